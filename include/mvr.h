@@ -1,0 +1,145 @@
+/*
+ * mvr.h -- C ABI of the MI355X ColBERT late-interaction search path (libmvr).
+ *
+ * Plain pointers and sizes only.  Every function returns an mvr_status; the
+ * library never calls exit().  mvr_last_error() gives the message of the last
+ * failure on the calling thread.  One index handle may be used by one host
+ * thread at a time; concurrency happens on the device.
+ *
+ * Reference interfaces each entry point replaces (paths relative to the
+ * reference repository root):
+ *   mvr_index_create / mvr_index_load
+ *       IGPGPUPP ctor + IGPGPUPP::buildIndex  src/impl/IGPGPUPP.hpp:67-251
+ *       (npy loads :124-173, readIVFIndex src/include/struct/IVFIndex.hpp:27-59,
+ *        device upload src/include/alg/igp_gpu_pp/DataStructPP.hpp:27-205)
+ *   mvr_search
+ *       IGPGPUPP::search  src/impl/IGPGPUPP.hpp:253-332, i.e. per query
+ *       GPGPURetrievalPP::search  src/include/alg/igp_gpu_pp/IGPGPURetrievalPP.hpp:114-176
+ *       (centroid scoring GenerateVQScoreCandidatePP.hpp:305-366, filter
+ *        FilterPP.hpp:267-398, rerank RerankPP.hpp:271-475, host top-k
+ *        IGPGPURetrievalPP.hpp:153-161).  With clip_zero = 1 it is the basic
+ *       variant's search, src/impl/IGPGPUBasic.hpp:190-310.
+ *   mvr_decompress
+ *       ResidualCodePack::Decode  src/include/alg/refine/ResidualScalarQuantizationPack.hpp:330-378
+ *       and decompressAndAddResidual  src/include/alg/igp_gpu_pp/RerankPP.hpp:65-113
+ */
+#ifndef MVR_H
+#define MVR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  MVR_OK = 0,
+  MVR_ERR_INVALID = 1,   /* bad argument or shape */
+  MVR_ERR_IO = 2,        /* missing or malformed file */
+  MVR_ERR_DEVICE = 3,    /* HIP runtime error */
+  MVR_ERR_UNSUPPORTED = 4
+} mvr_status;
+
+typedef enum { MVR_FP32 = 0, MVR_BF16 = 1, MVR_FP8 = 2 } mvr_precision;
+
+typedef struct mvr_index mvr_index;
+
+/* Index in the reference layout (src/impl/IGPGPUPP.hpp:30-55).  Pointers are
+ * host pointers unless on_device != 0, in which case they are device pointers
+ * on `device`; either way the library copies them. */
+typedef struct {
+  uint32_t n_doc;
+  uint64_t n_tok;
+  uint32_t n_centroid;
+  uint32_t dim;              /* must be 128 */
+  uint32_t n_bit;            /* must be 2 */
+  const uint32_t *doclens;   /* [n_doc]                         item_n_vec_l.npy */
+  const float *centroids;    /* [n_centroid][dim]               centroid_l.npy */
+  const uint32_t *codes;     /* [n_tok]                         vq_code_l.npy */
+  const uint8_t *residual;   /* [n_tok][dim*n_bit/8], MSB-first residual_code_l.npy */
+  const float *weights;      /* [2^n_bit] or [dim][2^n_bit]     weight_l.npy */
+  uint32_t weights_per_dim;  /* 0: one array for all dims, 1: per dimension */
+  const uint64_t *ivf_offsets; /* [n_centroid]                  ivf.index */
+  const uint32_t *ivf_sizes;   /* [n_centroid] */
+  const uint32_t *ivf_ids;     /* [ivf_n_ele], docs ascending per list */
+  uint64_t ivf_n_ele;
+  int on_device;
+} mvr_index_desc;
+
+typedef struct {
+  uint32_t nprobe;      /* centroids probed per query token (-nprobe) */
+  uint32_t probe_topk;  /* docs kept by the filter and reranked (-probe_topk) */
+  uint32_t topk;        /* results per query (-topk) */
+  uint32_t precision;   /* mvr_precision of centroid scoring and MaxSim */
+  uint32_t rescore;     /* BF16/FP8: FP32 rescoring of the best `rescore` docs (0 = off) */
+  uint32_t clip_zero;   /* 1: MaxSim per token is max(0, .) (basic variant) */
+  uint32_t batch;       /* queries per device pass (0 = 1024) */
+  uint32_t reserved;
+} mvr_search_params;
+
+/* Optional per-stage outputs (host buffers, any may be NULL), for tests. */
+typedef struct {
+  uint32_t *top_ids;   /* [n_q][t_max][nprobe_max] probed centroid ids */
+  float *top_s;        /* [n_q][t_max][nprobe_max] centroid scores + 1 */
+  uint32_t *cand_n;    /* [n_q] unique filter candidates */
+  uint32_t *sel_ids;   /* [n_q][probe_topk] filter top docs, best first */
+  float *sel_F;        /* [n_q][probe_topk] their filter scores */
+  float *sel_score;    /* [n_q][probe_topk] their MaxSim scores (final precision) */
+} mvr_stage_out;
+
+/* Per-call timings in milliseconds (device event time per stage, summed over passes). */
+typedef struct {
+  double centroid_ms, filter_ms, select_ms, rerank_ms, topk_ms, total_ms;
+} mvr_timings;
+
+int mvr_device_count(int *count);
+const char *mvr_last_error(void);
+
+int mvr_index_create(const mvr_index_desc *desc, int device, mvr_index **out);
+/* Loads centroid_l.npy, vq_code_l.npy, weight_l.npy, residual_code_l.npy,
+ * item_n_vec_l.npy and ivf.index from index_dir (IGPGPUPP.cpp:33-73). */
+int mvr_index_load(const char *index_dir, int device, mvr_index **out);
+void mvr_index_free(mvr_index *ix);
+int mvr_index_info(const mvr_index *ix, uint32_t *n_doc, uint64_t *n_tok, uint32_t *n_centroid,
+                   uint32_t *max_list, uint32_t *max_doclen);
+
+/* Batched search.  Q is [n_q][t_max][dim] (rows t >= q_ntok[q] are ignored);
+ * q_ntok (NULL: all t_max) and q_nprobe (NULL: params->nprobe) are host
+ * arrays.  With on_device = 0, Q and the outputs are host buffers and the call
+ * returns when results are on the host.  With on_device != 0 they are device
+ * buffers and work is queued on `stream` (a hipStream_t, NULL = default)
+ * without host synchronisation.  Outputs are [n_q][topk], best first;
+ * missing results are (UINT32_MAX, -inf). */
+int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, const uint32_t *q_ntok,
+               const uint32_t *q_nprobe, const mvr_search_params *params, uint32_t *out_ids,
+               float *out_scores, int on_device, void *stream, mvr_stage_out *stages,
+               mvr_timings *timings);
+
+/* FP32 decompression of whole documents: out receives sum(doclens[doc_ids])
+ * rows of dim floats, documents in the given order (host buffers). */
+int mvr_decompress(mvr_index *ix, const uint32_t *doc_ids, uint32_t n, float *out);
+
+/* ---- synthetic corpus and index build on the device (bench and tests) ---- */
+
+/* Mixture tokens: row k of out (device, [n_tok][dim]) is token tok_begin+k of
+ * the stream pair (stream_comp, stream_noise): normalise(centers[comp] +
+ * noise_std * z) with comp and z from the counter-based generator. */
+int mvr_synth_tokens(uint64_t seed, uint64_t stream_comp, uint64_t stream_noise,
+                     const float *centers_dev, uint32_t n_comp, uint32_t dim, double noise_std,
+                     uint64_t tok_begin, uint64_t n_tok, float *out_dev, void *stream);
+/* argmax_c dot(x_i, C_c) (lowest c on ties), FP32, device buffers. */
+int mvr_assign(const float *x_dev, uint64_t n, const float *C_dev, uint32_t n_c, uint32_t dim,
+               uint32_t *codes_dev, void *stream);
+/* argmax_c (dot(x_i, C_c) - half_norm[c]) for Lloyd iterations (half_norm may be NULL). */
+int mvr_assign_bias(const float *x_dev, uint64_t n, const float *C_dev, const float *half_norm_dev,
+                    uint32_t n_c, uint32_t dim, uint32_t *codes_dev, void *stream);
+/* Packed 2-bit residual codes of x against C[codes] with cutoffs [dim][3]
+ * (per_dim = 1) or [3]; device buffers. */
+int mvr_compress(const float *x_dev, const uint32_t *codes_dev, uint64_t n, const float *C_dev,
+                 uint32_t dim, const float *cutoffs_dev, int per_dim, uint8_t *out_dev, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MVR_H */

@@ -1,0 +1,80 @@
+"""Build recipe of the native parts (run from any directory).
+
+Outputs go to multi-vector-retrieval-gpu_amd/lib/:
+  libmvr.so                      HIP kernels + C ABI (hipcc, gfx950)
+  IGPGPUPP, IGPGPUBasic          command-line drop-ins (g++, link libmvr.so)
+  GPGPUBuildIndexPackGPU*.so     pybind11 index-build module (g++)
+
+Usage: python multi-vector-retrieval-gpu_amd/build.py [lib|cli|pymod|all]
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+import sysconfig
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+LIB = os.path.join(PKG, "lib")
+INCLUDE = os.path.join(ROOT, "include")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# -packed-fp32-ops: no v_pk_*_f32 (unreliable on these boxes);
+# -ffp-contract=off: only explicit fmaf() fuses, matching the CPU oracle.
+HIP_FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-ffp-contract=off",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
+             "-Wno-unused-command-line-argument"]
+HIP_SOURCES = ["mvr_kernels.hip", "mvr_capi.hip"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_lib() -> str:
+    os.makedirs(LIB, exist_ok=True)
+    out = os.path.join(LIB, "libmvr.so")
+    objs = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(LIB, src.replace(".hip", ".o"))
+        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    tmp = out + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_cli() -> list[str]:
+    outs = []
+    for name, variant in (("IGPGPUPP", "MVR_VARIANT_PP"), ("IGPGPUBasic", "MVR_VARIANT_BASIC")):
+        out = os.path.join(LIB, name)
+        _run(["g++", "-O2", "-std=c++17", f"-D{variant}=1", "-I", INCLUDE,
+              os.path.join(CSRC, "igpgpu_cli.cpp"), "-o", out, "-L", LIB, "-lmvr",
+              "-Wl,-rpath,$ORIGIN"])
+        outs.append(out)
+    return outs
+
+
+def build_pymod() -> str:
+    import pybind11
+    suffix = sysconfig.get_config_var("EXT_SUFFIX")
+    out = os.path.join(LIB, "GPGPUBuildIndexPackGPU" + suffix)
+    _run(["g++", "-O3", "-std=c++17", "-shared", "-fPIC", "-fopenmp", "-ffp-contract=off",
+          "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
+          os.path.join(CSRC, "build_index_module.cpp"), "-o", out])
+    return out
+
+
+def build_all():
+    build_lib()
+    build_cli()
+    build_pymod()
+
+
+if __name__ == "__main__":
+    what = sys.argv[1] if len(sys.argv) > 1 else "all"
+    {"lib": build_lib, "cli": build_cli, "pymod": build_pymod, "all": build_all}[what]()

@@ -1,0 +1,586 @@
+// C ABI of libmvr: index upload/load, batched search orchestration,
+// decompression and the device-side synthetic index helpers.  Host-only code
+// apart from the launch calls into mvr_kernels.hip.
+//
+// Replaces (reference paths): IGPGPUPP ctor/buildIndex/search
+// (src/impl/IGPGPUPP.hpp:67-332), the per-thread GPGPURetrievalPP pipeline
+// (src/include/alg/igp_gpu_pp/IGPGPURetrievalPP.hpp:73-176) and the device
+// structs of DataStructPP.hpp:16-205.  Differences by design: the whole index
+// (including residual codes) is device resident, queries are processed in
+// batches of up to `batch` per device pass, and errors are returned as status
+// codes instead of exit().
+#include "../../include/mvr.h"
+#include "mvr_internal.h"
+#include "npy_io.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <string>
+#include <vector>
+
+using namespace mvr;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess)                                                                \
+      return fail(MVR_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                  __FILE__, __LINE__);                                                   \
+  } while (0)
+
+// Growable device buffer.
+struct DBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      if (e != hipSuccess) return e;
+      p = nullptr;
+      cap = 0;
+    }
+    const size_t b = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) cap = b;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+// Growable pinned host buffer (metadata staging).
+struct HBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    const size_t b = std::max<size_t>(bytes, 256);
+    hipError_t e = hipHostMalloc(&p, b, hipHostMallocDefault);
+    if (e == hipSuccess) cap = b;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
+struct Workspace {
+  DBuf Q, meta, part_ids, part_s, top_ids, top_s, cand_cnt, cand_keys, sel_ids, sel_F, sel_n,
+      sel_score, out_ids, out_s;
+  HBuf staging;
+  hipEvent_t staging_done = nullptr;
+  void release() {
+    for (DBuf *b : {&Q, &meta, &part_ids, &part_s, &top_ids, &top_s, &cand_cnt, &cand_keys,
+                    &sel_ids, &sel_F, &sel_n, &sel_score, &out_ids, &out_s})
+      b->release();
+    staging.release();
+    if (staging_done) (void)hipEventDestroy(staging_done);
+    staging_done = nullptr;
+  }
+};
+
+}  // namespace
+
+struct mvr_index {
+  int device = 0;
+  DevIndex d;
+  std::vector<uint32_t> h_doclens;
+  Workspace ws;
+};
+
+namespace {
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+void free_devindex(DevIndex &d) {
+  for (void *p : {(void *)d.C, (void *)d.C_bf16, (void *)d.C_fp8, (void *)d.C_fp8_scale,
+                  (void *)d.codes, (void *)d.residual, (void *)d.W, (void *)d.doclens,
+                  (void *)d.docoff, (void *)d.ivf_off, (void *)d.ivf_size, (void *)d.ivf_ids,
+                  (void *)d.seg_off})
+    if (p) (void)hipFree(p);
+  d = DevIndex();
+}
+
+template <class T>
+hipError_t upload(T **dst, const T *src, size_t n, bool src_on_device) {
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(dst), std::max<size_t>(n, 1) * sizeof(T));
+  if (e != hipSuccess || n == 0) return e;
+  return hipMemcpy(*dst, src, n * sizeof(T),
+                   src_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice);
+}
+
+template <class T> std::vector<T> fetch(const T *src, size_t n, bool on_device) {
+  std::vector<T> v(n);
+  if (n == 0) return v;
+  if (on_device) (void)hipMemcpy(v.data(), src, n * sizeof(T), hipMemcpyDeviceToHost);
+  else std::memcpy(v.data(), src, n * sizeof(T));
+  return v;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mvr_last_error(void) { return g_err.c_str(); }
+
+int mvr_device_count(int *count) {
+  if (!count) return fail(MVR_ERR_INVALID, "count is NULL");
+  HIP_TRY(hipGetDeviceCount(count));
+  return MVR_OK;
+}
+
+int mvr_index_create(const mvr_index_desc *desc, int device, mvr_index **out) {
+  if (!desc || !out) return fail(MVR_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (desc->dim != kDim) return fail(MVR_ERR_UNSUPPORTED, "dim %u unsupported (kernels need 128)", desc->dim);
+  if (desc->n_bit != 2) return fail(MVR_ERR_UNSUPPORTED, "n_bit %u unsupported (kernels need 2)", desc->n_bit);
+  if (desc->n_doc == 0 || desc->n_centroid == 0)
+    return fail(MVR_ERR_INVALID, "empty index (n_doc %u, n_centroid %u)", desc->n_doc, desc->n_centroid);
+  if (!desc->doclens || !desc->centroids || !desc->codes || !desc->residual || !desc->weights ||
+      !desc->ivf_offsets || !desc->ivf_sizes || (!desc->ivf_ids && desc->ivf_n_ele))
+    return fail(MVR_ERR_INVALID, "NULL index array");
+  int n_dev = 0;
+  HIP_TRY(hipGetDeviceCount(&n_dev));
+  if (device < 0 || device >= n_dev) return fail(MVR_ERR_INVALID, "device %d out of range (%d)", device, n_dev);
+  DeviceGuard g(device);
+  const bool od = desc->on_device != 0;
+
+  mvr_index *ix = new mvr_index();
+  ix->device = device;
+  DevIndex &d = ix->d;
+  d.n_doc = desc->n_doc;
+  d.n_tok = desc->n_tok;
+  d.n_c = desc->n_centroid;
+  d.n_ele = desc->ivf_n_ele;
+  d.n_seg = (d.n_doc + kSeg - 1) / kSeg;
+
+  auto bail = [&](int code) {
+    free_devindex(ix->d);
+    delete ix;
+    return code;
+  };
+#define UP(dst, src, n)                                                                      \
+  do {                                                                                       \
+    hipError_t _e = upload(&(dst), (src), (n), od);                                          \
+    if (_e != hipSuccess)                                                                    \
+      return bail(fail(MVR_ERR_DEVICE, "upload of %s failed: %s", #src, hipGetErrorString(_e))); \
+  } while (0)
+
+  // host-side metadata: doc lengths / offsets, list sizes, decode table
+  ix->h_doclens = fetch(desc->doclens, d.n_doc, od);
+  std::vector<uint64_t> docoff(d.n_doc);
+  uint64_t acc = 0;
+  for (uint32_t i = 0; i < d.n_doc; ++i) {
+    docoff[i] = acc;
+    acc += ix->h_doclens[i];
+    d.max_doclen = std::max(d.max_doclen, ix->h_doclens[i]);
+  }
+  if (acc != d.n_tok)
+    return bail(fail(MVR_ERR_INVALID, "sum of doc lengths %llu != n_tok %llu",
+                     (unsigned long long)acc, (unsigned long long)d.n_tok));
+  std::vector<uint32_t> sizes = fetch(desc->ivf_sizes, d.n_c, od);
+  std::vector<uint64_t> offs = fetch(desc->ivf_offsets, d.n_c, od);
+  for (uint32_t c = 0; c < d.n_c; ++c) {
+    d.max_list = std::max(d.max_list, sizes[c]);
+    if (offs[c] + sizes[c] > d.n_ele)
+      return bail(fail(MVR_ERR_INVALID, "IVF list %u exceeds n_ele", c));
+  }
+  std::vector<float> W(kDim * 4);
+  {
+    const size_t nw = desc->weights_per_dim ? kDim * 4 : 4;
+    std::vector<float> w = fetch(desc->weights, nw, od);
+    for (uint32_t dd = 0; dd < kDim; ++dd)
+      for (uint32_t b = 0; b < 4; ++b) W[dd * 4 + b] = w[(desc->weights_per_dim ? dd * 4 : 0) + b];
+  }
+
+  UP(d.C, desc->centroids, (size_t)d.n_c * kDim);
+  UP(d.codes, desc->codes, d.n_tok);
+  UP(d.residual, desc->residual, d.n_tok * kResBytes);
+  UP(d.doclens, desc->doclens, d.n_doc);
+  UP(d.ivf_off, desc->ivf_offsets, d.n_c);
+  UP(d.ivf_size, desc->ivf_sizes, d.n_c);
+  UP(d.ivf_ids, desc->ivf_ids, d.n_ele);
+#undef UP
+  hipError_t e = upload(&d.docoff, docoff.data(), docoff.size(), false);
+  if (e == hipSuccess) e = upload(&d.W, W.data(), W.size(), false);
+  if (e == hipSuccess)
+    e = hipMalloc(reinterpret_cast<void **>(&d.seg_off), (size_t)d.n_c * (d.n_seg + 1) * sizeof(uint32_t));
+  if (e == hipSuccess) e = launch_seg_offsets(d, nullptr);
+  if (e == hipSuccess) e = launch_convert_centroids(d, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e != hipSuccess) return bail(fail(MVR_ERR_DEVICE, "index setup failed: %s", hipGetErrorString(e)));
+  *out = ix;
+  return MVR_OK;
+}
+
+int mvr_index_load(const char *index_dir, int device, mvr_index **out) {
+  if (!index_dir || !out) return fail(MVR_ERR_INVALID, "NULL argument");
+  const std::string dir(index_dir);
+  NpyArray cent, codes, weight, res, lens;
+  std::string err;
+  if (!(err = npy_load(dir + "/centroid_l.npy", cent)).empty()) return fail(MVR_ERR_IO, "%s", err.c_str());
+  if (!(err = npy_load(dir + "/vq_code_l.npy", codes)).empty()) return fail(MVR_ERR_IO, "%s", err.c_str());
+  if (!(err = npy_load(dir + "/weight_l.npy", weight)).empty()) return fail(MVR_ERR_IO, "%s", err.c_str());
+  if (!(err = npy_load(dir + "/residual_code_l.npy", res)).empty()) return fail(MVR_ERR_IO, "%s", err.c_str());
+  if (!(err = npy_load(dir + "/item_n_vec_l.npy", lens)).empty()) return fail(MVR_ERR_IO, "%s", err.c_str());
+  if (cent.word_size != 4 || cent.descr[1] != 'f' || cent.shape.size() != 2)
+    return fail(MVR_ERR_IO, "centroid_l.npy must be float32 [n_centroid, dim]");
+  if (codes.word_size != 4 || codes.shape.size() != 1)
+    return fail(MVR_ERR_IO, "vq_code_l.npy must be (u)int32 [n_vec]");
+  if (weight.word_size != 4 || weight.descr[1] != 'f' || weight.shape.empty() || weight.shape.size() > 2)
+    return fail(MVR_ERR_IO, "weight_l.npy must be float32 [2^b] or [dim, 2^b]");
+  if (res.word_size != 1 || res.shape.size() != 1)
+    return fail(MVR_ERR_IO, "residual_code_l.npy must be uint8 [n_vec * packed]");
+  if (lens.word_size != 4 || lens.shape.size() != 1)
+    return fail(MVR_ERR_IO, "item_n_vec_l.npy must be uint32 [n_item]");
+  const uint32_t n_c = (uint32_t)cent.shape[0], dim = (uint32_t)cent.shape[1];
+  const uint64_t n_tok = codes.shape[0];
+  if (res.shape[0] != n_tok * (dim / 4))
+    return fail(MVR_ERR_IO, "residual_code_l.npy has %zu bytes, expected %llu", res.shape[0],
+                (unsigned long long)(n_tok * (dim / 4)));
+  const uint32_t *cd = codes.as<uint32_t>();
+  for (uint64_t i = 0; i < n_tok; ++i)
+    if (cd[i] >= n_c) return fail(MVR_ERR_IO, "vq code %u at %llu >= n_centroid %u", cd[i], (unsigned long long)i, n_c);
+
+  // ivf.index: u32 n_centroid, u64 n_ele, u64[n_c] offsets, u32[n_c] sizes,
+  // u32[n_ele] ids (src/include/struct/IVFIndex.hpp:168-183)
+  std::ifstream in(dir + "/ivf.index", std::ios::binary);
+  if (!in) return fail(MVR_ERR_IO, "cannot open %s/ivf.index", index_dir);
+  uint32_t ivf_nc = 0;
+  uint64_t n_ele = 0;
+  in.read(reinterpret_cast<char *>(&ivf_nc), 4);
+  in.read(reinterpret_cast<char *>(&n_ele), 8);
+  if (!in || ivf_nc != n_c) return fail(MVR_ERR_IO, "ivf.index header mismatch (n_centroid %u vs %u)", ivf_nc, n_c);
+  std::vector<uint64_t> off(n_c);
+  std::vector<uint32_t> size(n_c), ids(n_ele);
+  in.read(reinterpret_cast<char *>(off.data()), n_c * 8);
+  in.read(reinterpret_cast<char *>(size.data()), n_c * 4);
+  in.read(reinterpret_cast<char *>(ids.data()), n_ele * 4);
+  if (!in) return fail(MVR_ERR_IO, "ivf.index truncated");
+
+  mvr_index_desc desc{};
+  desc.n_doc = (uint32_t)lens.shape[0];
+  desc.n_tok = n_tok;
+  desc.n_centroid = n_c;
+  desc.dim = dim;
+  desc.n_bit = 2;
+  desc.doclens = lens.as<uint32_t>();
+  desc.centroids = cent.as<float>();
+  desc.codes = cd;
+  desc.residual = res.as<uint8_t>();
+  desc.weights = weight.as<float>();
+  desc.weights_per_dim = weight.shape.size() == 2 ? 1u : 0u;
+  if (desc.weights_per_dim ? (weight.shape[0] != dim || weight.shape[1] != 4) : weight.shape[0] != 4)
+    return fail(MVR_ERR_UNSUPPORTED, "weight_l.npy shape does not match 2-bit codes");
+  desc.ivf_offsets = off.data();
+  desc.ivf_sizes = size.data();
+  desc.ivf_ids = ids.data();
+  desc.ivf_n_ele = n_ele;
+  desc.on_device = 0;
+  return mvr_index_create(&desc, device, out);
+}
+
+void mvr_index_free(mvr_index *ix) {
+  if (!ix) return;
+  DeviceGuard g(ix->device);
+  (void)hipDeviceSynchronize();
+  ix->ws.release();
+  free_devindex(ix->d);
+  delete ix;
+}
+
+int mvr_index_info(const mvr_index *ix, uint32_t *n_doc, uint64_t *n_tok, uint32_t *n_centroid,
+                   uint32_t *max_list, uint32_t *max_doclen) {
+  if (!ix) return fail(MVR_ERR_INVALID, "NULL index");
+  if (n_doc) *n_doc = ix->d.n_doc;
+  if (n_tok) *n_tok = ix->d.n_tok;
+  if (n_centroid) *n_centroid = ix->d.n_c;
+  if (max_list) *max_list = ix->d.max_list;
+  if (max_doclen) *max_doclen = ix->d.max_doclen;
+  return MVR_OK;
+}
+
+int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, const uint32_t *q_ntok,
+               const uint32_t *q_nprobe, const mvr_search_params *prm, uint32_t *out_ids,
+               float *out_scores, int on_device, void *stream, mvr_stage_out *stages,
+               mvr_timings *timings) {
+  if (!ix || !prm) return fail(MVR_ERR_INVALID, "NULL argument");
+  if (n_q == 0) return MVR_OK;
+  if (!Q || !out_ids || !out_scores) return fail(MVR_ERR_INVALID, "NULL query or output buffer");
+  const DevIndex &d = ix->d;
+  const uint32_t R = prm->probe_topk, k = prm->topk;
+  if (t_max == 0 || t_max > kMaxQTok) return fail(MVR_ERR_INVALID, "query tokens %u not in [1, 32]", t_max);
+  if (R < k) return fail(MVR_ERR_INVALID, "probe_topk %u is smaller than topk %u", R, k);
+  if (k == 0) return fail(MVR_ERR_INVALID, "topk must be positive");
+  if (R > kMaxSelect) return fail(MVR_ERR_UNSUPPORTED, "probe_topk %u > %u", R, kMaxSelect);
+  if (prm->precision != MVR_FP32) return fail(MVR_ERR_UNSUPPORTED, "precision %u not built", prm->precision);
+  if (stages && on_device) return fail(MVR_ERR_INVALID, "stage outputs need host buffers");
+  uint32_t P = prm->nprobe;
+  std::vector<uint32_t> np(n_q), nt(n_q);
+  for (uint32_t q = 0; q < n_q; ++q) {
+    np[q] = q_nprobe ? q_nprobe[q] : prm->nprobe;
+    nt[q] = q_ntok ? q_ntok[q] : t_max;
+    if (np[q] == 0 || np[q] > kMaxProbe) return fail(MVR_ERR_INVALID, "nprobe %u not in [1, 32]", np[q]);
+    if (np[q] > d.n_c) return fail(MVR_ERR_INVALID, "nprobe %u is larger than n_centroid %u", np[q], d.n_c);
+    if (nt[q] == 0 || nt[q] > t_max) return fail(MVR_ERR_INVALID, "query %u has %u tokens (t_max %u)", q, nt[q], t_max);
+    P = std::max(P, np[q]);
+  }
+  P = std::min(P, d.n_c);
+  const uint32_t B = prm->batch ? prm->batch : 1024;
+  DeviceGuard g(ix->device);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  Workspace &ws = ix->ws;
+
+  // ---- per-call metadata (host) : q_ntok, q_np, cand_off, row_map ----
+  bool uniform = true;
+  for (uint32_t q = 0; q < n_q; ++q) uniform &= (nt[q] == t_max);
+  std::vector<uint64_t> coff(n_q);
+  std::vector<uint32_t> rmap;
+  std::vector<uint32_t> rows_begin;  // per batch: first row-map entry
+  size_t max_pool = 0, max_rows = 0;
+  for (uint32_t b0 = 0; b0 < n_q; b0 += B) {
+    const uint32_t nb = std::min(B, n_q - b0);
+    uint64_t pool = 0;
+    rows_begin.push_back((uint32_t)rmap.size());
+    size_t rows = 0;
+    for (uint32_t i = 0; i < nb; ++i) {
+      const uint32_t q = b0 + i;
+      coff[q] = pool;
+      const uint64_t bound = std::min<uint64_t>(d.n_doc, (uint64_t)nt[q] * np[q] * d.max_list);
+      pool += bound;
+      if (!uniform)
+        for (uint32_t t = 0; t < nt[q]; ++t) rmap.push_back(i * t_max + t);
+      rows += uniform ? t_max : nt[q];
+    }
+    max_pool = std::max<size_t>(max_pool, pool);
+    max_rows = std::max(max_rows, rows);
+  }
+  const size_t meta_u32 = 2 * (size_t)n_q + rmap.size();
+  const size_t meta_bytes = (size_t)n_q * 8 + meta_u32 * 4;
+  // wait until the previous call's staging copy finished before reuse
+  if (ws.staging_done) HIP_TRY(hipEventSynchronize(ws.staging_done));
+  else HIP_TRY(hipEventCreateWithFlags(&ws.staging_done, hipEventDisableTiming));
+  HIP_TRY(ws.staging.ensure(meta_bytes));
+  uint8_t *hs = reinterpret_cast<uint8_t *>(ws.staging.p);
+  std::memcpy(hs, coff.data(), (size_t)n_q * 8);
+  std::memcpy(hs + (size_t)n_q * 8, nt.data(), (size_t)n_q * 4);
+  std::memcpy(hs + (size_t)n_q * 12, np.data(), (size_t)n_q * 4);
+  if (!rmap.empty()) std::memcpy(hs + (size_t)n_q * 16, rmap.data(), rmap.size() * 4);
+  HIP_TRY(ws.meta.ensure(meta_bytes));
+  HIP_TRY(hipMemcpyAsync(ws.meta.p, hs, meta_bytes, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipEventRecord(ws.staging_done, s));
+  const uint64_t *d_coff = reinterpret_cast<const uint64_t *>(ws.meta.p);
+  const uint32_t *d_nt = reinterpret_cast<const uint32_t *>(ws.meta.as<uint8_t>() + (size_t)n_q * 8);
+  const uint32_t *d_np = d_nt + n_q;
+  const uint32_t *d_rmap = d_np + n_q;
+
+  // ---- workspace ----
+  const uint32_t n_rows_max = (uint32_t)max_rows;
+  const uint32_t rows_blocks = (n_rows_max + 255) / 256;
+  uint32_t n_split = std::max(1u, std::min(16u, (1024u + rows_blocks - 1) / std::max(1u, rows_blocks)));
+  n_split = std::min(n_split, std::max(1u, d.n_c / 256));
+  const size_t Bc = std::min<size_t>(B, n_q);
+  HIP_TRY(ws.part_ids.ensure((size_t)n_split * n_rows_max * P * 4));
+  HIP_TRY(ws.part_s.ensure((size_t)n_split * n_rows_max * P * 4));
+  HIP_TRY(ws.top_ids.ensure(Bc * t_max * P * 4));
+  HIP_TRY(ws.top_s.ensure(Bc * t_max * P * 4));
+  HIP_TRY(ws.cand_cnt.ensure(Bc * 4));
+  HIP_TRY(ws.cand_keys.ensure(max_pool * 8));
+  HIP_TRY(ws.sel_ids.ensure(Bc * R * 4));
+  HIP_TRY(ws.sel_F.ensure(Bc * R * 4));
+  HIP_TRY(ws.sel_n.ensure(Bc * 4));
+  HIP_TRY(ws.sel_score.ensure(Bc * R * 4));
+  if (!on_device) {
+    HIP_TRY(ws.Q.ensure(Bc * t_max * kDim * 4));
+    HIP_TRY(ws.out_ids.ensure(Bc * k * 4));
+    HIP_TRY(ws.out_s.ensure(Bc * k * 4));
+  }
+
+  hipEvent_t ev[6] = {};
+  if (timings) {
+    for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
+    *timings = mvr_timings{};
+  }
+  auto rec = [&](int i) -> hipError_t { return timings ? hipEventRecord(ev[i], s) : hipSuccess; };
+  auto acc_ms = [&](double &dst, int a, int b) -> hipError_t {
+    if (!timings) return hipSuccess;
+    hipError_t e = hipEventSynchronize(ev[b]);
+    if (e != hipSuccess) return e;
+    float ms = 0.f;
+    e = hipEventElapsedTime(&ms, ev[a], ev[b]);
+    dst += ms;
+    return e;
+  };
+
+  int bi = 0;
+  for (uint32_t b0 = 0; b0 < n_q; b0 += B, ++bi) {
+    const uint32_t nb = std::min(B, n_q - b0);
+    const float *Qb;
+    uint32_t *oid;
+    float *os;
+    if (on_device) {
+      Qb = Q + (size_t)b0 * t_max * kDim;
+      oid = out_ids + (size_t)b0 * k;
+      os = out_scores + (size_t)b0 * k;
+    } else {
+      HIP_TRY(hipMemcpyAsync(ws.Q.p, Q + (size_t)b0 * t_max * kDim, (size_t)nb * t_max * kDim * 4,
+                             hipMemcpyHostToDevice, s));
+      Qb = ws.Q.as<float>();
+      oid = ws.out_ids.as<uint32_t>();
+      os = ws.out_s.as<float>();
+    }
+    const uint32_t *rm = uniform ? nullptr : d_rmap + rows_begin[bi];
+    uint32_t n_rows = 0;
+    if (uniform) n_rows = nb * t_max;
+    else for (uint32_t i = 0; i < nb; ++i) n_rows += nt[b0 + i];
+    const uint32_t *bnt = d_nt + b0, *bnp = d_np + b0;
+    const uint64_t *bcoff = d_coff + b0;
+
+    HIP_TRY(rec(0));
+    HIP_TRY(launch_centroid_topk(Qb, rm, n_rows, nb * t_max, d, P, prm->precision, n_split,
+                                 ws.part_ids.as<uint32_t>(), ws.part_s.as<float>(),
+                                 ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), nullptr, s));
+    HIP_TRY(rec(1));
+    HIP_TRY(launch_fill_u32(ws.cand_cnt.as<uint32_t>(), 0u, nb, s));
+    HIP_TRY(launch_filter(ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), t_max, P, bnt, bnp, nb,
+                          d, bcoff, ws.cand_cnt.as<uint32_t>(), ws.cand_keys.as<uint64_t>(), s));
+    HIP_TRY(rec(2));
+    HIP_TRY(launch_select(ws.cand_keys.as<uint64_t>(), bcoff, ws.cand_cnt.as<uint32_t>(), nb, R,
+                          ws.sel_ids.as<uint32_t>(), ws.sel_F.as<float>(), ws.sel_n.as<uint32_t>(), s));
+    HIP_TRY(rec(3));
+    HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R,
+                          d, prm->precision, (int)prm->clip_zero, ws.sel_score.as<float>(), s));
+    HIP_TRY(rec(4));
+    HIP_TRY(launch_topk(ws.sel_ids.as<uint32_t>(), ws.sel_score.as<float>(), ws.sel_n.as<uint32_t>(),
+                        R, nb, k, oid, os, s));
+    HIP_TRY(rec(5));
+    if (!on_device) {
+      HIP_TRY(hipMemcpyAsync(out_ids + (size_t)b0 * k, oid, (size_t)nb * k * 4, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(out_scores + (size_t)b0 * k, os, (size_t)nb * k * 4, hipMemcpyDeviceToHost, s));
+    }
+    if (stages) {
+      HIP_TRY(hipStreamSynchronize(s));
+      if (stages->top_ids)
+        HIP_TRY(hipMemcpy(stages->top_ids + (size_t)b0 * t_max * P, ws.top_ids.p, (size_t)nb * t_max * P * 4, hipMemcpyDeviceToHost));
+      if (stages->top_s)
+        HIP_TRY(hipMemcpy(stages->top_s + (size_t)b0 * t_max * P, ws.top_s.p, (size_t)nb * t_max * P * 4, hipMemcpyDeviceToHost));
+      if (stages->cand_n)
+        HIP_TRY(hipMemcpy(stages->cand_n + b0, ws.cand_cnt.p, (size_t)nb * 4, hipMemcpyDeviceToHost));
+      if (stages->sel_ids)
+        HIP_TRY(hipMemcpy(stages->sel_ids + (size_t)b0 * R, ws.sel_ids.p, (size_t)nb * R * 4, hipMemcpyDeviceToHost));
+      if (stages->sel_F)
+        HIP_TRY(hipMemcpy(stages->sel_F + (size_t)b0 * R, ws.sel_F.p, (size_t)nb * R * 4, hipMemcpyDeviceToHost));
+      if (stages->sel_score)
+        HIP_TRY(hipMemcpy(stages->sel_score + (size_t)b0 * R, ws.sel_score.p, (size_t)nb * R * 4, hipMemcpyDeviceToHost));
+    }
+    if (timings) {
+      HIP_TRY(acc_ms(timings->centroid_ms, 0, 1));
+      HIP_TRY(acc_ms(timings->filter_ms, 1, 2));
+      HIP_TRY(acc_ms(timings->select_ms, 2, 3));
+      HIP_TRY(acc_ms(timings->rerank_ms, 3, 4));
+      HIP_TRY(acc_ms(timings->topk_ms, 4, 5));
+      HIP_TRY(acc_ms(timings->total_ms, 0, 5));
+    }
+  }
+  if (timings)
+    for (auto &e : ev) (void)hipEventDestroy(e);
+  if (!on_device) HIP_TRY(hipStreamSynchronize(s));
+  return MVR_OK;
+}
+
+int mvr_decompress(mvr_index *ix, const uint32_t *doc_ids, uint32_t n, float *out) {
+  if (!ix || (!doc_ids && n) || (!out && n)) return fail(MVR_ERR_INVALID, "NULL argument");
+  if (n == 0) return MVR_OK;
+  std::vector<uint64_t> toks;
+  uint64_t acc = 0;
+  std::vector<uint64_t> docoff(ix->d.n_doc);
+  for (uint32_t i = 0; i < ix->d.n_doc; ++i) { docoff[i] = acc; acc += ix->h_doclens[i]; }
+  for (uint32_t i = 0; i < n; ++i) {
+    if (doc_ids[i] >= ix->d.n_doc) return fail(MVR_ERR_INVALID, "doc id %u >= n_doc %u", doc_ids[i], ix->d.n_doc);
+    for (uint32_t t = 0; t < ix->h_doclens[doc_ids[i]]; ++t) toks.push_back(docoff[doc_ids[i]] + t);
+  }
+  DeviceGuard g(ix->device);
+  uint64_t *dt = nullptr;
+  float *dout = nullptr;
+  HIP_TRY(hipMalloc(reinterpret_cast<void **>(&dt), std::max<size_t>(1, toks.size()) * 8));
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(&dout), std::max<size_t>(1, toks.size()) * kDim * 4);
+  if (e == hipSuccess) e = hipMemcpy(dt, toks.data(), toks.size() * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_decompress(dt, toks.size(), ix->d, dout, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(out, dout, toks.size() * kDim * 4, hipMemcpyDeviceToHost);
+  (void)hipFree(dt);
+  if (dout) (void)hipFree(dout);
+  if (e != hipSuccess) return fail(MVR_ERR_DEVICE, "decompress failed: %s", hipGetErrorString(e));
+  return MVR_OK;
+}
+
+int mvr_synth_tokens(uint64_t seed, uint64_t stream_comp, uint64_t stream_noise,
+                     const float *centers_dev, uint32_t n_comp, uint32_t dim, double noise_std,
+                     uint64_t tok_begin, uint64_t n_tok, float *out_dev, void *stream) {
+  if (dim != kDim) return fail(MVR_ERR_UNSUPPORTED, "dim must be 128");
+  if (!centers_dev || !out_dev || n_comp == 0) return fail(MVR_ERR_INVALID, "bad arguments");
+  HIP_TRY(launch_synth_tokens(seed, stream_comp, stream_noise, centers_dev, n_comp, noise_std,
+                              tok_begin, n_tok, out_dev, reinterpret_cast<hipStream_t>(stream)));
+  return MVR_OK;
+}
+
+int mvr_assign_bias(const float *x_dev, uint64_t n, const float *C_dev, const float *half_norm_dev,
+                    uint32_t n_c, uint32_t dim, uint32_t *codes_dev, void *stream) {
+  if (dim != kDim) return fail(MVR_ERR_UNSUPPORTED, "dim must be 128");
+  if (!x_dev || !C_dev || !codes_dev || n_c == 0) return fail(MVR_ERR_INVALID, "bad arguments");
+  HIP_TRY(launch_assign(x_dev, n, C_dev, half_norm_dev, n_c, codes_dev,
+                        reinterpret_cast<hipStream_t>(stream)));
+  return MVR_OK;
+}
+
+int mvr_assign(const float *x_dev, uint64_t n, const float *C_dev, uint32_t n_c, uint32_t dim,
+               uint32_t *codes_dev, void *stream) {
+  return mvr_assign_bias(x_dev, n, C_dev, nullptr, n_c, dim, codes_dev, stream);
+}
+
+int mvr_compress(const float *x_dev, const uint32_t *codes_dev, uint64_t n, const float *C_dev,
+                 uint32_t dim, const float *cutoffs_dev, int per_dim, uint8_t *out_dev, void *stream) {
+  if (dim != kDim) return fail(MVR_ERR_UNSUPPORTED, "dim must be 128");
+  if (!x_dev || !codes_dev || !C_dev || !cutoffs_dev || !out_dev) return fail(MVR_ERR_INVALID, "bad arguments");
+  HIP_TRY(launch_compress(x_dev, codes_dev, n, C_dev, cutoffs_dev, per_dim, out_dev,
+                          reinterpret_cast<hipStream_t>(stream)));
+  return MVR_OK;
+}
+
+}  // extern "C"

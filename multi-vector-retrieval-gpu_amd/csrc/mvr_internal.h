@@ -1,0 +1,80 @@
+// Internal declarations shared by the libmvr translation units (kernel
+// launchers and the C-ABI layer).  All kernels are written for gfx950
+// (wave64, 160 KB LDS per CU) and specialise dim = 128, 2-bit residuals.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace mvr {
+
+constexpr uint32_t kDim = 128;             // embedding width
+constexpr uint32_t kResBytes = 32;         // packed 2-bit residual bytes per token
+constexpr uint32_t kMaxQTok = 32;          // query tokens (bit mask width in the filter)
+constexpr uint32_t kMaxProbe = 32;         // largest nprobe
+constexpr uint32_t kSeg = 4096;            // docs per filter segment (LDS state)
+constexpr uint32_t kMaxSelect = 4096;      // largest probe_topk
+constexpr double kFxScale = 4294967296.0;  // filter fixed point: 2^32
+
+// Device-resident index (layout mirrors the reference files).
+struct DevIndex {
+  uint32_t n_doc = 0, n_c = 0, n_seg = 0, max_list = 0, max_doclen = 0;
+  uint64_t n_tok = 0, n_ele = 0;
+  float *C = nullptr;          // [n_c][128] f32
+  uint16_t *C_bf16 = nullptr;  // [n_c][128] bf16 (RNE of C)
+  uint8_t *C_fp8 = nullptr;    // [n_c][128] e4m3 of C / scale
+  float *C_fp8_scale = nullptr;// [n_c]
+  uint32_t *codes = nullptr;   // [n_tok]
+  uint8_t *residual = nullptr; // [n_tok][32]
+  float *W = nullptr;          // [128][4] decode table
+  uint32_t *doclens = nullptr; // [n_doc]
+  uint64_t *docoff = nullptr;  // [n_doc]
+  uint64_t *ivf_off = nullptr; // [n_c]
+  uint32_t *ivf_size = nullptr;// [n_c]
+  uint32_t *ivf_ids = nullptr; // [n_ele]
+  uint32_t *seg_off = nullptr; // [n_c][n_seg+1] list index where each segment starts
+};
+
+// ---- launchers (return hipError_t of the launch) ----
+hipError_t launch_seg_offsets(const DevIndex &ix, hipStream_t s);
+hipError_t launch_convert_centroids(const DevIndex &ix, hipStream_t s);
+
+// centroid scoring + top-P.  rows: n_rows query-token rows; row_map (nullable)
+// maps row -> index into Q rows and into the output rows.
+hipError_t launch_centroid_topk(const float *Q, const uint32_t *row_map, uint32_t n_rows,
+                                uint32_t n_out_rows, const DevIndex &ix, uint32_t P, int precision,
+                                uint32_t n_split, uint32_t *part_ids, float *part_s,
+                                uint32_t *top_ids, float *top_s, const uint16_t *Qbf,
+                                hipStream_t s);
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
+
+hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t_max, uint32_t P,
+                         const uint32_t *q_ntok, const uint32_t *q_np, uint32_t n_q,
+                         const DevIndex &ix, const uint64_t *cand_off, uint32_t *cand_cnt,
+                         uint64_t *cand_keys, hipStream_t s);
+
+hipError_t launch_select(const uint64_t *cand_keys, const uint64_t *cand_off,
+                         const uint32_t *cand_cnt, uint32_t n_q, uint32_t R, uint32_t *sel_ids,
+                         float *sel_F, uint32_t *sel_n, hipStream_t s);
+
+hipError_t launch_rerank(const float *Q, uint32_t t_max, const uint32_t *q_ntok, uint32_t n_q,
+                         const uint32_t *sel_ids, const uint32_t *sel_n, uint32_t R,
+                         const DevIndex &ix, int precision, int clip_zero, float *sel_score,
+                         hipStream_t s);
+
+hipError_t launch_topk(const uint32_t *ids, const float *score, const uint32_t *n_in, uint32_t R,
+                       uint32_t n_q, uint32_t k, uint32_t *out_ids, float *out_s, hipStream_t s);
+
+hipError_t launch_decompress(const uint64_t *tok_ids, uint64_t n, const DevIndex &ix, float *out,
+                             hipStream_t s);
+
+hipError_t launch_synth_tokens(uint64_t seed, uint64_t sc, uint64_t sn, const float *centers,
+                               uint32_t n_comp, double noise, uint64_t tok_begin, uint64_t n_tok,
+                               float *out, hipStream_t s);
+hipError_t launch_assign(const float *x, uint64_t n, const float *C, const float *bias,
+                         uint32_t n_c, uint32_t *codes, hipStream_t s);
+hipError_t launch_compress(const float *x, const uint32_t *codes, uint64_t n, const float *C,
+                           const float *cut, int per_dim, uint8_t *out, hipStream_t s);
+
+}  // namespace mvr

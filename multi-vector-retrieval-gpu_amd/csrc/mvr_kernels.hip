@@ -1,0 +1,840 @@
+// HIP kernels of the ColBERT late-interaction search path for gfx950.
+//
+// Stage map (reference file:line of the computation each kernel performs):
+//   k_centroid_topk_f32  S'[t][c] = Q_t.C_c + 1 and per-row top-P
+//                        (GenerateVQScoreCandidatePP.hpp:137-174)
+//   k_filter_seg         IVF probe, per-(doc, token) max and per-doc sum
+//                        (FilterPP.hpp:99-142, 366-379)
+//   k_select_topr        top-probe_topk by filter score (FilterPP.hpp:386-394)
+//   k_rerank_f32         fused 2-bit decompress + MaxSim + sum over tokens
+//                        (RerankPP.hpp:65-161, 416-471)
+//   k_topk_final         best-k by MaxSim score (IGPGPURetrievalPP.hpp:153-161)
+//
+// Arithmetic contract (shared with the CPU oracle so results match bit for
+// bit in FP32 mode): every dot product is a sequential fused multiply-add
+// chain over d = 0..127 from 0.0f; filter sums are exact int64 fixed point
+// (2^-32) rounded once to FP32; the MaxSim score is the left-to-right FP32 sum
+// over query tokens; all selections order by (score desc, id asc).
+#include "mvr_internal.h"
+
+#include <cfloat>
+#include <cmath>
+
+namespace mvr {
+
+namespace {
+
+constexpr uint32_t kWave = 64;
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ bool better(float s1, uint32_t i1, float s2, uint32_t i2) {
+  return (s1 > s2) || (s1 == s2 && i1 < i2);
+}
+
+// Order-preserving map of a float to uint32 (larger float -> larger uint).
+__device__ __forceinline__ uint32_t f2ord(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t o) {
+  return __uint_as_float((o & 0x80000000u) ? (o & 0x7FFFFFFFu) : ~o);
+}
+
+// Selection key: (score order bits, ~id), so that larger keys come first and
+// equal scores put the lower id first.
+__device__ __forceinline__ uint64_t make_key(float s, uint32_t id) {
+  return ((uint64_t)f2ord(s) << 32) | (uint64_t)(~id);
+}
+
+// ---------------------------------------------------------------------------
+// Centroid scoring + top-P, FP32 VALU.  Workgroup = 64 query-token rows x one
+// slice of centroids, 256 threads.  The 64 rows live in LDS transposed
+// ([d][row]); centroids stream through LDS in tiles of 64 (two halves of 64
+// dims).  Thread (tx, ty) accumulates a 4x4 register tile, each output a
+// sequential FMA chain over d.  After a tile the 64x64 scores go to LDS and
+// each wave maintains the top-P lists of 16 rows, one list element per lane
+// (lane j holds the j-th best): a candidate beating element P-1 is inserted
+// with one ballot + one shuffle.  MODE 0: score = dot + 1; MODE 1: score =
+// dot - bias[c] (k-means / code assignment, P = 1).  Each workgroup writes
+// sorted partial lists; k_topk_merge combines the centroid slices.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kRt = 64;   // rows per workgroup
+constexpr uint32_t kCt = 64;   // centroids per tile
+constexpr uint32_t kCtChunk = kCt;
+constexpr uint32_t kHalf = 64; // dims per LDS half tile
+
+__device__ __forceinline__ float readlane_f(float v, uint32_t l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ __forceinline__ uint32_t readlane_u(uint32_t v, uint32_t l) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, l);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_centroid_topk_f32(
+    const float *__restrict__ Q, const uint32_t *__restrict__ row_map, uint32_t n_rows,
+    const float *__restrict__ C, const float *__restrict__ bias, uint32_t n_c,
+    uint32_t c_per_split, uint32_t P, uint32_t *__restrict__ part_ids,
+    float *__restrict__ part_s) {
+  __shared__ __align__(16) float As[kDim][kRt + 4];
+  __shared__ __align__(16) float Bs[kHalf][kCt + 4];  // reused as the score tile [64][65]
+  const uint32_t tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const uint32_t wave = tid >> 6, lane = tid & 63;
+  const uint32_t row0 = blockIdx.x * kRt;
+
+#pragma nounroll
+  for (uint32_t k = tid; k < kRt * (kDim / 4); k += 256) {
+    const uint32_t r = k >> 5, d4 = k & 31, row = row0 + r;
+    float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (row < n_rows) {
+      const uint64_t qr = row_map ? row_map[row] : row;
+      v = reinterpret_cast<const float4 *>(Q + qr * kDim)[d4];
+    }
+    As[4 * d4 + 0][r] = v.x;
+    As[4 * d4 + 1][r] = v.y;
+    As[4 * d4 + 2][r] = v.z;
+    As[4 * d4 + 3][r] = v.w;
+  }
+
+  float ls[16];
+  uint32_t li[16];
+#pragma unroll
+  for (uint32_t rr = 0; rr < 16; ++rr) { ls[rr] = -INFINITY; li[rr] = 0xFFFFFFFFu; }
+
+  const uint32_t c_begin = blockIdx.y * c_per_split;
+  const uint32_t c_end = min(n_c, c_begin + c_per_split);
+  float *S = &Bs[0][0];
+  constexpr uint32_t kSs = kCt + 1;
+
+#pragma nounroll
+  for (uint32_t c0 = c_begin; c0 < c_end; c0 += kCt) {
+    float acc[4][4];
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i)
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) acc[i][j] = 0.f;
+#pragma unroll
+    for (uint32_t half = 0; half < 2; ++half) {
+      __syncthreads();
+#pragma nounroll
+      for (uint32_t k = tid; k < kCt * (kHalf / 4); k += 256) {
+        const uint32_t cc = k >> 4, d4 = k & 15, c = c0 + cc;
+        float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (c < c_end) v = reinterpret_cast<const float4 *>(C + (size_t)c * kDim + half * kHalf)[d4];
+        Bs[4 * d4 + 0][cc] = v.x;
+        Bs[4 * d4 + 1][cc] = v.y;
+        Bs[4 * d4 + 2][cc] = v.z;
+        Bs[4 * d4 + 3][cc] = v.w;
+      }
+      __syncthreads();
+#pragma unroll 8
+      for (uint32_t d = 0; d < kHalf; ++d) {
+        const float4 a = *reinterpret_cast<const float4 *>(&As[half * kHalf + d][ty * 4]);
+        const float4 b = *reinterpret_cast<const float4 *>(&Bs[d][tx * 4]);
+        const float av[4] = {a.x, a.y, a.z, a.w};
+        const float bv[4] = {b.x, b.y, b.z, b.w};
+#pragma unroll
+        for (uint32_t i = 0; i < 4; ++i)
+#pragma unroll
+          for (uint32_t j = 0; j < 4; ++j) acc[i][j] = __builtin_fmaf(av[i], bv[j], acc[i][j]);
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i)
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        const uint32_t c = tx * 4 + j, cid = c0 + c;
+        float s;
+        if (MODE == 0) s = acc[i][j] + 1.0f;
+        else s = bias ? acc[i][j] - bias[cid < n_c ? cid : 0] : acc[i][j];
+        S[(ty * 4 + i) * kSs + c] = s;
+      }
+    __syncthreads();
+    const uint32_t cid = c0 + lane;
+    const bool cvalid = cid < c_end;
+#pragma unroll
+    for (uint32_t rr = 0; rr < 16; ++rr) {
+      const uint32_t r = wave * 16 + rr;
+      if (row0 + r >= n_rows) continue;  // uniform per wave
+      const float s = S[r * kSs + lane];
+      float ts = readlane_f(ls[rr], P - 1);
+      uint32_t ti = readlane_u(li[rr], P - 1);
+      uint64_t bal = __ballot(cvalid && better(s, cid, ts, ti));
+      while (bal) {
+        const uint32_t l = (uint32_t)__builtin_ctzll(bal);
+        const float cs = readlane_f(s, l);
+        const uint32_t ci = c0 + l;
+        const uint32_t pos = (uint32_t)__popcll(__ballot(lane < P && better(ls[rr], li[rr], cs, ci)));
+        const float us = __shfl_up(ls[rr], 1);
+        const uint32_t ui = __shfl_up(li[rr], 1);
+        if (lane > pos) { ls[rr] = us; li[rr] = ui; }
+        else if (lane == pos) { ls[rr] = cs; li[rr] = ci; }
+        bal &= bal - 1;
+        ts = readlane_f(ls[rr], P - 1);
+        ti = readlane_u(li[rr], P - 1);
+        bal &= __ballot(cvalid && better(s, cid, ts, ti));
+      }
+    }
+  }
+#pragma unroll
+  for (uint32_t rr = 0; rr < 16; ++rr) {
+    const uint32_t row = row0 + wave * 16 + rr;
+    if (row < n_rows && lane < P) {
+      const size_t o = ((size_t)blockIdx.y * n_rows + row) * P + lane;
+      part_ids[o] = li[rr];
+      part_s[o] = ls[rr];
+    }
+  }
+}
+
+// Merge n_split sorted partial lists per row into the final top-P list.
+__global__ __launch_bounds__(256) void k_topk_merge(const uint32_t *__restrict__ part_ids,
+                                                    const float *__restrict__ part_s,
+                                                    uint32_t n_split, uint32_t n_rows, uint32_t P,
+                                                    const uint32_t *__restrict__ row_map,
+                                                    uint32_t *__restrict__ out_ids,
+                                                    float *__restrict__ out_s) {
+  const uint32_t row = blockIdx.x * 256u + threadIdx.x;
+  if (row >= n_rows) return;
+  const uint64_t orow = row_map ? row_map[row] : row;
+  uint32_t head[16];
+  for (uint32_t k = 0; k < n_split; ++k) head[k] = 0;
+  for (uint32_t j = 0; j < P; ++j) {
+    int bk = -1;
+    float bs = -INFINITY;
+    uint32_t bi = 0xFFFFFFFFu;
+    for (uint32_t k = 0; k < n_split; ++k) {
+      if (head[k] >= P) continue;
+      const size_t o = ((size_t)k * n_rows + row) * P + head[k];
+      const float s = part_s[o];
+      const uint32_t i = part_ids[o];
+      if (i == 0xFFFFFFFFu) continue;
+      if (bk < 0 || better(s, i, bs, bi)) { bk = (int)k; bs = s; bi = i; }
+    }
+    if (bk >= 0) head[bk]++;
+    out_ids[orow * P + j] = bi;
+    out_s[orow * P + j] = bs;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Filter.  Workgroup (segment, query) owns docs [seg*4096, seg*4096+4096) in
+// LDS: a 32-bit mask of query tokens already counted and an int64 fixed-point
+// accumulator.  Probe ranks are processed in order j = 0..P-1 with a barrier
+// in between; because each token's probe list is sorted by score, the first
+// rank whose IVF list holds the doc carries that token's maximum score, so the
+// atomicOr on the mask decides which hit adds (max over probes, then an exact
+// sum over tokens).  Each list's sub-range inside the segment comes from the
+// precomputed seg_off table.  Candidates are appended to the query's region of
+// the candidate pool as 64-bit selection keys.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_filter_seg(
+    const uint32_t *__restrict__ top_ids, const float *__restrict__ top_s, uint32_t t_max,
+    uint32_t P, const uint32_t *__restrict__ q_ntok, const uint32_t *__restrict__ q_np,
+    const uint64_t *__restrict__ ivf_off, const uint32_t *__restrict__ ivf_ids,
+    const uint32_t *__restrict__ seg_off, uint32_t n_seg, uint32_t n_doc,
+    const uint64_t *__restrict__ cand_off, uint32_t *__restrict__ cand_cnt,
+    uint64_t *__restrict__ cand_keys) {
+  __shared__ uint32_t mask[kSeg];
+  __shared__ unsigned long long acc[kSeg];
+  __shared__ uint64_t lbase[kMaxQTok];
+  __shared__ uint32_t lpre[kMaxQTok + 1];
+  __shared__ long long lfx[kMaxQTok];
+
+  const uint32_t seg = blockIdx.x, q = blockIdx.y;
+  const uint32_t T = q_ntok ? q_ntok[q] : t_max;
+  const uint32_t Pq = q_np ? q_np[q] : P;
+  const uint32_t seg_lo = seg * kSeg;
+  const uint32_t tid = threadIdx.x;
+
+  for (uint32_t i = tid; i < kSeg; i += 256) { mask[i] = 0u; acc[i] = 0ull; }
+
+  for (uint32_t j = 0; j < Pq; ++j) {
+    __syncthreads();
+    if (tid < kMaxQTok) {
+      uint32_t len = 0;
+      uint64_t base = 0;
+      long long fx = 0;
+      if (tid < T) {
+        const size_t o = ((size_t)q * t_max + tid) * P + j;
+        const uint32_t c = top_ids[o];
+        if (c != 0xFFFFFFFFu) {
+          const uint32_t *so = seg_off + (size_t)c * (n_seg + 1);
+          const uint32_t lo = so[seg], hi = so[seg + 1];
+          len = hi - lo;
+          base = ivf_off[c] + lo;
+          fx = (long long)((double)top_s[o] * kFxScale);
+        }
+      }
+      lbase[tid] = base;
+      lfx[tid] = fx;
+      // inclusive scan of len over the first wave-half (32 lanes)
+      uint32_t v = len;
+#pragma unroll
+      for (uint32_t off = 1; off < kMaxQTok; off <<= 1) {
+        const uint32_t u = __shfl_up(v, off, kMaxQTok);
+        if (tid >= off) v += u;
+      }
+      lpre[tid + 1] = v;
+      if (tid == 0) lpre[0] = 0;
+    }
+    __syncthreads();
+    const uint32_t total = lpre[kMaxQTok];
+    for (uint32_t i = tid; i < total; i += 256) {
+      // token t whose range holds i: largest t with lpre[t] <= i
+      uint32_t lo = 0, hi = kMaxQTok;
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (lpre[mid] <= i) lo = mid; else hi = mid;
+      }
+      const uint32_t t = lo;
+      const uint32_t doc = ivf_ids[lbase[t] + (i - lpre[t])];
+      const uint32_t dl = doc - seg_lo;
+      const uint32_t bit = 1u << t;
+      const uint32_t old = atomicOr(&mask[dl], bit);
+      if (!(old & bit)) atomicAdd(&acc[dl], (unsigned long long)lfx[t]);
+    }
+  }
+  __syncthreads();
+
+  const uint32_t lane = tid & (kWave - 1);
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (uint32_t dl = tid; dl < kSeg; dl += 256) {
+    const uint32_t doc = seg_lo + dl;
+    const bool has = doc < n_doc && mask[dl] != 0u;
+    const uint64_t bal = __ballot(has);
+    uint32_t base = 0;
+    if (lane == 0 && bal) base = atomicAdd(&cand_cnt[q], (uint32_t)__popcll(bal));
+    base = __shfl(base, 0);
+    if (has) {
+      const float F = (float)((double)(long long)acc[dl] * (1.0 / kFxScale));
+      cand_keys[cand_off[q] + base + __popcll(bal & lt)] = make_key(F, doc);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Bitonic sort (descending) of n (power of two) 64-bit keys in LDS.
+// ---------------------------------------------------------------------------
+__device__ void bitonic_desc(unsigned long long *s, uint32_t n) {
+  for (uint32_t k = 2; k <= n; k <<= 1) {
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint32_t ixj = i ^ j;
+        if (ixj > i) {
+          const unsigned long long a = s[i], b = s[ixj];
+          const bool up = (i & k) == 0;
+          if (up ? (a < b) : (a > b)) { s[i] = b; s[ixj] = a; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__device__ __forceinline__ uint32_t pow2_ceil(uint32_t n) {
+  uint32_t p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+// ---------------------------------------------------------------------------
+// Top-R selection per query over its candidate keys: MSB-first radix select
+// (11-bit digits, LDS histogram), keys above the pivot digit go straight to
+// the output; the pivot bucket is refined until it fits in LDS, then sorted.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kSelThreads = 512;
+constexpr uint32_t kSelScratch = 2048;
+
+__global__ __launch_bounds__(kSelThreads) void k_select_topr(
+    const uint64_t *__restrict__ cand_keys, const uint64_t *__restrict__ cand_off,
+    const uint32_t *__restrict__ cand_cnt, uint32_t R, uint32_t *__restrict__ sel_ids,
+    float *__restrict__ sel_F, uint32_t *__restrict__ sel_n) {
+  __shared__ unsigned long long outk[kMaxSelect];
+  __shared__ unsigned long long scr[kSelScratch];
+  __shared__ uint32_t hist[2048];
+  __shared__ uint32_t s_out_n, s_scr_n, s_digit, s_gt, s_eq;
+
+  const uint32_t q = blockIdx.x, tid = threadIdx.x;
+  const uint32_t n = cand_cnt[q];
+  const uint64_t *keys = cand_keys + cand_off[q];
+  const uint32_t n_sel = min(n, R);
+
+  if (tid == 0) { s_out_n = 0; s_scr_n = 0; }
+  __syncthreads();
+
+  if (n <= kMaxSelect) {
+    const uint32_t np = pow2_ceil(max(n, 2u));
+    for (uint32_t i = tid; i < np; i += kSelThreads) outk[i] = (i < n) ? keys[i] : 0ull;
+    __syncthreads();
+    bitonic_desc(outk, np);
+  } else {
+    // n > kMaxSelect >= R: the pivot search below always finds a digit.
+    uint64_t prefix = 0;
+    uint32_t shift = 64;
+    uint32_t need = R;
+    bool done = false;
+    while (!done) {
+      const uint32_t bits = shift >= 11 ? 11u : shift;
+      const uint32_t hi_shift = shift;  // bits above the current digit
+      shift -= bits;
+      const uint32_t nb = 1u << bits;
+      for (uint32_t b = tid; b < nb; b += kSelThreads) hist[b] = 0;
+      __syncthreads();
+      for (uint32_t i = tid; i < n; i += kSelThreads) {
+        const uint64_t k = keys[i];
+        if (hi_shift == 64 || (k >> hi_shift) == prefix)
+          atomicAdd(&hist[(uint32_t)(k >> shift) & (nb - 1)], 1u);
+      }
+      __syncthreads();
+      if (tid == 0) {
+        uint32_t cum = 0, b = nb;
+        while (b > 0) {
+          --b;
+          if (cum + hist[b] >= need) break;
+          cum += hist[b];
+        }
+        s_digit = b; s_gt = cum; s_eq = hist[b];
+      }
+      __syncthreads();
+      const uint32_t dsel = s_digit, gt = s_gt, eq = s_eq;
+      const bool take_eq = (gt + eq == need);
+      const bool eq_fits = eq <= kSelScratch;
+      for (uint32_t i = tid; i < n; i += kSelThreads) {
+        const uint64_t k = keys[i];
+        if (hi_shift == 64 || (k >> hi_shift) == prefix) {
+          const uint32_t dg = (uint32_t)(k >> shift) & (nb - 1);
+          if (dg > dsel || (dg == dsel && take_eq)) outk[atomicAdd(&s_out_n, 1u)] = k;
+          else if (dg == dsel && eq_fits) scr[atomicAdd(&s_scr_n, 1u)] = k;
+        }
+      }
+      __syncthreads();
+      need -= gt;
+      if (take_eq) {
+        done = true;
+      } else if (shift == 0 && !eq_fits) {
+        // only reachable with duplicate keys: take the first `need` of them
+        for (uint32_t i = tid; i < n; i += kSelThreads) {
+          const uint64_t k = keys[i];
+          if ((k >> hi_shift) == prefix && (uint32_t)(k & (nb - 1)) == dsel) {
+            const uint32_t slot = atomicAdd(&s_scr_n, 1u);
+            if (slot < need) outk[s_out_n + slot] = k;
+          }
+        }
+        __syncthreads();
+        done = true;
+      } else if (eq_fits) {
+        const uint32_t ns = s_scr_n;
+        const uint32_t np = pow2_ceil(max(ns, 2u));
+        for (uint32_t i = ns + tid; i < np; i += kSelThreads) scr[i] = 0ull;
+        __syncthreads();
+        bitonic_desc(scr, np);
+        const uint32_t base = s_out_n;
+        for (uint32_t i = tid; i < need; i += kSelThreads) outk[base + i] = scr[i];
+        __syncthreads();
+        done = true;
+      } else {
+        prefix = (prefix << bits) | dsel;
+      }
+    }
+    const uint32_t np = pow2_ceil(max(n_sel, 2u));
+    for (uint32_t i = n_sel + tid; i < np; i += kSelThreads) outk[i] = 0ull;
+    __syncthreads();
+    bitonic_desc(outk, np);
+  }
+  for (uint32_t i = tid; i < R; i += kSelThreads) {
+    const size_t o = (size_t)q * R + i;
+    if (i < n_sel) {
+      const unsigned long long k = outk[i];
+      sel_ids[o] = ~(uint32_t)(k & 0xFFFFFFFFull);
+      sel_F[o] = ord2f((uint32_t)(k >> 32));
+    } else {
+      sel_ids[o] = 0xFFFFFFFFu;
+      sel_F[o] = -INFINITY;
+    }
+  }
+  if (tid == 0) sel_n[q] = n_sel;
+}
+
+// ---------------------------------------------------------------------------
+// Fused decompress + MaxSim, FP32.  Workgroup (slice, query), 4 waves; each
+// wave takes candidate docs r = slice*4 + wave, stepping by 4*n_slice.  Lane
+// l owns query token t = l & 31 (its 128 values in VGPRs) and token parity
+// h = l >> 5: a step decodes two doc tokens (i, i+1) straight from the packed
+// codes into LDS (lane l writes dims 4t..4t+3 of token i+h: one code word, one
+// residual byte, one float4 of the centroid row) and each lane runs the dot
+// product of its query token with token i+h as one FMA chain, reading the
+// decoded token as LDS broadcasts.  No decompressed embedding touches HBM.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float pick4(uint32_t b, float w0, float w1, float w2, float w3) {
+  const float lo = (b & 1u) ? w1 : w0;
+  const float hi = (b & 1u) ? w3 : w2;
+  return (b & 2u) ? hi : lo;
+}
+
+__global__ __launch_bounds__(256) void k_rerank_f32(
+    const float *__restrict__ Q, uint32_t t_max, const uint32_t *__restrict__ q_ntok,
+    const uint32_t *__restrict__ sel_ids, const uint32_t *__restrict__ sel_n, uint32_t R,
+    uint32_t n_slice, const uint32_t *__restrict__ doclens, const uint64_t *__restrict__ docoff,
+    const uint32_t *__restrict__ codes, const uint8_t *__restrict__ residual,
+    const float *__restrict__ C, const float *__restrict__ W, int clip_zero,
+    float *__restrict__ sel_score) {
+  __shared__ __align__(16) float4 xs[4][2][kDim / 4];
+  __shared__ float Ms[4][kMaxQTok];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const uint32_t h = lane >> 5, t = lane & 31;
+  const uint32_t slice = blockIdx.x, q = blockIdx.y;
+  const uint32_t T = q_ntok ? q_ntok[q] : t_max;
+  const uint32_t n = sel_n[q];
+
+  float qv[kDim];
+  {
+    const float4 *q4 = reinterpret_cast<const float4 *>(Q + ((size_t)q * t_max + t) * kDim);
+#pragma unroll
+    for (uint32_t i = 0; i < kDim / 4; ++i) {
+      const float4 v = (t < T) ? q4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      qv[4 * i] = v.x; qv[4 * i + 1] = v.y; qv[4 * i + 2] = v.z; qv[4 * i + 3] = v.w;
+    }
+  }
+  // decode weights of dims 4t..4t+3
+  float w[4][4];
+#pragma unroll
+  for (uint32_t i = 0; i < 4; ++i)
+#pragma unroll
+    for (uint32_t b = 0; b < 4; ++b) w[i][b] = W[(4 * t + i) * 4 + b];
+
+  for (uint32_t r = slice * 4 + wave; r < n; r += n_slice * 4) {
+    const uint32_t doc = sel_ids[(size_t)q * R + r];
+    const uint32_t L = doclens[doc];
+    const uint64_t off = docoff[doc];
+    float M = clip_zero ? 0.0f : -FLT_MAX;
+    for (uint32_t i = 0; i < L; i += 2) {
+      const uint32_t tok = i + h;
+      const bool vt = tok < L;
+      if (vt) {
+        const uint64_t g = off + tok;
+        const uint32_t code = codes[g];
+        const uint32_t byte = residual[g * kResBytes + t];
+        const float4 c = reinterpret_cast<const float4 *>(C + (size_t)code * kDim)[t];
+        float4 x;
+        x.x = c.x + pick4((byte >> 6) & 3u, w[0][0], w[0][1], w[0][2], w[0][3]);
+        x.y = c.y + pick4((byte >> 4) & 3u, w[1][0], w[1][1], w[1][2], w[1][3]);
+        x.z = c.z + pick4((byte >> 2) & 3u, w[2][0], w[2][1], w[2][2], w[2][3]);
+        x.w = c.w + pick4(byte & 3u, w[3][0], w[3][1], w[3][2], w[3][3]);
+        xs[wave][h][t] = x;
+      }
+      wave_lds_sync();
+      if (vt) {
+        float a = 0.f;
+#pragma unroll
+        for (uint32_t d4 = 0; d4 < kDim / 4; ++d4) {
+          const float4 xv = xs[wave][h][d4];
+          a = __builtin_fmaf(qv[4 * d4 + 0], xv.x, a);
+          a = __builtin_fmaf(qv[4 * d4 + 1], xv.y, a);
+          a = __builtin_fmaf(qv[4 * d4 + 2], xv.z, a);
+          a = __builtin_fmaf(qv[4 * d4 + 3], xv.w, a);
+        }
+        M = fmaxf(M, a);
+      }
+      wave_lds_sync();
+    }
+    M = fmaxf(M, __shfl_xor(M, 32));
+    if (h == 0) Ms[wave][t] = M;
+    wave_lds_sync();
+    if (lane == 0) {
+      float sc = 0.0f;
+      for (uint32_t tt = 0; tt < T; ++tt) sc += Ms[wave][tt];
+      sel_score[(size_t)q * R + r] = sc;
+    }
+    wave_lds_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Final top-k per query: sort the n reranked (score, doc) keys in LDS.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kSelThreads) void k_topk_final(
+    const uint32_t *__restrict__ ids, const float *__restrict__ score,
+    const uint32_t *__restrict__ n_in, uint32_t R, uint32_t k, uint32_t *__restrict__ out_ids,
+    float *__restrict__ out_s) {
+  __shared__ unsigned long long s[kMaxSelect];
+  const uint32_t q = blockIdx.x, tid = threadIdx.x;
+  const uint32_t n = n_in[q];
+  const uint32_t np = pow2_ceil(max(n, 2u));
+  for (uint32_t i = tid; i < np; i += kSelThreads)
+    s[i] = (i < n) ? make_key(score[(size_t)q * R + i], ids[(size_t)q * R + i]) : 0ull;
+  __syncthreads();
+  bitonic_desc(s, np);
+  for (uint32_t i = tid; i < k; i += kSelThreads) {
+    const size_t o = (size_t)q * k + i;
+    if (i < n) {
+      out_ids[o] = ~(uint32_t)(s[i] & 0xFFFFFFFFull);
+      out_s[o] = ord2f((uint32_t)(s[i] >> 32));
+    } else {
+      out_ids[o] = 0xFFFFFFFFu;
+      out_s[o] = -INFINITY;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Index preparation.
+// ---------------------------------------------------------------------------
+// seg_off[c][s] = first position in list c whose doc id is >= s*4096.
+__global__ void k_seg_offsets(const uint64_t *__restrict__ ivf_off,
+                              const uint32_t *__restrict__ ivf_size,
+                              const uint32_t *__restrict__ ivf_ids, uint32_t n_c, uint32_t n_seg,
+                              uint32_t *__restrict__ seg_off) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t total = (uint64_t)n_c * (n_seg + 1);
+  if (g >= total) return;
+  const uint32_t c = (uint32_t)(g / (n_seg + 1)), s = (uint32_t)(g % (n_seg + 1));
+  const uint32_t *lst = ivf_ids + ivf_off[c];
+  const uint32_t sz = ivf_size[c];
+  const uint64_t key = (uint64_t)s * kSeg;
+  uint32_t lo = 0, hi = sz;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if ((uint64_t)lst[mid] < key) lo = mid + 1; else hi = mid;
+  }
+  seg_off[g] = lo;
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_decompress(const uint64_t *__restrict__ tok_ids, uint64_t n,
+                             const uint32_t *__restrict__ codes,
+                             const uint8_t *__restrict__ residual, const float *__restrict__ C,
+                             const float *__restrict__ W, float *__restrict__ out) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n * (kDim / 4)) return;
+  const uint64_t k = g / (kDim / 4);
+  const uint32_t j = (uint32_t)(g % (kDim / 4));
+  const uint64_t tok = tok_ids[k];
+  const uint32_t byte = residual[tok * kResBytes + j];
+  const float4 c = reinterpret_cast<const float4 *>(C + (size_t)codes[tok] * kDim)[j];
+  float4 x;
+  x.x = c.x + W[(4 * j + 0) * 4 + ((byte >> 6) & 3u)];
+  x.y = c.y + W[(4 * j + 1) * 4 + ((byte >> 4) & 3u)];
+  x.z = c.z + W[(4 * j + 2) * 4 + ((byte >> 2) & 3u)];
+  x.w = c.w + W[(4 * j + 3) * 4 + (byte & 3u)];
+  reinterpret_cast<float4 *>(out)[g] = x;
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic corpus and index build.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ double uniform01(uint64_t key, uint64_t i) {
+  return (double)(mix64(key + i) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+// One wave per token; lane l produces normals 2l and 2l+1 of the token with
+// one Box-Muller pair (same pairing as the CPU generator).
+__global__ __launch_bounds__(256) void k_synth_tokens(uint64_t seed, uint64_t sc, uint64_t sn,
+                                                      const float *__restrict__ centers,
+                                                      uint32_t n_comp, double noise,
+                                                      uint64_t tok_begin, uint64_t n_tok,
+                                                      float *__restrict__ out) {
+  __shared__ double red[4][kWave];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint64_t k = (uint64_t)blockIdx.x * 4 + wave;
+  const bool valid = k < n_tok;
+  const uint64_t i = tok_begin + (valid ? k : 0);
+  const uint64_t kc = mix64(seed ^ (sc * 0xD1B54A32D192ED03ull));
+  const uint64_t kn = mix64(seed ^ (sn * 0xD1B54A32D192ED03ull));
+  uint32_t comp = (uint32_t)(uniform01(kc, i) * (double)n_comp);
+  if (comp >= n_comp) comp = n_comp - 1;
+  const uint64_t pair = (i * kDim) / 2 + lane;
+  const double u1 = uniform01(kn, 2 * pair), u2 = uniform01(kn, 2 * pair + 1);
+  const double r = sqrt(-2.0 * log(1.0 - u1));
+  const double a = 6.283185307179586 * u2;
+  const double v0 = (double)centers[(size_t)comp * kDim + 2 * lane] + noise * (r * cos(a));
+  const double v1 = (double)centers[(size_t)comp * kDim + 2 * lane + 1] + noise * (r * sin(a));
+  red[wave][lane] = v0 * v0 + v1 * v1;
+  wave_lds_sync();
+  double nrm = 0.0;
+  for (uint32_t l = 0; l < kWave; ++l) nrm += red[wave][l];
+  nrm = sqrt(nrm);
+  if (valid) {
+    out[k * kDim + 2 * lane] = (float)(v0 / nrm);
+    out[k * kDim + 2 * lane + 1] = (float)(v1 / nrm);
+  }
+}
+
+// One thread per (token, packed byte): bucket = number of cutoffs <= residual
+// (first cutoff strictly greater), packed MSB-first.
+__global__ void k_compress(const float *__restrict__ x, const uint32_t *__restrict__ codes,
+                           uint64_t n, const float *__restrict__ C, const float *__restrict__ cut,
+                           int per_dim, uint8_t *__restrict__ out) {
+  const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= n * kResBytes) return;
+  const uint64_t i = g / kResBytes;
+  const uint32_t p = (uint32_t)(g % kResBytes);
+  const float *xi = x + i * kDim;
+  const float *ci = C + (size_t)codes[i] * kDim;
+  uint32_t byte = 0;
+  for (uint32_t v = 0; v < 4; ++v) {
+    const uint32_t d = 4 * p + v;
+    const float e = xi[d] - ci[d];
+    const float *ct = cut + (per_dim ? d * 3 : 0);
+    uint32_t b = 3;
+    if (ct[0] > e) b = 0;
+    else if (ct[1] > e) b = 1;
+    else if (ct[2] > e) b = 2;
+    byte = (byte << 2) | b;
+  }
+  out[g] = (uint8_t)byte;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers.
+// ---------------------------------------------------------------------------
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_fill_u32<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(p, v, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_seg_offsets(const DevIndex &ix, hipStream_t s) {
+  const uint64_t total = (uint64_t)ix.n_c * (ix.n_seg + 1);
+  k_seg_offsets<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(ix.ivf_off, ix.ivf_size,
+                                                                ix.ivf_ids, ix.n_c, ix.n_seg,
+                                                                ix.seg_off);
+  return hipGetLastError();
+}
+
+hipError_t launch_centroid_topk(const float *Q, const uint32_t *row_map, uint32_t n_rows,
+                                uint32_t n_out_rows, const DevIndex &ix, uint32_t P, int precision,
+                                uint32_t n_split, uint32_t *part_ids, float *part_s,
+                                uint32_t *top_ids, float *top_s, const uint16_t *Qbf,
+                                hipStream_t s) {
+  (void)n_out_rows;
+  (void)Qbf;
+  if (precision != 0) return hipErrorNotSupported;
+  if (n_rows == 0) return hipSuccess;
+  if (P > 64) return hipErrorInvalidValue;
+  const uint32_t cps = ((ix.n_c + n_split - 1) / n_split + kCtChunk - 1) / kCtChunk * kCtChunk;
+  const uint32_t ns = (ix.n_c + cps - 1) / cps;
+  dim3 grid((n_rows + kRt - 1) / kRt, ns);
+  k_centroid_topk_f32<0><<<grid, 256, 0, s>>>(Q, row_map, n_rows, ix.C, nullptr, ix.n_c, cps, P,
+                                               part_ids, part_s);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_topk_merge<<<(n_rows + 255) / 256, 256, 0, s>>>(part_ids, part_s, ns, n_rows, P, row_map,
+                                                    top_ids, top_s);
+  return hipGetLastError();
+}
+
+hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t_max, uint32_t P,
+                         const uint32_t *q_ntok, const uint32_t *q_np, uint32_t n_q,
+                         const DevIndex &ix, const uint64_t *cand_off, uint32_t *cand_cnt,
+                         uint64_t *cand_keys, hipStream_t s) {
+  if (n_q == 0) return hipSuccess;
+  dim3 grid(ix.n_seg, n_q);
+  k_filter_seg<<<grid, 256, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np, ix.ivf_off,
+                                    ix.ivf_ids, ix.seg_off, ix.n_seg, ix.n_doc, cand_off, cand_cnt,
+                                    cand_keys);
+  return hipGetLastError();
+}
+
+hipError_t launch_select(const uint64_t *cand_keys, const uint64_t *cand_off,
+                         const uint32_t *cand_cnt, uint32_t n_q, uint32_t R, uint32_t *sel_ids,
+                         float *sel_F, uint32_t *sel_n, hipStream_t s) {
+  if (n_q == 0) return hipSuccess;
+  k_select_topr<<<n_q, kSelThreads, 0, s>>>(cand_keys, cand_off, cand_cnt, R, sel_ids, sel_F,
+                                            sel_n);
+  return hipGetLastError();
+}
+
+hipError_t launch_rerank(const float *Q, uint32_t t_max, const uint32_t *q_ntok, uint32_t n_q,
+                         const uint32_t *sel_ids, const uint32_t *sel_n, uint32_t R,
+                         const DevIndex &ix, int precision, int clip_zero, float *sel_score,
+                         hipStream_t s) {
+  if (precision != 0) return hipErrorNotSupported;
+  if (n_q == 0) return hipSuccess;
+  const uint32_t n_slice = R >= 256 ? 4 : 1;
+  dim3 grid(n_slice, n_q);
+  k_rerank_f32<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, sel_ids, sel_n, R, n_slice, ix.doclens,
+                                    ix.docoff, ix.codes, ix.residual, ix.C, ix.W, clip_zero,
+                                    sel_score);
+  return hipGetLastError();
+}
+
+hipError_t launch_topk(const uint32_t *ids, const float *score, const uint32_t *n_in, uint32_t R,
+                       uint32_t n_q, uint32_t k, uint32_t *out_ids, float *out_s, hipStream_t s) {
+  if (n_q == 0) return hipSuccess;
+  k_topk_final<<<n_q, kSelThreads, 0, s>>>(ids, score, n_in, R, k, out_ids, out_s);
+  return hipGetLastError();
+}
+
+hipError_t launch_decompress(const uint64_t *tok_ids, uint64_t n, const DevIndex &ix, float *out,
+                             hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint64_t total = n * (kDim / 4);
+  k_decompress<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(tok_ids, n, ix.codes, ix.residual,
+                                                               ix.C, ix.W, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_synth_tokens(uint64_t seed, uint64_t sc, uint64_t sn, const float *centers,
+                               uint32_t n_comp, double noise, uint64_t tok_begin, uint64_t n_tok,
+                               float *out, hipStream_t s) {
+  if (n_tok == 0) return hipSuccess;
+  k_synth_tokens<<<(unsigned)((n_tok + 3) / 4), 256, 0, s>>>(seed, sc, sn, centers, n_comp, noise,
+                                                             tok_begin, n_tok, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_assign(const float *x, uint64_t n, const float *C, const float *bias,
+                         uint32_t n_c, uint32_t *codes, hipStream_t s) {
+  // Rows go in launches of at most 2^20 so the per-row score scratch stays small.
+  const uint32_t kRowsPerLaunch = 1u << 20;
+  float *ps = nullptr;
+  hipError_t e = hipMalloc(reinterpret_cast<void **>(&ps), sizeof(float) * kRowsPerLaunch);
+  if (e != hipSuccess) return e;
+  const uint32_t cps = (n_c + kCtChunk - 1) / kCtChunk * kCtChunk;
+  for (uint64_t r0 = 0; r0 < n; r0 += kRowsPerLaunch) {
+    const uint32_t nr = (uint32_t)((n - r0) < kRowsPerLaunch ? (n - r0) : kRowsPerLaunch);
+    dim3 grid((nr + kRt - 1) / kRt, 1);
+    k_centroid_topk_f32<1><<<grid, 256, 0, s>>>(x + r0 * kDim, nullptr, nr, C, bias, n_c, cps, 1,
+                                                codes + r0, ps);
+    e = hipGetLastError();
+    if (e != hipSuccess) break;
+  }
+  hipError_t e2 = hipStreamSynchronize(s);
+  hipError_t e3 = hipFree(ps);
+  return e != hipSuccess ? e : (e2 != hipSuccess ? e2 : e3);
+}
+
+hipError_t launch_compress(const float *x, const uint32_t *codes, uint64_t n, const float *C,
+                           const float *cut, int per_dim, uint8_t *out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint64_t total = n * kResBytes;
+  k_compress<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(x, codes, n, C, cut, per_dim, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_convert_centroids(const DevIndex &ix, hipStream_t s) {
+  (void)ix;
+  (void)s;
+  return hipSuccess;
+}
+
+}  // namespace mvr

@@ -1,0 +1,93 @@
+"""ctypes binding of libmvr (include/mvr.h).  No fallback: if the library is
+missing or fails to load, every call raises."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_DIR = os.path.join(_PKG, "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libmvr.so")
+
+MVR_OK = 0
+MVR_FP32, MVR_BF16, MVR_FP8 = 0, 1, 2
+
+
+class MvrError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"libmvr status {status}: {msg}")
+        self.status = status
+
+
+class IndexDesc(C.Structure):
+    _fields_ = [("n_doc", C.c_uint32), ("n_tok", C.c_uint64), ("n_centroid", C.c_uint32),
+                ("dim", C.c_uint32), ("n_bit", C.c_uint32), ("doclens", C.c_void_p),
+                ("centroids", C.c_void_p), ("codes", C.c_void_p), ("residual", C.c_void_p),
+                ("weights", C.c_void_p), ("weights_per_dim", C.c_uint32),
+                ("ivf_offsets", C.c_void_p), ("ivf_sizes", C.c_void_p), ("ivf_ids", C.c_void_p),
+                ("ivf_n_ele", C.c_uint64), ("on_device", C.c_int)]
+
+
+class SearchParams(C.Structure):
+    _fields_ = [("nprobe", C.c_uint32), ("probe_topk", C.c_uint32), ("topk", C.c_uint32),
+                ("precision", C.c_uint32), ("rescore", C.c_uint32), ("clip_zero", C.c_uint32),
+                ("batch", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class StageOut(C.Structure):
+    _fields_ = [("top_ids", C.c_void_p), ("top_s", C.c_void_p), ("cand_n", C.c_void_p),
+                ("sel_ids", C.c_void_p), ("sel_F", C.c_void_p), ("sel_score", C.c_void_p)]
+
+
+class Timings(C.Structure):
+    _fields_ = [("centroid_ms", C.c_double), ("filter_ms", C.c_double), ("select_ms", C.c_double),
+                ("rerank_ms", C.c_double), ("topk_ms", C.c_double), ("total_ms", C.c_double)]
+
+
+# Every symbol declared in include/mvr.h, with its ctypes signature.
+SIGNATURES = {
+    "mvr_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "mvr_last_error": (C.c_char_p, []),
+    "mvr_index_create": (C.c_int, [C.POINTER(IndexDesc), C.c_int, C.POINTER(C.c_void_p)]),
+    "mvr_index_load": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "mvr_index_free": (None, [C.c_void_p]),
+    "mvr_index_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
+                                 C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "mvr_search": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                             C.POINTER(SearchParams), C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                             C.POINTER(StageOut), C.POINTER(Timings)]),
+    "mvr_decompress": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "mvr_synth_tokens": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint32,
+                                   C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, C.c_void_p,
+                                   C.c_void_p]),
+    "mvr_assign": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p,
+                             C.c_void_p]),
+    "mvr_assign_bias": (C.c_int, [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint32,
+                                  C.c_uint32, C.c_void_p, C.c_void_p]),
+    "mvr_compress": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32,
+                               C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libmvr.so once; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() or "
+                           f"python multi-vector-retrieval-gpu_amd/build.py lib")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(status: int):
+    if status != MVR_OK:
+        raise MvrError(status, lib().mvr_last_error().decode())

@@ -1,0 +1,128 @@
+"""GPU parity of the search path against the CPU oracle (bit-exact in FP32).
+
+Every stage of the HIP pipeline is compared with the oracle on the same
+seed-generated index: probed centroid ids and scores, the number of filter
+candidates, the filter's top-R ids and scores, the reranked MaxSim scores and
+the final top-k.  Calls go through the C ABI (libmvr) via the mvr package.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _index(ix):
+    import mvr
+    return mvr.Index.from_arrays(ix.doclens, ix.C, ix.codes, ix.residual, ix.weights, ix.ivf)
+
+
+def _oracle_stages(ix, Q, P, R, k, q_ntok=None, q_nprobe=None, clip_zero=False):
+    from oracle import oracle as O
+    n_q, t_max, _ = Q.shape
+    out = []
+    for q in range(n_q):
+        T = t_max if q_ntok is None else int(q_ntok[q])
+        Pq = P if q_nprobe is None else int(q_nprobe[q])
+        top_ids, top_s = O.centroid_topk(Q[q, :T], ix.C, Pq)
+        cand, F = O.filter_all(top_ids, top_s, Pq, ix.ivf, ix.n_doc)
+        sel, selF = O.select_top(cand, F, R)
+        sc = ix.rerank(Q[q, :T], sel, clip_zero=clip_zero)
+        fid, fs = O.select_top(sel, sc, k)
+        out.append(dict(top_ids=top_ids, top_s=top_s, cand_n=cand.size, sel=sel, selF=selF, sc=sc,
+                        fid=fid, fs=fs))
+    return out
+
+
+def _check(st, ids, sc, ref, q_ntok=None, q_nprobe=None, P=None):
+    for q, r in enumerate(ref):
+        T = r["top_ids"].shape[0]
+        Pq = r["top_ids"].shape[1]
+        np.testing.assert_array_equal(st["top_ids"][q, :T, :Pq], r["top_ids"], err_msg=f"q{q} top ids")
+        np.testing.assert_array_equal(st["top_s"][q, :T, :Pq], r["top_s"], err_msg=f"q{q} top scores")
+        assert st["cand_n"][q] == r["cand_n"], f"q{q} candidates"
+        n = r["sel"].size
+        np.testing.assert_array_equal(st["sel_ids"][q, :n], r["sel"], err_msg=f"q{q} filter ids")
+        np.testing.assert_array_equal(st["sel_F"][q, :n], r["selF"], err_msg=f"q{q} filter scores")
+        np.testing.assert_array_equal(st["sel_score"][q, :n], r["sc"], err_msg=f"q{q} MaxSim")
+        kk = r["fid"].size
+        np.testing.assert_array_equal(ids[q, :kk], r["fid"], err_msg=f"q{q} top-k ids")
+        np.testing.assert_array_equal(sc[q, :kk], r["fs"], err_msg=f"q{q} top-k scores")
+        assert (ids[q, kk:] == 0xFFFFFFFF).all()
+
+
+def test_small_fp32_bit_exact(small):
+    corpus, ix, Q = small
+    g = _index(ix)
+    ids, sc, st = g.search(Q, 4, 64, 10, stages=True)
+    _check(st, ids, sc, _oracle_stages(ix, Q, 4, 64, 10))
+
+
+def test_config1_fp32_bit_exact(c1):
+    """Config 1: 8,192 docs, 1,024 centroids, 64 queries, P=4, R=256, k=10."""
+    corpus, ix, Q = c1
+    g = _index(ix)
+    ids, sc, st = g.search(Q, 4, 256, 10, stages=True)
+    _check(st, ids, sc, _oracle_stages(ix, Q, 4, 256, 10))
+
+
+def test_config1_search_batch_matches_oracle_batch(c1):
+    corpus, ix, Q = c1
+    g = _index(ix)
+    ids, sc = g.search(Q, 8, 512, 100, batch=16)
+    oi, os_ = ix.search(Q, 8, 512, 100)
+    np.testing.assert_array_equal(ids, oi)
+    np.testing.assert_array_equal(sc, os_)
+
+
+def test_ragged_tokens_and_per_query_nprobe(small):
+    """Uneven token counts and per-query nprobe (the batching workload's shape)."""
+    corpus, ix, Q = small
+    rng = np.random.default_rng(7)
+    q_ntok = rng.integers(8, 33, Q.shape[0]).astype(np.uint32)
+    q_np = rng.choice([2, 4, 8, 16, 32], Q.shape[0]).astype(np.uint32)
+    g = _index(ix)
+    ids, sc, st = g.search(Q, 2, 100, 20, q_ntok=q_ntok, q_nprobe=q_np, stages=True)
+    ref = _oracle_stages(ix, Q, 2, 100, 20, q_ntok, q_np)
+    _check(st, ids, sc, ref)
+
+
+def test_basic_variant_clip_zero(small):
+    """IGPGPUBasic semantics: per-token max clipped at 0 (zero padding)."""
+    corpus, ix, Q = small
+    g = _index(ix)
+    Qn = -Q  # negated queries make many per-token maxima negative
+    ids, sc, st = g.search(Qn, 4, 64, 10, clip_zero=True, stages=True)
+    _check(st, ids, sc, _oracle_stages(ix, Qn, 4, 64, 10, clip_zero=True))
+
+
+def test_fewer_candidates_than_probe_topk(small):
+    """R larger than the candidate count: outputs padded with (UINT32_MAX, -inf)."""
+    corpus, ix, Q = small
+    g = _index(ix)
+    ids, sc, st = g.search(Q[:4], 1, 1024, 1024, stages=True)
+    ref = _oracle_stages(ix, Q[:4], 1, 1024, 1024)
+    _check(st, ids, sc, ref)
+    for q, r in enumerate(ref):
+        assert r["cand_n"] < 1024
+        assert np.isneginf(sc[q, r["cand_n"]:]).all()
+
+
+def test_decompress_bit_exact(c1):
+    corpus, ix, Q = c1
+    g = _index(ix)
+    docs = np.array([0, 5, 8191, 17, 17], np.uint32)
+    out = g.decompress(docs)
+    ref = np.concatenate([ix.decompress_doc(int(d)) for d in docs])
+    np.testing.assert_array_equal(out, ref)
+
+
+def test_invalid_arguments_raise(small):
+    import mvr
+    corpus, ix, Q = small
+    g = _index(ix)
+    with pytest.raises(mvr.MvrError):
+        g.search(Q, 4, 5, 10)            # probe_topk < topk
+    with pytest.raises(mvr.MvrError):
+        g.search(Q, 200, 64, 10)         # nprobe > n_centroid
+    with pytest.raises(mvr.MvrError):
+        g.search(np.zeros((1, 33, 128), np.float32), 4, 64, 10)

@@ -1,0 +1,99 @@
+"""Pins of the oracle's codec against known answers and closed forms.
+
+Reference: ResidualScalarQuantizationPack.hpp:17-237 (quantiles, bucket rule,
+MSB-first packing), DataStructPP.hpp:147-205 (decode table), RerankPP.hpp:65-113
+(x = C[code] + w[bucket]).
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def test_unpack_known_answer_msb_first():
+    # byte 0b00011011 holds buckets (0, 1, 2, 3) for dims 4j..4j+3
+    dim = 128
+    res = np.zeros((1, 32), np.uint8)
+    res[0, 5] = 0b00011011
+    C = np.zeros((1, dim), np.float32)
+    W = O.decode_table(np.array([10.0, 11.0, 12.0, 13.0], np.float32), dim, 2)
+    x = O.decompress_tokens(np.array([0], np.uint64), np.array([0], np.uint32), res, C, W, 2)[0]
+    np.testing.assert_array_equal(x[20:24], [10.0, 11.0, 12.0, 13.0])
+    assert (x[:20] == 10.0).all() and (x[24:] == 10.0).all()
+
+
+def test_bucket_rule_first_cutoff_strictly_greater():
+    cut = np.array([-1.0, 0.0, 1.0], np.float32)
+    cases = {-2.0: 0, -1.0: 1, -0.5: 1, 0.0: 2, 0.5: 2, 1.0: 3, 2.0: 3}
+    for e, b in cases.items():
+        assert O.lib().orc_bucket_of(cut, 3, e) == b, e
+
+
+def test_pooled_quantiles_known_answer():
+    # 8 values 0..7: cutoffs at ranks floor(i/4*8) = 2,4,6; weights at floor((i+.5)/4*8) = 1,3,5,7
+    v = np.array([5, 2, 7, 0, 3, 6, 1, 4], np.float32)
+    cut, w = O.quantize_pooled(v, 2)
+    np.testing.assert_array_equal(cut, [2, 4, 6])
+    np.testing.assert_array_equal(w, [1, 3, 5, 7])
+
+
+def test_pooled_quantiles_fraction_below_cutoff():
+    rng = np.random.default_rng(0)
+    v = rng.standard_normal(10007).astype(np.float32)
+    cut, w = O.quantize_pooled(v, 2)
+    for i, c in enumerate(cut, start=1):
+        assert (v < c).sum() == int(np.floor(i / 4 * v.size))  # distinct values: rank = count below
+    assert (np.diff(cut) > 0).all() and (np.diff(w) > 0).all()
+    assert w[0] < cut[0] < w[1] < cut[1] < w[2] < cut[2] < w[3]
+
+
+def test_perdim_quantiles_and_bucket_means():
+    rng = np.random.default_rng(1)
+    n, dim = 4001, 128
+    e = (rng.standard_normal((n, dim)) * np.linspace(0.01, 0.2, dim)).astype(np.float32)
+    cut, w = O.quantize_perdim(e, 2)
+    for d in (0, 63, 127):
+        col = e[:, d]
+        for i in range(3):
+            assert (col < cut[d, i]).sum() == int(np.floor((i + 1) / 4 * n))
+        b = np.searchsorted(cut[d], col, side="right")   # count of cutoffs <= value
+        for k in range(4):
+            np.testing.assert_allclose(w[d, k], col[b == k].astype(np.float64).mean(), rtol=1e-6)
+
+
+def test_compress_known_answer():
+    dim = 128
+    C = np.zeros((2, dim), np.float32)
+    C[1] = 0.5
+    cut = np.array([-0.1, 0.0, 0.1], np.float32)
+    # buckets cycle 3,2,1,0 over dims -> byte 0b11100100 = 0xE4
+    e = np.tile(np.array([0.2, 0.05, -0.05, -0.2], np.float32), dim // 4)
+    x = np.stack([C[0] + e, C[1] + e])
+    codes = np.array([0, 1], np.uint32)
+    out = O.compress(x, codes, C, cut, 2)
+    assert out.shape == (2, 32)
+    assert (out == 0xE4).all()
+    # per-dimension cutoffs: dim 0 moved so that 0.2 falls in bucket 2
+    cutd = np.tile(cut, (dim, 1))
+    cutd[0] = [-0.1, 0.0, 0.3]
+    out2 = O.compress(x, codes, C, cutd, 2)
+    assert out2[0, 0] == 0b10100100 and (out2[:, 1:] == 0xE4).all()
+
+
+def test_encode_decode_roundtrip_error_bounded():
+    """Decoding an encoded vector returns C + weight of its bucket; with
+    bucket-mean weights the error per dim is bounded by the bucket width."""
+    rng = np.random.default_rng(2)
+    dim, n = 128, 500
+    C = rng.standard_normal((4, dim)).astype(np.float32)
+    codes = rng.integers(0, 4, n).astype(np.uint32)
+    e = (rng.standard_normal((n, dim)) * 0.05).astype(np.float32)
+    x = C[codes] + e
+    cut, w = O.quantize_perdim(e, 2)
+    res = O.compress(x, codes, C, cut, 2)
+    W = O.decode_table(w, dim, 2)
+    xd = O.decompress_tokens(np.arange(n, dtype=np.uint64), codes, res, C, W, 2)
+    b = np.stack([np.searchsorted(cut[d], (x[:, d] - C[codes, d]), side="right") for d in range(dim)], 1)
+    np.testing.assert_array_equal(xd, (C[codes] + W[np.arange(dim)[None, :], b]).astype(np.float32))
+    inner = (b == 1) | (b == 2)
+    assert np.abs(xd - x)[inner].max() <= (cut[:, 2] - cut[:, 0]).max()
