@@ -88,9 +88,12 @@ typedef struct {
   float *sel_score;    /* [n_q][probe_topk] their MaxSim scores (final precision) */
 } mvr_stage_out;
 
-/* Per-call timings in milliseconds (device event time per stage, summed over passes). */
+/* Per-call device time per stage in milliseconds (HIP events on the call's
+ * stream, summed over passes) and the work done: reranked doc tokens (sum of
+ * the lengths of every reranked doc) and filter candidates, summed over queries. */
 typedef struct {
   double centroid_ms, filter_ms, select_ms, rerank_ms, topk_ms, total_ms;
+  uint64_t rerank_tokens, candidates;
 } mvr_timings;
 
 int mvr_device_count(int *count);
@@ -119,6 +122,15 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
 /* FP32 decompression of whole documents: out receives sum(doclens[doc_ids])
  * rows of dim floats, documents in the given order (host buffers). */
 int mvr_decompress(mvr_index *ix, const uint32_t *doc_ids, uint32_t n, float *out);
+
+/* FP64 exhaustive MaxSim over raw FP32 embeddings (the accuracy oracle recall
+ * is measured against; Distance.hpp:200-218 vectorSetDistance summed over
+ * query tokens): out[q][doc] = sum_{t < q_ntok[q]} max_i <Q[q][t], X_i> in
+ * double.  Device buffers: Q [n_q][t_max][128] f32, q_ntok [n_q] (NULL: t_max),
+ * X [sum(doclens)][128] f32, doclens [n_doc], out [n_q][n_doc] f64. */
+int mvr_exhaustive_f64(const float *Q_dev, uint32_t n_q, uint32_t t_max, const uint32_t *q_ntok_dev,
+                       const float *X_dev, const uint32_t *doclens_dev, uint32_t n_doc,
+                       double *out_dev, void *stream);
 
 /* ---- synthetic corpus and index build on the device (bench and tests) ---- */
 

@@ -69,12 +69,22 @@ def build_pymod() -> str:
     return out
 
 
+def build_probe() -> str:
+    """Test-only MFMA layout probe (tests/probes/libmvr_probe.so)."""
+    src = os.path.join(ROOT, "tests", "probes", "mfma_probe.hip")
+    out = os.path.join(ROOT, "tests", "probes", "libmvr_probe.so")
+    _run([HIPCC, *HIP_FLAGS, "-shared", src, "-o", out])
+    return out
+
+
 def build_all():
     build_lib()
     build_cli()
     build_pymod()
+    build_probe()
 
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
-    {"lib": build_lib, "cli": build_cli, "pymod": build_pymod, "all": build_all}[what]()
+    {"lib": build_lib, "cli": build_cli, "pymod": build_pymod, "probe": build_probe,
+     "all": build_all}[what]()

@@ -94,12 +94,13 @@ struct HBuf {
 
 struct Workspace {
   DBuf Q, meta, part_ids, part_s, top_ids, top_s, cand_cnt, cand_keys, sel_ids, sel_F, sel_n,
-      sel_score, out_ids, out_s;
+      sel_score, out_ids, out_s, counters, res_ids, res_s, res_n, bmax;
   HBuf staging;
   hipEvent_t staging_done = nullptr;
   void release() {
     for (DBuf *b : {&Q, &meta, &part_ids, &part_s, &top_ids, &top_s, &cand_cnt, &cand_keys,
-                    &sel_ids, &sel_F, &sel_n, &sel_score, &out_ids, &out_s})
+                    &sel_ids, &sel_F, &sel_n, &sel_score, &out_ids, &out_s, &counters, &res_ids,
+                    &res_s, &res_n, &bmax})
       b->release();
     staging.release();
     if (staging_done) (void)hipEventDestroy(staging_done);
@@ -129,11 +130,19 @@ struct DeviceGuard {
   }
 };
 
+// float -> bfloat16 bits, round to nearest even (finite inputs)
+uint16_t f32_to_bf16(float f) {
+  uint32_t b;
+  std::memcpy(&b, &f, 4);
+  b += 0x7FFFu + ((b >> 16) & 1u);
+  return (uint16_t)(b >> 16);
+}
+
 void free_devindex(DevIndex &d) {
   for (void *p : {(void *)d.C, (void *)d.C_bf16, (void *)d.C_fp8, (void *)d.C_fp8_scale,
                   (void *)d.codes, (void *)d.residual, (void *)d.W, (void *)d.doclens,
                   (void *)d.docoff, (void *)d.ivf_off, (void *)d.ivf_size, (void *)d.ivf_ids,
-                  (void *)d.seg_off})
+                  (void *)d.seg_off, (void *)d.lut_bf16})
     if (p) (void)hipFree(p);
   d = DevIndex();
 }
@@ -238,8 +247,25 @@ int mvr_index_create(const mvr_index_desc *desc, int device, mvr_index **out) {
   UP(d.ivf_size, desc->ivf_sizes, d.n_c);
   UP(d.ivf_ids, desc->ivf_ids, d.n_ele);
 #undef UP
+  // bf16 residual table: entry [pos][byte] holds the four decoded values of
+  // dims 4*pos .. 4*pos+3 (MSB-first buckets), two bf16 per 32-bit word
+  std::vector<uint32_t> lut(kResBytes * 256 * 2);
+  for (uint32_t pos = 0; pos < kResBytes; ++pos)
+    for (uint32_t byte = 0; byte < 256; ++byte) {
+      uint16_t v[4];
+      for (uint32_t k = 0; k < 4; ++k) {
+        const uint32_t dd = 4 * pos + k, b = (byte >> (6 - 2 * k)) & 3u;
+        v[k] = f32_to_bf16(W[dd * 4 + b]);
+      }
+      lut[(pos * 256 + byte) * 2 + 0] = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+      lut[(pos * 256 + byte) * 2 + 1] = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+    }
   hipError_t e = upload(&d.docoff, docoff.data(), docoff.size(), false);
   if (e == hipSuccess) e = upload(&d.W, W.data(), W.size(), false);
+  if (e == hipSuccess)
+    e = upload(reinterpret_cast<uint32_t **>(&d.lut_bf16), lut.data(), lut.size(), false);
+  if (e == hipSuccess)
+    e = hipMalloc(reinterpret_cast<void **>(&d.C_bf16), (size_t)d.n_c * kDim * sizeof(uint16_t));
   if (e == hipSuccess)
     e = hipMalloc(reinterpret_cast<void **>(&d.seg_off), (size_t)d.n_c * (d.n_seg + 1) * sizeof(uint32_t));
   if (e == hipSuccess) e = launch_seg_offsets(d, nullptr);
@@ -350,7 +376,12 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   if (R < k) return fail(MVR_ERR_INVALID, "probe_topk %u is smaller than topk %u", R, k);
   if (k == 0) return fail(MVR_ERR_INVALID, "topk must be positive");
   if (R > kMaxSelect) return fail(MVR_ERR_UNSUPPORTED, "probe_topk %u > %u", R, kMaxSelect);
-  if (prm->precision != MVR_FP32) return fail(MVR_ERR_UNSUPPORTED, "precision %u not built", prm->precision);
+  if (prm->precision != MVR_FP32 && prm->precision != MVR_BF16)
+    return fail(MVR_ERR_UNSUPPORTED, "precision %u not built", prm->precision);
+  // BF16/FP8: the best `rescore` docs by the low-precision MaxSim are scored
+  // again in FP32 and the final top-k comes from those FP32 scores
+  const uint32_t R2 = (prm->precision != MVR_FP32 && prm->rescore) ? std::min(prm->rescore, R) : 0;
+  if (R2 && R2 < k) return fail(MVR_ERR_INVALID, "rescore %u is smaller than topk %u", R2, k);
   if (stages && on_device) return fail(MVR_ERR_INVALID, "stage outputs need host buffers");
   uint32_t P = prm->nprobe;
   std::vector<uint32_t> np(n_q), nt(n_q);
@@ -427,26 +458,31 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   HIP_TRY(ws.sel_F.ensure(Bc * R * 4));
   HIP_TRY(ws.sel_n.ensure(Bc * 4));
   HIP_TRY(ws.sel_score.ensure(Bc * R * 4));
+  if (prm->precision != MVR_FP32) HIP_TRY(ws.bmax.ensure(Bc * t_max * ((d.n_c + 31) / 32) * 4));
+  if (R2) {
+    HIP_TRY(ws.res_ids.ensure(Bc * R2 * 4));
+    HIP_TRY(ws.res_s.ensure(Bc * R2 * 4));
+    HIP_TRY(ws.res_n.ensure(Bc * 4));
+  }
   if (!on_device) {
     HIP_TRY(ws.Q.ensure(Bc * t_max * kDim * 4));
     HIP_TRY(ws.out_ids.ensure(Bc * k * 4));
     HIP_TRY(ws.out_s.ensure(Bc * k * 4));
   }
 
-  hipEvent_t ev[6] = {};
+  // Stage timings: six events per device pass, read once at the end of the
+  // call so that timing adds no host synchronisation between passes.
+  const uint32_t n_pass = (n_q + B - 1) / B;
+  std::vector<hipEvent_t> ev;
   if (timings) {
+    ev.resize((size_t)n_pass * 6);
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
     *timings = mvr_timings{};
+    HIP_TRY(ws.counters.ensure(16));
+    HIP_TRY(launch_fill_u32(ws.counters.as<uint32_t>(), 0u, 4, s));
   }
-  auto rec = [&](int i) -> hipError_t { return timings ? hipEventRecord(ev[i], s) : hipSuccess; };
-  auto acc_ms = [&](double &dst, int a, int b) -> hipError_t {
-    if (!timings) return hipSuccess;
-    hipError_t e = hipEventSynchronize(ev[b]);
-    if (e != hipSuccess) return e;
-    float ms = 0.f;
-    e = hipEventElapsedTime(&ms, ev[a], ev[b]);
-    dst += ms;
-    return e;
+  auto rec = [&](uint32_t pass, int i) -> hipError_t {
+    return timings ? hipEventRecord(ev[(size_t)pass * 6 + i], s) : hipSuccess;
   };
 
   int bi = 0;
@@ -473,24 +509,40 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     const uint32_t *bnt = d_nt + b0, *bnp = d_np + b0;
     const uint64_t *bcoff = d_coff + b0;
 
-    HIP_TRY(rec(0));
-    HIP_TRY(launch_centroid_topk(Qb, rm, n_rows, nb * t_max, d, P, prm->precision, n_split,
-                                 ws.part_ids.as<uint32_t>(), ws.part_s.as<float>(),
-                                 ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), nullptr, s));
-    HIP_TRY(rec(1));
+    HIP_TRY(rec(bi, 0));
+    if (prm->precision == MVR_FP32)
+      HIP_TRY(launch_centroid_topk(Qb, rm, n_rows, nb * t_max, d, P, prm->precision, n_split,
+                                   ws.part_ids.as<uint32_t>(), ws.part_s.as<float>(),
+                                   ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), nullptr, s));
+    else
+      HIP_TRY(launch_centroid_topk_bf16(Qb, nb, t_max, rm, n_rows, d, P, ws.bmax.as<float>(),
+                                        ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), s));
+    HIP_TRY(rec(bi, 1));
     HIP_TRY(launch_fill_u32(ws.cand_cnt.as<uint32_t>(), 0u, nb, s));
     HIP_TRY(launch_filter(ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), t_max, P, bnt, bnp, nb,
                           d, bcoff, ws.cand_cnt.as<uint32_t>(), ws.cand_keys.as<uint64_t>(), s));
-    HIP_TRY(rec(2));
+    HIP_TRY(rec(bi, 2));
     HIP_TRY(launch_select(ws.cand_keys.as<uint64_t>(), bcoff, ws.cand_cnt.as<uint32_t>(), nb, R,
                           ws.sel_ids.as<uint32_t>(), ws.sel_F.as<float>(), ws.sel_n.as<uint32_t>(), s));
-    HIP_TRY(rec(3));
+    HIP_TRY(rec(bi, 3));
     HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R,
                           d, prm->precision, (int)prm->clip_zero, ws.sel_score.as<float>(), s));
-    HIP_TRY(rec(4));
-    HIP_TRY(launch_topk(ws.sel_ids.as<uint32_t>(), ws.sel_score.as<float>(), ws.sel_n.as<uint32_t>(),
-                        R, nb, k, oid, os, s));
-    HIP_TRY(rec(5));
+    if (R2) {
+      // best R2 by the low-precision score, FP32 MaxSim of those, top-k of the FP32 scores
+      HIP_TRY(launch_topk(ws.sel_ids.as<uint32_t>(), ws.sel_score.as<float>(), ws.sel_n.as<uint32_t>(),
+                          R, nb, R2, ws.res_ids.as<uint32_t>(), ws.res_s.as<float>(),
+                          ws.res_n.as<uint32_t>(), s));
+      HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2,
+                            d, MVR_FP32, (int)prm->clip_zero, ws.res_s.as<float>(), s));
+    }
+    HIP_TRY(rec(bi, 4));
+    if (R2)
+      HIP_TRY(launch_topk(ws.res_ids.as<uint32_t>(), ws.res_s.as<float>(), ws.res_n.as<uint32_t>(), R2,
+                          nb, k, oid, os, nullptr, s));
+    else
+      HIP_TRY(launch_topk(ws.sel_ids.as<uint32_t>(), ws.sel_score.as<float>(), ws.sel_n.as<uint32_t>(),
+                          R, nb, k, oid, os, nullptr, s));
+    HIP_TRY(rec(bi, 5));
     if (!on_device) {
       HIP_TRY(hipMemcpyAsync(out_ids + (size_t)b0 * k, oid, (size_t)nb * k * 4, hipMemcpyDeviceToHost, s));
       HIP_TRY(hipMemcpyAsync(out_scores + (size_t)b0 * k, os, (size_t)nb * k * 4, hipMemcpyDeviceToHost, s));
@@ -510,17 +562,28 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       if (stages->sel_score)
         HIP_TRY(hipMemcpy(stages->sel_score + (size_t)b0 * R, ws.sel_score.p, (size_t)nb * R * 4, hipMemcpyDeviceToHost));
     }
-    if (timings) {
-      HIP_TRY(acc_ms(timings->centroid_ms, 0, 1));
-      HIP_TRY(acc_ms(timings->filter_ms, 1, 2));
-      HIP_TRY(acc_ms(timings->select_ms, 2, 3));
-      HIP_TRY(acc_ms(timings->rerank_ms, 3, 4));
-      HIP_TRY(acc_ms(timings->topk_ms, 4, 5));
-      HIP_TRY(acc_ms(timings->total_ms, 0, 5));
-    }
+    if (timings)
+      HIP_TRY(launch_count_work(ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R, nb, d,
+                                ws.cand_cnt.as<uint32_t>(), ws.counters.as<unsigned long long>(), s));
   }
-  if (timings)
+  if (timings) {
+    HIP_TRY(hipEventSynchronize(ev.back()));
+    for (uint32_t b = 0; b < n_pass; ++b) {
+      float ms[5];
+      for (int i = 0; i < 5; ++i) HIP_TRY(hipEventElapsedTime(&ms[i], ev[b * 6 + i], ev[b * 6 + i + 1]));
+      timings->centroid_ms += ms[0];
+      timings->filter_ms += ms[1];
+      timings->select_ms += ms[2];
+      timings->rerank_ms += ms[3];
+      timings->topk_ms += ms[4];
+      timings->total_ms += ms[0] + ms[1] + ms[2] + ms[3] + ms[4];
+    }
+    unsigned long long cnt[2] = {0, 0};
+    HIP_TRY(hipMemcpy(cnt, ws.counters.p, 16, hipMemcpyDeviceToHost));
+    timings->rerank_tokens = cnt[0];
+    timings->candidates = cnt[1];
     for (auto &e : ev) (void)hipEventDestroy(e);
+  }
   if (!on_device) HIP_TRY(hipStreamSynchronize(s));
   return MVR_OK;
 }
@@ -547,6 +610,28 @@ int mvr_decompress(mvr_index *ix, const uint32_t *doc_ids, uint32_t n, float *ou
   (void)hipFree(dt);
   if (dout) (void)hipFree(dout);
   if (e != hipSuccess) return fail(MVR_ERR_DEVICE, "decompress failed: %s", hipGetErrorString(e));
+  return MVR_OK;
+}
+
+int mvr_exhaustive_f64(const float *Q_dev, uint32_t n_q, uint32_t t_max, const uint32_t *q_ntok_dev,
+                       const float *X_dev, const uint32_t *doclens_dev, uint32_t n_doc,
+                       double *out_dev, void *stream) {
+  if (!Q_dev || !X_dev || !doclens_dev || !out_dev) return fail(MVR_ERR_INVALID, "NULL argument");
+  if (t_max == 0 || t_max > kMaxQTok) return fail(MVR_ERR_INVALID, "t_max %u not in [1, 32]", t_max);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  std::vector<uint32_t> lens(n_doc);
+  HIP_TRY(hipMemcpy(lens.data(), doclens_dev, (size_t)n_doc * 4, hipMemcpyDeviceToHost));
+  std::vector<uint64_t> off(n_doc);
+  uint64_t acc = 0;
+  for (uint32_t i = 0; i < n_doc; ++i) { off[i] = acc; acc += lens[i]; }
+  uint64_t *doff = nullptr;
+  HIP_TRY(hipMalloc(reinterpret_cast<void **>(&doff), std::max<size_t>(1, n_doc) * 8));
+  hipError_t e = hipMemcpy(doff, off.data(), (size_t)n_doc * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = launch_exhaustive_f64(Q_dev, n_q, t_max, q_ntok_dev, X_dev, doclens_dev, doff, n_doc, out_dev, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(doff);
+  if (e != hipSuccess) return fail(MVR_ERR_DEVICE, "exhaustive MaxSim failed: %s", hipGetErrorString(e));
   return MVR_OK;
 }
 

@@ -33,6 +33,7 @@ struct DevIndex {
   uint32_t *ivf_size = nullptr;// [n_c]
   uint32_t *ivf_ids = nullptr; // [n_ele]
   uint32_t *seg_off = nullptr; // [n_c][n_seg+1] list index where each segment starts
+  uint2 *lut_bf16 = nullptr;   // [32][256]: 4 bf16 residual values of one packed byte per position
 };
 
 // ---- launchers (return hipError_t of the launch) ----
@@ -46,6 +47,13 @@ hipError_t launch_centroid_topk(const float *Q, const uint32_t *row_map, uint32_
                                 uint32_t n_split, uint32_t *part_ids, float *part_s,
                                 uint32_t *top_ids, float *top_s, const uint16_t *Qbf,
                                 hipStream_t s);
+
+// BF16 centroid scoring: MFMA tile maxima into bmax [n_q*t_max][ceil(n_c/32)],
+// then per row exact top-P over the best P+2 tiles (bf16 inputs, FP32 FMA chain).
+hipError_t launch_centroid_topk_bf16(const float *Q, uint32_t n_q, uint32_t t_max,
+                                     const uint32_t *row_map, uint32_t n_rows, const DevIndex &ix,
+                                     uint32_t P, float *bmax, uint32_t *top_ids, float *top_s,
+                                     hipStream_t s);
 
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 
@@ -63,8 +71,19 @@ hipError_t launch_rerank(const float *Q, uint32_t t_max, const uint32_t *q_ntok,
                          const DevIndex &ix, int precision, int clip_zero, float *sel_score,
                          hipStream_t s);
 
+// best k of n_in[q] (score, id) pairs; out_n (nullable) receives min(n_in, k)
 hipError_t launch_topk(const uint32_t *ids, const float *score, const uint32_t *n_in, uint32_t R,
-                       uint32_t n_q, uint32_t k, uint32_t *out_ids, float *out_s, hipStream_t s);
+                       uint32_t n_q, uint32_t k, uint32_t *out_ids, float *out_s, uint32_t *out_n,
+                       hipStream_t s);
+
+// counters[0] += sum of doclens over the selected docs, counters[1] += candidates
+hipError_t launch_count_work(const uint32_t *sel_ids, const uint32_t *sel_n, uint32_t R, uint32_t n_q,
+                             const DevIndex &ix, const uint32_t *cand_cnt,
+                             unsigned long long *counters, hipStream_t s);
+
+hipError_t launch_exhaustive_f64(const float *Q, uint32_t n_q, uint32_t t_max,
+                                 const uint32_t *q_ntok, const float *X, const uint32_t *doclens,
+                                 const uint64_t *docoff, uint32_t n_doc, double *out, hipStream_t s);
 
 hipError_t launch_decompress(const uint64_t *tok_ids, uint64_t n, const DevIndex &ix, float *out,
                              hipStream_t s);

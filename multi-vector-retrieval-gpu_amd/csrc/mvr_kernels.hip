@@ -17,6 +17,7 @@
 // over query tokens; all selections order by (score desc, id asc).
 #include "mvr_internal.h"
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 
@@ -557,12 +558,318 @@ __global__ __launch_bounds__(256) void k_rerank_f32(
 }
 
 // ---------------------------------------------------------------------------
+// Fused decompress + MaxSim on MFMA (BF16 inputs, FP32 accumulation).
+// x_i = C[code_i] + r_i is never formed: S = C_rows.Q^T + R.Q^T with both
+// products on v_mfma_f32_32x32x16_bf16 into one accumulator.  A operand =
+// 32 doc tokens (rows), B operand = the 32 query tokens (columns, fixed per
+// wave).  Lane l = (row/col c = l & 31, half h = l >> 5): it loads its token's
+// code word, 16 residual bytes and 128 bytes of the bf16 centroid row (dims
+// [64h, 64h+64)), so the k index is permuted identically in A and B: step s,
+// half h covers dims 64h + 8s .. +7.  The residual operand comes from a
+// per-byte-position table in LDS (32 positions x 256 byte values x 4 bf16 =
+// 64 KB): one 8-byte LDS read decodes four dims.  The accumulator leaves each
+// lane 16 token rows of one query-token column, so the max over tokens is
+// local plus one cross-half shuffle.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float floatx16_t __attribute__((ext_vector_type(16)));
+
+__device__ __forceinline__ bf16x8_t as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8_t, v); }
+
+__global__ __launch_bounds__(256) void k_rerank_bf16(
+    const float *__restrict__ Q, uint32_t t_max, const uint32_t *__restrict__ q_ntok,
+    const uint32_t *__restrict__ sel_ids, const uint32_t *__restrict__ sel_n, uint32_t R,
+    uint32_t n_slice, const uint32_t *__restrict__ doclens, const uint64_t *__restrict__ docoff,
+    const uint32_t *__restrict__ codes, const uint8_t *__restrict__ residual,
+    const uint16_t *__restrict__ Cb, const uint2 *__restrict__ lut_g, int clip_zero,
+    float *__restrict__ sel_score) {
+  __shared__ uint2 lut[kResBytes * 256];
+  __shared__ float Ms[4][kMaxQTok];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const uint32_t h = lane >> 5, col = lane & 31;
+  const uint32_t slice = blockIdx.x, q = blockIdx.y;
+  const uint32_t T = q_ntok ? q_ntok[q] : t_max;
+  const uint32_t n = sel_n[q];
+
+  for (uint32_t i = tid; i < kResBytes * 256; i += 256) lut[i] = lut_g[i];
+  bf16x8_t qf[8];
+  {
+    const float *qr = Q + ((size_t)q * t_max + col) * kDim + 64 * h;
+#pragma unroll
+    for (uint32_t s = 0; s < 8; ++s) {
+      bf16x8_t v;
+#pragma unroll
+      for (uint32_t e = 0; e < 8; ++e) v[e] = (__bf16)(col < T ? qr[8 * s + e] : 0.0f);
+      qf[s] = v;
+    }
+  }
+  __syncthreads();
+
+  for (uint32_t r = slice * 4 + wave; r < n; r += n_slice * 4) {
+    const uint32_t doc = sel_ids[(size_t)q * R + r];
+    const uint32_t L = doclens[doc];
+    const uint64_t off = docoff[doc];
+    float M = clip_zero ? 0.0f : -FLT_MAX;
+    for (uint32_t i0 = 0; i0 < L; i0 += 32) {
+      const uint32_t i = i0 + col;
+      uint4 cv[8];
+      uint4 rb = make_uint4(0u, 0u, 0u, 0u);
+      if (i < L) {
+        const uint64_t g = off + i;
+        const uint32_t code = codes[g];
+        rb = *reinterpret_cast<const uint4 *>(residual + g * kResBytes + 16 * h);
+        const uint4 *cp = reinterpret_cast<const uint4 *>(Cb + (size_t)code * kDim + 64 * h);
+#pragma unroll
+        for (uint32_t s = 0; s < 8; ++s) cv[s] = cp[s];
+      } else {
+#pragma unroll
+        for (uint32_t s = 0; s < 8; ++s) cv[s] = make_uint4(0u, 0u, 0u, 0u);
+      }
+      const uint32_t rw[4] = {rb.x, rb.y, rb.z, rb.w};
+      floatx16_t acc = {};
+#pragma unroll
+      for (uint32_t s = 0; s < 8; ++s) {
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(cv[s]), qf[s], acc, 0, 0, 0);
+        const uint32_t word = rw[s >> 1];
+        const uint32_t b0 = (word >> (16 * (s & 1))) & 0xFFu;
+        const uint32_t b1 = (word >> (16 * (s & 1) + 8)) & 0xFFu;
+        const uint32_t pos = 16 * h + 2 * s;
+        const uint2 r0 = lut[pos * 256 + b0];
+        const uint2 r1 = lut[(pos + 1) * 256 + b1];
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(make_uint4(r0.x, r0.y, r1.x, r1.y)),
+                                                      qf[s], acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < 16; ++j) {
+        const uint32_t m = 8 * (j >> 2) + 4 * h + (j & 3);
+        if (i0 + m < L) M = fmaxf(M, acc[j]);
+      }
+    }
+    M = fmaxf(M, __shfl_xor(M, 32));
+    if (h == 0) Ms[wave][col] = M;
+    wave_lds_sync();
+    if (lane == 0) {
+      float sc = 0.0f;
+      for (uint32_t tt = 0; tt < T; ++tt) sc += Ms[wave][tt];
+      sel_score[(size_t)q * R + r] = sc;
+    }
+    wave_lds_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Centroid scoring + top-P in BF16, pass 1: block maxima on MFMA.
+// Workgroup = 8 waves = 8 queries x one slice of centroid tiles (32 centroids
+// each).  A tile (32 rows x 128 bf16) is staged once in LDS and read by all 8
+// waves as the A operand; each wave's B operand is its query's 32 tokens.
+// For each (query token, tile) the kernel stores only the tile maximum.  The
+// top-P centroids of a token always lie in its top-P tiles by maximum (a
+// tile holding one of the P best scores has a maximum at least that large),
+// so pass 2 only rescans those tiles.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kBmWaves = 8;
+
+__global__ __launch_bounds__(512) void k_ct_bf16_blockmax(
+    const float *__restrict__ Q, uint32_t t_max, uint32_t n_q, const uint16_t *__restrict__ Cb,
+    uint32_t n_c, uint32_t n_tiles, uint32_t tiles_per_split, float *__restrict__ bmax) {
+  __shared__ uint4 As[32][17];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const uint32_t h = lane >> 5, col = lane & 31;
+  const uint32_t q = blockIdx.x * kBmWaves + wave;
+  const bool vq = q < n_q;
+  bf16x8_t qf[8];
+  {
+    const bool vr = vq && col < t_max;
+    const float *qr = Q + ((size_t)(vq ? q : 0) * t_max + (vr ? col : 0)) * kDim + 64 * h;
+#pragma unroll
+    for (uint32_t s = 0; s < 8; ++s) {
+      bf16x8_t v;
+#pragma unroll
+      for (uint32_t e = 0; e < 8; ++e) v[e] = (__bf16)(vr ? qr[8 * s + e] : 0.0f);
+      qf[s] = v;
+    }
+  }
+  const uint32_t tb = blockIdx.y * tiles_per_split;
+  const uint32_t te = min(n_tiles, tb + tiles_per_split);
+  const uint4 *C4 = reinterpret_cast<const uint4 *>(Cb);
+  for (uint32_t tile = tb; tile < te; ++tile) {
+    __syncthreads();
+    {
+      const uint32_t r = tid >> 4, k = tid & 15, c = tile * 32 + r;
+      As[r][k] = c < n_c ? C4[(size_t)c * 16 + k] : make_uint4(0u, 0u, 0u, 0u);
+    }
+    __syncthreads();
+    floatx16_t acc = {};
+#pragma unroll
+    for (uint32_t s = 0; s < 8; ++s)
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(As[col][8 * h + s]), qf[s], acc, 0, 0, 0);
+    float mx = -INFINITY;
+#pragma unroll
+    for (uint32_t j = 0; j < 16; ++j) {
+      const uint32_t m = 8 * (j >> 2) + 4 * h + (j & 3);
+      if (tile * 32 + m < n_c) mx = fmaxf(mx, acc[j]);
+    }
+    mx = fmaxf(mx, __shfl_xor(mx, 32));
+    if (h == 0 && vq && col < t_max) bmax[((size_t)q * t_max + col) * n_tiles + tile] = mx;
+  }
+}
+
+// Pass 2: one wave per query-token row.  Select the best P+2 tiles by their
+// maximum (wave-held sorted list, lane j = j-th best), then score every
+// centroid of those tiles with bf16-rounded inputs as one sequential FP32 FMA
+// chain over d (the precision mode's exact definition, reproducible on the
+// CPU) and keep the top-P by (score desc, id asc).  Output S' = score + 1.
+__device__ __forceinline__ void wave_list_insert(float &ls, uint32_t &li, uint32_t lane, uint32_t P,
+                                                 float s, uint32_t id, bool valid) {
+  float ts = readlane_f(ls, P - 1);
+  uint32_t ti = readlane_u(li, P - 1);
+  uint64_t bal = __ballot(valid && better(s, id, ts, ti));
+  while (bal) {
+    const uint32_t l = (uint32_t)__builtin_ctzll(bal);
+    const float cs = readlane_f(s, l);
+    const uint32_t ci = readlane_u(id, l);
+    const uint32_t pos = (uint32_t)__popcll(__ballot(lane < P && better(ls, li, cs, ci)));
+    const float us = __shfl_up(ls, 1);
+    const uint32_t ui = __shfl_up(li, 1);
+    if (lane > pos) { ls = us; li = ui; }
+    else if (lane == pos) { ls = cs; li = ci; }
+    bal &= bal - 1;
+    ts = readlane_f(ls, P - 1);
+    ti = readlane_u(li, P - 1);
+    bal &= __ballot(valid && better(s, id, ts, ti));
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ct_bf16_select(
+    const float *__restrict__ Q, const uint32_t *__restrict__ row_map, uint32_t n_rows,
+    const uint16_t *__restrict__ Cb, uint32_t n_c, uint32_t n_tiles,
+    const float *__restrict__ bmax, uint32_t P, uint32_t *__restrict__ top_ids,
+    float *__restrict__ top_s) {
+  __shared__ float qs[4][kDim];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const uint32_t row = blockIdx.x * 4 + wave;
+  if (row >= n_rows) return;  // whole wave leaves; no block barrier below
+  const uint64_t qrow = row_map ? row_map[row] : row;
+  for (uint32_t d = lane; d < kDim; d += 64) qs[wave][d] = (float)(__bf16)Q[qrow * kDim + d];
+  wave_lds_sync();
+  // best P2 tiles by maximum
+  const uint32_t P2 = min(P + 2, min(n_tiles, 64u));
+  float ls = -INFINITY;
+  uint32_t li = 0xFFFFFFFFu;
+  const float *bm = bmax + qrow * n_tiles;
+  for (uint32_t base = 0; base < n_tiles; base += 64) {
+    const uint32_t tl = base + lane;
+    const bool v = tl < n_tiles;
+    wave_list_insert(ls, li, lane, P2, v ? bm[tl] : -INFINITY, tl, v);
+  }
+  // exact scores of the selected tiles' centroids, two tiles per round
+  float fs = -INFINITY;
+  uint32_t fi = 0xFFFFFFFFu;
+  for (uint32_t j = 0; j < P2; j += 2) {
+    const uint32_t jj = j + (lane >> 5);
+    const uint32_t tile = (uint32_t)__shfl((int)li, (int)min(jj, P2 - 1));
+    const uint32_t c = tile * 32 + (lane & 31);
+    const bool v = jj < P2 && tile != 0xFFFFFFFFu && c < n_c;
+    float a = 0.0f;
+    if (v) {
+      const uint4 *cr = reinterpret_cast<const uint4 *>(Cb + (size_t)c * kDim);
+#pragma unroll 4
+      for (uint32_t k = 0; k < 16; ++k) {
+        const uint4 w = cr[k];
+        const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (uint32_t e = 0; e < 4; ++e) {
+          a = __builtin_fmaf(qs[wave][8 * k + 2 * e], __uint_as_float(ww[e] << 16), a);
+          a = __builtin_fmaf(qs[wave][8 * k + 2 * e + 1], __uint_as_float(ww[e] & 0xFFFF0000u), a);
+        }
+      }
+    }
+    wave_list_insert(fs, fi, lane, P, a + 1.0f, c, v);
+  }
+  if (lane < P) {
+    top_ids[qrow * P + lane] = fi;
+    top_s[qrow * P + lane] = fs;
+  }
+}
+
+__global__ void k_to_bf16(const float *__restrict__ x, uint16_t *__restrict__ y, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = __builtin_bit_cast(uint16_t, (__bf16)x[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Exhaustive MaxSim in FP64 over raw FP32 embeddings (the accuracy oracle that
+// recall is measured against): out[q][doc] = sum_t max_i <Q_t, X_i> with every
+// product and sum in double (products of two floats are exact in double).
+// Same lane layout as k_rerank_f32: lane (t, h) keeps Q_t as 128 floats in
+// VGPRs, doc tokens go through LDS as doubles two at a time.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_exhaustive_f64(
+    const float *__restrict__ Q, uint32_t t_max, const uint32_t *__restrict__ q_ntok,
+    const float *__restrict__ X, const uint32_t *__restrict__ doclens,
+    const uint64_t *__restrict__ docoff, uint32_t n_doc, uint32_t n_slice,
+    double *__restrict__ out) {
+  __shared__ __align__(16) double xs[4][2][kDim];
+  __shared__ double Ms[4][kMaxQTok];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const uint32_t h = lane >> 5, t = lane & 31;
+  const uint32_t slice = blockIdx.x, q = blockIdx.y;
+  const uint32_t T = q_ntok ? q_ntok[q] : t_max;
+  float qv[kDim];
+  {
+    const float4 *q4 = reinterpret_cast<const float4 *>(Q + ((size_t)q * t_max + t) * kDim);
+#pragma unroll
+    for (uint32_t i = 0; i < kDim / 4; ++i) {
+      const float4 v = (t < T) ? q4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      qv[4 * i] = v.x; qv[4 * i + 1] = v.y; qv[4 * i + 2] = v.z; qv[4 * i + 3] = v.w;
+    }
+  }
+  for (uint32_t doc = slice * 4 + wave; doc < n_doc; doc += n_slice * 4) {
+    const uint32_t L = doclens[doc];
+    const uint64_t off = docoff[doc];
+    double M = -INFINITY;
+    for (uint32_t i = 0; i < L; i += 2) {
+      const uint32_t tok = i + h;
+      const bool vt = tok < L;
+      if (vt) {
+        const float4 x = reinterpret_cast<const float4 *>(X + (off + tok) * kDim)[t];
+        xs[wave][h][4 * t + 0] = (double)x.x;
+        xs[wave][h][4 * t + 1] = (double)x.y;
+        xs[wave][h][4 * t + 2] = (double)x.z;
+        xs[wave][h][4 * t + 3] = (double)x.w;
+      }
+      wave_lds_sync();
+      if (vt) {
+        double a = 0.0;
+#pragma unroll
+        for (uint32_t d = 0; d < kDim; d += 2) {
+          const double2 xv = *reinterpret_cast<const double2 *>(&xs[wave][h][d]);
+          a = __builtin_fma((double)qv[d], xv.x, a);
+          a = __builtin_fma((double)qv[d + 1], xv.y, a);
+        }
+        M = fmax(M, a);
+      }
+      wave_lds_sync();
+    }
+    M = fmax(M, __shfl_xor(M, 32));
+    if (h == 0) Ms[wave][t] = M;
+    wave_lds_sync();
+    if (lane == 0) {
+      double sc = 0.0;
+      for (uint32_t tt = 0; tt < T; ++tt) sc += Ms[wave][tt];
+      out[(size_t)q * n_doc + doc] = sc;
+    }
+    wave_lds_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Final top-k per query: sort the n reranked (score, doc) keys in LDS.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kSelThreads) void k_topk_final(
     const uint32_t *__restrict__ ids, const float *__restrict__ score,
     const uint32_t *__restrict__ n_in, uint32_t R, uint32_t k, uint32_t *__restrict__ out_ids,
-    float *__restrict__ out_s) {
+    float *__restrict__ out_s, uint32_t *__restrict__ out_n) {
   __shared__ unsigned long long s[kMaxSelect];
   const uint32_t q = blockIdx.x, tid = threadIdx.x;
   const uint32_t n = n_in[q];
@@ -581,6 +888,7 @@ __global__ __launch_bounds__(kSelThreads) void k_topk_final(
       out_s[o] = -INFINITY;
     }
   }
+  if (out_n && tid == 0) out_n[q] = min(n, k);
 }
 
 // ---------------------------------------------------------------------------
@@ -604,6 +912,20 @@ __global__ void k_seg_offsets(const uint64_t *__restrict__ ivf_off,
     if ((uint64_t)lst[mid] < key) lo = mid + 1; else hi = mid;
   }
   seg_off[g] = lo;
+}
+
+// counters[0] += sum of the selected docs' lengths, counters[1] += candidates.
+__global__ __launch_bounds__(256) void k_count_work(const uint32_t *__restrict__ sel_ids,
+                                                    const uint32_t *__restrict__ sel_n, uint32_t R,
+                                                    const uint32_t *__restrict__ doclens,
+                                                    const uint32_t *__restrict__ cand_cnt,
+                                                    unsigned long long *__restrict__ counters) {
+  const uint32_t q = blockIdx.x;
+  const uint32_t n = sel_n[q];
+  unsigned long long t = 0;
+  for (uint32_t r = threadIdx.x; r < n; r += 256) t += doclens[sel_ids[(size_t)q * R + r]];
+  if (t) atomicAdd(&counters[0], t);
+  if (threadIdx.x == 0) atomicAdd(&counters[1], (unsigned long long)cand_cnt[q]);
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
@@ -712,6 +1034,14 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
+hipError_t launch_count_work(const uint32_t *sel_ids, const uint32_t *sel_n, uint32_t R, uint32_t n_q,
+                             const DevIndex &ix, const uint32_t *cand_cnt,
+                             unsigned long long *counters, hipStream_t s) {
+  if (n_q == 0) return hipSuccess;
+  k_count_work<<<n_q, 256, 0, s>>>(sel_ids, sel_n, R, ix.doclens, cand_cnt, counters);
+  return hipGetLastError();
+}
+
 hipError_t launch_seg_offsets(const DevIndex &ix, hipStream_t s) {
   const uint64_t total = (uint64_t)ix.n_c * (ix.n_seg + 1);
   k_seg_offsets<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(ix.ivf_off, ix.ivf_size,
@@ -742,6 +1072,25 @@ hipError_t launch_centroid_topk(const float *Q, const uint32_t *row_map, uint32_
   return hipGetLastError();
 }
 
+hipError_t launch_centroid_topk_bf16(const float *Q, uint32_t n_q, uint32_t t_max,
+                                     const uint32_t *row_map, uint32_t n_rows, const DevIndex &ix,
+                                     uint32_t P, float *bmax, uint32_t *top_ids, float *top_s,
+                                     hipStream_t s) {
+  if (n_q == 0 || n_rows == 0) return hipSuccess;
+  if (!ix.C_bf16 || P > 62) return hipErrorInvalidValue;
+  const uint32_t n_tiles = (ix.n_c + 31) / 32;
+  const uint32_t qblocks = (n_q + kBmWaves - 1) / kBmWaves;
+  const uint32_t n_split = std::max(1u, std::min(n_tiles, (1024u + qblocks - 1) / qblocks));
+  const uint32_t tps = (n_tiles + n_split - 1) / n_split;
+  dim3 grid(qblocks, (n_tiles + tps - 1) / tps);
+  k_ct_bf16_blockmax<<<grid, 512, 0, s>>>(Q, t_max, n_q, ix.C_bf16, ix.n_c, n_tiles, tps, bmax);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_ct_bf16_select<<<(n_rows + 3) / 4, 256, 0, s>>>(Q, row_map, n_rows, ix.C_bf16, ix.n_c, n_tiles,
+                                                    bmax, P, top_ids, top_s);
+  return hipGetLastError();
+}
+
 hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t_max, uint32_t P,
                          const uint32_t *q_ntok, const uint32_t *q_np, uint32_t n_q,
                          const DevIndex &ix, const uint64_t *cand_off, uint32_t *cand_cnt,
@@ -767,20 +1116,40 @@ hipError_t launch_rerank(const float *Q, uint32_t t_max, const uint32_t *q_ntok,
                          const uint32_t *sel_ids, const uint32_t *sel_n, uint32_t R,
                          const DevIndex &ix, int precision, int clip_zero, float *sel_score,
                          hipStream_t s) {
-  if (precision != 0) return hipErrorNotSupported;
   if (n_q == 0) return hipSuccess;
   const uint32_t n_slice = R >= 256 ? 4 : 1;
   dim3 grid(n_slice, n_q);
-  k_rerank_f32<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, sel_ids, sel_n, R, n_slice, ix.doclens,
-                                    ix.docoff, ix.codes, ix.residual, ix.C, ix.W, clip_zero,
-                                    sel_score);
+  if (precision == 0) {
+    k_rerank_f32<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, sel_ids, sel_n, R, n_slice, ix.doclens,
+                                      ix.docoff, ix.codes, ix.residual, ix.C, ix.W, clip_zero,
+                                      sel_score);
+  } else if (precision == 1) {
+    if (!ix.C_bf16 || !ix.lut_bf16) return hipErrorInvalidValue;
+    k_rerank_bf16<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, sel_ids, sel_n, R, n_slice, ix.doclens,
+                                       ix.docoff, ix.codes, ix.residual, ix.C_bf16, ix.lut_bf16,
+                                       clip_zero, sel_score);
+  } else {
+    return hipErrorNotSupported;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_exhaustive_f64(const float *Q, uint32_t n_q, uint32_t t_max,
+                                 const uint32_t *q_ntok, const float *X, const uint32_t *doclens,
+                                 const uint64_t *docoff, uint32_t n_doc, double *out,
+                                 hipStream_t s) {
+  if (n_q == 0 || n_doc == 0) return hipSuccess;
+  const uint32_t n_slice = std::max(1u, std::min((n_doc + 3) / 4, 8192u / n_q));
+  dim3 grid(n_slice, n_q);
+  k_exhaustive_f64<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, X, doclens, docoff, n_doc, n_slice, out);
   return hipGetLastError();
 }
 
 hipError_t launch_topk(const uint32_t *ids, const float *score, const uint32_t *n_in, uint32_t R,
-                       uint32_t n_q, uint32_t k, uint32_t *out_ids, float *out_s, hipStream_t s) {
+                       uint32_t n_q, uint32_t k, uint32_t *out_ids, float *out_s, uint32_t *out_n,
+                       hipStream_t s) {
   if (n_q == 0) return hipSuccess;
-  k_topk_final<<<n_q, kSelThreads, 0, s>>>(ids, score, n_in, R, k, out_ids, out_s);
+  k_topk_final<<<n_q, kSelThreads, 0, s>>>(ids, score, n_in, R, k, out_ids, out_s, out_n);
   return hipGetLastError();
 }
 
@@ -832,9 +1201,10 @@ hipError_t launch_compress(const float *x, const uint32_t *codes, uint64_t n, co
 }
 
 hipError_t launch_convert_centroids(const DevIndex &ix, hipStream_t s) {
-  (void)ix;
-  (void)s;
-  return hipSuccess;
+  const size_t n = (size_t)ix.n_c * kDim;
+  if (!ix.C_bf16) return hipSuccess;
+  k_to_bf16<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(ix.C, ix.C_bf16, n);
+  return hipGetLastError();
 }
 
 }  // namespace mvr

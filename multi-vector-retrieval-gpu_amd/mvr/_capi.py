@@ -41,7 +41,8 @@ class StageOut(C.Structure):
 
 class Timings(C.Structure):
     _fields_ = [("centroid_ms", C.c_double), ("filter_ms", C.c_double), ("select_ms", C.c_double),
-                ("rerank_ms", C.c_double), ("topk_ms", C.c_double), ("total_ms", C.c_double)]
+                ("rerank_ms", C.c_double), ("topk_ms", C.c_double), ("total_ms", C.c_double),
+                ("rerank_tokens", C.c_uint64), ("candidates", C.c_uint64)]
 
 
 # Every symbol declared in include/mvr.h, with its ctypes signature.
@@ -57,6 +58,8 @@ SIGNATURES = {
                              C.POINTER(SearchParams), C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                              C.POINTER(StageOut), C.POINTER(Timings)]),
     "mvr_decompress": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "mvr_exhaustive_f64": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "mvr_synth_tokens": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint32,
                                    C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, C.c_void_p,
                                    C.c_void_p]),

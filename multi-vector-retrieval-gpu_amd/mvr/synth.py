@@ -181,9 +181,12 @@ def build_corpus(torch, seed, n_doc, n_comp, n_c, doclens, n_iter=10, kmeans_sam
     torch.cuda.synchronize()
     log(f"kmeans: {n_c} centroids on {sample.shape[0]} tokens, {n_iter} iterations")
     codes = torch.empty(n_tok, dtype=torch.int32, device=dev)
-    for t0 in range(0, n_tok, chunk):
+    for i, t0 in enumerate(range(0, n_tok, chunk)):
         n = min(chunk, n_tok - t0)
         assign(torch, X[t0:t0 + n], C_, None, out=codes[t0:t0 + n])
+        if i % 4 == 3:
+            torch.cuda.synchronize()
+            log(f"assign: {t0 + n}/{n_tok} tokens")
     torch.cuda.synchronize()
     log("assign: done")
     cs = sample if sample is not X else X[:: max(1, n_tok // (1 << 20))]

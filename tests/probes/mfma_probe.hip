@@ -1,0 +1,58 @@
+// Lane-layout probe for the MFMA builtins the bf16/fp8 kernels rely on
+// (test infrastructure, built into tests/probes/libmvr_probe.so).  One wave;
+// every index stays inside the 64-lane buffers the host passes.
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+__global__ void k_probe_bf16(const bf16x8 *a, const bf16x8 *b, floatx16 *d) {
+  const uint32_t l = threadIdx.x;  // 64 lanes
+  floatx16 acc = {};
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[l], b[l], acc, 0, 0, 0);
+  d[l] = acc;
+}
+
+__global__ void k_probe_fp8(const long *a, const long *b, floatx16 *d) {
+  const uint32_t l = threadIdx.x;
+  floatx16 acc = {};
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a[l], b[l], acc, 0, 0, 0);
+  d[l] = acc;
+}
+
+// Packs pairs of floats to fp8 with the hardware conversion (e4m3).
+__global__ void k_probe_cvt_fp8(const float *x, uint32_t *out, uint32_t n_pairs) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_pairs) return;
+  out[i] = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(x[2 * i], x[2 * i + 1], 0, false) & 0xFFFFu;
+}
+
+extern "C" int probe_mfma(int kind, const void *a, const void *b, float *d) {
+  const size_t op = kind == 0 ? 16 : 8;
+  void *da, *db, *dd;
+  if (hipMalloc(&da, 64 * op) || hipMalloc(&db, 64 * op) || hipMalloc(&dd, 64 * 64)) return 1;
+  hipMemcpy(da, a, 64 * op, hipMemcpyHostToDevice);
+  hipMemcpy(db, b, 64 * op, hipMemcpyHostToDevice);
+  if (kind == 0)
+    k_probe_bf16<<<1, 64>>>((const bf16x8 *)da, (const bf16x8 *)db, (floatx16 *)dd);
+  else
+    k_probe_fp8<<<1, 64>>>((const long *)da, (const long *)db, (floatx16 *)dd);
+  hipError_t e = hipMemcpy(d, dd, 64 * 64, hipMemcpyDeviceToHost);
+  hipFree(da);
+  hipFree(db);
+  hipFree(dd);
+  return e != hipSuccess;
+}
+
+extern "C" int probe_cvt_fp8(const float *x, uint32_t *out, uint32_t n_pairs) {
+  float *dx;
+  uint32_t *dout;
+  if (hipMalloc((void **)&dx, n_pairs * 8) || hipMalloc((void **)&dout, n_pairs * 4)) return 1;
+  hipMemcpy(dx, x, n_pairs * 8, hipMemcpyHostToDevice);
+  k_probe_cvt_fp8<<<(n_pairs + 63) / 64, 64>>>(dx, dout, n_pairs);
+  hipError_t e = hipMemcpy(out, dout, n_pairs * 4, hipMemcpyDeviceToHost);
+  hipFree(dx);
+  hipFree(dout);
+  return e != hipSuccess;
+}

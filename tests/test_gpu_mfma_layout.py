@@ -1,0 +1,84 @@
+"""In-bounds check of the MFMA lane layouts and the FP8 encoding the bf16/fp8
+kernels assume on gfx950 (one wave, exact small-integer inputs).
+
+Assumed (v_mfma_f32_32x32x16_{bf16,fp8_fp8}):
+  A: lane l holds A[l % 32][8*(l // 32) + e], e = 0..7
+  B: lane l holds B[8*(l // 32) + e][l % 32]
+  D: lane l, element j holds D[8*(j // 4) + 4*(l // 32) + j % 4][l % 32]
+FP8: OCP e4m3 (bias 7, 1.0 = 0x38).
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def probe():
+    return ctypes.CDLL(os.path.join(HERE, "probes", "libmvr_probe.so"))
+
+
+def bf16_bits(x):
+    return (np.asarray(x, np.float32).view(np.uint32) >> 16).astype(np.uint16)
+
+
+def pack_operands(A, B):
+    a = np.zeros((64, 8), A.dtype)
+    b = np.zeros((64, 8), B.dtype)
+    for l in range(64):
+        for e in range(8):
+            a[l, e] = A[l % 32, 8 * (l // 32) + e]
+            b[l, e] = B[8 * (l // 32) + e, l % 32]
+    return a, b
+
+
+def unpack_d(d):
+    D = np.zeros((32, 32), np.float32)
+    for l in range(64):
+        for j in range(16):
+            D[8 * (j // 4) + 4 * (l // 32) + j % 4, l % 32] = d[l, j]
+    return D
+
+
+def test_bf16_32x32x16_layout(probe):
+    rng = np.random.default_rng(0)
+    A = rng.integers(-8, 9, (32, 16)).astype(np.float32)
+    B = rng.integers(-8, 9, (16, 32)).astype(np.float32)
+    a, b = pack_operands(bf16_bits(A), bf16_bits(B))
+    d = np.zeros((64, 16), np.float32)
+    assert probe.probe_mfma(0, a.ctypes.data, b.ctypes.data, d.ctypes.data) == 0
+    np.testing.assert_array_equal(unpack_d(d), A @ B)
+
+
+def ocp_e4m3(v):
+    """OCP FP8 E4M3 encoding of exactly representable values."""
+    if v == 0:
+        return 0
+    s = 0x80 if v < 0 else 0
+    v = abs(v)
+    e = int(np.floor(np.log2(v)))
+    m = v / 2.0 ** e - 1.0
+    return s | ((e + 7) << 3) | int(round(m * 8))
+
+
+def test_fp8_conversion_is_ocp_e4m3(probe):
+    vals = np.array([1.0, 2.0, -3.0, 0.5, 448.0, -0.015625, 1.125, 6.0], np.float32)
+    out = np.zeros(vals.size // 2, np.uint32)
+    assert probe.probe_cvt_fp8(vals.ctypes.data, out.ctypes.data, vals.size // 2) == 0
+    got = [(int(o) >> (8 * k)) & 0xFF for o in out for k in range(2)]
+    assert got == [ocp_e4m3(float(v)) for v in vals]
+
+
+def test_fp8_32x32x16_layout(probe):
+    rng = np.random.default_rng(1)
+    A = rng.integers(-4, 5, (32, 16)).astype(np.float32)
+    B = rng.integers(-4, 5, (16, 32)).astype(np.float32)
+    enc = np.vectorize(ocp_e4m3)
+    a, b = pack_operands(enc(A).astype(np.uint8), enc(B).astype(np.uint8))
+    d = np.zeros((64, 16), np.float32)
+    assert probe.probe_mfma(1, a.ctypes.data, b.ctypes.data, d.ctypes.data) == 0
+    np.testing.assert_array_equal(unpack_d(d), A @ B)

@@ -99,8 +99,9 @@ def test_fewer_candidates_than_probe_topk(small):
     """R larger than the candidate count: outputs padded with (UINT32_MAX, -inf)."""
     corpus, ix, Q = small
     g = _index(ix)
-    ids, sc, st = g.search(Q[:4], 1, 1024, 1024, stages=True)
-    ref = _oracle_stages(ix, Q[:4], 1, 1024, 1024)
+    Q1 = np.ascontiguousarray(Q[:4, :1])     # one token, one probe: a few hundred candidates
+    ids, sc, st = g.search(Q1, 1, 1024, 1024, stages=True)
+    ref = _oracle_stages(ix, Q1, 1, 1024, 1024)
     _check(st, ids, sc, ref)
     for q, r in enumerate(ref):
         assert r["cand_n"] < 1024
@@ -126,3 +127,67 @@ def test_invalid_arguments_raise(small):
         g.search(Q, 200, 64, 10)         # nprobe > n_centroid
     with pytest.raises(mvr.MvrError):
         g.search(np.zeros((1, 33, 128), np.float32), 4, 64, 10)
+
+
+def _bf16_maxsim(ix, Qq, docs, clip_zero=False):
+    """MaxSim with bf16 inputs as the BF16 kernel defines it: query, centroid
+    and residual values rounded to bf16 (nearest even), products and sums in
+    float64: s = bf16(q).(bf16(C[code]) + bf16(w[d][bucket]))."""
+    from oracle import oracle as O
+    qb = O.bf16_round(Qq).astype(np.float64)
+    Cb = O.bf16_round(ix.C).astype(np.float64)
+    Wb = O.bf16_round(ix.W).astype(np.float64)
+    out = []
+    for d in docs:
+        off, L = int(ix.docoff[d]), int(ix.doclens[d])
+        res = ix.residual[off:off + L]
+        b = np.stack([(res[:, j // 4] >> (6 - 2 * (j % 4))) & 3 for j in range(128)], 1)
+        X = Cb[ix.codes[off:off + L]] + Wb[np.arange(128)[None, :], b]
+        m = (qb @ X.T).max(axis=1)
+        if clip_zero:
+            m = np.maximum(m, 0)
+        out.append(m.sum())
+    return np.array(out)
+
+
+def test_bf16_rerank_scores_within_tolerance(small):
+    corpus, ix, Q = small
+    g = _index(ix)
+    ids, sc, st = g.search(Q, 4, 64, 10, precision="bf16", rescore=0, stages=True)
+    for q in range(Q.shape[0]):
+        n = int((st["sel_ids"][q] != 0xFFFFFFFF).sum())
+        ref = _bf16_maxsim(ix, Q[q], st["sel_ids"][q, :n])
+        np.testing.assert_allclose(st["sel_score"][q, :n], ref, rtol=0, atol=2e-4)
+
+
+def test_bf16_with_fp32_rescore_returns_fp32_scores(c1):
+    """BF16 MaxSim over R=256, FP32 rescoring of the best 64, top-10: the
+    returned scores are the FP32 (oracle) scores of the returned docs and
+    match the FP32 pipeline's top-10."""
+    corpus, ix, Q = c1
+    g = _index(ix)
+    ids, sc = g.search(Q, 4, 256, 10, precision="bf16", rescore=64)
+    oi, os_ = ix.search(Q, 4, 256, 10)
+    for q in range(Q.shape[0]):
+        np.testing.assert_array_equal(sc[q], ix.rerank(Q[q], ids[q]))
+    same = np.mean([set(a.tolist()) == set(b.tolist()) for a, b in zip(ids, oi)])
+    assert same >= 0.95, same
+
+
+def test_bf16_centroid_topk_matches_rounded_input_oracle(c1):
+    """BF16 centroid scoring: exact top-P of S' = fmaf-chain(bf16(q), bf16(C)) + 1
+    (the oracle's FP32 scoring applied to bf16-rounded inputs); the MFMA tile
+    maxima only choose which tiles are rescanned, so a difference can only
+    come from a near-tie at the tile boundary."""
+    from oracle import oracle as O
+    corpus, ix, Q = c1
+    g = _index(ix)
+    P = 8
+    ids, sc, st = g.search(Q, P, 64, 10, precision="bf16", stages=True)
+    ti, ts = O.centroid_topk(O.bf16_round(Q), O.bf16_round(ix.C), P)
+    ti = ti.reshape(Q.shape[0], Q.shape[1], P)
+    ts = ts.reshape(Q.shape[0], Q.shape[1], P)
+    mism = (st["top_ids"] != ti).any(axis=2)
+    assert mism.mean() <= 0.002, mism.mean()
+    ok = ~mism
+    np.testing.assert_array_equal(st["top_s"][ok], ts[ok])
