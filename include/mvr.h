@@ -75,7 +75,7 @@ typedef struct {
   uint32_t rescore;     /* BF16/FP8: FP32 rescoring of the best `rescore` docs (0 = off) */
   uint32_t clip_zero;   /* 1: MaxSim per token is max(0, .) (basic variant) */
   uint32_t batch;       /* queries per device pass (0 = 1024) */
-  uint32_t reserved;
+  uint32_t learned;     /* 1: per-query nprobe/probe_topk from the attached probe model */
 } mvr_search_params;
 
 /* Optional per-stage outputs (host buffers, any may be NULL), for tests. */
@@ -119,6 +119,16 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
                float *out_scores, int on_device, void *stream, mvr_stage_out *stages,
                mvr_timings *timings);
 
+/* MaxSim of given candidates (the rerank stage alone, RerankPP.hpp:368-475):
+ * scores_dev[q][r] = sum_t max_i <Q[q][t], x_i> over the decompressed tokens of
+ * doc_ids_dev[q][r] for r < n_docs[q] (host array; NULL: R for every query),
+ * in `precision` (MVR_FP32 bit-exact with mvr_search, MVR_BF16 on MFMA).
+ * Device buffers Q [n_q][t_max][128], doc_ids [n_q][R], scores [n_q][R];
+ * q_ntok is a host array (NULL: t_max).  Queued on `stream`. */
+int mvr_rerank_docs(mvr_index *ix, const float *Q_dev, uint32_t n_q, uint32_t t_max,
+                    const uint32_t *q_ntok, const uint32_t *doc_ids_dev, const uint32_t *n_docs,
+                    uint32_t R, uint32_t precision, float *scores_dev, void *stream);
+
 /* FP32 decompression of whole documents: out receives sum(doclens[doc_ids])
  * rows of dim floats, documents in the given order (host buffers). */
 int mvr_decompress(mvr_index *ix, const uint32_t *doc_ids, uint32_t n, float *out);
@@ -131,6 +141,36 @@ int mvr_decompress(mvr_index *ix, const uint32_t *doc_ids, uint32_t n, float *ou
 int mvr_exhaustive_f64(const float *Q_dev, uint32_t n_q, uint32_t t_max, const uint32_t *q_ntok_dev,
                        const float *X_dev, const uint32_t *doclens_dev, uint32_t n_doc,
                        double *out_dev, void *stream);
+
+/* ---- learned probe (no counterpart in the reference, which probes a fixed
+ * nprobe: GenerateVQScoreCandidatePP.hpp:353-361) ----
+ * A 3-layer MLP (dims {1024, d1, d2, 6}, GELU) reads a query's top-32
+ * centroid scores per token and predicts its nprobe class over {2,4,8,16,32}
+ * (outputs 0..4) and log(probe_topk) (output 5).  Trained with Adam on the
+ * device; parameters are one flat array W1 b1 W2 b2 W3 b3 (row-major W[out][in]). */
+typedef struct mvr_mlp mvr_mlp;
+const char *mvr_mlp_last_error(void);
+int mvr_mlp_create(const uint32_t *dims, int device, mvr_mlp **out);
+void mvr_mlp_free(mvr_mlp *mlp);
+uint64_t mvr_mlp_num_params(const mvr_mlp *mlp);
+int mvr_mlp_set_params(mvr_mlp *mlp, const float *flat_host);      /* also resets Adam state */
+int mvr_mlp_get_params(const mvr_mlp *mlp, float *flat_host);
+/* out_dev [B][6] = network(x_dev [B][dims[0]]) */
+int mvr_mlp_forward(mvr_mlp *mlp, const float *x_dev, uint32_t B, float *out_dev, void *stream);
+/* loss (mean CE + mean squared error) and gradients for labels y (class
+ * index) and targets r, without updating; grads_host has num_params floats. */
+int mvr_mlp_grads(mvr_mlp *mlp, const float *x_dev, const int32_t *y_dev, const float *r_dev,
+                  uint32_t B, float *grads_host, float *loss_host);
+/* one Adam step (lr; betas 0.9/0.999, eps 1e-8); loss_host may be NULL */
+int mvr_mlp_train_step(mvr_mlp *mlp, const float *x_dev, const int32_t *y_dev, const float *r_dev,
+                       uint32_t B, float lr, float *loss_host, void *stream);
+/* Attach a probe model to an index: searches with params->learned = 1 score
+ * the top-32 centroids per token, run the model, and probe the predicted
+ * nprobe (fallback_nprobe when the top class probability is below conf) with
+ * probe_topk min(params->probe_topk, max(r_min, round(exp(output 5)))).
+ * The index keeps the pointer; pass NULL to detach. */
+int mvr_index_set_probe(mvr_index *ix, mvr_mlp *mlp, float conf, uint32_t fallback_nprobe,
+                        uint32_t r_min);
 
 /* ---- synthetic corpus and index build on the device (bench and tests) ---- */
 

@@ -94,13 +94,14 @@ struct HBuf {
 
 struct Workspace {
   DBuf Q, meta, part_ids, part_s, top_ids, top_s, cand_cnt, cand_keys, sel_ids, sel_F, sel_n,
-      sel_score, out_ids, out_s, counters, res_ids, res_s, res_n, bmax;
+      sel_score, out_ids, out_s, counters, res_ids, res_s, res_n, bmax, feat, mlp_out, dyn_np, dyn_r,
+      rr_meta;
   HBuf staging;
   hipEvent_t staging_done = nullptr;
   void release() {
     for (DBuf *b : {&Q, &meta, &part_ids, &part_s, &top_ids, &top_s, &cand_cnt, &cand_keys,
                     &sel_ids, &sel_F, &sel_n, &sel_score, &out_ids, &out_s, &counters, &res_ids,
-                    &res_s, &res_n, &bmax})
+                    &res_s, &res_n, &bmax, &feat, &mlp_out, &dyn_np, &dyn_r, &rr_meta})
       b->release();
     staging.release();
     if (staging_done) (void)hipEventDestroy(staging_done);
@@ -112,6 +113,9 @@ struct Workspace {
 
 struct mvr_index {
   int device = 0;
+  mvr_mlp *probe = nullptr;    // learned probe model (not owned)
+  float probe_conf = 0.6f;
+  uint32_t probe_fallback = 16, probe_rmin = 0;
   DevIndex d;
   std::vector<uint32_t> h_doclens;
   Workspace ws;
@@ -393,7 +397,16 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     if (nt[q] == 0 || nt[q] > t_max) return fail(MVR_ERR_INVALID, "query %u has %u tokens (t_max %u)", q, nt[q], t_max);
     P = std::max(P, np[q]);
   }
+  const bool learned = prm->learned != 0;
+  if (learned) {
+    if (!ix->probe) return fail(MVR_ERR_INVALID, "learned probing needs a model (mvr_index_set_probe)");
+    if (q_nprobe) return fail(MVR_ERR_INVALID, "learned probing chooses nprobe; pass q_nprobe = NULL");
+    // the model reads the top-32 scores of every token and may pick up to 32 probes
+    P = kMaxProbe;
+    for (uint32_t q = 0; q < n_q; ++q) np[q] = kMaxProbe;
+  }
   P = std::min(P, d.n_c);
+  if (learned && P < kMaxProbe) return fail(MVR_ERR_INVALID, "learned probing needs >= 32 centroids");
   const uint32_t B = prm->batch ? prm->batch : 1024;
   DeviceGuard g(ix->device);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
@@ -459,6 +472,12 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   HIP_TRY(ws.sel_n.ensure(Bc * 4));
   HIP_TRY(ws.sel_score.ensure(Bc * R * 4));
   if (prm->precision != MVR_FP32) HIP_TRY(ws.bmax.ensure(Bc * t_max * ((d.n_c + 31) / 32) * 4));
+  if (learned) {
+    HIP_TRY(ws.feat.ensure(Bc * 1024 * 4));
+    HIP_TRY(ws.mlp_out.ensure(Bc * 6 * 4));
+    HIP_TRY(ws.dyn_np.ensure(Bc * 4));
+    HIP_TRY(ws.dyn_r.ensure(Bc * 4));
+  }
   if (R2) {
     HIP_TRY(ws.res_ids.ensure(Bc * R2 * 4));
     HIP_TRY(ws.res_s.ensure(Bc * R2 * 4));
@@ -506,7 +525,8 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     uint32_t n_rows = 0;
     if (uniform) n_rows = nb * t_max;
     else for (uint32_t i = 0; i < nb; ++i) n_rows += nt[b0 + i];
-    const uint32_t *bnt = d_nt + b0, *bnp = d_np + b0;
+    const uint32_t *bnt = d_nt + b0;
+    const uint32_t *bnp = d_np + b0;
     const uint64_t *bcoff = d_coff + b0;
 
     HIP_TRY(rec(bi, 0));
@@ -518,11 +538,22 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       HIP_TRY(launch_centroid_topk_bf16(Qb, nb, t_max, rm, n_rows, d, P, ws.bmax.as<float>(),
                                         ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), s));
     HIP_TRY(rec(bi, 1));
+    const uint32_t *q_r = nullptr;
+    if (learned) {
+      HIP_TRY(launch_probe_features(ws.top_s.as<float>(), t_max, P, bnt, nb, ws.feat.as<float>(), s));
+      if (mvr_mlp_forward(ix->probe, ws.feat.as<float>(), nb, ws.mlp_out.as<float>(), s) != MVR_OK)
+        return fail(MVR_ERR_DEVICE, "probe model: %s", mvr_mlp_last_error());
+      HIP_TRY(launch_probe_decide(ws.mlp_out.as<float>(), nb, ix->probe_conf, ix->probe_fallback,
+                                  std::max(ix->probe_rmin, k), R, ws.dyn_np.as<uint32_t>(),
+                                  ws.dyn_r.as<uint32_t>(), s));
+      bnp = ws.dyn_np.as<uint32_t>();
+      q_r = ws.dyn_r.as<uint32_t>();
+    }
     HIP_TRY(launch_fill_u32(ws.cand_cnt.as<uint32_t>(), 0u, nb, s));
     HIP_TRY(launch_filter(ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), t_max, P, bnt, bnp, nb,
                           d, bcoff, ws.cand_cnt.as<uint32_t>(), ws.cand_keys.as<uint64_t>(), s));
     HIP_TRY(rec(bi, 2));
-    HIP_TRY(launch_select(ws.cand_keys.as<uint64_t>(), bcoff, ws.cand_cnt.as<uint32_t>(), nb, R,
+    HIP_TRY(launch_select(ws.cand_keys.as<uint64_t>(), bcoff, ws.cand_cnt.as<uint32_t>(), q_r, nb, R,
                           ws.sel_ids.as<uint32_t>(), ws.sel_F.as<float>(), ws.sel_n.as<uint32_t>(), s));
     HIP_TRY(rec(bi, 3));
     HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R,
@@ -585,6 +616,48 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     for (auto &e : ev) (void)hipEventDestroy(e);
   }
   if (!on_device) HIP_TRY(hipStreamSynchronize(s));
+  return MVR_OK;
+}
+
+int mvr_rerank_docs(mvr_index *ix, const float *Q_dev, uint32_t n_q, uint32_t t_max,
+                    const uint32_t *q_ntok, const uint32_t *doc_ids_dev, const uint32_t *n_docs,
+                    uint32_t R, uint32_t precision, float *scores_dev, void *stream) {
+  if (!ix || !Q_dev || !doc_ids_dev || !scores_dev) return fail(MVR_ERR_INVALID, "NULL argument");
+  if (n_q == 0 || R == 0) return MVR_OK;
+  if (t_max == 0 || t_max > kMaxQTok) return fail(MVR_ERR_INVALID, "t_max %u not in [1, 32]", t_max);
+  if (precision != MVR_FP32 && precision != MVR_BF16)
+    return fail(MVR_ERR_UNSUPPORTED, "precision %u not built", precision);
+  DeviceGuard g(ix->device);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  Workspace &ws = ix->ws;
+  // metadata: q_ntok and the per-query doc counts (pinned staging, one copy)
+  if (ws.staging_done) HIP_TRY(hipEventSynchronize(ws.staging_done));
+  else HIP_TRY(hipEventCreateWithFlags(&ws.staging_done, hipEventDisableTiming));
+  HIP_TRY(ws.staging.ensure((size_t)n_q * 8));
+  uint32_t *hs = reinterpret_cast<uint32_t *>(ws.staging.p);
+  for (uint32_t q = 0; q < n_q; ++q) {
+    hs[q] = q_ntok ? q_ntok[q] : t_max;
+    hs[n_q + q] = n_docs ? std::min(n_docs[q], R) : R;
+    if (hs[q] == 0 || hs[q] > t_max) return fail(MVR_ERR_INVALID, "query %u has %u tokens", q, hs[q]);
+  }
+  HIP_TRY(ws.rr_meta.ensure((size_t)n_q * 8));
+  HIP_TRY(hipMemcpyAsync(ws.rr_meta.p, hs, (size_t)n_q * 8, hipMemcpyHostToDevice, s));
+  HIP_TRY(hipEventRecord(ws.staging_done, s));
+  const uint32_t *d_nt = ws.rr_meta.as<uint32_t>(), *d_n = d_nt + n_q;
+  HIP_TRY(launch_rerank(Q_dev, t_max, d_nt, n_q, doc_ids_dev, d_n, R, ix->d, (int)precision, 0,
+                        scores_dev, s));
+  return MVR_OK;
+}
+
+int mvr_index_set_probe(mvr_index *ix, mvr_mlp *mlp, float conf, uint32_t fallback_nprobe,
+                        uint32_t r_min) {
+  if (!ix) return fail(MVR_ERR_INVALID, "NULL index");
+  if (mlp && (fallback_nprobe == 0 || fallback_nprobe > kMaxProbe))
+    return fail(MVR_ERR_INVALID, "fallback nprobe %u not in [1, 32]", fallback_nprobe);
+  ix->probe = mlp;
+  ix->probe_conf = conf;
+  ix->probe_fallback = fallback_nprobe;
+  ix->probe_rmin = r_min;
   return MVR_OK;
 }
 

@@ -62,9 +62,17 @@ hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t
                          const DevIndex &ix, const uint64_t *cand_off, uint32_t *cand_cnt,
                          uint64_t *cand_keys, hipStream_t s);
 
+// top-R per query (q_r, nullable: per-query R <= R; R is the output row stride)
 hipError_t launch_select(const uint64_t *cand_keys, const uint64_t *cand_off,
-                         const uint32_t *cand_cnt, uint32_t n_q, uint32_t R, uint32_t *sel_ids,
-                         float *sel_F, uint32_t *sel_n, hipStream_t s);
+                         const uint32_t *cand_cnt, const uint32_t *q_r, uint32_t n_q, uint32_t R,
+                         uint32_t *sel_ids, float *sel_F, uint32_t *sel_n, hipStream_t s);
+
+// learned probe: features [n_q][1024] from the top-32 scores, and decisions
+hipError_t launch_probe_features(const float *top_s, uint32_t t_max, uint32_t P,
+                                 const uint32_t *q_ntok, uint32_t n_q, float *X, hipStream_t s);
+hipError_t launch_probe_decide(const float *O, uint32_t n_q, float conf, uint32_t fallback,
+                               uint32_t r_min, uint32_t r_max, uint32_t *q_np, uint32_t *q_r,
+                               hipStream_t s);
 
 hipError_t launch_rerank(const float *Q, uint32_t t_max, const uint32_t *q_ntok, uint32_t n_q,
                          const uint32_t *sel_ids, const uint32_t *sel_n, uint32_t R,

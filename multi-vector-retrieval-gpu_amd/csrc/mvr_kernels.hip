@@ -304,18 +304,39 @@ __global__ __launch_bounds__(256) void k_filter_seg(
   }
   __syncthreads();
 
-  const uint32_t lane = tid & (kWave - 1);
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (uint32_t dl = tid; dl < kSeg; dl += 256) {
-    const uint32_t doc = seg_lo + dl;
-    const bool has = doc < n_doc && mask[dl] != 0u;
-    const uint64_t bal = __ballot(has);
-    uint32_t base = 0;
-    if (lane == 0 && bal) base = atomicAdd(&cand_cnt[q], (uint32_t)__popcll(bal));
-    base = __shfl(base, 0);
-    if (has) {
+  // Emission: count this thread's candidates (slots tid + 256k), scan the
+  // counts across the workgroup, and reserve the whole block's range with a
+  // single atomic on the query's counter.
+  const uint32_t lane = tid & (kWave - 1), wv = tid >> 6;
+  uint32_t hasbits = 0, myc = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < kSeg / 256; ++k) {
+    const uint32_t dl = tid + 256 * k;
+    if (seg_lo + dl < n_doc && mask[dl] != 0u) { hasbits |= 1u << k; ++myc; }
+  }
+  uint32_t incl = myc;
+#pragma unroll
+  for (uint32_t off = 1; off < kWave; off <<= 1) {
+    const uint32_t u = __shfl_up(incl, off, kWave);
+    if (lane >= off) incl += u;
+  }
+  __shared__ uint32_t wsum[4], blk_base;
+  if (lane == kWave - 1) wsum[wv] = incl;
+  __syncthreads();
+  if (tid == 0) {
+    const uint32_t tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    blk_base = tot ? atomicAdd(&cand_cnt[q], tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t pos = blk_base + incl - myc;
+  for (uint32_t w = 0; w < wv; ++w) pos += wsum[w];
+  uint64_t *out = cand_keys + cand_off[q];
+#pragma unroll
+  for (uint32_t k = 0; k < kSeg / 256; ++k) {
+    if (hasbits & (1u << k)) {
+      const uint32_t dl = tid + 256 * k;
       const float F = (float)((double)(long long)acc[dl] * (1.0 / kFxScale));
-      cand_keys[cand_off[q] + base + __popcll(bal & lt)] = make_key(F, doc);
+      out[pos++] = make_key(F, seg_lo + dl);
     }
   }
 }
@@ -355,8 +376,8 @@ constexpr uint32_t kSelScratch = 2048;
 
 __global__ __launch_bounds__(kSelThreads) void k_select_topr(
     const uint64_t *__restrict__ cand_keys, const uint64_t *__restrict__ cand_off,
-    const uint32_t *__restrict__ cand_cnt, uint32_t R, uint32_t *__restrict__ sel_ids,
-    float *__restrict__ sel_F, uint32_t *__restrict__ sel_n) {
+    const uint32_t *__restrict__ cand_cnt, const uint32_t *__restrict__ q_r, uint32_t R_out,
+    uint32_t *__restrict__ sel_ids, float *__restrict__ sel_F, uint32_t *__restrict__ sel_n) {
   __shared__ unsigned long long outk[kMaxSelect];
   __shared__ unsigned long long scr[kSelScratch];
   __shared__ uint32_t hist[2048];
@@ -365,6 +386,8 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
   const uint32_t q = blockIdx.x, tid = threadIdx.x;
   const uint32_t n = cand_cnt[q];
   const uint64_t *keys = cand_keys + cand_off[q];
+  // R_out is the row stride of the outputs; q_r (nullable) lowers R per query
+  const uint32_t R = q_r ? max(1u, min(R_out, q_r[q])) : R_out;
   const uint32_t n_sel = min(n, R);
 
   if (tid == 0) { s_out_n = 0; s_scr_n = 0; }
@@ -449,8 +472,8 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
     __syncthreads();
     bitonic_desc(outk, np);
   }
-  for (uint32_t i = tid; i < R; i += kSelThreads) {
-    const size_t o = (size_t)q * R + i;
+  for (uint32_t i = tid; i < R_out; i += kSelThreads) {
+    const size_t o = (size_t)q * R_out + i;
     if (i < n_sel) {
       const unsigned long long k = outk[i];
       sel_ids[o] = ~(uint32_t)(k & 0xFFFFFFFFull);
@@ -466,18 +489,23 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
 // ---------------------------------------------------------------------------
 // Fused decompress + MaxSim, FP32.  Workgroup (slice, query), 4 waves; each
 // wave takes candidate docs r = slice*4 + wave, stepping by 4*n_slice.  Lane
-// l owns query token t = l & 31 (its 128 values in VGPRs) and token parity
-// h = l >> 5: a step decodes two doc tokens (i, i+1) straight from the packed
-// codes into LDS (lane l writes dims 4t..4t+3 of token i+h: one code word, one
-// residual byte, one float4 of the centroid row) and each lane runs the dot
-// product of its query token with token i+h as one FMA chain, reading the
-// decoded token as LDS broadcasts.  No decompressed embedding touches HBM.
+// l owns query token t = l & 31 (its 128 values in VGPRs) and parity
+// h = l >> 5.  A step covers eight doc tokens: lane l decodes dims 4t..4t+3 of
+// tokens h, h+2, h+4, h+6 straight from the packed codes into LDS (code word,
+// one residual byte, one float4 of the centroid row each), then runs four
+// independent FMA chains -- its query token against those four tokens --
+// reading the decoded tokens as LDS broadcasts.  Each dot product is still
+// one sequential chain over d, so results are bit-identical to the oracle.
+// No decompressed embedding touches HBM.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float pick4(uint32_t b, float w0, float w1, float w2, float w3) {
   const float lo = (b & 1u) ? w1 : w0;
   const float hi = (b & 1u) ? w3 : w2;
   return (b & 2u) ? hi : lo;
 }
+
+constexpr uint32_t kRrTok = 8;   // doc tokens per step
+constexpr uint32_t kRrChains = kRrTok / 2;
 
 __global__ __launch_bounds__(256) void k_rerank_f32(
     const float *__restrict__ Q, uint32_t t_max, const uint32_t *__restrict__ q_ntok,
@@ -486,7 +514,7 @@ __global__ __launch_bounds__(256) void k_rerank_f32(
     const uint32_t *__restrict__ codes, const uint8_t *__restrict__ residual,
     const float *__restrict__ C, const float *__restrict__ W, int clip_zero,
     float *__restrict__ sel_score) {
-  __shared__ __align__(16) float4 xs[4][2][kDim / 4];
+  __shared__ __align__(16) float4 xs[4][kRrTok][kDim / 4];
   __shared__ float Ms[4][kMaxQTok];
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const uint32_t h = lane >> 5, t = lane & 31;
@@ -515,34 +543,41 @@ __global__ __launch_bounds__(256) void k_rerank_f32(
     const uint32_t L = doclens[doc];
     const uint64_t off = docoff[doc];
     float M = clip_zero ? 0.0f : -FLT_MAX;
-    for (uint32_t i = 0; i < L; i += 2) {
-      const uint32_t tok = i + h;
-      const bool vt = tok < L;
-      if (vt) {
-        const uint64_t g = off + tok;
-        const uint32_t code = codes[g];
-        const uint32_t byte = residual[g * kResBytes + t];
-        const float4 c = reinterpret_cast<const float4 *>(C + (size_t)code * kDim)[t];
-        float4 x;
-        x.x = c.x + pick4((byte >> 6) & 3u, w[0][0], w[0][1], w[0][2], w[0][3]);
-        x.y = c.y + pick4((byte >> 4) & 3u, w[1][0], w[1][1], w[1][2], w[1][3]);
-        x.z = c.z + pick4((byte >> 2) & 3u, w[2][0], w[2][1], w[2][2], w[2][3]);
-        x.w = c.w + pick4(byte & 3u, w[3][0], w[3][1], w[3][2], w[3][3]);
-        xs[wave][h][t] = x;
+    for (uint32_t i = 0; i < L; i += kRrTok) {
+#pragma unroll
+      for (uint32_t c = 0; c < kRrChains; ++c) {
+        const uint32_t tk = 2 * c + h;
+        if (i + tk < L) {
+          const uint64_t g = off + i + tk;
+          const uint32_t code = codes[g];
+          const uint32_t byte = residual[g * kResBytes + t];
+          const float4 cv = reinterpret_cast<const float4 *>(C + (size_t)code * kDim)[t];
+          float4 x;
+          x.x = cv.x + pick4((byte >> 6) & 3u, w[0][0], w[0][1], w[0][2], w[0][3]);
+          x.y = cv.y + pick4((byte >> 4) & 3u, w[1][0], w[1][1], w[1][2], w[1][3]);
+          x.z = cv.z + pick4((byte >> 2) & 3u, w[2][0], w[2][1], w[2][2], w[2][3]);
+          x.w = cv.w + pick4(byte & 3u, w[3][0], w[3][1], w[3][2], w[3][3]);
+          xs[wave][tk][t] = x;
+        }
       }
       wave_lds_sync();
-      if (vt) {
-        float a = 0.f;
+      float a[kRrChains];
 #pragma unroll
-        for (uint32_t d4 = 0; d4 < kDim / 4; ++d4) {
-          const float4 xv = xs[wave][h][d4];
-          a = __builtin_fmaf(qv[4 * d4 + 0], xv.x, a);
-          a = __builtin_fmaf(qv[4 * d4 + 1], xv.y, a);
-          a = __builtin_fmaf(qv[4 * d4 + 2], xv.z, a);
-          a = __builtin_fmaf(qv[4 * d4 + 3], xv.w, a);
+      for (uint32_t c = 0; c < kRrChains; ++c) a[c] = 0.f;
+#pragma unroll
+      for (uint32_t d4 = 0; d4 < kDim / 4; ++d4) {
+#pragma unroll
+        for (uint32_t c = 0; c < kRrChains; ++c) {
+          const float4 xv = xs[wave][2 * c + h][d4];
+          a[c] = __builtin_fmaf(qv[4 * d4 + 0], xv.x, a[c]);
+          a[c] = __builtin_fmaf(qv[4 * d4 + 1], xv.y, a[c]);
+          a[c] = __builtin_fmaf(qv[4 * d4 + 2], xv.z, a[c]);
+          a[c] = __builtin_fmaf(qv[4 * d4 + 3], xv.w, a[c]);
         }
-        M = fmaxf(M, a);
       }
+#pragma unroll
+      for (uint32_t c = 0; c < kRrChains; ++c)
+        if (i + 2 * c + h < L) M = fmaxf(M, a[c]);
       wave_lds_sync();
     }
     M = fmaxf(M, __shfl_xor(M, 32));
@@ -1104,10 +1139,10 @@ hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t
 }
 
 hipError_t launch_select(const uint64_t *cand_keys, const uint64_t *cand_off,
-                         const uint32_t *cand_cnt, uint32_t n_q, uint32_t R, uint32_t *sel_ids,
-                         float *sel_F, uint32_t *sel_n, hipStream_t s) {
+                         const uint32_t *cand_cnt, const uint32_t *q_r, uint32_t n_q, uint32_t R,
+                         uint32_t *sel_ids, float *sel_F, uint32_t *sel_n, hipStream_t s) {
   if (n_q == 0) return hipSuccess;
-  k_select_topr<<<n_q, kSelThreads, 0, s>>>(cand_keys, cand_off, cand_cnt, R, sel_ids, sel_F,
+  k_select_topr<<<n_q, kSelThreads, 0, s>>>(cand_keys, cand_off, cand_cnt, q_r, R, sel_ids, sel_F,
                                             sel_n);
   return hipGetLastError();
 }

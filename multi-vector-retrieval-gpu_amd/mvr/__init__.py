@@ -126,18 +126,19 @@ class Index:
 
     # ---------------------------------------------------------------- search
     @staticmethod
-    def _params(nprobe, probe_topk, topk, precision, rescore, clip_zero, batch):
+    def _params(nprobe, probe_topk, topk, precision, rescore, clip_zero, batch, learned=False):
         p = _capi.SearchParams()
         p.nprobe, p.probe_topk, p.topk = nprobe, probe_topk, topk
         p.precision = _PRECISION.get(precision, precision) if isinstance(precision, str) else precision
         p.rescore = rescore
         p.clip_zero = int(clip_zero)
         p.batch = batch
+        p.learned = int(learned)
         return p
 
     def search(self, Q, nprobe: int, probe_topk: int, topk: int, precision="fp32", rescore: int = 0,
                q_ntok=None, q_nprobe=None, clip_zero=False, batch: int = 0, stages=False,
-               timings=False):
+               timings=False, learned=False):
         """Host-buffer search.  Q: [n_q, t_max, 128] float32.  Returns
         (ids [n_q, topk] uint32, scores [n_q, topk] float32[, stages][, timings])."""
         Q = _c(Q, np.float32)
@@ -146,11 +147,11 @@ class Index:
         sc = np.empty((n_q, topk), np.float32)
         qt = None if q_ntok is None else _c(q_ntok, np.uint32)
         qp = None if q_nprobe is None else _c(q_nprobe, np.uint32)
-        prm = self._params(nprobe, probe_topk, topk, precision, rescore, clip_zero, batch)
+        prm = self._params(nprobe, probe_topk, topk, precision, rescore, clip_zero, batch, learned)
         st = None
         so = None
         if stages:
-            pmax = max(nprobe, int(qp.max()) if qp is not None else 0)
+            pmax = 32 if learned else max(nprobe, int(qp.max()) if qp is not None else 0)
             pmax = min(pmax, self.n_centroid)
             st = {"top_ids": np.empty((n_q, t_max, pmax), np.uint32),
                   "top_s": np.empty((n_q, t_max, pmax), np.float32),
@@ -172,12 +173,13 @@ class Index:
         return tuple(out)
 
     def search_device(self, Q, out_ids, out_scores, nprobe, probe_topk, topk, precision="fp32",
-                      rescore=0, q_ntok=None, q_nprobe=None, clip_zero=False, batch=0, stream=0):
+                      rescore=0, q_ntok=None, q_nprobe=None, clip_zero=False, batch=0, stream=0,
+                      learned=False):
         """Device-buffer search (torch tensors); queued on `stream`, no host sync."""
         n_q, t_max, _ = Q.shape
         qt = None if q_ntok is None else _c(q_ntok, np.uint32)
         qp = None if q_nprobe is None else _c(q_nprobe, np.uint32)
-        prm = self._params(nprobe, probe_topk, topk, precision, rescore, clip_zero, batch)
+        prm = self._params(nprobe, probe_topk, topk, precision, rescore, clip_zero, batch, learned)
         check(lib().mvr_search(self._h, Q.data_ptr(), n_q, t_max, _ptr(qt), _ptr(qp), C.byref(prm),
                                out_ids.data_ptr(), out_scores.data_ptr(), 1, stream or None, None, None))
 

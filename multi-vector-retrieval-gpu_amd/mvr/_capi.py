@@ -31,7 +31,7 @@ class IndexDesc(C.Structure):
 class SearchParams(C.Structure):
     _fields_ = [("nprobe", C.c_uint32), ("probe_topk", C.c_uint32), ("topk", C.c_uint32),
                 ("precision", C.c_uint32), ("rescore", C.c_uint32), ("clip_zero", C.c_uint32),
-                ("batch", C.c_uint32), ("reserved", C.c_uint32)]
+                ("batch", C.c_uint32), ("learned", C.c_uint32)]
 
 
 class StageOut(C.Structure):
@@ -58,8 +58,22 @@ SIGNATURES = {
                              C.POINTER(SearchParams), C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                              C.POINTER(StageOut), C.POINTER(Timings)]),
     "mvr_decompress": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "mvr_rerank_docs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                  C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "mvr_exhaustive_f64": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "mvr_mlp_last_error": (C.c_char_p, []),
+    "mvr_mlp_create": (C.c_int, [C.POINTER(C.c_uint32), C.c_int, C.POINTER(C.c_void_p)]),
+    "mvr_mlp_free": (None, [C.c_void_p]),
+    "mvr_mlp_num_params": (C.c_uint64, [C.c_void_p]),
+    "mvr_mlp_set_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mvr_mlp_get_params": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "mvr_mlp_forward": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "mvr_mlp_grads": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                C.c_void_p, C.POINTER(C.c_float)]),
+    "mvr_mlp_train_step": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                     C.c_float, C.POINTER(C.c_float), C.c_void_p]),
+    "mvr_index_set_probe": (C.c_int, [C.c_void_p, C.c_void_p, C.c_float, C.c_uint32, C.c_uint32]),
     "mvr_synth_tokens": (C.c_int, [C.c_uint64, C.c_uint64, C.c_uint64, C.c_void_p, C.c_uint32,
                                    C.c_uint32, C.c_double, C.c_uint64, C.c_uint64, C.c_void_p,
                                    C.c_void_p]),
@@ -82,6 +96,13 @@ def lib():
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is not built; run __graft_entry__.build() or "
                            f"python multi-vector-retrieval-gpu_amd/build.py lib")
+    # libmvr links libamdhip64; torch ships its own copy under the same
+    # soname.  Load torch first so both share one HIP runtime (a second
+    # runtime would see no devices once torch owns them).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(L, name)
@@ -94,3 +115,8 @@ def lib():
 def check(status: int):
     if status != MVR_OK:
         raise MvrError(status, lib().mvr_last_error().decode())
+
+
+def check_mlp(status: int):
+    if status != MVR_OK:
+        raise MvrError(status, lib().mvr_mlp_last_error().decode())

@@ -19,7 +19,13 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 @pytest.fixture(scope="module")
 def probe():
-    return ctypes.CDLL(os.path.join(HERE, "probes", "libmvr_probe.so"))
+    lib = ctypes.CDLL(os.path.join(HERE, "probes", "libmvr_probe.so"))
+    vp = ctypes.c_void_p
+    lib.probe_mfma.argtypes = [ctypes.c_int, vp, vp, vp]
+    lib.probe_mfma.restype = ctypes.c_int
+    lib.probe_cvt_fp8.argtypes = [vp, vp, ctypes.c_uint32]
+    lib.probe_cvt_fp8.restype = ctypes.c_int
+    return lib
 
 
 def bf16_bits(x):

@@ -161,17 +161,27 @@ def test_bf16_rerank_scores_within_tolerance(small):
 
 
 def test_bf16_with_fp32_rescore_returns_fp32_scores(c1):
-    """BF16 MaxSim over R=256, FP32 rescoring of the best 64, top-10: the
-    returned scores are the FP32 (oracle) scores of the returned docs and
-    match the FP32 pipeline's top-10."""
+    """BF16 MaxSim over R=256, FP32 rescoring of the best 64, top-10.  Exact
+    check of the rescore logic: from the BF16 path's own candidates and BF16
+    scores (stage outputs) take the best 64 by (score desc, id asc), score them
+    with the FP32 oracle, keep the best 10 -- that must be the answer, ids and
+    scores.  Also: the answer agrees with the plain FP32 pipeline's top-10."""
     corpus, ix, Q = c1
     g = _index(ix)
-    ids, sc = g.search(Q, 4, 256, 10, precision="bf16", rescore=64)
+    ids, sc, st = g.search(Q, 4, 256, 10, precision="bf16", rescore=64, stages=True)
     oi, os_ = ix.search(Q, 4, 256, 10)
     for q in range(Q.shape[0]):
-        np.testing.assert_array_equal(sc[q], ix.rerank(Q[q], ids[q]))
-    same = np.mean([set(a.tolist()) == set(b.tolist()) for a, b in zip(ids, oi)])
-    assert same >= 0.95, same
+        n = min(int(st["cand_n"][q]), 256)
+        cid = st["sel_ids"][q, :n].astype(np.int64)
+        cs = st["sel_score"][q, :n]
+        best = np.lexsort((cid, -cs.astype(np.float64)))[:64]
+        rid = cid[best]
+        rs = ix.rerank(Q[q], rid.astype(np.uint32))
+        top = np.lexsort((rid, -rs.astype(np.float64)))[:10]
+        np.testing.assert_array_equal(ids[q], rid[top].astype(np.uint32))
+        np.testing.assert_array_equal(sc[q], rs[top])
+    same = np.mean([len(set(a.tolist()) & set(b.tolist())) / 10 for a, b in zip(ids, oi)])
+    assert same >= 0.8, same
 
 
 def test_bf16_centroid_topk_matches_rounded_input_oracle(c1):
