@@ -53,13 +53,18 @@ def main():
     ap.add_argument("--rescore", type=int, default=200)
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--recall-queries", type=int, default=256)
-    ap.add_argument("--cpu-queries", type=int, default=32)
+    ap.add_argument("--cpu-queries", type=int, default=640)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path is query "
               f"data-parallel and would shard by replicating the index per GPU.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
+        return
 
     import torch
     import mvr
@@ -99,13 +104,34 @@ def main():
         ix.search_device(Q, out_ids, out_s, P, R, k, args.precision, rescore, batch=batch, stream=stream)
 
     for _ in range(args.warmup):
-        step()
+        step(timings=True)
     torch.cuda.synchronize()
-    # timed region: K whole-batch searches, per-pass latency from HIP events
-    ev = []
+    # Timed region: K whole-batch searches.  Each call records HIP events
+    # around every stage of every 1,024-query pass on the search stream (read
+    # back once per call), so the rerank kernel's launch durations come from
+    # the timed work itself.
+    stage_keys = ("centroid_ms", "filter_ms", "select_ms", "rerank_ms", "rescore_ms", "topk_ms", "total_ms")
+    acc = {f: 0.0 for f in stage_keys}
+    n_launch = 0
+    tm = None
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
+        tm = step(timings=True)
+        for f in stage_keys:
+            acc[f] += tm[f]
+        n_launch += tm["n_pass"]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed * 1e3 / args.steps
+    qps = n_q * args.steps / elapsed
+    log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, {qps:.0f} QPS")
+    stages = {f: acc[f] / args.steps for f in stage_keys}
+    log(f"[bench] stages (ms/step): " + ", ".join(f"{f}={stages[f]:.2f}" for f in stage_keys))
+
+    # per-pass latency: single 1,024-query passes bracketed by events on the search stream
+    ev = []
+    for _ in range(max(2, min(args.steps, 8))):
         for b0 in range(0, n_q, batch):
             e0 = torch.cuda.Event(enable_timing=True)
             e1 = torch.cuda.Event(enable_timing=True)
@@ -115,39 +141,20 @@ def main():
             e1.record()
             ev.append((e0, e1))
     torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    ms_per_step = elapsed * 1e3 / args.steps
-    qps = n_q * args.steps / elapsed
     lat = np.array([a.elapsed_time(b) for a, b in ev])
-    log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, {qps:.0f} QPS")
 
-    # one instrumented step: per-stage device time and work counters
-    tm = step(timings=True)
-    torch.cuda.synchronize()
-    rr_ms = tm["rerank_ms"]
-    n_pass = (n_q + batch - 1) // batch
-    flops = 2.0 * 32 * tm["rerank_tokens"] * 128
-    bytes_alg = 36.0 * tm["rerank_tokens"]
+    # roofline of the MaxSim kernel: algorithmic flops / bytes per launch over
+    # its average launch duration in the timed region
+    tok_per_launch = tm["rerank_tokens"] / tm["n_pass"]
+    avg_launch_ms = acc["rerank_ms"] / n_launch
+    flops = 2.0 * 32 * 128 * tok_per_launch
+    bytes_alg = 36.0 * tok_per_launch
     peak = PEAK_TFLOPS[args.precision]
-    achieved_tf = flops / (rr_ms / n_pass * 1e-3) / n_pass / 1e12 if rr_ms > 0 else 0.0
-    log(f"[bench] stages (ms/step): " + ", ".join(f"{f}={tm[f]:.2f}" for f in
-                                                 ("centroid_ms", "filter_ms", "select_ms", "rerank_ms", "topk_ms")))
+    achieved_tf = flops / (avg_launch_ms * 1e-3) / 1e12
+    kname = "k_rerank_f32" if args.precision == "fp32" else f"k_rerank_{args.precision}"
+    traffic = measured_traffic(kname, args.config, args.precision)
 
-    # recall@k against FP64 exhaustive MaxSim over the original embeddings
-    nr = min(args.recall_queries, n_q)
-    recall = None
-    if nr > 0 and corpus.X is not None:
-        ids_h = out_ids[:nr].cpu().numpy()
-        dl = torch.from_numpy(doclens.astype(np.int32)).cuda()
-        scores = torch.empty((nr, cfg["n_doc"]), dtype=torch.float64, device="cuda")
-        t1 = time.time()
-        from mvr import _capi
-        _capi.check(_capi.lib().mvr_exhaustive_f64(Q.data_ptr(), nr, 32, None, corpus.X.data_ptr(),
-                                                   dl.data_ptr(), cfg["n_doc"], scores.data_ptr(), None))
-        gnd = torch.topk(scores, k, dim=1).indices.cpu().numpy()
-        recall = float(np.mean([len(set(a.tolist()) & set(b.tolist())) / k for a, b in zip(ids_h, gnd)]))
-        log(f"[bench] recall@{k} vs FP64 exhaustive on {nr} queries: {recall:.4f} ({time.time() - t1:.1f}s)")
-
+    recall = recall_report(torch, args, cfg, corpus, ix, Q, out_ids, k, doclens)
     cpu = None
     if not args.no_cpu and args.cpu_queries > 0:
         cpu = cpu_baseline(torch, corpus, Q, P, R, k, args.cpu_queries)
@@ -172,17 +179,89 @@ def main():
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3),
                        "unit": f"per {batch}-query device pass"},
-        "recall_at_k": recall,
-        "stages_ms_per_step": {f: round(tm[f], 3) for f in
-                               ("centroid_ms", "filter_ms", "select_ms", "rerank_ms", "topk_ms", "total_ms")},
-        "roofline": {"kernel": "k_rerank_f32" if args.precision == "fp32" else f"k_rerank_{args.precision}",
-                     "bound": "mfma", "achieved": round(achieved_tf, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved_tf / peak, 4), "traffic": None,
-                     "hbm_achieved_GBs": round(bytes_alg / (rr_ms * 1e-3) / 1e9, 1) if rr_ms > 0 else None,
-                     "rerank_tokens_per_step": int(tm["rerank_tokens"])},
+        "recall_at_k": recall.get("search_vs_fp64") if recall else None,
+        "recall": recall,
+        "stages_ms_per_step": {f: round(stages[f], 3) for f in stage_keys},
+        "roofline": {"kernel": kname, "bound": "mfma", "achieved": round(achieved_tf, 3), "peak": peak,
+                     "unit": "TFLOP/s", "frac": round(achieved_tf / peak, 4), "traffic": traffic,
+                     "avg_launch_ms": round(avg_launch_ms, 4), "launches": n_launch,
+                     "doc_tokens_per_launch": int(tok_per_launch),
+                     "flop_per_doc_token": 2 * 32 * 128, "alg_bytes_per_doc_token": 36,
+                     "alg_GBs": round(bytes_alg / (avg_launch_ms * 1e-3) / 1e9, 1)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def measured_traffic(kname, config, precision):
+    """HBM bytes per launch of `kname` from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration),
+    or None when that workload has not been profiled."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            rows = json.load(f)
+    except (OSError, ValueError):
+        return None
+    for r in rows:
+        if r.get("kernel") == kname and r.get("config") == config and r.get("precision") == precision:
+            return r.get("bytes_per_launch")
+    return None
+
+
+def recall_report(torch, args, cfg, corpus, ix, Q, out_ids, k, doclens):
+    """recall@k of the search against FP64 exhaustive MaxSim over the original
+    embeddings, and the two numbers that bound it: exhaustive FP32 MaxSim over
+    the decompressed (2-bit) index against the same FP64 truth (the ceiling any
+    search over this index can reach), and the search against that exhaustive
+    decompressed ranking (what the probing loses)."""
+    nr = min(args.recall_queries, Q.shape[0])
+    if nr <= 0 or corpus.X is None:
+        return None
+    from mvr import _capi
+    import ctypes as C
+    t1 = time.time()
+    n_doc = cfg["n_doc"]
+    ids_h = out_ids[:nr].cpu().numpy()
+    dl = torch.from_numpy(doclens.astype(np.int32)).cuda()
+    s64 = torch.empty((nr, n_doc), dtype=torch.float64, device="cuda")
+    _capi.check(_capi.lib().mvr_exhaustive_f64(Q.data_ptr(), nr, 32, None, corpus.X.data_ptr(),
+                                               dl.data_ptr(), n_doc, s64.data_ptr(), None))
+    gnd = torch.topk(s64, k, dim=1).indices.cpu().numpy()
+    del s64
+    docs = torch.arange(n_doc, dtype=torch.int32, device="cuda")
+    dec = np.empty((nr, k), np.int64)
+    stream = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    for q0 in range(0, nr, 64):
+        nb = min(64, nr - q0)
+        s32 = torch.empty((nb, n_doc), dtype=torch.float32, device="cuda")
+        _capi.check(_capi.lib().mvr_rerank_docs(ix._h, Q[q0:q0 + nb].contiguous().data_ptr(), nb, 32, None,
+                                                docs.expand(nb, n_doc).contiguous().data_ptr(), None, n_doc,
+                                                0, s32.data_ptr(), stream))
+        dec[q0:q0 + nb] = torch.topk(s32, k, dim=1).indices.cpu().numpy()
+
+    def ov(a, b):
+        return float(np.mean([len(set(x.tolist()) & set(y.tolist())) / k for x, y in zip(a, b)]))
+
+    rep = {"queries": nr, "k": k, "search_vs_fp64": ov(ids_h, gnd), "exhaustive_2bit_vs_fp64": ov(dec, gnd),
+           "search_vs_exhaustive_2bit": ov(ids_h, dec)}
+    log(f"[bench] recall@{k} on {nr} queries: search vs FP64 exhaustive {rep['search_vs_fp64']:.4f}; "
+        f"exhaustive over the 2-bit index vs FP64 {rep['exhaustive_2bit_vs_fp64']:.4f}; search vs "
+        f"exhaustive 2-bit {rep['search_vs_exhaustive_2bit']:.4f} ({time.time() - t1:.1f}s)")
+    return rep
+
+
+def copy_probe(mib):
+    """A plain device copy of known size, the calibration run for FETCH_SIZE /
+    WRITE_SIZE counters."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "launches": 3}), flush=True)
 
 
 def cpu_baseline(torch, corpus, Q, P, R, k, n_sample):
@@ -201,7 +280,7 @@ def cpu_baseline(torch, corpus, Q, P, R, k, n_sample):
                  corpus.C.cpu().numpy(), corpus.W.cpu().numpy(), 2, ivf)
     Qh = Q[:n_sample].cpu().numpy()
     log(f"[bench] cpu baseline: index on host in {time.time() - t0:.1f}s, {cores} cores")
-    ids, s, secs = ix.search(Qh, P, R, k, n_threads=cores, timing=True)
+    ix.search(Qh[:16], P, R, k, n_threads=cores)  # warm the host index pages
     t1 = time.time()
     ids, s, secs = ix.search(Qh, P, R, k, n_threads=cores, timing=True)
     wall = time.time() - t1

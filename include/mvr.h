@@ -90,10 +90,13 @@ typedef struct {
 
 /* Per-call device time per stage in milliseconds (HIP events on the call's
  * stream, summed over passes) and the work done: reranked doc tokens (sum of
- * the lengths of every reranked doc) and filter candidates, summed over queries. */
+ * the lengths of every reranked doc) and filter candidates, summed over queries.
+ * rerank_ms is the MaxSim kernel alone; rescore_ms the FP32 rescore of a
+ * BF16/FP8 search (its top-N selection and FP32 MaxSim; 0 otherwise). */
 typedef struct {
-  double centroid_ms, filter_ms, select_ms, rerank_ms, topk_ms, total_ms;
+  double centroid_ms, filter_ms, select_ms, rerank_ms, rescore_ms, topk_ms, total_ms;
   uint64_t rerank_tokens, candidates;
+  uint32_t n_pass;     /* device passes the call made */
 } mvr_timings;
 
 int mvr_device_count(int *count);

@@ -489,19 +489,20 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     HIP_TRY(ws.out_s.ensure(Bc * k * 4));
   }
 
-  // Stage timings: six events per device pass, read once at the end of the
+  // Stage timings: seven events per device pass, read once at the end of the
   // call so that timing adds no host synchronisation between passes.
   const uint32_t n_pass = (n_q + B - 1) / B;
+  constexpr int kNev = 7;
   std::vector<hipEvent_t> ev;
   if (timings) {
-    ev.resize((size_t)n_pass * 6);
+    ev.resize((size_t)n_pass * kNev);
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
     *timings = mvr_timings{};
     HIP_TRY(ws.counters.ensure(16));
     HIP_TRY(launch_fill_u32(ws.counters.as<uint32_t>(), 0u, 4, s));
   }
   auto rec = [&](uint32_t pass, int i) -> hipError_t {
-    return timings ? hipEventRecord(ev[(size_t)pass * 6 + i], s) : hipSuccess;
+    return timings ? hipEventRecord(ev[(size_t)pass * kNev + i], s) : hipSuccess;
   };
 
   int bi = 0;
@@ -558,6 +559,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     HIP_TRY(rec(bi, 3));
     HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R,
                           d, prm->precision, (int)prm->clip_zero, ws.sel_score.as<float>(), s));
+    HIP_TRY(rec(bi, 4));
     if (R2) {
       // best R2 by the low-precision score, FP32 MaxSim of those, top-k of the FP32 scores
       HIP_TRY(launch_topk(ws.sel_ids.as<uint32_t>(), ws.sel_score.as<float>(), ws.sel_n.as<uint32_t>(),
@@ -566,14 +568,14 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2,
                             d, MVR_FP32, (int)prm->clip_zero, ws.res_s.as<float>(), s));
     }
-    HIP_TRY(rec(bi, 4));
+    HIP_TRY(rec(bi, 5));
     if (R2)
       HIP_TRY(launch_topk(ws.res_ids.as<uint32_t>(), ws.res_s.as<float>(), ws.res_n.as<uint32_t>(), R2,
                           nb, k, oid, os, nullptr, s));
     else
       HIP_TRY(launch_topk(ws.sel_ids.as<uint32_t>(), ws.sel_score.as<float>(), ws.sel_n.as<uint32_t>(),
                           R, nb, k, oid, os, nullptr, s));
-    HIP_TRY(rec(bi, 5));
+    HIP_TRY(rec(bi, 6));
     if (!on_device) {
       HIP_TRY(hipMemcpyAsync(out_ids + (size_t)b0 * k, oid, (size_t)nb * k * 4, hipMemcpyDeviceToHost, s));
       HIP_TRY(hipMemcpyAsync(out_scores + (size_t)b0 * k, os, (size_t)nb * k * 4, hipMemcpyDeviceToHost, s));
@@ -600,15 +602,18 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   if (timings) {
     HIP_TRY(hipEventSynchronize(ev.back()));
     for (uint32_t b = 0; b < n_pass; ++b) {
-      float ms[5];
-      for (int i = 0; i < 5; ++i) HIP_TRY(hipEventElapsedTime(&ms[i], ev[b * 6 + i], ev[b * 6 + i + 1]));
+      float ms[kNev - 1];
+      for (int i = 0; i < kNev - 1; ++i)
+        HIP_TRY(hipEventElapsedTime(&ms[i], ev[b * kNev + i], ev[b * kNev + i + 1]));
       timings->centroid_ms += ms[0];
       timings->filter_ms += ms[1];
       timings->select_ms += ms[2];
       timings->rerank_ms += ms[3];
-      timings->topk_ms += ms[4];
-      timings->total_ms += ms[0] + ms[1] + ms[2] + ms[3] + ms[4];
+      timings->rescore_ms += ms[4];
+      timings->topk_ms += ms[5];
+      timings->total_ms += ms[0] + ms[1] + ms[2] + ms[3] + ms[4] + ms[5];
     }
+    timings->n_pass = n_pass;
     unsigned long long cnt[2] = {0, 0};
     HIP_TRY(hipMemcpy(cnt, ws.counters.p, 16, hipMemcpyDeviceToHost));
     timings->rerank_tokens = cnt[0];

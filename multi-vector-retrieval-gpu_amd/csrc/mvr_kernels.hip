@@ -1152,7 +1152,9 @@ hipError_t launch_rerank(const float *Q, uint32_t t_max, const uint32_t *q_ntok,
                          const DevIndex &ix, int precision, int clip_zero, float *sel_score,
                          hipStream_t s) {
   if (n_q == 0) return hipSuccess;
-  const uint32_t n_slice = R >= 256 ? 4 : 1;
+  // about 8k workgroups when the lists are long (exhaustive scoring), else
+  // 4 per query (16 waves sharing one candidate list)
+  const uint32_t n_slice = R >= 256 ? std::max(4u, std::min(R / 256, std::max(1u, 8192u / n_q))) : 1;
   dim3 grid(n_slice, n_q);
   if (precision == 0) {
     k_rerank_f32<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, sel_ids, sel_n, R, n_slice, ix.doclens,

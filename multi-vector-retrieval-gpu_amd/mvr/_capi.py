@@ -41,8 +41,9 @@ class StageOut(C.Structure):
 
 class Timings(C.Structure):
     _fields_ = [("centroid_ms", C.c_double), ("filter_ms", C.c_double), ("select_ms", C.c_double),
-                ("rerank_ms", C.c_double), ("topk_ms", C.c_double), ("total_ms", C.c_double),
-                ("rerank_tokens", C.c_uint64), ("candidates", C.c_uint64)]
+                ("rerank_ms", C.c_double), ("rescore_ms", C.c_double), ("topk_ms", C.c_double),
+                ("total_ms", C.c_double), ("rerank_tokens", C.c_uint64), ("candidates", C.c_uint64),
+                ("n_pass", C.c_uint32)]
 
 
 # Every symbol declared in include/mvr.h, with its ctypes signature.

@@ -10,6 +10,14 @@ for p in (ROOT, PKG):
     if p not in sys.path:
         sys.path.insert(0, p)
 
+# torch ships its own HIP runtime under the soname libamdhip64.so.7.  Load it
+# before any test opens a HIP library of this tree (libmvr, the MFMA probe), so
+# the process has one runtime and torch still sees the device.
+try:
+    import torch  # noqa: F401
+except ImportError:  # pragma: no cover
+    pass
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs on the GPU box)")
