@@ -93,13 +93,13 @@ struct HBuf {
 };
 
 struct Workspace {
-  DBuf Q, meta, part_ids, part_s, top_ids, top_s, cand_cnt, cand_keys, sel_ids, sel_F, sel_n,
+  DBuf Q, meta, part_ids, part_s, top_ids, top_s, cand_cnt, cand_tot, cand_keys, sel_ids, sel_F, sel_n,
       sel_score, out_ids, out_s, counters, res_ids, res_s, res_n, bmax, feat, mlp_out, dyn_np, dyn_r,
       rr_meta;
   HBuf staging;
   hipEvent_t staging_done = nullptr;
   void release() {
-    for (DBuf *b : {&Q, &meta, &part_ids, &part_s, &top_ids, &top_s, &cand_cnt, &cand_keys,
+    for (DBuf *b : {&Q, &meta, &part_ids, &part_s, &top_ids, &top_s, &cand_cnt, &cand_tot, &cand_keys,
                     &sel_ids, &sel_F, &sel_n, &sel_score, &out_ids, &out_s, &counters, &res_ids,
                     &res_s, &res_n, &bmax, &feat, &mlp_out, &dyn_np, &dyn_r, &rr_meta})
       b->release();
@@ -407,51 +407,47 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   }
   P = std::min(P, d.n_c);
   if (learned && P < kMaxProbe) return fail(MVR_ERR_INVALID, "learned probing needs >= 32 centroids");
-  const uint32_t B = prm->batch ? prm->batch : 1024;
+  // queries per device pass; the candidate pool holds a 4,096-slot region
+  // per (query, doc segment), so a pass is also capped by the pool budget
+  const uint64_t pool_per_q = (uint64_t)d.n_seg * kSeg * 8;
+  const uint32_t B = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>(prm->batch ? prm->batch : 1024, kPoolBudget / pool_per_q));
   DeviceGuard g(ix->device);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   Workspace &ws = ix->ws;
 
-  // ---- per-call metadata (host) : q_ntok, q_np, cand_off, row_map ----
+  // ---- per-call metadata (host) : q_ntok, q_np, row_map ----
   bool uniform = true;
   for (uint32_t q = 0; q < n_q; ++q) uniform &= (nt[q] == t_max);
-  std::vector<uint64_t> coff(n_q);
   std::vector<uint32_t> rmap;
   std::vector<uint32_t> rows_begin;  // per batch: first row-map entry
-  size_t max_pool = 0, max_rows = 0;
+  size_t max_rows = 0;
   for (uint32_t b0 = 0; b0 < n_q; b0 += B) {
     const uint32_t nb = std::min(B, n_q - b0);
-    uint64_t pool = 0;
     rows_begin.push_back((uint32_t)rmap.size());
     size_t rows = 0;
     for (uint32_t i = 0; i < nb; ++i) {
       const uint32_t q = b0 + i;
-      coff[q] = pool;
-      const uint64_t bound = std::min<uint64_t>(d.n_doc, (uint64_t)nt[q] * np[q] * d.max_list);
-      pool += bound;
       if (!uniform)
         for (uint32_t t = 0; t < nt[q]; ++t) rmap.push_back(i * t_max + t);
       rows += uniform ? t_max : nt[q];
     }
-    max_pool = std::max<size_t>(max_pool, pool);
     max_rows = std::max(max_rows, rows);
   }
   const size_t meta_u32 = 2 * (size_t)n_q + rmap.size();
-  const size_t meta_bytes = (size_t)n_q * 8 + meta_u32 * 4;
+  const size_t meta_bytes = meta_u32 * 4;
   // wait until the previous call's staging copy finished before reuse
   if (ws.staging_done) HIP_TRY(hipEventSynchronize(ws.staging_done));
   else HIP_TRY(hipEventCreateWithFlags(&ws.staging_done, hipEventDisableTiming));
   HIP_TRY(ws.staging.ensure(meta_bytes));
   uint8_t *hs = reinterpret_cast<uint8_t *>(ws.staging.p);
-  std::memcpy(hs, coff.data(), (size_t)n_q * 8);
-  std::memcpy(hs + (size_t)n_q * 8, nt.data(), (size_t)n_q * 4);
-  std::memcpy(hs + (size_t)n_q * 12, np.data(), (size_t)n_q * 4);
-  if (!rmap.empty()) std::memcpy(hs + (size_t)n_q * 16, rmap.data(), rmap.size() * 4);
+  std::memcpy(hs, nt.data(), (size_t)n_q * 4);
+  std::memcpy(hs + (size_t)n_q * 4, np.data(), (size_t)n_q * 4);
+  if (!rmap.empty()) std::memcpy(hs + (size_t)n_q * 8, rmap.data(), rmap.size() * 4);
   HIP_TRY(ws.meta.ensure(meta_bytes));
   HIP_TRY(hipMemcpyAsync(ws.meta.p, hs, meta_bytes, hipMemcpyHostToDevice, s));
   HIP_TRY(hipEventRecord(ws.staging_done, s));
-  const uint64_t *d_coff = reinterpret_cast<const uint64_t *>(ws.meta.p);
-  const uint32_t *d_nt = reinterpret_cast<const uint32_t *>(ws.meta.as<uint8_t>() + (size_t)n_q * 8);
+  const uint32_t *d_nt = ws.meta.as<uint32_t>();
   const uint32_t *d_np = d_nt + n_q;
   const uint32_t *d_rmap = d_np + n_q;
 
@@ -465,8 +461,9 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   HIP_TRY(ws.part_s.ensure((size_t)n_split * n_rows_max * P * 4));
   HIP_TRY(ws.top_ids.ensure(Bc * t_max * P * 4));
   HIP_TRY(ws.top_s.ensure(Bc * t_max * P * 4));
-  HIP_TRY(ws.cand_cnt.ensure(Bc * 4));
-  HIP_TRY(ws.cand_keys.ensure(max_pool * 8));
+  HIP_TRY(ws.cand_cnt.ensure(Bc * d.n_seg * 4));
+  HIP_TRY(ws.cand_tot.ensure(Bc * 4));
+  HIP_TRY(ws.cand_keys.ensure(Bc * pool_per_q));
   HIP_TRY(ws.sel_ids.ensure(Bc * R * 4));
   HIP_TRY(ws.sel_F.ensure(Bc * R * 4));
   HIP_TRY(ws.sel_n.ensure(Bc * 4));
@@ -528,7 +525,6 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     else for (uint32_t i = 0; i < nb; ++i) n_rows += nt[b0 + i];
     const uint32_t *bnt = d_nt + b0;
     const uint32_t *bnp = d_np + b0;
-    const uint64_t *bcoff = d_coff + b0;
 
     HIP_TRY(rec(bi, 0));
     if (prm->precision == MVR_FP32)
@@ -550,12 +546,12 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       bnp = ws.dyn_np.as<uint32_t>();
       q_r = ws.dyn_r.as<uint32_t>();
     }
-    HIP_TRY(launch_fill_u32(ws.cand_cnt.as<uint32_t>(), 0u, nb, s));
     HIP_TRY(launch_filter(ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), t_max, P, bnt, bnp, nb,
-                          d, bcoff, ws.cand_cnt.as<uint32_t>(), ws.cand_keys.as<uint64_t>(), s));
+                          d, ws.cand_cnt.as<uint32_t>(), ws.cand_keys.as<uint64_t>(), s));
     HIP_TRY(rec(bi, 2));
-    HIP_TRY(launch_select(ws.cand_keys.as<uint64_t>(), bcoff, ws.cand_cnt.as<uint32_t>(), q_r, nb, R,
-                          ws.sel_ids.as<uint32_t>(), ws.sel_F.as<float>(), ws.sel_n.as<uint32_t>(), s));
+    HIP_TRY(launch_select(ws.cand_keys.as<uint64_t>(), ws.cand_cnt.as<uint32_t>(), d.n_seg, q_r, nb, R,
+                          ws.sel_ids.as<uint32_t>(), ws.sel_F.as<float>(), ws.sel_n.as<uint32_t>(),
+                          ws.cand_tot.as<uint32_t>(), s));
     HIP_TRY(rec(bi, 3));
     HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R,
                           d, prm->precision, (int)prm->clip_zero, ws.sel_score.as<float>(), s));
@@ -587,7 +583,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       if (stages->top_s)
         HIP_TRY(hipMemcpy(stages->top_s + (size_t)b0 * t_max * P, ws.top_s.p, (size_t)nb * t_max * P * 4, hipMemcpyDeviceToHost));
       if (stages->cand_n)
-        HIP_TRY(hipMemcpy(stages->cand_n + b0, ws.cand_cnt.p, (size_t)nb * 4, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(stages->cand_n + b0, ws.cand_tot.p, (size_t)nb * 4, hipMemcpyDeviceToHost));
       if (stages->sel_ids)
         HIP_TRY(hipMemcpy(stages->sel_ids + (size_t)b0 * R, ws.sel_ids.p, (size_t)nb * R * 4, hipMemcpyDeviceToHost));
       if (stages->sel_F)
@@ -597,7 +593,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     }
     if (timings)
       HIP_TRY(launch_count_work(ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R, nb, d,
-                                ws.cand_cnt.as<uint32_t>(), ws.counters.as<unsigned long long>(), s));
+                                ws.cand_tot.as<uint32_t>(), ws.counters.as<unsigned long long>(), s));
   }
   if (timings) {
     HIP_TRY(hipEventSynchronize(ev.back()));

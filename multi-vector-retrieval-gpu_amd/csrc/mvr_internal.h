@@ -13,6 +13,7 @@ constexpr uint32_t kResBytes = 32;         // packed 2-bit residual bytes per to
 constexpr uint32_t kMaxQTok = 32;          // query tokens (bit mask width in the filter)
 constexpr uint32_t kMaxProbe = 32;         // largest nprobe
 constexpr uint32_t kSeg = 4096;            // docs per filter segment (LDS state)
+constexpr uint64_t kPoolBudget = 16ull << 30;  // candidate pool bytes per device pass
 constexpr uint32_t kMaxSelect = 4096;      // largest probe_topk
 constexpr double kFxScale = 4294967296.0;  // filter fixed point: 2^32
 
@@ -59,13 +60,14 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 
 hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t_max, uint32_t P,
                          const uint32_t *q_ntok, const uint32_t *q_np, uint32_t n_q,
-                         const DevIndex &ix, const uint64_t *cand_off, uint32_t *cand_cnt,
-                         uint64_t *cand_keys, hipStream_t s);
+                         const DevIndex &ix, uint32_t *seg_cnt, uint64_t *cand_keys, hipStream_t s);
 
 // top-R per query (q_r, nullable: per-query R <= R; R is the output row stride)
-hipError_t launch_select(const uint64_t *cand_keys, const uint64_t *cand_off,
-                         const uint32_t *cand_cnt, const uint32_t *q_r, uint32_t n_q, uint32_t R,
-                         uint32_t *sel_ids, float *sel_F, uint32_t *sel_n, hipStream_t s);
+// over the per-(query, segment) candidate regions; cand_tot[q] receives the
+// query's candidate count
+hipError_t launch_select(const uint64_t *cand_keys, const uint32_t *seg_cnt, uint32_t n_seg,
+                         const uint32_t *q_r, uint32_t n_q, uint32_t R, uint32_t *sel_ids,
+                         float *sel_F, uint32_t *sel_n, uint32_t *cand_tot, hipStream_t s);
 
 // learned probe: features [n_q][1024] from the top-32 scores, and decisions
 hipError_t launch_probe_features(const float *top_s, uint32_t t_max, uint32_t P,

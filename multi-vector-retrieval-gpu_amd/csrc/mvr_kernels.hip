@@ -227,117 +227,154 @@ __global__ __launch_bounds__(256) void k_topk_merge(const uint32_t *__restrict__
 // ---------------------------------------------------------------------------
 // Filter.  Workgroup (segment, query) owns docs [seg*4096, seg*4096+4096) in
 // LDS: a 32-bit mask of query tokens already counted and an int64 fixed-point
-// accumulator.  Probe ranks are processed in order j = 0..P-1 with a barrier
-// in between; because each token's probe list is sorted by score, the first
-// rank whose IVF list holds the doc carries that token's maximum score, so the
+// accumulator.  Probe ranks are applied in order j = 0..P-1 with a barrier in
+// between; because each token's probe list is sorted by score, the first rank
+// whose IVF list holds the doc carries that token's maximum score, so the
 // atomicOr on the mask decides which hit adds (max over probes, then an exact
 // sum over tokens).  Each list's sub-range inside the segment comes from the
-// precomputed seg_off table.  Candidates are appended to the query's region of
-// the candidate pool as 64-bit selection keys.
+// precomputed seg_off table.
+//
+// Work split: up to 8 ranks x 32 token slots are resolved at once (one
+// thread per (rank, token): centroid id, sub-range, fixed-point score).  In a
+// rank, 8 threads share a token's run, thread `sub` taking entries sub,
+// sub + 8, ...; the first two entries of every run of the chunk are loaded
+// into registers before any rank is applied, so a workgroup waits on global
+// memory about three times per chunk, and no thread searches for its run.
+//
+// XCD-aware order: consecutive workgroup ids go to the 8 XCDs in turn, so the
+// 1-D id is remapped to give each XCD a contiguous run of (query, segment)
+// tiles; neighbouring segments of one query then share the L2 lines of the
+// seg_off rows and of the query's probe lists.
+//
+// Candidates go to the workgroup's own 4,096-slot region of the candidate
+// pool (query-major, then segment) as 64-bit selection keys, with the count
+// in seg_cnt[q][seg]: no atomics on shared counters, and each wave writes its
+// candidates as contiguous runs.
 // ---------------------------------------------------------------------------
+constexpr uint32_t kFiltRounds = 8;   // probe ranks resolved per chunk (8 x 32 token slots)
+constexpr uint32_t kFiltHold = 2;     // list entries per thread and rank loaded up front
+
 __global__ __launch_bounds__(256) void k_filter_seg(
     const uint32_t *__restrict__ top_ids, const float *__restrict__ top_s, uint32_t t_max,
     uint32_t P, const uint32_t *__restrict__ q_ntok, const uint32_t *__restrict__ q_np,
     const uint64_t *__restrict__ ivf_off, const uint32_t *__restrict__ ivf_ids,
-    const uint32_t *__restrict__ seg_off, uint32_t n_seg, uint32_t n_doc,
-    const uint64_t *__restrict__ cand_off, uint32_t *__restrict__ cand_cnt,
-    uint64_t *__restrict__ cand_keys) {
+    const uint32_t *__restrict__ seg_off, uint32_t n_seg, uint32_t n_doc, uint32_t n_q,
+    uint32_t *__restrict__ seg_cnt, uint64_t *__restrict__ cand_keys) {
   __shared__ uint32_t mask[kSeg];
   __shared__ unsigned long long acc[kSeg];
-  __shared__ uint64_t lbase[kMaxQTok];
-  __shared__ uint32_t lpre[kMaxQTok + 1];
-  __shared__ long long lfx[kMaxQTok];
+  __shared__ uint64_t pstart[kFiltRounds * kMaxQTok];
+  __shared__ uint32_t plen[kFiltRounds * kMaxQTok];
+  __shared__ long long pfx[kFiltRounds * kMaxQTok];
 
-  const uint32_t seg = blockIdx.x, q = blockIdx.y;
+  const uint32_t n_wg = n_seg * n_q;
+  const uint32_t per_xcd = (n_wg + 7) / 8;
+  const uint32_t L = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+  if (L >= n_wg) return;  // uniform for the whole workgroup
+  const uint32_t q = L / n_seg, seg = L - q * n_seg;
   const uint32_t T = q_ntok ? q_ntok[q] : t_max;
   const uint32_t Pq = q_np ? q_np[q] : P;
   const uint32_t seg_lo = seg * kSeg;
-  const uint32_t tid = threadIdx.x;
+  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
+  // in a rank, thread (pr, sub) takes entries sub, sub + 8, ... of token pr's list
+  const uint32_t pr = tid >> 3, sub = tid & 7u;
 
   for (uint32_t i = tid; i < kSeg; i += 256) { mask[i] = 0u; acc[i] = 0ull; }
 
-  for (uint32_t j = 0; j < Pq; ++j) {
-    __syncthreads();
-    if (tid < kMaxQTok) {
+  for (uint32_t j0 = 0; j0 < Pq; j0 += kFiltRounds) {
+    // resolve slot tid = (rank j0 + tid / 32, token tid % 32)
+    {
+      const uint32_t j = j0 + (tid >> 5), t = tid & 31u;
       uint32_t len = 0;
-      uint64_t base = 0;
+      uint64_t start = 0;
       long long fx = 0;
-      if (tid < T) {
-        const size_t o = ((size_t)q * t_max + tid) * P + j;
+      if (j < Pq && t < T) {
+        const size_t o = ((size_t)q * t_max + t) * P + j;
         const uint32_t c = top_ids[o];
         if (c != 0xFFFFFFFFu) {
           const uint32_t *so = seg_off + (size_t)c * (n_seg + 1);
           const uint32_t lo = so[seg], hi = so[seg + 1];
           len = hi - lo;
-          base = ivf_off[c] + lo;
+          start = ivf_off[c] + lo;
           fx = (long long)((double)top_s[o] * kFxScale);
         }
       }
-      lbase[tid] = base;
-      lfx[tid] = fx;
-      // inclusive scan of len over the first wave-half (32 lanes)
-      uint32_t v = len;
-#pragma unroll
-      for (uint32_t off = 1; off < kMaxQTok; off <<= 1) {
-        const uint32_t u = __shfl_up(v, off, kMaxQTok);
-        if (tid >= off) v += u;
-      }
-      lpre[tid + 1] = v;
-      if (tid == 0) lpre[0] = 0;
+      __syncthreads();  // the previous chunk's readers are done
+      pstart[tid] = start;
+      plen[tid] = len;
+      pfx[tid] = fx;
+      __syncthreads();
     }
-    __syncthreads();
-    const uint32_t total = lpre[kMaxQTok];
-    for (uint32_t i = tid; i < total; i += 256) {
-      // token t whose range holds i: largest t with lpre[t] <= i
-      uint32_t lo = 0, hi = kMaxQTok;
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (lpre[mid] <= i) lo = mid; else hi = mid;
+    const uint32_t nr = min(kFiltRounds, Pq - j0);
+    // load the first kFiltHold entries of every (rank, token) run of this chunk
+    uint32_t ent[kFiltRounds][kFiltHold];
+#pragma unroll
+    for (uint32_t r = 0; r < kFiltRounds; ++r) {
+      const uint32_t slot = r * kMaxQTok + pr;
+      const uint32_t len = (r < nr) ? plen[slot] : 0u;
+      const uint64_t st = pstart[slot];
+#pragma unroll
+      for (uint32_t h = 0; h < kFiltHold; ++h) {
+        const uint32_t i = sub + 8 * h;
+        ent[r][h] = (i < len) ? ivf_ids[st + i] : 0xFFFFFFFFu;
       }
-      const uint32_t t = lo;
-      const uint32_t doc = ivf_ids[lbase[t] + (i - lpre[t])];
-      const uint32_t dl = doc - seg_lo;
-      const uint32_t bit = 1u << t;
-      const uint32_t old = atomicOr(&mask[dl], bit);
-      if (!(old & bit)) atomicAdd(&acc[dl], (unsigned long long)lfx[t]);
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kFiltRounds; ++r) {
+      if (r < nr) {
+        const uint32_t slot = r * kMaxQTok + pr;
+        const uint32_t bit = 1u << pr;
+        const unsigned long long add = (unsigned long long)pfx[slot];
+#pragma unroll
+        for (uint32_t h = 0; h < kFiltHold; ++h) {
+          if (ent[r][h] != 0xFFFFFFFFu) {
+            const uint32_t dl = (ent[r][h] - seg_lo) & (kSeg - 1);
+            const uint32_t old = atomicOr(&mask[dl], bit);
+            if (!(old & bit)) atomicAdd(&acc[dl], add);
+          }
+        }
+        // long runs: the rest of the entries, loaded here
+        const uint32_t len = plen[slot];
+        const uint64_t st = pstart[slot];
+        for (uint32_t i = sub + 8 * kFiltHold; i < len; i += 8) {
+          const uint32_t dl = (ivf_ids[st + i] - seg_lo) & (kSeg - 1);
+          const uint32_t old = atomicOr(&mask[dl], bit);
+          if (!(old & bit)) atomicAdd(&acc[dl], add);
+        }
+        __syncthreads();
+      }
     }
   }
   __syncthreads();
 
-  // Emission: count this thread's candidates (slots tid + 256k), scan the
-  // counts across the workgroup, and reserve the whole block's range with a
-  // single atomic on the query's counter.
-  const uint32_t lane = tid & (kWave - 1), wv = tid >> 6;
+  // Emission: slot k of every lane in turn; a wave's hits for one k are
+  // written as one contiguous run (ballot + popcount), waves in order.
   uint32_t hasbits = 0, myc = 0;
 #pragma unroll
   for (uint32_t k = 0; k < kSeg / 256; ++k) {
-    const uint32_t dl = tid + 256 * k;
+    const uint32_t dl = lane + kWave * (wv + 4 * k);
     if (seg_lo + dl < n_doc && mask[dl] != 0u) { hasbits |= 1u << k; ++myc; }
   }
-  uint32_t incl = myc;
+  uint32_t wcount = myc;
 #pragma unroll
-  for (uint32_t off = 1; off < kWave; off <<= 1) {
-    const uint32_t u = __shfl_up(incl, off, kWave);
-    if (lane >= off) incl += u;
-  }
-  __shared__ uint32_t wsum[4], blk_base;
-  if (lane == kWave - 1) wsum[wv] = incl;
+  for (uint32_t off = 1; off < kWave; off <<= 1) wcount += __shfl_xor(wcount, off, kWave);
+  __shared__ uint32_t wsum[4];
+  if (lane == 0) wsum[wv] = wcount;
   __syncthreads();
-  if (tid == 0) {
-    const uint32_t tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-    blk_base = tot ? atomicAdd(&cand_cnt[q], tot) : 0u;
-  }
-  __syncthreads();
-  uint32_t pos = blk_base + incl - myc;
+  uint32_t pos = 0;
   for (uint32_t w = 0; w < wv; ++w) pos += wsum[w];
-  uint64_t *out = cand_keys + cand_off[q];
+  if (tid == 0) seg_cnt[(size_t)q * n_seg + seg] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+  uint64_t *out = cand_keys + ((size_t)q * n_seg + seg) * kSeg;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
 #pragma unroll
   for (uint32_t k = 0; k < kSeg / 256; ++k) {
-    if (hasbits & (1u << k)) {
-      const uint32_t dl = tid + 256 * k;
+    const bool hit = (hasbits >> k) & 1u;
+    const uint64_t b = __ballot(hit);
+    if (hit) {
+      const uint32_t dl = lane + kWave * (wv + 4 * k);
       const float F = (float)((double)(long long)acc[dl] * (1.0 / kFxScale));
-      out[pos++] = make_key(F, seg_lo + dl);
+      out[pos + __popcll(b & lt_mask)] = make_key(F, seg_lo + dl);
     }
+    pos += __popcll(b);
   }
 }
 
@@ -375,17 +412,37 @@ constexpr uint32_t kSelThreads = 512;
 constexpr uint32_t kSelScratch = 2048;
 
 __global__ __launch_bounds__(kSelThreads) void k_select_topr(
-    const uint64_t *__restrict__ cand_keys, const uint64_t *__restrict__ cand_off,
-    const uint32_t *__restrict__ cand_cnt, const uint32_t *__restrict__ q_r, uint32_t R_out,
-    uint32_t *__restrict__ sel_ids, float *__restrict__ sel_F, uint32_t *__restrict__ sel_n) {
+    const uint64_t *__restrict__ cand_keys, const uint32_t *__restrict__ seg_cnt, uint32_t n_seg,
+    const uint32_t *__restrict__ q_r, uint32_t R_out, uint32_t *__restrict__ sel_ids,
+    float *__restrict__ sel_F, uint32_t *__restrict__ sel_n, uint32_t *__restrict__ cand_tot) {
   __shared__ unsigned long long outk[kMaxSelect];
   __shared__ unsigned long long scr[kSelScratch];
   __shared__ uint32_t hist[2048];
   __shared__ uint32_t s_out_n, s_scr_n, s_digit, s_gt, s_eq;
+  __shared__ uint32_t s_n;
 
   const uint32_t q = blockIdx.x, tid = threadIdx.x;
-  const uint32_t n = cand_cnt[q];
-  const uint64_t *keys = cand_keys + cand_off[q];
+  const uint32_t *cnt = seg_cnt + (size_t)q * n_seg;
+  const uint64_t *keys = cand_keys + (size_t)q * n_seg * kSeg;
+  // the query's candidates: cnt[sg] keys at keys + sg * kSeg for every segment
+  if (tid == 0) s_n = 0;
+  __syncthreads();
+  {
+    uint32_t part = 0;
+    for (uint32_t sg = tid; sg < n_seg; sg += kSelThreads) part += cnt[sg];
+    if (part) atomicAdd(&s_n, part);
+  }
+  __syncthreads();
+  const uint32_t n = s_n;
+  if (tid == 0 && cand_tot) cand_tot[q] = n;
+  // visit every candidate key (segment by segment)
+  auto for_each_key = [&](auto &&body) {
+    for (uint32_t sg = 0; sg < n_seg; ++sg) {
+      const uint32_t c = cnt[sg];
+      const uint64_t *kk = keys + (size_t)sg * kSeg;
+      for (uint32_t i = tid; i < c; i += kSelThreads) body(kk[i]);
+    }
+  };
   // R_out is the row stride of the outputs; q_r (nullable) lowers R per query
   const uint32_t R = q_r ? max(1u, min(R_out, q_r[q])) : R_out;
   const uint32_t n_sel = min(n, R);
@@ -395,7 +452,14 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
 
   if (n <= kMaxSelect) {
     const uint32_t np = pow2_ceil(max(n, 2u));
-    for (uint32_t i = tid; i < np; i += kSelThreads) outk[i] = (i < n) ? keys[i] : 0ull;
+    uint32_t base = 0;
+    for (uint32_t sg = 0; sg < n_seg; ++sg) {
+      const uint32_t c = cnt[sg];
+      const uint64_t *kk = keys + (size_t)sg * kSeg;
+      for (uint32_t i = tid; i < c; i += kSelThreads) outk[base + i] = kk[i];
+      base += c;
+    }
+    for (uint32_t i = n + tid; i < np; i += kSelThreads) outk[i] = 0ull;
     __syncthreads();
     bitonic_desc(outk, np);
   } else {
@@ -411,11 +475,10 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
       const uint32_t nb = 1u << bits;
       for (uint32_t b = tid; b < nb; b += kSelThreads) hist[b] = 0;
       __syncthreads();
-      for (uint32_t i = tid; i < n; i += kSelThreads) {
-        const uint64_t k = keys[i];
+      for_each_key([&](uint64_t k) {
         if (hi_shift == 64 || (k >> hi_shift) == prefix)
           atomicAdd(&hist[(uint32_t)(k >> shift) & (nb - 1)], 1u);
-      }
+      });
       __syncthreads();
       if (tid == 0) {
         uint32_t cum = 0, b = nb;
@@ -430,27 +493,25 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
       const uint32_t dsel = s_digit, gt = s_gt, eq = s_eq;
       const bool take_eq = (gt + eq == need);
       const bool eq_fits = eq <= kSelScratch;
-      for (uint32_t i = tid; i < n; i += kSelThreads) {
-        const uint64_t k = keys[i];
+      for_each_key([&](uint64_t k) {
         if (hi_shift == 64 || (k >> hi_shift) == prefix) {
           const uint32_t dg = (uint32_t)(k >> shift) & (nb - 1);
           if (dg > dsel || (dg == dsel && take_eq)) outk[atomicAdd(&s_out_n, 1u)] = k;
           else if (dg == dsel && eq_fits) scr[atomicAdd(&s_scr_n, 1u)] = k;
         }
-      }
+      });
       __syncthreads();
       need -= gt;
       if (take_eq) {
         done = true;
       } else if (shift == 0 && !eq_fits) {
         // only reachable with duplicate keys: take the first `need` of them
-        for (uint32_t i = tid; i < n; i += kSelThreads) {
-          const uint64_t k = keys[i];
+        for_each_key([&](uint64_t k) {
           if ((k >> hi_shift) == prefix && (uint32_t)(k & (nb - 1)) == dsel) {
             const uint32_t slot = atomicAdd(&s_scr_n, 1u);
             if (slot < need) outk[s_out_n + slot] = k;
           }
-        }
+        });
         __syncthreads();
         done = true;
       } else if (eq_fits) {
@@ -1128,22 +1189,23 @@ hipError_t launch_centroid_topk_bf16(const float *Q, uint32_t n_q, uint32_t t_ma
 
 hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t_max, uint32_t P,
                          const uint32_t *q_ntok, const uint32_t *q_np, uint32_t n_q,
-                         const DevIndex &ix, const uint64_t *cand_off, uint32_t *cand_cnt,
-                         uint64_t *cand_keys, hipStream_t s) {
+                         const DevIndex &ix, uint32_t *seg_cnt, uint64_t *cand_keys, hipStream_t s) {
   if (n_q == 0) return hipSuccess;
-  dim3 grid(ix.n_seg, n_q);
+  const uint64_t n_wg = (uint64_t)ix.n_seg * n_q;
+  if (n_wg > 0x7FFFFFF8ull) return hipErrorInvalidValue;
+  const uint32_t grid = (uint32_t)((n_wg + 7) / 8 * 8);  // the kernel's XCD remap needs a multiple of 8
   k_filter_seg<<<grid, 256, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np, ix.ivf_off,
-                                    ix.ivf_ids, ix.seg_off, ix.n_seg, ix.n_doc, cand_off, cand_cnt,
+                                    ix.ivf_ids, ix.seg_off, ix.n_seg, ix.n_doc, n_q, seg_cnt,
                                     cand_keys);
   return hipGetLastError();
 }
 
-hipError_t launch_select(const uint64_t *cand_keys, const uint64_t *cand_off,
-                         const uint32_t *cand_cnt, const uint32_t *q_r, uint32_t n_q, uint32_t R,
-                         uint32_t *sel_ids, float *sel_F, uint32_t *sel_n, hipStream_t s) {
+hipError_t launch_select(const uint64_t *cand_keys, const uint32_t *seg_cnt, uint32_t n_seg,
+                         const uint32_t *q_r, uint32_t n_q, uint32_t R, uint32_t *sel_ids,
+                         float *sel_F, uint32_t *sel_n, uint32_t *cand_tot, hipStream_t s) {
   if (n_q == 0) return hipSuccess;
-  k_select_topr<<<n_q, kSelThreads, 0, s>>>(cand_keys, cand_off, cand_cnt, q_r, R, sel_ids, sel_F,
-                                            sel_n);
+  k_select_topr<<<n_q, kSelThreads, 0, s>>>(cand_keys, seg_cnt, n_seg, q_r, R, sel_ids, sel_F,
+                                            sel_n, cand_tot);
   return hipGetLastError();
 }
 
@@ -1152,9 +1214,9 @@ hipError_t launch_rerank(const float *Q, uint32_t t_max, const uint32_t *q_ntok,
                          const DevIndex &ix, int precision, int clip_zero, float *sel_score,
                          hipStream_t s) {
   if (n_q == 0) return hipSuccess;
-  // about 8k workgroups when the lists are long (exhaustive scoring), else
-  // 4 per query (16 waves sharing one candidate list)
-  const uint32_t n_slice = R >= 256 ? std::max(4u, std::min(R / 256, std::max(1u, 8192u / n_q))) : 1;
+  // one workgroup per 64 candidates of a query, capped at about 8k
+  // workgroups in all (at least 4 per query)
+  const uint32_t n_slice = std::min((R + 63) / 64, std::max(4u, 8192u / n_q));
   dim3 grid(n_slice, n_q);
   if (precision == 0) {
     k_rerank_f32<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, sel_ids, sel_n, R, n_slice, ix.doclens,
