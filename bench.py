@@ -29,7 +29,7 @@ for p in (ROOT, PKG):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
-PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0, "fp8": 5000.0}
+PEAK_TFLOPS = {"fp32": 157.3, "fp32_mfma": 157.3, "bf16": 2500.0, "fp8": 5000.0}
 
 CONFIGS = {
     "c2": dict(n_doc=1 << 20, n_comp=65536, n_c=65536, lognormal=True, n_q=4096, P=8, R=1000, k=100,
@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp8"])
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32_mfma", "bf16", "fp8"])
     ap.add_argument("--rescore", type=int, default=200)
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--recall-queries", type=int, default=256)
@@ -88,7 +88,7 @@ def main():
     torch.cuda.synchronize()
     log(f"[bench] setup {time.time() - t_setup:.1f}s; max IVF list {ix.max_list}")
     stream = torch.cuda.current_stream().cuda_stream
-    rescore = args.rescore if args.precision != "fp32" else 0
+    rescore = args.rescore if args.precision in ("bf16", "fp8") else 0
     batch = 1024
 
     def step(timings=False):
@@ -151,7 +151,8 @@ def main():
     bytes_alg = 36.0 * tok_per_launch
     peak = PEAK_TFLOPS[args.precision]
     achieved_tf = flops / (avg_launch_ms * 1e-3) / 1e12
-    kname = "k_rerank_f32" if args.precision == "fp32" else f"k_rerank_{args.precision}"
+    kname = {"fp32": "k_rerank_f32", "fp32_mfma": "k_rerank_f32mfma"}.get(args.precision,
+                                                                       f"k_rerank_{args.precision}")
     traffic = measured_traffic(kname, args.config, args.precision)
 
     recall = recall_report(torch, args, cfg, corpus, ix, Q, out_ids, k, doclens)
@@ -170,7 +171,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.precision,
+        "dtype": "fp32" if args.precision.startswith("fp32") else args.precision,
         "data": "synthetic (seeded mixture corpus and queries, index built on the device)",
         "config": {"workload": f"{args.config}: {cfg['n_doc']} docs / {int(doclens.sum())} tokens, "
                                f"{cfg['n_c']} centroids, 2-bit residuals; {n_q} queries x 32 tokens; "

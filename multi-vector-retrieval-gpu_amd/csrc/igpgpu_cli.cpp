@@ -12,7 +12,7 @@
 //
 // Flag grammar follows deps/cmd_line_parser/include/parser.hpp:20-60: each
 // flag once, "-name value", unknown flags are errors, every required flag must
-// appear.  Extra optional flags: -precision fp32|bf16|fp8, -rescore N,
+// appear.  Extra optional flags: -precision fp32|fp32_mfma|bf16|fp8, -rescore N,
 // -batch N.  The dataset root /home/{username}/Dataset/multi-vector-retrieval-GPU
 // can be replaced with the MVR_DATASET_ROOT environment variable.
 #include "mvr.h"
@@ -143,7 +143,7 @@ int main(int argc, char **argv) {
   p.add("-probe_topk", "topk of the probe", true);
   if (!kBasic) p.add("-n_thread", "number of cpu thread", true);
   p.add("-is_profile", "is profile the result", true);
-  p.add("-precision", "fp32 (default), bf16 or fp8 centroid scoring and MaxSim", false);
+  p.add("-precision", "fp32 (default), fp32_mfma, bf16 or fp8 centroid scoring and MaxSim", false);
   p.add("-rescore", "bf16/fp8: rescore this many docs in fp32 (default 200)", false);
   p.add("-batch", "queries per device pass (default 1024)", false);
   if (!p.parse(argc, argv)) {
@@ -160,6 +160,7 @@ int main(int argc, char **argv) {
   uint32_t precision = MVR_FP32;
   if (prec == "bf16") precision = MVR_BF16;
   else if (prec == "fp8") precision = MVR_FP8;
+  else if (prec == "fp32_mfma") precision = MVR_FP32_MFMA;
   else if (prec != "fp32") { std::cerr << "Error parsing command line: bad -precision" << std::endl; return 1; }
   const uint32_t rescore = p.get("-rescore").empty() ? 200 : p.u32("-rescore");
   const uint32_t batch = p.get("-batch").empty() ? 1024 : std::max(1u, p.u32("-batch"));
@@ -209,7 +210,7 @@ int main(int argc, char **argv) {
   prm.probe_topk = probe_topk;
   prm.topk = topk;
   prm.precision = precision;
-  prm.rescore = precision == MVR_FP32 ? 0 : std::min(rescore, probe_topk);
+  prm.rescore = (precision == MVR_FP32 || precision == MVR_FP32_MFMA) ? 0 : std::min(rescore, probe_topk);
   prm.clip_zero = kBasic ? 1 : 0;
   prm.batch = batch;
 

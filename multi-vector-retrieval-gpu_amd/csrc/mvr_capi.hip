@@ -380,11 +380,12 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   if (R < k) return fail(MVR_ERR_INVALID, "probe_topk %u is smaller than topk %u", R, k);
   if (k == 0) return fail(MVR_ERR_INVALID, "topk must be positive");
   if (R > kMaxSelect) return fail(MVR_ERR_UNSUPPORTED, "probe_topk %u > %u", R, kMaxSelect);
-  if (prm->precision != MVR_FP32 && prm->precision != MVR_BF16)
+  if (prm->precision != MVR_FP32 && prm->precision != MVR_BF16 && prm->precision != MVR_FP32_MFMA)
     return fail(MVR_ERR_UNSUPPORTED, "precision %u not built", prm->precision);
+  const bool fp32_mode = prm->precision == MVR_FP32 || prm->precision == MVR_FP32_MFMA;
   // BF16/FP8: the best `rescore` docs by the low-precision MaxSim are scored
-  // again in FP32 and the final top-k comes from those FP32 scores
-  const uint32_t R2 = (prm->precision != MVR_FP32 && prm->rescore) ? std::min(prm->rescore, R) : 0;
+  // again in FP32 (on MFMA) and the final top-k comes from those FP32 scores
+  const uint32_t R2 = (!fp32_mode && prm->rescore) ? std::min(prm->rescore, R) : 0;
   if (R2 && R2 < k) return fail(MVR_ERR_INVALID, "rescore %u is smaller than topk %u", R2, k);
   if (stages && on_device) return fail(MVR_ERR_INVALID, "stage outputs need host buffers");
   uint32_t P = prm->nprobe;
@@ -468,7 +469,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   HIP_TRY(ws.sel_F.ensure(Bc * R * 4));
   HIP_TRY(ws.sel_n.ensure(Bc * 4));
   HIP_TRY(ws.sel_score.ensure(Bc * R * 4));
-  if (prm->precision != MVR_FP32) HIP_TRY(ws.bmax.ensure(Bc * t_max * ((d.n_c + 31) / 32) * 4));
+  if (!fp32_mode) HIP_TRY(ws.bmax.ensure(Bc * t_max * ((d.n_c + 31) / 32) * 4));
   if (learned) {
     HIP_TRY(ws.feat.ensure(Bc * 1024 * 4));
     HIP_TRY(ws.mlp_out.ensure(Bc * 6 * 4));
@@ -527,8 +528,8 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     const uint32_t *bnp = d_np + b0;
 
     HIP_TRY(rec(bi, 0));
-    if (prm->precision == MVR_FP32)
-      HIP_TRY(launch_centroid_topk(Qb, rm, n_rows, nb * t_max, d, P, prm->precision, n_split,
+    if (fp32_mode)
+      HIP_TRY(launch_centroid_topk(Qb, rm, n_rows, nb * t_max, d, P, MVR_FP32, n_split,
                                    ws.part_ids.as<uint32_t>(), ws.part_s.as<float>(),
                                    ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), nullptr, s));
     else
@@ -562,7 +563,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
                           R, nb, R2, ws.res_ids.as<uint32_t>(), ws.res_s.as<float>(),
                           ws.res_n.as<uint32_t>(), s));
       HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2,
-                            d, MVR_FP32, (int)prm->clip_zero, ws.res_s.as<float>(), s));
+                            d, MVR_FP32_MFMA, (int)prm->clip_zero, ws.res_s.as<float>(), s));
     }
     HIP_TRY(rec(bi, 5));
     if (R2)
@@ -626,7 +627,7 @@ int mvr_rerank_docs(mvr_index *ix, const float *Q_dev, uint32_t n_q, uint32_t t_
   if (!ix || !Q_dev || !doc_ids_dev || !scores_dev) return fail(MVR_ERR_INVALID, "NULL argument");
   if (n_q == 0 || R == 0) return MVR_OK;
   if (t_max == 0 || t_max > kMaxQTok) return fail(MVR_ERR_INVALID, "t_max %u not in [1, 32]", t_max);
-  if (precision != MVR_FP32 && precision != MVR_BF16)
+  if (precision != MVR_FP32 && precision != MVR_BF16 && precision != MVR_FP32_MFMA)
     return fail(MVR_ERR_UNSUPPORTED, "precision %u not built", precision);
   DeviceGuard g(ix->device);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);

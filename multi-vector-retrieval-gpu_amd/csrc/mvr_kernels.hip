@@ -754,6 +754,100 @@ __global__ __launch_bounds__(256) void k_rerank_bf16(
 }
 
 // ---------------------------------------------------------------------------
+// Fused decompress + MaxSim on FP32 MFMA (v_mfma_f32_32x32x2_f32).  Same tile
+// scheme as the BF16 kernel: A = 32 doc tokens (rows), B = the 32 query tokens
+// (columns, 64 FP32 values per lane held for the whole workgroup).  Lane l =
+// (row/col c = l & 31, half h = l >> 5) decodes its token's dims
+// [64h, 64h + 64) in registers -- x = C[code] + w[d][bucket], the exact FP32
+// values of the decompressed embedding -- from one 16-byte residual load and
+// 16 float4 loads of the centroid row; instruction s, half h covers dim
+// 64h + s in both operands.  Dot products are summed in the MFMA's order, so
+// scores agree with the sequential-FMA kernel to about 1e-7 relative (not
+// bit-identical); the max over tokens is local plus one cross-half shuffle
+// and the sum over query tokens runs left to right.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_rerank_f32mfma(
+    const float *__restrict__ Q, uint32_t t_max, const uint32_t *__restrict__ q_ntok,
+    const uint32_t *__restrict__ sel_ids, const uint32_t *__restrict__ sel_n, uint32_t R,
+    uint32_t n_slice, const uint32_t *__restrict__ doclens, const uint64_t *__restrict__ docoff,
+    const uint32_t *__restrict__ codes, const uint8_t *__restrict__ residual,
+    const float *__restrict__ C, const float *__restrict__ W, int clip_zero,
+    float *__restrict__ sel_score) {
+  __shared__ float wl[kDim * 4];
+  __shared__ float Ms[4][kMaxQTok];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const uint32_t h = lane >> 5, col = lane & 31;
+  const uint32_t slice = blockIdx.x, q = blockIdx.y;
+  const uint32_t T = q_ntok ? q_ntok[q] : t_max;
+  const uint32_t n = sel_n[q];
+
+  for (uint32_t i = tid; i < kDim * 4; i += 256) wl[i] = W[i];
+  float qf[64];
+  {
+    const float4 *qr = reinterpret_cast<const float4 *>(Q + ((size_t)q * t_max + col) * kDim + 64 * h);
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) {
+      const float4 v = (col < T) ? qr[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+      qf[4 * s] = v.x; qf[4 * s + 1] = v.y; qf[4 * s + 2] = v.z; qf[4 * s + 3] = v.w;
+    }
+  }
+  __syncthreads();
+
+  for (uint32_t r = slice * 4 + wave; r < n; r += n_slice * 4) {
+    const uint32_t doc = sel_ids[(size_t)q * R + r];
+    const uint32_t L = doclens[doc];
+    const uint64_t off = docoff[doc];
+    float M = clip_zero ? 0.0f : -FLT_MAX;
+    for (uint32_t i0 = 0; i0 < L; i0 += 32) {
+      const uint32_t i = i0 + col;
+      float4 cv[16];
+      uint4 rb = make_uint4(0u, 0u, 0u, 0u);
+      if (i < L) {
+        const uint64_t g = off + i;
+        const uint32_t code = codes[g];
+        rb = *reinterpret_cast<const uint4 *>(residual + g * kResBytes + 16 * h);
+        const float4 *cp = reinterpret_cast<const float4 *>(C + (size_t)code * kDim + 64 * h);
+#pragma unroll
+        for (uint32_t s = 0; s < 16; ++s) cv[s] = cp[s];
+      } else {
+#pragma unroll
+        for (uint32_t s = 0; s < 16; ++s) cv[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+      const uint32_t rw[4] = {rb.x, rb.y, rb.z, rb.w};
+      floatx16_t acc = {};
+#pragma unroll
+      for (uint32_t s = 0; s < 16; ++s) {
+        // residual byte 16h + s holds dims 64h + 4s .. +3, most significant bits first
+        const uint32_t byte = (rw[s >> 2] >> (8 * (s & 3))) & 0xFFu;
+        const float *wd = wl + 4 * (64 * h + 4 * s);  // weights of dim 64h + 4s, 4 per dim
+        const float x0 = cv[s].x + wd[(byte >> 6) & 3u];
+        const float x1 = cv[s].y + wd[4 + ((byte >> 4) & 3u)];
+        const float x2 = cv[s].z + wd[8 + ((byte >> 2) & 3u)];
+        const float x3 = cv[s].w + wd[12 + (byte & 3u)];
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x0, qf[4 * s], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x1, qf[4 * s + 1], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x2, qf[4 * s + 2], acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x3, qf[4 * s + 3], acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < 16; ++j) {
+        const uint32_t m = 8 * (j >> 2) + 4 * h + (j & 3);
+        if (i0 + m < L) M = fmaxf(M, acc[j]);
+      }
+    }
+    M = fmaxf(M, __shfl_xor(M, 32));
+    if (h == 0) Ms[wave][col] = M;
+    wave_lds_sync();
+    if (lane == 0) {
+      float sc = 0.0f;
+      for (uint32_t tt = 0; tt < T; ++tt) sc += Ms[wave][tt];
+      sel_score[(size_t)q * R + r] = sc;
+    }
+    wave_lds_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Centroid scoring + top-P in BF16, pass 1: block maxima on MFMA.
 // Workgroup = 8 waves = 8 queries x one slice of centroid tiles (32 centroids
 // each).  A tile (32 rows x 128 bf16) is staged once in LDS and read by all 8
@@ -1222,6 +1316,10 @@ hipError_t launch_rerank(const float *Q, uint32_t t_max, const uint32_t *q_ntok,
     k_rerank_f32<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, sel_ids, sel_n, R, n_slice, ix.doclens,
                                       ix.docoff, ix.codes, ix.residual, ix.C, ix.W, clip_zero,
                                       sel_score);
+  } else if (precision == 3) {
+    k_rerank_f32mfma<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, sel_ids, sel_n, R, n_slice, ix.doclens,
+                                          ix.docoff, ix.codes, ix.residual, ix.C, ix.W, clip_zero,
+                                          sel_score);
   } else if (precision == 1) {
     if (!ix.C_bf16 || !ix.lut_bf16) return hipErrorInvalidValue;
     k_rerank_bf16<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, sel_ids, sel_n, R, n_slice, ix.doclens,

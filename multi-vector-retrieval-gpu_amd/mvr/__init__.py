@@ -18,12 +18,12 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _capi
-from ._capi import MVR_BF16, MVR_FP8, MVR_FP32, MvrError, check, lib
+from ._capi import MVR_BF16, MVR_FP8, MVR_FP32, MVR_FP32_MFMA, MvrError, check, lib
 
-__all__ = ["Index", "IGPGPUPP", "IGPGPUPPResult", "MvrError", "MVR_FP32", "MVR_BF16", "MVR_FP8",
+__all__ = ["Index", "IGPGPUPP", "IGPGPUPPResult", "MvrError", "MVR_FP32", "MVR_BF16", "MVR_FP8", "MVR_FP32_MFMA",
            "build_index_module", "device_count"]
 
-_PRECISION = {"fp32": MVR_FP32, "bf16": MVR_BF16, "fp8": MVR_FP8}
+_PRECISION = {"fp32": MVR_FP32, "bf16": MVR_BF16, "fp8": MVR_FP8, "fp32_mfma": MVR_FP32_MFMA}
 
 
 def _ptr(a):

@@ -10,7 +10,7 @@ LIB_DIR = os.path.join(_PKG, "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libmvr.so")
 
 MVR_OK = 0
-MVR_FP32, MVR_BF16, MVR_FP8 = 0, 1, 2
+MVR_FP32, MVR_BF16, MVR_FP8, MVR_FP32_MFMA = 0, 1, 2, 3
 
 
 class MvrError(RuntimeError):
