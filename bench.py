@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "fp32_mfma", "bf16", "fp8"])
+    ap.add_argument("--precision", default="fp32_mfma", choices=["fp32", "fp32_mfma", "bf16", "fp8"])
     ap.add_argument("--rescore", type=int, default=200)
     ap.add_argument("--seed", type=int, default=20261203)
     ap.add_argument("--recall-queries", type=int, default=256)

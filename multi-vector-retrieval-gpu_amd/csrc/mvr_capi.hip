@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -244,6 +245,15 @@ int mvr_index_create(const mvr_index_desc *desc, int device, mvr_index **out) {
   }
 
   UP(d.C, desc->centroids, (size_t)d.n_c * kDim);
+  {
+    // largest centroid norm (host copy: the descriptor may hold device pointers)
+    const std::vector<float> Ch = fetch(desc->centroids, (size_t)d.n_c * kDim, od);
+    for (uint32_t c = 0; c < d.n_c; ++c) {
+      double nn = 0.0;
+      for (uint32_t k = 0; k < kDim; ++k) nn += (double)Ch[(size_t)c * kDim + k] * Ch[(size_t)c * kDim + k];
+      d.c_norm_max = std::max(d.c_norm_max, (float)std::sqrt(nn));
+    }
+  }
   UP(d.codes, desc->codes, d.n_tok);
   UP(d.residual, desc->residual, d.n_tok * kResBytes);
   UP(d.doclens, desc->doclens, d.n_doc);
@@ -383,6 +393,13 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   if (prm->precision != MVR_FP32 && prm->precision != MVR_BF16 && prm->precision != MVR_FP32_MFMA)
     return fail(MVR_ERR_UNSUPPORTED, "precision %u not built", prm->precision);
   const bool fp32_mode = prm->precision == MVR_FP32 || prm->precision == MVR_FP32_MFMA;
+  // FP32 centroid top-P: BF16 MFMA tile maxima + exact FP32 rescan (same
+  // result as the all-FP32 scan); MVR_CT_SCAN=1 forces the all-FP32 scan
+  static const bool force_scan = [] {
+    const char *e = std::getenv("MVR_CT_SCAN");
+    return e && e[0] == '1';
+  }();
+  const bool fast_ct = !force_scan;
   // BF16/FP8: the best `rescore` docs by the low-precision MaxSim are scored
   // again in FP32 (on MFMA) and the final top-k comes from those FP32 scores
   const uint32_t R2 = (!fp32_mode && prm->rescore) ? std::min(prm->rescore, R) : 0;
@@ -469,7 +486,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   HIP_TRY(ws.sel_F.ensure(Bc * R * 4));
   HIP_TRY(ws.sel_n.ensure(Bc * 4));
   HIP_TRY(ws.sel_score.ensure(Bc * R * 4));
-  if (!fp32_mode) HIP_TRY(ws.bmax.ensure(Bc * t_max * ((d.n_c + 31) / 32) * 4));
+  if (!fp32_mode || fast_ct) HIP_TRY(ws.bmax.ensure(Bc * t_max * ((d.n_c + 31) / 32) * 4));
   if (learned) {
     HIP_TRY(ws.feat.ensure(Bc * 1024 * 4));
     HIP_TRY(ws.mlp_out.ensure(Bc * 6 * 4));
@@ -528,13 +545,13 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     const uint32_t *bnp = d_np + b0;
 
     HIP_TRY(rec(bi, 0));
-    if (fp32_mode)
+    if (fp32_mode && !fast_ct)
       HIP_TRY(launch_centroid_topk(Qb, rm, n_rows, nb * t_max, d, P, MVR_FP32, n_split,
                                    ws.part_ids.as<uint32_t>(), ws.part_s.as<float>(),
                                    ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), nullptr, s));
     else
       HIP_TRY(launch_centroid_topk_bf16(Qb, nb, t_max, rm, n_rows, d, P, ws.bmax.as<float>(),
-                                        ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), s));
+                                        ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), fp32_mode, s));
     HIP_TRY(rec(bi, 1));
     const uint32_t *q_r = nullptr;
     if (learned) {

@@ -21,6 +21,7 @@ constexpr double kFxScale = 4294967296.0;  // filter fixed point: 2^32
 struct DevIndex {
   uint32_t n_doc = 0, n_c = 0, n_seg = 0, max_list = 0, max_doclen = 0;
   uint64_t n_tok = 0, n_ele = 0;
+  float c_norm_max = 0.0f;     // max_c ||C_c||_2 (bounds the centroid-score error)
   float *C = nullptr;          // [n_c][128] f32
   uint16_t *C_bf16 = nullptr;  // [n_c][128] bf16 (RNE of C)
   uint8_t *C_fp8 = nullptr;    // [n_c][128] e4m3 of C / scale
@@ -51,10 +52,12 @@ hipError_t launch_centroid_topk(const float *Q, const uint32_t *row_map, uint32_
 
 // BF16 centroid scoring: MFMA tile maxima into bmax [n_q*t_max][ceil(n_c/32)],
 // then per row exact top-P over the best P+2 tiles (bf16 inputs, FP32 FMA chain).
+// exact_f32: the top-P is exact on FP32 inputs (BF16 MFMA tile maxima only
+// choose which tiles are rescanned); else exact on bf16-rounded inputs
 hipError_t launch_centroid_topk_bf16(const float *Q, uint32_t n_q, uint32_t t_max,
                                      const uint32_t *row_map, uint32_t n_rows, const DevIndex &ix,
                                      uint32_t P, float *bmax, uint32_t *top_ids, float *top_s,
-                                     hipStream_t s);
+                                     bool exact_f32, hipStream_t s);
 
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 

@@ -904,11 +904,19 @@ __global__ __launch_bounds__(512) void k_ct_bf16_blockmax(
   }
 }
 
-// Pass 2: one wave per query-token row.  Select the best P+2 tiles by their
-// maximum (wave-held sorted list, lane j = j-th best), then score every
-// centroid of those tiles with bf16-rounded inputs as one sequential FP32 FMA
-// chain over d (the precision mode's exact definition, reproducible on the
-// CPU) and keep the top-P by (score desc, id asc).  Output S' = score + 1.
+// Pass 2: one wave per query-token row.  The tile maxima of pass 1 are
+// approximate (bf16 inputs, FP32 accumulation in the matrix unit's order);
+// |approx - exact| <= eps for every centroid, with
+//   eps = margin * ||q|| * max_c ||C_c||
+// (Cauchy-Schwarz on the rounding-error bound of a 128-term dot product).
+// If A_P is the P-th largest approximate tile maximum, every centroid of the
+// exact top-P (ties broken by id) lies in a tile whose approximate maximum is
+// >= A_P - 2 eps: P tiles reach A_P, so P distinct centroids score >= A_P - eps
+// exactly, hence the exact P-th score tau >= A_P - eps, and a centroid scoring
+// >= tau has a tile maximum >= tau - eps.  Every centroid of those tiles is
+// scored exactly -- one sequential FP32 FMA chain over d, on FP32 inputs
+// (exact_f32) or bf16-rounded inputs (the BF16 mode's definition) -- and the
+// top-P by (score desc, id asc) is kept.  Output S' = score + 1.
 __device__ __forceinline__ void wave_list_insert(float &ls, uint32_t &li, uint32_t lane, uint32_t P,
                                                  float s, uint32_t id, bool valid) {
   float ts = readlane_f(ls, P - 1);
@@ -930,51 +938,80 @@ __device__ __forceinline__ void wave_list_insert(float &ls, uint32_t &li, uint32
   }
 }
 
-__global__ __launch_bounds__(256) void k_ct_bf16_select(
+template <bool kF32>
+__global__ __launch_bounds__(256) void k_ct_select(
     const float *__restrict__ Q, const uint32_t *__restrict__ row_map, uint32_t n_rows,
-    const uint16_t *__restrict__ Cb, uint32_t n_c, uint32_t n_tiles,
-    const float *__restrict__ bmax, uint32_t P, uint32_t *__restrict__ top_ids,
-    float *__restrict__ top_s) {
+    const float *__restrict__ C, const uint16_t *__restrict__ Cb, uint32_t n_c, uint32_t n_tiles,
+    const float *__restrict__ bmax, uint32_t P, float margin, float c_norm_max,
+    uint32_t *__restrict__ top_ids, float *__restrict__ top_s) {
   __shared__ float qs[4][kDim];
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t row = blockIdx.x * 4 + wave;
   if (row >= n_rows) return;  // whole wave leaves; no block barrier below
   const uint64_t qrow = row_map ? row_map[row] : row;
-  for (uint32_t d = lane; d < kDim; d += 64) qs[wave][d] = (float)(__bf16)Q[qrow * kDim + d];
+  float nn = 0.0f;
+  for (uint32_t d = lane; d < kDim; d += 64) {
+    const float v = kF32 ? Q[qrow * kDim + d] : (float)(__bf16)Q[qrow * kDim + d];
+    qs[wave][d] = v;
+    nn += v * v;
+  }
+#pragma unroll
+  for (uint32_t off = 32; off > 0; off >>= 1) nn += __shfl_xor(nn, off);
   wave_lds_sync();
-  // best P2 tiles by maximum
-  const uint32_t P2 = min(P + 2, min(n_tiles, 64u));
+  // A_P: the P-th largest approximate tile maximum
+  const uint32_t P1 = min(P, n_tiles);
   float ls = -INFINITY;
   uint32_t li = 0xFFFFFFFFu;
   const float *bm = bmax + qrow * n_tiles;
   for (uint32_t base = 0; base < n_tiles; base += 64) {
     const uint32_t tl = base + lane;
     const bool v = tl < n_tiles;
-    wave_list_insert(ls, li, lane, P2, v ? bm[tl] : -INFINITY, tl, v);
+    wave_list_insert(ls, li, lane, P1, v ? bm[tl] : -INFINITY, tl, v);
   }
-  // exact scores of the selected tiles' centroids, two tiles per round
+  // 1.01: slack for rounding in the bound itself
+  const float thr = readlane_f(ls, P1 - 1) - 2.02f * margin * sqrtf(nn) * c_norm_max;
+  // exact scores of every centroid in a tile reaching thr, two tiles per round
   float fs = -INFINITY;
   uint32_t fi = 0xFFFFFFFFu;
-  for (uint32_t j = 0; j < P2; j += 2) {
-    const uint32_t jj = j + (lane >> 5);
-    const uint32_t tile = (uint32_t)__shfl((int)li, (int)min(jj, P2 - 1));
-    const uint32_t c = tile * 32 + (lane & 31);
-    const bool v = jj < P2 && tile != 0xFFFFFFFFu && c < n_c;
-    float a = 0.0f;
-    if (v) {
-      const uint4 *cr = reinterpret_cast<const uint4 *>(Cb + (size_t)c * kDim);
+  for (uint32_t base = 0; base < n_tiles; base += 64) {
+    const uint32_t tl = base + lane;
+    uint64_t bal = __ballot(tl < n_tiles && bm[tl] >= thr);
+    while (bal) {
+      const uint32_t t0 = base + (uint32_t)__builtin_ctzll(bal);
+      bal &= bal - 1;
+      uint32_t t1 = 0xFFFFFFFFu;
+      if (bal) { t1 = base + (uint32_t)__builtin_ctzll(bal); bal &= bal - 1; }
+      const uint32_t tile = (lane < 32) ? t0 : t1;
+      const uint32_t c = tile * 32 + (lane & 31);
+      const bool v = tile != 0xFFFFFFFFu && c < n_c;
+      float a = 0.0f;
+      if (v) {
+        if (kF32) {
+          const float4 *cr = reinterpret_cast<const float4 *>(C + (size_t)c * kDim);
+#pragma unroll 8
+          for (uint32_t k = 0; k < kDim / 4; ++k) {
+            const float4 w = cr[k];
+            a = __builtin_fmaf(qs[wave][4 * k], w.x, a);
+            a = __builtin_fmaf(qs[wave][4 * k + 1], w.y, a);
+            a = __builtin_fmaf(qs[wave][4 * k + 2], w.z, a);
+            a = __builtin_fmaf(qs[wave][4 * k + 3], w.w, a);
+          }
+        } else {
+          const uint4 *cr = reinterpret_cast<const uint4 *>(Cb + (size_t)c * kDim);
 #pragma unroll 4
-      for (uint32_t k = 0; k < 16; ++k) {
-        const uint4 w = cr[k];
-        const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
+          for (uint32_t k = 0; k < 16; ++k) {
+            const uint4 w = cr[k];
+            const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-        for (uint32_t e = 0; e < 4; ++e) {
-          a = __builtin_fmaf(qs[wave][8 * k + 2 * e], __uint_as_float(ww[e] << 16), a);
-          a = __builtin_fmaf(qs[wave][8 * k + 2 * e + 1], __uint_as_float(ww[e] & 0xFFFF0000u), a);
+            for (uint32_t e = 0; e < 4; ++e) {
+              a = __builtin_fmaf(qs[wave][8 * k + 2 * e], __uint_as_float(ww[e] << 16), a);
+              a = __builtin_fmaf(qs[wave][8 * k + 2 * e + 1], __uint_as_float(ww[e] & 0xFFFF0000u), a);
+            }
+          }
         }
       }
+      wave_list_insert(fs, fi, lane, P, a + 1.0f, c, v);
     }
-    wave_list_insert(fs, fi, lane, P, a + 1.0f, c, v);
   }
   if (lane < P) {
     top_ids[qrow * P + lane] = fi;
@@ -1265,9 +1302,9 @@ hipError_t launch_centroid_topk(const float *Q, const uint32_t *row_map, uint32_
 hipError_t launch_centroid_topk_bf16(const float *Q, uint32_t n_q, uint32_t t_max,
                                      const uint32_t *row_map, uint32_t n_rows, const DevIndex &ix,
                                      uint32_t P, float *bmax, uint32_t *top_ids, float *top_s,
-                                     hipStream_t s) {
+                                     bool exact_f32, hipStream_t s) {
   if (n_q == 0 || n_rows == 0) return hipSuccess;
-  if (!ix.C_bf16 || P > 62) return hipErrorInvalidValue;
+  if (!ix.C_bf16 || P > 64) return hipErrorInvalidValue;
   const uint32_t n_tiles = (ix.n_c + 31) / 32;
   const uint32_t qblocks = (n_q + kBmWaves - 1) / kBmWaves;
   const uint32_t n_split = std::max(1u, std::min(n_tiles, (1024u + qblocks - 1) / qblocks));
@@ -1276,8 +1313,18 @@ hipError_t launch_centroid_topk_bf16(const float *Q, uint32_t n_q, uint32_t t_ma
   k_ct_bf16_blockmax<<<grid, 512, 0, s>>>(Q, t_max, n_q, ix.C_bf16, ix.n_c, n_tiles, tps, bmax);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_ct_bf16_select<<<(n_rows + 3) / 4, 256, 0, s>>>(Q, row_map, n_rows, ix.C_bf16, ix.n_c, n_tiles,
-                                                    bmax, P, top_ids, top_s);
+  // error bounds per unit ||q|| ||C||: FP32 inputs vs bf16 MFMA = input rounding
+  // (2u + u^2, u = 2^-8) plus two 128-term FP32 accumulations; bf16 inputs =
+  // accumulation order only (products of bf16 values are exact in FP32), with
+  // a wide allowance for the matrix unit's internal summation
+  if (exact_f32)
+    k_ct_select<true><<<(n_rows + 3) / 4, 256, 0, s>>>(Q, row_map, n_rows, ix.C, ix.C_bf16, ix.n_c,
+                                                       n_tiles, bmax, P, 0.008f, ix.c_norm_max,
+                                                       top_ids, top_s);
+  else
+    k_ct_select<false><<<(n_rows + 3) / 4, 256, 0, s>>>(Q, row_map, n_rows, ix.C, ix.C_bf16, ix.n_c,
+                                                        n_tiles, bmax, P, 1e-4f, ix.c_norm_max * 1.004f,
+                                                        top_ids, top_s);
   return hipGetLastError();
 }
 
