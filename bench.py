@@ -36,6 +36,10 @@ CONFIGS = {
                kmeans_sample=1 << 21, n_iter=4),
     "c1": dict(n_doc=8192, n_comp=1024, n_c=1024, lognormal=False, n_q=64, P=4, R=256, k=10,
                kmeans_sample=None, n_iter=10),
+    # uneven batches on the C2 corpus: 16,384 queries, U{8..32} tokens, nprobe
+    # uniform over {2, 4, 8, 16, 32}
+    "c4": dict(n_doc=1 << 20, n_comp=65536, n_c=65536, lognormal=True, n_q=16384, P=32, R=1000, k=100,
+               kmeans_sample=1 << 21, n_iter=4, ragged=True),
 }
 
 
@@ -55,6 +59,9 @@ def main():
     ap.add_argument("--recall-queries", type=int, default=256)
     ap.add_argument("--cpu-queries", type=int, default=640)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--persistent", action="store_true",
+                    help="MaxSim + top-k as one persistent work-queue kernel per pass")
+    ap.add_argument("--graph", action="store_true", help="replay each device pass from a hipGraph")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -83,6 +90,10 @@ def main():
     ix = corpus.index
     n_q, P, R, k = cfg["n_q"], cfg["P"], cfg["R"], cfg["k"]
     Q = synth.make_queries(torch, args.seed, corpus.centers, n_q)
+    q_nt = q_np = None
+    if cfg.get("ragged"):
+        q_nt, q_np = synth.query_shapes(args.seed, n_q)
+        log(f"[bench] ragged queries: tokens {q_nt.min()}..{q_nt.max()}, nprobe {sorted(set(q_np.tolist()))}")
     out_ids = torch.empty((n_q, k), dtype=torch.int32, device="cuda")
     out_s = torch.empty((n_q, k), dtype=torch.float32, device="cuda")
     torch.cuda.synchronize()
@@ -96,12 +107,16 @@ def main():
             import ctypes as C
             from mvr import _capi
             tm = _capi.Timings()
-            prm = mvr.Index._params(P, R, k, args.precision, rescore, False, batch)
-            _capi.check(_capi.lib().mvr_search(ix._h, Q.data_ptr(), n_q, 32, None, None, C.byref(prm),
+            prm = mvr.Index._params(P, R, k, args.precision, rescore, False, batch, False, args.persistent,
+                                    args.graph)
+            qt = None if q_nt is None else q_nt.ctypes.data_as(C.c_void_p)
+            qp = None if q_np is None else q_np.ctypes.data_as(C.c_void_p)
+            _capi.check(_capi.lib().mvr_search(ix._h, Q.data_ptr(), n_q, 32, qt, qp, C.byref(prm),
                                                out_ids.data_ptr(), out_s.data_ptr(), 1, C.c_void_p(stream),
                                                None, C.byref(tm)))
             return {f: getattr(tm, f) for f, _ in _capi.Timings._fields_}
-        ix.search_device(Q, out_ids, out_s, P, R, k, args.precision, rescore, batch=batch, stream=stream)
+        ix.search_device(Q, out_ids, out_s, P, R, k, args.precision, rescore, q_ntok=q_nt, q_nprobe=q_np,
+                         batch=batch, stream=stream, persistent=args.persistent, graph=args.graph)
 
     for _ in range(args.warmup):
         step(timings=True)
@@ -137,7 +152,10 @@ def main():
             e1 = torch.cuda.Event(enable_timing=True)
             e0.record()
             ix.search_device(Q[b0:b0 + batch], out_ids[b0:b0 + batch], out_s[b0:b0 + batch], P, R, k,
-                             args.precision, rescore, batch=batch, stream=stream)
+                             args.precision, rescore, batch=batch, stream=stream,
+                             q_ntok=None if q_nt is None else q_nt[b0:b0 + batch],
+                             q_nprobe=None if q_np is None else q_np[b0:b0 + batch],
+                             persistent=args.persistent, graph=args.graph)
             e1.record()
             ev.append((e0, e1))
     torch.cuda.synchronize()
@@ -145,19 +163,21 @@ def main():
 
     # roofline of the MaxSim kernel: algorithmic flops / bytes per launch over
     # its average launch duration in the timed region
+    # (query tokens x doc tokens) pairs: 2 x 128 flops each
     tok_per_launch = tm["rerank_tokens"] / tm["n_pass"]
     avg_launch_ms = acc["rerank_ms"] / n_launch
-    flops = 2.0 * 32 * 128 * tok_per_launch
+    flops = 2.0 * 128 * tm["rerank_pairs"] / tm["n_pass"]
     bytes_alg = 36.0 * tok_per_launch
     peak = PEAK_TFLOPS[args.precision]
-    achieved_tf = flops / (avg_launch_ms * 1e-3) / 1e12
+    # a replayed graph has no stage boundaries: no per-kernel figure then
+    achieved_tf = flops / (avg_launch_ms * 1e-3) / 1e12 if avg_launch_ms > 0 else None
     kname = {"fp32": "k_rerank_f32", "fp32_mfma": "k_rerank_f32mfma"}.get(args.precision,
                                                                        f"k_rerank_{args.precision}")
     traffic = measured_traffic(kname, args.config, args.precision)
 
-    recall = recall_report(torch, args, cfg, corpus, ix, Q, out_ids, k, doclens)
+    recall = recall_report(torch, args, cfg, corpus, ix, Q, out_ids, k, doclens, q_nt)
     cpu = None
-    if not args.no_cpu and args.cpu_queries > 0:
+    if not args.no_cpu and args.cpu_queries > 0 and q_nt is None:
         cpu = cpu_baseline(torch, corpus, Q, P, R, k, args.cpu_queries)
 
     line = {
@@ -174,8 +194,11 @@ def main():
         "dtype": "fp32" if args.precision.startswith("fp32") else args.precision,
         "data": "synthetic (seeded mixture corpus and queries, index built on the device)",
         "config": {"workload": f"{args.config}: {cfg['n_doc']} docs / {int(doclens.sum())} tokens, "
-                               f"{cfg['n_c']} centroids, 2-bit residuals; {n_q} queries x 32 tokens; "
-                               f"nprobe {P}, probe_topk {R}, k {k}",
+                               f"{cfg['n_c']} centroids, 2-bit residuals; {n_q} queries x "
+                               + ("U{8..32} tokens, nprobe U{2,4,8,16,32}" if q_nt is not None
+                                  else f"32 tokens; nprobe {P}") + f", probe_topk {R}, k {k}",
+                   "persistent": args.persistent, "graph": args.graph,
+                   "tuning": (ix.get_tuning(args.precision) if args.persistent else None),
                    "batch_per_pass": batch, "precision": args.precision, "rescore": rescore},
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3),
@@ -183,12 +206,14 @@ def main():
         "recall_at_k": recall.get("search_vs_fp64") if recall else None,
         "recall": recall,
         "stages_ms_per_step": {f: round(stages[f], 3) for f in stage_keys},
-        "roofline": {"kernel": kname, "bound": "mfma", "achieved": round(achieved_tf, 3), "peak": peak,
-                     "unit": "TFLOP/s", "frac": round(achieved_tf / peak, 4), "traffic": traffic,
-                     "avg_launch_ms": round(avg_launch_ms, 4), "launches": n_launch,
+        "roofline": {"kernel": kname, "bound": "mfma",
+                     "achieved": None if achieved_tf is None else round(achieved_tf, 3), "peak": peak,
+                     "unit": "TFLOP/s", "frac": None if achieved_tf is None else round(achieved_tf / peak, 4),
+                     "traffic": traffic, "avg_launch_ms": round(avg_launch_ms, 4), "launches": n_launch,
                      "doc_tokens_per_launch": int(tok_per_launch),
-                     "flop_per_doc_token": 2 * 32 * 128, "alg_bytes_per_doc_token": 36,
-                     "alg_GBs": round(bytes_alg / (avg_launch_ms * 1e-3) / 1e9, 1)},
+                     "flop_per_launch": flops, "flop_per_doc_token_and_query_token": 2 * 128,
+                     "alg_bytes_per_doc_token": 36,
+                     "alg_GBs": round(bytes_alg / (avg_launch_ms * 1e-3) / 1e9, 1) if avg_launch_ms > 0 else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -210,7 +235,7 @@ def measured_traffic(kname, config, precision):
     return None
 
 
-def recall_report(torch, args, cfg, corpus, ix, Q, out_ids, k, doclens):
+def recall_report(torch, args, cfg, corpus, ix, Q, out_ids, k, doclens, q_nt=None):
     """recall@k of the search against FP64 exhaustive MaxSim over the original
     embeddings, and the two numbers that bound it: exhaustive FP32 MaxSim over
     the decompressed (2-bit) index against the same FP64 truth (the ceiling any
@@ -226,8 +251,9 @@ def recall_report(torch, args, cfg, corpus, ix, Q, out_ids, k, doclens):
     ids_h = out_ids[:nr].cpu().numpy()
     dl = torch.from_numpy(doclens.astype(np.int32)).cuda()
     s64 = torch.empty((nr, n_doc), dtype=torch.float64, device="cuda")
-    _capi.check(_capi.lib().mvr_exhaustive_f64(Q.data_ptr(), nr, 32, None, corpus.X.data_ptr(),
-                                               dl.data_ptr(), n_doc, s64.data_ptr(), None))
+    qt_d = None if q_nt is None else torch.from_numpy(q_nt[:nr].astype(np.int32)).cuda()
+    _capi.check(_capi.lib().mvr_exhaustive_f64(Q.data_ptr(), nr, 32, None if qt_d is None else qt_d.data_ptr(),
+                                               corpus.X.data_ptr(), dl.data_ptr(), n_doc, s64.data_ptr(), None))
     gnd = torch.topk(s64, k, dim=1).indices.cpu().numpy()
     del s64
     docs = torch.arange(n_doc, dtype=torch.int32, device="cuda")
@@ -236,7 +262,9 @@ def recall_report(torch, args, cfg, corpus, ix, Q, out_ids, k, doclens):
     for q0 in range(0, nr, 64):
         nb = min(64, nr - q0)
         s32 = torch.empty((nb, n_doc), dtype=torch.float32, device="cuda")
-        _capi.check(_capi.lib().mvr_rerank_docs(ix._h, Q[q0:q0 + nb].contiguous().data_ptr(), nb, 32, None,
+        qt_h = None if q_nt is None else np.ascontiguousarray(q_nt[q0:q0 + nb], np.uint32)
+        _capi.check(_capi.lib().mvr_rerank_docs(ix._h, Q[q0:q0 + nb].contiguous().data_ptr(), nb, 32,
+                                                None if qt_h is None else qt_h.ctypes.data_as(C.c_void_p),
                                                 docs.expand(nb, n_doc).contiguous().data_ptr(), None, n_doc,
                                                 0, s32.data_ptr(), stream))
         dec[q0:q0 + nb] = torch.topk(s32, k, dim=1).indices.cpu().numpy()

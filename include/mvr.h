@@ -81,6 +81,10 @@ typedef struct {
   uint32_t clip_zero;   /* 1: MaxSim per token is max(0, .) (basic variant) */
   uint32_t batch;       /* queries per device pass (0 = 1024) */
   uint32_t learned;     /* 1: per-query nprobe/probe_topk from the attached probe model */
+  uint32_t persistent;  /* 1: MaxSim (and the rescore) + top-k as one persistent work-queue
+                           kernel per pass (MVR_FP32_MFMA / MVR_BF16, topk and rescore <= 256) */
+  uint32_t graph;       /* 1: each device pass is replayed from a captured hipGraph
+                           (cached per pass shape; not with learned probing) */
 } mvr_search_params;
 
 /* Optional per-stage outputs (host buffers, any may be NULL), for tests. */
@@ -97,11 +101,13 @@ typedef struct {
  * stream, summed over passes) and the work done: reranked doc tokens (sum of
  * the lengths of every reranked doc) and filter candidates, summed over queries.
  * rerank_ms is the MaxSim kernel alone; rescore_ms the FP32 rescore of a
- * BF16/FP8 search (its top-N selection and FP32 MaxSim; 0 otherwise). */
+ * BF16/FP8 search (its top-N selection and FP32 MaxSim; 0 otherwise).  A pass
+ * replayed from a hipGraph only adds to total_ms. */
 typedef struct {
   double centroid_ms, filter_ms, select_ms, rerank_ms, rescore_ms, topk_ms, total_ms;
   uint64_t rerank_tokens, candidates;
-  uint32_t n_pass;     /* device passes the call made */
+  uint32_t n_pass;        /* device passes the call made */
+  uint64_t rerank_pairs;  /* sum over queries of (query tokens x reranked doc tokens) */
 } mvr_timings;
 
 int mvr_device_count(int *count);
@@ -137,6 +143,16 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
 int mvr_rerank_docs(mvr_index *ix, const float *Q_dev, uint32_t n_q, uint32_t t_max,
                     const uint32_t *q_ntok, const uint32_t *doc_ids_dev, const uint32_t *n_docs,
                     uint32_t R, uint32_t precision, float *scores_dev, void *stream);
+
+/* Work-queue MaxSim tuning per precision (no counterpart in the reference,
+ * which launches one kernel per query): workgroups per CU of the persistent
+ * grid and 32-token tiles in flight per wave.  The first persistent search in
+ * a precision times the candidate settings on its first pass and keeps the
+ * fastest; set_tuning fixes them instead (wgs_per_cu in {1, 2}, tiles in
+ * {1, 2}).  get_tuning writes (wgs_per_cu, tiles) and fails with
+ * MVR_ERR_INVALID until a setting exists. */
+int mvr_index_set_tuning(mvr_index *ix, uint32_t precision, uint32_t wgs_per_cu, uint32_t tiles);
+int mvr_index_get_tuning(const mvr_index *ix, uint32_t precision, uint32_t out[2]);
 
 /* FP32 decompression of whole documents: out receives sum(doclens[doc_ids])
  * rows of dim floats, documents in the given order (host buffers). */

@@ -93,17 +93,37 @@ struct HBuf {
   }
 };
 
+// A captured device pass, replayed while the pass shape and the workspace
+// buffers stay the same.
+struct PassGraph {
+  std::vector<uint64_t> key;
+  hipGraphExec_t exec = nullptr;
+};
+
 struct Workspace {
   DBuf Q, meta, part_ids, part_s, top_ids, top_s, cand_cnt, cand_tot, cand_keys, sel_ids, sel_F, sel_n,
       sel_score, out_ids, out_s, counters, res_ids, res_s, res_n, bmax, feat, mlp_out, dyn_np, dyn_r,
-      rr_meta;
+      rr_meta, queue, pass_nt, pass_np, pass_rm;
   HBuf staging;
   hipEvent_t staging_done = nullptr;
+  std::vector<PassGraph> graphs;  // most recently used last
+  std::vector<uint64_t> graph_sig;  // buffer addresses the graphs were captured with
+  hipStream_t cap_stream = nullptr;
+  std::vector<DBuf *> all() {
+    return {&Q, &meta, &part_ids, &part_s, &top_ids, &top_s, &cand_cnt, &cand_tot, &cand_keys,
+            &sel_ids, &sel_F, &sel_n, &sel_score, &out_ids, &out_s, &counters, &res_ids, &res_s,
+            &res_n, &bmax, &feat, &mlp_out, &dyn_np, &dyn_r, &rr_meta, &queue, &pass_nt, &pass_np,
+            &pass_rm};
+  }
+  void drop_graphs() {
+    for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
+    graphs.clear();
+  }
   void release() {
-    for (DBuf *b : {&Q, &meta, &part_ids, &part_s, &top_ids, &top_s, &cand_cnt, &cand_tot, &cand_keys,
-                    &sel_ids, &sel_F, &sel_n, &sel_score, &out_ids, &out_s, &counters, &res_ids,
-                    &res_s, &res_n, &bmax, &feat, &mlp_out, &dyn_np, &dyn_r, &rr_meta})
-      b->release();
+    drop_graphs();
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    cap_stream = nullptr;
+    for (DBuf *b : all()) b->release();
     staging.release();
     if (staging_done) (void)hipEventDestroy(staging_done);
     staging_done = nullptr;
@@ -120,6 +140,8 @@ struct mvr_index {
   DevIndex d;
   std::vector<uint32_t> h_doclens;
   Workspace ws;
+  PersistCfg tune[4];          // work-queue MaxSim settings per precision
+  bool tuned[4] = {false, false, false, false};
 };
 
 namespace {
@@ -204,6 +226,12 @@ int mvr_index_create(const mvr_index_desc *desc, int device, mvr_index **out) {
   d.n_c = desc->n_centroid;
   d.n_ele = desc->ivf_n_ele;
   d.n_seg = (d.n_doc + kSeg - 1) / kSeg;
+  {
+    int n_cu = 0;
+    if (hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || n_cu <= 0)
+      n_cu = 256;
+    d.n_cu = (uint32_t)n_cu;
+  }
 
   auto bail = [&](int code) {
     free_devindex(ix->d);
@@ -498,10 +526,22 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     HIP_TRY(ws.res_s.ensure(Bc * R2 * 4));
     HIP_TRY(ws.res_n.ensure(Bc * 4));
   }
-  if (!on_device) {
+  // persistent work-queue MaxSim: BF16 / FP32-MFMA, top-k (and rescore) <= 256
+  const bool persistent = prm->persistent &&
+                          (prm->precision == MVR_BF16 || prm->precision == MVR_FP32_MFMA) &&
+                          k <= kPersistTopkMax && R2 <= kPersistTopkMax;
+  // graph replay: not with learned probing (the model's launches) or stage outputs
+  const bool use_graph = prm->graph && !learned && !stages;
+  if (!on_device || use_graph) {
     HIP_TRY(ws.Q.ensure(Bc * t_max * kDim * 4));
     HIP_TRY(ws.out_ids.ensure(Bc * k * 4));
     HIP_TRY(ws.out_s.ensure(Bc * k * 4));
+  }
+  if (persistent) HIP_TRY(ws.queue.ensure(4));
+  if (use_graph) {
+    HIP_TRY(ws.pass_nt.ensure(Bc * 4));
+    HIP_TRY(ws.pass_np.ensure(Bc * 4));
+    HIP_TRY(ws.pass_rm.ensure(Bc * t_max * 4));
   }
 
   // Stage timings: seven events per device pass, read once at the end of the
@@ -513,12 +553,106 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     ev.resize((size_t)n_pass * kNev);
     for (auto &e : ev) HIP_TRY(hipEventCreate(&e));
     *timings = mvr_timings{};
-    HIP_TRY(ws.counters.ensure(16));
-    HIP_TRY(launch_fill_u32(ws.counters.as<uint32_t>(), 0u, 4, s));
+    HIP_TRY(ws.counters.ensure(24));
+    HIP_TRY(launch_fill_u32(ws.counters.as<uint32_t>(), 0u, 6, s));
   }
   auto rec = [&](uint32_t pass, int i) -> hipError_t {
     return timings ? hipEventRecord(ev[(size_t)pass * kNev + i], s) : hipSuccess;
   };
+
+  // One device pass over nb queries whose inputs are in place.  `timed`
+  // records the stage events (not while a graph is being captured).
+  auto run_pass = [&](hipStream_t st, const float *Qb, const uint32_t *rm, uint32_t n_rows,
+                      const uint32_t *bnt, const uint32_t *bnp, uint32_t nb, uint32_t *oid, float *os,
+                      int bi, bool timed) -> int {
+    auto mark = [&](int i) -> hipError_t { return timed ? rec(bi, i) : hipSuccess; };
+    HIP_TRY(mark(0));
+    if (fp32_mode && !fast_ct)
+      HIP_TRY(launch_centroid_topk(Qb, rm, n_rows, nb * t_max, d, P, MVR_FP32, n_split,
+                                   ws.part_ids.as<uint32_t>(), ws.part_s.as<float>(),
+                                   ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), nullptr, st));
+    else
+      HIP_TRY(launch_centroid_topk_bf16(Qb, nb, t_max, rm, n_rows, d, P, ws.bmax.as<float>(),
+                                        ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), fp32_mode, st));
+    HIP_TRY(mark(1));
+    const uint32_t *q_r = nullptr;
+    if (learned) {
+      HIP_TRY(launch_probe_features(ws.top_s.as<float>(), t_max, P, bnt, nb, ws.feat.as<float>(), st));
+      if (mvr_mlp_forward(ix->probe, ws.feat.as<float>(), nb, ws.mlp_out.as<float>(), st) != MVR_OK)
+        return fail(MVR_ERR_DEVICE, "probe model: %s", mvr_mlp_last_error());
+      HIP_TRY(launch_probe_decide(ws.mlp_out.as<float>(), nb, ix->probe_conf, ix->probe_fallback,
+                                  std::max(ix->probe_rmin, k), R, ws.dyn_np.as<uint32_t>(),
+                                  ws.dyn_r.as<uint32_t>(), st));
+      bnp = ws.dyn_np.as<uint32_t>();
+      q_r = ws.dyn_r.as<uint32_t>();
+    }
+    HIP_TRY(launch_filter(ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), t_max, P, bnt, bnp, nb,
+                          d, ws.cand_cnt.as<uint32_t>(), ws.cand_keys.as<uint64_t>(), st));
+    HIP_TRY(mark(2));
+    HIP_TRY(launch_select(ws.cand_keys.as<uint64_t>(), ws.cand_cnt.as<uint32_t>(), d.n_seg, q_r, nb, R,
+                          ws.sel_ids.as<uint32_t>(), ws.sel_F.as<float>(), ws.sel_n.as<uint32_t>(),
+                          ws.cand_tot.as<uint32_t>(), st));
+    HIP_TRY(mark(3));
+    const int clip = (int)prm->clip_zero;
+    if (persistent) {
+      const PersistCfg &c1 = ix->tune[prm->precision];
+      uint32_t *qu = ws.queue.as<uint32_t>();
+      if (R2) {
+        // phase 1: low-precision MaxSim of all candidates, fused top-R2;
+        // phase 2: FP32 MaxSim of those, fused top-k
+        HIP_TRY(launch_persist(Qb, t_max, bnt, ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R, nb, d,
+                               (int)prm->precision, clip, c1, qu, ws.sel_score.as<float>(), R2,
+                               ws.res_ids.as<uint32_t>(), ws.res_s.as<float>(), ws.res_n.as<uint32_t>(), R2,
+                               st));
+        HIP_TRY(mark(4));
+        const PersistCfg &c2 = ix->tuned[MVR_FP32_MFMA] ? ix->tune[MVR_FP32_MFMA] : c1;
+        HIP_TRY(launch_persist(Qb, t_max, bnt, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2, nb, d,
+                               MVR_FP32_MFMA, clip, c2, qu, ws.res_s.as<float>(), k, oid, os, nullptr,
+                               k, st));
+        HIP_TRY(mark(5));
+      } else {
+        HIP_TRY(launch_persist(Qb, t_max, bnt, ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R, nb, d,
+                               (int)prm->precision, clip, c1, qu, ws.sel_score.as<float>(), k, oid,
+                               os, nullptr, k, st));
+        HIP_TRY(mark(4));
+        HIP_TRY(mark(5));
+      }
+      HIP_TRY(mark(6));
+      return MVR_OK;
+    }
+    HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R,
+                          d, prm->precision, clip, ws.sel_score.as<float>(), st));
+    HIP_TRY(mark(4));
+    if (R2) {
+      // best R2 by the low-precision score, FP32 MaxSim of those, top-k of the FP32 scores
+      HIP_TRY(launch_topk(ws.sel_ids.as<uint32_t>(), ws.sel_score.as<float>(), ws.sel_n.as<uint32_t>(),
+                          R, nb, R2, ws.res_ids.as<uint32_t>(), ws.res_s.as<float>(),
+                          ws.res_n.as<uint32_t>(), st));
+      HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2,
+                            d, MVR_FP32_MFMA, clip, ws.res_s.as<float>(), st));
+    }
+    HIP_TRY(mark(5));
+    if (R2)
+      HIP_TRY(launch_topk(ws.res_ids.as<uint32_t>(), ws.res_s.as<float>(), ws.res_n.as<uint32_t>(), R2,
+                          nb, k, oid, os, nullptr, st));
+    else
+      HIP_TRY(launch_topk(ws.sel_ids.as<uint32_t>(), ws.sel_score.as<float>(), ws.sel_n.as<uint32_t>(),
+                          R, nb, k, oid, os, nullptr, st));
+    HIP_TRY(mark(6));
+    return MVR_OK;
+  };
+
+  // Graphs are only valid for the buffers they were captured with.
+  if (use_graph) {
+    std::vector<uint64_t> sig;
+    for (DBuf *b : ws.all()) sig.push_back(reinterpret_cast<uint64_t>(b->p));
+    if (sig != ws.graph_sig) {
+      ws.drop_graphs();
+      ws.graph_sig = sig;
+    }
+    if (!ws.cap_stream) HIP_TRY(hipStreamCreateWithFlags(&ws.cap_stream, hipStreamNonBlocking));
+  }
+  std::vector<char> pass_graphed(n_pass, 0);
 
   int bi = 0;
   for (uint32_t b0 = 0; b0 < n_q; b0 += B, ++bi) {
@@ -526,13 +660,13 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     const float *Qb;
     uint32_t *oid;
     float *os;
-    if (on_device) {
+    if (on_device && !use_graph) {
       Qb = Q + (size_t)b0 * t_max * kDim;
       oid = out_ids + (size_t)b0 * k;
       os = out_scores + (size_t)b0 * k;
     } else {
       HIP_TRY(hipMemcpyAsync(ws.Q.p, Q + (size_t)b0 * t_max * kDim, (size_t)nb * t_max * kDim * 4,
-                             hipMemcpyHostToDevice, s));
+                             on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
       Qb = ws.Q.as<float>();
       oid = ws.out_ids.as<uint32_t>();
       os = ws.out_s.as<float>();
@@ -544,52 +678,85 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     const uint32_t *bnt = d_nt + b0;
     const uint32_t *bnp = d_np + b0;
 
-    HIP_TRY(rec(bi, 0));
-    if (fp32_mode && !fast_ct)
-      HIP_TRY(launch_centroid_topk(Qb, rm, n_rows, nb * t_max, d, P, MVR_FP32, n_split,
-                                   ws.part_ids.as<uint32_t>(), ws.part_s.as<float>(),
-                                   ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), nullptr, s));
-    else
-      HIP_TRY(launch_centroid_topk_bf16(Qb, nb, t_max, rm, n_rows, d, P, ws.bmax.as<float>(),
-                                        ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), fp32_mode, s));
-    HIP_TRY(rec(bi, 1));
-    const uint32_t *q_r = nullptr;
-    if (learned) {
-      HIP_TRY(launch_probe_features(ws.top_s.as<float>(), t_max, P, bnt, nb, ws.feat.as<float>(), s));
-      if (mvr_mlp_forward(ix->probe, ws.feat.as<float>(), nb, ws.mlp_out.as<float>(), s) != MVR_OK)
-        return fail(MVR_ERR_DEVICE, "probe model: %s", mvr_mlp_last_error());
-      HIP_TRY(launch_probe_decide(ws.mlp_out.as<float>(), nb, ix->probe_conf, ix->probe_fallback,
-                                  std::max(ix->probe_rmin, k), R, ws.dyn_np.as<uint32_t>(),
-                                  ws.dyn_r.as<uint32_t>(), s));
-      bnp = ws.dyn_np.as<uint32_t>();
-      q_r = ws.dyn_r.as<uint32_t>();
+    if (persistent && !ix->tuned[prm->precision]) {
+      // autotune on this pass: every setting once to warm up, then timed;
+      // results are identical for every setting, so the pass stays valid
+      hipEvent_t e0, e1;
+      HIP_TRY(hipEventCreate(&e0));
+      HIP_TRY(hipEventCreate(&e1));
+      float best = 1e30f;
+      PersistCfg best_c;
+      for (int rep = 0; rep < 2; ++rep)
+        for (uint32_t tps = 1; tps <= 2; ++tps)
+          for (uint32_t wpc = 1; wpc <= 2; ++wpc) {
+              PersistCfg c;
+              c.n_wg = wpc * d.n_cu;
+              c.tps = tps;
+              ix->tune[prm->precision] = c;
+              HIP_TRY(hipEventRecord(e0, s));
+              const int rc = run_pass(s, Qb, rm, n_rows, bnt, bnp, nb, oid, os, bi, false);
+              if (rc != MVR_OK) return rc;
+              HIP_TRY(hipEventRecord(e1, s));
+              HIP_TRY(hipEventSynchronize(e1));
+              float ms = 0.0f;
+              HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+              if (rep == 1 && ms < best) { best = ms; best_c = c; }
+            }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      ix->tune[prm->precision] = best_c;
+      ix->tuned[prm->precision] = true;
     }
-    HIP_TRY(launch_filter(ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), t_max, P, bnt, bnp, nb,
-                          d, ws.cand_cnt.as<uint32_t>(), ws.cand_keys.as<uint64_t>(), s));
-    HIP_TRY(rec(bi, 2));
-    HIP_TRY(launch_select(ws.cand_keys.as<uint64_t>(), ws.cand_cnt.as<uint32_t>(), d.n_seg, q_r, nb, R,
-                          ws.sel_ids.as<uint32_t>(), ws.sel_F.as<float>(), ws.sel_n.as<uint32_t>(),
-                          ws.cand_tot.as<uint32_t>(), s));
-    HIP_TRY(rec(bi, 3));
-    HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R,
-                          d, prm->precision, (int)prm->clip_zero, ws.sel_score.as<float>(), s));
-    HIP_TRY(rec(bi, 4));
-    if (R2) {
-      // best R2 by the low-precision score, FP32 MaxSim of those, top-k of the FP32 scores
-      HIP_TRY(launch_topk(ws.sel_ids.as<uint32_t>(), ws.sel_score.as<float>(), ws.sel_n.as<uint32_t>(),
-                          R, nb, R2, ws.res_ids.as<uint32_t>(), ws.res_s.as<float>(),
-                          ws.res_n.as<uint32_t>(), s));
-      HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2,
-                            d, MVR_FP32_MFMA, (int)prm->clip_zero, ws.res_s.as<float>(), s));
+
+    if (use_graph) {
+      // inputs of the pass into fixed buffers, then replay (or capture) the pass
+      HIP_TRY(hipMemcpyAsync(ws.pass_nt.p, bnt, (size_t)nb * 4, hipMemcpyDeviceToDevice, s));
+      HIP_TRY(hipMemcpyAsync(ws.pass_np.p, bnp, (size_t)nb * 4, hipMemcpyDeviceToDevice, s));
+      if (rm) HIP_TRY(hipMemcpyAsync(ws.pass_rm.p, rm, (size_t)n_rows * 4, hipMemcpyDeviceToDevice, s));
+      const PersistCfg &tc = ix->tune[prm->precision];
+      const std::vector<uint64_t> key = {nb, n_rows, t_max, P, R, R2, k, prm->precision, prm->clip_zero,
+                                         uniform ? 1u : 0u, persistent ? 1u : 0u, fast_ct ? 1u : 0u,
+                                         tc.n_wg, tc.tps};
+      hipGraphExec_t exec = nullptr;
+      for (size_t gi = 0; gi < ws.graphs.size(); ++gi)
+        if (ws.graphs[gi].key == key) {
+          exec = ws.graphs[gi].exec;
+          std::rotate(ws.graphs.begin() + gi, ws.graphs.begin() + gi + 1, ws.graphs.end());
+          break;
+        }
+      if (!exec) {
+        hipGraph_t graph = nullptr;
+        HIP_TRY(hipStreamBeginCapture(ws.cap_stream, hipStreamCaptureModeRelaxed));
+        const int rc = run_pass(ws.cap_stream, ws.Q.as<float>(), rm ? ws.pass_rm.as<uint32_t>() : nullptr,
+                                n_rows, ws.pass_nt.as<uint32_t>(), ws.pass_np.as<uint32_t>(), nb,
+                                ws.out_ids.as<uint32_t>(), ws.out_s.as<float>(), bi, false);
+        const hipError_t ce = hipStreamEndCapture(ws.cap_stream, &graph);
+        if (rc != MVR_OK) {
+          if (graph) (void)hipGraphDestroy(graph);
+          return rc;
+        }
+        HIP_TRY(ce);
+        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HIP_TRY(ie);
+        if (ws.graphs.size() >= 64) {
+          (void)hipGraphExecDestroy(ws.graphs.front().exec);
+          ws.graphs.erase(ws.graphs.begin());
+        }
+        ws.graphs.push_back(PassGraph{key, exec});
+      }
+      HIP_TRY(rec(bi, 0));
+      HIP_TRY(hipGraphLaunch(exec, s));
+      for (int i = 1; i < kNev; ++i) HIP_TRY(rec(bi, i));
+      pass_graphed[bi] = 1;
+      if (on_device) {
+        HIP_TRY(hipMemcpyAsync(out_ids + (size_t)b0 * k, oid, (size_t)nb * k * 4, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(out_scores + (size_t)b0 * k, os, (size_t)nb * k * 4, hipMemcpyDeviceToDevice, s));
+      }
+    } else {
+      const int rc = run_pass(s, Qb, rm, n_rows, bnt, bnp, nb, oid, os, bi, true);
+      if (rc != MVR_OK) return rc;
     }
-    HIP_TRY(rec(bi, 5));
-    if (R2)
-      HIP_TRY(launch_topk(ws.res_ids.as<uint32_t>(), ws.res_s.as<float>(), ws.res_n.as<uint32_t>(), R2,
-                          nb, k, oid, os, nullptr, s));
-    else
-      HIP_TRY(launch_topk(ws.sel_ids.as<uint32_t>(), ws.sel_score.as<float>(), ws.sel_n.as<uint32_t>(),
-                          R, nb, k, oid, os, nullptr, s));
-    HIP_TRY(rec(bi, 6));
     if (!on_device) {
       HIP_TRY(hipMemcpyAsync(out_ids + (size_t)b0 * k, oid, (size_t)nb * k * 4, hipMemcpyDeviceToHost, s));
       HIP_TRY(hipMemcpyAsync(out_scores + (size_t)b0 * k, os, (size_t)nb * k * 4, hipMemcpyDeviceToHost, s));
@@ -611,7 +778,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     }
     if (timings)
       HIP_TRY(launch_count_work(ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R, nb, d,
-                                ws.cand_tot.as<uint32_t>(), ws.counters.as<unsigned long long>(), s));
+                                ws.cand_tot.as<uint32_t>(), d_nt + b0, ws.counters.as<unsigned long long>(), s));
   }
   if (timings) {
     HIP_TRY(hipEventSynchronize(ev.back()));
@@ -619,22 +786,46 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       float ms[kNev - 1];
       for (int i = 0; i < kNev - 1; ++i)
         HIP_TRY(hipEventElapsedTime(&ms[i], ev[b * kNev + i], ev[b * kNev + i + 1]));
+      timings->total_ms += ms[0] + ms[1] + ms[2] + ms[3] + ms[4] + ms[5];
+      if (pass_graphed[b]) continue;  // a replayed graph has no stage boundaries
       timings->centroid_ms += ms[0];
       timings->filter_ms += ms[1];
       timings->select_ms += ms[2];
       timings->rerank_ms += ms[3];
       timings->rescore_ms += ms[4];
       timings->topk_ms += ms[5];
-      timings->total_ms += ms[0] + ms[1] + ms[2] + ms[3] + ms[4] + ms[5];
     }
     timings->n_pass = n_pass;
-    unsigned long long cnt[2] = {0, 0};
-    HIP_TRY(hipMemcpy(cnt, ws.counters.p, 16, hipMemcpyDeviceToHost));
+    unsigned long long cnt[3] = {0, 0, 0};
+    HIP_TRY(hipMemcpy(cnt, ws.counters.p, 24, hipMemcpyDeviceToHost));
     timings->rerank_tokens = cnt[0];
     timings->candidates = cnt[1];
+    timings->rerank_pairs = cnt[2];
     for (auto &e : ev) (void)hipEventDestroy(e);
   }
   if (!on_device) HIP_TRY(hipStreamSynchronize(s));
+  return MVR_OK;
+}
+
+int mvr_index_set_tuning(mvr_index *ix, uint32_t precision, uint32_t wgs_per_cu, uint32_t tiles) {
+  if (!ix) return fail(MVR_ERR_INVALID, "NULL index");
+  if (precision != MVR_BF16 && precision != MVR_FP32_MFMA)
+    return fail(MVR_ERR_UNSUPPORTED, "no work-queue kernel for precision %u", precision);
+  if (wgs_per_cu < 1 || wgs_per_cu > 2) return fail(MVR_ERR_INVALID, "wgs_per_cu %u not in {1,2}", wgs_per_cu);
+  if (tiles < 1 || tiles > 2) return fail(MVR_ERR_INVALID, "tiles %u not in {1,2}", tiles);
+  PersistCfg c;
+  c.n_wg = wgs_per_cu * ix->d.n_cu;
+  c.tps = tiles;
+  ix->tune[precision] = c;
+  ix->tuned[precision] = true;
+  return MVR_OK;
+}
+
+int mvr_index_get_tuning(const mvr_index *ix, uint32_t precision, uint32_t out[2]) {
+  if (!ix || !out) return fail(MVR_ERR_INVALID, "NULL argument");
+  if (precision > 3 || !ix->tuned[precision]) return fail(MVR_ERR_INVALID, "precision %u not tuned", precision);
+  out[0] = ix->tune[precision].n_wg / std::max(1u, ix->d.n_cu);
+  out[1] = ix->tune[precision].tps;
   return MVR_OK;
 }
 

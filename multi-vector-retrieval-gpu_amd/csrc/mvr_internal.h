@@ -22,6 +22,7 @@ struct DevIndex {
   uint32_t n_doc = 0, n_c = 0, n_seg = 0, max_list = 0, max_doclen = 0;
   uint64_t n_tok = 0, n_ele = 0;
   float c_norm_max = 0.0f;     // max_c ||C_c||_2 (bounds the centroid-score error)
+  uint32_t n_cu = 0;           // compute units of the device
   float *C = nullptr;          // [n_c][128] f32
   uint16_t *C_bf16 = nullptr;  // [n_c][128] bf16 (RNE of C)
   uint8_t *C_fp8 = nullptr;    // [n_c][128] e4m3 of C / scale
@@ -72,6 +73,19 @@ hipError_t launch_select(const uint64_t *cand_keys, const uint32_t *seg_cnt, uin
                          const uint32_t *q_r, uint32_t n_q, uint32_t R, uint32_t *sel_ids,
                          float *sel_F, uint32_t *sel_n, uint32_t *cand_tot, hipStream_t s);
 
+// persistent work-queue MaxSim + top-kk (kk <= 256) over whole queries;
+// precision 1 (BF16) or 3 (FP32 MFMA).  queue [1] is scratch.
+struct PersistCfg {
+  uint32_t n_wg = 0;   // workgroups in the grid
+  uint32_t tps = 1;    // 32-token tiles in flight per wave
+};
+constexpr uint32_t kPersistTopkMax = 256;
+hipError_t launch_persist(const float *Q, uint32_t t_max, const uint32_t *q_ntok,
+                          const uint32_t *in_ids, const uint32_t *in_n, uint32_t R_in, uint32_t nb,
+                          const DevIndex &ix, int precision, int clip_zero, const PersistCfg &cfg,
+                          uint32_t *queue, float *score, uint32_t kk, uint32_t *out_ids, float *out_s,
+                          uint32_t *out_n, uint32_t out_stride, hipStream_t s);
+
 // learned probe: features [n_q][1024] from the top-32 scores, and decisions
 hipError_t launch_probe_features(const float *top_s, uint32_t t_max, uint32_t P,
                                  const uint32_t *q_ntok, uint32_t n_q, float *X, hipStream_t s);
@@ -91,7 +105,7 @@ hipError_t launch_topk(const uint32_t *ids, const float *score, const uint32_t *
 
 // counters[0] += sum of doclens over the selected docs, counters[1] += candidates
 hipError_t launch_count_work(const uint32_t *sel_ids, const uint32_t *sel_n, uint32_t R, uint32_t n_q,
-                             const DevIndex &ix, const uint32_t *cand_cnt,
+                             const DevIndex &ix, const uint32_t *cand_cnt, const uint32_t *q_ntok,
                              unsigned long long *counters, hipStream_t s);
 
 hipError_t launch_exhaustive_f64(const float *Q, uint32_t n_q, uint32_t t_max,

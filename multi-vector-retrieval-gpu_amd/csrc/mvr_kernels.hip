@@ -672,6 +672,80 @@ typedef float floatx16_t __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ bf16x8_t as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8_t, v); }
 
+// Per-query-token max over one doc's tokens on BF16 MFMA tiles; returns, in
+// lanes 0..31, the max for query token `col` (after the cross-half max).
+// TPS tiles of 32 doc tokens are in flight per step (independent
+// accumulators, so one tile's loads overlap the other's MFMAs).
+template <int TPS>
+__device__ __forceinline__ float maxsim_doc_bf16(uint32_t L, uint64_t off, uint32_t h, uint32_t col,
+                                                 const uint32_t *__restrict__ codes,
+                                                 const uint8_t *__restrict__ residual,
+                                                 const uint16_t *__restrict__ Cb, const uint2 *lut,
+                                                 const bf16x8_t (&qf)[8], int clip_zero) {
+  float M = clip_zero ? 0.0f : -FLT_MAX;
+  for (uint32_t i0 = 0; i0 < L; i0 += 32 * TPS) {
+    uint4 cv[TPS][8];
+    uint4 rb[TPS];
+#pragma unroll
+    for (int u = 0; u < TPS; ++u) {
+      const uint32_t i = i0 + 32 * u + col;
+      rb[u] = make_uint4(0u, 0u, 0u, 0u);
+      if (i < L) {
+        const uint64_t g = off + i;
+        const uint32_t code = codes[g];
+        rb[u] = *reinterpret_cast<const uint4 *>(residual + g * kResBytes + 16 * h);
+        const uint4 *cp = reinterpret_cast<const uint4 *>(Cb + (size_t)code * kDim + 64 * h);
+#pragma unroll
+        for (uint32_t s = 0; s < 8; ++s) cv[u][s] = cp[s];
+      } else {
+#pragma unroll
+        for (uint32_t s = 0; s < 8; ++s) cv[u][s] = make_uint4(0u, 0u, 0u, 0u);
+      }
+    }
+    floatx16_t acc[TPS];
+#pragma unroll
+    for (int u = 0; u < TPS; ++u) acc[u] = floatx16_t{};
+#pragma unroll
+    for (uint32_t s = 0; s < 8; ++s) {
+#pragma unroll
+      for (int u = 0; u < TPS; ++u) {
+        const uint32_t rw[4] = {rb[u].x, rb[u].y, rb[u].z, rb[u].w};
+        acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(cv[u][s]), qf[s], acc[u], 0, 0, 0);
+        const uint32_t word = rw[s >> 1];
+        const uint32_t b0 = (word >> (16 * (s & 1))) & 0xFFu;
+        const uint32_t b1 = (word >> (16 * (s & 1) + 8)) & 0xFFu;
+        const uint32_t pos = 16 * h + 2 * s;
+        const uint2 r0 = lut[pos * 256 + b0];
+        const uint2 r1 = lut[(pos + 1) * 256 + b1];
+        acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(make_uint4(r0.x, r0.y, r1.x, r1.y)),
+                                                         qf[s], acc[u], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < TPS; ++u)
+#pragma unroll
+      for (uint32_t j = 0; j < 16; ++j) {
+        const uint32_t m = 32 * u + 8 * (j >> 2) + 4 * h + (j & 3);
+        if (i0 + m < L) M = fmaxf(M, acc[u][j]);
+      }
+  }
+  return fmaxf(M, __shfl_xor(M, 32));
+}
+
+// Query tokens as the B operand: lane (col, h) holds Q[col][64h .. 64h+63] in
+// bf16 (zero past the query's T tokens).
+__device__ __forceinline__ void load_q_bf16(const float *__restrict__ Q, uint32_t q, uint32_t t_max,
+                                            uint32_t T, uint32_t h, uint32_t col, bf16x8_t (&qf)[8]) {
+  const float *qr = Q + ((size_t)q * t_max + col) * kDim + 64 * h;
+#pragma unroll
+  for (uint32_t s = 0; s < 8; ++s) {
+    bf16x8_t v;
+#pragma unroll
+    for (uint32_t e = 0; e < 8; ++e) v[e] = (__bf16)(col < T ? qr[8 * s + e] : 0.0f);
+    qf[s] = v;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_rerank_bf16(
     const float *__restrict__ Q, uint32_t t_max, const uint32_t *__restrict__ q_ntok,
     const uint32_t *__restrict__ sel_ids, const uint32_t *__restrict__ sel_n, uint32_t R,
@@ -689,59 +763,13 @@ __global__ __launch_bounds__(256) void k_rerank_bf16(
 
   for (uint32_t i = tid; i < kResBytes * 256; i += 256) lut[i] = lut_g[i];
   bf16x8_t qf[8];
-  {
-    const float *qr = Q + ((size_t)q * t_max + col) * kDim + 64 * h;
-#pragma unroll
-    for (uint32_t s = 0; s < 8; ++s) {
-      bf16x8_t v;
-#pragma unroll
-      for (uint32_t e = 0; e < 8; ++e) v[e] = (__bf16)(col < T ? qr[8 * s + e] : 0.0f);
-      qf[s] = v;
-    }
-  }
+  load_q_bf16(Q, q, t_max, T, h, col, qf);
   __syncthreads();
 
   for (uint32_t r = slice * 4 + wave; r < n; r += n_slice * 4) {
     const uint32_t doc = sel_ids[(size_t)q * R + r];
-    const uint32_t L = doclens[doc];
-    const uint64_t off = docoff[doc];
-    float M = clip_zero ? 0.0f : -FLT_MAX;
-    for (uint32_t i0 = 0; i0 < L; i0 += 32) {
-      const uint32_t i = i0 + col;
-      uint4 cv[8];
-      uint4 rb = make_uint4(0u, 0u, 0u, 0u);
-      if (i < L) {
-        const uint64_t g = off + i;
-        const uint32_t code = codes[g];
-        rb = *reinterpret_cast<const uint4 *>(residual + g * kResBytes + 16 * h);
-        const uint4 *cp = reinterpret_cast<const uint4 *>(Cb + (size_t)code * kDim + 64 * h);
-#pragma unroll
-        for (uint32_t s = 0; s < 8; ++s) cv[s] = cp[s];
-      } else {
-#pragma unroll
-        for (uint32_t s = 0; s < 8; ++s) cv[s] = make_uint4(0u, 0u, 0u, 0u);
-      }
-      const uint32_t rw[4] = {rb.x, rb.y, rb.z, rb.w};
-      floatx16_t acc = {};
-#pragma unroll
-      for (uint32_t s = 0; s < 8; ++s) {
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(cv[s]), qf[s], acc, 0, 0, 0);
-        const uint32_t word = rw[s >> 1];
-        const uint32_t b0 = (word >> (16 * (s & 1))) & 0xFFu;
-        const uint32_t b1 = (word >> (16 * (s & 1) + 8)) & 0xFFu;
-        const uint32_t pos = 16 * h + 2 * s;
-        const uint2 r0 = lut[pos * 256 + b0];
-        const uint2 r1 = lut[(pos + 1) * 256 + b1];
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(make_uint4(r0.x, r0.y, r1.x, r1.y)),
-                                                      qf[s], acc, 0, 0, 0);
-      }
-#pragma unroll
-      for (uint32_t j = 0; j < 16; ++j) {
-        const uint32_t m = 8 * (j >> 2) + 4 * h + (j & 3);
-        if (i0 + m < L) M = fmaxf(M, acc[j]);
-      }
-    }
-    M = fmaxf(M, __shfl_xor(M, 32));
+    const float M = maxsim_doc_bf16<1>(doclens[doc], docoff[doc], h, col, codes, residual, Cb, lut, qf,
+                                       clip_zero);
     if (h == 0) Ms[wave][col] = M;
     wave_lds_sync();
     if (lane == 0) {
@@ -766,6 +794,76 @@ __global__ __launch_bounds__(256) void k_rerank_bf16(
 // bit-identical); the max over tokens is local plus one cross-half shuffle
 // and the sum over query tokens runs left to right.
 // ---------------------------------------------------------------------------
+// Per-query-token max over one doc's tokens on FP32 MFMA tiles (see above);
+// wl = the [128][4] decode weights in LDS; TPS tiles in flight per step.
+template <int TPS>
+__device__ __forceinline__ float maxsim_doc_f32mfma(uint32_t L, uint64_t off, uint32_t h, uint32_t col,
+                                                    const uint32_t *__restrict__ codes,
+                                                    const uint8_t *__restrict__ residual,
+                                                    const float *__restrict__ C, const float *wl,
+                                                    const float (&qf)[64], int clip_zero) {
+  float M = clip_zero ? 0.0f : -FLT_MAX;
+  for (uint32_t i0 = 0; i0 < L; i0 += 32 * TPS) {
+    float4 cv[TPS][16];
+    uint4 rb[TPS];
+#pragma unroll
+    for (int u = 0; u < TPS; ++u) {
+      const uint32_t i = i0 + 32 * u + col;
+      rb[u] = make_uint4(0u, 0u, 0u, 0u);
+      if (i < L) {
+        const uint64_t g = off + i;
+        const uint32_t code = codes[g];
+        rb[u] = *reinterpret_cast<const uint4 *>(residual + g * kResBytes + 16 * h);
+        const float4 *cp = reinterpret_cast<const float4 *>(C + (size_t)code * kDim + 64 * h);
+#pragma unroll
+        for (uint32_t s = 0; s < 16; ++s) cv[u][s] = cp[s];
+      } else {
+#pragma unroll
+        for (uint32_t s = 0; s < 16; ++s) cv[u][s] = make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+    }
+    floatx16_t acc[TPS];
+#pragma unroll
+    for (int u = 0; u < TPS; ++u) acc[u] = floatx16_t{};
+#pragma unroll
+    for (uint32_t s = 0; s < 16; ++s) {
+#pragma unroll
+      for (int u = 0; u < TPS; ++u) {
+        const uint32_t rw[4] = {rb[u].x, rb[u].y, rb[u].z, rb[u].w};
+        // residual byte 16h + s holds dims 64h + 4s .. +3, most significant bits first
+        const uint32_t byte = (rw[s >> 2] >> (8 * (s & 3))) & 0xFFu;
+        const float *wd = wl + 4 * (64 * h + 4 * s);  // weights of dim 64h + 4s, 4 per dim
+        const float x0 = cv[u][s].x + wd[(byte >> 6) & 3u];
+        const float x1 = cv[u][s].y + wd[4 + ((byte >> 4) & 3u)];
+        const float x2 = cv[u][s].z + wd[8 + ((byte >> 2) & 3u)];
+        const float x3 = cv[u][s].w + wd[12 + (byte & 3u)];
+        acc[u] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0, qf[4 * s], acc[u], 0, 0, 0);
+        acc[u] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1, qf[4 * s + 1], acc[u], 0, 0, 0);
+        acc[u] = __builtin_amdgcn_mfma_f32_32x32x2f32(x2, qf[4 * s + 2], acc[u], 0, 0, 0);
+        acc[u] = __builtin_amdgcn_mfma_f32_32x32x2f32(x3, qf[4 * s + 3], acc[u], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < TPS; ++u)
+#pragma unroll
+      for (uint32_t j = 0; j < 16; ++j) {
+        const uint32_t m = 32 * u + 8 * (j >> 2) + 4 * h + (j & 3);
+        if (i0 + m < L) M = fmaxf(M, acc[u][j]);
+      }
+  }
+  return fmaxf(M, __shfl_xor(M, 32));
+}
+
+__device__ __forceinline__ void load_q_f32(const float *__restrict__ Q, uint32_t q, uint32_t t_max,
+                                           uint32_t T, uint32_t h, uint32_t col, float (&qf)[64]) {
+  const float4 *qr = reinterpret_cast<const float4 *>(Q + ((size_t)q * t_max + col) * kDim + 64 * h);
+#pragma unroll
+  for (uint32_t s = 0; s < 16; ++s) {
+    const float4 v = (col < T) ? qr[s] : make_float4(0.f, 0.f, 0.f, 0.f);
+    qf[4 * s] = v.x; qf[4 * s + 1] = v.y; qf[4 * s + 2] = v.z; qf[4 * s + 3] = v.w;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_rerank_f32mfma(
     const float *__restrict__ Q, uint32_t t_max, const uint32_t *__restrict__ q_ntok,
     const uint32_t *__restrict__ sel_ids, const uint32_t *__restrict__ sel_n, uint32_t R,
@@ -783,59 +881,13 @@ __global__ __launch_bounds__(256) void k_rerank_f32mfma(
 
   for (uint32_t i = tid; i < kDim * 4; i += 256) wl[i] = W[i];
   float qf[64];
-  {
-    const float4 *qr = reinterpret_cast<const float4 *>(Q + ((size_t)q * t_max + col) * kDim + 64 * h);
-#pragma unroll
-    for (uint32_t s = 0; s < 16; ++s) {
-      const float4 v = (col < T) ? qr[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-      qf[4 * s] = v.x; qf[4 * s + 1] = v.y; qf[4 * s + 2] = v.z; qf[4 * s + 3] = v.w;
-    }
-  }
+  load_q_f32(Q, q, t_max, T, h, col, qf);
   __syncthreads();
 
   for (uint32_t r = slice * 4 + wave; r < n; r += n_slice * 4) {
     const uint32_t doc = sel_ids[(size_t)q * R + r];
-    const uint32_t L = doclens[doc];
-    const uint64_t off = docoff[doc];
-    float M = clip_zero ? 0.0f : -FLT_MAX;
-    for (uint32_t i0 = 0; i0 < L; i0 += 32) {
-      const uint32_t i = i0 + col;
-      float4 cv[16];
-      uint4 rb = make_uint4(0u, 0u, 0u, 0u);
-      if (i < L) {
-        const uint64_t g = off + i;
-        const uint32_t code = codes[g];
-        rb = *reinterpret_cast<const uint4 *>(residual + g * kResBytes + 16 * h);
-        const float4 *cp = reinterpret_cast<const float4 *>(C + (size_t)code * kDim + 64 * h);
-#pragma unroll
-        for (uint32_t s = 0; s < 16; ++s) cv[s] = cp[s];
-      } else {
-#pragma unroll
-        for (uint32_t s = 0; s < 16; ++s) cv[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-      const uint32_t rw[4] = {rb.x, rb.y, rb.z, rb.w};
-      floatx16_t acc = {};
-#pragma unroll
-      for (uint32_t s = 0; s < 16; ++s) {
-        // residual byte 16h + s holds dims 64h + 4s .. +3, most significant bits first
-        const uint32_t byte = (rw[s >> 2] >> (8 * (s & 3))) & 0xFFu;
-        const float *wd = wl + 4 * (64 * h + 4 * s);  // weights of dim 64h + 4s, 4 per dim
-        const float x0 = cv[s].x + wd[(byte >> 6) & 3u];
-        const float x1 = cv[s].y + wd[4 + ((byte >> 4) & 3u)];
-        const float x2 = cv[s].z + wd[8 + ((byte >> 2) & 3u)];
-        const float x3 = cv[s].w + wd[12 + (byte & 3u)];
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x0, qf[4 * s], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x1, qf[4 * s + 1], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x2, qf[4 * s + 2], acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x3, qf[4 * s + 3], acc, 0, 0, 0);
-      }
-#pragma unroll
-      for (uint32_t j = 0; j < 16; ++j) {
-        const uint32_t m = 8 * (j >> 2) + 4 * h + (j & 3);
-        if (i0 + m < L) M = fmaxf(M, acc[j]);
-      }
-    }
-    M = fmaxf(M, __shfl_xor(M, 32));
+    const float M = maxsim_doc_f32mfma<1>(doclens[doc], docoff[doc], h, col, codes, residual, C, wl, qf,
+                                          clip_zero);
     if (h == 0) Ms[wave][col] = M;
     wave_lds_sync();
     if (lane == 0) {
@@ -844,6 +896,171 @@ __global__ __launch_bounds__(256) void k_rerank_f32mfma(
       sel_score[(size_t)q * R + r] = sc;
     }
     wave_lds_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Block-wide top-kk (kk <= 256) of n unique keys make_key(sc[i], ids[i]),
+// written best first.  n <= 512: one bitonic sort in LDS.  Otherwise MSB-first
+// radix select with 8-bit digits: keys above the pivot digit go to the output,
+// the pivot bucket is refined until it fits the 512-key scratch, then sorted.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kTopkMax = 256;
+constexpr uint32_t kTopkScr = 512;
+
+struct TopkLds {
+  unsigned long long out[kTopkMax];
+  unsigned long long scr[kTopkScr];
+  uint32_t hist[256];
+  uint32_t misc[8];
+};
+
+__device__ void block_topk_small(const float *__restrict__ sc, const uint32_t *__restrict__ ids,
+                                 uint32_t n, uint32_t kk, TopkLds &L, uint32_t *__restrict__ o_ids,
+                                 float *__restrict__ o_s, uint32_t *__restrict__ o_n) {
+  const uint32_t tid = threadIdx.x, nt = blockDim.x;
+  const uint32_t n_sel = min(n, kk);
+  unsigned long long *res;
+  if (n <= kTopkScr) {
+    const uint32_t np = pow2_ceil(max(n, 2u));
+    for (uint32_t i = tid; i < np; i += nt) L.scr[i] = (i < n) ? make_key(sc[i], ids[i]) : 0ull;
+    __syncthreads();
+    bitonic_desc(L.scr, np);
+    res = L.scr;
+  } else {
+    if (tid == 0) L.misc[0] = 0;
+    uint64_t prefix = 0;
+    uint32_t shift = 64, need = kk;
+    for (;;) {
+      const uint32_t hi = shift;
+      shift -= 8;
+      for (uint32_t b = tid; b < 256; b += nt) L.hist[b] = 0;
+      __syncthreads();
+      for (uint32_t i = tid; i < n; i += nt) {
+        const uint64_t k = make_key(sc[i], ids[i]);
+        if (hi == 64 || (k >> hi) == prefix) atomicAdd(&L.hist[(uint32_t)(k >> shift) & 255u], 1u);
+      }
+      __syncthreads();
+      if (tid == 0) {
+        uint32_t cum = 0, b = 256;
+        while (b > 0) {
+          --b;
+          if (cum + L.hist[b] >= need) break;
+          cum += L.hist[b];
+        }
+        L.misc[2] = b; L.misc[3] = cum; L.misc[4] = L.hist[b]; L.misc[1] = 0;
+      }
+      __syncthreads();
+      const uint32_t dsel = L.misc[2], gt = L.misc[3], eq = L.misc[4];
+      const bool take_eq = gt + eq == need;
+      const bool eq_fits = eq <= kTopkScr;
+      for (uint32_t i = tid; i < n; i += nt) {
+        const uint64_t k = make_key(sc[i], ids[i]);
+        if (hi == 64 || (k >> hi) == prefix) {
+          const uint32_t dg = (uint32_t)(k >> shift) & 255u;
+          if (dg > dsel || (dg == dsel && take_eq)) L.out[atomicAdd(&L.misc[0], 1u)] = k;
+          else if (dg == dsel && eq_fits) L.scr[atomicAdd(&L.misc[1], 1u)] = k;
+        }
+      }
+      __syncthreads();
+      need -= gt;
+      if (take_eq) break;
+      if (eq_fits) {
+        const uint32_t ns = L.misc[1];
+        const uint32_t np = pow2_ceil(max(ns, 2u));
+        for (uint32_t i = ns + tid; i < np; i += nt) L.scr[i] = 0ull;
+        __syncthreads();
+        bitonic_desc(L.scr, np);
+        const uint32_t base = L.misc[0];
+        for (uint32_t i = tid; i < need; i += nt) L.out[base + i] = L.scr[i];
+        __syncthreads();
+        break;
+      }
+      prefix = (prefix << 8) | dsel;  // keys are unique, so the bucket shrinks to fit
+    }
+    const uint32_t np = pow2_ceil(max(n_sel, 2u));
+    for (uint32_t i = n_sel + tid; i < np; i += nt) L.out[i] = 0ull;
+    __syncthreads();
+    bitonic_desc(L.out, np);
+    res = L.out;
+  }
+  for (uint32_t i = tid; i < kk; i += nt) {
+    if (i < n_sel) {
+      o_ids[i] = ~(uint32_t)(res[i] & 0xFFFFFFFFull);
+      o_s[i] = ord2f((uint32_t)(res[i] >> 32));
+    } else {
+      o_ids[i] = 0xFFFFFFFFu;
+      o_s[i] = -INFINITY;
+    }
+  }
+  if (o_n && tid == 0) *o_n = n_sel;
+}
+
+// ---------------------------------------------------------------------------
+// Persistent fused decompress + MaxSim + top-k.  A fixed grid of workgroups
+// (sized to the resident capacity) pulls queries from a device queue; the 4
+// waves of a workgroup score the query's candidates (one doc per wave at a
+// time), then the same workgroup selects the query's top-kk from those scores
+// -- no cross-workgroup hand-off, so candidate lists of very different lengths
+// spread over the chip by the queue alone.  The decode tables are loaded into
+// LDS once per workgroup, the query operand once per query.
+// PREC 1: BF16 MFMA; PREC 3: FP32 MFMA.  TPS: 32-token tiles in flight.
+// ---------------------------------------------------------------------------
+template <int PREC, int TPS>
+__global__ __launch_bounds__(256) void k_rerank_persist(
+    const float *__restrict__ Q, uint32_t t_max, const uint32_t *__restrict__ q_ntok,
+    const uint32_t *__restrict__ in_ids, const uint32_t *__restrict__ in_n, uint32_t R_in,
+    uint32_t nb, uint32_t *__restrict__ queue, const uint32_t *__restrict__ doclens,
+    const uint64_t *__restrict__ docoff, const uint32_t *__restrict__ codes,
+    const uint8_t *__restrict__ residual, const float *__restrict__ C,
+    const float *__restrict__ W, const uint16_t *__restrict__ Cb, const uint2 *__restrict__ lut_g,
+    int clip_zero, float *__restrict__ score, uint32_t kk, uint32_t *__restrict__ out_ids,
+    float *__restrict__ out_s, uint32_t *__restrict__ out_n, uint32_t out_stride) {
+  __shared__ uint2 lut[PREC == 1 ? kResBytes * 256 : 1];
+  __shared__ float wl[PREC == 3 ? kDim * 4 : 1];
+  __shared__ float Ms[4][kMaxQTok];
+  __shared__ TopkLds tk;
+  __shared__ uint32_t s_q;
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const uint32_t h = lane >> 5, col = lane & 31;
+  if constexpr (PREC == 1)
+    for (uint32_t i = tid; i < kResBytes * 256; i += 256) lut[i] = lut_g[i];
+  else
+    for (uint32_t i = tid; i < kDim * 4; i += 256) wl[i] = W[i];
+  bf16x8_t qb[PREC == 1 ? 8 : 1];
+  float qf[PREC == 3 ? 64 : 1];
+  for (;;) {
+    __syncthreads();  // the previous query's top-k is done with the LDS
+    if (tid == 0) s_q = atomicAdd(queue, 1u);
+    __syncthreads();
+    const uint32_t q = s_q;
+    if (q >= nb) break;
+    const uint32_t n = min(in_n[q], R_in);
+    const uint32_t T = q_ntok ? q_ntok[q] : t_max;
+    if constexpr (PREC == 1) load_q_bf16(Q, q, t_max, T, h, col, qb);
+    else load_q_f32(Q, q, t_max, T, h, col, qf);
+    float *qs = score + (size_t)q * R_in;
+    for (uint32_t r = wave; r < n; r += 4) {
+      const uint32_t doc = in_ids[(size_t)q * R_in + r];
+      float M;
+      if constexpr (PREC == 1)
+        M = maxsim_doc_bf16<TPS>(doclens[doc], docoff[doc], h, col, codes, residual, Cb, lut, qb,
+                                 clip_zero);
+      else
+        M = maxsim_doc_f32mfma<TPS>(doclens[doc], docoff[doc], h, col, codes, residual, C, wl, qf,
+                                    clip_zero);
+      if (h == 0) Ms[wave][col] = M;
+      wave_lds_sync();
+      if (lane == 0) {
+        float sc = 0.0f;
+        for (uint32_t tt = 0; tt < T; ++tt) sc += Ms[wave][tt];
+        qs[r] = sc;
+      }
+      wave_lds_sync();
+    }
+    __syncthreads();  // the query's scores are stored (same workgroup reads them)
+    block_topk_small(qs, in_ids + (size_t)q * R_in, n, kk, tk, out_ids + (size_t)q * out_stride,
+                     out_s + (size_t)q * out_stride, out_n ? out_n + q : nullptr);
   }
 }
 
@@ -1146,12 +1363,16 @@ __global__ __launch_bounds__(256) void k_count_work(const uint32_t *__restrict__
                                                     const uint32_t *__restrict__ sel_n, uint32_t R,
                                                     const uint32_t *__restrict__ doclens,
                                                     const uint32_t *__restrict__ cand_cnt,
+                                                    const uint32_t *__restrict__ q_ntok,
                                                     unsigned long long *__restrict__ counters) {
   const uint32_t q = blockIdx.x;
   const uint32_t n = sel_n[q];
   unsigned long long t = 0;
   for (uint32_t r = threadIdx.x; r < n; r += 256) t += doclens[sel_ids[(size_t)q * R + r]];
-  if (t) atomicAdd(&counters[0], t);
+  if (t) {
+    atomicAdd(&counters[0], t);
+    atomicAdd(&counters[2], t * q_ntok[q]);
+  }
   if (threadIdx.x == 0) atomicAdd(&counters[1], (unsigned long long)cand_cnt[q]);
 }
 
@@ -1262,10 +1483,10 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s) {
 }
 
 hipError_t launch_count_work(const uint32_t *sel_ids, const uint32_t *sel_n, uint32_t R, uint32_t n_q,
-                             const DevIndex &ix, const uint32_t *cand_cnt,
+                             const DevIndex &ix, const uint32_t *cand_cnt, const uint32_t *q_ntok,
                              unsigned long long *counters, hipStream_t s) {
   if (n_q == 0) return hipSuccess;
-  k_count_work<<<n_q, 256, 0, s>>>(sel_ids, sel_n, R, ix.doclens, cand_cnt, counters);
+  k_count_work<<<n_q, 256, 0, s>>>(sel_ids, sel_n, R, ix.doclens, cand_cnt, q_ntok, counters);
   return hipGetLastError();
 }
 
@@ -1375,6 +1596,30 @@ hipError_t launch_rerank(const float *Q, uint32_t t_max, const uint32_t *q_ntok,
   } else {
     return hipErrorNotSupported;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_persist(const float *Q, uint32_t t_max, const uint32_t *q_ntok,
+                          const uint32_t *in_ids, const uint32_t *in_n, uint32_t R_in, uint32_t nb,
+                          const DevIndex &ix, int precision, int clip_zero, const PersistCfg &cfg,
+                          uint32_t *queue, float *score, uint32_t kk, uint32_t *out_ids, float *out_s,
+                          uint32_t *out_n, uint32_t out_stride, hipStream_t s) {
+  if (nb == 0) return hipSuccess;
+  if (kk == 0 || kk > kTopkMax || cfg.n_wg == 0) return hipErrorInvalidValue;
+  if (precision == 1 && (!ix.C_bf16 || !ix.lut_bf16)) return hipErrorInvalidValue;
+  k_fill_u32<<<1, 64, 0, s>>>(queue, 0u, 1);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const uint32_t grid = std::min(cfg.n_wg, nb);
+#define MVR_PERSIST_ARGS                                                                           \
+  Q, t_max, q_ntok, in_ids, in_n, R_in, nb, queue, ix.doclens, ix.docoff, ix.codes, ix.residual, \
+      ix.C, ix.W, ix.C_bf16, ix.lut_bf16, clip_zero, score, kk, out_ids, out_s, out_n, out_stride
+  if (precision == 1 && cfg.tps == 1) k_rerank_persist<1, 1><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
+  else if (precision == 1) k_rerank_persist<1, 2><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
+  else if (precision == 3 && cfg.tps == 1) k_rerank_persist<3, 1><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
+  else if (precision == 3) k_rerank_persist<3, 2><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
+  else return hipErrorNotSupported;
+#undef MVR_PERSIST_ARGS
   return hipGetLastError();
 }
 

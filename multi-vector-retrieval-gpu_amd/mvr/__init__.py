@@ -126,7 +126,8 @@ class Index:
 
     # ---------------------------------------------------------------- search
     @staticmethod
-    def _params(nprobe, probe_topk, topk, precision, rescore, clip_zero, batch, learned=False):
+    def _params(nprobe, probe_topk, topk, precision, rescore, clip_zero, batch, learned=False,
+                persistent=False, graph=False):
         p = _capi.SearchParams()
         p.nprobe, p.probe_topk, p.topk = nprobe, probe_topk, topk
         p.precision = _PRECISION.get(precision, precision) if isinstance(precision, str) else precision
@@ -134,11 +135,26 @@ class Index:
         p.clip_zero = int(clip_zero)
         p.batch = batch
         p.learned = int(learned)
+        p.persistent = int(persistent)
+        p.graph = int(graph)
         return p
+
+    def set_tuning(self, precision, wgs_per_cu, tiles):
+        """Fix the work-queue MaxSim settings of a precision."""
+        prec = _PRECISION.get(precision, precision) if isinstance(precision, str) else precision
+        check(lib().mvr_index_set_tuning(self._h, prec, wgs_per_cu, tiles))
+
+    def get_tuning(self, precision):
+        """(workgroups per CU, tiles in flight) or None before tuning."""
+        prec = _PRECISION.get(precision, precision) if isinstance(precision, str) else precision
+        out = (C.c_uint32 * 2)()
+        if lib().mvr_index_get_tuning(self._h, prec, out) != 0:
+            return None
+        return tuple(int(v) for v in out)
 
     def search(self, Q, nprobe: int, probe_topk: int, topk: int, precision="fp32", rescore: int = 0,
                q_ntok=None, q_nprobe=None, clip_zero=False, batch: int = 0, stages=False,
-               timings=False, learned=False):
+               timings=False, learned=False, persistent=False, graph=False):
         """Host-buffer search.  Q: [n_q, t_max, 128] float32.  Returns
         (ids [n_q, topk] uint32, scores [n_q, topk] float32[, stages][, timings])."""
         Q = _c(Q, np.float32)
@@ -147,7 +163,8 @@ class Index:
         sc = np.empty((n_q, topk), np.float32)
         qt = None if q_ntok is None else _c(q_ntok, np.uint32)
         qp = None if q_nprobe is None else _c(q_nprobe, np.uint32)
-        prm = self._params(nprobe, probe_topk, topk, precision, rescore, clip_zero, batch, learned)
+        prm = self._params(nprobe, probe_topk, topk, precision, rescore, clip_zero, batch, learned,
+                           persistent, graph)
         st = None
         so = None
         if stages:
@@ -174,12 +191,13 @@ class Index:
 
     def search_device(self, Q, out_ids, out_scores, nprobe, probe_topk, topk, precision="fp32",
                       rescore=0, q_ntok=None, q_nprobe=None, clip_zero=False, batch=0, stream=0,
-                      learned=False):
+                      learned=False, persistent=False, graph=False):
         """Device-buffer search (torch tensors); queued on `stream`, no host sync."""
         n_q, t_max, _ = Q.shape
         qt = None if q_ntok is None else _c(q_ntok, np.uint32)
         qp = None if q_nprobe is None else _c(q_nprobe, np.uint32)
-        prm = self._params(nprobe, probe_topk, topk, precision, rescore, clip_zero, batch, learned)
+        prm = self._params(nprobe, probe_topk, topk, precision, rescore, clip_zero, batch, learned,
+                           persistent, graph)
         check(lib().mvr_search(self._h, Q.data_ptr(), n_q, t_max, _ptr(qt), _ptr(qp), C.byref(prm),
                                out_ids.data_ptr(), out_scores.data_ptr(), 1, stream or None, None, None))
 

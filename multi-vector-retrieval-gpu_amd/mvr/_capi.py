@@ -31,7 +31,8 @@ class IndexDesc(C.Structure):
 class SearchParams(C.Structure):
     _fields_ = [("nprobe", C.c_uint32), ("probe_topk", C.c_uint32), ("topk", C.c_uint32),
                 ("precision", C.c_uint32), ("rescore", C.c_uint32), ("clip_zero", C.c_uint32),
-                ("batch", C.c_uint32), ("learned", C.c_uint32)]
+                ("batch", C.c_uint32), ("learned", C.c_uint32), ("persistent", C.c_uint32),
+                ("graph", C.c_uint32)]
 
 
 class StageOut(C.Structure):
@@ -43,7 +44,7 @@ class Timings(C.Structure):
     _fields_ = [("centroid_ms", C.c_double), ("filter_ms", C.c_double), ("select_ms", C.c_double),
                 ("rerank_ms", C.c_double), ("rescore_ms", C.c_double), ("topk_ms", C.c_double),
                 ("total_ms", C.c_double), ("rerank_tokens", C.c_uint64), ("candidates", C.c_uint64),
-                ("n_pass", C.c_uint32)]
+                ("n_pass", C.c_uint32), ("rerank_pairs", C.c_uint64)]
 
 
 # Every symbol declared in include/mvr.h, with its ctypes signature.
@@ -59,6 +60,8 @@ SIGNATURES = {
                              C.POINTER(SearchParams), C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                              C.POINTER(StageOut), C.POINTER(Timings)]),
     "mvr_decompress": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
+    "mvr_index_set_tuning": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "mvr_index_get_tuning": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "mvr_rerank_docs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "mvr_exhaustive_f64": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,

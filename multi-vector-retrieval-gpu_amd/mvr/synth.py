@@ -205,6 +205,18 @@ def build_corpus(torch, seed, n_doc, n_comp, n_c, doclens, n_iter=10, kmeans_sam
                         residual, W, ivf)
 
 
+def query_shapes(seed, n_q, t_lo=8, t_hi=32, probes=(2, 4, 8, 16, 32)):
+    """Per-query token counts U{t_lo..t_hi} and nprobe uniform over `probes`
+    (the uneven batching workload)."""
+    idx = np.arange(n_q, dtype=np.uint64)
+    u = uniform_np(seed, STREAM_QLEN, idx)
+    nt = np.minimum(t_lo + np.floor(u * (t_hi - t_lo + 1)), t_hi).astype(np.uint32)
+    v = uniform_np(seed, STREAM_QPROBE, idx)
+    pr = np.asarray(probes, np.uint32)
+    npr = pr[np.minimum(np.floor(v * len(pr)).astype(np.int64), len(pr) - 1)]
+    return nt, npr
+
+
 def make_queries(torch, seed, centers, n_q, T=32, noise=0.15):
     Q = synth_tokens(torch, seed, STREAM_QCOMP, STREAM_QNOISE, centers, n_q * T, noise)
     return Q.view(n_q, T, 128)

@@ -1,0 +1,84 @@
+"""Batched execution on the GPU: the persistent work-queue MaxSim + top-k
+kernel, its tuning settings and hipGraph replay of device passes.  Every
+variant must return exactly what the per-query launch path returns (same
+MaxSim arithmetic, same (score desc, id asc) order), which the parity tests
+tie to the CPU oracle.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _index(ix):
+    import mvr
+    return mvr.Index.from_arrays(ix.doclens, ix.C, ix.codes, ix.residual, ix.weights, ix.ivf)
+
+
+def _ragged(Q, seed):
+    rng = np.random.default_rng(seed)
+    qt = rng.integers(8, Q.shape[1] + 1, Q.shape[0]).astype(np.uint32)
+    qp = rng.choice(np.array([2, 4, 8, 16, 32], np.uint32), Q.shape[0])
+    return qt, qp
+
+
+@pytest.mark.parametrize("prec,rescore", [("fp32_mfma", 0), ("bf16", 0), ("bf16", 64)])
+def test_persistent_matches_launch_path_every_setting(c1, prec, rescore):
+    corpus, ix, Q = c1
+    g = _index(ix)
+    qt, qp = _ragged(Q, 1)
+    ref_ids, ref_sc = g.search(Q, 8, 256, 10, precision=prec, rescore=rescore, q_ntok=qt, q_nprobe=qp,
+                               batch=24)
+    for wpc, tps in itertools.product((1, 2), (1, 2)):
+        g.set_tuning(prec, wpc, tps)
+        ids, sc = g.search(Q, 8, 256, 10, precision=prec, rescore=rescore, q_ntok=qt, q_nprobe=qp,
+                           batch=24, persistent=True)
+        np.testing.assert_array_equal(ids, ref_ids, err_msg=f"{prec} wpc={wpc} tps={tps}")
+        np.testing.assert_array_equal(sc, ref_sc, err_msg=f"{prec} wpc={wpc} tps={tps}")
+
+
+def test_autotuner_picks_a_valid_setting(small):
+    corpus, ix, Q = small
+    g = _index(ix)
+    assert g.get_tuning("bf16") is None
+    ref = g.search(Q, 4, 64, 10, precision="bf16")
+    got = g.search(Q, 4, 64, 10, precision="bf16", persistent=True)
+    wpc, tps = g.get_tuning("bf16")
+    assert wpc in (1, 2) and tps in (1, 2)
+    np.testing.assert_array_equal(got[0], ref[0])
+    np.testing.assert_array_equal(got[1], ref[1])
+
+
+@pytest.mark.parametrize("persistent", [False, True])
+def test_graph_replays_match_eager(c1, persistent):
+    """Each pass captured once and replayed: three rounds of replays with
+    eager searches and synchronisations in between, every replay checked."""
+    corpus, ix, Q = c1
+    g = _index(ix)
+    qt, qp = _ragged(Q, 2)
+    kw = dict(precision="bf16", rescore=64, q_ntok=qt, q_nprobe=qp, batch=16)
+    ref_ids, ref_sc = g.search(Q, 8, 256, 10, **kw)
+    for _ in range(3):
+        ids, sc = g.search(Q, 8, 256, 10, graph=True, persistent=persistent, **kw)
+        np.testing.assert_array_equal(ids, ref_ids)
+        np.testing.assert_array_equal(sc, ref_sc)
+        e_ids, e_sc = g.search(Q[:8], 4, 64, 10, precision="fp32_mfma")  # eager work in between
+        assert (e_ids != 0xFFFFFFFF).all()
+
+
+def test_graph_on_device_buffers(c1):
+    import torch
+    corpus, ix, Q = c1
+    g = _index(ix)
+    ref_ids, ref_sc = g.search(Q, 4, 256, 10, precision="fp32_mfma")
+    Qd = torch.from_numpy(np.ascontiguousarray(Q)).cuda()
+    for _ in range(3):
+        oi = torch.empty((Q.shape[0], 10), dtype=torch.int32, device="cuda")
+        os_ = torch.empty((Q.shape[0], 10), dtype=torch.float32, device="cuda")
+        g.search_device(Qd, oi, os_, 4, 256, 10, precision="fp32_mfma", batch=32, graph=True,
+                        persistent=True, stream=torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(oi.cpu().numpy().view(np.uint32), ref_ids)
+        np.testing.assert_array_equal(os_.cpu().numpy(), ref_sc)
