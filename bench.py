@@ -38,6 +38,13 @@ CONFIGS = {
                kmeans_sample=None, n_iter=10),
     # uneven batches on the C2 corpus: 16,384 queries, U{8..32} tokens, nprobe
     # uniform over {2, 4, 8, 16, 32}
+    # learned probe, as specified: config-1 corpus (mixture recipe), 262,144
+    # training queries labelled by recall@100 >= 0.99 against exhaustive MaxSim
+    "c5": dict(n_doc=8192, n_comp=1024, n_c=1024, lognormal=False, n_q=4096, P=4, R=256, k=100,
+               kmeans_sample=None, n_iter=10, learned=True, n_train=262144, target=0.99),
+    # learned probe on a topic-structured corpus where query difficulty varies
+    "c5t": dict(n_doc=8192, n_comp=1024, n_c=1024, lognormal=False, n_q=2048, P=4, R=2048, k=10,
+                kmeans_sample=None, n_iter=10, learned=True, n_train=65536, target=0.9, topic=True),
     "c4": dict(n_doc=1 << 20, n_comp=65536, n_c=65536, lognormal=True, n_q=16384, P=32, R=1000, k=100,
                kmeans_sample=1 << 21, n_iter=4, ragged=True),
 }
@@ -62,6 +69,7 @@ def main():
     ap.add_argument("--persistent", action="store_true",
                     help="MaxSim + top-k as one persistent work-queue kernel per pass")
     ap.add_argument("--graph", action="store_true", help="replay each device pass from a hipGraph")
+    ap.add_argument("--epochs", type=int, default=10, help="training epochs of the learned-probe configs")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -78,6 +86,9 @@ def main():
     from mvr import synth
 
     cfg = CONFIGS[args.config]
+    if cfg.get("learned"):
+        bench_learned(torch, args, cfg)
+        return
     t_setup = time.time()
     if cfg["lognormal"]:
         doclens = synth.doclens_lognormal(args.seed, cfg["n_doc"], 64.0, 0.29, 8, 300)
@@ -216,6 +227,40 @@ def main():
                      "alg_GBs": round(bytes_alg / (avg_launch_ms * 1e-3) / 1e9, 1) if avg_launch_ms > 0 else None},
         "cpu_baseline": cpu,
     }
+    print(json.dumps(line), flush=True)
+
+
+def bench_learned(torch, args, cfg):
+    """Learned probing against fixed nprobe (recall@k vs exhaustive MaxSim over
+    the index, queries/s, mean nprobe and rerank depth); one JSON line."""
+    from mvr import probe, synth
+    t0 = time.time()
+    doclens = synth.doclens_uniform(args.seed, cfg["n_doc"], 16, 128)
+    if cfg.get("topic"):
+        corpus = synth.topic_corpus(torch, args.seed, cfg["n_doc"], cfg["n_comp"], cfg["n_c"], doclens,
+                                    n_topics=128, comps_per_topic=16, p_topic=0.5, log=lambda m: log("[bench]", m))
+        kw = dict(noise_lo=0.05, noise_hi=0.6, p_lo=0.3, p_hi=1.0)
+        Qtr = synth.topic_queries(torch, corpus, args.seed + 1, cfg["n_train"], **kw)
+        Qev = synth.topic_queries(torch, corpus, args.seed + 2, cfg["n_q"], **kw)
+    else:
+        corpus = synth.build_corpus(torch, args.seed, cfg["n_doc"], cfg["n_comp"], cfg["n_c"], doclens,
+                                    n_iter=cfg["n_iter"], log=lambda m: log("[bench]", m))
+        Qtr = synth.make_queries(torch, args.seed + 1, corpus.centers, cfg["n_train"])
+        Qev = synth.make_queries(torch, args.seed + 2, corpus.centers, cfg["n_q"])
+    log(f"[bench] corpus and {cfg['n_train']} + {cfg['n_q']} queries in {time.time() - t0:.1f}s")
+    out = probe.learned_vs_fixed(torch, corpus.index, Qtr, Qev, cfg["k"], cfg["R"], cfg["n_doc"],
+                                 target=cfg["target"], epochs=args.epochs, log=lambda m: log("[bench]", m))
+    out.pop("model")
+    learned = out["eval_full_depth"][-1]
+    line = {"metric": f"learned-probe recall@{cfg['k']} and queries/s against fixed nprobe",
+            "value": round(learned["qps"], 1), "unit": "queries/s", "n_gpus": 1, "higher_is_better": True,
+            "dtype": "bf16", "data": "synthetic" + (" topic-structured" if cfg.get("topic") else " mixture"),
+            "config": {"workload": f"{args.config}: {cfg['n_doc']} docs, {cfg['n_c']} centroids, "
+                                   f"{cfg['n_train']} training / {cfg['n_q']} evaluation queries, "
+                                   f"labels: smallest nprobe with recall@{cfg['k']} >= {cfg['target']} at "
+                                   f"rerank depth {cfg['R']}; MLP 1024-512-256-6, Adam lr 1e-3, batch 1024, "
+                                   f"{args.epochs} epochs, fallback nprobe 16 below softmax 0.6"},
+            **out}
     print(json.dumps(line), flush=True)
 
 

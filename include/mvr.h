@@ -95,6 +95,8 @@ typedef struct {
   uint32_t *sel_ids;   /* [n_q][probe_topk] filter top docs, best first */
   float *sel_F;        /* [n_q][probe_topk] their filter scores */
   float *sel_score;    /* [n_q][probe_topk] their MaxSim scores (final precision) */
+  uint32_t *nprobe;    /* [n_q] nprobe used (the model's choice with learned probing) */
+  uint32_t *rerank_n;  /* [n_q] docs reranked (min(candidates, per-query probe_topk)) */
 } mvr_stage_out;
 
 /* Per-call device time per stage in milliseconds (HIP events on the call's

@@ -775,6 +775,11 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
         HIP_TRY(hipMemcpy(stages->sel_F + (size_t)b0 * R, ws.sel_F.p, (size_t)nb * R * 4, hipMemcpyDeviceToHost));
       if (stages->sel_score)
         HIP_TRY(hipMemcpy(stages->sel_score + (size_t)b0 * R, ws.sel_score.p, (size_t)nb * R * 4, hipMemcpyDeviceToHost));
+      if (stages->nprobe)
+        HIP_TRY(hipMemcpy(stages->nprobe + b0, learned ? ws.dyn_np.p : (const void *)(d_np + b0), (size_t)nb * 4,
+                          hipMemcpyDeviceToHost));
+      if (stages->rerank_n)
+        HIP_TRY(hipMemcpy(stages->rerank_n + b0, ws.sel_n.p, (size_t)nb * 4, hipMemcpyDeviceToHost));
     }
     if (timings)
       HIP_TRY(launch_count_work(ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R, nb, d,

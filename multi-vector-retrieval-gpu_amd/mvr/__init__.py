@@ -175,9 +175,12 @@ class Index:
                   "cand_n": np.empty(n_q, np.uint32),
                   "sel_ids": np.empty((n_q, probe_topk), np.uint32),
                   "sel_F": np.empty((n_q, probe_topk), np.float32),
-                  "sel_score": np.empty((n_q, probe_topk), np.float32)}
+                  "sel_score": np.empty((n_q, probe_topk), np.float32),
+                  "nprobe": np.empty(n_q, np.uint32),
+                  "rerank_n": np.empty(n_q, np.uint32)}
             so = _capi.StageOut(*[st[k].ctypes.data for k in
-                                  ("top_ids", "top_s", "cand_n", "sel_ids", "sel_F", "sel_score")])
+                                  ("top_ids", "top_s", "cand_n", "sel_ids", "sel_F", "sel_score", "nprobe",
+                                   "rerank_n")])
         tm = _capi.Timings() if timings else None
         check(lib().mvr_search(self._h, _ptr(Q), n_q, t_max, _ptr(qt), _ptr(qp), C.byref(prm),
                                _ptr(ids), _ptr(sc), 0, None, C.byref(so) if so else None,

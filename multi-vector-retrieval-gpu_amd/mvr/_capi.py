@@ -37,7 +37,8 @@ class SearchParams(C.Structure):
 
 class StageOut(C.Structure):
     _fields_ = [("top_ids", C.c_void_p), ("top_s", C.c_void_p), ("cand_n", C.c_void_p),
-                ("sel_ids", C.c_void_p), ("sel_F", C.c_void_p), ("sel_score", C.c_void_p)]
+                ("sel_ids", C.c_void_p), ("sel_F", C.c_void_p), ("sel_score", C.c_void_p),
+                ("nprobe", C.c_void_p), ("rerank_n", C.c_void_p)]
 
 
 class Timings(C.Structure):

@@ -206,3 +206,63 @@ def train(torch, model, feats, cls, logr, epochs=8, batch=1024, lr=1e-3, seed=0,
         if log:
             log(f"probe epoch {ep}: loss {losses[-1]:.4f}")
     return losses
+
+
+def evaluate(torch, index, Q, k, R, gnd, nprobes=CLASSES, learned=False, precision="bf16", rescore=0,
+             batch=1024, reps=3):
+    """recall@k against `gnd` [n_q, k] and queries/s of fixed-nprobe searches
+    (and of learned probing when a model is attached).  Returns a list of
+    dicts: mode, nprobe (mean), rerank (mean docs reranked), recall, qps."""
+    import time as _time
+    n_q = Q.shape[0]
+    Qh = Q.cpu().numpy()
+    dev = Q.device
+    oi = torch.empty((n_q, k), dtype=torch.int32, device=dev)
+    os_ = torch.empty((n_q, k), dtype=torch.float32, device=dev)
+    stream = torch.cuda.current_stream().cuda_stream
+    modes = [("fixed", P) for P in nprobes] + ([("learned", None)] if learned else [])
+    out = []
+    for mode, P in modes:
+        lp = mode == "learned"
+        nprobe = 32 if lp else P
+        ids, _, st = index.search(Qh, nprobe, R, k, precision=precision, rescore=rescore, batch=batch,
+                                  stages=True, learned=lp)
+        rec = float(np.mean([len(set(a.tolist()) & set(b.tolist())) / k for a, b in zip(ids, gnd)]))
+        best = None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = _time.perf_counter()
+            index.search_device(Q, oi, os_, nprobe, R, k, precision=precision, rescore=rescore, batch=batch,
+                                stream=stream, learned=lp)
+            torch.cuda.synchronize()
+            dt = _time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        out.append({"mode": mode, "nprobe": float(st["nprobe"].mean()), "rerank": float(st["rerank_n"].mean()),
+                    "recall": rec, "qps": n_q / best})
+    return out
+
+
+def learned_vs_fixed(torch, index, Q_train, Q_eval, k, R_max, n_doc, target=0.9, epochs=20, batch=1024,
+                     lr=1e-3, conf=0.6, fallback=16, r_margin=0.0, seed=1, log=None):
+    """The learned-probe experiment end to end: labels from exhaustive MaxSim
+    over the index for the training queries, Adam training on the device,
+    then recall@k (against exhaustive MaxSim) of learned probing next to fixed
+    nprobe at the full rerank depth R_max and at the learned mean depth.
+    r_margin is added to the log-R labels (a safety factor on the cutoff)."""
+    feats, cls, logr, rec, _ = make_labels(torch, index, Q_train, k, R_max, n_doc, target=target, log=log)
+    model = ProbeModel(seed=seed)
+    losses = train(torch, model, feats, cls, logr + r_margin, epochs=epochs, batch=batch, lr=lr, log=log)
+    _, _, _, _, gnd = make_labels(torch, index, Q_eval, k, R_max, n_doc, target=target)
+    model.attach(index, conf=conf, fallback=fallback, r_min=k)
+    res = evaluate(torch, index, Q_eval, k, R_max, gnd, learned=True)
+    learned = res[-1]
+    r_eq = max(k, int(round(learned["rerank"])))
+    res_eq = evaluate(torch, index, Q_eval, k, r_eq, gnd, learned=False)
+    index_detach = getattr(index, "_h", None)
+    if index_detach is not None:
+        check(lib().mvr_index_set_probe(index._h, None, conf, fallback, 0))
+    return {"class_counts": np.bincount(cls, minlength=len(CLASSES)).tolist(),
+            "train_recall_by_class": rec.mean(axis=0).round(4).tolist(),
+            "losses": [round(float(l), 4) for l in losses],
+            "eval_full_depth": res, "rerank_depth_of_learned": r_eq, "eval_same_depth": res_eq,
+            "model": model}

@@ -161,17 +161,11 @@ class DeviceCorpus:
     ivf: tuple
 
 
-def build_corpus(torch, seed, n_doc, n_comp, n_c, doclens, n_iter=10, kmeans_sample=None,
-                 keep_embeddings=True, chunk=1 << 22, log=print):
-    dev = torch.device("cuda")
-    n_tok = int(doclens.sum())
-    centers = gen_centers(torch, seed, n_comp)
-    X = torch.empty((n_tok, 128), dtype=torch.float32, device=dev)
-    for t0 in range(0, n_tok, chunk):
-        n = min(chunk, n_tok - t0)
-        synth_tokens(torch, seed, STREAM_TOKCOMP, STREAM_TOKNOISE, centers, n, tok_begin=t0, out=X[t0:t0 + n])
-    torch.cuda.synchronize()
-    log(f"synth: {n_tok} tokens")
+def index_from_tokens(torch, X, doclens, n_c, n_iter=10, kmeans_sample=None, chunk=1 << 22, log=print):
+    """IGP index of token embeddings X [n_tok, 128] (device): k-means on a
+    sample, codes, per-dimension 2-bit codec, residuals and IVF."""
+    dev = X.device
+    n_tok = X.shape[0]
     if kmeans_sample and kmeans_sample < n_tok:
         sel = (torch.arange(kmeans_sample, device=dev, dtype=torch.int64) * n_tok) // kmeans_sample
         sample = X[sel].contiguous()
@@ -201,6 +195,22 @@ def build_corpus(torch, seed, n_doc, n_comp, n_c, doclens, n_iter=10, kmeans_sam
     torch.cuda.synchronize()
     log(f"codec + ivf: {ivf[2].numel()} IVF entries")
     index = Index.from_device(dl, C_, codes, residual, W, ivf)
+    return index, C_, codes, residual, W, ivf
+
+
+def build_corpus(torch, seed, n_doc, n_comp, n_c, doclens, n_iter=10, kmeans_sample=None,
+                 keep_embeddings=True, chunk=1 << 22, log=print):
+    dev = torch.device("cuda")
+    n_tok = int(doclens.sum())
+    centers = gen_centers(torch, seed, n_comp)
+    X = torch.empty((n_tok, 128), dtype=torch.float32, device=dev)
+    for t0 in range(0, n_tok, chunk):
+        n = min(chunk, n_tok - t0)
+        synth_tokens(torch, seed, STREAM_TOKCOMP, STREAM_TOKNOISE, centers, n, tok_begin=t0, out=X[t0:t0 + n])
+    torch.cuda.synchronize()
+    log(f"synth: {n_tok} tokens")
+    index, C_, codes, residual, W, ivf = index_from_tokens(torch, X, doclens, n_c, n_iter, kmeans_sample,
+                                                           chunk, log)
     return DeviceCorpus(seed, doclens, centers, X if keep_embeddings else None, index, C_, codes,
                         residual, W, ivf)
 
@@ -215,6 +225,59 @@ def query_shapes(seed, n_q, t_lo=8, t_hi=32, probes=(2, 4, 8, 16, 32)):
     pr = np.asarray(probes, np.uint32)
     npr = pr[np.minimum(np.floor(v * len(pr)).astype(np.int64), len(pr) - 1)]
     return nt, npr
+
+
+def topic_corpus(torch, seed, n_doc, n_comp, n_c, doclens, n_topics=128, comps_per_topic=16, noise=0.15,
+                 p_topic=1.0, n_iter=10, log=print):
+    """A corpus with topical structure (for the learned-probe evaluation, where
+    query difficulty has to vary): topic t owns `comps_per_topic` mixture
+    components drawn at random (topics overlap); a doc picks a topic and each
+    of its tokens one of the topic's components (with probability p_topic,
+    else any component), plus N(0, noise^2) per dim, L2-normalised.  Seeded
+    torch generator on the device."""
+    dev = torch.device("cuda")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    centers = gen_centers(torch, seed, n_comp)
+    topics = torch.randint(0, n_comp, (n_topics, comps_per_topic), generator=gen, device=dev)
+    dl = torch.from_numpy(doclens.astype(np.int64)).to(dev)
+    doc_topic = torch.randint(0, n_topics, (n_doc,), generator=gen, device=dev)
+    tok_topic = torch.repeat_interleave(doc_topic, dl)
+    n_tok = tok_topic.numel()
+    pick = torch.randint(0, comps_per_topic, (n_tok,), generator=gen, device=dev)
+    comp = topics[tok_topic, pick]
+    if p_topic < 1.0:
+        off = torch.rand((n_tok,), generator=gen, device=dev) >= p_topic
+        comp = torch.where(off, torch.randint(0, n_comp, (n_tok,), generator=gen, device=dev), comp)
+    X = centers[comp] + noise * torch.randn((n_tok, 128), generator=gen, device=dev)
+    X = torch.nn.functional.normalize(X, dim=1).contiguous()
+    log(f"topic corpus: {n_tok} tokens, {n_topics} topics x {comps_per_topic} components")
+    index, C_, codes, residual, W, ivf = index_from_tokens(torch, X, doclens, n_c, n_iter, None, 1 << 22, log)
+    corpus = DeviceCorpus(seed, doclens, centers, X, index, C_, codes, residual, W, ivf)
+    corpus.topics = topics
+    corpus.doc_topic = doc_topic
+    return corpus
+
+
+def topic_queries(torch, corpus, seed, n_q, T=32, noise_lo=0.05, noise_hi=0.4, p_lo=1.0, p_hi=1.0):
+    """Queries of the topic corpus: a topic, T tokens from its components and
+    a per-query noise level uniform in [noise_lo, noise_hi]; each token comes
+    from the topic with a per-query probability uniform in [p_lo, p_hi], else
+    from any component.  Noisier, less topical queries have flatter
+    centroid-score profiles and need more probes."""
+    dev = torch.device("cuda")
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed)
+    n_topics, cpt = corpus.topics.shape
+    qt = torch.randint(0, n_topics, (n_q,), generator=gen, device=dev)
+    pick = torch.randint(0, cpt, (n_q, T), generator=gen, device=dev)
+    comp = corpus.topics[qt[:, None], pick]
+    pq = p_lo + (p_hi - p_lo) * torch.rand((n_q, 1), generator=gen, device=dev)
+    off = torch.rand((n_q, T), generator=gen, device=dev) >= pq
+    comp = torch.where(off, torch.randint(0, corpus.centers.shape[0], (n_q, T), generator=gen, device=dev), comp)
+    sig = noise_lo + (noise_hi - noise_lo) * torch.rand((n_q, 1, 1), generator=gen, device=dev)
+    Q = corpus.centers[comp] + sig * torch.randn((n_q, T, 128), generator=gen, device=dev)
+    return torch.nn.functional.normalize(Q, dim=2).contiguous()
 
 
 def make_queries(torch, seed, centers, n_q, T=32, noise=0.15):
