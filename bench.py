@@ -172,6 +172,14 @@ def main():
     torch.cuda.synchronize()
     lat = np.array([a.elapsed_time(b) for a, b in ev])
 
+    # the same search through host buffers (PCIe copies of queries and results
+    # included), as the reference's executables use it
+    Qh = Q.cpu().numpy()
+    t_h = time.perf_counter()
+    ix.search(Qh, P, R, k, args.precision, rescore, q_ntok=q_nt, q_nprobe=q_np, batch=batch,
+              persistent=args.persistent, graph=args.graph)
+    qps_host = n_q / (time.perf_counter() - t_h)
+
     # roofline of the MaxSim kernel: algorithmic flops / bytes per launch over
     # its average launch duration in the timed region
     # (query tokens x doc tokens) pairs: 2 x 128 flops each
@@ -211,6 +219,7 @@ def main():
                    "persistent": args.persistent, "graph": args.graph,
                    "tuning": (ix.get_tuning(args.precision) if args.persistent else None),
                    "batch_per_pass": batch, "precision": args.precision, "rescore": rescore},
+        "qps_host_buffers": round(qps_host, 1),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3),
                        "unit": f"per {batch}-query device pass"},
