@@ -226,14 +226,8 @@ def main():
         "recall_at_k": recall.get("search_vs_fp64") if recall else None,
         "recall": recall,
         "stages_ms_per_step": {f: round(stages[f], 3) for f in stage_keys},
-        "roofline": {"kernel": kname, "bound": "mfma",
-                     "achieved": None if achieved_tf is None else round(achieved_tf, 3), "peak": peak,
-                     "unit": "TFLOP/s", "frac": None if achieved_tf is None else round(achieved_tf / peak, 4),
-                     "traffic": traffic, "avg_launch_ms": round(avg_launch_ms, 4), "launches": n_launch,
-                     "doc_tokens_per_launch": int(tok_per_launch),
-                     "flop_per_launch": flops, "flop_per_doc_token_and_query_token": 2 * 128,
-                     "alg_bytes_per_doc_token": 36,
-                     "alg_GBs": round(bytes_alg / (avg_launch_ms * 1e-3) / 1e9, 1) if avg_launch_ms > 0 else None},
+        "roofline": roofline(kname, achieved_tf, peak, bytes_alg, avg_launch_ms, n_launch, tok_per_launch,
+                             flops, traffic),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
@@ -271,6 +265,26 @@ def bench_learned(torch, args, cfg):
                                    f"{args.epochs} epochs, fallback nprobe 16 below softmax 0.6"},
             **out}
     print(json.dumps(line), flush=True)
+
+
+def roofline(kname, achieved_tf, peak_tf, bytes_alg, avg_launch_ms, n_launch, tok_per_launch, flops, traffic):
+    """Both roofs of the MaxSim kernel -- matrix-unit FLOP/s and HBM bytes/s
+    (algorithmic: 36 B of codes per doc token) -- with the binding one (the
+    larger fraction) as the headline."""
+    if achieved_tf is None:
+        return {"kernel": kname, "bound": None, "achieved": None, "peak": None, "unit": None, "frac": None,
+                "traffic": traffic, "note": "graph replay: no per-kernel timing"}
+    gbs = bytes_alg / (avg_launch_ms * 1e-3) / 1e9
+    mfma = {"bound": "mfma", "achieved": round(achieved_tf, 3), "peak": peak_tf, "unit": "TFLOP/s",
+            "frac": round(achieved_tf / peak_tf, 4)}
+    hbm = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    main, other = (mfma, hbm) if mfma["frac"] >= hbm["frac"] else (hbm, mfma)
+    return {"kernel": kname, **main, "traffic": traffic, "other_roof": other,
+            "avg_launch_ms": round(avg_launch_ms, 4), "launches": n_launch,
+            "doc_tokens_per_launch": int(tok_per_launch), "flop_per_launch": flops,
+            "flop_per_doc_token_and_query_token": 2 * 128, "alg_bytes_per_doc_token": 36,
+            "measured_GBs": round(traffic / (avg_launch_ms * 1e-3) / 1e9, 1) if traffic else None}
 
 
 def measured_traffic(kname, config, precision):
