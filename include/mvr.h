@@ -44,8 +44,11 @@ typedef enum {
 /* MVR_FP32: MaxSim as sequential FP32 FMA chains, bit-identical to the CPU
  * restatement of the reference.  MVR_FP32_MFMA: the same FP32 decompressed
  * embeddings and FP32 queries on v_mfma_f32_32x32x2_f32 (sums in the matrix
- * unit's order; within 1e-5 relative of MVR_FP32).  MVR_BF16 / MVR_FP8: low
- * precision centroid scoring and MaxSim, optionally rescored in FP32. */
+ * unit's order; within 1e-5 relative of MVR_FP32).  MVR_BF16: centroid scoring
+ * (exact on bf16-rounded inputs) and MaxSim on BF16 MFMA.  MVR_FP8: MaxSim on
+ * FP8 (OCP e4m3) MFMA with per-centroid, per-query-token and residual scales;
+ * centroid top-P exact in FP32.  Low-precision modes optionally rescore their
+ * best `rescore` docs in FP32. */
 typedef enum { MVR_FP32 = 0, MVR_BF16 = 1, MVR_FP8 = 2, MVR_FP32_MFMA = 3 } mvr_precision;
 
 typedef struct mvr_index mvr_index;
@@ -138,8 +141,8 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
 /* MaxSim of given candidates (the rerank stage alone, RerankPP.hpp:368-475):
  * scores_dev[q][r] = sum_t max_i <Q[q][t], x_i> over the decompressed tokens of
  * doc_ids_dev[q][r] for r < n_docs[q] (host array; NULL: R for every query),
- * in `precision` (MVR_FP32 bit-exact with mvr_search, MVR_FP32_MFMA and
- * MVR_BF16 on MFMA).
+ * in `precision` (MVR_FP32 bit-exact with mvr_search; MVR_FP32_MFMA,
+ * MVR_BF16 and MVR_FP8 on MFMA).
  * Device buffers Q [n_q][t_max][128], doc_ids [n_q][R], scores [n_q][R];
  * q_ntok is a host array (NULL: t_max).  Queued on `stream`. */
 int mvr_rerank_docs(mvr_index *ix, const float *Q_dev, uint32_t n_q, uint32_t t_max,

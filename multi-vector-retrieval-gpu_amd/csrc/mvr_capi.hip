@@ -169,7 +169,7 @@ void free_devindex(DevIndex &d) {
   for (void *p : {(void *)d.C, (void *)d.C_bf16, (void *)d.C_fp8, (void *)d.C_fp8_scale,
                   (void *)d.codes, (void *)d.residual, (void *)d.W, (void *)d.doclens,
                   (void *)d.docoff, (void *)d.ivf_off, (void *)d.ivf_size, (void *)d.ivf_ids,
-                  (void *)d.seg_off, (void *)d.lut_bf16})
+                  (void *)d.seg_off, (void *)d.lut_bf16, (void *)d.lut_fp8})
     if (p) (void)hipFree(p);
   d = DevIndex();
 }
@@ -308,6 +308,14 @@ int mvr_index_create(const mvr_index_desc *desc, int device, mvr_index **out) {
     e = upload(reinterpret_cast<uint32_t **>(&d.lut_bf16), lut.data(), lut.size(), false);
   if (e == hipSuccess)
     e = hipMalloc(reinterpret_cast<void **>(&d.C_bf16), (size_t)d.n_c * kDim * sizeof(uint16_t));
+  {
+    float wmax = 0.0f;
+    for (float v : W) wmax = std::max(wmax, std::fabs(v));
+    d.fp8_res_scale = wmax / 448.0f;
+  }
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d.C_fp8), (size_t)d.n_c * kDim);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d.C_fp8_scale), (size_t)d.n_c * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d.lut_fp8), (size_t)kResBytes * 256 * 4);
   if (e == hipSuccess)
     e = hipMalloc(reinterpret_cast<void **>(&d.seg_off), (size_t)d.n_c * (d.n_seg + 1) * sizeof(uint32_t));
   if (e == hipSuccess) e = launch_seg_offsets(d, nullptr);
@@ -418,8 +426,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   if (R < k) return fail(MVR_ERR_INVALID, "probe_topk %u is smaller than topk %u", R, k);
   if (k == 0) return fail(MVR_ERR_INVALID, "topk must be positive");
   if (R > kMaxSelect) return fail(MVR_ERR_UNSUPPORTED, "probe_topk %u > %u", R, kMaxSelect);
-  if (prm->precision != MVR_FP32 && prm->precision != MVR_BF16 && prm->precision != MVR_FP32_MFMA)
-    return fail(MVR_ERR_UNSUPPORTED, "precision %u not built", prm->precision);
+  if (prm->precision > MVR_FP32_MFMA) return fail(MVR_ERR_INVALID, "unknown precision %u", prm->precision);
   const bool fp32_mode = prm->precision == MVR_FP32 || prm->precision == MVR_FP32_MFMA;
   // FP32 centroid top-P: BF16 MFMA tile maxima + exact FP32 rescan (same
   // result as the all-FP32 scan); MVR_CT_SCAN=1 forces the all-FP32 scan
@@ -527,8 +534,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     HIP_TRY(ws.res_n.ensure(Bc * 4));
   }
   // persistent work-queue MaxSim: BF16 / FP32-MFMA, top-k (and rescore) <= 256
-  const bool persistent = prm->persistent &&
-                          (prm->precision == MVR_BF16 || prm->precision == MVR_FP32_MFMA) &&
+  const bool persistent = prm->persistent && prm->precision != MVR_FP32 &&
                           k <= kPersistTopkMax && R2 <= kPersistTopkMax;
   // graph replay: not with learned probing (the model's launches) or stage outputs
   const bool use_graph = prm->graph && !learned && !stages;
@@ -573,7 +579,8 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
                                    ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), nullptr, st));
     else
       HIP_TRY(launch_centroid_topk_bf16(Qb, nb, t_max, rm, n_rows, d, P, ws.bmax.as<float>(),
-                                        ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), fp32_mode, st));
+                                        ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(),
+                                        fp32_mode || prm->precision == MVR_FP8, st));
     HIP_TRY(mark(1));
     const uint32_t *q_r = nullptr;
     if (learned) {
@@ -814,7 +821,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
 
 int mvr_index_set_tuning(mvr_index *ix, uint32_t precision, uint32_t wgs_per_cu, uint32_t tiles) {
   if (!ix) return fail(MVR_ERR_INVALID, "NULL index");
-  if (precision != MVR_BF16 && precision != MVR_FP32_MFMA)
+  if (precision != MVR_BF16 && precision != MVR_FP8 && precision != MVR_FP32_MFMA)
     return fail(MVR_ERR_UNSUPPORTED, "no work-queue kernel for precision %u", precision);
   if (wgs_per_cu < 1 || wgs_per_cu > 2) return fail(MVR_ERR_INVALID, "wgs_per_cu %u not in {1,2}", wgs_per_cu);
   if (tiles < 1 || tiles > 2) return fail(MVR_ERR_INVALID, "tiles %u not in {1,2}", tiles);
@@ -840,8 +847,7 @@ int mvr_rerank_docs(mvr_index *ix, const float *Q_dev, uint32_t n_q, uint32_t t_
   if (!ix || !Q_dev || !doc_ids_dev || !scores_dev) return fail(MVR_ERR_INVALID, "NULL argument");
   if (n_q == 0 || R == 0) return MVR_OK;
   if (t_max == 0 || t_max > kMaxQTok) return fail(MVR_ERR_INVALID, "t_max %u not in [1, 32]", t_max);
-  if (precision != MVR_FP32 && precision != MVR_BF16 && precision != MVR_FP32_MFMA)
-    return fail(MVR_ERR_UNSUPPORTED, "precision %u not built", precision);
+  if (precision > MVR_FP32_MFMA) return fail(MVR_ERR_INVALID, "unknown precision %u", precision);
   DeviceGuard g(ix->device);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   Workspace &ws = ix->ws;

@@ -37,6 +37,8 @@ struct DevIndex {
   uint32_t *ivf_ids = nullptr; // [n_ele]
   uint32_t *seg_off = nullptr; // [n_c][n_seg+1] list index where each segment starts
   uint2 *lut_bf16 = nullptr;   // [32][256]: 4 bf16 residual values of one packed byte per position
+  uint32_t *lut_fp8 = nullptr; // [32][256]: 4 e4m3 residual values / fp8_res_scale
+  float fp8_res_scale = 0.0f;  // max |W| / 448
 };
 
 // ---- launchers (return hipError_t of the launch) ----

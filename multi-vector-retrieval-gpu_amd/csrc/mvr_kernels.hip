@@ -900,6 +900,171 @@ __global__ __launch_bounds__(256) void k_rerank_f32mfma(
 }
 
 // ---------------------------------------------------------------------------
+// Fused decompress + MaxSim on FP8 MFMA (OCP e4m3 inputs, FP32 accumulation),
+// v_mfma_f32_32x32x16_fp8_fp8 in the BF16 kernel's tile scheme.  Scales:
+// per centroid row (s_c = max|C_c| / 448, stored with the fp8 rows), per query
+// token (s_q = max|q_t| / 448, computed when the query is loaded) and one for
+// the residual weights (s_r).  x = C[code] + r is never formed: two
+// accumulators hold C8.q8 and R8.q8, combined per output as
+// s_q (s_c acc_C + s_r acc_R), which applies each scale to whole rows or
+// columns of the products.  A centroid row costs 64 bytes per lane pair
+// (128 B per token) instead of 256 B in BF16.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t fp8x2(float a, float b) {
+  return (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(a, b, 0, false) & 0xFFFFu;
+}
+
+// Query tokens as the fp8 B operand (lane (col, h): dims 64h + 8s .. +7 in
+// qf8[s]) and the token's scale.
+__device__ __forceinline__ void load_q_fp8(const float *__restrict__ Q, uint32_t q, uint32_t t_max,
+                                           uint32_t T, uint32_t h, uint32_t col, long (&qf8)[8],
+                                           float &sq) {
+  const float4 *qr = reinterpret_cast<const float4 *>(Q + ((size_t)q * t_max + col) * kDim + 64 * h);
+  float v[64];
+  float amax = 0.0f;
+#pragma unroll
+  for (uint32_t i = 0; i < 16; ++i) {
+    const float4 x = (col < T) ? qr[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    v[4 * i] = x.x; v[4 * i + 1] = x.y; v[4 * i + 2] = x.z; v[4 * i + 3] = x.w;
+    amax = fmaxf(amax, fmaxf(fmaxf(fabsf(x.x), fabsf(x.y)), fmaxf(fabsf(x.z), fabsf(x.w))));
+  }
+  amax = fmaxf(amax, __shfl_xor(amax, 32));
+  sq = amax / 448.0f;
+  const float inv = amax > 0.0f ? 448.0f / amax : 0.0f;
+#pragma unroll
+  for (uint32_t s = 0; s < 8; ++s) {
+    const uint32_t lo = fp8x2(v[8 * s] * inv, v[8 * s + 1] * inv) | (fp8x2(v[8 * s + 2] * inv, v[8 * s + 3] * inv) << 16);
+    const uint32_t hi = fp8x2(v[8 * s + 4] * inv, v[8 * s + 5] * inv) | (fp8x2(v[8 * s + 6] * inv, v[8 * s + 7] * inv) << 16);
+    qf8[s] = (long)(((uint64_t)hi << 32) | lo);
+  }
+}
+
+// Per-query-token max over one doc on FP8 tiles; rsc = this wave's LDS row
+// scales [TPS][32].
+template <int TPS>
+__device__ __forceinline__ float maxsim_doc_fp8(uint32_t L, uint64_t off, uint32_t h, uint32_t col,
+                                                const uint32_t *__restrict__ codes,
+                                                const uint8_t *__restrict__ residual,
+                                                const uint8_t *__restrict__ C8,
+                                                const float *__restrict__ csc, const uint32_t *lut8,
+                                                float sr, const long (&qf8)[8], float sq, float *rsc,
+                                                int clip_zero) {
+  float M = clip_zero ? 0.0f : -FLT_MAX;
+  for (uint32_t i0 = 0; i0 < L; i0 += 32 * TPS) {
+    uint4 cv[TPS][4];
+    uint4 rb[TPS];
+#pragma unroll
+    for (int u = 0; u < TPS; ++u) {
+      const uint32_t i = i0 + 32 * u + col;
+      rb[u] = make_uint4(0u, 0u, 0u, 0u);
+      float sc = 0.0f;
+      if (i < L) {
+        const uint64_t g = off + i;
+        const uint32_t code = codes[g];
+        rb[u] = *reinterpret_cast<const uint4 *>(residual + g * kResBytes + 16 * h);
+        const uint4 *cp = reinterpret_cast<const uint4 *>(C8 + (size_t)code * kDim + 64 * h);
+#pragma unroll
+        for (uint32_t s = 0; s < 4; ++s) cv[u][s] = cp[s];
+        sc = csc[code];
+      } else {
+#pragma unroll
+        for (uint32_t s = 0; s < 4; ++s) cv[u][s] = make_uint4(0u, 0u, 0u, 0u);
+      }
+      if (h == 0) rsc[32 * u + col] = sc;
+    }
+    floatx16_t accC[TPS], accR[TPS];
+#pragma unroll
+    for (int u = 0; u < TPS; ++u) { accC[u] = floatx16_t{}; accR[u] = floatx16_t{}; }
+#pragma unroll
+    for (uint32_t s = 0; s < 8; ++s) {
+#pragma unroll
+      for (int u = 0; u < TPS; ++u) {
+        const uint4 c4 = cv[u][s >> 1];
+        const long a = (long)((s & 1) ? (((uint64_t)c4.w << 32) | c4.z) : (((uint64_t)c4.y << 32) | c4.x));
+        accC[u] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a, qf8[s], accC[u], 0, 0, 0);
+        const uint32_t rw[4] = {rb[u].x, rb[u].y, rb[u].z, rb[u].w};
+        const uint32_t word = rw[s >> 1];
+        const uint32_t b0 = (word >> (16 * (s & 1))) & 0xFFu;
+        const uint32_t b1 = (word >> (16 * (s & 1) + 8)) & 0xFFu;
+        const uint32_t pos = 16 * h + 2 * s;
+        const long r = (long)(((uint64_t)lut8[(pos + 1) * 256 + b1] << 32) | lut8[pos * 256 + b0]);
+        accR[u] = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(r, qf8[s], accR[u], 0, 0, 0);
+      }
+    }
+    wave_lds_sync();
+#pragma unroll
+    for (int u = 0; u < TPS; ++u)
+#pragma unroll
+      for (uint32_t j = 0; j < 16; ++j) {
+        const uint32_t m = 8 * (j >> 2) + 4 * h + (j & 3);
+        if (i0 + 32 * u + m < L) M = fmaxf(M, sq * (rsc[32 * u + m] * accC[u][j] + sr * accR[u][j]));
+      }
+    wave_lds_sync();
+  }
+  return fmaxf(M, __shfl_xor(M, 32));
+}
+
+__global__ __launch_bounds__(256) void k_rerank_fp8(
+    const float *__restrict__ Q, uint32_t t_max, const uint32_t *__restrict__ q_ntok,
+    const uint32_t *__restrict__ sel_ids, const uint32_t *__restrict__ sel_n, uint32_t R,
+    uint32_t n_slice, const uint32_t *__restrict__ doclens, const uint64_t *__restrict__ docoff,
+    const uint32_t *__restrict__ codes, const uint8_t *__restrict__ residual,
+    const uint8_t *__restrict__ C8, const float *__restrict__ csc, const uint32_t *__restrict__ lut_g,
+    float sr, int clip_zero, float *__restrict__ sel_score) {
+  __shared__ uint32_t lut[kResBytes * 256];
+  __shared__ float Ms[4][kMaxQTok];
+  __shared__ float rsc[4][32];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const uint32_t h = lane >> 5, col = lane & 31;
+  const uint32_t slice = blockIdx.x, q = blockIdx.y;
+  const uint32_t T = q_ntok ? q_ntok[q] : t_max;
+  const uint32_t n = sel_n[q];
+  for (uint32_t i = tid; i < kResBytes * 256; i += 256) lut[i] = lut_g[i];
+  long qf8[8];
+  float sq;
+  load_q_fp8(Q, q, t_max, T, h, col, qf8, sq);
+  __syncthreads();
+  for (uint32_t r = slice * 4 + wave; r < n; r += n_slice * 4) {
+    const uint32_t doc = sel_ids[(size_t)q * R + r];
+    const float M = maxsim_doc_fp8<1>(doclens[doc], docoff[doc], h, col, codes, residual, C8, csc, lut, sr,
+                                      qf8, sq, rsc[wave], clip_zero);
+    if (h == 0) Ms[wave][col] = M;
+    wave_lds_sync();
+    if (lane == 0) {
+      float sc = 0.0f;
+      for (uint32_t tt = 0; tt < T; ++tt) sc += Ms[wave][tt];
+      sel_score[(size_t)q * R + r] = sc;
+    }
+    wave_lds_sync();
+  }
+}
+
+// FP8 copies of the centroids (one wave per centroid) and of the residual
+// weight table.
+__global__ void k_to_fp8_rows(const float *__restrict__ C, uint32_t n_c, uint8_t *__restrict__ C8,
+                              float *__restrict__ csc) {
+  const uint32_t c = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (c >= n_c) return;
+  const float a = C[(size_t)c * kDim + 2 * lane], b = C[(size_t)c * kDim + 2 * lane + 1];
+  float m = fmaxf(fabsf(a), fabsf(b));
+#pragma unroll
+  for (uint32_t off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+  const float inv = m > 0.0f ? 448.0f / m : 0.0f;
+  reinterpret_cast<uint16_t *>(C8 + (size_t)c * kDim)[lane] = (uint16_t)fp8x2(a * inv, b * inv);
+  if (lane == 0) csc[c] = m / 448.0f;
+}
+
+__global__ void k_lut_fp8(const float *__restrict__ W, float inv_sr, uint32_t *__restrict__ lut) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kResBytes * 256) return;
+  const uint32_t pos = i >> 8, byte = i & 255u;
+  float v[4];
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) v[k] = W[(4 * pos + k) * 4 + ((byte >> (6 - 2 * k)) & 3u)] * inv_sr;
+  lut[i] = fp8x2(v[0], v[1]) | (fp8x2(v[2], v[3]) << 16);
+}
+
+// ---------------------------------------------------------------------------
 // Block-wide top-kk (kk <= 256) of n unique keys make_key(sc[i], ids[i]),
 // written best first.  n <= 512: one bitonic sort in LDS.  Otherwise MSB-first
 // radix select with 8-bit digits: keys above the pivot digit go to the output,
@@ -1014,21 +1179,28 @@ __global__ __launch_bounds__(256) void k_rerank_persist(
     const uint64_t *__restrict__ docoff, const uint32_t *__restrict__ codes,
     const uint8_t *__restrict__ residual, const float *__restrict__ C,
     const float *__restrict__ W, const uint16_t *__restrict__ Cb, const uint2 *__restrict__ lut_g,
-    int clip_zero, float *__restrict__ score, uint32_t kk, uint32_t *__restrict__ out_ids,
+    const uint8_t *__restrict__ C8, const float *__restrict__ csc, const uint32_t *__restrict__ lut8_g,
+    float sr, int clip_zero, float *__restrict__ score, uint32_t kk, uint32_t *__restrict__ out_ids,
     float *__restrict__ out_s, uint32_t *__restrict__ out_n, uint32_t out_stride) {
   __shared__ uint2 lut[PREC == 1 ? kResBytes * 256 : 1];
+  __shared__ uint32_t lut8[PREC == 2 ? kResBytes * 256 : 1];
   __shared__ float wl[PREC == 3 ? kDim * 4 : 1];
   __shared__ float Ms[4][kMaxQTok];
+  __shared__ float rsc[4][PREC == 2 ? 32 * TPS : 1];
   __shared__ TopkLds tk;
   __shared__ uint32_t s_q;
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const uint32_t h = lane >> 5, col = lane & 31;
   if constexpr (PREC == 1)
     for (uint32_t i = tid; i < kResBytes * 256; i += 256) lut[i] = lut_g[i];
+  else if constexpr (PREC == 2)
+    for (uint32_t i = tid; i < kResBytes * 256; i += 256) lut8[i] = lut8_g[i];
   else
     for (uint32_t i = tid; i < kDim * 4; i += 256) wl[i] = W[i];
   bf16x8_t qb[PREC == 1 ? 8 : 1];
   float qf[PREC == 3 ? 64 : 1];
+  long q8[PREC == 2 ? 8 : 1];
+  float sq = 0.0f;
   for (;;) {
     __syncthreads();  // the previous query's top-k is done with the LDS
     if (tid == 0) s_q = atomicAdd(queue, 1u);
@@ -1038,6 +1210,7 @@ __global__ __launch_bounds__(256) void k_rerank_persist(
     const uint32_t n = min(in_n[q], R_in);
     const uint32_t T = q_ntok ? q_ntok[q] : t_max;
     if constexpr (PREC == 1) load_q_bf16(Q, q, t_max, T, h, col, qb);
+    else if constexpr (PREC == 2) load_q_fp8(Q, q, t_max, T, h, col, q8, sq);
     else load_q_f32(Q, q, t_max, T, h, col, qf);
     float *qs = score + (size_t)q * R_in;
     for (uint32_t r = wave; r < n; r += 4) {
@@ -1046,6 +1219,9 @@ __global__ __launch_bounds__(256) void k_rerank_persist(
       if constexpr (PREC == 1)
         M = maxsim_doc_bf16<TPS>(doclens[doc], docoff[doc], h, col, codes, residual, Cb, lut, qb,
                                  clip_zero);
+      else if constexpr (PREC == 2)
+        M = maxsim_doc_fp8<TPS>(doclens[doc], docoff[doc], h, col, codes, residual, C8, csc, lut8, sr, q8,
+                                sq, rsc[wave], clip_zero);
       else
         M = maxsim_doc_f32mfma<TPS>(doclens[doc], docoff[doc], h, col, codes, residual, C, wl, qf,
                                     clip_zero);
@@ -1588,6 +1764,11 @@ hipError_t launch_rerank(const float *Q, uint32_t t_max, const uint32_t *q_ntok,
     k_rerank_f32mfma<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, sel_ids, sel_n, R, n_slice, ix.doclens,
                                           ix.docoff, ix.codes, ix.residual, ix.C, ix.W, clip_zero,
                                           sel_score);
+  } else if (precision == 2) {
+    if (!ix.C_fp8 || !ix.lut_fp8) return hipErrorInvalidValue;
+    k_rerank_fp8<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, sel_ids, sel_n, R, n_slice, ix.doclens, ix.docoff,
+                                      ix.codes, ix.residual, ix.C_fp8, ix.C_fp8_scale, ix.lut_fp8,
+                                      ix.fp8_res_scale, clip_zero, sel_score);
   } else if (precision == 1) {
     if (!ix.C_bf16 || !ix.lut_bf16) return hipErrorInvalidValue;
     k_rerank_bf16<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, sel_ids, sel_n, R, n_slice, ix.doclens,
@@ -1607,15 +1788,19 @@ hipError_t launch_persist(const float *Q, uint32_t t_max, const uint32_t *q_ntok
   if (nb == 0) return hipSuccess;
   if (kk == 0 || kk > kTopkMax || cfg.n_wg == 0) return hipErrorInvalidValue;
   if (precision == 1 && (!ix.C_bf16 || !ix.lut_bf16)) return hipErrorInvalidValue;
+  if (precision == 2 && (!ix.C_fp8 || !ix.lut_fp8)) return hipErrorInvalidValue;
   k_fill_u32<<<1, 64, 0, s>>>(queue, 0u, 1);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const uint32_t grid = std::min(cfg.n_wg, nb);
 #define MVR_PERSIST_ARGS                                                                           \
   Q, t_max, q_ntok, in_ids, in_n, R_in, nb, queue, ix.doclens, ix.docoff, ix.codes, ix.residual, \
-      ix.C, ix.W, ix.C_bf16, ix.lut_bf16, clip_zero, score, kk, out_ids, out_s, out_n, out_stride
+      ix.C, ix.W, ix.C_bf16, ix.lut_bf16, ix.C_fp8, ix.C_fp8_scale, ix.lut_fp8, ix.fp8_res_scale,   \
+      clip_zero, score, kk, out_ids, out_s, out_n, out_stride
   if (precision == 1 && cfg.tps == 1) k_rerank_persist<1, 1><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
   else if (precision == 1) k_rerank_persist<1, 2><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
+  else if (precision == 2 && cfg.tps == 1) k_rerank_persist<2, 1><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
+  else if (precision == 2) k_rerank_persist<2, 2><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
   else if (precision == 3 && cfg.tps == 1) k_rerank_persist<3, 1><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
   else if (precision == 3) k_rerank_persist<3, 2><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
   else return hipErrorNotSupported;
@@ -1691,9 +1876,22 @@ hipError_t launch_compress(const float *x, const uint32_t *codes, uint64_t n, co
 
 hipError_t launch_convert_centroids(const DevIndex &ix, hipStream_t s) {
   const size_t n = (size_t)ix.n_c * kDim;
-  if (!ix.C_bf16) return hipSuccess;
-  k_to_bf16<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(ix.C, ix.C_bf16, n);
-  return hipGetLastError();
+  if (ix.C_bf16) {
+    k_to_bf16<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(ix.C, ix.C_bf16, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (ix.C_fp8 && ix.C_fp8_scale) {
+    k_to_fp8_rows<<<(ix.n_c + 3) / 4, 256, 0, s>>>(ix.C, ix.n_c, ix.C_fp8, ix.C_fp8_scale);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (ix.lut_fp8) {
+    k_lut_fp8<<<(kResBytes * 256 + 255) / 256, 256, 0, s>>>(ix.W, ix.fp8_res_scale > 0 ? 1.0f / ix.fp8_res_scale : 0.0f,
+                                                             ix.lut_fp8);
+    return hipGetLastError();
+  }
+  return hipSuccess;
 }
 
 }  // namespace mvr

@@ -322,6 +322,29 @@ def bf16_round(x: np.ndarray) -> np.ndarray:
     return b.astype(np.uint32).view(np.float32)
 
 
+def e4m3_round(x: np.ndarray) -> np.ndarray:
+    """Round float32 to OCP FP8 E4M3 (3 mantissa bits, bias 7, subnormals down
+    to 2^-9, saturating at +-448), nearest even; returned as float32.  Model
+    of the FP8 precision mode (no counterpart in the reference)."""
+    x = np.asarray(x, np.float32).astype(np.float64)
+    a = np.minimum(np.abs(x), 448.0)
+    _, e = np.frexp(a)                      # a = m 2^e, m in [0.5, 1)
+    step = np.exp2(np.maximum(e - 4, -9).astype(np.float64))
+    q = np.round(a / step) * step           # numpy rounds half to even
+    return (np.sign(x) * np.minimum(q, 448.0)).astype(np.float32)
+
+
+def fp8_scaled(x: np.ndarray, axis=-1):
+    """Per-row fp8 encoding as the FP8 mode does it: s = max|x| / 448 and
+    e4m3(x * (448 / max|x|)) in float32 arithmetic; returns (codes as float32
+    values, s)."""
+    x = np.asarray(x, np.float32)
+    amax = np.max(np.abs(x), axis=axis, keepdims=True).astype(np.float32)
+    with np.errstate(divide="ignore"):
+        inv = np.where(amax > 0, np.float32(448.0) / amax, np.float32(0.0)).astype(np.float32)
+    return e4m3_round((x * inv).astype(np.float32)), (amax / np.float32(448.0)).astype(np.float32)
+
+
 def recall_at_k(est: np.ndarray, gnd: np.ndarray) -> float:
     """|gnd ∩ est| / k averaged over queries (script/evaluation/performance_metric.py:259-272)."""
     k = gnd.shape[1]

@@ -24,7 +24,7 @@ def _ragged(Q, seed):
     return qt, qp
 
 
-@pytest.mark.parametrize("prec,rescore", [("fp32_mfma", 0), ("bf16", 0), ("bf16", 64)])
+@pytest.mark.parametrize("prec,rescore", [("fp32_mfma", 0), ("bf16", 0), ("bf16", 64), ("fp8", 64)])
 def test_persistent_matches_launch_path_every_setting(c1, prec, rescore):
     corpus, ix, Q = c1
     g = _index(ix)

@@ -193,6 +193,68 @@ def _bf16_maxsim(ix, Qq, docs, clip_zero=False):
     return np.array(out)
 
 
+def _fp8_maxsim(ix, Qq, docs, clip_zero=False):
+    """MaxSim as the FP8 kernel defines it, in float64: query tokens and
+    centroid rows encoded per row (s = max|.|/448, e4m3 of x*448/max|.|),
+    residual weights with one global scale, score = s_q (s_c C8.q8 + s_r R8.q8)."""
+    from oracle import oracle as O
+    q8, sq = O.fp8_scaled(Qq)
+    c8, sc = O.fp8_scaled(ix.C)
+    wmax = np.float32(np.abs(ix.W).max())
+    sr = np.float32(wmax / np.float32(448.0))
+    w8 = O.e4m3_round((ix.W * (np.float32(1.0) / sr)).astype(np.float32)).astype(np.float64)
+    q8, sq, c8, sc = q8.astype(np.float64), sq.astype(np.float64), c8.astype(np.float64), sc.astype(np.float64)
+    out = []
+    for d in docs:
+        off, L = int(ix.docoff[d]), int(ix.doclens[d])
+        res = ix.residual[off:off + L]
+        b = np.stack([(res[:, j // 4] >> (6 - 2 * (j % 4))) & 3 for j in range(128)], 1)
+        codes = ix.codes[off:off + L]
+        SC = (q8 @ c8[codes].T) * sc[codes, 0][None, :]
+        SR = (q8 @ w8[np.arange(128)[None, :], b].T) * float(sr)
+        m = (sq * (SC + SR)).max(axis=1)
+        if clip_zero:
+            m = np.maximum(m, 0)
+        out.append(m.sum())
+    return np.array(out)
+
+
+def test_fp8_rerank_matches_fp8_model(small):
+    """FP8 MaxSim against its float64 definition on e4m3-encoded inputs (the
+    encoding modelled bit for bit on the CPU); differences come only from FP32
+    accumulation in the matrix unit."""
+    corpus, ix, Q = small
+    g = _index(ix)
+    ids, sc, st = g.search(Q, 4, 64, 10, precision="fp8", rescore=0, stages=True)
+    for q in range(Q.shape[0]):
+        n = int((st["sel_ids"][q] != 0xFFFFFFFF).sum())
+        ref = _fp8_maxsim(ix, Q[q], st["sel_ids"][q, :n])
+        np.testing.assert_allclose(st["sel_score"][q, :n], ref, rtol=1e-4, atol=1e-4)
+
+
+def test_fp8_with_fp32_rescore(c1):
+    """FP8 MaxSim over R=256 then FP32 rescoring of the best 64: the answer is
+    the FP32 top-10 of the FP8 path's best 64 (ties within 1e-5), the FP8
+    scores stay close to FP32, and the centroid stage is the exact FP32 one."""
+    from oracle import oracle as O
+    corpus, ix, Q = c1
+    g = _index(ix)
+    ids, sc, st = g.search(Q, 4, 256, 10, precision="fp8", rescore=64, stages=True)
+    ti, ts = O.centroid_topk(Q, ix.C, 4)
+    np.testing.assert_array_equal(st["top_ids"], ti.reshape(Q.shape[0], Q.shape[1], 4))
+    for q in range(Q.shape[0]):
+        n = min(int(st["cand_n"][q]), 256)
+        cid = st["sel_ids"][q, :n].astype(np.int64)
+        cs = st["sel_score"][q, :n]
+        exact = ix.rerank(Q[q], cid.astype(np.uint32))
+        assert np.median(np.abs(cs - exact) / np.abs(exact)) < 0.02
+        best = np.lexsort((cid, -cs.astype(np.float64)))[:64]
+        rid = cid[best]
+        rs = exact[best]
+        top = np.lexsort((rid, -rs.astype(np.float64)))[:10]
+        _check_topk_up_to_ties(ix, Q[q], ids[q], sc[q], rid[top].astype(np.uint32), rs[top], msg=f"q{q}")
+
+
 def test_bf16_rerank_scores_within_tolerance(small):
     corpus, ix, Q = small
     g = _index(ix)

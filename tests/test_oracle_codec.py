@@ -97,3 +97,29 @@ def test_encode_decode_roundtrip_error_bounded():
     np.testing.assert_array_equal(xd, (C[codes] + W[np.arange(dim)[None, :], b]).astype(np.float32))
     inner = (b == 1) | (b == 2)
     assert np.abs(xd - x)[inner].max() <= (cut[:, 2] - cut[:, 0]).max()
+
+
+def test_e4m3_round_known_values():
+    """OCP E4M3: exact values survive; halfway cases round to even; tiny
+    values go to subnormal steps of 2^-9; large values saturate at 448."""
+    from oracle import oracle as O
+    exact = np.array([1.0, 1.125, -3.0, 448.0, 2.0 ** -9, 0.015625, 240.0], np.float32)
+    np.testing.assert_array_equal(O.e4m3_round(exact), exact)
+    # 1 + 1/16 is halfway between 1 and 1.125 -> even mantissa (1.0);
+    # 1 + 3/16 is halfway between 1.125 and 1.25 -> 1.25
+    np.testing.assert_array_equal(O.e4m3_round(np.array([1.0625, 1.1875], np.float32)), [1.0, 1.25])
+    np.testing.assert_array_equal(O.e4m3_round(np.array([3 * 2.0 ** -10, 1000.0, -500.0], np.float32)),
+                                  np.array([2 * 2.0 ** -9, 448.0, -448.0], np.float32))
+    # every representable positive value is a fixed point, and the grid has 126 finite positives
+    vals = []
+    for e in range(-6, 9):
+        for m in range(8):
+            v = (1 + m / 8) * 2.0 ** e
+            if v <= 448:
+                vals.append(v)
+    vals += [m * 2.0 ** -9 for m in range(1, 8)]
+    v = np.array(sorted(vals), np.float32)
+    assert v.size == 126
+    np.testing.assert_array_equal(O.e4m3_round(v), v)
+    mids = ((v[:-1].astype(np.float64) + v[1:]) / 2 * 0.999).astype(np.float32)
+    np.testing.assert_array_equal(O.e4m3_round(mids), v[:-1])
