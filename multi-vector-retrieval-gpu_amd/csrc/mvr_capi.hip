@@ -103,7 +103,7 @@ struct PassGraph {
 struct Workspace {
   DBuf Q, meta, part_ids, part_s, top_ids, top_s, cand_cnt, cand_tot, cand_keys, sel_ids, sel_F, sel_n,
       sel_score, out_ids, out_s, counters, res_ids, res_s, res_n, bmax, feat, mlp_out, dyn_np, dyn_r,
-      rr_meta, queue, pass_nt, pass_np, pass_rm;
+      rr_meta, queue, pass_nt, pass_np, pass_rm, xcd_part, xcd_queue;
   HBuf staging;
   hipEvent_t staging_done = nullptr;
   std::vector<PassGraph> graphs;  // most recently used last
@@ -113,7 +113,7 @@ struct Workspace {
     return {&Q, &meta, &part_ids, &part_s, &top_ids, &top_s, &cand_cnt, &cand_tot, &cand_keys,
             &sel_ids, &sel_F, &sel_n, &sel_score, &out_ids, &out_s, &counters, &res_ids, &res_s,
             &res_n, &bmax, &feat, &mlp_out, &dyn_np, &dyn_r, &rr_meta, &queue, &pass_nt, &pass_np,
-            &pass_rm};
+            &pass_rm, &xcd_part, &xcd_queue};
   }
   void drop_graphs() {
     for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
@@ -142,6 +142,8 @@ struct mvr_index {
   Workspace ws;
   PersistCfg tune[4];          // work-queue MaxSim settings per precision
   bool tuned[4] = {false, false, false, false};
+  XcdLayout xl;                // slice-grouped token copy (built on first use)
+  bool xl_failed = false;
 };
 
 namespace {
@@ -398,9 +400,39 @@ void mvr_index_free(mvr_index *ix) {
   DeviceGuard g(ix->device);
   (void)hipDeviceSynchronize();
   ix->ws.release();
+  for (void *p : {(void *)ix->xl.codes_s, (void *)ix->xl.residual_s, (void *)ix->xl.sl_off})
+    if (p) (void)hipFree(p);
   free_devindex(ix->d);
   delete ix;
 }
+
+namespace {
+
+// The slice-grouped token copy for the XCD-sliced MaxSim (built on first use;
+// false when the index cannot have one, e.g. docs longer than 65,535 tokens).
+bool ensure_xcd_layout(mvr_index *ix, hipStream_t s) {
+  if (ix->xl.codes_s) return true;
+  if (ix->xl_failed) return false;
+  const DevIndex &d = ix->d;
+  XcdLayout xl;
+  hipError_t e = d.max_doclen < 65536 ? hipSuccess : hipErrorInvalidValue;
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&xl.codes_s), d.n_tok * 4);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&xl.residual_s), d.n_tok * kResBytes);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&xl.sl_off), (size_t)d.n_doc * 9 * 2);
+  if (e == hipSuccess) e = launch_slice_docs(d, xl.codes_s, xl.residual_s, xl.sl_off, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    for (void *p : {(void *)xl.codes_s, (void *)xl.residual_s, (void *)xl.sl_off})
+      if (p) (void)hipFree(p);
+    ix->xl_failed = true;
+    (void)hipGetLastError();
+    return false;
+  }
+  ix->xl = xl;
+  return true;
+}
+
+}  // namespace
 
 int mvr_index_info(const mvr_index *ix, uint32_t *n_doc, uint64_t *n_tok, uint32_t *n_centroid,
                    uint32_t *max_list, uint32_t *max_doclen) {
@@ -538,12 +570,23 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
                           k <= kPersistTopkMax && R2 <= kPersistTopkMax;
   // graph replay: not with learned probing (the model's launches) or stage outputs
   const bool use_graph = prm->graph && !learned && !stages;
+  // XCD-sliced MaxSim for the MFMA precisions: opt in with MVR_MAXSIM_SLICED=1
+  // (it cuts the fabric traffic of the centroid gathers by more than half but
+  // is not faster than the per-doc kernels on MI355X, see the design notes)
+  const char *sliced_var = std::getenv("MVR_MAXSIM_SLICED");
+  const bool sliced_env = sliced_var && sliced_var[0] == '1';
+  const bool sliced = !persistent && prm->precision != MVR_FP32 && sliced_env && ensure_xcd_layout(ix, s);
+  const uint32_t xcd_wgs = std::max(1u, d.n_cu / 8) * 2;
   if (!on_device || use_graph) {
     HIP_TRY(ws.Q.ensure(Bc * t_max * kDim * 4));
     HIP_TRY(ws.out_ids.ensure(Bc * k * 4));
     HIP_TRY(ws.out_s.ensure(Bc * k * 4));
   }
   if (persistent) HIP_TRY(ws.queue.ensure(4));
+  if (sliced) {
+    HIP_TRY(ws.xcd_queue.ensure(8 * 4));
+    HIP_TRY(ws.xcd_part.ensure(Bc * std::max<size_t>(R, R2) * kXcdParts * 4));
+  }
   if (use_graph) {
     HIP_TRY(ws.pass_nt.ensure(Bc * 4));
     HIP_TRY(ws.pass_np.ensure(Bc * 4));
@@ -627,16 +670,26 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       HIP_TRY(mark(6));
       return MVR_OK;
     }
-    HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R,
-                          d, prm->precision, clip, ws.sel_score.as<float>(), st));
+    if (sliced)
+      HIP_TRY(launch_rerank_xcd(Qb, t_max, bnt, ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R, nb, d,
+                                ix->xl, (int)prm->precision, clip, xcd_wgs, ws.xcd_queue.as<uint32_t>(),
+                                ws.xcd_part.as<float>(), ws.sel_score.as<float>(), st));
+    else
+      HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R,
+                            d, prm->precision, clip, ws.sel_score.as<float>(), st));
     HIP_TRY(mark(4));
     if (R2) {
       // best R2 by the low-precision score, FP32 MaxSim of those, top-k of the FP32 scores
       HIP_TRY(launch_topk(ws.sel_ids.as<uint32_t>(), ws.sel_score.as<float>(), ws.sel_n.as<uint32_t>(),
                           R, nb, R2, ws.res_ids.as<uint32_t>(), ws.res_s.as<float>(),
                           ws.res_n.as<uint32_t>(), st));
-      HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2,
-                            d, MVR_FP32_MFMA, clip, ws.res_s.as<float>(), st));
+      if (sliced)
+        HIP_TRY(launch_rerank_xcd(Qb, t_max, bnt, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2, nb, d,
+                                  ix->xl, MVR_FP32_MFMA, clip, xcd_wgs, ws.xcd_queue.as<uint32_t>(),
+                                  ws.xcd_part.as<float>(), ws.res_s.as<float>(), st));
+      else
+        HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2,
+                              d, MVR_FP32_MFMA, clip, ws.res_s.as<float>(), st));
     }
     HIP_TRY(mark(5));
     if (R2)
@@ -723,7 +776,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       const PersistCfg &tc = ix->tune[prm->precision];
       const std::vector<uint64_t> key = {nb, n_rows, t_max, P, R, R2, k, prm->precision, prm->clip_zero,
                                          uniform ? 1u : 0u, persistent ? 1u : 0u, fast_ct ? 1u : 0u,
-                                         tc.n_wg, tc.tps};
+                                         tc.n_wg, tc.tps, sliced ? 1u : 0u};
       hipGraphExec_t exec = nullptr;
       for (size_t gi = 0; gi < ws.graphs.size(); ++gi)
         if (ws.graphs[gi].key == key) {
