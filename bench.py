@@ -47,6 +47,12 @@ CONFIGS = {
                 kmeans_sample=None, n_iter=10, learned=True, n_train=65536, target=0.9, topic=True),
     "c4": dict(n_doc=1 << 20, n_comp=65536, n_c=65536, lognormal=True, n_q=16384, P=32, R=1000, k=100,
                kmeans_sample=1 << 21, n_iter=4, ragged=True),
+    # the PLAID path (ColBERTv2 IndexScorer.rank, CPU-branch semantics) on a
+    # PLAID index of the c2 corpus, searcher defaults for k
+    "plaid": dict(n_doc=1 << 20, n_comp=65536, n_c=65536, lognormal=True, n_q=4096, k=10,
+                  kmeans_sample=1 << 21, n_iter=4, plaid=True),
+    "plaid100": dict(n_doc=1 << 20, n_comp=65536, n_c=65536, lognormal=True, n_q=4096, k=100,
+                     kmeans_sample=1 << 21, n_iter=4, plaid=True),
 }
 
 
@@ -88,6 +94,9 @@ def main():
     cfg = CONFIGS[args.config]
     if cfg.get("learned"):
         bench_learned(torch, args, cfg)
+        return
+    if cfg.get("plaid"):
+        bench_plaid(torch, args, cfg)
         return
     t_setup = time.time()
     if cfg["lognormal"]:
@@ -265,6 +274,77 @@ def bench_learned(torch, args, cfg):
                                    f"{args.epochs} epochs, fallback nprobe 16 below softmax 0.6"},
             **out}
     print(json.dumps(line), flush=True)
+
+
+def bench_plaid(torch, args, cfg):
+    """PLAID search (two-stage centroid filter, FP32 decompression, L2
+    normalisation, zero-initialised MaxSim) on a PLAID index of the c2
+    corpus; the C oracle of the same search on the host cores beside it,
+    with the GPU results checked bit for bit on that sample.  One JSON line."""
+    from mvr import plaid, synth
+    t0 = time.time()
+    doclens = synth.doclens_lognormal(args.seed, cfg["n_doc"], 64.0, 0.29, 8, 300)
+    corpus = synth.build_corpus(torch, args.seed, cfg["n_doc"], cfg["n_comp"], cfg["n_c"], doclens,
+                                n_iter=cfg["n_iter"], kmeans_sample=cfg["kmeans_sample"],
+                                log=lambda m: log("[bench]", m))
+    pa = synth.plaid_from_corpus(torch, corpus, log=lambda m: log("[bench]", m))
+    n_q, k = cfg["n_q"], cfg["k"]
+    Q = synth.make_queries(torch, args.seed, corpus.centers, n_q)
+    corpus.index.close()
+    corpus = None
+    torch.cuda.empty_cache()
+    g = plaid.PlaidIndex(**pa.index_kwargs())
+    ncells, thr, ndocs = plaid.searcher_defaults(k)
+    log(f"[bench] plaid setup {time.time() - t0:.1f}s: {int(doclens.sum())} tokens, ncells {ncells}, "
+        f"threshold {thr}, ndocs {ndocs}")
+    for _ in range(args.warmup):
+        g.search(Q, k=k)
+    torch.cuda.synchronize()
+    keys = ("candidates_ms", "filter_ms", "refine_ms", "total_ms", "n_refine_ivf", "n_refine_filter",
+            "n_vec_score_refine")
+    acc = {f: 0.0 for f in keys}
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        ids, sc, tm = g.search(Q, k=k, timings=True)
+        for f in keys:
+            acc[f] += tm[f]
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t1
+    qps = n_q * args.steps / elapsed
+    st = {f: acc[f] / args.steps for f in keys}
+    log(f"[bench] plaid {args.steps} steps: {elapsed * 1e3 / args.steps:.2f} ms/step, {qps:.0f} QPS; "
+        + ", ".join(f"{f}={st[f]:.2f}" for f in keys[:4]))
+    cpu = None
+    if not args.no_cpu and args.cpu_queries > 0:
+        from oracle import plaid as OP
+        cores = len(os.sched_getaffinity(0))
+        n_s = min(args.cpu_queries, n_q)
+        oix = OP.PlaidIndex(**pa.index_kwargs())
+        Qh = Q[:n_s].cpu().numpy()
+        oix.search(Qh[:16], k, ncells, thr, ndocs, n_threads=cores)
+        t2 = time.time()
+        rid, rsc, _ = oix.search(Qh, k, ncells, thr, ndocs, n_threads=cores)
+        wall = time.time() - t2
+        same = bool(np.array_equal(rid, ids[:n_s]) and np.array_equal(rsc, sc[:n_s]))
+        cpu = {"value": round(n_s / wall, 2), "unit": "queries/s", "cores": cores, "kind": "port",
+               "sample": f"first {n_s} queries, same PLAID index and settings",
+               "seconds": round(wall, 2), "gpu_results_bitexact": same}
+        log(f"[bench] plaid cpu baseline {n_s / wall:.1f} QPS on {cores} cores; GPU results bit-exact: {same}")
+    line = {
+        "metric": f"PLAID search queries/sec (searcher defaults for k={k}), 1 GPU",
+        "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / args.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32", "data": "synthetic (c2 corpus as a PLAID index, built on the device)",
+        "config": {"workload": f"{args.config}: {cfg['n_doc']} docs / {int(doclens.sum())} tokens, "
+                               f"{cfg['n_c']} centroids, 2-bit residuals; {n_q} queries x 32 tokens; "
+                               f"ncells {ncells}, centroid_score_threshold {thr}, ndocs {ndocs}, k {k}"},
+        "stages_ms_per_step": {f: round(st[f], 3) for f in keys[:4]},
+        "per_query": {"n_refine_ivf": st["n_refine_ivf"] / n_q, "n_refine_filter": st["n_refine_filter"] / n_q,
+                      "n_vec_score_refine": st["n_vec_score_refine"] / n_q},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    g.close()
 
 
 def roofline(kname, achieved_tf, peak_tf, bytes_alg, avg_launch_ms, n_launch, tok_per_launch, flops, traffic):

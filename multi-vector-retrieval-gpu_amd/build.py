@@ -26,7 +26,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
              "-Wno-unused-command-line-argument"]
-HIP_SOURCES = ["mvr_kernels.hip", "mvr_capi.hip", "mvr_mlp.hip"]
+HIP_SOURCES = ["mvr_kernels.hip", "mvr_capi.hip", "mvr_mlp.hip", "mvr_plaid.hip"]
 
 
 def _run(cmd):

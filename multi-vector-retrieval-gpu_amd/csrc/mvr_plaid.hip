@@ -1,0 +1,1142 @@
+// PLAID (ColBERTv2) search path and residual codec ops for gfx950, with their
+// C ABI (include/mvr_plaid.h).
+//
+// Replaces (reference paths, baseline/ColBERT/colbert/):
+//   indexing/codecs/decompress_residuals.cu:8-75  -> k_pl_decompress_f16
+//   indexing/codecs/packbits.cu:10-57             -> k_pl_packbits
+//   indexing/codecs/residual.py:189-224 (binarize) -> k_pl_binarize
+//   search/decompress_residuals.cpp:27-155        -> k_pl_decompress_pids
+//   modeling/segmented_maxsim.cpp:22-93           -> k_pl_segmented_maxsim
+//   search/filter_pids.cpp:30-193                 -> k_pl_approx (both stages) + k_select_topr
+//   search/candidate_generation.py:13-66          -> k_pl_cscores, k_pl_cells, k_pl_candidates
+//   search/index_storage.py:101-198 (CPU branch)  -> the batched pipeline of mvr_plaid_search
+//
+// Design: the reference runs one query at a time through pthread loops
+// (filter, decompress, MaxSim) and a priority_queue merge.  Here a device pass
+// takes a batch of queries through seven kernels with no host round trip:
+// centroid scores S = C.Q^T for the whole batch, per-token top-ncells and the
+// IVF union (a per-query pid bitmap whose first setter appends the pid), the
+// two filter stages as one wave per pid (dedup of codes in an LDS bitmap,
+// 32 lanes = 32 query tokens reading one 128-byte row of S per code), exact
+// top-n selection on 64-bit (score, pid) keys, and a refine kernel that
+// decompresses, L2-normalises and scores the surviving docs without writing
+// any decompressed embedding to HBM.
+#include "../../include/mvr.h"
+#include "../../include/mvr_plaid.h"
+#include "mvr_internal.h"
+
+#include <algorithm>
+#include <cfloat>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+namespace mvr {
+namespace {
+
+constexpr uint32_t kPlMaxCells = 8;      // largest ncells
+constexpr uint32_t kPlMaxBmBytes = 65536;  // LDS of one workgroup's dedup bitmaps
+
+__device__ __forceinline__ void pl_wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Order-preserving float -> uint32 map (larger float, larger uint).
+__device__ __forceinline__ uint32_t pl_f2ord(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ __forceinline__ float pl_readlane(float v, uint32_t l) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+
+// std::max(a, b) of the reference (returns a unless a < b).
+__device__ __forceinline__ float pl_max(float a, float b) { return (a < b) ? b : a; }
+
+__device__ __forceinline__ float h2f(uint16_t h) {
+  return (float)__builtin_bit_cast(_Float16, h);
+}
+__device__ __forceinline__ uint16_t f2h(float f) {  // round to nearest even
+  return __builtin_bit_cast(uint16_t, (_Float16)f);
+}
+
+// ---------------------------------------------------------------------------
+// Codec ops
+// ---------------------------------------------------------------------------
+// decompress_residuals.cu:8-40, one thread per packed byte (as the
+// reference); the 256 x vpb table bw[lut[rbm[v]][k]] sits in LDS.
+__global__ __launch_bounds__(256) void k_pl_decompress_f16(
+    const uint8_t *__restrict__ res, uint64_t n_bytes, uint32_t pk, const uint16_t *__restrict__ bw,
+    const uint8_t *__restrict__ rbm, const uint8_t *__restrict__ lut, uint32_t vpb,
+    const int32_t *__restrict__ codes, const uint16_t *__restrict__ C, uint32_t n_c, uint32_t dim,
+    uint16_t *__restrict__ out) {
+  __shared__ uint16_t tab[256 * 8];
+  for (uint32_t e = threadIdx.x; e < 256 * vpb; e += blockDim.x) {
+    const uint32_t v = e / vpb, k = e % vpb;
+    tab[e] = bw[lut[(uint32_t)rbm[v] * vpb + k]];
+  }
+  __syncthreads();
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n_bytes; e += stride) {
+    const uint64_t i = e / pk;
+    const uint32_t j = (uint32_t)(e - i * pk);
+    const uint32_t code = (uint32_t)codes[i];
+    const uint32_t v = res[e];
+    const uint16_t *c = C + (size_t)code * dim + (size_t)j * vpb;
+    uint16_t *o = out + i * dim + (size_t)j * vpb;
+    for (uint32_t k = 0; k < vpb; ++k) {
+      const float cf = code < n_c ? h2f(c[k]) : 0.0f;
+      o[k] = f2h(h2f(tab[v * vpb + k]) + cf);
+    }
+  }
+}
+
+// packbits.cu:10-37: output byte j = input bytes 8j..8j+7 (nonzero -> 1), MSB first.
+__global__ __launch_bounds__(256) void k_pl_packbits(const uint8_t *__restrict__ in, uint64_t n_out,
+                                                     uint8_t *__restrict__ out) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n_out; j += stride) {
+    const uint8_t *p = in + 8 * j;
+    uint32_t v = 0;
+#pragma unroll
+    for (uint32_t b = 0; b < 8; ++b) v = (v << 1) | (p[b] != 0 ? 1u : 0u);
+    out[j] = (uint8_t)v;
+  }
+}
+
+// residual.py:189-224: one thread per output byte.
+__global__ __launch_bounds__(256) void k_pl_binarize(const float *__restrict__ e, uint64_t n_out,
+                                                     uint32_t row_bytes, uint32_t dim,
+                                                     const float *__restrict__ cut, uint32_t n_cut,
+                                                     uint32_t nbits, uint8_t *__restrict__ out) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < n_out; o += stride) {
+    const uint64_t i = o / row_bytes;
+    const uint32_t j = (uint32_t)(o - i * row_bytes);
+    uint32_t v = 0, last_d = 0xFFFFFFFFu, b = 0;
+    for (uint32_t p = 8 * j; p < 8 * j + 8; ++p) {
+      const uint32_t d = p / nbits, k = p - d * nbits;
+      if (d != last_d) {
+        const float x = e[i * dim + d];
+        b = 0;
+        for (uint32_t c = 0; c < n_cut; ++c) b += (cut[c] < x) ? 1u : 0u;
+        last_d = d;
+      }
+      v = (v << 1) | ((b >> k) & 1u);
+    }
+    out[o] = (uint8_t)v;
+  }
+}
+
+// cum[0] = 0, cum[i+1] = cum[i] + lengths[pids ? pids[i] : i]; one workgroup.
+__global__ __launch_bounds__(1024) void k_pl_cumsum(const int32_t *__restrict__ pids, uint32_t n,
+                                                    const int64_t *__restrict__ lengths,
+                                                    int64_t *__restrict__ cum) {
+  __shared__ int64_t part[1024];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t chunk = (n + 1023) / 1024;
+  const uint32_t b = min(n, tid * chunk), e = min(n, b + chunk);
+  int64_t s = 0;
+  for (uint32_t i = b; i < e; ++i) s += lengths[pids ? pids[i] : i];
+  part[tid] = s;
+  __syncthreads();
+  for (uint32_t off = 1; off < 1024; off <<= 1) {
+    const int64_t add = tid >= off ? part[tid - off] : 0;
+    __syncthreads();
+    part[tid] += add;
+    __syncthreads();
+  }
+  int64_t run = tid ? part[tid - 1] : 0;
+  if (tid == 0) cum[0] = 0;
+  for (uint32_t i = b; i < e; ++i) {
+    run += lengths[pids ? pids[i] : i];
+    cum[i + 1] = run;
+  }
+}
+
+// search/decompress_residuals.cpp:36-75, one workgroup per pid.
+__global__ __launch_bounds__(256) void k_pl_decompress_pids(
+    const int32_t *__restrict__ pids, const int64_t *__restrict__ cum, const int64_t *__restrict__ lengths,
+    const int64_t *__restrict__ offsets, const float *__restrict__ bw, const uint8_t *__restrict__ rbm,
+    const uint8_t *__restrict__ lut, uint32_t vpb, const uint8_t *__restrict__ res, uint32_t pk,
+    const int32_t *__restrict__ codes, const float *__restrict__ C, uint32_t n_c, uint32_t dim,
+    float *__restrict__ out) {
+  __shared__ float tab[256 * 8];
+  for (uint32_t e = threadIdx.x; e < 256 * vpb; e += blockDim.x) {
+    const uint32_t v = e / vpb, k = e % vpb;
+    tab[e] = bw[lut[(uint32_t)rbm[v] * vpb + k]];
+  }
+  __syncthreads();
+  const uint32_t i = blockIdx.x;
+  const int32_t pid = pids[i];
+  const int64_t L = lengths[pid], off = offsets[pid];
+  const uint64_t row0 = (uint64_t)cum[i];
+  const uint64_t n_e = (uint64_t)L * pk;
+  for (uint64_t e = threadIdx.x; e < n_e; e += blockDim.x) {
+    const uint64_t j = e / pk;
+    const uint32_t kb = (uint32_t)(e - j * pk);
+    const uint64_t tok = (uint64_t)off + j;
+    const uint32_t code = (uint32_t)codes[tok];
+    const uint32_t v = res[tok * pk + kb];
+    float *o = out + (row0 + j) * dim + (size_t)kb * vpb;
+    const float *c = C + (size_t)code * dim + (size_t)kb * vpb;
+    for (uint32_t l = 0; l < vpb; ++l) o[l] = tab[v * vpb + l] + (code < n_c ? c[l] : 0.0f);
+  }
+}
+
+// segmented_maxsim.cpp:22-46: one wave per doc, lane t = query token.
+__global__ __launch_bounds__(256) void k_pl_segmented_maxsim(const float *__restrict__ s, uint32_t nq,
+                                                             const int64_t *__restrict__ cum,
+                                                             uint32_t ndocs, float *__restrict__ out) {
+  const uint32_t lane = threadIdx.x & 63;
+  const uint32_t d = blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (d >= ndocs) return;
+  float m = 0.0f;
+  if (lane < nq)
+    for (int64_t r = cum[d]; r < cum[d + 1]; ++r) m = pl_max(m, s[(uint64_t)r * nq + lane]);
+  float sum = 0.0f;
+  for (uint32_t t = 0; t < nq; ++t) sum += pl_readlane(m, t);
+  if (lane == 0) out[d] = sum;
+}
+
+// ---------------------------------------------------------------------------
+// Candidate generation (candidate_generation.py:13-66)
+// ---------------------------------------------------------------------------
+// S[q][c][t] = FMA chain over d of C[c][d] * Q[q][t][d]; idx[q][c] =
+// max_t S >= threshold.  Workgroup = 64 centroids of one query; thread
+// (cc, tg) computes 8 tokens of centroid cc.
+__global__ __launch_bounds__(256) void k_pl_cscores(const float *__restrict__ Q, uint32_t nq,
+                                                    const float *__restrict__ C, uint32_t n_c, float thr,
+                                                    float *__restrict__ S, uint8_t *__restrict__ idx) {
+  __shared__ __align__(16) float Qs[kDim][kMaxQTok];
+  const uint32_t tid = threadIdx.x, q = blockIdx.y;
+  for (uint32_t e = tid; e < kDim * kMaxQTok; e += 256) {
+    const uint32_t t = e / kDim, d = e % kDim;
+    Qs[d][t] = t < nq ? Q[((size_t)q * nq + t) * kDim + d] : 0.0f;
+  }
+  __syncthreads();
+  const uint32_t cc = tid >> 2, tg = tid & 3, c = blockIdx.x * 64 + cc;
+  float acc[8];
+#pragma unroll
+  for (uint32_t e = 0; e < 8; ++e) acc[e] = 0.0f;
+  if (c < n_c) {
+    const float4 *crow = reinterpret_cast<const float4 *>(C + (size_t)c * kDim);
+#pragma unroll 4
+    for (uint32_t d4 = 0; d4 < kDim / 4; ++d4) {
+      const float4 cv = crow[d4];
+      const float cx[4] = {cv.x, cv.y, cv.z, cv.w};
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        const float4 qa = *reinterpret_cast<const float4 *>(&Qs[4 * d4 + u][tg * 8]);
+        const float4 qb = *reinterpret_cast<const float4 *>(&Qs[4 * d4 + u][tg * 8 + 4]);
+        acc[0] = __builtin_fmaf(cx[u], qa.x, acc[0]);
+        acc[1] = __builtin_fmaf(cx[u], qa.y, acc[1]);
+        acc[2] = __builtin_fmaf(cx[u], qa.z, acc[2]);
+        acc[3] = __builtin_fmaf(cx[u], qa.w, acc[3]);
+        acc[4] = __builtin_fmaf(cx[u], qb.x, acc[4]);
+        acc[5] = __builtin_fmaf(cx[u], qb.y, acc[5]);
+        acc[6] = __builtin_fmaf(cx[u], qb.z, acc[6]);
+        acc[7] = __builtin_fmaf(cx[u], qb.w, acc[7]);
+      }
+    }
+  }
+  float m = -INFINITY;
+  if (c < n_c) {
+    float *srow = S + ((size_t)q * n_c + c) * nq;
+#pragma unroll
+    for (uint32_t e = 0; e < 8; ++e) {
+      const uint32_t t = tg * 8 + e;
+      if (t < nq) {
+        srow[t] = acc[e];
+        m = fmaxf(m, acc[e]);
+      }
+    }
+  }
+  m = fmaxf(m, __shfl_xor(m, 1));
+  m = fmaxf(m, __shfl_xor(m, 2));
+  if (c < n_c && tg == 0) idx[(size_t)q * n_c + c] = m >= thr ? 1 : 0;
+}
+
+__device__ __forceinline__ bool pl_better(float s1, uint32_t i1, float s2, uint32_t i2) {
+  return (s1 > s2) || (s1 == s2 && i1 < i2);
+}
+
+// Per query token the best NC centroids by (score desc, centroid asc), their
+// union as an ascending cell list.  One wave per query: lane (t, h) scans the
+// centroids c = h (mod 2) of token t, then the two halves merge.
+template <uint32_t NC>
+__global__ __launch_bounds__(64) void k_pl_cells(const float *__restrict__ S, uint32_t n_c, uint32_t nq,
+                                                 uint32_t *__restrict__ cells, uint32_t cell_cap,
+                                                 uint32_t *__restrict__ cell_n) {
+  extern __shared__ uint32_t bm[];
+  const uint32_t lane = threadIdx.x, q = blockIdx.x;
+  const uint32_t t = lane & 31, h = lane >> 5;
+  const uint32_t words = (n_c + 31) / 32;
+  for (uint32_t w = lane; w < words; w += 64) bm[w] = 0;
+  float bs[NC];
+  uint32_t bi[NC];
+#pragma unroll
+  for (uint32_t j = 0; j < NC; ++j) { bs[j] = -INFINITY; bi[j] = 0xFFFFFFFFu; }
+  if (t < nq) {
+    const float *Sq = S + (size_t)q * n_c * nq + t;
+    for (uint32_t c = h; c < n_c; c += 2) {
+      float cv = Sq[(size_t)c * nq];
+      uint32_t ci = c;
+      if (pl_better(cv, ci, bs[NC - 1], bi[NC - 1])) {
+#pragma unroll
+        for (uint32_t j = 0; j < NC; ++j) {
+          if (pl_better(cv, ci, bs[j], bi[j])) {
+            const float ts = bs[j]; const uint32_t ti = bi[j];
+            bs[j] = cv; bi[j] = ci; cv = ts; ci = ti;
+          }
+        }
+      }
+    }
+  }
+  // merge the odd-centroid list of lane t + 32 into lane t's
+  float os[NC];
+  uint32_t oi[NC];
+#pragma unroll
+  for (uint32_t j = 0; j < NC; ++j) {
+    os[j] = __shfl(bs[j], (int)(t + 32));
+    oi[j] = (uint32_t)__shfl((int)bi[j], (int)(t + 32));
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < NC; ++j) {
+    float cv = os[j];
+    uint32_t ci = oi[j];
+#pragma unroll
+    for (uint32_t u = 0; u < NC; ++u) {
+      if (pl_better(cv, ci, bs[u], bi[u])) {
+        const float ts = bs[u]; const uint32_t ti = bi[u];
+        bs[u] = cv; bi[u] = ci; cv = ts; ci = ti;
+      }
+    }
+  }
+  pl_wave_sync();
+  if (h == 0 && t < nq) {
+#pragma unroll
+    for (uint32_t j = 0; j < NC; ++j)
+      if (bi[j] < n_c) atomicOr(&bm[bi[j] >> 5], 1u << (bi[j] & 31));
+  }
+  pl_wave_sync();
+  uint32_t total = 0;
+  for (uint32_t w0 = 0; w0 < words; w0 += 64) {
+    const uint32_t w = w0 + lane;
+    const uint32_t word = w < words ? bm[w] : 0u;
+    const uint32_t cnt = __popc(word);
+    uint32_t incl = cnt;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+      const uint32_t v = (uint32_t)__shfl_up((int)incl, off);
+      if (lane >= off) incl += v;
+    }
+    uint32_t pos = total + incl - cnt;
+    uint32_t bits = word;
+    while (bits) {
+      const uint32_t b = __ffs(bits) - 1;
+      bits &= bits - 1;
+      if (pos < cell_cap) cells[(size_t)q * cell_cap + pos] = w * 32 + b;
+      ++pos;
+    }
+    total += (uint32_t)__shfl((int)incl, 63);
+  }
+  if (lane == 0) cell_n[q] = min(total, cell_cap);
+}
+
+// Union of the IVF lists of a query's cells: the first setter of a pid's bit
+// appends the pid (one wave-aggregated atomic per 64 list entries).
+__global__ __launch_bounds__(256) void k_pl_candidates(
+    const uint32_t *__restrict__ cells, uint32_t cell_cap, const uint32_t *__restrict__ cell_n,
+    const int32_t *__restrict__ ivf, const int64_t *__restrict__ ivf_off,
+    const int64_t *__restrict__ ivf_len, uint32_t *__restrict__ pid_bm, uint32_t bm_words,
+    uint32_t *__restrict__ cand, uint32_t cand_cap, uint32_t *__restrict__ cand_n) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = blockIdx.y;
+  const uint32_t nw = gridDim.x * 4;
+  const uint32_t n = cell_n[q];
+  uint32_t *bmq = pid_bm + (size_t)q * bm_words;
+  for (uint32_t ci = blockIdx.x * 4 + wave; ci < n; ci += nw) {
+    const uint32_t c = cells[(size_t)q * cell_cap + ci];
+    const int64_t off = ivf_off[c], len = ivf_len[c];
+    for (int64_t j0 = 0; j0 < len; j0 += 64) {
+      const int64_t j = j0 + lane;
+      uint32_t pid = 0;
+      bool isnew = false;
+      if (j < len) {
+        pid = (uint32_t)ivf[off + j];
+        const uint32_t bit = 1u << (pid & 31);
+        isnew = !(atomicOr(&bmq[pid >> 5], bit) & bit);
+      }
+      const uint64_t mask = __ballot(isnew);
+      if (mask) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&cand_n[q], (uint32_t)__popcll(mask));
+        base = (uint32_t)__shfl((int)base, 0);
+        if (isnew) {
+          const uint32_t pos = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+          if (pos < cand_cap) cand[(size_t)q * cand_cap + pos] = pid;
+        }
+      }
+    }
+  }
+}
+
+// seg_cnt[q][sg] = number of keys in segment sg when the query has n keys
+// laid out flat (segment sg = keys [sg*kSeg, (sg+1)*kSeg)).
+__global__ void k_pl_segcnt(const uint32_t *__restrict__ n_in, uint32_t n_fixed, uint32_t n_seg,
+                            uint32_t n_q, uint32_t *__restrict__ seg_cnt) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (uint64_t)n_q * n_seg) return;
+  const uint32_t q = (uint32_t)(e / n_seg), sg = (uint32_t)(e % n_seg);
+  const uint32_t n = n_in ? n_in[q] : n_fixed;
+  const uint32_t lo = sg * kSeg;
+  seg_cnt[e] = n > lo ? min(n - lo, kSeg) : 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Filter stages (filter_pids.cpp:30-80): one wave per pid.  The wave walks
+// the doc's codes 64 at a time; a code that passes idx (stage 1) and was not
+// seen before in this doc (LDS bitmap, one per wave) is new; new codes are
+// consumed two at a time, lanes (t, h) taking max with S[code_h][t].
+// key = (score order bits << 32) | pid: larger keys first = (score, pid)
+// descending, the priority_queue order.  kInv: input ids are selection
+// outputs (~pid).
+// ---------------------------------------------------------------------------
+template <bool kIdx, bool kInv>
+__global__ __launch_bounds__(256) void k_pl_approx(
+    const uint32_t *__restrict__ in_ids, const uint32_t *__restrict__ in_n, uint32_t n_fixed,
+    uint64_t in_stride, const float *__restrict__ S, uint64_t s_stride, const uint8_t *__restrict__ idx,
+    uint64_t idx_stride, uint32_t n_c, uint32_t nq, uint32_t n_doc, const int32_t *__restrict__ codes,
+    const int64_t *__restrict__ doclens, const int64_t *__restrict__ offsets, uint32_t *__restrict__ pid_bm,
+    uint32_t bm_words, uint32_t waves_per_block, unsigned long long *__restrict__ keys, uint64_t key_stride,
+    unsigned long long *__restrict__ nvec) {
+  extern __shared__ uint32_t dyn_bm[];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (wave >= waves_per_block) return;
+  const uint32_t t = lane & 31, h = lane >> 5, q = blockIdx.y;
+  const uint32_t words = (n_c + 31) / 32;
+  uint32_t *my = dyn_bm + (size_t)wave * words;
+  for (uint32_t w = lane; w < words; w += 64) my[w] = 0;
+  pl_wave_sync();
+  const uint32_t n = in_n ? in_n[q] : n_fixed;
+  const float *Sq = S + (size_t)q * s_stride;
+  const uint8_t *iq = idx ? idx + (size_t)q * idx_stride : nullptr;
+  unsigned long long nv = 0;
+  for (uint32_t i = blockIdx.x * waves_per_block + wave; i < n; i += gridDim.x * waves_per_block) {
+    const uint32_t raw = in_ids[(size_t)q * in_stride + i];
+    const uint32_t pid = kInv ? ~raw : raw;
+    const bool ok = pid < n_doc;
+    if (pid_bm && ok && lane == 0) pid_bm[(size_t)q * bm_words + (pid >> 5)] = 0;
+    const uint32_t L = ok ? (uint32_t)doclens[pid] : 0u;
+    const int64_t off = ok ? offsets[pid] : 0;
+    float m = -9999.0f;
+    uint32_t nu = 0;
+    for (uint32_t base = 0; base < L; base += 64) {
+      const uint32_t j = base + lane;
+      uint32_t code = 0;
+      bool pass = false;
+      if (j < L) {
+        code = (uint32_t)codes[off + j];
+        pass = code < n_c && (!kIdx || iq[code]);
+      }
+      bool isnew = false;
+      if (pass) {
+        const uint32_t bit = 1u << (code & 31);
+        isnew = !(atomicOr(&my[code >> 5], bit) & bit);
+      }
+      uint64_t mask = __ballot(isnew);
+      nu += (uint32_t)__popcll(mask);
+      while (mask) {
+        const uint32_t l0 = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+        mask &= mask - 1;
+        uint32_t l1 = l0;
+        if (mask) {
+          l1 = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+          mask &= mask - 1;
+        }
+        const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)code, l0);
+        const uint32_t c1 = (uint32_t)__builtin_amdgcn_readlane((int)code, l1);
+        const uint32_t cc = h ? c1 : c0;
+        if (t < nq) m = pl_max(m, Sq[(size_t)cc * nq + t]);
+      }
+    }
+    pl_wave_sync();
+    for (uint32_t base = 0; base < L; base += 64) {
+      const uint32_t j = base + lane;
+      if (j < L) {
+        const uint32_t code = (uint32_t)codes[off + j];
+        if (code < n_c) my[code >> 5] = 0;
+      }
+    }
+    pl_wave_sync();
+    m = pl_max(m, __shfl_xor(m, 32));
+    float score = 0.0f;
+    for (uint32_t tt = 0; tt < nq; ++tt) score += pl_readlane(m, tt);
+    if (lane == 0) {
+      keys[(size_t)q * key_stride + i] = ((unsigned long long)pl_f2ord(score) << 32) | pid;
+      nv += nu;
+    }
+  }
+  if (lane == 0 && nv) atomicAdd(&nvec[q], nv * nq);
+}
+
+__global__ void k_pl_unkey(const uint32_t *__restrict__ sel, const uint32_t *__restrict__ n_sel,
+                           int32_t *__restrict__ out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_sel[0]) out[i] = (int32_t)~sel[i];
+}
+
+// ---------------------------------------------------------------------------
+// Refine (index_storage.py:164-190, CPU branch): per surviving doc, decompress
+// (x = w[bucket] + C[code]), y = x / max(||x||, 1e-12), s_t = FMA chain of
+// y . Q_t, M_t = max(0, max_i s_t), score = sum over t left to right.
+// Wave layout as the IGP rerank: lane (t, h) holds Q_t (128 floats) and the
+// wave takes 8 doc tokens per step: decode into LDS, 8 lanes run the norm
+// chains, every lane divides its 4 dims of 4 tokens, then four dot chains.
+// tab4[v] = the four weights of raw PLAID byte v (rbm, lut and bw folded).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kPlTok = 8;
+
+__global__ __launch_bounds__(256) void k_pl_refine(
+    const float *__restrict__ Q, uint32_t nq, const uint32_t *__restrict__ sel_ids,
+    const uint32_t *__restrict__ sel_n, uint32_t R, uint32_t n_slice, uint32_t n_doc,
+    const int64_t *__restrict__ doclens, const int64_t *__restrict__ offsets,
+    const int32_t *__restrict__ codes, const uint8_t *__restrict__ res, const float *__restrict__ C,
+    const float4 *__restrict__ tab4, uint32_t *__restrict__ out_ids, float *__restrict__ out_score) {
+  __shared__ __align__(16) float4 xs[4][kPlTok][kDim / 4];
+  __shared__ float dens[4][kPlTok];
+  __shared__ float Ms[4][kMaxQTok];
+  __shared__ float4 tab[256];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const uint32_t h = lane >> 5, t = lane & 31;
+  const uint32_t slice = blockIdx.x, q = blockIdx.y;
+  tab[tid] = tab4[tid];
+  __syncthreads();
+  const uint32_t n = sel_n[q];
+  float qv[kDim];
+  {
+    const float4 *q4 = reinterpret_cast<const float4 *>(Q + ((size_t)q * nq + (t < nq ? t : 0)) * kDim);
+#pragma unroll
+    for (uint32_t i = 0; i < kDim / 4; ++i) {
+      const float4 v = (t < nq) ? q4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      qv[4 * i] = v.x; qv[4 * i + 1] = v.y; qv[4 * i + 2] = v.z; qv[4 * i + 3] = v.w;
+    }
+  }
+  for (uint32_t r = slice * 4 + wave; r < n; r += n_slice * 4) {
+    const uint32_t pid = ~sel_ids[(size_t)q * R + r];
+    const uint32_t L = pid < n_doc ? (uint32_t)doclens[pid] : 0u;
+    const int64_t off = pid < n_doc ? offsets[pid] : 0;
+    float M = 0.0f;
+    for (uint32_t i = 0; i < L; i += kPlTok) {
+#pragma unroll
+      for (uint32_t c = 0; c < kPlTok / 2; ++c) {
+        const uint32_t tk = 2 * c + h;
+        if (i + tk < L) {
+          const uint64_t g = (uint64_t)off + i + tk;
+          const uint32_t code = (uint32_t)codes[g];
+          const float4 w = tab[res[g * kResBytes + t]];
+          const float4 cv = reinterpret_cast<const float4 *>(C + (size_t)code * kDim)[t];
+          xs[wave][tk][t] = make_float4(w.x + cv.x, w.y + cv.y, w.z + cv.z, w.w + cv.w);
+        }
+      }
+      pl_wave_sync();
+      if (lane < kPlTok && i + lane < L) {
+        float n2 = 0.0f;
+#pragma unroll 8
+        for (uint32_t d4 = 0; d4 < kDim / 4; ++d4) {
+          const float4 v = xs[wave][lane][d4];
+          n2 = __builtin_fmaf(v.x, v.x, n2);
+          n2 = __builtin_fmaf(v.y, v.y, n2);
+          n2 = __builtin_fmaf(v.z, v.z, n2);
+          n2 = __builtin_fmaf(v.w, v.w, n2);
+        }
+        // sqrtf and '/' compile to correctly rounded sequences (HIP's default
+        // -fhip-fp32-correctly-rounded-divide-sqrt); __fsqrt_rn does not
+        dens[wave][lane] = fmaxf(sqrtf(n2), 1e-12f);
+      }
+      pl_wave_sync();
+#pragma unroll
+      for (uint32_t c = 0; c < kPlTok / 2; ++c) {
+        const uint32_t tk = 2 * c + h;
+        if (i + tk < L) {
+          const float den = dens[wave][tk];
+          const float4 v = xs[wave][tk][t];
+          xs[wave][tk][t] = make_float4(v.x / den, v.y / den, v.z / den, v.w / den);
+        }
+      }
+      pl_wave_sync();
+      float a[kPlTok / 2];
+#pragma unroll
+      for (uint32_t c = 0; c < kPlTok / 2; ++c) a[c] = 0.f;
+#pragma unroll
+      for (uint32_t d4 = 0; d4 < kDim / 4; ++d4) {
+#pragma unroll
+        for (uint32_t c = 0; c < kPlTok / 2; ++c) {
+          const float4 xv = xs[wave][2 * c + h][d4];
+          a[c] = __builtin_fmaf(xv.x, qv[4 * d4 + 0], a[c]);
+          a[c] = __builtin_fmaf(xv.y, qv[4 * d4 + 1], a[c]);
+          a[c] = __builtin_fmaf(xv.z, qv[4 * d4 + 2], a[c]);
+          a[c] = __builtin_fmaf(xv.w, qv[4 * d4 + 3], a[c]);
+        }
+      }
+#pragma unroll
+      for (uint32_t c = 0; c < kPlTok / 2; ++c)
+        if (i + 2 * c + h < L) M = pl_max(M, a[c]);
+      pl_wave_sync();
+    }
+    M = pl_max(M, __shfl_xor(M, 32));
+    if (h == 0) Ms[wave][t] = M;
+    pl_wave_sync();
+    if (lane == 0) {
+      float sc = 0.0f;
+      for (uint32_t tt = 0; tt < nq; ++tt) sc += Ms[wave][tt];
+      out_score[(size_t)q * R + r] = sc;
+      out_ids[(size_t)q * R + r] = pid;
+    }
+    pl_wave_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+thread_local std::string g_pl_err;
+
+int pl_fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_pl_err = buf;
+  return code;
+}
+
+#define PL_TRY(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess)                                                                   \
+      return pl_fail(MVR_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
+                     __FILE__, __LINE__);                                                   \
+  } while (0)
+
+struct PBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+  hipError_t ensure(size_t bytes) {
+    if (bytes <= cap) return hipSuccess;
+    if (p) {
+      hipError_t e = hipFree(p);
+      if (e != hipSuccess) return e;
+      p = nullptr;
+      cap = 0;
+    }
+    const size_t b = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&p, b);
+    if (e == hipSuccess) cap = b;
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    cap = 0;
+  }
+  template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+inline uint32_t grid_for(uint64_t n, uint32_t block) {
+  const uint64_t g = (n + block - 1) / block;
+  return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(g, 1u << 20));
+}
+
+// waves per workgroup of k_pl_approx for a dedup bitmap of n_c bits
+inline uint32_t approx_waves(uint32_t n_c) {
+  const uint32_t bytes = ((n_c + 31) / 32) * 4;
+  return std::max(1u, std::min(4u, kPlMaxBmBytes / std::max(bytes, 4u)));
+}
+
+// Scratch of the single ops (per host thread, reallocated on a device change).
+struct OpScratch {
+  int device = -1;
+  PBuf cum, keys, seg, sel1_ids, sel1_F, sel1_n, keys2, sel2_ids, sel2_F, sel2_n, nvec;
+  hipError_t bind() {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev != device) {
+      for (PBuf *b : {&cum, &keys, &seg, &sel1_ids, &sel1_F, &sel1_n, &keys2, &sel2_ids, &sel2_F, &sel2_n,
+                      &nvec})
+        b->release();
+      device = dev;
+    }
+    return hipSuccess;
+  }
+};
+thread_local OpScratch g_ops;
+
+// Launch the two filter stages for n_q queries.  Stage-1 input: pids
+// (in_ids, in_n / n_fixed, row stride in_stride) with their S, idx rows.
+hipError_t launch_filter_stages(const uint32_t *in_ids, const uint32_t *in_n, uint32_t n_fixed,
+                                uint64_t in_stride, uint32_t in_cap, const float *S, const uint8_t *idx,
+                                uint32_t n_c, uint32_t nq, uint32_t n_doc, const int32_t *codes,
+                                const int64_t *doclens, const int64_t *offsets, uint32_t *pid_bm,
+                                uint32_t bm_words, uint32_t n_q, uint32_t ndocs, uint64_t *keys1,
+                                uint32_t *seg1, uint32_t *sel1_ids, float *sel1_F, uint32_t *sel1_n,
+                                uint64_t *keys2, uint32_t *sel2_ids, float *sel2_F, uint32_t *sel2_n,
+                                unsigned long long *nvec, hipStream_t s) {
+  const uint32_t n_seg1 = std::max(1u, (in_cap + kSeg - 1) / kSeg);
+  const uint32_t wpb = approx_waves(n_c);
+  const size_t lds = (size_t)wpb * ((n_c + 31) / 32) * 4;
+  const uint32_t gx = std::max(1u, std::min(64u, 4096u / std::max(1u, n_q)));
+  dim3 grid(gx, n_q);
+  k_pl_approx<true, false><<<grid, 256, lds, s>>>(in_ids, in_n, n_fixed, in_stride, S, (uint64_t)n_c * nq,
+                                                  idx, n_c, n_c, nq, n_doc, codes, doclens, offsets, pid_bm,
+                                                  bm_words, wpb, (unsigned long long *)keys1,
+                                                  (uint64_t)n_seg1 * kSeg, nvec);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  k_pl_segcnt<<<grid_for((uint64_t)n_q * n_seg1, 256), 256, 0, s>>>(in_n, n_fixed, n_seg1, n_q, seg1);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_select(keys1, seg1, n_seg1, nullptr, n_q, ndocs, sel1_ids, sel1_F, sel1_n, nullptr, s)) !=
+      hipSuccess)
+    return e;
+  k_pl_approx<false, true><<<grid, 256, lds, s>>>(sel1_ids, sel1_n, 0, ndocs, S, (uint64_t)n_c * nq, nullptr,
+                                                  0, n_c, nq, n_doc, codes, doclens, offsets, nullptr, 0, wpb,
+                                                  (unsigned long long *)keys2, kSeg, nvec);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return launch_select(keys2, sel1_n, 1, nullptr, n_q, ndocs / 4, sel2_ids, sel2_F, sel2_n, nullptr, s);
+}
+
+}  // namespace
+}  // namespace mvr
+
+using namespace mvr;
+
+// ---------------------------------------------------------------------------
+// Batched search handle
+// ---------------------------------------------------------------------------
+struct mvr_plaid_index {
+  int device = 0;
+  uint32_t n_doc = 0, n_c = 0;
+  uint64_t n_tok = 0, n_ivf = 0;
+  uint32_t bm_words = 0;
+  float *C = nullptr;
+  int32_t *codes = nullptr;
+  uint8_t *res = nullptr;
+  int64_t *doclens = nullptr, *offsets = nullptr;
+  int32_t *ivf = nullptr;
+  int64_t *ivf_off = nullptr, *ivf_len = nullptr;
+  float4 *tab4 = nullptr;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool bm_dirty = true;  // pid bitmap must be cleared before the next pass
+  PBuf Q, S, idx, cells, cell_n, pid_bm, cand, cand_n, keys1, seg1, sel1_ids, sel1_F, sel1_n, keys2, sel2_ids,
+      sel2_F, sel2_n, ids2, score2, out_ids, out_s, nvec;
+  std::vector<PBuf *> bufs() {
+    return {&Q, &S, &idx, &cells, &cell_n, &pid_bm, &cand, &cand_n, &keys1, &seg1, &sel1_ids, &sel1_F, &sel1_n,
+            &keys2, &sel2_ids, &sel2_F, &sel2_n, &ids2, &score2, &out_ids, &out_s, &nvec};
+  }
+};
+
+extern "C" {
+
+const char *mvr_plaid_last_error(void) { return g_pl_err.c_str(); }
+
+int mvr_plaid_decompress_residuals(const uint8_t *binary_residuals, uint64_t n, const uint16_t *bucket_weights,
+                                   const uint8_t *reversed_bit_map, const uint8_t *bucket_weight_combinations,
+                                   const int32_t *codes, const uint16_t *centroids, uint32_t dim,
+                                   uint32_t nbits, uint16_t *out, void *stream) {
+  if (!(nbits == 1 || nbits == 2 || nbits == 4 || nbits == 8))
+    return pl_fail(MVR_ERR_INVALID, "nbits %u must be 1, 2, 4 or 8", nbits);
+  if (dim == 0 || dim % 8) return pl_fail(MVR_ERR_INVALID, "dim %u must be a positive multiple of 8", dim);
+  if (n == 0) return MVR_OK;
+  if (!binary_residuals || !bucket_weights || !reversed_bit_map || !bucket_weight_combinations || !codes ||
+      !centroids || !out)
+    return pl_fail(MVR_ERR_INVALID, "NULL argument");
+  const uint32_t vpb = 8 / nbits, pk = dim / vpb;
+  const uint64_t n_bytes = n * pk;
+  k_pl_decompress_f16<<<grid_for(n_bytes, 256), 256, 0, (hipStream_t)stream>>>(
+      binary_residuals, n_bytes, pk, bucket_weights, reversed_bit_map, bucket_weight_combinations, vpb, codes,
+      centroids, 0xFFFFFFFFu, dim, out);
+  PL_TRY(hipGetLastError());
+  return MVR_OK;
+}
+
+int mvr_plaid_packbits(const uint8_t *residuals, uint64_t m, uint8_t *packed, void *stream) {
+  if (m % 32) return pl_fail(MVR_ERR_INVALID, "length %llu is not a multiple of 32", (unsigned long long)m);
+  if (m == 0) return MVR_OK;
+  if (!residuals || !packed) return pl_fail(MVR_ERR_INVALID, "NULL argument");
+  k_pl_packbits<<<grid_for(m / 8, 256), 256, 0, (hipStream_t)stream>>>(residuals, m / 8, packed);
+  PL_TRY(hipGetLastError());
+  return MVR_OK;
+}
+
+int mvr_plaid_binarize(const float *residuals, uint64_t n, uint32_t dim, const float *bucket_cutoffs,
+                       uint32_t n_cutoffs, uint32_t nbits, uint8_t *out, void *stream) {
+  if (nbits == 0 || nbits > 8) return pl_fail(MVR_ERR_INVALID, "nbits %u out of range", nbits);
+  if (dim == 0 || (dim * nbits) % 8) return pl_fail(MVR_ERR_INVALID, "dim * nbits must be a multiple of 8");
+  if (n_cutoffs >= (1u << nbits)) return pl_fail(MVR_ERR_INVALID, "%u cutoffs for %u bits", n_cutoffs, nbits);
+  if (n == 0) return MVR_OK;
+  if (!residuals || !out || (n_cutoffs && !bucket_cutoffs)) return pl_fail(MVR_ERR_INVALID, "NULL argument");
+  const uint32_t rb = dim * nbits / 8;
+  k_pl_binarize<<<grid_for(n * rb, 256), 256, 0, (hipStream_t)stream>>>(residuals, n * rb, rb, dim,
+                                                                         bucket_cutoffs, n_cutoffs, nbits, out);
+  PL_TRY(hipGetLastError());
+  return MVR_OK;
+}
+
+int mvr_plaid_lengths_cumsum(const int32_t *pids, uint32_t n, const int64_t *lengths, int64_t *cum,
+                             int64_t *total_host, void *stream) {
+  if (!lengths || !cum) return pl_fail(MVR_ERR_INVALID, "NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  k_pl_cumsum<<<1, 1024, 0, s>>>(pids, n, lengths, cum);
+  PL_TRY(hipGetLastError());
+  if (total_host) {
+    PL_TRY(hipMemcpyAsync(total_host, cum + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    PL_TRY(hipStreamSynchronize(s));
+  }
+  return MVR_OK;
+}
+
+int mvr_plaid_decompress_pids(const int32_t *pids, uint32_t npids, const int64_t *cum, const int64_t *lengths,
+                              const int64_t *offsets, const float *bucket_weights,
+                              const uint8_t *reversed_bit_map, const uint8_t *bucket_weight_combinations,
+                              const uint8_t *binary_residuals, const int32_t *codes, const float *centroids,
+                              uint32_t dim, uint32_t nbits, float *out, void *stream) {
+  if (!(nbits == 1 || nbits == 2 || nbits == 4 || nbits == 8))
+    return pl_fail(MVR_ERR_INVALID, "nbits %u must be 1, 2, 4 or 8", nbits);
+  if (dim == 0 || dim % 8) return pl_fail(MVR_ERR_INVALID, "dim %u must be a positive multiple of 8", dim);
+  if (npids == 0) return MVR_OK;
+  if (!pids || !cum || !lengths || !offsets || !bucket_weights || !reversed_bit_map ||
+      !bucket_weight_combinations || !binary_residuals || !codes || !centroids || !out)
+    return pl_fail(MVR_ERR_INVALID, "NULL argument");
+  const uint32_t vpb = 8 / nbits, pk = dim / vpb;
+  k_pl_decompress_pids<<<npids, 256, 0, (hipStream_t)stream>>>(
+      pids, cum, lengths, offsets, bucket_weights, reversed_bit_map, bucket_weight_combinations, vpb,
+      binary_residuals, pk, codes, centroids, 0xFFFFFFFFu, dim, out);
+  PL_TRY(hipGetLastError());
+  return MVR_OK;
+}
+
+int mvr_plaid_segmented_maxsim(const float *scores, uint32_t nq, const int64_t *lengths, uint32_t ndocs,
+                               float *out, void *stream) {
+  if (nq > 64) return pl_fail(MVR_ERR_UNSUPPORTED, "nq %u > 64", nq);
+  if (ndocs == 0) return MVR_OK;
+  if (!scores || !lengths || !out) return pl_fail(MVR_ERR_INVALID, "NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  PL_TRY(g_ops.bind());
+  PL_TRY(g_ops.cum.ensure(sizeof(int64_t) * ((size_t)ndocs + 1)));
+  k_pl_cumsum<<<1, 1024, 0, s>>>(nullptr, ndocs, lengths, g_ops.cum.as<int64_t>());
+  PL_TRY(hipGetLastError());
+  k_pl_segmented_maxsim<<<(ndocs + 3) / 4, 256, 0, s>>>(scores, nq, g_ops.cum.as<int64_t>(), ndocs, out);
+  PL_TRY(hipGetLastError());
+  PL_TRY(hipStreamSynchronize(s));  // the scratch cum is reused by the next call
+  return MVR_OK;
+}
+
+int mvr_plaid_filter_pids(const int32_t *pids, uint32_t npids, const float *centroid_scores, uint32_t n_centroid,
+                          uint32_t nq, const int32_t *codes, const int64_t *doclens, const int64_t *offsets,
+                          const uint8_t *idx, uint32_t ndocs, int32_t *out_pids, uint32_t *out_n,
+                          uint64_t *n_vec_score_refine, void *stream) {
+  // n_doc bound: pids are trusted to index doclens/offsets as in the reference
+  if (!out_n) return pl_fail(MVR_ERR_INVALID, "out_n is NULL");
+  *out_n = 0;
+  if (n_vec_score_refine) *n_vec_score_refine = 0;
+  if (nq == 0 || nq > kMaxQTok) return pl_fail(MVR_ERR_UNSUPPORTED, "nq %u must be 1..32", nq);
+  if (ndocs > kMaxSelect) return pl_fail(MVR_ERR_UNSUPPORTED, "ndocs %u > %u", ndocs, kMaxSelect);
+  if (n_centroid == 0 || (size_t)((n_centroid + 31) / 32) * 4 > kPlMaxBmBytes)
+    return pl_fail(MVR_ERR_UNSUPPORTED, "n_centroid %u out of range (1..%u)", n_centroid, kPlMaxBmBytes * 8);
+  if (npids == 0 || ndocs < 4) return MVR_OK;
+  if (!pids || !centroid_scores || !codes || !doclens || !offsets || !idx || !out_pids)
+    return pl_fail(MVR_ERR_INVALID, "NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  OpScratch &w = g_ops;
+  PL_TRY(w.bind());
+  const uint32_t n_seg1 = (npids + kSeg - 1) / kSeg;
+  PL_TRY(w.keys.ensure(sizeof(uint64_t) * (size_t)n_seg1 * kSeg));
+  PL_TRY(w.seg.ensure(sizeof(uint32_t) * n_seg1));
+  PL_TRY(w.sel1_ids.ensure(sizeof(uint32_t) * ndocs));
+  PL_TRY(w.sel1_F.ensure(sizeof(float) * ndocs));
+  PL_TRY(w.sel1_n.ensure(sizeof(uint32_t)));
+  PL_TRY(w.keys2.ensure(sizeof(uint64_t) * kSeg));
+  PL_TRY(w.sel2_ids.ensure(sizeof(uint32_t) * (ndocs / 4)));
+  PL_TRY(w.sel2_F.ensure(sizeof(float) * (ndocs / 4)));
+  PL_TRY(w.sel2_n.ensure(sizeof(uint32_t)));
+  PL_TRY(w.nvec.ensure(sizeof(unsigned long long)));
+  PL_TRY(launch_fill_u32(w.nvec.as<uint32_t>(), 0, 2, s));
+  PL_TRY(launch_filter_stages(reinterpret_cast<const uint32_t *>(pids), nullptr, npids, 0, npids,
+                              centroid_scores, idx, n_centroid, nq, 0x7FFFFFFFu, codes, doclens, offsets,
+                              nullptr, 0, 1, ndocs, w.keys.as<uint64_t>(), w.seg.as<uint32_t>(),
+                              w.sel1_ids.as<uint32_t>(), w.sel1_F.as<float>(), w.sel1_n.as<uint32_t>(),
+                              w.keys2.as<uint64_t>(), w.sel2_ids.as<uint32_t>(), w.sel2_F.as<float>(),
+                              w.sel2_n.as<uint32_t>(), w.nvec.as<unsigned long long>(), s));
+  k_pl_unkey<<<(ndocs / 4 + 255) / 256, 256, 0, s>>>(w.sel2_ids.as<uint32_t>(), w.sel2_n.as<uint32_t>(),
+                                                      out_pids);
+  PL_TRY(hipGetLastError());
+  uint32_t n_out = 0;
+  unsigned long long nv = 0;
+  PL_TRY(hipMemcpyAsync(&n_out, w.sel2_n.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  PL_TRY(hipMemcpyAsync(&nv, w.nvec.p, sizeof(nv), hipMemcpyDeviceToHost, s));
+  PL_TRY(hipStreamSynchronize(s));
+  *out_n = n_out;
+  if (n_vec_score_refine) *n_vec_score_refine = nv;
+  return MVR_OK;
+}
+
+int mvr_plaid_index_create(const mvr_plaid_index_desc *d, int device, mvr_plaid_index **out) {
+  if (!d || !out) return pl_fail(MVR_ERR_INVALID, "NULL argument");
+  *out = nullptr;
+  if (d->dim != kDim || d->nbits != 2)
+    return pl_fail(MVR_ERR_UNSUPPORTED, "search kernels need dim 128 and nbits 2 (got %u, %u)", d->dim, d->nbits);
+  if (d->n_doc == 0 || d->n_centroid == 0) return pl_fail(MVR_ERR_INVALID, "empty index");
+  if ((size_t)((d->n_centroid + 31) / 32) * 4 > kPlMaxBmBytes)
+    return pl_fail(MVR_ERR_UNSUPPORTED, "n_centroid %u > %u", d->n_centroid, kPlMaxBmBytes * 8);
+  if (d->n_doc > 0x7FFFFFFFu) return pl_fail(MVR_ERR_UNSUPPORTED, "n_doc too large");
+  if (!d->doclens || !d->codes || !d->residuals || !d->centroids || !d->bucket_weights || !d->ivf ||
+      !d->ivf_lengths)
+    return pl_fail(MVR_ERR_INVALID, "NULL index array");
+  std::vector<int64_t> offsets(d->n_doc), ivf_off(d->n_centroid);
+  uint64_t tot = 0;
+  for (uint32_t i = 0; i < d->n_doc; ++i) {
+    if (d->doclens[i] < 0) return pl_fail(MVR_ERR_INVALID, "negative doclen at %u", i);
+    offsets[i] = (int64_t)tot;
+    tot += (uint64_t)d->doclens[i];
+  }
+  if (tot != d->n_tok)
+    return pl_fail(MVR_ERR_INVALID, "sum of doclens %llu != n_tok %llu", (unsigned long long)tot,
+                   (unsigned long long)d->n_tok);
+  uint64_t itot = 0;
+  for (uint32_t c = 0; c < d->n_centroid; ++c) {
+    if (d->ivf_lengths[c] < 0) return pl_fail(MVR_ERR_INVALID, "negative ivf length at %u", c);
+    ivf_off[c] = (int64_t)itot;
+    itot += (uint64_t)d->ivf_lengths[c];
+  }
+  if (itot != d->n_ivf) return pl_fail(MVR_ERR_INVALID, "sum of ivf_lengths != n_ivf");
+  for (uint64_t i = 0; i < d->n_tok; ++i)
+    if (d->codes[i] < 0 || (uint32_t)d->codes[i] >= d->n_centroid)
+      return pl_fail(MVR_ERR_INVALID, "code %d at %llu out of range", d->codes[i], (unsigned long long)i);
+  for (uint64_t i = 0; i < d->n_ivf; ++i)
+    if (d->ivf[i] < 0 || (uint32_t)d->ivf[i] >= d->n_doc)
+      return pl_fail(MVR_ERR_INVALID, "ivf pid %d at %llu out of range", d->ivf[i], (unsigned long long)i);
+  // raw PLAID byte v -> weights of its four dims: reversed_bit_map swaps the
+  // two bits of each 2-bit group (residual.py:51-72) and the product table
+  // reads the groups MSB first (residual.py:76-91)
+  std::vector<float> tab(256 * 4);
+  for (uint32_t v = 0; v < 256; ++v)
+    for (uint32_t k = 0; k < 4; ++k) {
+      const uint32_t g = (v >> (6 - 2 * k)) & 3u;
+      const uint32_t b = ((g & 1u) << 1) | (g >> 1);
+      tab[v * 4 + k] = d->bucket_weights[b];
+    }
+
+  int n_dev = 0;
+  PL_TRY(hipGetDeviceCount(&n_dev));
+  if (device < 0 || device >= n_dev) return pl_fail(MVR_ERR_INVALID, "device %d out of range (%d)", device, n_dev);
+  PL_TRY(hipSetDevice(device));
+  auto *ix = new mvr_plaid_index();
+  ix->device = device;
+  ix->n_doc = d->n_doc;
+  ix->n_c = d->n_centroid;
+  ix->n_tok = d->n_tok;
+  ix->n_ivf = d->n_ivf;
+  ix->bm_words = (d->n_doc + 31) / 32;
+  auto bail = [&](int code) {
+    mvr_plaid_index_free(ix);
+    return code;
+  };
+  auto up = [&](void **dst, const void *src, size_t bytes) -> hipError_t {
+    hipError_t e = hipMalloc(dst, std::max<size_t>(bytes, 256));
+    if (e != hipSuccess) return e;
+    return bytes ? hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+  };
+  hipError_t e = hipSuccess;
+  if ((e = up((void **)&ix->C, d->centroids, sizeof(float) * (size_t)d->n_centroid * kDim)) != hipSuccess ||
+      (e = up((void **)&ix->codes, d->codes, sizeof(int32_t) * d->n_tok)) != hipSuccess ||
+      (e = up((void **)&ix->res, d->residuals, (size_t)kResBytes * d->n_tok)) != hipSuccess ||
+      (e = up((void **)&ix->doclens, d->doclens, sizeof(int64_t) * d->n_doc)) != hipSuccess ||
+      (e = up((void **)&ix->offsets, offsets.data(), sizeof(int64_t) * d->n_doc)) != hipSuccess ||
+      (e = up((void **)&ix->ivf, d->ivf, sizeof(int32_t) * d->n_ivf)) != hipSuccess ||
+      (e = up((void **)&ix->ivf_off, ivf_off.data(), sizeof(int64_t) * d->n_centroid)) != hipSuccess ||
+      (e = up((void **)&ix->ivf_len, d->ivf_lengths, sizeof(int64_t) * d->n_centroid)) != hipSuccess ||
+      (e = up((void **)&ix->tab4, tab.data(), sizeof(float) * tab.size())) != hipSuccess ||
+      (e = hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking)) != hipSuccess)
+    return bail(pl_fail(MVR_ERR_DEVICE, "index upload failed: %s", hipGetErrorString(e)));
+  for (auto &ev : ix->ev)
+    if ((e = hipEventCreate(&ev)) != hipSuccess)
+      return bail(pl_fail(MVR_ERR_DEVICE, "event create failed: %s", hipGetErrorString(e)));
+  *out = ix;
+  return MVR_OK;
+}
+
+void mvr_plaid_index_free(mvr_plaid_index *ix) {
+  if (!ix) return;
+  (void)hipSetDevice(ix->device);
+  if (ix->stream) (void)hipStreamSynchronize(ix->stream);
+  for (void *p : {(void *)ix->C, (void *)ix->codes, (void *)ix->res, (void *)ix->doclens, (void *)ix->offsets,
+                  (void *)ix->ivf, (void *)ix->ivf_off, (void *)ix->ivf_len, (void *)ix->tab4})
+    if (p) (void)hipFree(p);
+  for (PBuf *b : ix->bufs()) b->release();
+  for (auto &ev : ix->ev)
+    if (ev) (void)hipEventDestroy(ev);
+  if (ix->stream) (void)hipStreamDestroy(ix->stream);
+  delete ix;
+}
+
+int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t nq, const mvr_plaid_params *p,
+                     int32_t *out_ids, float *out_scores, int q_on_device, uint64_t *stats,
+                     mvr_plaid_timings *timings) {
+  if (!ix || !p || !out_ids || !out_scores) return pl_fail(MVR_ERR_INVALID, "NULL argument");
+  if (timings) *timings = mvr_plaid_timings{};
+  if (n_q == 0) return MVR_OK;
+  if (!Q) return pl_fail(MVR_ERR_INVALID, "Q is NULL");
+  if (nq == 0 || nq > kMaxQTok) return pl_fail(MVR_ERR_UNSUPPORTED, "nq %u must be 1..32", nq);
+  if (p->k == 0) return pl_fail(MVR_ERR_INVALID, "k must be positive");
+  if (p->ncells == 0 || p->ncells > kPlMaxCells)
+    return pl_fail(MVR_ERR_UNSUPPORTED, "ncells %u must be 1..%u", p->ncells, kPlMaxCells);
+  if (p->ndocs < 4 || p->ndocs > kMaxSelect)
+    return pl_fail(MVR_ERR_UNSUPPORTED, "ndocs %u must be 4..%u", p->ndocs, kMaxSelect);
+  if (p->k > kMaxSelect) return pl_fail(MVR_ERR_UNSUPPORTED, "k %u > %u", p->k, kMaxSelect);
+  PL_TRY(hipSetDevice(ix->device));
+  hipStream_t s = ix->stream;
+  const uint32_t n_c = ix->n_c, n_doc = ix->n_doc, ndocs = p->ndocs, R2 = ndocs / 4, k = p->k;
+  const uint32_t cell_cap = nq * p->ncells;
+  const uint32_t n_seg1 = (n_doc + kSeg - 1) / kSeg;
+  // device bytes per query in a pass
+  const size_t per_q = sizeof(float) * kDim * nq + sizeof(float) * (size_t)n_c * nq + n_c +
+                       sizeof(uint32_t) * (cell_cap + 1) + sizeof(uint32_t) * ix->bm_words +
+                       sizeof(uint32_t) * (n_doc + 1) + sizeof(uint64_t) * (size_t)n_seg1 * kSeg +
+                       sizeof(uint32_t) * n_seg1 + 12 * (size_t)ndocs + 8 * (size_t)kSeg + 16 * (size_t)R2 +
+                       8 * (size_t)k + 16;
+  uint32_t B = p->batch ? p->batch : 1024;
+  B = std::min<uint32_t>(B, n_q);
+  B = (uint32_t)std::max<size_t>(1, std::min<size_t>(B, kPoolBudget / per_q));
+  PL_TRY(ix->Q.ensure(sizeof(float) * (size_t)B * nq * kDim));
+  PL_TRY(ix->S.ensure(sizeof(float) * (size_t)B * n_c * nq));
+  PL_TRY(ix->idx.ensure((size_t)B * n_c));
+  PL_TRY(ix->cells.ensure(sizeof(uint32_t) * (size_t)B * cell_cap));
+  PL_TRY(ix->cell_n.ensure(sizeof(uint32_t) * B));
+  if (ix->pid_bm.cap < sizeof(uint32_t) * (size_t)B * ix->bm_words) ix->bm_dirty = true;
+  PL_TRY(ix->pid_bm.ensure(sizeof(uint32_t) * (size_t)B * ix->bm_words));
+  PL_TRY(ix->cand.ensure(sizeof(uint32_t) * (size_t)B * n_doc));
+  PL_TRY(ix->cand_n.ensure(sizeof(uint32_t) * B));
+  PL_TRY(ix->keys1.ensure(sizeof(uint64_t) * (size_t)B * n_seg1 * kSeg));
+  PL_TRY(ix->seg1.ensure(sizeof(uint32_t) * (size_t)B * n_seg1));
+  PL_TRY(ix->sel1_ids.ensure(sizeof(uint32_t) * (size_t)B * ndocs));
+  PL_TRY(ix->sel1_F.ensure(sizeof(float) * (size_t)B * ndocs));
+  PL_TRY(ix->sel1_n.ensure(sizeof(uint32_t) * B));
+  PL_TRY(ix->keys2.ensure(sizeof(uint64_t) * (size_t)B * kSeg));
+  PL_TRY(ix->sel2_ids.ensure(sizeof(uint32_t) * (size_t)B * R2));
+  PL_TRY(ix->sel2_F.ensure(sizeof(float) * (size_t)B * R2));
+  PL_TRY(ix->sel2_n.ensure(sizeof(uint32_t) * B));
+  PL_TRY(ix->ids2.ensure(sizeof(uint32_t) * (size_t)B * R2));
+  PL_TRY(ix->score2.ensure(sizeof(float) * (size_t)B * R2));
+  PL_TRY(ix->out_ids.ensure(sizeof(uint32_t) * (size_t)B * k));
+  PL_TRY(ix->out_s.ensure(sizeof(float) * (size_t)B * k));
+  PL_TRY(ix->nvec.ensure(sizeof(unsigned long long) * B));
+  if (ix->bm_dirty) {
+    PL_TRY(launch_fill_u32(ix->pid_bm.as<uint32_t>(), 0, ix->pid_bm.cap / 4, s));
+    ix->bm_dirty = false;
+  }
+  std::vector<uint32_t> h_cand(B), h_fin(B);
+  std::vector<unsigned long long> h_nv(B);
+  double t_cand = 0, t_filt = 0, t_ref = 0;
+  auto t0 = std::chrono::steady_clock::now();
+  for (uint32_t q0 = 0; q0 < n_q; q0 += B) {
+    const uint32_t b = std::min(B, n_q - q0);
+    const float *Qb = Q + (size_t)q0 * nq * kDim;
+    PL_TRY(hipMemcpyAsync(ix->Q.p, Qb, sizeof(float) * (size_t)b * nq * kDim,
+                          q_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    // a failed pass leaves pid bits set; the next call clears the bitmap
+    ix->bm_dirty = true;
+    PL_TRY(hipEventRecord(ix->ev[0], s));
+    k_pl_cscores<<<dim3((n_c + 63) / 64, b), 256, 0, s>>>(ix->Q.as<float>(), nq, ix->C, n_c,
+                                                          p->centroid_score_threshold, ix->S.as<float>(),
+                                                          ix->idx.as<uint8_t>());
+    PL_TRY(hipGetLastError());
+    const size_t cell_lds = sizeof(uint32_t) * ((n_c + 31) / 32);
+    switch (p->ncells) {
+#define PL_CELLS(NC)                                                                                       \
+  case NC:                                                                                                 \
+    k_pl_cells<NC><<<b, 64, cell_lds, s>>>(ix->S.as<float>(), n_c, nq, ix->cells.as<uint32_t>(), cell_cap, \
+                                           ix->cell_n.as<uint32_t>());                                     \
+    break;
+      PL_CELLS(1) PL_CELLS(2) PL_CELLS(3) PL_CELLS(4) PL_CELLS(5) PL_CELLS(6) PL_CELLS(7) PL_CELLS(8)
+#undef PL_CELLS
+    }
+    PL_TRY(hipGetLastError());
+    PL_TRY(launch_fill_u32(ix->cand_n.as<uint32_t>(), 0, b, s));
+    PL_TRY(launch_fill_u32(ix->nvec.as<uint32_t>(), 0, 2 * (size_t)b, s));
+    {
+      const uint32_t gx = std::max(1u, std::min(16u, 2048u / b));
+      k_pl_candidates<<<dim3(gx, b), 256, 0, s>>>(ix->cells.as<uint32_t>(), cell_cap, ix->cell_n.as<uint32_t>(),
+                                                  ix->ivf, ix->ivf_off, ix->ivf_len, ix->pid_bm.as<uint32_t>(),
+                                                  ix->bm_words, ix->cand.as<uint32_t>(), n_doc,
+                                                  ix->cand_n.as<uint32_t>());
+      PL_TRY(hipGetLastError());
+    }
+    PL_TRY(hipEventRecord(ix->ev[1], s));
+    PL_TRY(launch_filter_stages(ix->cand.as<uint32_t>(), ix->cand_n.as<uint32_t>(), 0, n_doc, n_doc,
+                                ix->S.as<float>(), ix->idx.as<uint8_t>(), n_c, nq, n_doc, ix->codes, ix->doclens,
+                                ix->offsets, ix->pid_bm.as<uint32_t>(), ix->bm_words, b, ndocs,
+                                ix->keys1.as<uint64_t>(), ix->seg1.as<uint32_t>(), ix->sel1_ids.as<uint32_t>(),
+                                ix->sel1_F.as<float>(), ix->sel1_n.as<uint32_t>(), ix->keys2.as<uint64_t>(),
+                                ix->sel2_ids.as<uint32_t>(), ix->sel2_F.as<float>(), ix->sel2_n.as<uint32_t>(),
+                                ix->nvec.as<unsigned long long>(), s));
+    PL_TRY(hipEventRecord(ix->ev[2], s));
+    {
+      const uint32_t n_slice = std::max(1u, std::min((R2 + 31) / 32, std::max(4u, 8192u / b)));
+      k_pl_refine<<<dim3(n_slice, b), 256, 0, s>>>(ix->Q.as<float>(), nq, ix->sel2_ids.as<uint32_t>(),
+                                                   ix->sel2_n.as<uint32_t>(), R2, n_slice, n_doc, ix->doclens,
+                                                   ix->offsets, ix->codes, ix->res, ix->C, ix->tab4,
+                                                   ix->ids2.as<uint32_t>(), ix->score2.as<float>());
+      PL_TRY(hipGetLastError());
+    }
+    PL_TRY(launch_topk(ix->ids2.as<uint32_t>(), ix->score2.as<float>(), ix->sel2_n.as<uint32_t>(), R2, b, k,
+                       ix->out_ids.as<uint32_t>(), ix->out_s.as<float>(), nullptr, s));
+    PL_TRY(hipEventRecord(ix->ev[3], s));
+    PL_TRY(hipMemcpyAsync(out_ids + (size_t)q0 * k, ix->out_ids.p, sizeof(int32_t) * (size_t)b * k,
+                          hipMemcpyDeviceToHost, s));
+    PL_TRY(hipMemcpyAsync(out_scores + (size_t)q0 * k, ix->out_s.p, sizeof(float) * (size_t)b * k,
+                          hipMemcpyDeviceToHost, s));
+    PL_TRY(hipMemcpyAsync(h_cand.data(), ix->cand_n.p, sizeof(uint32_t) * b, hipMemcpyDeviceToHost, s));
+    PL_TRY(hipMemcpyAsync(h_fin.data(), ix->sel2_n.p, sizeof(uint32_t) * b, hipMemcpyDeviceToHost, s));
+    PL_TRY(hipMemcpyAsync(h_nv.data(), ix->nvec.p, sizeof(unsigned long long) * b, hipMemcpyDeviceToHost, s));
+    PL_TRY(hipStreamSynchronize(s));
+    ix->bm_dirty = false;  // stage 1 cleared every word it set
+    float ms = 0;
+    PL_TRY(hipEventElapsedTime(&ms, ix->ev[0], ix->ev[1]));
+    t_cand += ms;
+    PL_TRY(hipEventElapsedTime(&ms, ix->ev[1], ix->ev[2]));
+    t_filt += ms;
+    PL_TRY(hipEventElapsedTime(&ms, ix->ev[2], ix->ev[3]));
+    t_ref += ms;
+    for (uint32_t i = 0; i < b; ++i) {
+      if (stats) {
+        stats[(size_t)(q0 + i) * 3 + 0] = h_cand[i];
+        stats[(size_t)(q0 + i) * 3 + 1] = h_fin[i];
+        stats[(size_t)(q0 + i) * 3 + 2] = h_nv[i];
+      }
+      if (timings) {
+        timings->n_refine_ivf += h_cand[i];
+        timings->n_refine_filter += h_fin[i];
+        timings->n_vec_score_refine += h_nv[i];
+      }
+    }
+  }
+  if (timings) {
+    timings->candidates_ms = t_cand;
+    timings->filter_ms = t_filt;
+    timings->refine_ms = t_ref;
+    timings->total_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  return MVR_OK;
+}
+
+}  // extern "C"

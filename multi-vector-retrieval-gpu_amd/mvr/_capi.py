@@ -1,4 +1,4 @@
-"""ctypes binding of libmvr (include/mvr.h).  No fallback: if the library is
+"""ctypes binding of libmvr (include/mvr.h, include/mvr_plaid.h).  No fallback: if the library is
 missing or fails to load, every call raises."""
 from __future__ import annotations
 
@@ -48,7 +48,28 @@ class Timings(C.Structure):
                 ("n_pass", C.c_uint32), ("rerank_pairs", C.c_uint64)]
 
 
-# Every symbol declared in include/mvr.h, with its ctypes signature.
+class PlaidIndexDesc(C.Structure):
+    _fields_ = [("n_doc", C.c_uint32), ("n_centroid", C.c_uint32), ("dim", C.c_uint32), ("nbits", C.c_uint32),
+                ("n_tok", C.c_uint64), ("n_ivf", C.c_uint64), ("doclens", C.c_void_p), ("codes", C.c_void_p),
+                ("residuals", C.c_void_p), ("centroids", C.c_void_p), ("bucket_weights", C.c_void_p),
+                ("ivf", C.c_void_p), ("ivf_lengths", C.c_void_p)]
+
+
+class PlaidParams(C.Structure):
+    _fields_ = [("k", C.c_uint32), ("ncells", C.c_uint32), ("ndocs", C.c_uint32), ("batch", C.c_uint32),
+                ("centroid_score_threshold", C.c_float)]
+
+
+class PlaidTimings(C.Structure):
+    _fields_ = [("candidates_ms", C.c_double), ("filter_ms", C.c_double), ("refine_ms", C.c_double),
+                ("total_ms", C.c_double), ("n_refine_ivf", C.c_uint64), ("n_refine_filter", C.c_uint64),
+                ("n_vec_score_refine", C.c_uint64)]
+
+
+_vp, _u32, _u64 = C.c_void_p, C.c_uint32, C.c_uint64
+
+# Every symbol declared in include/mvr.h and include/mvr_plaid.h, with its
+# ctypes signature.
 SIGNATURES = {
     "mvr_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "mvr_last_error": (C.c_char_p, []),
@@ -88,6 +109,21 @@ SIGNATURES = {
                                   C.c_uint32, C.c_void_p, C.c_void_p]),
     "mvr_compress": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32,
                                C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    # include/mvr_plaid.h
+    "mvr_plaid_last_error": (C.c_char_p, []),
+    "mvr_plaid_decompress_residuals": (C.c_int, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp, _vp]),
+    "mvr_plaid_packbits": (C.c_int, [_vp, _u64, _vp, _vp]),
+    "mvr_plaid_binarize": (C.c_int, [_vp, _u64, _u32, _vp, _u32, _u32, _vp, _vp]),
+    "mvr_plaid_lengths_cumsum": (C.c_int, [_vp, _u32, _vp, _vp, C.POINTER(C.c_int64), _vp]),
+    "mvr_plaid_decompress_pids": (C.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32,
+                                            _vp, _vp]),
+    "mvr_plaid_segmented_maxsim": (C.c_int, [_vp, _u32, _vp, _u32, _vp, _vp]),
+    "mvr_plaid_filter_pids": (C.c_int, [_vp, _u32, _vp, _u32, _u32, _vp, _vp, _vp, _vp, _u32, _vp,
+                                        C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), _vp]),
+    "mvr_plaid_index_create": (C.c_int, [C.POINTER(PlaidIndexDesc), C.c_int, C.POINTER(C.c_void_p)]),
+    "mvr_plaid_index_free": (None, [_vp]),
+    "mvr_plaid_search": (C.c_int, [_vp, _vp, _u32, _u32, C.POINTER(PlaidParams), _vp, _vp, C.c_int, _vp,
+                                   C.POINTER(PlaidTimings)]),
 }
 
 _lib = None
@@ -120,6 +156,11 @@ def lib():
 def check(status: int):
     if status != MVR_OK:
         raise MvrError(status, lib().mvr_last_error().decode())
+
+
+def check_plaid(status: int):
+    if status != MVR_OK:
+        raise MvrError(status, lib().mvr_plaid_last_error().decode())
 
 
 def check_mlp(status: int):

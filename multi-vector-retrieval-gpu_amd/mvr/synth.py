@@ -283,3 +283,51 @@ def topic_queries(torch, corpus, seed, n_q, T=32, noise_lo=0.05, noise_hi=0.4, p
 def make_queries(torch, seed, centers, n_q, T=32, noise=0.15):
     Q = synth_tokens(torch, seed, STREAM_QCOMP, STREAM_QNOISE, centers, n_q * T, noise)
     return Q.view(n_q, T, 128)
+
+
+@dataclass
+class PlaidArrays:
+    """Host arrays of a PLAID index (the layout IndexScorer loads)."""
+    doclens: np.ndarray         # int64 [n_doc]
+    codes: np.ndarray           # int32 [n_tok]
+    residuals: np.ndarray       # uint8 [n_tok, 32], PLAID bit order
+    centroids: np.ndarray       # float32 [n_c, 128] (float16 values, as centroids.pt stores them)
+    bucket_cutoffs: np.ndarray  # float32 [3]
+    bucket_weights: np.ndarray  # float32 [4] (float16 values, as buckets.pt stores them)
+    ivf: np.ndarray             # int32 [n_ivf]
+    ivf_lengths: np.ndarray     # int64 [n_c]
+
+    def index_kwargs(self):
+        return dict(doclens=self.doclens, codes=self.codes, residuals=self.residuals, centroids=self.centroids,
+                    bucket_weights=self.bucket_weights, ivf=self.ivf, ivf_lengths=self.ivf_lengths)
+
+
+def plaid_from_corpus(torch, corpus: DeviceCorpus, sample=1 << 17, chunk=1 << 22, log=print) -> PlaidArrays:
+    """A PLAID index of a device corpus's embeddings with its centroids and
+    codes: centroids rounded to float16 (centroids.pt), residuals against
+    them binarized on the GPU with pooled cutoffs / weights at the quantiles
+    i/4 and (i + 1/2)/4 of a residual sample (collection_indexer.py:320-345),
+    the pid IVF of ivf.pid.pt (indexing/utils.py:8-53)."""
+    from . import plaid
+    X, n_tok = corpus.X, corpus.X.shape[0]
+    C16 = corpus.C.half().float().contiguous()
+    codes = corpus.codes
+    sel = (torch.arange(min(sample, n_tok), device=X.device, dtype=torch.int64) * n_tok) // min(sample, n_tok)
+    E = (X[sel] - C16[codes[sel].long()]).reshape(-1)
+    E = E[torch.randperm(E.numel(), device=E.device, generator=torch.Generator(device=E.device).manual_seed(0))
+          [: 1 << 23]]
+    qs = torch.arange(0, 4, device=E.device, dtype=torch.float32) / 4
+    cut = E.quantile(qs[1:]).float()
+    w = E.quantile(qs + 0.125).half().float()
+    residuals = torch.empty((n_tok, 32), dtype=torch.uint8, device=X.device)
+    for t0 in range(0, n_tok, chunk):
+        n = min(chunk, n_tok - t0)
+        e = X[t0:t0 + n] - C16[codes[t0:t0 + n].long()]
+        residuals[t0:t0 + n] = plaid.binarize(e, cut, 2)
+    torch.cuda.synchronize()
+    off, sizes, ids = corpus.ivf
+    log(f"plaid: {n_tok} tokens binarized, cutoffs {cut.tolist()}, weights {w.tolist()}")
+    return PlaidArrays(doclens=corpus.doclens.astype(np.int64), codes=codes.cpu().numpy().astype(np.int32),
+                       residuals=residuals.cpu().numpy(), centroids=C16.cpu().numpy(),
+                       bucket_cutoffs=cut.cpu().numpy(), bucket_weights=w.cpu().numpy(),
+                       ivf=ids.cpu().numpy().astype(np.int32), ivf_lengths=sizes.cpu().numpy().astype(np.int64))

@@ -56,3 +56,28 @@ extern "C" int probe_cvt_fp8(const float *x, uint32_t *out, uint32_t n_pairs) {
   hipFree(dout);
   return e != hipSuccess;
 }
+
+// Single-precision square root and division: the default operators (what the
+// PLAID refine kernel uses) next to the __fsqrt_rn / __fdiv_rn intrinsics.
+__global__ void k_probe_divsqrt(const float *a, const float *b, float *out, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  out[4 * i + 0] = __fsqrt_rn(a[i]);
+  out[4 * i + 1] = __fdiv_rn(a[i], b[i]);
+  out[4 * i + 2] = sqrtf(a[i]);
+  out[4 * i + 3] = a[i] / b[i];
+}
+
+extern "C" int probe_divsqrt(const float *a, const float *b, float *out, uint32_t n) {
+  float *da, *db, *dout;
+  if (hipMalloc((void **)&da, n * 4) || hipMalloc((void **)&db, n * 4) || hipMalloc((void **)&dout, n * 16))
+    return 1;
+  hipMemcpy(da, a, n * 4, hipMemcpyHostToDevice);
+  hipMemcpy(db, b, n * 4, hipMemcpyHostToDevice);
+  k_probe_divsqrt<<<(n + 255) / 256, 256>>>(da, db, dout, n);
+  hipError_t e = hipMemcpy(out, dout, (size_t)n * 16, hipMemcpyDeviceToHost);
+  hipFree(da);
+  hipFree(db);
+  hipFree(dout);
+  return e != hipSuccess;
+}

@@ -1,6 +1,7 @@
-"""The C-ABI library loads and exports every symbol include/mvr.h declares
+"""The C-ABI library loads and exports every symbol include/*.h declares
 (no compute calls: this runs without a GPU)."""
 import ctypes
+import glob
 import os
 import re
 
@@ -8,7 +9,7 @@ from conftest import PKG, ROOT
 
 
 def _declared():
-    src = open(os.path.join(ROOT, "include", "mvr.h")).read()
+    src = "".join(open(p).read() for p in sorted(glob.glob(os.path.join(ROOT, "include", "*.h"))))
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     return sorted(set(re.findall(r"\b(mvr_[a-z_0-9]+)\s*\(", src)))
 

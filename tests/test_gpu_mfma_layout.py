@@ -6,6 +6,8 @@ Assumed (v_mfma_f32_32x32x16_{bf16,fp8_fp8}):
   B: lane l holds B[8*(l // 32) + e][l % 32]
   D: lane l, element j holds D[8*(j // 4) + 4*(l // 32) + j % 4][l % 32]
 FP8: OCP e4m3 (bias 7, 1.0 = 0x38).
+FP32 sqrtf and '/' in device code are correctly rounded (the PLAID refine
+kernel's normalisation relies on it; __fsqrt_rn is not, on this toolchain).
 """
 import ctypes
 import os
@@ -25,6 +27,8 @@ def probe():
     lib.probe_mfma.restype = ctypes.c_int
     lib.probe_cvt_fp8.argtypes = [vp, vp, ctypes.c_uint32]
     lib.probe_cvt_fp8.restype = ctypes.c_int
+    lib.probe_divsqrt.argtypes = [vp, vp, vp, ctypes.c_uint32]
+    lib.probe_divsqrt.restype = ctypes.c_int
     return lib
 
 
@@ -88,3 +92,15 @@ def test_fp8_32x32x16_layout(probe):
     d = np.zeros((64, 16), np.float32)
     assert probe.probe_mfma(1, a.ctypes.data, b.ctypes.data, d.ctypes.data) == 0
     np.testing.assert_array_equal(unpack_d(d), A @ B)
+
+
+def test_fp32_sqrt_and_division_correctly_rounded(probe):
+    rng = np.random.default_rng(0)
+    n = 1 << 18
+    a = (np.abs(rng.standard_normal(n)) * np.exp2(rng.integers(-20, 20, n))).astype(np.float32)
+    b = ((rng.standard_normal(n) + 0.01) * np.exp2(rng.integers(-10, 10, n))).astype(np.float32)
+    a[:4] = [0.0, 1e-40, 2.0, 3.4e38]        # zero, subnormal, exact, large
+    out = np.empty((n, 4), np.float32)
+    assert probe.probe_divsqrt(a.ctypes.data, b.ctypes.data, out.ctypes.data, n) == 0
+    np.testing.assert_array_equal(out[:, 2].view(np.uint32), np.sqrt(a).view(np.uint32))
+    np.testing.assert_array_equal(out[:, 3].view(np.uint32), (a / b).view(np.uint32))

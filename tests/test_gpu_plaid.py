@@ -1,0 +1,235 @@
+"""GPU parity of the PLAID path (mvr.plaid, libmvr's mvr_plaid_* C ABI)
+against the CPU oracle (oracle/plaid_oracle.c), on the lotte-500 PLAID index
+fixture and on synthetic PLAID indices.
+
+Every op is integer/byte work or a fixed-order FP32 computation, so the bar
+is bit equality: FP16 decompression, packbits, binarize, FP32 decompression,
+segmented MaxSim, the two-stage filter (pids, order, n_vec_score_refine) and
+the batched search (pids, scores, per-query counts).
+"""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lotte500.npz")
+
+
+@pytest.fixture(scope="module")
+def fx():
+    return dict(np.load(FIX))
+
+
+@pytest.fixture(scope="module")
+def opix(fx):
+    from oracle import plaid as P
+    return P.PlaidIndex(fx["doclens"], fx["codes"], fx["residuals"], fx["centroids"].astype(np.float32),
+                        fx["bucket_weights"].astype(np.float32), fx["ivf"], fx["ivf_lengths"])
+
+
+@pytest.fixture(scope="module")
+def gpix(fx):
+    from mvr import plaid
+    ix = plaid.PlaidIndex(fx["doclens"], fx["codes"], fx["residuals"], fx["centroids"].astype(np.float32),
+                          fx["bucket_weights"].astype(np.float32), fx["ivf"], fx["ivf_lengths"])
+    yield ix
+    ix.close()
+
+
+def _synthetic_plaid(seed, n_doc, n_c, lo=4, hi=90):
+    """A PLAID index over random unit vectors: codes by inner-product argmax,
+    residual bytes by the oracle's binarize with pooled quantile cutoffs."""
+    from oracle import oracle as O, plaid as P
+    rng = np.random.default_rng(seed)
+    doclens = rng.integers(lo, hi, n_doc).astype(np.int64)
+    n_tok = int(doclens.sum())
+    X = rng.standard_normal((n_tok, 128)).astype(np.float32)
+    X /= np.linalg.norm(X, axis=1, keepdims=True)
+    C_ = X[rng.choice(n_tok, n_c, replace=False)].copy()
+    codes = O.assign_ip(X, C_).astype(np.int32)
+    E = X - C_[codes]
+    cut = np.quantile(E, [0.25, 0.5, 0.75]).astype(np.float32)
+    w = np.quantile(E, [0.125, 0.375, 0.625, 0.875]).astype(np.float32)
+    res = P.binarize(E, cut, 2)
+    ivf_sizes = []
+    ivf = []
+    pid_of = np.repeat(np.arange(n_doc), doclens)
+    for c in range(n_c):
+        p = np.unique(pid_of[codes == c])
+        ivf.append(p)
+        ivf_sizes.append(p.size)
+    ivf = np.concatenate(ivf).astype(np.int32)
+    return dict(doclens=doclens, codes=codes, residuals=res, centroids=C_, bucket_weights=w, ivf=ivf,
+                ivf_lengths=np.array(ivf_sizes, np.int64), X=X)
+
+
+# ------------------------------------------------------------- codec ops
+def test_decompress_residuals_cuda_bitexact_on_fixture(fx):
+    import torch
+    from mvr import plaid
+    from oracle import plaid as P
+    rbm, lut = P.reversed_bit_map(2), P.decompression_lookup_table(2)
+    got = plaid.decompress_residuals_cuda(torch.from_numpy(fx["residuals"]).cuda(),
+                                          torch.from_numpy(fx["bucket_weights"]).cuda(),
+                                          torch.from_numpy(rbm).cuda(), torch.from_numpy(lut).cuda(),
+                                          torch.from_numpy(fx["codes"]).cuda(),
+                                          torch.from_numpy(fx["centroids"]).cuda(), 128, 2)
+    torch.cuda.synchronize()
+    ref = P.decompress_f16(fx["residuals"], fx["bucket_weights"].view(np.uint16), rbm, lut, fx["codes"],
+                           fx["centroids"].view(np.uint16))
+    np.testing.assert_array_equal(got.cpu().numpy().view(np.uint16), ref)
+
+
+@pytest.mark.parametrize("nbits", [1, 4])
+def test_decompress_residuals_cuda_other_widths(nbits):
+    import torch
+    from mvr import plaid
+    from oracle import plaid as P
+    rng = np.random.default_rng(nbits)
+    n, dim, n_c = 777, 64, 50
+    res = rng.integers(0, 256, (n, dim * nbits // 8)).astype(np.uint8)
+    bw = rng.standard_normal(1 << nbits).astype(np.float16)
+    C16 = rng.standard_normal((n_c, dim)).astype(np.float16)
+    codes = rng.integers(0, n_c, n).astype(np.int32)
+    rbm, lut = P.reversed_bit_map(nbits), P.decompression_lookup_table(nbits)
+    got = plaid.decompress_residuals_cuda(res, bw, rbm, lut, codes, C16, dim, nbits)
+    ref = P.decompress_f16(res, bw.view(np.uint16), rbm, lut, codes, C16.view(np.uint16), dim, nbits)
+    np.testing.assert_array_equal(got.cpu().numpy().view(np.uint16), ref)
+
+
+def test_packbits_cuda_matches_numpy():
+    from mvr import plaid
+    rng = np.random.default_rng(7)
+    bits = rng.integers(0, 2, 32 * 12345).astype(np.uint8)
+    bits[rng.random(bits.size) < 0.05] = 200
+    got = plaid.packbits_cuda(bits).cpu().numpy()
+    np.testing.assert_array_equal(got, np.packbits(bits != 0))
+
+
+@pytest.mark.parametrize("nbits", [1, 2, 4])
+def test_binarize_bitexact(nbits):
+    from mvr import plaid
+    from oracle import plaid as P
+    rng = np.random.default_rng(10 + nbits)
+    n_cut = (1 << nbits) - 1
+    cut = np.sort(rng.standard_normal(n_cut).astype(np.float32) * 0.05)
+    e = (rng.standard_normal((2000, 128)) * 0.05).astype(np.float32)
+    e[0, :n_cut] = cut                       # exactly on the cutoffs (bucketize right=False)
+    got = plaid.binarize(e, cut, nbits).cpu().numpy()
+    np.testing.assert_array_equal(got, P.binarize(e, cut, nbits))
+
+
+def test_decompress_pids_bitexact_on_fixture(opix):
+    from mvr import plaid
+    from oracle import plaid as P
+    rng = np.random.default_rng(3)
+    pids = rng.integers(0, opix.n_doc, 300).astype(np.int32)   # any order, repeats
+    got = plaid.decompress_residuals(pids, opix.doclens, opix.offsets, opix.bucket_weights, opix.rbm, opix.lut,
+                                     opix.residuals, opix.codes, opix.centroids, 128, 2).cpu().numpy()
+    ref = P.decompress_pids(pids, opix.doclens, opix.offsets, opix.bucket_weights, opix.rbm, opix.lut,
+                            opix.residuals, opix.codes, opix.centroids)
+    np.testing.assert_array_equal(got, ref)
+
+
+@pytest.mark.parametrize("nq", [32, 20, 64])
+def test_segmented_maxsim_bitexact(nq):
+    from mvr import plaid
+    from oracle import plaid as P
+    rng = np.random.default_rng(nq)
+    lengths = rng.integers(0, 40, 500).astype(np.int64)
+    lengths[:3] = [0, 1, 0]
+    s = rng.standard_normal((int(lengths.sum()), nq)).astype(np.float32)
+    got = plaid.segmented_maxsim(s, lengths).cpu().numpy()
+    np.testing.assert_array_equal(got, P.segmented_maxsim(s, lengths))
+
+
+# ------------------------------------------------------------ filter_pids
+def test_filter_pids_bitexact_on_fixture_queries(fx, opix):
+    from mvr import plaid
+    from oracle import plaid as P
+    for qi in range(4):
+        S = P.centroid_scores(opix.centroids, fx["queries"][qi])
+        for thr, ndocs in ((0.5, 256), (0.45, 1024), (0.3, 64), (0.5, 4)):
+            idx = (S.max(1) >= thr).astype(np.uint8)
+            pids = np.arange(opix.n_doc, dtype=np.int32)
+            got, nv = plaid.filter_pids(pids, S, opix.codes, opix.doclens, opix.offsets, idx.astype(bool), ndocs)
+            ref, _, nv_ref = P.filter_pids(pids, S, opix.codes, opix.doclens, opix.offsets, idx, ndocs)
+            np.testing.assert_array_equal(got.cpu().numpy(), ref)
+            assert nv == nv_ref
+
+
+def test_filter_pids_ties_and_short_input():
+    from mvr import plaid
+    from oracle import plaid as P
+    rng = np.random.default_rng(11)
+    n_c, nq, n_doc = 64, 8, 3000
+    doclens = rng.integers(0, 12, n_doc).astype(np.int64)
+    offsets = np.concatenate([[0], np.cumsum(doclens)[:-1]]).astype(np.int64)
+    codes = rng.integers(0, n_c, int(doclens.sum())).astype(np.int32)
+    S = (rng.integers(0, 3, (n_c, nq)) * 0.25).astype(np.float32)     # many equal scores
+    idx = (rng.random(n_c) < 0.6).astype(np.uint8)
+    for npids, ndocs in ((2500, 1000), (100, 1000), (2999, 4096), (9, 8)):
+        pids = np.sort(rng.choice(n_doc, npids, replace=False)).astype(np.int32)
+        got, nv = plaid.filter_pids(pids, S, codes, doclens, offsets, idx.astype(bool), ndocs)
+        ref, _, nv_ref = P.filter_pids(pids, S, codes, doclens, offsets, idx, ndocs)
+        np.testing.assert_array_equal(got.cpu().numpy(), ref)
+        assert nv == nv_ref
+
+
+# -------------------------------------------------------- batched search
+@pytest.mark.parametrize("k", [10, 100, 1000])
+def test_search_bitexact_on_fixture(fx, opix, gpix, k):
+    from oracle import plaid as P
+    Q = fx["queries"]
+    ncells, thr, ndocs = P.searcher_defaults(k)
+    ids, sc, st = gpix.search(Q, k=k, return_stats=True)
+    rid, rsc, rst = opix.search(Q, k, ncells, thr, ndocs)
+    np.testing.assert_array_equal(ids, rid)
+    np.testing.assert_array_equal(sc, rsc)
+    np.testing.assert_array_equal(st, rst)
+
+
+def test_search_batches_and_nq_below_32(fx, opix, gpix):
+    from oracle import plaid as P
+    Q = np.ascontiguousarray(fx["queries"][:, :20])
+    a = gpix.search(Q, k=10, batch=3, return_stats=True)
+    b = gpix.search(Q, k=10, batch=1024, return_stats=True)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    rid, rsc, rst = opix.search(Q, 10, *P.searcher_defaults(10))
+    np.testing.assert_array_equal(a[0], rid)
+    np.testing.assert_array_equal(a[1], rsc)
+    np.testing.assert_array_equal(a[2], rst)
+
+
+def test_search_bitexact_on_synthetic_index():
+    from mvr import plaid
+    from oracle import plaid as P
+    d = _synthetic_plaid(5, 3000, 512)
+    X = d.pop("X")
+    o = P.PlaidIndex(**d)
+    g = plaid.PlaidIndex(**d)
+    rng = np.random.default_rng(9)
+    # queries: noisy copies of doc tokens so the filter has real work
+    Q = X[rng.choice(X.shape[0], 70 * 32)].reshape(70, 32, 128)
+    Q = Q + 0.3 * rng.standard_normal(Q.shape).astype(np.float32)
+    Q = (Q / np.linalg.norm(Q, axis=2, keepdims=True)).astype(np.float32)
+    for k, ncells, thr, ndocs in ((10, 1, 0.5, 256), (100, 2, 0.45, 1024), (10, 4, 0.2, 4096), (5, 3, 0.9, 64)):
+        ids, sc, st = g.search(Q, k=k, ncells=ncells, centroid_score_threshold=thr, ndocs=ndocs, batch=32,
+                               return_stats=True)
+        rid, rsc, rst = o.search(Q, k, ncells, thr, ndocs)
+        np.testing.assert_array_equal(ids, rid)
+        np.testing.assert_array_equal(sc, rsc)
+        np.testing.assert_array_equal(st, rst)
+    g.close()
+
+
+def test_rank_single_query_matches_search(fx, gpix):
+    import torch
+    Q = fx["queries"][:1]
+    pids, scores = gpix.rank(torch.from_numpy(Q).cuda(), k=10)
+    ids, sc = gpix.search(Q, k=10)
+    assert pids == ids[0].tolist() and scores == sc[0].tolist()
+    assert len(pids) == 10 and scores == sorted(scores, reverse=True)
