@@ -8,7 +8,8 @@
 //   search/decompress_residuals.cpp:27-155        -> k_pl_decompress_pids
 //   modeling/segmented_maxsim.cpp:22-93           -> k_pl_segmented_maxsim
 //   search/filter_pids.cpp:30-193                 -> k_pl_approx (both stages) + k_select_topr
-//   search/candidate_generation.py:13-66          -> k_pl_cscores, k_pl_cells, k_pl_candidates
+//   search/candidate_generation.py:13-66          -> k_centroid_topk_f32<2> (mvr_kernels.hip),
+//                                                    k_pl_cells, k_pl_candidates
 //   search/index_storage.py:101-198 (CPU branch)  -> the batched pipeline of mvr_plaid_search
 //
 // Design: the reference runs one query at a time through pthread loops
@@ -208,122 +209,21 @@ __global__ __launch_bounds__(256) void k_pl_segmented_maxsim(const float *__rest
 // ---------------------------------------------------------------------------
 // Candidate generation (candidate_generation.py:13-66)
 // ---------------------------------------------------------------------------
-// S[q][c][t] = FMA chain over d of C[c][d] * Q[q][t][d]; idx[q][c] =
-// max_t S >= threshold.  Workgroup = 64 centroids of one query; thread
-// (cc, tg) computes 8 tokens of centroid cc.
-__global__ __launch_bounds__(256) void k_pl_cscores(const float *__restrict__ Q, uint32_t nq,
-                                                    const float *__restrict__ C, uint32_t n_c, float thr,
-                                                    float *__restrict__ S, uint8_t *__restrict__ idx) {
-  __shared__ __align__(16) float Qs[kDim][kMaxQTok];
-  const uint32_t tid = threadIdx.x, q = blockIdx.y;
-  for (uint32_t e = tid; e < kDim * kMaxQTok; e += 256) {
-    const uint32_t t = e / kDim, d = e % kDim;
-    Qs[d][t] = t < nq ? Q[((size_t)q * nq + t) * kDim + d] : 0.0f;
-  }
-  __syncthreads();
-  const uint32_t cc = tid >> 2, tg = tid & 3, c = blockIdx.x * 64 + cc;
-  float acc[8];
-#pragma unroll
-  for (uint32_t e = 0; e < 8; ++e) acc[e] = 0.0f;
-  if (c < n_c) {
-    const float4 *crow = reinterpret_cast<const float4 *>(C + (size_t)c * kDim);
-#pragma unroll 4
-    for (uint32_t d4 = 0; d4 < kDim / 4; ++d4) {
-      const float4 cv = crow[d4];
-      const float cx[4] = {cv.x, cv.y, cv.z, cv.w};
-#pragma unroll
-      for (uint32_t u = 0; u < 4; ++u) {
-        const float4 qa = *reinterpret_cast<const float4 *>(&Qs[4 * d4 + u][tg * 8]);
-        const float4 qb = *reinterpret_cast<const float4 *>(&Qs[4 * d4 + u][tg * 8 + 4]);
-        acc[0] = __builtin_fmaf(cx[u], qa.x, acc[0]);
-        acc[1] = __builtin_fmaf(cx[u], qa.y, acc[1]);
-        acc[2] = __builtin_fmaf(cx[u], qa.z, acc[2]);
-        acc[3] = __builtin_fmaf(cx[u], qa.w, acc[3]);
-        acc[4] = __builtin_fmaf(cx[u], qb.x, acc[4]);
-        acc[5] = __builtin_fmaf(cx[u], qb.y, acc[5]);
-        acc[6] = __builtin_fmaf(cx[u], qb.z, acc[6]);
-        acc[7] = __builtin_fmaf(cx[u], qb.w, acc[7]);
-      }
-    }
-  }
-  float m = -INFINITY;
-  if (c < n_c) {
-    float *srow = S + ((size_t)q * n_c + c) * nq;
-#pragma unroll
-    for (uint32_t e = 0; e < 8; ++e) {
-      const uint32_t t = tg * 8 + e;
-      if (t < nq) {
-        srow[t] = acc[e];
-        m = fmaxf(m, acc[e]);
-      }
-    }
-  }
-  m = fmaxf(m, __shfl_xor(m, 1));
-  m = fmaxf(m, __shfl_xor(m, 2));
-  if (c < n_c && tg == 0) idx[(size_t)q * n_c + c] = m >= thr ? 1 : 0;
-}
-
-__device__ __forceinline__ bool pl_better(float s1, uint32_t i1, float s2, uint32_t i2) {
-  return (s1 > s2) || (s1 == s2 && i1 < i2);
-}
-
-// Per query token the best NC centroids by (score desc, centroid asc), their
-// union as an ascending cell list.  One wave per query: lane (t, h) scans the
-// centroids c = h (mod 2) of token t, then the two halves merge.
-template <uint32_t NC>
-__global__ __launch_bounds__(64) void k_pl_cells(const float *__restrict__ S, uint32_t n_c, uint32_t nq,
-                                                 uint32_t *__restrict__ cells, uint32_t cell_cap,
-                                                 uint32_t *__restrict__ cell_n) {
+// The union of the query's per-token top-ncells lists (from the centroid
+// kernel) as an ascending cell list; one wave per query, an LDS bitmap of the
+// n_c centroids.
+__global__ __launch_bounds__(64) void k_pl_cells(const uint32_t *__restrict__ top_ids, uint32_t P,
+                                                 uint32_t nq, uint32_t n_c, uint32_t *__restrict__ cells,
+                                                 uint32_t cell_cap, uint32_t *__restrict__ cell_n) {
   extern __shared__ uint32_t bm[];
   const uint32_t lane = threadIdx.x, q = blockIdx.x;
-  const uint32_t t = lane & 31, h = lane >> 5;
   const uint32_t words = (n_c + 31) / 32;
   for (uint32_t w = lane; w < words; w += 64) bm[w] = 0;
-  float bs[NC];
-  uint32_t bi[NC];
-#pragma unroll
-  for (uint32_t j = 0; j < NC; ++j) { bs[j] = -INFINITY; bi[j] = 0xFFFFFFFFu; }
-  if (t < nq) {
-    const float *Sq = S + (size_t)q * n_c * nq + t;
-    for (uint32_t c = h; c < n_c; c += 2) {
-      float cv = Sq[(size_t)c * nq];
-      uint32_t ci = c;
-      if (pl_better(cv, ci, bs[NC - 1], bi[NC - 1])) {
-#pragma unroll
-        for (uint32_t j = 0; j < NC; ++j) {
-          if (pl_better(cv, ci, bs[j], bi[j])) {
-            const float ts = bs[j]; const uint32_t ti = bi[j];
-            bs[j] = cv; bi[j] = ci; cv = ts; ci = ti;
-          }
-        }
-      }
-    }
-  }
-  // merge the odd-centroid list of lane t + 32 into lane t's
-  float os[NC];
-  uint32_t oi[NC];
-#pragma unroll
-  for (uint32_t j = 0; j < NC; ++j) {
-    os[j] = __shfl(bs[j], (int)(t + 32));
-    oi[j] = (uint32_t)__shfl((int)bi[j], (int)(t + 32));
-  }
-#pragma unroll
-  for (uint32_t j = 0; j < NC; ++j) {
-    float cv = os[j];
-    uint32_t ci = oi[j];
-#pragma unroll
-    for (uint32_t u = 0; u < NC; ++u) {
-      if (pl_better(cv, ci, bs[u], bi[u])) {
-        const float ts = bs[u]; const uint32_t ti = bi[u];
-        bs[u] = cv; bi[u] = ci; cv = ts; ci = ti;
-      }
-    }
-  }
   pl_wave_sync();
-  if (h == 0 && t < nq) {
-#pragma unroll
-    for (uint32_t j = 0; j < NC; ++j)
-      if (bi[j] < n_c) atomicOr(&bm[bi[j] >> 5], 1u << (bi[j] & 31));
+  for (uint32_t e = lane; e < nq * P; e += 64) {
+    const uint32_t t = e / P, j = e - t * P;
+    const uint32_t c = top_ids[((size_t)q * kMaxQTok + t) * P + j];
+    if (c < n_c) atomicOr(&bm[c >> 5], 1u << (c & 31));
   }
   pl_wave_sync();
   uint32_t total = 0;
@@ -411,7 +311,8 @@ __global__ void k_pl_segcnt(const uint32_t *__restrict__ n_in, uint32_t n_fixed,
 template <bool kIdx, bool kInv>
 __global__ __launch_bounds__(256) void k_pl_approx(
     const uint32_t *__restrict__ in_ids, const uint32_t *__restrict__ in_n, uint32_t n_fixed,
-    uint64_t in_stride, const float *__restrict__ S, uint64_t s_stride, const uint8_t *__restrict__ idx,
+    uint64_t in_stride, const uint32_t *__restrict__ slots, const uint32_t *__restrict__ slot_n,
+    const float *__restrict__ S, uint64_t s_stride, const uint8_t *__restrict__ idx,
     uint64_t idx_stride, uint32_t n_c, uint32_t nq, uint32_t n_doc, const int32_t *__restrict__ codes,
     const int64_t *__restrict__ doclens, const int64_t *__restrict__ offsets, uint32_t *__restrict__ pid_bm,
     uint32_t bm_words, uint32_t waves_per_block, unsigned long long *__restrict__ keys, uint64_t key_stride,
@@ -424,11 +325,13 @@ __global__ __launch_bounds__(256) void k_pl_approx(
   uint32_t *my = dyn_bm + (size_t)wave * words;
   for (uint32_t w = lane; w < words; w += 64) my[w] = 0;
   pl_wave_sync();
-  const uint32_t n = in_n ? in_n[q] : n_fixed;
+  // with slots: visit only the listed input positions (slots[q][j], j < slot_n[q])
+  const uint32_t n = slots ? slot_n[q] : (in_n ? in_n[q] : n_fixed);
   const float *Sq = S + (size_t)q * s_stride;
   const uint8_t *iq = idx ? idx + (size_t)q * idx_stride : nullptr;
   unsigned long long nv = 0;
-  for (uint32_t i = blockIdx.x * waves_per_block + wave; i < n; i += gridDim.x * waves_per_block) {
+  for (uint32_t jj = blockIdx.x * waves_per_block + wave; jj < n; jj += gridDim.x * waves_per_block) {
+    const uint32_t i = slots ? slots[(size_t)q * in_stride + jj] : jj;
     const uint32_t raw = in_ids[(size_t)q * in_stride + i];
     const uint32_t pid = kInv ? ~raw : raw;
     const bool ok = pid < n_doc;
@@ -436,7 +339,7 @@ __global__ __launch_bounds__(256) void k_pl_approx(
     const uint32_t L = ok ? (uint32_t)doclens[pid] : 0u;
     const int64_t off = ok ? offsets[pid] : 0;
     float m = -9999.0f;
-    uint32_t nu = 0;
+    uint32_t nu = 0;  // distinct passing codes (wave-uniform)
     for (uint32_t base = 0; base < L; base += 64) {
       const uint32_t j = base + lane;
       uint32_t code = 0;
@@ -466,24 +369,127 @@ __global__ __launch_bounds__(256) void k_pl_approx(
         if (t < nq) m = pl_max(m, Sq[(size_t)cc * nq + t]);
       }
     }
-    pl_wave_sync();
-    for (uint32_t base = 0; base < L; base += 64) {
-      const uint32_t j = base + lane;
-      if (j < L) {
-        const uint32_t code = (uint32_t)codes[off + j];
-        if (code < n_c) my[code >> 5] = 0;
+    // a doc without passing codes keeps every maximum at -9999: its score is
+    // the exact sum -9999 * nq (integers below 2^24), and no bit was set
+    float score = -9999.0f * (float)nq;
+    if (nu) {
+      pl_wave_sync();
+      for (uint32_t base = 0; base < L; base += 64) {
+        const uint32_t j = base + lane;
+        if (j < L) {
+          const uint32_t code = (uint32_t)codes[off + j];
+          if (code < n_c) my[code >> 5] = 0;
+        }
       }
+      pl_wave_sync();
+      m = pl_max(m, __shfl_xor(m, 32));
+      score = 0.0f;
+      for (uint32_t tt = 0; tt < nq; ++tt) score += pl_readlane(m, tt);
     }
-    pl_wave_sync();
-    m = pl_max(m, __shfl_xor(m, 32));
-    float score = 0.0f;
-    for (uint32_t tt = 0; tt < nq; ++tt) score += pl_readlane(m, tt);
     if (lane == 0) {
       keys[(size_t)q * key_stride + i] = ((unsigned long long)pl_f2ord(score) << 32) | pid;
       nv += nu;
     }
   }
   if (lane == 0 && nv) atomicAdd(&nvec[q], nv * nq);
+}
+
+// Docs holding an idx-passing code are exactly the docs in the IVF lists of
+// the passing centroids (a doc is in IVF[c] iff one of its codes is c).  One
+// workgroup per query lists the passing centroids and, when their lists hold
+// fewer entries than the query's candidates hold tokens, marks those docs in
+// pass_bm (pmode = 1); otherwise every candidate is scanned (pmode = 0).
+constexpr uint32_t kPassCap = 1024;
+
+__global__ __launch_bounds__(256) void k_pl_passmark(
+    const uint8_t *__restrict__ idx, uint32_t n_c, const int32_t *__restrict__ ivf,
+    const int64_t *__restrict__ ivf_off, const int64_t *__restrict__ ivf_len, const uint32_t *__restrict__ cand_n,
+    float avg_len, uint32_t *__restrict__ pass_list, uint32_t *__restrict__ pass_cnt, uint32_t *__restrict__ pmode,
+    uint32_t *__restrict__ pass_bm, uint32_t bm_words) {
+  __shared__ uint32_t s_list[kPassCap];
+  __shared__ uint32_t s_n;
+  __shared__ unsigned long long s_len;
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, q = blockIdx.x;
+  if (tid == 0) { s_n = 0; s_len = 0; }
+  __syncthreads();
+  const uint8_t *iq = idx + (size_t)q * n_c;
+  for (uint32_t c = tid; c < n_c; c += 256) {
+    if (iq[c]) {
+      const uint32_t slot = atomicAdd(&s_n, 1u);
+      if (slot < kPassCap) s_list[slot] = c;
+      atomicAdd(&s_len, (unsigned long long)ivf_len[c]);
+    }
+  }
+  __syncthreads();
+  const uint32_t n = s_n;
+  const bool use = n <= kPassCap && (double)s_len <= (double)cand_n[q] * (double)avg_len;
+  if (tid == 0) {
+    pmode[q] = use ? 1u : 0u;
+    pass_cnt[q] = use ? n : 0u;
+  }
+  if (!use) return;
+  for (uint32_t i = tid; i < n; i += 256) pass_list[(size_t)q * kPassCap + i] = s_list[i];
+  uint32_t *bq = pass_bm + (size_t)q * bm_words;
+  for (uint32_t ci = wave; ci < n; ci += 4) {
+    const uint32_t c = s_list[ci];
+    const int64_t off = ivf_off[c], len = ivf_len[c];
+    for (int64_t j = lane; j < len; j += 64) {
+      const uint32_t pid = (uint32_t)ivf[off + j];
+      atomicOr(&bq[pid >> 5], 1u << (pid & 31));
+    }
+  }
+}
+
+// Zeroes the words k_pl_passmark set.
+__global__ __launch_bounds__(256) void k_pl_passclear(const uint32_t *__restrict__ pass_list,
+                                                      const uint32_t *__restrict__ pass_cnt,
+                                                      const int32_t *__restrict__ ivf,
+                                                      const int64_t *__restrict__ ivf_off,
+                                                      const int64_t *__restrict__ ivf_len,
+                                                      uint32_t *__restrict__ pass_bm, uint32_t bm_words) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = blockIdx.x;
+  const uint32_t n = pass_cnt[q];
+  uint32_t *bq = pass_bm + (size_t)q * bm_words;
+  for (uint32_t ci = wave; ci < n; ci += 4) {
+    const uint32_t c = pass_list[(size_t)q * kPassCap + ci];
+    const int64_t off = ivf_off[c], len = ivf_len[c];
+    for (int64_t j = lane; j < len; j += 64) bq[(uint32_t)ivf[off + j] >> 5] = 0;
+  }
+}
+
+// Stage-1 split, one thread per candidate: clears the candidate's word of
+// pid_bm; a candidate without a passing code gets its final key at once
+// (score -9999 * nq, exact in FP32 for nq <= 32), the others are listed in
+// heavy[q] for the wave scan.
+__global__ __launch_bounds__(256) void k_pl_split(
+    const uint32_t *__restrict__ cand, uint64_t cand_stride, const uint32_t *__restrict__ cand_n,
+    const uint32_t *__restrict__ pmode, const uint32_t *__restrict__ pass_bm, uint32_t *__restrict__ pid_bm,
+    uint32_t bm_words, uint32_t nq, unsigned long long *__restrict__ keys, uint64_t key_stride,
+    uint32_t *__restrict__ heavy, uint32_t *__restrict__ heavy_n) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, q = blockIdx.y;
+  const uint32_t n = cand_n[q];
+  const bool mark = pmode[q] != 0;
+  const unsigned long long ck = (unsigned long long)pl_f2ord(-9999.0f * (float)nq) << 32;
+  const uint32_t *bq = pass_bm + (size_t)q * bm_words;
+  for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+    const uint32_t i = base + tid;
+    const bool v = i < n;
+    uint32_t pid = 0;
+    bool hv = false;
+    if (v) {
+      pid = cand[(size_t)q * cand_stride + i];
+      pid_bm[(size_t)q * bm_words + (pid >> 5)] = 0;
+      hv = !mark || ((bq[pid >> 5] >> (pid & 31)) & 1u);
+      if (!hv) keys[(size_t)q * key_stride + i] = ck | pid;
+    }
+    const uint64_t m = __ballot(hv);
+    if (m) {
+      uint32_t b0 = 0;
+      if (lane == 0) b0 = atomicAdd(&heavy_n[q], (uint32_t)__popcll(m));
+      b0 = (uint32_t)__shfl((int)b0, 0);
+      if (hv) heavy[(size_t)q * cand_stride + b0 + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+    }
+  }
 }
 
 __global__ void k_pl_unkey(const uint32_t *__restrict__ sel, const uint32_t *__restrict__ n_sel,
@@ -521,7 +527,7 @@ __global__ __launch_bounds__(256) void k_pl_refine(
   const uint32_t n = sel_n[q];
   float qv[kDim];
   {
-    const float4 *q4 = reinterpret_cast<const float4 *>(Q + ((size_t)q * nq + (t < nq ? t : 0)) * kDim);
+    const float4 *q4 = reinterpret_cast<const float4 *>(Q + ((size_t)q * kMaxQTok + (t < nq ? t : 0)) * kDim);
 #pragma unroll
     for (uint32_t i = 0; i < kDim / 4; ++i) {
       const float4 v = (t < nq) ? q4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -682,8 +688,11 @@ thread_local OpScratch g_ops;
 
 // Launch the two filter stages for n_q queries.  Stage-1 input: pids
 // (in_ids, in_n / n_fixed, row stride in_stride) with their S, idx rows.
+// slots / slot_n (nullable): stage 1 scans only those input positions; the
+// keys of the others are already in keys1 (k_pl_split).
 hipError_t launch_filter_stages(const uint32_t *in_ids, const uint32_t *in_n, uint32_t n_fixed,
-                                uint64_t in_stride, uint32_t in_cap, const float *S, const uint8_t *idx,
+                                uint64_t in_stride, uint32_t in_cap, const uint32_t *slots,
+                                const uint32_t *slot_n, const float *S, const uint8_t *idx,
                                 uint32_t n_c, uint32_t nq, uint32_t n_doc, const int32_t *codes,
                                 const int64_t *doclens, const int64_t *offsets, uint32_t *pid_bm,
                                 uint32_t bm_words, uint32_t n_q, uint32_t ndocs, uint64_t *keys1,
@@ -695,7 +704,8 @@ hipError_t launch_filter_stages(const uint32_t *in_ids, const uint32_t *in_n, ui
   const size_t lds = (size_t)wpb * ((n_c + 31) / 32) * 4;
   const uint32_t gx = std::max(1u, std::min(64u, 4096u / std::max(1u, n_q)));
   dim3 grid(gx, n_q);
-  k_pl_approx<true, false><<<grid, 256, lds, s>>>(in_ids, in_n, n_fixed, in_stride, S, (uint64_t)n_c * nq,
+  k_pl_approx<true, false><<<grid, 256, lds, s>>>(in_ids, in_n, n_fixed, in_stride, slots, slot_n, S,
+                                                  (uint64_t)n_c * nq,
                                                   idx, n_c, n_c, nq, n_doc, codes, doclens, offsets, pid_bm,
                                                   bm_words, wpb, (unsigned long long *)keys1,
                                                   (uint64_t)n_seg1 * kSeg, nvec);
@@ -706,7 +716,8 @@ hipError_t launch_filter_stages(const uint32_t *in_ids, const uint32_t *in_n, ui
   if ((e = launch_select(keys1, seg1, n_seg1, nullptr, n_q, ndocs, sel1_ids, sel1_F, sel1_n, nullptr, s)) !=
       hipSuccess)
     return e;
-  k_pl_approx<false, true><<<grid, 256, lds, s>>>(sel1_ids, sel1_n, 0, ndocs, S, (uint64_t)n_c * nq, nullptr,
+  k_pl_approx<false, true><<<grid, 256, lds, s>>>(sel1_ids, sel1_n, 0, ndocs, nullptr, nullptr, S,
+                                                  (uint64_t)n_c * nq, nullptr,
                                                   0, n_c, nq, n_doc, codes, doclens, offsets, nullptr, 0, wpb,
                                                   (unsigned long long *)keys2, kSeg, nvec);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -736,10 +747,13 @@ struct mvr_plaid_index {
   hipStream_t stream = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   bool bm_dirty = true;  // pid bitmap must be cleared before the next pass
+  uint32_t q_nq = 0;     // query rows last copied into the padded Q buffer
+  float avg_len = 0.0f;  // mean doc length (tokens)
+  PBuf part_ids, part_s, top_ids, top_s, pass_list, pass_cnt, pmode, pass_bm, heavy, heavy_n;
   PBuf Q, S, idx, cells, cell_n, pid_bm, cand, cand_n, keys1, seg1, sel1_ids, sel1_F, sel1_n, keys2, sel2_ids,
       sel2_F, sel2_n, ids2, score2, out_ids, out_s, nvec;
   std::vector<PBuf *> bufs() {
-    return {&Q, &S, &idx, &cells, &cell_n, &pid_bm, &cand, &cand_n, &keys1, &seg1, &sel1_ids, &sel1_F, &sel1_n,
+    return {&part_ids, &part_s, &top_ids, &top_s, &pass_list, &pass_cnt, &pmode, &pass_bm, &heavy, &heavy_n, &Q, &S, &idx, &cells, &cell_n, &pid_bm, &cand, &cand_n, &keys1, &seg1, &sel1_ids, &sel1_F, &sel1_n,
             &keys2, &sel2_ids, &sel2_F, &sel2_n, &ids2, &score2, &out_ids, &out_s, &nvec};
   }
 };
@@ -870,8 +884,8 @@ int mvr_plaid_filter_pids(const int32_t *pids, uint32_t npids, const float *cent
   PL_TRY(w.sel2_n.ensure(sizeof(uint32_t)));
   PL_TRY(w.nvec.ensure(sizeof(unsigned long long)));
   PL_TRY(launch_fill_u32(w.nvec.as<uint32_t>(), 0, 2, s));
-  PL_TRY(launch_filter_stages(reinterpret_cast<const uint32_t *>(pids), nullptr, npids, 0, npids,
-                              centroid_scores, idx, n_centroid, nq, 0x7FFFFFFFu, codes, doclens, offsets,
+  PL_TRY(launch_filter_stages(reinterpret_cast<const uint32_t *>(pids), nullptr, npids, 0, npids, nullptr,
+                              nullptr, centroid_scores, idx, n_centroid, nq, 0x7FFFFFFFu, codes, doclens, offsets,
                               nullptr, 0, 1, ndocs, w.keys.as<uint64_t>(), w.seg.as<uint32_t>(),
                               w.sel1_ids.as<uint32_t>(), w.sel1_F.as<float>(), w.sel1_n.as<uint32_t>(),
                               w.keys2.as<uint64_t>(), w.sel2_ids.as<uint32_t>(), w.sel2_F.as<float>(),
@@ -921,9 +935,26 @@ int mvr_plaid_index_create(const mvr_plaid_index_desc *d, int device, mvr_plaid_
   for (uint64_t i = 0; i < d->n_tok; ++i)
     if (d->codes[i] < 0 || (uint32_t)d->codes[i] >= d->n_centroid)
       return pl_fail(MVR_ERR_INVALID, "code %d at %llu out of range", d->codes[i], (unsigned long long)i);
-  for (uint64_t i = 0; i < d->n_ivf; ++i)
-    if (d->ivf[i] < 0 || (uint32_t)d->ivf[i] >= d->n_doc)
-      return pl_fail(MVR_ERR_INVALID, "ivf pid %d at %llu out of range", d->ivf[i], (unsigned long long)i);
+  // The search relies on IVF[c] being exactly the docs that hold code c, in
+  // ascending order (ivf.pid.pt, indexing/utils.py:8-53): check it.
+  {
+    std::vector<int64_t> last(d->n_centroid, -1), fill(d->n_centroid, 0);
+    uint64_t tok = 0;
+    for (uint32_t doc = 0; doc < d->n_doc; ++doc) {
+      for (int64_t j = 0; j < d->doclens[doc]; ++j, ++tok) {
+        const uint32_t c = (uint32_t)d->codes[tok];
+        if (last[c] == (int64_t)doc) continue;
+        last[c] = doc;
+        if (fill[c] >= d->ivf_lengths[c] || d->ivf[ivf_off[c] + fill[c]] != (int32_t)doc)
+          return pl_fail(MVR_ERR_INVALID, "ivf list of centroid %u does not hold exactly its docs (doc %u)", c, doc);
+        ++fill[c];
+      }
+    }
+    for (uint32_t c = 0; c < d->n_centroid; ++c)
+      if (fill[c] != d->ivf_lengths[c])
+        return pl_fail(MVR_ERR_INVALID, "ivf list of centroid %u has %lld entries, its docs %lld", c,
+                       (long long)d->ivf_lengths[c], (long long)fill[c]);
+  }
   // raw PLAID byte v -> weights of its four dims: reversed_bit_map swaps the
   // two bits of each 2-bit group (residual.py:51-72) and the product table
   // reads the groups MSB first (residual.py:76-91)
@@ -946,6 +977,7 @@ int mvr_plaid_index_create(const mvr_plaid_index_desc *d, int device, mvr_plaid_
   ix->n_tok = d->n_tok;
   ix->n_ivf = d->n_ivf;
   ix->bm_words = (d->n_doc + 31) / 32;
+  ix->avg_len = (float)((double)d->n_tok / (double)d->n_doc);
   auto bail = [&](int code) {
     mvr_plaid_index_free(ix);
     return code;
@@ -1008,7 +1040,9 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
   const uint32_t cell_cap = nq * p->ncells;
   const uint32_t n_seg1 = (n_doc + kSeg - 1) / kSeg;
   // device bytes per query in a pass
-  const size_t per_q = sizeof(float) * kDim * nq + sizeof(float) * (size_t)n_c * nq + n_c +
+  const size_t per_q = sizeof(float) * kDim * kMaxQTok + sizeof(float) * (size_t)n_c * nq + n_c +
+                       (kPlaidMaxSplit + 1) * 8 * (size_t)kMaxQTok * p->ncells +
+                       sizeof(uint32_t) * ((size_t)n_doc + ix->bm_words + kPassCap + 3) +
                        sizeof(uint32_t) * (cell_cap + 1) + sizeof(uint32_t) * ix->bm_words +
                        sizeof(uint32_t) * (n_doc + 1) + sizeof(uint64_t) * (size_t)n_seg1 * kSeg +
                        sizeof(uint32_t) * n_seg1 + 12 * (size_t)ndocs + 8 * (size_t)kSeg + 16 * (size_t)R2 +
@@ -1016,13 +1050,33 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
   uint32_t B = p->batch ? p->batch : 1024;
   B = std::min<uint32_t>(B, n_q);
   B = (uint32_t)std::max<size_t>(1, std::min<size_t>(B, kPoolBudget / per_q));
-  PL_TRY(ix->Q.ensure(sizeof(float) * (size_t)B * nq * kDim));
+  if (ix->Q.cap < sizeof(float) * (size_t)B * kMaxQTok * kDim) {
+    PL_TRY(ix->Q.ensure(sizeof(float) * (size_t)B * kMaxQTok * kDim));
+    PL_TRY(launch_fill_u32(ix->Q.as<uint32_t>(), 0, ix->Q.cap / 4, s));
+    ix->q_nq = kMaxQTok;
+  }
+  if (nq != ix->q_nq) {  // padding rows of an earlier, wider batch
+    PL_TRY(launch_fill_u32(ix->Q.as<uint32_t>(), 0, ix->Q.cap / 4, s));
+    ix->q_nq = nq;
+  }
+  PL_TRY(ix->part_ids.ensure(sizeof(uint32_t) * (size_t)kPlaidMaxSplit * B * kMaxQTok * p->ncells));
+  PL_TRY(ix->part_s.ensure(sizeof(float) * (size_t)kPlaidMaxSplit * B * kMaxQTok * p->ncells));
+  PL_TRY(ix->top_ids.ensure(sizeof(uint32_t) * (size_t)B * kMaxQTok * p->ncells));
+  PL_TRY(ix->top_s.ensure(sizeof(float) * (size_t)B * kMaxQTok * p->ncells));
   PL_TRY(ix->S.ensure(sizeof(float) * (size_t)B * n_c * nq));
   PL_TRY(ix->idx.ensure((size_t)B * n_c));
   PL_TRY(ix->cells.ensure(sizeof(uint32_t) * (size_t)B * cell_cap));
   PL_TRY(ix->cell_n.ensure(sizeof(uint32_t) * B));
-  if (ix->pid_bm.cap < sizeof(uint32_t) * (size_t)B * ix->bm_words) ix->bm_dirty = true;
+  if (ix->pid_bm.cap < sizeof(uint32_t) * (size_t)B * ix->bm_words ||
+      ix->pass_bm.cap < sizeof(uint32_t) * (size_t)B * ix->bm_words)
+    ix->bm_dirty = true;
   PL_TRY(ix->pid_bm.ensure(sizeof(uint32_t) * (size_t)B * ix->bm_words));
+  PL_TRY(ix->pass_bm.ensure(sizeof(uint32_t) * (size_t)B * ix->bm_words));
+  PL_TRY(ix->pass_list.ensure(sizeof(uint32_t) * (size_t)B * kPassCap));
+  PL_TRY(ix->pass_cnt.ensure(sizeof(uint32_t) * B));
+  PL_TRY(ix->pmode.ensure(sizeof(uint32_t) * B));
+  PL_TRY(ix->heavy.ensure(sizeof(uint32_t) * (size_t)B * n_doc));
+  PL_TRY(ix->heavy_n.ensure(sizeof(uint32_t) * B));
   PL_TRY(ix->cand.ensure(sizeof(uint32_t) * (size_t)B * n_doc));
   PL_TRY(ix->cand_n.ensure(sizeof(uint32_t) * B));
   PL_TRY(ix->keys1.ensure(sizeof(uint64_t) * (size_t)B * n_seg1 * kSeg));
@@ -1041,6 +1095,7 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
   PL_TRY(ix->nvec.ensure(sizeof(unsigned long long) * B));
   if (ix->bm_dirty) {
     PL_TRY(launch_fill_u32(ix->pid_bm.as<uint32_t>(), 0, ix->pid_bm.cap / 4, s));
+    PL_TRY(launch_fill_u32(ix->pass_bm.as<uint32_t>(), 0, ix->pass_bm.cap / 4, s));
     ix->bm_dirty = false;
   }
   std::vector<uint32_t> h_cand(B), h_fin(B);
@@ -1050,25 +1105,20 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
   for (uint32_t q0 = 0; q0 < n_q; q0 += B) {
     const uint32_t b = std::min(B, n_q - q0);
     const float *Qb = Q + (size_t)q0 * nq * kDim;
-    PL_TRY(hipMemcpyAsync(ix->Q.p, Qb, sizeof(float) * (size_t)b * nq * kDim,
-                          q_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    // queries padded to 32 rows (rows t >= nq stay zero)
+    PL_TRY(hipMemcpy2DAsync(ix->Q.p, sizeof(float) * kMaxQTok * kDim, Qb, sizeof(float) * nq * kDim,
+                            sizeof(float) * nq * kDim, b,
+                            q_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     // a failed pass leaves pid bits set; the next call clears the bitmap
     ix->bm_dirty = true;
     PL_TRY(hipEventRecord(ix->ev[0], s));
-    k_pl_cscores<<<dim3((n_c + 63) / 64, b), 256, 0, s>>>(ix->Q.as<float>(), nq, ix->C, n_c,
-                                                          p->centroid_score_threshold, ix->S.as<float>(),
-                                                          ix->idx.as<uint8_t>());
-    PL_TRY(hipGetLastError());
-    const size_t cell_lds = sizeof(uint32_t) * ((n_c + 31) / 32);
-    switch (p->ncells) {
-#define PL_CELLS(NC)                                                                                       \
-  case NC:                                                                                                 \
-    k_pl_cells<NC><<<b, 64, cell_lds, s>>>(ix->S.as<float>(), n_c, nq, ix->cells.as<uint32_t>(), cell_cap, \
-                                           ix->cell_n.as<uint32_t>());                                     \
-    break;
-      PL_CELLS(1) PL_CELLS(2) PL_CELLS(3) PL_CELLS(4) PL_CELLS(5) PL_CELLS(6) PL_CELLS(7) PL_CELLS(8)
-#undef PL_CELLS
-    }
+    PL_TRY(launch_centroid_scores_plaid(ix->Q.as<float>(), b, nq, ix->C, n_c, p->ncells,
+                                        p->centroid_score_threshold, ix->part_ids.as<uint32_t>(),
+                                        ix->part_s.as<float>(), ix->top_ids.as<uint32_t>(), ix->top_s.as<float>(),
+                                        ix->S.as<float>(), ix->idx.as<uint8_t>(), s));
+    k_pl_cells<<<b, 64, sizeof(uint32_t) * ((n_c + 31) / 32), s>>>(ix->top_ids.as<uint32_t>(), p->ncells, nq, n_c,
+                                                                   ix->cells.as<uint32_t>(), cell_cap,
+                                                                   ix->cell_n.as<uint32_t>());
     PL_TRY(hipGetLastError());
     PL_TRY(launch_fill_u32(ix->cand_n.as<uint32_t>(), 0, b, s));
     PL_TRY(launch_fill_u32(ix->nvec.as<uint32_t>(), 0, 2 * (size_t)b, s));
@@ -1081,13 +1131,33 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
       PL_TRY(hipGetLastError());
     }
     PL_TRY(hipEventRecord(ix->ev[1], s));
+    k_pl_passmark<<<b, 256, 0, s>>>(ix->idx.as<uint8_t>(), n_c, ix->ivf, ix->ivf_off, ix->ivf_len,
+                                    ix->cand_n.as<uint32_t>(), ix->avg_len, ix->pass_list.as<uint32_t>(),
+                                    ix->pass_cnt.as<uint32_t>(), ix->pmode.as<uint32_t>(),
+                                    ix->pass_bm.as<uint32_t>(), ix->bm_words);
+    PL_TRY(hipGetLastError());
+    PL_TRY(launch_fill_u32(ix->heavy_n.as<uint32_t>(), 0, b, s));
+    {
+      const uint32_t n_seg1 = (n_doc + kSeg - 1) / kSeg;
+      const uint32_t gx = std::max(1u, std::min(64u, 16384u / b));
+      k_pl_split<<<dim3(gx, b), 256, 0, s>>>(ix->cand.as<uint32_t>(), n_doc, ix->cand_n.as<uint32_t>(),
+                                             ix->pmode.as<uint32_t>(), ix->pass_bm.as<uint32_t>(),
+                                             ix->pid_bm.as<uint32_t>(), ix->bm_words, nq,
+                                             ix->keys1.as<unsigned long long>(), (uint64_t)n_seg1 * kSeg,
+                                             ix->heavy.as<uint32_t>(), ix->heavy_n.as<uint32_t>());
+      PL_TRY(hipGetLastError());
+    }
     PL_TRY(launch_filter_stages(ix->cand.as<uint32_t>(), ix->cand_n.as<uint32_t>(), 0, n_doc, n_doc,
+                                ix->heavy.as<uint32_t>(), ix->heavy_n.as<uint32_t>(),
                                 ix->S.as<float>(), ix->idx.as<uint8_t>(), n_c, nq, n_doc, ix->codes, ix->doclens,
-                                ix->offsets, ix->pid_bm.as<uint32_t>(), ix->bm_words, b, ndocs,
+                                ix->offsets, nullptr, ix->bm_words, b, ndocs,
                                 ix->keys1.as<uint64_t>(), ix->seg1.as<uint32_t>(), ix->sel1_ids.as<uint32_t>(),
                                 ix->sel1_F.as<float>(), ix->sel1_n.as<uint32_t>(), ix->keys2.as<uint64_t>(),
                                 ix->sel2_ids.as<uint32_t>(), ix->sel2_F.as<float>(), ix->sel2_n.as<uint32_t>(),
                                 ix->nvec.as<unsigned long long>(), s));
+    k_pl_passclear<<<b, 256, 0, s>>>(ix->pass_list.as<uint32_t>(), ix->pass_cnt.as<uint32_t>(), ix->ivf,
+                                     ix->ivf_off, ix->ivf_len, ix->pass_bm.as<uint32_t>(), ix->bm_words);
+    PL_TRY(hipGetLastError());
     PL_TRY(hipEventRecord(ix->ev[2], s));
     {
       const uint32_t n_slice = std::max(1u, std::min((R2 + 31) / 32, std::max(4u, 8192u / b)));
@@ -1108,7 +1178,7 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
     PL_TRY(hipMemcpyAsync(h_fin.data(), ix->sel2_n.p, sizeof(uint32_t) * b, hipMemcpyDeviceToHost, s));
     PL_TRY(hipMemcpyAsync(h_nv.data(), ix->nvec.p, sizeof(unsigned long long) * b, hipMemcpyDeviceToHost, s));
     PL_TRY(hipStreamSynchronize(s));
-    ix->bm_dirty = false;  // stage 1 cleared every word it set
+    ix->bm_dirty = false;  // k_pl_split and k_pl_passclear cleared every word set in the pass
     float ms = 0;
     PL_TRY(hipEventElapsedTime(&ms, ix->ev[0], ix->ev[1]));
     t_cand += ms;
