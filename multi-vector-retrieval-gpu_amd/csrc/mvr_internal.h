@@ -62,23 +62,6 @@ hipError_t launch_centroid_topk_bf16(const float *Q, uint32_t n_q, uint32_t t_ma
                                      uint32_t P, float *bmax, uint32_t *top_ids, float *top_s,
                                      bool exact_f32, hipStream_t s);
 
-// PLAID centroid scores (MODE 2 of the FP32 centroid kernel): for n_q
-// queries laid out as 32 rows each (rows t >= nq zero), S [n_q][n_c][nq] =
-// FMA-chain dots, idx [n_q][n_c] = max_t S >= thr, and the top-P centroids of
-// every row (t < nq) by (score desc, id asc) in top_ids/top_s [n_q*32][P].
-// part_ids/part_s hold [kPlaidMaxSplit][n_q*32][P] partial lists.
-struct PlaidScores {
-  float *S = nullptr;
-  uint8_t *idx = nullptr;
-  uint32_t nq = 0;
-  float thr = 0.0f;
-};
-constexpr uint32_t kPlaidMaxSplit = 16;
-hipError_t launch_centroid_scores_plaid(const float *Q, uint32_t n_q, uint32_t nq, const float *C, uint32_t n_c,
-                                        uint32_t P, float thr, uint32_t *part_ids, float *part_s,
-                                        uint32_t *top_ids, float *top_s, float *S, uint8_t *idx,
-                                        hipStream_t s);
-
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 
 hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t_max, uint32_t P,

@@ -63,9 +63,6 @@ __device__ __forceinline__ uint64_t make_key(float s, uint32_t id) {
 // with one ballot + one shuffle.  MODE 0: score = dot + 1; MODE 1: score =
 // dot - bias[c] (k-means / code assignment, P = 1).  Each workgroup writes
 // sorted partial lists; k_topk_merge combines the centroid slices.
-// MODE 2 (PLAID centroid scores): score = dot; rows are query-major with 32
-// rows per query (rows t >= po.nq are padding and keep no list); every tile
-// also goes to po.S [q][c][t] and po.idx[q][c] = max_t dot >= po.thr.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kRt = 64;   // rows per workgroup
 constexpr uint32_t kCt = 64;   // centroids per tile
@@ -84,7 +81,7 @@ __global__ __launch_bounds__(256) void k_centroid_topk_f32(
     const float *__restrict__ Q, const uint32_t *__restrict__ row_map, uint32_t n_rows,
     const float *__restrict__ C, const float *__restrict__ bias, uint32_t n_c,
     uint32_t c_per_split, uint32_t P, uint32_t *__restrict__ part_ids,
-    float *__restrict__ part_s, PlaidScores po) {
+    float *__restrict__ part_s) {
   __shared__ __align__(16) float As[kDim][kRt + 4];
   __shared__ __align__(16) float Bs[kHalf][kCt + 4];  // reused as the score tile [64][65]
   const uint32_t tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
@@ -156,34 +153,16 @@ __global__ __launch_bounds__(256) void k_centroid_topk_f32(
         const uint32_t c = tx * 4 + j, cid = c0 + c;
         float s;
         if (MODE == 0) s = acc[i][j] + 1.0f;
-        else if (MODE == 1) s = bias ? acc[i][j] - bias[cid < n_c ? cid : 0] : acc[i][j];
-        else s = acc[i][j];
+        else s = bias ? acc[i][j] - bias[cid < n_c ? cid : 0] : acc[i][j];
         S[(ty * 4 + i) * kSs + c] = s;
       }
     __syncthreads();
-    if (MODE == 2) {
-      // the tile to S[q][c][t] (runs of nq consecutive floats per centroid)
-      for (uint32_t e = tid; e < kRt * kCt; e += 256) {
-        const uint32_t r = e & (kRt - 1), c = e / kRt, cid = c0 + c, row = row0 + r;
-        if (row < n_rows && cid < c_end && (row & 31u) < po.nq)
-          po.S[((size_t)(row >> 5) * n_c + cid) * po.nq + (row & 31u)] = S[r * kSs + c];
-      }
-      if (tid < 2 * kCt) {
-        const uint32_t qq = tid / kCt, c = tid % kCt, cid = c0 + c, row = row0 + qq * 32;
-        if (row < n_rows && cid < c_end) {
-          float m = -INFINITY;
-          for (uint32_t t = 0; t < po.nq; ++t) m = fmaxf(m, S[(qq * 32 + t) * kSs + c]);
-          po.idx[(size_t)(row >> 5) * n_c + cid] = m >= po.thr ? 1 : 0;
-        }
-      }
-    }
     const uint32_t cid = c0 + lane;
     const bool cvalid = cid < c_end;
 #pragma unroll
     for (uint32_t rr = 0; rr < 16; ++rr) {
       const uint32_t r = wave * 16 + rr;
       if (row0 + r >= n_rows) continue;  // uniform per wave
-      if (MODE == 2 && ((row0 + r) & 31u) >= po.nq) continue;
       const float s = S[r * kSs + lane];
       float ts = readlane_f(ls[rr], P - 1);
       uint32_t ti = readlane_u(li[rr], P - 1);
@@ -2065,31 +2044,11 @@ hipError_t launch_centroid_topk(const float *Q, const uint32_t *row_map, uint32_
   const uint32_t ns = (ix.n_c + cps - 1) / cps;
   dim3 grid((n_rows + kRt - 1) / kRt, ns);
   k_centroid_topk_f32<0><<<grid, 256, 0, s>>>(Q, row_map, n_rows, ix.C, nullptr, ix.n_c, cps, P,
-                                               part_ids, part_s, PlaidScores{});
+                                               part_ids, part_s);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   k_topk_merge<<<(n_rows + 255) / 256, 256, 0, s>>>(part_ids, part_s, ns, n_rows, P, row_map,
                                                     top_ids, top_s);
-  return hipGetLastError();
-}
-
-hipError_t launch_centroid_scores_plaid(const float *Q, uint32_t n_q, uint32_t nq, const float *C, uint32_t n_c,
-                                        uint32_t P, float thr, uint32_t *part_ids, float *part_s,
-                                        uint32_t *top_ids, float *top_s, float *S, uint8_t *idx,
-                                        hipStream_t s) {
-  if (n_q == 0) return hipSuccess;
-  if (P == 0 || P > 64 || nq == 0 || nq > 32) return hipErrorInvalidValue;
-  const uint32_t n_rows = n_q * 32;
-  const uint32_t row_blocks = (n_rows + kRt - 1) / kRt;
-  const uint32_t want = std::max(1u, std::min(kPlaidMaxSplit, 4096u / row_blocks));
-  const uint32_t cps = ((n_c + want - 1) / want + kCtChunk - 1) / kCtChunk * kCtChunk;
-  const uint32_t ns = (n_c + cps - 1) / cps;
-  dim3 grid(row_blocks, ns);
-  k_centroid_topk_f32<2><<<grid, 256, 0, s>>>(Q, nullptr, n_rows, C, nullptr, n_c, cps, P, part_ids, part_s,
-                                               PlaidScores{S, idx, nq, thr});
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  k_topk_merge<<<(n_rows + 255) / 256, 256, 0, s>>>(part_ids, part_s, ns, n_rows, P, nullptr, top_ids, top_s);
   return hipGetLastError();
 }
 
@@ -2289,7 +2248,7 @@ hipError_t launch_assign(const float *x, uint64_t n, const float *C, const float
     const uint32_t nr = (uint32_t)((n - r0) < kRowsPerLaunch ? (n - r0) : kRowsPerLaunch);
     dim3 grid((nr + kRt - 1) / kRt, 1);
     k_centroid_topk_f32<1><<<grid, 256, 0, s>>>(x + r0 * kDim, nullptr, nr, C, bias, n_c, cps, 1,
-                                                codes + r0, ps, PlaidScores{});
+                                                codes + r0, ps);
     e = hipGetLastError();
     if (e != hipSuccess) break;
   }

@@ -8,8 +8,8 @@
 //   search/decompress_residuals.cpp:27-155        -> k_pl_decompress_pids
 //   modeling/segmented_maxsim.cpp:22-93           -> k_pl_segmented_maxsim
 //   search/filter_pids.cpp:30-193                 -> k_pl_approx (both stages) + k_select_topr
-//   search/candidate_generation.py:13-66          -> k_centroid_topk_f32<2> (mvr_kernels.hip),
-//                                                    k_pl_cells, k_pl_candidates
+//   search/candidate_generation.py:13-66          -> k_pl_sgemm, k_pl_cell_part/_merge, k_pl_cells,
+//                                                    k_pl_candidates
 //   search/index_storage.py:101-198 (CPU branch)  -> the batched pipeline of mvr_plaid_search
 //
 // Design: the reference runs one query at a time through pthread loops
@@ -209,8 +209,226 @@ __global__ __launch_bounds__(256) void k_pl_segmented_maxsim(const float *__rest
 // ---------------------------------------------------------------------------
 // Candidate generation (candidate_generation.py:13-66)
 // ---------------------------------------------------------------------------
-// The union of the query's per-token top-ncells lists (from the centroid
-// kernel) as an ascending cell list; one wave per query, an LDS bitmap of the
+// ---------------------------------------------------------------------------
+// Centroid scores S = C.Q^T (candidate_generation.py:14) for a batch:
+// S[q][c][t] = sequential FMA chain over d = 0..127 of C[c][d] * Q[q][t][d],
+// idx[q][c] = max_t S >= thr (index_storage.py:117).  Queries come padded to
+// 32 rows.  Workgroup tile = 4 queries (128 rows) x 128 centroids; thread
+// (tx, ty) of 16 x 16 keeps an 8 x 8 register tile: rows ty*8 .. ty*8+7 (so
+// wave w holds all 32 rows of query w) and centroids tx*4+u and 64+tx*4+u
+// (u < 4; consecutive lanes read consecutive float4s of the centroid tile).
+// K goes through LDS in chunks of 32 dims; per dim a thread reads 4 float4s
+// for 64 FMAs.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kGk = 32;        // dims per LDS chunk
+constexpr uint32_t kGt = 128;       // rows / centroids per tile
+constexpr uint32_t kGp = kGt + 4;   // padded LDS row
+
+// top1 (nullable): per row the tile's best (score, centroid) as the key
+// (score order bits << 32 | ~centroid), written to top1[q][tile][t], so that
+// ncells = 1 needs no second pass over S.
+__global__ __launch_bounds__(256) void k_pl_sgemm(const float *__restrict__ Q, uint32_t n_q, uint32_t nq,
+                                                  const float *__restrict__ C, uint32_t n_c, float thr,
+                                                  float *__restrict__ S, uint8_t *__restrict__ idx,
+                                                  unsigned long long *__restrict__ top1) {
+  __shared__ __align__(16) float As[kGk][kGp];
+  __shared__ __align__(16) float Bs[kGk][kGp];
+  const uint32_t tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const uint32_t q0 = blockIdx.y * 4, c0 = blockIdx.x * kGt;
+  const uint32_t n_rows = n_q * kMaxQTok;
+  float acc[8][8];
+#pragma unroll
+  for (uint32_t i = 0; i < 8; ++i)
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j) acc[i][j] = 0.0f;
+  const float4 *Q4 = reinterpret_cast<const float4 *>(Q);
+  const float4 *C4 = reinterpret_cast<const float4 *>(C);
+#pragma nounroll
+  for (uint32_t k0 = 0; k0 < kDim; k0 += kGk) {
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const uint32_t e = tid + 256 * u, r = e >> 3, d4 = e & 7;
+      const uint32_t row = q0 * kMaxQTok + r, c = c0 + r;
+      const float4 a = row < n_rows ? Q4[(size_t)row * (kDim / 4) + k0 / 4 + d4] : make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 b = c < n_c ? C4[(size_t)c * (kDim / 4) + k0 / 4 + d4] : make_float4(0.f, 0.f, 0.f, 0.f);
+      As[4 * d4 + 0][r] = a.x; As[4 * d4 + 1][r] = a.y; As[4 * d4 + 2][r] = a.z; As[4 * d4 + 3][r] = a.w;
+      Bs[4 * d4 + 0][r] = b.x; Bs[4 * d4 + 1][r] = b.y; Bs[4 * d4 + 2][r] = b.z; Bs[4 * d4 + 3][r] = b.w;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (uint32_t d = 0; d < kGk; ++d) {
+      const float4 a0 = *reinterpret_cast<const float4 *>(&As[d][ty * 8]);
+      const float4 a1 = *reinterpret_cast<const float4 *>(&As[d][ty * 8 + 4]);
+      const float4 b0 = *reinterpret_cast<const float4 *>(&Bs[d][tx * 4]);
+      const float4 b1 = *reinterpret_cast<const float4 *>(&Bs[d][64 + tx * 4]);
+      const float av[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+      const float bv[8] = {b0.x, b0.y, b0.z, b0.w, b1.x, b1.y, b1.z, b1.w};
+#pragma unroll
+      for (uint32_t i = 0; i < 8; ++i)
+#pragma unroll
+        for (uint32_t j = 0; j < 8; ++j) acc[i][j] = __builtin_fmaf(bv[j], av[i], acc[i][j]);
+    }
+    __syncthreads();
+  }
+  const uint32_t qq = ty >> 2, t0 = (ty & 3) * 8, q = q0 + qq;
+  const bool qv = q < n_q;
+#pragma unroll
+  for (uint32_t j = 0; j < 8; ++j) {
+    const uint32_t c = c0 + (j < 4 ? tx * 4 + j : 64 + tx * 4 + (j - 4));
+    float m = -INFINITY;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i)
+      if (t0 + i < nq) m = fmaxf(m, acc[i][j]);
+    m = fmaxf(m, __shfl_xor(m, 16));
+    m = fmaxf(m, __shfl_xor(m, 32));
+    if (qv && c < n_c) {
+      float *srow = S + ((size_t)q * n_c + c) * nq;
+      if (nq == kMaxQTok) {
+        reinterpret_cast<float4 *>(srow + t0)[0] = make_float4(acc[0][j], acc[1][j], acc[2][j], acc[3][j]);
+        reinterpret_cast<float4 *>(srow + t0)[1] = make_float4(acc[4][j], acc[5][j], acc[6][j], acc[7][j]);
+      } else {
+#pragma unroll
+        for (uint32_t i = 0; i < 8; ++i)
+          if (t0 + i < nq) srow[t0 + i] = acc[i][j];
+      }
+      if ((ty & 3) == 0) idx[(size_t)q * n_c + c] = m >= thr ? 1 : 0;
+    }
+  }
+  if (top1) {
+    const uint32_t n_tiles = gridDim.x;
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i) {
+      unsigned long long best = 0ull;
+#pragma unroll
+      for (uint32_t j = 0; j < 8; ++j) {
+        const uint32_t c = c0 + (j < 4 ? tx * 4 + j : 64 + tx * 4 + (j - 4));
+        const unsigned long long key = ((unsigned long long)pl_f2ord(acc[i][j]) << 32) | (uint32_t)~c;
+        if (c < n_c && key > best) best = key;
+      }
+#pragma unroll
+      for (uint32_t off = 1; off < 16; off <<= 1) {
+        const unsigned long long o = __shfl_xor(best, off);
+        best = o > best ? o : best;
+      }
+      if (tx == 0 && qv && t0 + i < nq) top1[((size_t)q * n_tiles + blockIdx.x) * kMaxQTok + t0 + i] = best;
+    }
+  }
+}
+
+// ncells = 1: per (query, token) the best of the tiles' keys; top_ids[q*32+t].
+__global__ __launch_bounds__(256) void k_pl_top1_merge(const unsigned long long *__restrict__ top1, uint32_t n_q,
+                                                       uint32_t nq, uint32_t n_tiles, uint32_t *__restrict__ top_ids) {
+  const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t q = e / kMaxQTok, t = e % kMaxQTok;
+  if (q >= n_q || t >= nq) return;
+  unsigned long long best = 0ull;
+  for (uint32_t i = 0; i < n_tiles; ++i) {
+    const unsigned long long k = top1[((size_t)q * n_tiles + i) * kMaxQTok + t];
+    best = k > best ? k : best;
+  }
+  top_ids[(size_t)q * kMaxQTok + t] = best ? ~(uint32_t)(best & 0xFFFFFFFFull) : 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ bool pl_better(float s1, uint32_t i1, float s2, uint32_t i2) {
+  return (s1 > s2) || (s1 == s2 && i1 < i2);
+}
+
+// Per query token the best NC centroids of one chunk of centroids by (score
+// desc, centroid asc): one wave per (query, chunk), lane (t, h) scanning the
+// chunk's centroids c = h (mod 2) for token t, then the two halves merge.
+// Lists go to part[q][chunk][t][NC].
+constexpr uint32_t kCellChunk = 1024;
+
+template <uint32_t NC>
+__global__ __launch_bounds__(256) void k_pl_cell_part(const float *__restrict__ S, uint32_t n_c, uint32_t nq,
+                                                      uint32_t n_chunk, uint32_t *__restrict__ part_ids,
+                                                      float *__restrict__ part_s) {
+  const uint32_t lane = threadIdx.x & 63, t = lane & 31, h = lane >> 5;
+  const uint32_t ch = blockIdx.x * 4 + (threadIdx.x >> 6), q = blockIdx.y;
+  if (ch >= n_chunk) return;
+  const uint32_t c_begin = ch * kCellChunk, c_end = min(n_c, c_begin + kCellChunk);
+  float bs[NC];
+  uint32_t bi[NC];
+#pragma unroll
+  for (uint32_t j = 0; j < NC; ++j) { bs[j] = -INFINITY; bi[j] = 0xFFFFFFFFu; }
+  if (t < nq) {
+    const float *Sq = S + (size_t)q * n_c * nq + t;
+#pragma unroll 8
+    for (uint32_t c = c_begin + h; c < c_end; c += 2) {
+      float cv = Sq[(size_t)c * nq];
+      uint32_t ci = c;
+      if (pl_better(cv, ci, bs[NC - 1], bi[NC - 1])) {
+#pragma unroll
+        for (uint32_t j = 0; j < NC; ++j) {
+          if (pl_better(cv, ci, bs[j], bi[j])) {
+            const float ts = bs[j]; const uint32_t ti = bi[j];
+            bs[j] = cv; bi[j] = ci; cv = ts; ci = ti;
+          }
+        }
+      }
+    }
+  }
+  // read the whole partner list before any insert changes the lists
+  float os[NC];
+  uint32_t oi[NC];
+#pragma unroll
+  for (uint32_t j = 0; j < NC; ++j) {
+    os[j] = __shfl(bs[j], (int)(t + 32));
+    oi[j] = (uint32_t)__shfl((int)bi[j], (int)(t + 32));
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < NC; ++j) {
+    float cv = os[j];
+    uint32_t ci = oi[j];
+#pragma unroll
+    for (uint32_t u = 0; u < NC; ++u) {
+      if (pl_better(cv, ci, bs[u], bi[u])) {
+        const float ts = bs[u]; const uint32_t ti = bi[u];
+        bs[u] = cv; bi[u] = ci; cv = ts; ci = ti;
+      }
+    }
+  }
+  if (h == 0 && t < nq) {
+    const size_t o = (((size_t)q * n_chunk + ch) * kMaxQTok + t) * NC;
+#pragma unroll
+    for (uint32_t j = 0; j < NC; ++j) { part_ids[o + j] = bi[j]; part_s[o + j] = bs[j]; }
+  }
+}
+
+// Merge of the chunk lists: one thread per (query, token) writes the best NC
+// to top_ids[q*32 + t][NC].
+template <uint32_t NC>
+__global__ __launch_bounds__(256) void k_pl_cell_merge(const uint32_t *__restrict__ part_ids,
+                                                       const float *__restrict__ part_s, uint32_t n_q,
+                                                       uint32_t nq, uint32_t n_chunk,
+                                                       uint32_t *__restrict__ top_ids) {
+  const uint32_t e = blockIdx.x * 256 + threadIdx.x;
+  const uint32_t q = e / kMaxQTok, t = e % kMaxQTok;
+  if (q >= n_q || t >= nq) return;
+  float bs[NC];
+  uint32_t bi[NC];
+#pragma unroll
+  for (uint32_t j = 0; j < NC; ++j) { bs[j] = -INFINITY; bi[j] = 0xFFFFFFFFu; }
+  for (uint32_t ch = 0; ch < n_chunk; ++ch) {
+    const size_t o = (((size_t)q * n_chunk + ch) * kMaxQTok + t) * NC;
+    for (uint32_t k = 0; k < NC; ++k) {
+      float cv = part_s[o + k];
+      uint32_t ci = part_ids[o + k];
+      if (!pl_better(cv, ci, bs[NC - 1], bi[NC - 1])) break;  // the chunk list is sorted
+#pragma unroll
+      for (uint32_t u = 0; u < NC; ++u) {
+        if (pl_better(cv, ci, bs[u], bi[u])) {
+          const float ts = bs[u]; const uint32_t ti = bi[u];
+          bs[u] = cv; bi[u] = ci; cv = ts; ci = ti;
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (uint32_t j = 0; j < NC; ++j) top_ids[((size_t)q * kMaxQTok + t) * NC + j] = bi[j];
+}
+
+// The union of the query's per-token top-ncells lists as an ascending cell list; one wave per query, an LDS bitmap of the
 // n_c centroids.
 __global__ __launch_bounds__(64) void k_pl_cells(const uint32_t *__restrict__ top_ids, uint32_t P,
                                                  uint32_t nq, uint32_t n_c, uint32_t *__restrict__ cells,
@@ -749,11 +967,11 @@ struct mvr_plaid_index {
   bool bm_dirty = true;  // pid bitmap must be cleared before the next pass
   uint32_t q_nq = 0;     // query rows last copied into the padded Q buffer
   float avg_len = 0.0f;  // mean doc length (tokens)
-  PBuf part_ids, part_s, top_ids, top_s, pass_list, pass_cnt, pmode, pass_bm, heavy, heavy_n;
+  PBuf part_ids, part_s, top_ids, top1, pass_list, pass_cnt, pmode, pass_bm, heavy, heavy_n;
   PBuf Q, S, idx, cells, cell_n, pid_bm, cand, cand_n, keys1, seg1, sel1_ids, sel1_F, sel1_n, keys2, sel2_ids,
       sel2_F, sel2_n, ids2, score2, out_ids, out_s, nvec;
   std::vector<PBuf *> bufs() {
-    return {&part_ids, &part_s, &top_ids, &top_s, &pass_list, &pass_cnt, &pmode, &pass_bm, &heavy, &heavy_n, &Q, &S, &idx, &cells, &cell_n, &pid_bm, &cand, &cand_n, &keys1, &seg1, &sel1_ids, &sel1_F, &sel1_n,
+    return {&part_ids, &part_s, &top_ids, &top1, &pass_list, &pass_cnt, &pmode, &pass_bm, &heavy, &heavy_n, &Q, &S, &idx, &cells, &cell_n, &pid_bm, &cand, &cand_n, &keys1, &seg1, &sel1_ids, &sel1_F, &sel1_n,
             &keys2, &sel2_ids, &sel2_F, &sel2_n, &ids2, &score2, &out_ids, &out_s, &nvec};
   }
 };
@@ -1041,7 +1259,8 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
   const uint32_t n_seg1 = (n_doc + kSeg - 1) / kSeg;
   // device bytes per query in a pass
   const size_t per_q = sizeof(float) * kDim * kMaxQTok + sizeof(float) * (size_t)n_c * nq + n_c +
-                       (kPlaidMaxSplit + 1) * 8 * (size_t)kMaxQTok * p->ncells +
+                       (((size_t)n_c + kCellChunk - 1) / kCellChunk + 1) * 8 * (size_t)kMaxQTok * p->ncells +
+                       8 * (((size_t)n_c + kGt - 1) / kGt) * kMaxQTok +
                        sizeof(uint32_t) * ((size_t)n_doc + ix->bm_words + kPassCap + 3) +
                        sizeof(uint32_t) * (cell_cap + 1) + sizeof(uint32_t) * ix->bm_words +
                        sizeof(uint32_t) * (n_doc + 1) + sizeof(uint64_t) * (size_t)n_seg1 * kSeg +
@@ -1059,10 +1278,11 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
     PL_TRY(launch_fill_u32(ix->Q.as<uint32_t>(), 0, ix->Q.cap / 4, s));
     ix->q_nq = nq;
   }
-  PL_TRY(ix->part_ids.ensure(sizeof(uint32_t) * (size_t)kPlaidMaxSplit * B * kMaxQTok * p->ncells));
-  PL_TRY(ix->part_s.ensure(sizeof(float) * (size_t)kPlaidMaxSplit * B * kMaxQTok * p->ncells));
+  const uint32_t n_chunk = (n_c + kCellChunk - 1) / kCellChunk;
+  PL_TRY(ix->part_ids.ensure(sizeof(uint32_t) * (size_t)n_chunk * B * kMaxQTok * p->ncells));
+  PL_TRY(ix->part_s.ensure(sizeof(float) * (size_t)n_chunk * B * kMaxQTok * p->ncells));
   PL_TRY(ix->top_ids.ensure(sizeof(uint32_t) * (size_t)B * kMaxQTok * p->ncells));
-  PL_TRY(ix->top_s.ensure(sizeof(float) * (size_t)B * kMaxQTok * p->ncells));
+  PL_TRY(ix->top1.ensure(sizeof(unsigned long long) * (size_t)B * ((n_c + kGt - 1) / kGt) * kMaxQTok));
   PL_TRY(ix->S.ensure(sizeof(float) * (size_t)B * n_c * nq));
   PL_TRY(ix->idx.ensure((size_t)B * n_c));
   PL_TRY(ix->cells.ensure(sizeof(uint32_t) * (size_t)B * cell_cap));
@@ -1112,10 +1332,32 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
     // a failed pass leaves pid bits set; the next call clears the bitmap
     ix->bm_dirty = true;
     PL_TRY(hipEventRecord(ix->ev[0], s));
-    PL_TRY(launch_centroid_scores_plaid(ix->Q.as<float>(), b, nq, ix->C, n_c, p->ncells,
-                                        p->centroid_score_threshold, ix->part_ids.as<uint32_t>(),
-                                        ix->part_s.as<float>(), ix->top_ids.as<uint32_t>(), ix->top_s.as<float>(),
-                                        ix->S.as<float>(), ix->idx.as<uint8_t>(), s));
+    const uint32_t n_tiles = (n_c + kGt - 1) / kGt;
+    k_pl_sgemm<<<dim3(n_tiles, (b + 3) / 4), 256, 0, s>>>(
+        ix->Q.as<float>(), b, nq, ix->C, n_c, p->centroid_score_threshold, ix->S.as<float>(), ix->idx.as<uint8_t>(),
+        p->ncells == 1 ? ix->top1.as<unsigned long long>() : nullptr);
+    PL_TRY(hipGetLastError());
+    if (p->ncells == 1) {
+      k_pl_top1_merge<<<(b * kMaxQTok + 255) / 256, 256, 0, s>>>(ix->top1.as<unsigned long long>(), b, nq, n_tiles,
+                                                                 ix->top_ids.as<uint32_t>());
+      PL_TRY(hipGetLastError());
+    } else {
+      const uint32_t n_chunk = (n_c + kCellChunk - 1) / kCellChunk;
+      const dim3 gp((n_chunk + 3) / 4, b);
+      const uint32_t gm = (b * kMaxQTok + 255) / 256;
+      switch (p->ncells) {
+#define PL_CELLS(NC)                                                                                          \
+  case NC:                                                                                                    \
+    k_pl_cell_part<NC><<<gp, 256, 0, s>>>(ix->S.as<float>(), n_c, nq, n_chunk, ix->part_ids.as<uint32_t>(),   \
+                                          ix->part_s.as<float>());                                            \
+    k_pl_cell_merge<NC><<<gm, 256, 0, s>>>(ix->part_ids.as<uint32_t>(), ix->part_s.as<float>(), b, nq,        \
+                                           n_chunk, ix->top_ids.as<uint32_t>());                              \
+    break;
+        PL_CELLS(1) PL_CELLS(2) PL_CELLS(3) PL_CELLS(4) PL_CELLS(5) PL_CELLS(6) PL_CELLS(7) PL_CELLS(8)
+#undef PL_CELLS
+      }
+      PL_TRY(hipGetLastError());
+    }
     k_pl_cells<<<b, 64, sizeof(uint32_t) * ((n_c + 31) / 32), s>>>(ix->top_ids.as<uint32_t>(), p->ncells, nq, n_c,
                                                                    ix->cells.as<uint32_t>(), cell_cap,
                                                                    ix->cell_n.as<uint32_t>());
@@ -1160,6 +1402,7 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
     PL_TRY(hipGetLastError());
     PL_TRY(hipEventRecord(ix->ev[2], s));
     {
+      // about 8 docs per wave: each wave holds its query's 32 rows in registers
       const uint32_t n_slice = std::max(1u, std::min((R2 + 31) / 32, std::max(4u, 8192u / b)));
       k_pl_refine<<<dim3(n_slice, b), 256, 0, s>>>(ix->Q.as<float>(), nq, ix->sel2_ids.as<uint32_t>(),
                                                    ix->sel2_n.as<uint32_t>(), R2, n_slice, n_doc, ix->doclens,
