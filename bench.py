@@ -301,7 +301,7 @@ def bench_plaid(torch, args, cfg):
         g.search(Q, k=k)
     torch.cuda.synchronize()
     keys = ("candidates_ms", "filter_ms", "refine_ms", "total_ms", "n_refine_ivf", "n_refine_filter",
-            "n_vec_score_refine")
+            "n_vec_score_refine", "scores_ms", "n_pass")
     acc = {f: 0.0 for f in keys}
     t1 = time.perf_counter()
     for _ in range(args.steps):
@@ -330,6 +330,17 @@ def bench_plaid(torch, args, cfg):
                "sample": f"first {n_s} queries, same PLAID index and settings",
                "seconds": round(wall, 2), "gpu_results_bitexact": same}
         log(f"[bench] plaid cpu baseline {n_s / wall:.1f} QPS on {cores} cores; GPU results bit-exact: {same}")
+    # roofline of the dominant kernel, the exact FP32 centroid-score GEMM
+    # (k_pl_sgemm): 2 x 128 x 32 x n_c flops per query, over its launch
+    # durations (HIP events around it on the search stream)
+    n_c = cfg["n_c"]
+    flops_launch = 2.0 * 128 * 32 * n_c * n_q / st["n_pass"]
+    avg_ms = st["scores_ms"] / st["n_pass"]
+    ach = flops_launch / (avg_ms * 1e-3) / 1e12
+    roof = {"bound": "mfma", "achieved": round(ach, 2), "peak": 157.3, "unit": "TFLOP/s",
+            "frac": round(ach / 157.3, 3), "traffic": None, "kernel": "k_pl_sgemm",
+            "note": "FP32 FMA chains on the vector ALUs (exact order); peak is MI355X's FP32 figure",
+            "avg_launch_ms": round(avg_ms, 3), "launches_per_step": st["n_pass"]}
     line = {
         "metric": f"PLAID search queries/sec (searcher defaults for k={k}), 1 GPU",
         "value": round(qps, 1), "unit": "queries/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -339,6 +350,7 @@ def bench_plaid(torch, args, cfg):
                                f"{cfg['n_c']} centroids, 2-bit residuals; {n_q} queries x 32 tokens; "
                                f"ncells {ncells}, centroid_score_threshold {thr}, ndocs {ndocs}, k {k}"},
         "stages_ms_per_step": {f: round(st[f], 3) for f in keys[:4]},
+        "roofline": roof,
         "per_query": {"n_refine_ivf": st["n_refine_ivf"] / n_q, "n_refine_filter": st["n_refine_filter"] / n_q,
                       "n_vec_score_refine": st["n_vec_score_refine"] / n_q},
         "cpu_baseline": cpu,

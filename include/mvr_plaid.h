@@ -122,10 +122,13 @@ typedef struct {
   const uint8_t *residuals;      /* [n_tok][dim*nbits/8], PLAID bit order */
   const float *centroids;        /* [n_centroid][dim] (the CPU branch widens f16 to f32) */
   const float *bucket_weights;   /* [2^nbits] */
-  const int32_t *ivf;            /* [n_ivf] pids of each centroid, ascending */
+  const int32_t *ivf;            /* [n_ivf] per centroid: exactly the docs holding it, ascending */
   const int64_t *ivf_lengths;    /* [n_centroid] */
 } mvr_plaid_index_desc;
 
+/* Checks the arrays first (codes in range, doclens summing to n_tok, every
+ * IVF list equal to the docs that hold its centroid; MVR_ERR_INVALID
+ * otherwise), then uploads them to `device`. */
 int mvr_plaid_index_create(const mvr_plaid_index_desc *desc, int device, mvr_plaid_index **out);
 void mvr_plaid_index_free(mvr_plaid_index *ix);
 
@@ -140,6 +143,8 @@ typedef struct {
 typedef struct {
   double candidates_ms, filter_ms, refine_ms, total_ms;
   uint64_t n_refine_ivf, n_refine_filter, n_vec_score_refine; /* sums over the queries */
+  double scores_ms;  /* the centroid-score kernel alone (part of candidates_ms) */
+  uint32_t n_pass;   /* device passes */
 } mvr_plaid_timings;
 
 /* IndexScorer.rank (index_storage.py:82-198, CPU branch) for n_q queries of

@@ -963,7 +963,7 @@ struct mvr_plaid_index {
   int64_t *ivf_off = nullptr, *ivf_len = nullptr;
   float4 *tab4 = nullptr;
   hipStream_t stream = nullptr;
-  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool bm_dirty = true;  // pid bitmap must be cleared before the next pass
   uint32_t q_nq = 0;     // query rows last copied into the padded Q buffer
   float avg_len = 0.0f;  // mean doc length (tokens)
@@ -1320,7 +1320,8 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
   }
   std::vector<uint32_t> h_cand(B), h_fin(B);
   std::vector<unsigned long long> h_nv(B);
-  double t_cand = 0, t_filt = 0, t_ref = 0;
+  double t_cand = 0, t_filt = 0, t_ref = 0, t_sc = 0;
+  uint32_t n_pass = 0;
   auto t0 = std::chrono::steady_clock::now();
   for (uint32_t q0 = 0; q0 < n_q; q0 += B) {
     const uint32_t b = std::min(B, n_q - q0);
@@ -1337,6 +1338,7 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
         ix->Q.as<float>(), b, nq, ix->C, n_c, p->centroid_score_threshold, ix->S.as<float>(), ix->idx.as<uint8_t>(),
         p->ncells == 1 ? ix->top1.as<unsigned long long>() : nullptr);
     PL_TRY(hipGetLastError());
+    PL_TRY(hipEventRecord(ix->ev[4], s));
     if (p->ncells == 1) {
       k_pl_top1_merge<<<(b * kMaxQTok + 255) / 256, 256, 0, s>>>(ix->top1.as<unsigned long long>(), b, nq, n_tiles,
                                                                  ix->top_ids.as<uint32_t>());
@@ -1429,6 +1431,9 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
     t_filt += ms;
     PL_TRY(hipEventElapsedTime(&ms, ix->ev[2], ix->ev[3]));
     t_ref += ms;
+    PL_TRY(hipEventElapsedTime(&ms, ix->ev[0], ix->ev[4]));
+    t_sc += ms;
+    ++n_pass;
     for (uint32_t i = 0; i < b; ++i) {
       if (stats) {
         stats[(size_t)(q0 + i) * 3 + 0] = h_cand[i];
@@ -1446,6 +1451,8 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
     timings->candidates_ms = t_cand;
     timings->filter_ms = t_filt;
     timings->refine_ms = t_ref;
+    timings->scores_ms = t_sc;
+    timings->n_pass = n_pass;
     timings->total_ms =
         std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }

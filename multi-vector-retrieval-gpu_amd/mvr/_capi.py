@@ -63,7 +63,7 @@ class PlaidParams(C.Structure):
 class PlaidTimings(C.Structure):
     _fields_ = [("candidates_ms", C.c_double), ("filter_ms", C.c_double), ("refine_ms", C.c_double),
                 ("total_ms", C.c_double), ("n_refine_ivf", C.c_uint64), ("n_refine_filter", C.c_uint64),
-                ("n_vec_score_refine", C.c_uint64)]
+                ("n_vec_score_refine", C.c_uint64), ("scores_ms", C.c_double), ("n_pass", C.c_uint32)]
 
 
 _vp, _u32, _u64 = C.c_void_p, C.c_uint32, C.c_uint64
