@@ -202,6 +202,21 @@ int mvr_mlp_train_step(mvr_mlp *mlp, const float *x_dev, const int32_t *y_dev, c
 int mvr_index_set_probe(mvr_index *ix, mvr_mlp *mlp, float conf, uint32_t fallback_nprobe,
                         uint32_t r_min);
 
+/* ---- exhaustive MaxSim (the groundtruth module) ---- */
+
+/* All-pairs FP32 MaxSim, replacing GPUComputeScore::computeItemScore
+ * (src/include/compute_score/GPUComputeScore.hpp:196-246) behind
+ * BruteForceProgressive.DocRetrieval.computeScore
+ * (src/app/BruteForceProgressive.cpp:16-33): scores[q][i] = left-to-right sum
+ * over the T query tokens of max over item i's vectors of dot(Q_q,t, x), each
+ * dot one FMA chain over d.  Host arrays: Q f32 [n_q][T][dim] (T <= 32,
+ * dim = 128), X f32 [sum lens][dim] (items back to back), lens [n_item]
+ * (every item >= 1 vector), scores f32 [n_q][n_item].  Runs on `device`;
+ * errors are described by mvr_maxsim_last_error(). */
+int mvr_maxsim_scores_host(const float *Q, uint32_t n_q, uint32_t T, uint32_t dim, const float *X,
+                           const uint32_t *lens, uint32_t n_item, float *scores, int device);
+const char *mvr_maxsim_last_error(void);
+
 /* ---- synthetic corpus and index build on the device (bench and tests) ---- */
 
 /* Mixture tokens: row k of out (device, [n_tok][dim]) is token tok_begin+k of
@@ -220,6 +235,13 @@ int mvr_assign_bias(const float *x_dev, uint64_t n, const float *C_dev, const fl
  * (per_dim = 1) or [3]; device buffers. */
 int mvr_compress(const float *x_dev, const uint32_t *codes_dev, uint64_t n, const float *C_dev,
                  uint32_t dim, const float *cutoffs_dev, int per_dim, uint8_t *out_dev, void *stream);
+/* The same on host arrays, run on `device` in chunks of 4M vectors: the GPU
+ * residual coder behind CompressResidualCode.compute_residual_code
+ * (compressSQCode, src/include/alg/refine/ResidualScalarQuantizationPackGPU.hpp:30-80;
+ * bound at src/app/build_index/GPGPUBuildIndexPackGPU.cpp:14-42).  Codes are
+ * checked against n_c; dim must be 128 (2-bit codes, 32 bytes per vector). */
+int mvr_compress_host(const float *x, const uint32_t *codes, uint64_t n, const float *C, uint32_t n_c,
+                      uint32_t dim, const float *cutoffs, int per_dim, uint8_t *out, int device);
 
 #ifdef __cplusplus
 }

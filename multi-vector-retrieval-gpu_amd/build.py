@@ -4,6 +4,7 @@ Outputs go to multi-vector-retrieval-gpu_amd/lib/:
   libmvr.so                      HIP kernels + C ABI (hipcc, gfx950)
   IGPGPUPP, IGPGPUBasic          command-line drop-ins (g++, link libmvr.so)
   GPGPUBuildIndexPackGPU*.so     pybind11 index-build module (g++)
+  BruteForceProgressive*.so      pybind11 exhaustive-MaxSim module (g++, calls libmvr)
 
 Usage: python multi-vector-retrieval-gpu_amd/build.py [lib|cli|pymod|all]
 """
@@ -26,7 +27,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = ["-O3", "--offload-arch=gfx950", "-std=c++17", "-fPIC", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall",
              "-Wno-unused-command-line-argument"]
-HIP_SOURCES = ["mvr_kernels.hip", "mvr_capi.hip", "mvr_mlp.hip", "mvr_plaid.hip"]
+HIP_SOURCES = ["mvr_kernels.hip", "mvr_capi.hip", "mvr_mlp.hip", "mvr_plaid.hip", "mvr_exhaustive.hip"]
 
 
 def _run(cmd):
@@ -65,7 +66,18 @@ def build_pymod() -> str:
     out = os.path.join(LIB, "GPGPUBuildIndexPackGPU" + suffix)
     _run(["g++", "-O3", "-std=c++17", "-shared", "-fPIC", "-fopenmp", "-ffp-contract=off",
           "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
-          os.path.join(CSRC, "build_index_module.cpp"), "-o", out])
+          os.path.join(CSRC, "build_index_module.cpp"), "-o", out, "-ldl"])
+    return out
+
+
+def build_bfmodule() -> str:
+    """BruteForceProgressive pybind module (exhaustive MaxSim over libmvr)."""
+    import pybind11
+    suffix = sysconfig.get_config_var("EXT_SUFFIX")
+    out = os.path.join(LIB, "BruteForceProgressive" + suffix)
+    _run(["g++", "-O3", "-std=c++17", "-shared", "-fPIC", "-fopenmp",
+          "-I", pybind11.get_include(), "-I", sysconfig.get_paths()["include"],
+          os.path.join(CSRC, "bruteforce_module.cpp"), "-o", out, "-ldl"])
     return out
 
 
@@ -81,10 +93,11 @@ def build_all():
     build_lib()
     build_cli()
     build_pymod()
+    build_bfmodule()
     build_probe()
 
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
-    {"lib": build_lib, "cli": build_cli, "pymod": build_pymod, "probe": build_probe,
+    {"lib": build_lib, "cli": build_cli, "pymod": build_pymod, "bf": build_bfmodule, "probe": build_probe,
      "all": build_all}[what]()

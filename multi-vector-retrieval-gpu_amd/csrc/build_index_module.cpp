@@ -8,16 +8,22 @@
 //   IVFIndex(vq_code_l, item_n_vec_l, n_vec, n_item, n_centroid).build() / .save(location)
 //       src/include/struct/IVFIndex.hpp:78-183
 // plus compute_quantized_scalar_per_dim (per-dimension cutoffs, bucket-mean
-// weights) for the synthetic configurations.  Arrays are taken C-contiguous
+// weights) for the synthetic configurations.  compute_residual_code runs on
+// the GPU through libmvr (mvr_compress_host) when one is present, as the
+// reference's does (MVR_BUILD_DEVICE=cpu|gpu|auto); `.device` tells where the
+// last call ran.  Both give the same bytes.  Arrays are taken C-contiguous
 // with forcecast (src/include/struct/TypeDef.hpp:13-21).  Invalid arguments
 // raise ValueError instead of exit(-1).
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <stdexcept>
@@ -123,6 +129,48 @@ py::tuple compute_quantized_scalar_per_dim(const arr_f32 &x, const arr_f32 &C, c
   return py::make_tuple(cut_arr, w_arr);
 }
 
+// The GPU residual coder of libmvr (mvr_compress_host), loaded from the
+// directory of this module on first use.  torch is imported first when it is
+// installed, so that libmvr binds to the HIP runtime torch already carries.
+struct GpuCoder {
+  using compress_fn = int (*)(const float *, const uint32_t *, uint64_t, const float *, uint32_t, uint32_t,
+                              const float *, int, uint8_t *, int);
+  compress_fn compress = nullptr;
+  int (*device_count)(int *) = nullptr;
+  const char *(*last_error)() = nullptr;
+  bool tried = false;
+  int n_dev = 0;
+};
+
+GpuCoder &gpu_coder() {
+  static GpuCoder g;
+  if (g.tried) return g;
+  g.tried = true;
+  try {
+    py::module_::import("torch");
+  } catch (...) {
+  }
+  Dl_info info;
+  if (!dladdr(reinterpret_cast<void *>(&gpu_coder), &info) || !info.dli_fname) return g;
+  std::string dir(info.dli_fname);
+  dir = dir.substr(0, dir.find_last_of('/') + 1);
+  void *h = dlopen((dir + "libmvr.so").c_str(), RTLD_NOW | RTLD_LOCAL);
+  if (!h) return g;
+  g.compress = reinterpret_cast<GpuCoder::compress_fn>(dlsym(h, "mvr_compress_host"));
+  g.device_count = reinterpret_cast<int (*)(int *)>(dlsym(h, "mvr_device_count"));
+  g.last_error = reinterpret_cast<const char *(*)()>(dlsym(h, "mvr_last_error"));
+  if (!g.compress || !g.device_count || !g.last_error || g.device_count(&g.n_dev) != 0) g.n_dev = 0;
+  return g;
+}
+
+// MVR_BUILD_DEVICE: "cpu" (host threads), "gpu" (device 0, error when there
+// is none), or unset / "auto" (the GPU when one is there and the shape is the
+// device kernel's: 2-bit codes of 128-dim vectors).
+std::string build_device_mode() {
+  const char *e = std::getenv("MVR_BUILD_DEVICE");
+  return e ? std::string(e) : std::string("auto");
+}
+
 class CompressResidualCode {
  public:
   uint32_t n_centroid_ = 0, vec_dim_ = 0, n_bit_ = 0, n_val_per_byte_ = 0, n_packed_val_per_vec_ = 0;
@@ -154,7 +202,9 @@ class CompressResidualCode {
     n_packed_val_per_vec_ = (vec_dim_ + n_val_per_byte_ - 1) / n_val_per_byte_;
   }
 
-  py::tuple compute_residual_code(const arr_f32 &x, const arr_u32 &codes) const {
+  std::string device_ = "cpu";  // where the last compute_residual_code ran
+
+  py::tuple compute_residual_code(const arr_f32 &x, const arr_u32 &codes) {
     if (x.ndim() != 2 || (uint32_t)x.shape(1) != vec_dim_ || codes.ndim() != 1 || codes.shape(0) != x.shape(0))
       throw py::value_error("expected vec_l [n, dim] and code_l [n]");
     const size_t n = x.shape(0);
@@ -163,6 +213,29 @@ class CompressResidualCode {
     for (size_t i = 0; i < n; ++i)
       if (kp[i] >= n_centroid_) throw py::value_error("code_l holds an id >= number of centroids");
     std::vector<uint8_t> out(n * n_packed_val_per_vec_);
+    const std::string mode = build_device_mode();
+    if (mode != "cpu" && mode != "gpu" && mode != "auto")
+      throw py::value_error("MVR_BUILD_DEVICE must be cpu, gpu or auto");
+    const bool shape_ok = n_bit_ == 2 && vec_dim_ == 128;
+    if (mode != "cpu" && n > 0) {
+      GpuCoder &g = gpu_coder();
+      const bool avail = g.compress && g.n_dev > 0;
+      if (mode == "gpu" && (!avail || !shape_ok))
+        throw std::runtime_error(!avail ? "MVR_BUILD_DEVICE=gpu but no GPU / libmvr.so is available"
+                                        : "the GPU residual coder needs n_bit 2 and dim 128");
+      if (avail && shape_ok) {
+        int st;
+        {
+          py::gil_scoped_release nogil;
+          st = g.compress(xp, kp, n, centroid_l_.data(), n_centroid_, vec_dim_, cutoff_l_.data(),
+                          cut_stride_ ? 1 : 0, out.data(), 0);
+        }
+        if (st != 0) throw std::runtime_error(std::string("GPU residual coder: ") + g.last_error());
+        device_ = "gpu";
+        return py::make_tuple(out);
+      }
+    }
+    device_ = "cpu";
     const uint8_t mask = (uint8_t)((1u << n_bit_) - 1);
 #pragma omp parallel for schedule(static)
     for (long long i = 0; i < (long long)n; ++i) {
@@ -250,6 +323,7 @@ PYBIND11_MODULE(GPGPUBuildIndexPackGPU, m) {
       .def(py::init<const arr_f32 &, const arr_f32 &, const arr_f32 &, uint32_t>(),
            py::arg("centroid_l"), py::arg("cutoff_l"), py::arg("weight_l"), py::arg("n_bit"))
       .def_readonly("n_val_per_vec", &CompressResidualCode::n_packed_val_per_vec_)
+      .def_readonly("device", &CompressResidualCode::device_)
       .def("compute_residual_code", &CompressResidualCode::compute_residual_code, py::arg("vec_l"),
            py::arg("code_l"));
   py::class_<IVFIndex>(m, "IVFIndex")

@@ -1015,4 +1015,56 @@ int mvr_compress(const float *x_dev, const uint32_t *codes_dev, uint64_t n, cons
   return MVR_OK;
 }
 
+int mvr_compress_host(const float *x, const uint32_t *codes, uint64_t n, const float *C, uint32_t n_c,
+                      uint32_t dim, const float *cutoffs, int per_dim, uint8_t *out, int device) {
+  if (dim != kDim) return fail(MVR_ERR_UNSUPPORTED, "dim must be 128");
+  if (n == 0) return MVR_OK;
+  if (!x || !codes || !C || !cutoffs || !out || n_c == 0) return fail(MVR_ERR_INVALID, "bad arguments");
+  for (uint64_t i = 0; i < n; ++i)
+    if (codes[i] >= n_c)
+      return fail(MVR_ERR_INVALID, "code %u at %llu >= n_centroid %u", codes[i], (unsigned long long)i, n_c);
+  int n_dev = 0;
+  HIP_TRY(hipGetDeviceCount(&n_dev));
+  if (device < 0 || device >= n_dev) return fail(MVR_ERR_INVALID, "device %d out of range (%d)", device, n_dev);
+  HIP_TRY(hipSetDevice(device));
+  const uint64_t chunk = std::min<uint64_t>(n, 1ull << 22);  // 2 GB of embeddings per step
+  float *dx = nullptr, *dC = nullptr, *dcut = nullptr;
+  uint32_t *dcodes = nullptr;
+  uint8_t *dout = nullptr;
+  hipStream_t s = nullptr;
+  auto cleanup = [&]() {
+    if (s) (void)hipStreamSynchronize(s);
+    for (void *p : {(void *)dx, (void *)dC, (void *)dcut, (void *)dcodes, (void *)dout})
+      if (p) (void)hipFree(p);
+    if (s) (void)hipStreamDestroy(s);
+  };
+  const size_t cut_n = per_dim ? (size_t)kDim * 3 : 3;
+  hipError_t e = hipSuccess;
+  if ((e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking)) != hipSuccess ||
+      (e = hipMalloc(&dx, sizeof(float) * chunk * kDim)) != hipSuccess ||
+      (e = hipMalloc(&dcodes, sizeof(uint32_t) * chunk)) != hipSuccess ||
+      (e = hipMalloc(&dout, (size_t)kResBytes * chunk)) != hipSuccess ||
+      (e = hipMalloc(&dC, sizeof(float) * (size_t)n_c * kDim)) != hipSuccess ||
+      (e = hipMalloc(&dcut, sizeof(float) * cut_n)) != hipSuccess ||
+      (e = hipMemcpyAsync(dC, C, sizeof(float) * (size_t)n_c * kDim, hipMemcpyHostToDevice, s)) != hipSuccess ||
+      (e = hipMemcpyAsync(dcut, cutoffs, sizeof(float) * cut_n, hipMemcpyHostToDevice, s)) != hipSuccess) {
+    cleanup();
+    return fail(MVR_ERR_DEVICE, "compress setup failed: %s", hipGetErrorString(e));
+  }
+  for (uint64_t i0 = 0; i0 < n; i0 += chunk) {
+    const uint64_t m = std::min(chunk, n - i0);
+    if ((e = hipMemcpyAsync(dx, x + i0 * kDim, sizeof(float) * m * kDim, hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(dcodes, codes + i0, sizeof(uint32_t) * m, hipMemcpyHostToDevice, s)) != hipSuccess ||
+        (e = launch_compress(dx, dcodes, m, dC, dcut, per_dim, dout, s)) != hipSuccess ||
+        (e = hipMemcpyAsync(out + i0 * kResBytes, dout, (size_t)kResBytes * m, hipMemcpyDeviceToHost, s)) !=
+            hipSuccess ||
+        (e = hipStreamSynchronize(s)) != hipSuccess) {
+      cleanup();
+      return fail(MVR_ERR_DEVICE, "compress failed: %s", hipGetErrorString(e));
+    }
+  }
+  cleanup();
+  return MVR_OK;
+}
+
 }  // extern "C"

@@ -268,6 +268,18 @@ class IGPGPUPP:
         return IGPGPUPPResult(sc.ravel(), ids.ravel(), np.full(n_query, dt), dt, n_query, topk)
 
 
+def bruteforce_module():
+    """Import the BruteForceProgressive pybind module (exhaustive MaxSim,
+    DocRetrieval) from the built lib/."""
+    path = os.path.join(_capi.LIB_DIR, "BruteForceProgressive" + sysconfig.get_config_var("EXT_SUFFIX"))
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built")
+    spec = importlib.util.spec_from_file_location("BruteForceProgressive", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
 def build_index_module():
     """Import the GPGPUBuildIndexPackGPU pybind module from the built lib/."""
     path = os.path.join(_capi.LIB_DIR, "GPGPUBuildIndexPackGPU" + sysconfig.get_config_var("EXT_SUFFIX"))

@@ -109,6 +109,11 @@ SIGNATURES = {
                                   C.c_uint32, C.c_void_p, C.c_void_p]),
     "mvr_compress": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32,
                                C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    "mvr_compress_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint32, C.c_uint32,
+                                    C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
+    "mvr_maxsim_scores_host": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                         C.c_uint32, C.c_void_p, C.c_int]),
+    "mvr_maxsim_last_error": (C.c_char_p, []),
     # include/mvr_plaid.h
     "mvr_plaid_last_error": (C.c_char_p, []),
     "mvr_plaid_decompress_residuals": (C.c_int, [_vp, _u64, _vp, _vp, _vp, _vp, _vp, _u32, _u32, _vp, _vp]),

@@ -785,3 +785,46 @@ void orc_exhaustive_f64(const float *Qall, uint32_t n_q, uint32_t Tmax, const ui
     free(sc); free(M);
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* All-pairs FP32 MaxSim of the exhaustive module (BruteForceProgressive):    */
+/* src/include/compute_score/GPUComputeScore.hpp:196-246 (SGEMM of the query  */
+/* tokens against every item token, select_max_functor :63-89 per item and    */
+/* query token, thrust::plus over query tokens from 0.0f).                    */
+/* ------------------------------------------------------------------------ */
+
+/* scores[q][i] = ((0 + M[0]) + M[1]) + ... over t < T, M[t] = max over item
+ * i's tokens x of dot(Q_q,t, x), each dot one FMA chain over d.  Items are
+ * laid out back to back in X [sum lens][dim]; every item has >= 1 token. */
+void orc_maxsim_scores_f32(const float *Qall, uint32_t n_q, uint32_t T, uint32_t dim, const float *X,
+                           const uint32_t *lens, uint32_t n_item, int n_threads, float *scores) {
+  uint64_t *off = (uint64_t *)malloc(sizeof(uint64_t) * (n_item + 1));
+  off[0] = 0;
+  for (uint32_t i = 0; i < n_item; i++) off[i + 1] = off[i] + lens[i];
+#ifdef _OPENMP
+  if (n_threads > 0) omp_set_num_threads(n_threads);
+#endif
+#pragma omp parallel
+  {
+    float *M = (float *)malloc(sizeof(float) * (T ? T : 1));
+#pragma omp for schedule(dynamic, 1)
+    for (int64_t q = 0; q < (int64_t)n_q; q++) {
+      const float *Q = Qall + (size_t)q * T * dim;
+      for (uint32_t i = 0; i < n_item; i++) {
+        for (uint32_t t = 0; t < T; t++) M[t] = -INFINITY;
+        for (uint64_t r = off[i]; r < off[i + 1]; r++) {
+          const float *x = X + r * dim;
+          for (uint32_t t = 0; t < T; t++) {
+            const float s = orc_dot_f32(Q + (size_t)t * dim, x, dim);
+            if (s > M[t]) M[t] = s;
+          }
+        }
+        float sc = 0.0f;
+        for (uint32_t t = 0; t < T; t++) sc += M[t];
+        scores[(size_t)q * n_item + i] = sc;
+      }
+    }
+    free(M);
+  }
+  free(off);
+}

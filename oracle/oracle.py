@@ -77,6 +77,8 @@ def lib():
                                     _u32p, _f32p, C.c_void_p]),
         "orc_exhaustive_f64": (None, [_f32p, C.c_uint32, C.c_uint32, C.c_void_p, _f32p, _u32p, _u64p,
                                       C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, _u32p, _f64p]),
+        "orc_maxsim_scores_f32": (None, [_f32p, C.c_uint32, C.c_uint32, C.c_uint32, _f32p, _u32p, C.c_uint32,
+                                         C.c_int, _f32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -352,3 +354,16 @@ def recall_at_k(est: np.ndarray, gnd: np.ndarray) -> float:
     for e, g in zip(est, gnd):
         tot += len(set(e[:k].tolist()) & set(g.tolist())) / k
     return tot / len(gnd)
+
+
+def maxsim_scores_f32(Q, X, lens, n_threads=0) -> np.ndarray:
+    """All-pairs FP32 MaxSim scores [n_q, n_item] of the exhaustive module
+    (GPUComputeScore.hpp:196-246): FMA-chain dots, max per item and query
+    token, left-to-right sum over query tokens."""
+    Q = _c(Q, np.float32)
+    X = _c(X, np.float32)
+    lens = _c(lens, np.uint32)
+    n_q, T, dim = Q.shape
+    out = np.empty((n_q, lens.size), np.float32)
+    lib().orc_maxsim_scores_f32(Q, n_q, T, dim, X, lens, lens.size, n_threads, out)
+    return out
