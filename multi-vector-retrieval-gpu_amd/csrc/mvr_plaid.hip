@@ -9,7 +9,7 @@
 //   modeling/segmented_maxsim.cpp:22-93           -> k_pl_segmented_maxsim
 //   search/filter_pids.cpp:30-193                 -> k_pl_approx (both stages) + k_select_topr
 //   search/candidate_generation.py:13-66          -> k_pl_sgemm, k_pl_cell_part/_merge, k_pl_cells,
-//                                                    k_pl_candidates
+//                                                    k_pl_mark, k_pl_compact
 //   search/index_storage.py:101-198 (CPU branch)  -> the batched pipeline of mvr_plaid_search
 //
 // Design: the reference runs one query at a time through pthread loops
@@ -468,13 +468,12 @@ __global__ __launch_bounds__(64) void k_pl_cells(const uint32_t *__restrict__ to
   if (lane == 0) cell_n[q] = min(total, cell_cap);
 }
 
-// Union of the IVF lists of a query's cells: the first setter of a pid's bit
-// appends the pid (one wave-aggregated atomic per 64 list entries).
-__global__ __launch_bounds__(256) void k_pl_candidates(
-    const uint32_t *__restrict__ cells, uint32_t cell_cap, const uint32_t *__restrict__ cell_n,
-    const int32_t *__restrict__ ivf, const int64_t *__restrict__ ivf_off,
-    const int64_t *__restrict__ ivf_len, uint32_t *__restrict__ pid_bm, uint32_t bm_words,
-    uint32_t *__restrict__ cand, uint32_t cand_cap, uint32_t *__restrict__ cand_n) {
+// Union of the IVF lists of a query's cells, as bits of a per-query pid bitmap.
+__global__ __launch_bounds__(256) void k_pl_mark(const uint32_t *__restrict__ cells, uint32_t cell_cap,
+                                                 const uint32_t *__restrict__ cell_n, const int32_t *__restrict__ ivf,
+                                                 const int64_t *__restrict__ ivf_off,
+                                                 const int64_t *__restrict__ ivf_len, uint32_t *__restrict__ pid_bm,
+                                                 uint32_t bm_words) {
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6, q = blockIdx.y;
   const uint32_t nw = gridDim.x * 4;
   const uint32_t n = cell_n[q];
@@ -482,27 +481,65 @@ __global__ __launch_bounds__(256) void k_pl_candidates(
   for (uint32_t ci = blockIdx.x * 4 + wave; ci < n; ci += nw) {
     const uint32_t c = cells[(size_t)q * cell_cap + ci];
     const int64_t off = ivf_off[c], len = ivf_len[c];
-    for (int64_t j0 = 0; j0 < len; j0 += 64) {
-      const int64_t j = j0 + lane;
-      uint32_t pid = 0;
-      bool isnew = false;
-      if (j < len) {
-        pid = (uint32_t)ivf[off + j];
-        const uint32_t bit = 1u << (pid & 31);
-        isnew = !(atomicOr(&bmq[pid >> 5], bit) & bit);
-      }
-      const uint64_t mask = __ballot(isnew);
-      if (mask) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(&cand_n[q], (uint32_t)__popcll(mask));
-        base = (uint32_t)__shfl((int)base, 0);
-        if (isnew) {
-          const uint32_t pos = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-          if (pos < cand_cap) cand[(size_t)q * cand_cap + pos] = pid;
-        }
-      }
+    for (int64_t j = lane; j < len; j += 64) {
+      const uint32_t pid = (uint32_t)ivf[off + j];
+      atomicOr(&bmq[pid >> 5], 1u << (pid & 31));
     }
   }
+}
+
+// Exclusive prefix sum over a 1024-thread workgroup; *total gets the sum.
+__device__ __forceinline__ uint32_t pl_block_scan(uint32_t v, uint32_t *warp_sums, uint32_t *total) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint32_t incl = v;
+#pragma unroll
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint32_t u = (uint32_t)__shfl_up((int)incl, off);
+    if (lane >= off) incl += u;
+  }
+  if (lane == 63) warp_sums[wave] = incl;
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (uint32_t w = 0; w < 16; ++w) {
+    const uint32_t x = warp_sums[w];
+    before += w < wave ? x : 0u;
+    all += x;
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
+}
+
+// The query's candidates in descending pid order (the set of
+// candidate_generation.py:56-64, which sorts ascending), one workgroup of
+// 1024 threads per query; the bitmap words are zeroed as they are read.
+__global__ __launch_bounds__(1024) void k_pl_compact(uint32_t *__restrict__ pid_bm, uint32_t bm_words,
+                                                     uint32_t *__restrict__ cand, uint32_t cand_cap,
+                                                     uint32_t *__restrict__ cand_n) {
+  __shared__ uint32_t warp_sums[16];
+  const uint32_t tid = threadIdx.x, q = blockIdx.x;
+  uint32_t *bmq = pid_bm + (size_t)q * bm_words;
+  uint32_t *cq = cand + (size_t)q * cand_cap;
+  uint32_t total = 0;
+  for (uint32_t base = 0; base < bm_words; base += 1024) {
+    const uint32_t k = base + tid;
+    const uint32_t w = bm_words - 1 - k;  // descending words
+    uint32_t word = 0;
+    if (k < bm_words) {
+      word = bmq[w];
+      if (word) bmq[w] = 0;
+    }
+    uint32_t sum;
+    uint32_t pos = total + pl_block_scan((uint32_t)__popc(word), warp_sums, &sum);
+    while (word) {
+      const uint32_t b = 31u - (uint32_t)__clz(word);
+      word &= ~(1u << b);
+      if (pos < cand_cap) cq[pos] = w * 32 + b;
+      ++pos;
+    }
+    total += sum;
+  }
+  if (tid == 0) cand_n[q] = min(total, cand_cap);
 }
 
 // seg_cnt[q][sg] = number of keys in segment sg when the query has n keys
@@ -515,6 +552,21 @@ __global__ void k_pl_segcnt(const uint32_t *__restrict__ n_in, uint32_t n_fixed,
   const uint32_t n = n_in ? n_in[q] : n_fixed;
   const uint32_t lo = sg * kSeg;
   seg_cnt[e] = n > lo ? min(n - lo, kSeg) : 0u;
+}
+
+// Segment counts of the split layout: heavy keys in segments 0 .. n_seg-2
+// (heavy_n of them, flat), light keys in the last segment (light_n).
+__global__ void k_pl_segcnt_split(const uint32_t *__restrict__ heavy_n, const uint32_t *__restrict__ light_n,
+                                  uint32_t n_seg, uint32_t n_q, uint32_t *__restrict__ seg_cnt) {
+  const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (uint64_t)n_q * n_seg) return;
+  const uint32_t q = (uint32_t)(e / n_seg), sg = (uint32_t)(e % n_seg);
+  if (sg + 1 == n_seg) {
+    seg_cnt[e] = light_n[q];
+  } else {
+    const uint32_t n = heavy_n[q], lo = sg * kSeg;
+    seg_cnt[e] = n > lo ? min(n - lo, kSeg) : 0u;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -543,7 +595,8 @@ __global__ __launch_bounds__(256) void k_pl_approx(
   uint32_t *my = dyn_bm + (size_t)wave * words;
   for (uint32_t w = lane; w < words; w += 64) my[w] = 0;
   pl_wave_sync();
-  // with slots: visit only the listed input positions (slots[q][j], j < slot_n[q])
+  // with slots: visit only the listed input positions (slots[q][j], j < slot_n[q]);
+  // the key of the j-th visited input goes to keys[q][j]
   const uint32_t n = slots ? slot_n[q] : (in_n ? in_n[q] : n_fixed);
   const float *Sq = S + (size_t)q * s_stride;
   const uint8_t *iq = idx ? idx + (size_t)q * idx_stride : nullptr;
@@ -605,7 +658,7 @@ __global__ __launch_bounds__(256) void k_pl_approx(
       for (uint32_t tt = 0; tt < nq; ++tt) score += pl_readlane(m, tt);
     }
     if (lane == 0) {
-      keys[(size_t)q * key_stride + i] = ((unsigned long long)pl_f2ord(score) << 32) | pid;
+      keys[(size_t)q * key_stride + jj] = ((unsigned long long)pl_f2ord(score) << 32) | pid;
       nv += nu;
     }
   }
@@ -675,38 +728,47 @@ __global__ __launch_bounds__(256) void k_pl_passclear(const uint32_t *__restrict
   }
 }
 
-// Stage-1 split, one thread per candidate: clears the candidate's word of
-// pid_bm; a candidate without a passing code gets its final key at once
-// (score -9999 * nq, exact in FP32 for nq <= 32), the others are listed in
-// heavy[q] for the wave scan.
-__global__ __launch_bounds__(256) void k_pl_split(
+// Stage-1 split over the descending-pid candidate list, one workgroup of
+// 1024 threads per query.  Candidates holding a passing code (all of them
+// when pmode is 0) are listed in heavy[q] for the wave scan.  The others keep
+// every maximum at -9999, so their score is exactly -9999 * nq and they rank
+// by pid: only the first ndocs of them (the largest pids) can reach the
+// stage-1 top-ndocs, and their keys go to keys[q][light_base + r].
+__global__ __launch_bounds__(1024) void k_pl_split(
     const uint32_t *__restrict__ cand, uint64_t cand_stride, const uint32_t *__restrict__ cand_n,
-    const uint32_t *__restrict__ pmode, const uint32_t *__restrict__ pass_bm, uint32_t *__restrict__ pid_bm,
-    uint32_t bm_words, uint32_t nq, unsigned long long *__restrict__ keys, uint64_t key_stride,
-    uint32_t *__restrict__ heavy, uint32_t *__restrict__ heavy_n) {
-  const uint32_t tid = threadIdx.x, lane = tid & 63, q = blockIdx.y;
+    const uint32_t *__restrict__ pmode, const uint32_t *__restrict__ pass_bm, uint32_t bm_words, uint32_t nq,
+    uint32_t ndocs, unsigned long long *__restrict__ keys, uint64_t key_stride, uint64_t light_base,
+    uint32_t *__restrict__ heavy, uint32_t *__restrict__ heavy_n, uint32_t *__restrict__ light_n) {
+  __shared__ uint32_t warp_sums[16];
+  const uint32_t tid = threadIdx.x, q = blockIdx.x;
   const uint32_t n = cand_n[q];
   const bool mark = pmode[q] != 0;
   const unsigned long long ck = (unsigned long long)pl_f2ord(-9999.0f * (float)nq) << 32;
   const uint32_t *bq = pass_bm + (size_t)q * bm_words;
-  for (uint32_t base = blockIdx.x * 256; base < n; base += gridDim.x * 256) {
+  uint32_t h_tot = 0, l_tot = 0;
+  for (uint32_t base = 0; base < n; base += 1024) {
     const uint32_t i = base + tid;
     const bool v = i < n;
     uint32_t pid = 0;
     bool hv = false;
     if (v) {
       pid = cand[(size_t)q * cand_stride + i];
-      pid_bm[(size_t)q * bm_words + (pid >> 5)] = 0;
       hv = !mark || ((bq[pid >> 5] >> (pid & 31)) & 1u);
-      if (!hv) keys[(size_t)q * key_stride + i] = ck | pid;
     }
-    const uint64_t m = __ballot(hv);
-    if (m) {
-      uint32_t b0 = 0;
-      if (lane == 0) b0 = atomicAdd(&heavy_n[q], (uint32_t)__popcll(m));
-      b0 = (uint32_t)__shfl((int)b0, 0);
-      if (hv) heavy[(size_t)q * cand_stride + b0 + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = i;
+    // heavy count in the low 16 bits, light count in the high 16 (<= 1024 each)
+    uint32_t sum;
+    const uint32_t ex = pl_block_scan(v ? (hv ? 1u : 0x10000u) : 0u, warp_sums, &sum);
+    if (hv) heavy[(size_t)q * cand_stride + h_tot + (ex & 0xFFFFu)] = i;
+    if (v && !hv) {
+      const uint32_t r = l_tot + (ex >> 16);
+      if (r < ndocs) keys[(size_t)q * key_stride + light_base + r] = ck | pid;
     }
+    h_tot += sum & 0xFFFFu;
+    l_tot += sum >> 16;
+  }
+  if (tid == 0) {
+    heavy_n[q] = h_tot;
+    light_n[q] = min(l_tot, ndocs);
   }
 }
 
@@ -721,11 +783,11 @@ __global__ void k_pl_unkey(const uint32_t *__restrict__ sel, const uint32_t *__r
 // (x = w[bucket] + C[code]), y = x / max(||x||, 1e-12), s_t = FMA chain of
 // y . Q_t, M_t = max(0, max_i s_t), score = sum over t left to right.
 // Wave layout as the IGP rerank: lane (t, h) holds Q_t (128 floats) and the
-// wave takes 8 doc tokens per step: decode into LDS, 8 lanes run the norm
-// chains, every lane divides its 4 dims of 4 tokens, then four dot chains.
+// wave takes 16 doc tokens per step: decode into LDS, 16 lanes run the norm
+// chains, every lane divides its 4 dims of 8 tokens, then eight dot chains.
 // tab4[v] = the four weights of raw PLAID byte v (rbm, lut and bw folded).
 // ---------------------------------------------------------------------------
-constexpr uint32_t kPlTok = 8;
+constexpr uint32_t kPlTok = 16;
 
 __global__ __launch_bounds__(256) void k_pl_refine(
     const float *__restrict__ Q, uint32_t nq, const uint32_t *__restrict__ sel_ids,
@@ -906,18 +968,21 @@ thread_local OpScratch g_ops;
 
 // Launch the two filter stages for n_q queries.  Stage-1 input: pids
 // (in_ids, in_n / n_fixed, row stride in_stride) with their S, idx rows.
-// slots / slot_n (nullable): stage 1 scans only those input positions; the
-// keys of the others are already in keys1 (k_pl_split).
+// slots / slot_n / light_n (nullable): stage 1 scans only the listed input
+// positions, writing their keys flat from keys1[q][0]; light_n[q] more keys
+// already sit in the extra last segment (k_pl_split).
 hipError_t launch_filter_stages(const uint32_t *in_ids, const uint32_t *in_n, uint32_t n_fixed,
                                 uint64_t in_stride, uint32_t in_cap, const uint32_t *slots,
-                                const uint32_t *slot_n, const float *S, const uint8_t *idx,
+                                const uint32_t *slot_n, const uint32_t *light_n, const float *S,
+                                const uint8_t *idx,
                                 uint32_t n_c, uint32_t nq, uint32_t n_doc, const int32_t *codes,
                                 const int64_t *doclens, const int64_t *offsets, uint32_t *pid_bm,
                                 uint32_t bm_words, uint32_t n_q, uint32_t ndocs, uint64_t *keys1,
                                 uint32_t *seg1, uint32_t *sel1_ids, float *sel1_F, uint32_t *sel1_n,
                                 uint64_t *keys2, uint32_t *sel2_ids, float *sel2_F, uint32_t *sel2_n,
                                 unsigned long long *nvec, hipStream_t s) {
-  const uint32_t n_seg1 = std::max(1u, (in_cap + kSeg - 1) / kSeg);
+  // with slots (split layout): heavy keys flat from 0, light keys in one more segment
+  const uint32_t n_seg1 = std::max(1u, (in_cap + kSeg - 1) / kSeg) + (slots ? 1u : 0u);
   const uint32_t wpb = approx_waves(n_c);
   const size_t lds = (size_t)wpb * ((n_c + 31) / 32) * 4;
   const uint32_t gx = std::max(1u, std::min(64u, 4096u / std::max(1u, n_q)));
@@ -929,7 +994,10 @@ hipError_t launch_filter_stages(const uint32_t *in_ids, const uint32_t *in_n, ui
                                                   (uint64_t)n_seg1 * kSeg, nvec);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  k_pl_segcnt<<<grid_for((uint64_t)n_q * n_seg1, 256), 256, 0, s>>>(in_n, n_fixed, n_seg1, n_q, seg1);
+  if (slots)
+    k_pl_segcnt_split<<<grid_for((uint64_t)n_q * n_seg1, 256), 256, 0, s>>>(slot_n, light_n, n_seg1, n_q, seg1);
+  else
+    k_pl_segcnt<<<grid_for((uint64_t)n_q * n_seg1, 256), 256, 0, s>>>(in_n, n_fixed, n_seg1, n_q, seg1);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = launch_select(keys1, seg1, n_seg1, nullptr, n_q, ndocs, sel1_ids, sel1_F, sel1_n, nullptr, s)) !=
       hipSuccess)
@@ -967,11 +1035,11 @@ struct mvr_plaid_index {
   bool bm_dirty = true;  // pid bitmap must be cleared before the next pass
   uint32_t q_nq = 0;     // query rows last copied into the padded Q buffer
   float avg_len = 0.0f;  // mean doc length (tokens)
-  PBuf part_ids, part_s, top_ids, top1, pass_list, pass_cnt, pmode, pass_bm, heavy, heavy_n;
+  PBuf part_ids, part_s, top_ids, top1, pass_list, pass_cnt, pmode, pass_bm, heavy, heavy_n, light_n;
   PBuf Q, S, idx, cells, cell_n, pid_bm, cand, cand_n, keys1, seg1, sel1_ids, sel1_F, sel1_n, keys2, sel2_ids,
       sel2_F, sel2_n, ids2, score2, out_ids, out_s, nvec;
   std::vector<PBuf *> bufs() {
-    return {&part_ids, &part_s, &top_ids, &top1, &pass_list, &pass_cnt, &pmode, &pass_bm, &heavy, &heavy_n, &Q, &S, &idx, &cells, &cell_n, &pid_bm, &cand, &cand_n, &keys1, &seg1, &sel1_ids, &sel1_F, &sel1_n,
+    return {&part_ids, &part_s, &top_ids, &top1, &pass_list, &pass_cnt, &pmode, &pass_bm, &heavy, &heavy_n, &light_n, &Q, &S, &idx, &cells, &cell_n, &pid_bm, &cand, &cand_n, &keys1, &seg1, &sel1_ids, &sel1_F, &sel1_n,
             &keys2, &sel2_ids, &sel2_F, &sel2_n, &ids2, &score2, &out_ids, &out_s, &nvec};
   }
 };
@@ -1103,7 +1171,7 @@ int mvr_plaid_filter_pids(const int32_t *pids, uint32_t npids, const float *cent
   PL_TRY(w.nvec.ensure(sizeof(unsigned long long)));
   PL_TRY(launch_fill_u32(w.nvec.as<uint32_t>(), 0, 2, s));
   PL_TRY(launch_filter_stages(reinterpret_cast<const uint32_t *>(pids), nullptr, npids, 0, npids, nullptr,
-                              nullptr, centroid_scores, idx, n_centroid, nq, 0x7FFFFFFFu, codes, doclens, offsets,
+                              nullptr, nullptr, centroid_scores, idx, n_centroid, nq, 0x7FFFFFFFu, codes, doclens, offsets,
                               nullptr, 0, 1, ndocs, w.keys.as<uint64_t>(), w.seg.as<uint32_t>(),
                               w.sel1_ids.as<uint32_t>(), w.sel1_F.as<float>(), w.sel1_n.as<uint32_t>(),
                               w.keys2.as<uint64_t>(), w.sel2_ids.as<uint32_t>(), w.sel2_F.as<float>(),
@@ -1263,8 +1331,8 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
                        8 * (((size_t)n_c + kGt - 1) / kGt) * kMaxQTok +
                        sizeof(uint32_t) * ((size_t)n_doc + ix->bm_words + kPassCap + 3) +
                        sizeof(uint32_t) * (cell_cap + 1) + sizeof(uint32_t) * ix->bm_words +
-                       sizeof(uint32_t) * (n_doc + 1) + sizeof(uint64_t) * (size_t)n_seg1 * kSeg +
-                       sizeof(uint32_t) * n_seg1 + 12 * (size_t)ndocs + 8 * (size_t)kSeg + 16 * (size_t)R2 +
+                       sizeof(uint32_t) * (n_doc + 1) + sizeof(uint64_t) * (size_t)(n_seg1 + 1) * kSeg +
+                       sizeof(uint32_t) * (n_seg1 + 2) + 12 * (size_t)ndocs + 8 * (size_t)kSeg + 16 * (size_t)R2 +
                        8 * (size_t)k + 16;
   uint32_t B = p->batch ? p->batch : 1024;
   B = std::min<uint32_t>(B, n_q);
@@ -1299,8 +1367,9 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
   PL_TRY(ix->heavy_n.ensure(sizeof(uint32_t) * B));
   PL_TRY(ix->cand.ensure(sizeof(uint32_t) * (size_t)B * n_doc));
   PL_TRY(ix->cand_n.ensure(sizeof(uint32_t) * B));
-  PL_TRY(ix->keys1.ensure(sizeof(uint64_t) * (size_t)B * n_seg1 * kSeg));
-  PL_TRY(ix->seg1.ensure(sizeof(uint32_t) * (size_t)B * n_seg1));
+  PL_TRY(ix->keys1.ensure(sizeof(uint64_t) * (size_t)B * (n_seg1 + 1) * kSeg));
+  PL_TRY(ix->seg1.ensure(sizeof(uint32_t) * (size_t)B * (n_seg1 + 1)));
+  PL_TRY(ix->light_n.ensure(sizeof(uint32_t) * B));
   PL_TRY(ix->sel1_ids.ensure(sizeof(uint32_t) * (size_t)B * ndocs));
   PL_TRY(ix->sel1_F.ensure(sizeof(float) * (size_t)B * ndocs));
   PL_TRY(ix->sel1_n.ensure(sizeof(uint32_t) * B));
@@ -1364,14 +1433,14 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
                                                                    ix->cells.as<uint32_t>(), cell_cap,
                                                                    ix->cell_n.as<uint32_t>());
     PL_TRY(hipGetLastError());
-    PL_TRY(launch_fill_u32(ix->cand_n.as<uint32_t>(), 0, b, s));
     PL_TRY(launch_fill_u32(ix->nvec.as<uint32_t>(), 0, 2 * (size_t)b, s));
     {
       const uint32_t gx = std::max(1u, std::min(16u, 2048u / b));
-      k_pl_candidates<<<dim3(gx, b), 256, 0, s>>>(ix->cells.as<uint32_t>(), cell_cap, ix->cell_n.as<uint32_t>(),
-                                                  ix->ivf, ix->ivf_off, ix->ivf_len, ix->pid_bm.as<uint32_t>(),
-                                                  ix->bm_words, ix->cand.as<uint32_t>(), n_doc,
-                                                  ix->cand_n.as<uint32_t>());
+      k_pl_mark<<<dim3(gx, b), 256, 0, s>>>(ix->cells.as<uint32_t>(), cell_cap, ix->cell_n.as<uint32_t>(), ix->ivf,
+                                            ix->ivf_off, ix->ivf_len, ix->pid_bm.as<uint32_t>(), ix->bm_words);
+      PL_TRY(hipGetLastError());
+      k_pl_compact<<<b, 1024, 0, s>>>(ix->pid_bm.as<uint32_t>(), ix->bm_words, ix->cand.as<uint32_t>(), n_doc,
+                                      ix->cand_n.as<uint32_t>());
       PL_TRY(hipGetLastError());
     }
     PL_TRY(hipEventRecord(ix->ev[1], s));
@@ -1380,19 +1449,17 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
                                     ix->pass_cnt.as<uint32_t>(), ix->pmode.as<uint32_t>(),
                                     ix->pass_bm.as<uint32_t>(), ix->bm_words);
     PL_TRY(hipGetLastError());
-    PL_TRY(launch_fill_u32(ix->heavy_n.as<uint32_t>(), 0, b, s));
     {
-      const uint32_t n_seg1 = (n_doc + kSeg - 1) / kSeg;
-      const uint32_t gx = std::max(1u, std::min(64u, 16384u / b));
-      k_pl_split<<<dim3(gx, b), 256, 0, s>>>(ix->cand.as<uint32_t>(), n_doc, ix->cand_n.as<uint32_t>(),
-                                             ix->pmode.as<uint32_t>(), ix->pass_bm.as<uint32_t>(),
-                                             ix->pid_bm.as<uint32_t>(), ix->bm_words, nq,
-                                             ix->keys1.as<unsigned long long>(), (uint64_t)n_seg1 * kSeg,
-                                             ix->heavy.as<uint32_t>(), ix->heavy_n.as<uint32_t>());
+      const uint32_t n_seg1 = (n_doc + kSeg - 1) / kSeg + 1;
+      k_pl_split<<<b, 1024, 0, s>>>(ix->cand.as<uint32_t>(), n_doc, ix->cand_n.as<uint32_t>(), ix->pmode.as<uint32_t>(),
+                                    ix->pass_bm.as<uint32_t>(), ix->bm_words, nq, ndocs,
+                                    ix->keys1.as<unsigned long long>(), (uint64_t)n_seg1 * kSeg,
+                                    (uint64_t)(n_seg1 - 1) * kSeg, ix->heavy.as<uint32_t>(),
+                                    ix->heavy_n.as<uint32_t>(), ix->light_n.as<uint32_t>());
       PL_TRY(hipGetLastError());
     }
     PL_TRY(launch_filter_stages(ix->cand.as<uint32_t>(), ix->cand_n.as<uint32_t>(), 0, n_doc, n_doc,
-                                ix->heavy.as<uint32_t>(), ix->heavy_n.as<uint32_t>(),
+                                ix->heavy.as<uint32_t>(), ix->heavy_n.as<uint32_t>(), ix->light_n.as<uint32_t>(),
                                 ix->S.as<float>(), ix->idx.as<uint8_t>(), n_c, nq, n_doc, ix->codes, ix->doclens,
                                 ix->offsets, nullptr, ix->bm_words, b, ndocs,
                                 ix->keys1.as<uint64_t>(), ix->seg1.as<uint32_t>(), ix->sel1_ids.as<uint32_t>(),
@@ -1404,8 +1471,8 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
     PL_TRY(hipGetLastError());
     PL_TRY(hipEventRecord(ix->ev[2], s));
     {
-      // about 8 docs per wave: each wave holds its query's 32 rows in registers
-      const uint32_t n_slice = std::max(1u, std::min((R2 + 31) / 32, std::max(4u, 8192u / b)));
+      // about 4 docs per wave: each wave holds its query's 32 rows in registers
+      const uint32_t n_slice = std::max(1u, std::min((R2 + 15) / 16, std::max(4u, 16384u / b)));
       k_pl_refine<<<dim3(n_slice, b), 256, 0, s>>>(ix->Q.as<float>(), nq, ix->sel2_ids.as<uint32_t>(),
                                                    ix->sel2_n.as<uint32_t>(), R2, n_slice, n_doc, ix->doclens,
                                                    ix->offsets, ix->codes, ix->res, ix->C, ix->tab4,
@@ -1423,7 +1490,7 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
     PL_TRY(hipMemcpyAsync(h_fin.data(), ix->sel2_n.p, sizeof(uint32_t) * b, hipMemcpyDeviceToHost, s));
     PL_TRY(hipMemcpyAsync(h_nv.data(), ix->nvec.p, sizeof(unsigned long long) * b, hipMemcpyDeviceToHost, s));
     PL_TRY(hipStreamSynchronize(s));
-    ix->bm_dirty = false;  // k_pl_split and k_pl_passclear cleared every word set in the pass
+    ix->bm_dirty = false;  // k_pl_compact and k_pl_passclear cleared every word set in the pass
     float ms = 0;
     PL_TRY(hipEventElapsedTime(&ms, ix->ev[0], ix->ev[1]));
     t_cand += ms;
