@@ -322,6 +322,7 @@ __global__ __launch_bounds__(256) void k_pl_top1_merge(const unsigned long long 
   const uint32_t q = e / kMaxQTok, t = e % kMaxQTok;
   if (q >= n_q || t >= nq) return;
   unsigned long long best = 0ull;
+#pragma unroll 8
   for (uint32_t i = 0; i < n_tiles; ++i) {
     const unsigned long long k = top1[((size_t)q * n_tiles + i) * kMaxQTok + t];
     best = k > best ? k : best;
@@ -626,18 +627,34 @@ __global__ __launch_bounds__(256) void k_pl_approx(
       }
       uint64_t mask = __ballot(isnew);
       nu += (uint32_t)__popcll(mask);
+      // up to 8 new codes per round (lane half h takes every other one), the
+      // S-row loads of a round issued before any is used
       while (mask) {
-        const uint32_t l0 = (uint32_t)__ffsll((unsigned long long)mask) - 1;
-        mask &= mask - 1;
-        uint32_t l1 = l0;
-        if (mask) {
-          l1 = (uint32_t)__ffsll((unsigned long long)mask) - 1;
-          mask &= mask - 1;
+        uint32_t cc[4];
+#pragma unroll
+        for (uint32_t k = 0; k < 4; ++k) {
+          uint32_t l0 = 64, l1 = 64;
+          if (mask) {
+            l0 = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+            mask &= mask - 1;
+            l1 = l0;
+            if (mask) {
+              l1 = (uint32_t)__ffsll((unsigned long long)mask) - 1;
+              mask &= mask - 1;
+            }
+          }
+          // an empty slot repeats the first code of the round (max is idempotent)
+          const uint32_t c0 = l0 < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)code, l0) : cc[0];
+          const uint32_t c1 = l1 < 64 ? (uint32_t)__builtin_amdgcn_readlane((int)code, l1) : cc[0];
+          cc[k] = h ? c1 : c0;
         }
-        const uint32_t c0 = (uint32_t)__builtin_amdgcn_readlane((int)code, l0);
-        const uint32_t c1 = (uint32_t)__builtin_amdgcn_readlane((int)code, l1);
-        const uint32_t cc = h ? c1 : c0;
-        if (t < nq) m = pl_max(m, Sq[(size_t)cc * nq + t]);
+        if (t < nq) {
+          float v[4];
+#pragma unroll
+          for (uint32_t k = 0; k < 4; ++k) v[k] = Sq[(size_t)cc[k] * nq + t];
+#pragma unroll
+          for (uint32_t k = 0; k < 4; ++k) m = pl_max(m, v[k]);
+        }
       }
     }
     // a doc without passing codes keeps every maximum at -9999: its score is
@@ -783,20 +800,48 @@ __global__ void k_pl_unkey(const uint32_t *__restrict__ sel, const uint32_t *__r
 // (x = w[bucket] + C[code]), y = x / max(||x||, 1e-12), s_t = FMA chain of
 // y . Q_t, M_t = max(0, max_i s_t), score = sum over t left to right.
 // Wave layout as the IGP rerank: lane (t, h) holds Q_t (128 floats) and the
-// wave takes 16 doc tokens per step: decode into LDS, 16 lanes run the norm
-// chains, every lane divides its 4 dims of 8 tokens, then eight dot chains.
+// wave takes 16 doc tokens per step: every lane decodes its 4 dims of 8
+// tokens and divides them by the token's denominator, then runs eight dot
+// chains.  The denominator max(||x||, 1e-12) depends on the token alone, so it
+// is computed once per index (k_pl_token_den, same FMA chain) and read here.
 // tab4[v] = the four weights of raw PLAID byte v (rbm, lut and bw folded).
 // ---------------------------------------------------------------------------
 constexpr uint32_t kPlTok = 16;
+
+// den[tok] = max(sqrtf(FMA chain over d of x_d^2), 1e-12), x the FP32
+// decompressed token (index_storage.py:182, residual.py:273-274).
+__global__ __launch_bounds__(256) void k_pl_token_den(const int32_t *__restrict__ codes, const uint8_t *__restrict__ res,
+                                                      uint64_t n_tok, const float *__restrict__ C,
+                                                      const float4 *__restrict__ tab4, float *__restrict__ den) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n_tok; g += stride) {
+    const float4 *c4 = reinterpret_cast<const float4 *>(C + (size_t)(uint32_t)codes[g] * kDim);
+    const uint8_t *rb = res + g * kResBytes;
+    float n2 = 0.0f;
+#pragma unroll 4
+    for (uint32_t j = 0; j < kResBytes; ++j) {
+      const float4 w = tab4[rb[j]];
+      const float4 cv = c4[j];
+      const float x0 = w.x + cv.x, x1 = w.y + cv.y, x2 = w.z + cv.z, x3 = w.w + cv.w;
+      n2 = __builtin_fmaf(x0, x0, n2);
+      n2 = __builtin_fmaf(x1, x1, n2);
+      n2 = __builtin_fmaf(x2, x2, n2);
+      n2 = __builtin_fmaf(x3, x3, n2);
+    }
+    // sqrtf and '/' compile to correctly rounded sequences (HIP's default
+    // -fhip-fp32-correctly-rounded-divide-sqrt); __fsqrt_rn does not
+    den[g] = fmaxf(sqrtf(n2), 1e-12f);
+  }
+}
 
 __global__ __launch_bounds__(256) void k_pl_refine(
     const float *__restrict__ Q, uint32_t nq, const uint32_t *__restrict__ sel_ids,
     const uint32_t *__restrict__ sel_n, uint32_t R, uint32_t n_slice, uint32_t n_doc,
     const int64_t *__restrict__ doclens, const int64_t *__restrict__ offsets,
     const int32_t *__restrict__ codes, const uint8_t *__restrict__ res, const float *__restrict__ C,
-    const float4 *__restrict__ tab4, uint32_t *__restrict__ out_ids, float *__restrict__ out_score) {
+    const float4 *__restrict__ tab4, const float *__restrict__ den, uint32_t *__restrict__ out_ids,
+    float *__restrict__ out_score) {
   __shared__ __align__(16) float4 xs[4][kPlTok][kDim / 4];
-  __shared__ float dens[4][kPlTok];
   __shared__ float Ms[4][kMaxQTok];
   __shared__ float4 tab[256];
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -828,32 +873,8 @@ __global__ __launch_bounds__(256) void k_pl_refine(
           const uint32_t code = (uint32_t)codes[g];
           const float4 w = tab[res[g * kResBytes + t]];
           const float4 cv = reinterpret_cast<const float4 *>(C + (size_t)code * kDim)[t];
-          xs[wave][tk][t] = make_float4(w.x + cv.x, w.y + cv.y, w.z + cv.z, w.w + cv.w);
-        }
-      }
-      pl_wave_sync();
-      if (lane < kPlTok && i + lane < L) {
-        float n2 = 0.0f;
-#pragma unroll 8
-        for (uint32_t d4 = 0; d4 < kDim / 4; ++d4) {
-          const float4 v = xs[wave][lane][d4];
-          n2 = __builtin_fmaf(v.x, v.x, n2);
-          n2 = __builtin_fmaf(v.y, v.y, n2);
-          n2 = __builtin_fmaf(v.z, v.z, n2);
-          n2 = __builtin_fmaf(v.w, v.w, n2);
-        }
-        // sqrtf and '/' compile to correctly rounded sequences (HIP's default
-        // -fhip-fp32-correctly-rounded-divide-sqrt); __fsqrt_rn does not
-        dens[wave][lane] = fmaxf(sqrtf(n2), 1e-12f);
-      }
-      pl_wave_sync();
-#pragma unroll
-      for (uint32_t c = 0; c < kPlTok / 2; ++c) {
-        const uint32_t tk = 2 * c + h;
-        if (i + tk < L) {
-          const float den = dens[wave][tk];
-          const float4 v = xs[wave][tk][t];
-          xs[wave][tk][t] = make_float4(v.x / den, v.y / den, v.z / den, v.w / den);
+          const float dn = den[g];
+          xs[wave][tk][t] = make_float4((w.x + cv.x) / dn, (w.y + cv.y) / dn, (w.z + cv.z) / dn, (w.w + cv.w) / dn);
         }
       }
       pl_wave_sync();
@@ -985,7 +1006,8 @@ hipError_t launch_filter_stages(const uint32_t *in_ids, const uint32_t *in_n, ui
   const uint32_t n_seg1 = std::max(1u, (in_cap + kSeg - 1) / kSeg) + (slots ? 1u : 0u);
   const uint32_t wpb = approx_waves(n_c);
   const size_t lds = (size_t)wpb * ((n_c + 31) / 32) * 4;
-  const uint32_t gx = std::max(1u, std::min(64u, 4096u / std::max(1u, n_q)));
+  // about 16k workgroups: one wave per doc is latency-bound, so many in flight
+  const uint32_t gx = std::max(1u, std::min(64u, 16384u / std::max(1u, n_q)));
   dim3 grid(gx, n_q);
   k_pl_approx<true, false><<<grid, 256, lds, s>>>(in_ids, in_n, n_fixed, in_stride, slots, slot_n, S,
                                                   (uint64_t)n_c * nq,
@@ -1030,6 +1052,7 @@ struct mvr_plaid_index {
   int32_t *ivf = nullptr;
   int64_t *ivf_off = nullptr, *ivf_len = nullptr;
   float4 *tab4 = nullptr;
+  float *den = nullptr;  // [n_tok] per-token normalisation denominators
   hipStream_t stream = nullptr;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool bm_dirty = true;  // pid bitmap must be cleared before the next pass
@@ -1283,8 +1306,15 @@ int mvr_plaid_index_create(const mvr_plaid_index_desc *d, int device, mvr_plaid_
       (e = up((void **)&ix->ivf_off, ivf_off.data(), sizeof(int64_t) * d->n_centroid)) != hipSuccess ||
       (e = up((void **)&ix->ivf_len, d->ivf_lengths, sizeof(int64_t) * d->n_centroid)) != hipSuccess ||
       (e = up((void **)&ix->tab4, tab.data(), sizeof(float) * tab.size())) != hipSuccess ||
+      (e = hipMalloc((void **)&ix->den, sizeof(float) * std::max<uint64_t>(d->n_tok, 1))) != hipSuccess ||
       (e = hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking)) != hipSuccess)
     return bail(pl_fail(MVR_ERR_DEVICE, "index upload failed: %s", hipGetErrorString(e)));
+  if (d->n_tok) {
+    k_pl_token_den<<<grid_for(d->n_tok, 256), 256, 0, ix->stream>>>(ix->codes, ix->res, d->n_tok, ix->C, ix->tab4,
+                                                                   ix->den);
+    if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(ix->stream)) != hipSuccess)
+      return bail(pl_fail(MVR_ERR_DEVICE, "token norms failed: %s", hipGetErrorString(e)));
+  }
   for (auto &ev : ix->ev)
     if ((e = hipEventCreate(&ev)) != hipSuccess)
       return bail(pl_fail(MVR_ERR_DEVICE, "event create failed: %s", hipGetErrorString(e)));
@@ -1297,7 +1327,7 @@ void mvr_plaid_index_free(mvr_plaid_index *ix) {
   (void)hipSetDevice(ix->device);
   if (ix->stream) (void)hipStreamSynchronize(ix->stream);
   for (void *p : {(void *)ix->C, (void *)ix->codes, (void *)ix->res, (void *)ix->doclens, (void *)ix->offsets,
-                  (void *)ix->ivf, (void *)ix->ivf_off, (void *)ix->ivf_len, (void *)ix->tab4})
+                  (void *)ix->ivf, (void *)ix->ivf_off, (void *)ix->ivf_len, (void *)ix->tab4, (void *)ix->den})
     if (p) (void)hipFree(p);
   for (PBuf *b : ix->bufs()) b->release();
   for (auto &ev : ix->ev)
@@ -1475,7 +1505,7 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
       const uint32_t n_slice = std::max(1u, std::min((R2 + 15) / 16, std::max(4u, 16384u / b)));
       k_pl_refine<<<dim3(n_slice, b), 256, 0, s>>>(ix->Q.as<float>(), nq, ix->sel2_ids.as<uint32_t>(),
                                                    ix->sel2_n.as<uint32_t>(), R2, n_slice, n_doc, ix->doclens,
-                                                   ix->offsets, ix->codes, ix->res, ix->C, ix->tab4,
+                                                   ix->offsets, ix->codes, ix->res, ix->C, ix->tab4, ix->den,
                                                    ix->ids2.as<uint32_t>(), ix->score2.as<float>());
       PL_TRY(hipGetLastError());
     }
