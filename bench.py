@@ -75,6 +75,9 @@ def main():
     ap.add_argument("--persistent", action="store_true",
                     help="MaxSim + top-k as one persistent work-queue kernel per pass")
     ap.add_argument("--graph", action="store_true", help="replay each device pass from a hipGraph")
+    ap.add_argument("--batch", type=int, default=1024, help="queries per device pass")
+    ap.add_argument("--residual-on-host", action="store_true",
+                    help="codes and residuals in pinned host memory, staged per device pass")
     ap.add_argument("--epochs", type=int, default=10, help="training epochs of the learned-probe configs")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration)")
@@ -108,6 +111,9 @@ def main():
                                 n_iter=cfg["n_iter"], kmeans_sample=cfg["kmeans_sample"],
                                 log=lambda m: log("[bench]", m))
     ix = corpus.index
+    if args.residual_on_host:
+        ix = mvr.Index.from_device(torch.from_numpy(doclens.astype(np.int32)).cuda(), corpus.C, corpus.codes,
+                                   corpus.residual, corpus.W, corpus.ivf, residual_on_host=True)
     n_q, P, R, k = cfg["n_q"], cfg["P"], cfg["R"], cfg["k"]
     Q = synth.make_queries(torch, args.seed, corpus.centers, n_q)
     q_nt = q_np = None
@@ -120,7 +126,7 @@ def main():
     log(f"[bench] setup {time.time() - t_setup:.1f}s; max IVF list {ix.max_list}")
     stream = torch.cuda.current_stream().cuda_stream
     rescore = args.rescore if args.precision in ("bf16", "fp8") else 0
-    batch = 1024
+    batch = args.batch
 
     def step(timings=False):
         if timings:
@@ -162,6 +168,10 @@ def main():
     qps = n_q * args.steps / elapsed
     log(f"[bench] {args.steps} steps: {ms_per_step:.2f} ms/step, {qps:.0f} QPS")
     stages = {f: acc[f] / args.steps for f in stage_keys}
+    if args.residual_on_host:
+        log(f"[bench] host residual staging: {tm['staging_ms']:.2f} ms, {tm['staged_docs']} docs / "
+            f"{tm['staged_tokens']} tokens per step ({36.0 * tm['staged_tokens'] / tm['staging_ms'] / 1e6:.1f} GB/s "
+            f"over the staging time)")
     log(f"[bench] stages (ms/step): " + ", ".join(f"{f}={stages[f]:.2f}" for f in stage_keys))
 
     # per-pass latency: single 1,024-query passes bracketed by events on the search stream
@@ -203,7 +213,8 @@ def main():
                                                                        f"k_rerank_{args.precision}")
     traffic = measured_traffic(kname, args.config, args.precision)
 
-    recall = recall_report(torch, args, cfg, corpus, ix, Q, out_ids, k, doclens, q_nt)
+    # the exhaustive 2-bit ranking needs the device-resident index
+    recall = recall_report(torch, args, cfg, corpus, corpus.index, Q, out_ids, k, doclens, q_nt)
     cpu = None
     if not args.no_cpu and args.cpu_queries > 0 and q_nt is None:
         cpu = cpu_baseline(torch, corpus, Q, P, R, k, args.cpu_queries)
@@ -227,7 +238,8 @@ def main():
                                   else f"32 tokens; nprobe {P}") + f", probe_topk {R}, k {k}",
                    "persistent": args.persistent, "graph": args.graph,
                    "tuning": (ix.get_tuning(args.precision) if args.persistent else None),
-                   "batch_per_pass": batch, "precision": args.precision, "rescore": rescore},
+                   "batch_per_pass": batch, "precision": args.precision, "rescore": rescore,
+                   "residual_on_host": args.residual_on_host},
         "qps_host_buffers": round(qps_host, 1),
         "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
                        "p99": round(float(np.percentile(lat, 99)), 3),
@@ -235,6 +247,9 @@ def main():
         "recall_at_k": recall.get("search_vs_fp64") if recall else None,
         "recall": recall,
         "stages_ms_per_step": {f: round(stages[f], 3) for f in stage_keys},
+        "host_staging": ({"ms_per_step": round(tm["staging_ms"], 3), "docs_per_step": tm["staged_docs"],
+                          "tokens_per_step": tm["staged_tokens"], "bytes_per_token": 36}
+                         if args.residual_on_host else None),
         "roofline": roofline(kname, achieved_tf, peak, bytes_alg, avg_launch_ms, n_launch, tok_per_launch,
                              flops, traffic),
         "cpu_baseline": cpu,

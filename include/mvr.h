@@ -8,7 +8,7 @@
  *
  * Reference interfaces each entry point replaces (paths relative to the
  * reference repository root):
- *   mvr_index_create / mvr_index_load
+ *   mvr_index_create / mvr_index_load (and the _ex variants)
  *       IGPGPUPP ctor + IGPGPUPP::buildIndex  src/impl/IGPGPUPP.hpp:67-251
  *       (npy loads :124-173, readIVFIndex src/include/struct/IVFIndex.hpp:27-59,
  *        device upload src/include/alg/igp_gpu_pp/DataStructPP.hpp:27-205)
@@ -107,12 +107,17 @@ typedef struct {
  * the lengths of every reranked doc) and filter candidates, summed over queries.
  * rerank_ms is the MaxSim kernel alone; rescore_ms the FP32 rescore of a
  * BF16/FP8 search (its top-N selection and FP32 MaxSim; 0 otherwise).  A pass
- * replayed from a hipGraph only adds to total_ms. */
+ * replayed from a hipGraph only adds to total_ms.  With host-resident
+ * residuals, rerank_ms also holds staging_ms, the time from the end of the
+ * selection to the staged docs being on the device, and staged_docs /
+ * staged_tokens count the docs copied from host memory (each once per pass). */
 typedef struct {
   double centroid_ms, filter_ms, select_ms, rerank_ms, rescore_ms, topk_ms, total_ms;
   uint64_t rerank_tokens, candidates;
   uint32_t n_pass;        /* device passes the call made */
   uint64_t rerank_pairs;  /* sum over queries of (query tokens x reranked doc tokens) */
+  double staging_ms;
+  uint64_t staged_docs, staged_tokens;
 } mvr_timings;
 
 int mvr_device_count(int *count);
@@ -122,6 +127,21 @@ int mvr_index_create(const mvr_index_desc *desc, int device, mvr_index **out);
 /* Loads centroid_l.npy, vq_code_l.npy, weight_l.npy, residual_code_l.npy,
  * item_n_vec_l.npy and ivf.index from index_dir (IGPGPUPP.cpp:33-73). */
 int mvr_index_load(const char *index_dir, int device, mvr_index **out);
+
+/* Index flags for the _ex variants.
+ * MVR_INDEX_RESIDUAL_ON_HOST: codes and residual bytes stay in host memory
+ * (the residuals pinned), for corpora whose 36 bytes per token do not fit in
+ * HBM; only centroids, doc lengths and the IVF lists go to the device.  It is
+ * the reference's host-resident-codes rerank (RerankPP::rerank and
+ * transmitSQCode, src/include/alg/igp_gpu_pp/RerankPP.hpp:289-361): each
+ * device pass of mvr_search reads back its selected doc ids, gathers those
+ * docs' codes and residual rows on the host and uploads them once for the
+ * MaxSim stages.  Results are identical to a device-resident index.  Graph
+ * replay is not used for such an index, and mvr_rerank_docs and
+ * mvr_decompress return MVR_ERR_UNSUPPORTED. */
+#define MVR_INDEX_RESIDUAL_ON_HOST 1u
+int mvr_index_create_ex(const mvr_index_desc *desc, int device, uint32_t flags, mvr_index **out);
+int mvr_index_load_ex(const char *index_dir, int device, uint32_t flags, mvr_index **out);
 void mvr_index_free(mvr_index *ix);
 int mvr_index_info(const mvr_index *ix, uint32_t *n_doc, uint64_t *n_tok, uint32_t *n_centroid,
                    uint32_t *max_list, uint32_t *max_doclen);

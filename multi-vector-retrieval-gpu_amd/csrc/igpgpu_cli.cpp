@@ -13,7 +13,7 @@
 // Flag grammar follows deps/cmd_line_parser/include/parser.hpp:20-60: each
 // flag once, "-name value", unknown flags are errors, every required flag must
 // appear.  Extra optional flags: -precision fp32|fp32_mfma|bf16|fp8, -rescore N,
-// -batch N.  The dataset root /home/{username}/Dataset/multi-vector-retrieval-GPU
+// -batch N, -residual_on_host true|false.  The dataset root /home/{username}/Dataset/multi-vector-retrieval-GPU
 // can be replaced with the MVR_DATASET_ROOT environment variable.
 #include "mvr.h"
 
@@ -146,6 +146,7 @@ int main(int argc, char **argv) {
   p.add("-precision", "fp32 (default), fp32_mfma, bf16 or fp8 centroid scoring and MaxSim", false);
   p.add("-rescore", "bf16/fp8: rescore this many docs in fp32 (default 200)", false);
   p.add("-batch", "queries per device pass (default 1024)", false);
+  p.add("-residual_on_host", "keep codes and residuals in host memory (default false)", false);
   if (!p.parse(argc, argv)) {
     std::cerr << "Error parsing command line" << std::endl;
     return 1;
@@ -164,6 +165,8 @@ int main(int argc, char **argv) {
   else if (prec != "fp32") { std::cerr << "Error parsing command line: bad -precision" << std::endl; return 1; }
   const uint32_t rescore = p.get("-rescore").empty() ? 200 : p.u32("-rescore");
   const uint32_t batch = p.get("-batch").empty() ? 1024 : std::max(1u, p.u32("-batch"));
+  const uint32_t ix_flags =
+      !p.get("-residual_on_host").empty() && p.boolean("-residual_on_host") ? MVR_INDEX_RESIDUAL_ON_HOST : 0u;
 
   const char *env_root = std::getenv("MVR_DATASET_ROOT");
   const std::string root = env_root ? std::string(env_root)
@@ -176,7 +179,7 @@ int main(int argc, char **argv) {
   const std::string qid_path = root + "/RawData/" + ds + "/document/queries.dev.tsv";
 
   mvr_index *ix = nullptr;
-  if (mvr_index_load(index_path.c_str(), 0, &ix) != MVR_OK) {
+  if (mvr_index_load_ex(index_path.c_str(), 0, ix_flags, &ix) != MVR_OK) {
     std::cerr << "error: " << mvr_last_error() << std::endl;
     return 1;
   }

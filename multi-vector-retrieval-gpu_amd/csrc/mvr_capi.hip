@@ -6,9 +6,10 @@
 // (src/impl/IGPGPUPP.hpp:67-332), the per-thread GPGPURetrievalPP pipeline
 // (src/include/alg/igp_gpu_pp/IGPGPURetrievalPP.hpp:73-176) and the device
 // structs of DataStructPP.hpp:16-205.  Differences by design: the whole index
-// (including residual codes) is device resident, queries are processed in
-// batches of up to `batch` per device pass, and errors are returned as status
-// codes instead of exit().
+// (including residual codes) is device resident unless the index is created
+// with MVR_INDEX_RESIDUAL_ON_HOST, queries are processed in batches of up to
+// `batch` per device pass, and errors are returned as status codes instead of
+// exit().
 #include "../../include/mvr.h"
 #include "mvr_internal.h"
 #include "npy_io.hpp"
@@ -103,8 +104,13 @@ struct PassGraph {
 struct Workspace {
   DBuf Q, meta, part_ids, part_s, top_ids, top_s, cand_cnt, cand_tot, cand_keys, sel_ids, sel_F, sel_n,
       sel_score, out_ids, out_s, counters, res_ids, res_s, res_n, bmax, feat, mlp_out, dyn_np, dyn_r,
-      rr_meta, queue, pass_nt, pass_np, pass_rm, xcd_part, xcd_queue;
+      rr_meta, queue, pass_nt, pass_np, pass_rm, xcd_part, xcd_queue, hs_bm, hs_bn, hs_bt, hs_wbase;
+  // host-resident residuals, per pass slot: compact lists and the staged docs
+  DBuf hs_docs[2], hs_lens[2], hs_offs[2], hs_cids[2], hs_cnt[2], hs_codes[2], hs_res[2];
   HBuf staging;
+  HBuf hs_cnt_h;  // [2][2] (docs, tokens) of each slot, read back
+  hipStream_t hs_stream = nullptr;              // the staging copies of a pipelined search
+  hipEvent_t hs_compacted[2] = {nullptr, nullptr}, hs_staged[2] = {nullptr, nullptr};
   hipEvent_t staging_done = nullptr;
   std::vector<PassGraph> graphs;  // most recently used last
   std::vector<uint64_t> graph_sig;  // buffer addresses the graphs were captured with
@@ -113,7 +119,9 @@ struct Workspace {
     return {&Q, &meta, &part_ids, &part_s, &top_ids, &top_s, &cand_cnt, &cand_tot, &cand_keys,
             &sel_ids, &sel_F, &sel_n, &sel_score, &out_ids, &out_s, &counters, &res_ids, &res_s,
             &res_n, &bmax, &feat, &mlp_out, &dyn_np, &dyn_r, &rr_meta, &queue, &pass_nt, &pass_np,
-            &pass_rm, &xcd_part, &xcd_queue};
+            &pass_rm, &xcd_part, &xcd_queue, &hs_bm, &hs_bn, &hs_bt, &hs_wbase, &hs_docs[0], &hs_docs[1],
+            &hs_lens[0], &hs_lens[1], &hs_offs[0], &hs_offs[1], &hs_cids[0], &hs_cids[1], &hs_cnt[0],
+            &hs_cnt[1], &hs_codes[0], &hs_codes[1], &hs_res[0], &hs_res[1]};
   }
   void drop_graphs() {
     for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
@@ -125,6 +133,14 @@ struct Workspace {
     cap_stream = nullptr;
     for (DBuf *b : all()) b->release();
     staging.release();
+    hs_cnt_h.release();
+    if (hs_stream) (void)hipStreamDestroy(hs_stream);
+    hs_stream = nullptr;
+    for (int i = 0; i < 2; ++i) {
+      if (hs_compacted[i]) (void)hipEventDestroy(hs_compacted[i]);
+      if (hs_staged[i]) (void)hipEventDestroy(hs_staged[i]);
+      hs_compacted[i] = hs_staged[i] = nullptr;
+    }
     if (staging_done) (void)hipEventDestroy(staging_done);
     staging_done = nullptr;
   }
@@ -144,6 +160,13 @@ struct mvr_index {
   bool tuned[4] = {false, false, false, false};
   XcdLayout xl;                // slice-grouped token copy (built on first use)
   bool xl_failed = false;
+  // MVR_INDEX_RESIDUAL_ON_HOST: residual bytes and a copy of the codes stay in
+  // host memory (residual pinned); each pass stages its selected docs
+  bool res_host = false;
+  uint8_t *h_res = nullptr;          // [n_tok][32], pinned and mapped
+  uint32_t *h_codes = nullptr;       // [n_tok], pinned and mapped
+  const uint8_t *h_res_dev = nullptr;  // their device-side addresses
+  const uint32_t *h_codes_dev = nullptr;
 };
 
 namespace {
@@ -205,8 +228,13 @@ int mvr_device_count(int *count) {
 }
 
 int mvr_index_create(const mvr_index_desc *desc, int device, mvr_index **out) {
+  return mvr_index_create_ex(desc, device, 0u, out);
+}
+
+int mvr_index_create_ex(const mvr_index_desc *desc, int device, uint32_t flags, mvr_index **out) {
   if (!desc || !out) return fail(MVR_ERR_INVALID, "NULL argument");
   *out = nullptr;
+  if (flags & ~MVR_INDEX_RESIDUAL_ON_HOST) return fail(MVR_ERR_INVALID, "unknown index flags 0x%x", flags);
   if (desc->dim != kDim) return fail(MVR_ERR_UNSUPPORTED, "dim %u unsupported (kernels need 128)", desc->dim);
   if (desc->n_bit != 2) return fail(MVR_ERR_UNSUPPORTED, "n_bit %u unsupported (kernels need 2)", desc->n_bit);
   if (desc->n_doc == 0 || desc->n_centroid == 0)
@@ -237,6 +265,8 @@ int mvr_index_create(const mvr_index_desc *desc, int device, mvr_index **out) {
 
   auto bail = [&](int code) {
     free_devindex(ix->d);
+    if (ix->h_res) (void)hipHostFree(ix->h_res);
+    if (ix->h_codes) (void)hipHostFree(ix->h_codes);
     delete ix;
     return code;
   };
@@ -284,8 +314,33 @@ int mvr_index_create(const mvr_index_desc *desc, int device, mvr_index **out) {
       d.c_norm_max = std::max(d.c_norm_max, (float)std::sqrt(nn));
     }
   }
-  UP(d.codes, desc->codes, d.n_tok);
-  UP(d.residual, desc->residual, d.n_tok * kResBytes);
+  if (flags & MVR_INDEX_RESIDUAL_ON_HOST) {
+    // pinned, mapped host copies that the per-pass gather kernel reads
+    ix->res_host = true;
+    const hipMemcpyKind kind = od ? hipMemcpyDeviceToHost : hipMemcpyHostToHost;
+    hipError_t e0 = hipHostMalloc(reinterpret_cast<void **>(&ix->h_res), std::max<size_t>(d.n_tok * kResBytes, 256),
+                                  hipHostMallocMapped);
+    if (e0 == hipSuccess)
+      e0 = hipHostMalloc(reinterpret_cast<void **>(&ix->h_codes), std::max<size_t>(d.n_tok * 4, 256),
+                         hipHostMallocMapped);
+    if (e0 == hipSuccess && d.n_tok) e0 = hipMemcpy(ix->h_res, desc->residual, d.n_tok * kResBytes, kind);
+    if (e0 == hipSuccess && d.n_tok) e0 = hipMemcpy(ix->h_codes, desc->codes, d.n_tok * 4, kind);
+    void *pr = nullptr, *pc = nullptr;
+    if (e0 == hipSuccess) e0 = hipHostGetDevicePointer(&pr, ix->h_res, 0);
+    if (e0 == hipSuccess) e0 = hipHostGetDevicePointer(&pc, ix->h_codes, 0);
+    if (e0 != hipSuccess)
+      return bail(fail(MVR_ERR_DEVICE, "host residual copy failed: %s", hipGetErrorString(e0)));
+    ix->h_res_dev = static_cast<const uint8_t *>(pr);
+    ix->h_codes_dev = static_cast<const uint32_t *>(pc);
+    if (!od) {
+      // codes in range (a device descriptor's codes are trusted like a device-resident index's)
+      for (uint64_t i = 0; i < d.n_tok; ++i)
+        if (ix->h_codes[i] >= d.n_c) return bail(fail(MVR_ERR_INVALID, "code %u out of range", ix->h_codes[i]));
+    }
+  } else {
+    UP(d.codes, desc->codes, d.n_tok);
+    UP(d.residual, desc->residual, d.n_tok * kResBytes);
+  }
   UP(d.doclens, desc->doclens, d.n_doc);
   UP(d.ivf_off, desc->ivf_offsets, d.n_c);
   UP(d.ivf_size, desc->ivf_sizes, d.n_c);
@@ -329,6 +384,10 @@ int mvr_index_create(const mvr_index_desc *desc, int device, mvr_index **out) {
 }
 
 int mvr_index_load(const char *index_dir, int device, mvr_index **out) {
+  return mvr_index_load_ex(index_dir, device, 0u, out);
+}
+
+int mvr_index_load_ex(const char *index_dir, int device, uint32_t flags, mvr_index **out) {
   if (!index_dir || !out) return fail(MVR_ERR_INVALID, "NULL argument");
   const std::string dir(index_dir);
   NpyArray cent, codes, weight, res, lens;
@@ -392,7 +451,7 @@ int mvr_index_load(const char *index_dir, int device, mvr_index **out) {
   desc.ivf_ids = ids.data();
   desc.ivf_n_ele = n_ele;
   desc.on_device = 0;
-  return mvr_index_create(&desc, device, out);
+  return mvr_index_create_ex(&desc, device, flags, out);
 }
 
 void mvr_index_free(mvr_index *ix) {
@@ -403,6 +462,8 @@ void mvr_index_free(mvr_index *ix) {
   for (void *p : {(void *)ix->xl.codes_s, (void *)ix->xl.residual_s, (void *)ix->xl.sl_off})
     if (p) (void)hipFree(p);
   free_devindex(ix->d);
+  if (ix->h_res) (void)hipHostFree(ix->h_res);
+  if (ix->h_codes) (void)hipHostFree(ix->h_codes);
   delete ix;
 }
 
@@ -430,6 +491,85 @@ bool ensure_xcd_layout(mvr_index *ix, hipStream_t s) {
   }
   ix->xl = xl;
   return true;
+}
+
+// The compact index of one pass over a host-resident-residual index.
+struct HostPass {
+  DevIndex d;                     // the index with codes/residual/doclens/docoff of the pass's docs
+  const uint32_t *ids = nullptr;  // [nb][R] compact ids of the selected docs
+  const uint32_t *docs = nullptr; // [n] compact id -> doc id (ascending)
+  uint32_t n = 0;
+};
+
+// Host-resident residuals (the reference's RerankPP::rerank / transmitSQCode,
+// src/include/alg/igp_gpu_pp/RerankPP.hpp:289-361, gathers the candidates'
+// residual codes on the host for every query and copies them over PCIe).  Here
+// the device marks the docs selected by the pass's queries in a bitmap and
+// ranks them (compact ids follow doc ids, so a ranking by (score, id) is the
+// same on compact and original ids); the host reads back only the pass's doc
+// and token counts to size the buffers; then a kernel reads those docs' codes
+// and residual rows straight from the pinned, mapped host arrays, each doc
+// once however many queries selected it.
+//
+// hs_compact queues the compaction of a pass into `slot` and the read-back of
+// its counts on st; hs_gather (once those counts are on the host) queues the
+// copy of the docs on gs.
+int hs_compact(mvr_index *ix, Workspace &ws, int slot, const uint32_t *sel_ids, const uint32_t *sel_n,
+               uint32_t R, uint32_t nb, hipStream_t st) {
+  const DevIndex &d = ix->d;
+  const uint32_t n_words = (d.n_doc + 31) / 32, nblk = (n_words + 255) / 256;
+  const size_t umax = std::min<size_t>(d.n_doc, (size_t)nb * R);
+  HIP_TRY(ws.hs_bm.ensure((size_t)n_words * 4));
+  HIP_TRY(ws.hs_bn.ensure((size_t)nblk * 4));
+  HIP_TRY(ws.hs_bt.ensure((size_t)nblk * 8));
+  HIP_TRY(ws.hs_wbase.ensure((size_t)n_words * 4));
+  HIP_TRY(ws.hs_docs[slot].ensure(umax * 4));
+  HIP_TRY(ws.hs_lens[slot].ensure(umax * 4));
+  HIP_TRY(ws.hs_offs[slot].ensure(umax * 8));
+  HIP_TRY(ws.hs_cids[slot].ensure((size_t)nb * R * 4));
+  HIP_TRY(ws.hs_cnt[slot].ensure(16));
+  HostStage hs;
+  hs.bm = ws.hs_bm.as<uint32_t>();
+  hs.b_n = ws.hs_bn.as<uint32_t>();
+  hs.b_t = ws.hs_bt.as<unsigned long long>();
+  hs.wbase = ws.hs_wbase.as<uint32_t>();
+  hs.docs = ws.hs_docs[slot].as<uint32_t>();
+  hs.lens = ws.hs_lens[slot].as<uint32_t>();
+  hs.offs = ws.hs_offs[slot].as<uint64_t>();
+  hs.cids = ws.hs_cids[slot].as<uint32_t>();
+  hs.cnt = ws.hs_cnt[slot].as<unsigned long long>();
+  HIP_TRY(launch_hs_compact(sel_ids, sel_n, R, nb, d.doclens, d.n_doc, hs, st));
+  HIP_TRY(hipMemcpyAsync(static_cast<char *>(ws.hs_cnt_h.p) + slot * 16, hs.cnt, 16, hipMemcpyDeviceToHost, st));
+  return MVR_OK;
+}
+
+int hs_gather(mvr_index *ix, Workspace &ws, int slot, uint32_t nb, uint32_t R, hipStream_t gs, HostPass &hp) {
+  const DevIndex &d = ix->d;
+  const unsigned long long *cnt =
+      reinterpret_cast<const unsigned long long *>(static_cast<const char *>(ws.hs_cnt_h.p) + slot * 16);
+  const uint32_t U = (uint32_t)cnt[0];
+  const uint64_t T = cnt[1];
+  if (U > std::min<size_t>(d.n_doc, (size_t)nb * R) || T > d.n_tok)
+    return fail(MVR_ERR_DEVICE, "pass compaction counted %u docs / %llu tokens", U, (unsigned long long)T);
+  HIP_TRY(ws.hs_codes[slot].ensure(T * 4));
+  HIP_TRY(ws.hs_res[slot].ensure(T * kResBytes));
+  HostStage hs;
+  hs.docs = ws.hs_docs[slot].as<uint32_t>();
+  hs.lens = ws.hs_lens[slot].as<uint32_t>();
+  hs.offs = ws.hs_offs[slot].as<uint64_t>();
+  HIP_TRY(launch_hs_gather(hs, U, d.docoff, ix->h_codes_dev, ix->h_res_dev, ws.hs_codes[slot].as<uint32_t>(),
+                           ws.hs_res[slot].as<uint8_t>(), gs));
+  hp.d = d;
+  hp.d.n_doc = U;
+  hp.d.n_tok = T;
+  hp.d.doclens = hs.lens;
+  hp.d.docoff = hs.offs;
+  hp.d.codes = ws.hs_codes[slot].as<uint32_t>();
+  hp.d.residual = ws.hs_res[slot].as<uint8_t>();
+  hp.ids = ws.hs_cids[slot].as<uint32_t>();
+  hp.docs = hs.docs;
+  hp.n = U;
+  return MVR_OK;
 }
 
 }  // namespace
@@ -547,12 +687,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   HIP_TRY(ws.top_ids.ensure(Bc * t_max * P * 4));
   HIP_TRY(ws.top_s.ensure(Bc * t_max * P * 4));
   HIP_TRY(ws.cand_cnt.ensure(Bc * d.n_seg * 4));
-  HIP_TRY(ws.cand_tot.ensure(Bc * 4));
   HIP_TRY(ws.cand_keys.ensure(Bc * pool_per_q));
-  HIP_TRY(ws.sel_ids.ensure(Bc * R * 4));
-  HIP_TRY(ws.sel_F.ensure(Bc * R * 4));
-  HIP_TRY(ws.sel_n.ensure(Bc * 4));
-  HIP_TRY(ws.sel_score.ensure(Bc * R * 4));
   if (!fp32_mode || fast_ct) HIP_TRY(ws.bmax.ensure(Bc * t_max * ((d.n_c + 31) / 32) * 4));
   if (learned) {
     HIP_TRY(ws.feat.ensure(Bc * 1024 * 4));
@@ -569,16 +704,47 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   const bool persistent = prm->persistent && prm->precision != MVR_FP32 &&
                           k <= kPersistTopkMax && R2 <= kPersistTopkMax;
   // graph replay: not with learned probing (the model's launches) or stage outputs
-  const bool use_graph = prm->graph && !learned && !stages;
+  // nor with host-resident residuals (each pass synchronises to stage its docs)
+  const bool use_graph = prm->graph && !learned && !stages && !ix->res_host;
   // XCD-sliced MaxSim for the MFMA precisions: opt in with MVR_MAXSIM_SLICED=1
   // (it cuts the fabric traffic of the centroid gathers by more than half but
   // is not faster than the per-doc kernels on MI355X, see the design notes)
   const char *sliced_var = std::getenv("MVR_MAXSIM_SLICED");
   const bool sliced_env = sliced_var && sliced_var[0] == '1';
-  const bool sliced = !persistent && prm->precision != MVR_FP32 && sliced_env && ensure_xcd_layout(ix, s);
+  const bool sliced =
+      !persistent && !ix->res_host && prm->precision != MVR_FP32 && sliced_env && ensure_xcd_layout(ix, s);
   const uint32_t xcd_wgs = std::max(1u, d.n_cu / 8) * 2;
+  const uint32_t n_pass = (n_q + B - 1) / B;
+  // host-resident residuals: two passes in flight, the staging copy of one (on
+  // a second stream) overlapping the device work of the other; the selection
+  // outputs and the query buffer then have one slot per pass in flight
+  const bool pipelined = ix->res_host && !stages && n_pass > 1 && !(persistent && !ix->tuned[prm->precision]);
+  const size_t n_slot = pipelined ? 2 : 1;
+  HIP_TRY(ws.cand_tot.ensure(n_slot * Bc * 4));
+  HIP_TRY(ws.sel_ids.ensure(n_slot * Bc * R * 4));
+  HIP_TRY(ws.sel_F.ensure(n_slot * Bc * R * 4));
+  HIP_TRY(ws.sel_n.ensure(n_slot * Bc * 4));
+  HIP_TRY(ws.sel_score.ensure(n_slot * Bc * R * 4));
+  if (ix->res_host) HIP_TRY(ws.hs_cnt_h.ensure(32));
+  if (pipelined) {
+    if (!ws.hs_stream) HIP_TRY(hipStreamCreateWithFlags(&ws.hs_stream, hipStreamNonBlocking));
+    for (int i = 0; i < 2; ++i) {
+      if (!ws.hs_compacted[i]) HIP_TRY(hipEventCreateWithFlags(&ws.hs_compacted[i], hipEventDisableTiming));
+      if (!ws.hs_staged[i]) HIP_TRY(hipEventCreateWithFlags(&ws.hs_staged[i], hipEventDisableTiming));
+    }
+  }
+  // per-slot views of the selection outputs
+  struct Sel {
+    uint32_t *ids, *n, *tot;
+    float *F, *score;
+  };
+  auto sel = [&](size_t slot) {
+    return Sel{ws.sel_ids.as<uint32_t>() + slot * Bc * R, ws.sel_n.as<uint32_t>() + slot * Bc,
+               ws.cand_tot.as<uint32_t>() + slot * Bc, ws.sel_F.as<float>() + slot * Bc * R,
+               ws.sel_score.as<float>() + slot * Bc * R};
+  };
   if (!on_device || use_graph) {
-    HIP_TRY(ws.Q.ensure(Bc * t_max * kDim * 4));
+    HIP_TRY(ws.Q.ensure(n_slot * Bc * t_max * kDim * 4));
     HIP_TRY(ws.out_ids.ensure(Bc * k * 4));
     HIP_TRY(ws.out_s.ensure(Bc * k * 4));
   }
@@ -593,10 +759,11 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     HIP_TRY(ws.pass_rm.ensure(Bc * t_max * 4));
   }
 
-  // Stage timings: seven events per device pass, read once at the end of the
-  // call so that timing adds no host synchronisation between passes.
-  const uint32_t n_pass = (n_q + B - 1) / B;
-  constexpr int kNev = 7;
+  // Stage timings: events per device pass, read once at the end of the call
+  // so that timing adds no host synchronisation between passes.  0..6: stage
+  // boundaries; host-resident residuals: 7 the staged docs on the device, 8
+  // the MaxSim stage starting.
+  constexpr int kNev = 9;
   std::vector<hipEvent_t> ev;
   if (timings) {
     ev.resize((size_t)n_pass * kNev);
@@ -605,16 +772,18 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     HIP_TRY(ws.counters.ensure(24));
     HIP_TRY(launch_fill_u32(ws.counters.as<uint32_t>(), 0u, 6, s));
   }
-  auto rec = [&](uint32_t pass, int i) -> hipError_t {
-    return timings ? hipEventRecord(ev[(size_t)pass * kNev + i], s) : hipSuccess;
+  auto rec = [&](uint32_t pass, int i, hipStream_t on = nullptr) -> hipError_t {
+    return timings ? hipEventRecord(ev[(size_t)pass * kNev + i], on ? on : s) : hipSuccess;
   };
 
-  // One device pass over nb queries whose inputs are in place.  `timed`
-  // records the stage events (not while a graph is being captured).
-  auto run_pass = [&](hipStream_t st, const float *Qb, const uint32_t *rm, uint32_t n_rows,
-                      const uint32_t *bnt, const uint32_t *bnp, uint32_t nb, uint32_t *oid, float *os,
-                      int bi, bool timed) -> int {
+  // A device pass over nb queries whose inputs are in place, in two halves.
+  // `timed` records the stage events (not while a graph is being captured).
+  // Front: centroid scoring, probing, filter and selection into `slot`.
+  auto run_front = [&](hipStream_t st, const float *Qb, const uint32_t *rm, uint32_t n_rows,
+                       const uint32_t *bnt, const uint32_t *bnp, uint32_t nb, int bi, bool timed,
+                       size_t slot) -> int {
     auto mark = [&](int i) -> hipError_t { return timed ? rec(bi, i) : hipSuccess; };
+    const Sel sl = sel(slot);
     HIP_TRY(mark(0));
     if (fp32_mode && !fast_ct)
       HIP_TRY(launch_centroid_topk(Qb, rm, n_rows, nb * t_max, d, P, MVR_FP32, n_split,
@@ -640,9 +809,20 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
                           d, ws.cand_cnt.as<uint32_t>(), ws.cand_keys.as<uint64_t>(), st));
     HIP_TRY(mark(2));
     HIP_TRY(launch_select(ws.cand_keys.as<uint64_t>(), ws.cand_cnt.as<uint32_t>(), d.n_seg, q_r, nb, R,
-                          ws.sel_ids.as<uint32_t>(), ws.sel_F.as<float>(), ws.sel_n.as<uint32_t>(),
-                          ws.cand_tot.as<uint32_t>(), st));
+                          sl.ids, sl.F, sl.n, sl.tot, st));
     HIP_TRY(mark(3));
+    return MVR_OK;
+  };
+
+  // Back: MaxSim of the selected docs (with host-resident residuals, of the
+  // pass's staged docs under their compact ids, mapped back at the end),
+  // the FP32 rescore and the final top-k.
+  auto run_back = [&](hipStream_t st, const float *Qb, const uint32_t *bnt, uint32_t nb, uint32_t *oid, float *os,
+                      int bi, bool timed, size_t slot, const HostPass *hp) -> int {
+    auto mark = [&](int i) -> hipError_t { return timed ? rec(bi, i) : hipSuccess; };
+    const Sel sl = sel(slot);
+    const uint32_t *sid = hp ? hp->ids : sl.ids;
+    const DevIndex &dx = hp ? hp->d : d;
     const int clip = (int)prm->clip_zero;
     if (persistent) {
       const PersistCfg &c1 = ix->tune[prm->precision];
@@ -650,56 +830,85 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       if (R2) {
         // phase 1: low-precision MaxSim of all candidates, fused top-R2;
         // phase 2: FP32 MaxSim of those, fused top-k
-        HIP_TRY(launch_persist(Qb, t_max, bnt, ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R, nb, d,
-                               (int)prm->precision, clip, c1, qu, ws.sel_score.as<float>(), R2,
+        HIP_TRY(launch_persist(Qb, t_max, bnt, sid, sl.n, R, nb, dx,
+                               (int)prm->precision, clip, c1, qu, sl.score, R2,
                                ws.res_ids.as<uint32_t>(), ws.res_s.as<float>(), ws.res_n.as<uint32_t>(), R2,
                                st));
         HIP_TRY(mark(4));
         const PersistCfg &c2 = ix->tuned[MVR_FP32_MFMA] ? ix->tune[MVR_FP32_MFMA] : c1;
-        HIP_TRY(launch_persist(Qb, t_max, bnt, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2, nb, d,
+        HIP_TRY(launch_persist(Qb, t_max, bnt, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2, nb, dx,
                                MVR_FP32_MFMA, clip, c2, qu, ws.res_s.as<float>(), k, oid, os, nullptr,
                                k, st));
         HIP_TRY(mark(5));
       } else {
-        HIP_TRY(launch_persist(Qb, t_max, bnt, ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R, nb, d,
-                               (int)prm->precision, clip, c1, qu, ws.sel_score.as<float>(), k, oid,
+        HIP_TRY(launch_persist(Qb, t_max, bnt, sid, sl.n, R, nb, dx,
+                               (int)prm->precision, clip, c1, qu, sl.score, k, oid,
                                os, nullptr, k, st));
         HIP_TRY(mark(4));
         HIP_TRY(mark(5));
       }
+      if (hp) HIP_TRY(launch_map_ids(oid, nb * k, hp->docs, hp->n, st));
       HIP_TRY(mark(6));
       return MVR_OK;
     }
     if (sliced)
-      HIP_TRY(launch_rerank_xcd(Qb, t_max, bnt, ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R, nb, d,
+      HIP_TRY(launch_rerank_xcd(Qb, t_max, bnt, sid, sl.n, R, nb, dx,
                                 ix->xl, (int)prm->precision, clip, xcd_wgs, ws.xcd_queue.as<uint32_t>(),
-                                ws.xcd_part.as<float>(), ws.sel_score.as<float>(), st));
+                                ws.xcd_part.as<float>(), sl.score, st));
     else
-      HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R,
-                            d, prm->precision, clip, ws.sel_score.as<float>(), st));
+      HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, sid, sl.n, R,
+                            dx, prm->precision, clip, sl.score, st));
     HIP_TRY(mark(4));
     if (R2) {
       // best R2 by the low-precision score, FP32 MaxSim of those, top-k of the FP32 scores
-      HIP_TRY(launch_topk(ws.sel_ids.as<uint32_t>(), ws.sel_score.as<float>(), ws.sel_n.as<uint32_t>(),
+      HIP_TRY(launch_topk(sid, sl.score, sl.n,
                           R, nb, R2, ws.res_ids.as<uint32_t>(), ws.res_s.as<float>(),
                           ws.res_n.as<uint32_t>(), st));
       if (sliced)
-        HIP_TRY(launch_rerank_xcd(Qb, t_max, bnt, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2, nb, d,
+        HIP_TRY(launch_rerank_xcd(Qb, t_max, bnt, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2, nb, dx,
                                   ix->xl, MVR_FP32_MFMA, clip, xcd_wgs, ws.xcd_queue.as<uint32_t>(),
                                   ws.xcd_part.as<float>(), ws.res_s.as<float>(), st));
       else
         HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2,
-                              d, MVR_FP32_MFMA, clip, ws.res_s.as<float>(), st));
+                              dx, MVR_FP32_MFMA, clip, ws.res_s.as<float>(), st));
     }
     HIP_TRY(mark(5));
     if (R2)
       HIP_TRY(launch_topk(ws.res_ids.as<uint32_t>(), ws.res_s.as<float>(), ws.res_n.as<uint32_t>(), R2,
                           nb, k, oid, os, nullptr, st));
     else
-      HIP_TRY(launch_topk(ws.sel_ids.as<uint32_t>(), ws.sel_score.as<float>(), ws.sel_n.as<uint32_t>(),
+      HIP_TRY(launch_topk(sid, sl.score, sl.n,
                           R, nb, k, oid, os, nullptr, st));
+    if (hp) HIP_TRY(launch_map_ids(oid, nb * k, hp->docs, hp->n, st));
     HIP_TRY(mark(6));
     return MVR_OK;
+  };
+
+  auto note_staged = [&](const HostPass &hp) {
+    if (timings) {
+      timings->staged_docs += hp.n;
+      timings->staged_tokens += hp.d.n_tok;
+    }
+  };
+
+  // One whole pass on st (slot 0).
+  auto run_pass = [&](hipStream_t st, const float *Qb, const uint32_t *rm, uint32_t n_rows,
+                      const uint32_t *bnt, const uint32_t *bnp, uint32_t nb, uint32_t *oid, float *os,
+                      int bi, bool timed) -> int {
+    int rc = run_front(st, Qb, rm, n_rows, bnt, bnp, nb, bi, timed, 0);
+    if (rc != MVR_OK) return rc;
+    if (!ix->res_host) return run_back(st, Qb, bnt, nb, oid, os, bi, timed, 0, nullptr);
+    const Sel sl = sel(0);
+    HostPass hp;
+    if ((rc = hs_compact(ix, ws, 0, sl.ids, sl.n, R, nb, st)) != MVR_OK) return rc;
+    HIP_TRY(hipStreamSynchronize(st));
+    if ((rc = hs_gather(ix, ws, 0, nb, R, st, hp)) != MVR_OK) return rc;
+    if (timed) {
+      HIP_TRY(rec(bi, 7));
+      HIP_TRY(rec(bi, 8));
+      note_staged(hp);
+    }
+    return run_back(st, Qb, bnt, nb, oid, os, bi, timed, 0, &hp);
   };
 
   // Graphs are only valid for the buffers they were captured with.
@@ -714,144 +923,219 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   }
   std::vector<char> pass_graphed(n_pass, 0);
 
-  int bi = 0;
-  for (uint32_t b0 = 0; b0 < n_q; b0 += B, ++bi) {
-    const uint32_t nb = std::min(B, n_q - b0);
-    const float *Qb;
-    uint32_t *oid;
-    float *os;
-    if (on_device && !use_graph) {
-      Qb = Q + (size_t)b0 * t_max * kDim;
-      oid = out_ids + (size_t)b0 * k;
-      os = out_scores + (size_t)b0 * k;
-    } else {
-      HIP_TRY(hipMemcpyAsync(ws.Q.p, Q + (size_t)b0 * t_max * kDim, (size_t)nb * t_max * kDim * 4,
-                             on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
-      Qb = ws.Q.as<float>();
-      oid = ws.out_ids.as<uint32_t>();
-      os = ws.out_s.as<float>();
-    }
-    const uint32_t *rm = uniform ? nullptr : d_rmap + rows_begin[bi];
-    uint32_t n_rows = 0;
-    if (uniform) n_rows = nb * t_max;
-    else for (uint32_t i = 0; i < nb; ++i) n_rows += nt[b0 + i];
-    const uint32_t *bnt = d_nt + b0;
-    const uint32_t *bnp = d_np + b0;
-
-    if (persistent && !ix->tuned[prm->precision]) {
-      // autotune on this pass: every setting once to warm up, then timed;
-      // results are identical for every setting, so the pass stays valid
-      hipEvent_t e0, e1;
-      HIP_TRY(hipEventCreate(&e0));
-      HIP_TRY(hipEventCreate(&e1));
-      float best = 1e30f;
-      PersistCfg best_c;
-      for (int rep = 0; rep < 2; ++rep)
-        for (uint32_t tps = 1; tps <= 2; ++tps)
-          for (uint32_t wpc = 1; wpc <= 2; ++wpc) {
-              PersistCfg c;
-              c.n_wg = wpc * d.n_cu;
-              c.tps = tps;
-              ix->tune[prm->precision] = c;
-              HIP_TRY(hipEventRecord(e0, s));
-              const int rc = run_pass(s, Qb, rm, n_rows, bnt, bnp, nb, oid, os, bi, false);
-              if (rc != MVR_OK) return rc;
-              HIP_TRY(hipEventRecord(e1, s));
-              HIP_TRY(hipEventSynchronize(e1));
-              float ms = 0.0f;
-              HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-              if (rep == 1 && ms < best) { best = ms; best_c = c; }
-            }
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
-      ix->tune[prm->precision] = best_c;
-      ix->tuned[prm->precision] = true;
-    }
-
-    if (use_graph) {
-      // inputs of the pass into fixed buffers, then replay (or capture) the pass
-      HIP_TRY(hipMemcpyAsync(ws.pass_nt.p, bnt, (size_t)nb * 4, hipMemcpyDeviceToDevice, s));
-      HIP_TRY(hipMemcpyAsync(ws.pass_np.p, bnp, (size_t)nb * 4, hipMemcpyDeviceToDevice, s));
-      if (rm) HIP_TRY(hipMemcpyAsync(ws.pass_rm.p, rm, (size_t)n_rows * 4, hipMemcpyDeviceToDevice, s));
-      const PersistCfg &tc = ix->tune[prm->precision];
-      const std::vector<uint64_t> key = {nb, n_rows, t_max, P, R, R2, k, prm->precision, prm->clip_zero,
-                                         uniform ? 1u : 0u, persistent ? 1u : 0u, fast_ct ? 1u : 0u,
-                                         tc.n_wg, tc.tps, sliced ? 1u : 0u};
-      hipGraphExec_t exec = nullptr;
-      for (size_t gi = 0; gi < ws.graphs.size(); ++gi)
-        if (ws.graphs[gi].key == key) {
-          exec = ws.graphs[gi].exec;
-          std::rotate(ws.graphs.begin() + gi, ws.graphs.begin() + gi + 1, ws.graphs.end());
-          break;
-        }
-      if (!exec) {
-        hipGraph_t graph = nullptr;
-        HIP_TRY(hipStreamBeginCapture(ws.cap_stream, hipStreamCaptureModeRelaxed));
-        const int rc = run_pass(ws.cap_stream, ws.Q.as<float>(), rm ? ws.pass_rm.as<uint32_t>() : nullptr,
-                                n_rows, ws.pass_nt.as<uint32_t>(), ws.pass_np.as<uint32_t>(), nb,
-                                ws.out_ids.as<uint32_t>(), ws.out_s.as<float>(), bi, false);
-        const hipError_t ce = hipStreamEndCapture(ws.cap_stream, &graph);
-        if (rc != MVR_OK) {
-          if (graph) (void)hipGraphDestroy(graph);
-          return rc;
-        }
-        HIP_TRY(ce);
-        const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        HIP_TRY(ie);
-        if (ws.graphs.size() >= 64) {
-          (void)hipGraphExecDestroy(ws.graphs.front().exec);
-          ws.graphs.erase(ws.graphs.begin());
-        }
-        ws.graphs.push_back(PassGraph{key, exec});
-      }
-      HIP_TRY(rec(bi, 0));
-      HIP_TRY(hipGraphLaunch(exec, s));
-      for (int i = 1; i < kNev; ++i) HIP_TRY(rec(bi, i));
-      pass_graphed[bi] = 1;
+  // Pipelined passes (host-resident residuals): the front of pass b+1 and
+  // its compaction are queued on s before the back of pass b, and pass b+1's
+  // staging copy runs on hs_stream while s works on pass b's MaxSim.
+  struct PassIn {
+    uint32_t b0 = 0, nb = 0, n_rows = 0;
+    const float *Qb = nullptr;
+    const uint32_t *rm = nullptr, *bnt = nullptr, *bnp = nullptr;
+  };
+  if (pipelined) {
+    hipStream_t gs = ws.hs_stream;
+    HostPass hps[2];
+    PassIn pin[2];
+    auto front_and_compact = [&](uint32_t bi) -> int {
+      const size_t slot = bi & 1u;
+      PassIn &pi = pin[slot];
+      pi.b0 = bi * B;
+      pi.nb = std::min(B, n_q - pi.b0);
       if (on_device) {
-        HIP_TRY(hipMemcpyAsync(out_ids + (size_t)b0 * k, oid, (size_t)nb * k * 4, hipMemcpyDeviceToDevice, s));
-        HIP_TRY(hipMemcpyAsync(out_scores + (size_t)b0 * k, os, (size_t)nb * k * 4, hipMemcpyDeviceToDevice, s));
+        pi.Qb = Q + (size_t)pi.b0 * t_max * kDim;
+      } else {
+        float *qs = ws.Q.as<float>() + slot * Bc * t_max * kDim;
+        HIP_TRY(hipMemcpyAsync(qs, Q + (size_t)pi.b0 * t_max * kDim, (size_t)pi.nb * t_max * kDim * 4,
+                               hipMemcpyHostToDevice, s));
+        pi.Qb = qs;
       }
-    } else {
-      const int rc = run_pass(s, Qb, rm, n_rows, bnt, bnp, nb, oid, os, bi, true);
+      pi.rm = uniform ? nullptr : d_rmap + rows_begin[bi];
+      pi.n_rows = 0;
+      if (uniform) pi.n_rows = pi.nb * t_max;
+      else for (uint32_t i = 0; i < pi.nb; ++i) pi.n_rows += nt[pi.b0 + i];
+      pi.bnt = d_nt + pi.b0;
+      pi.bnp = d_np + pi.b0;
+      int rc = run_front(s, pi.Qb, pi.rm, pi.n_rows, pi.bnt, pi.bnp, pi.nb, (int)bi, true, slot);
       if (rc != MVR_OK) return rc;
+      const Sel sl = sel(slot);
+      if ((rc = hs_compact(ix, ws, (int)slot, sl.ids, sl.n, R, pi.nb, s)) != MVR_OK) return rc;
+      HIP_TRY(hipEventRecord(ws.hs_compacted[slot], s));
+      return MVR_OK;
+    };
+    auto stage = [&](uint32_t bi) -> int {
+      const size_t slot = bi & 1u;
+      HIP_TRY(hipEventSynchronize(ws.hs_compacted[slot]));
+      const int rc = hs_gather(ix, ws, (int)slot, pin[slot].nb, R, gs, hps[slot]);
+      if (rc != MVR_OK) return rc;
+      HIP_TRY(rec(bi, 7, gs));
+      HIP_TRY(hipEventRecord(ws.hs_staged[slot], gs));
+      note_staged(hps[slot]);
+      return MVR_OK;
+    };
+    int rc = front_and_compact(0);
+    if (rc == MVR_OK) rc = stage(0);
+    for (uint32_t bi = 0; rc == MVR_OK && bi < n_pass; ++bi) {
+      const size_t slot = bi & 1u;
+      if (bi + 1 < n_pass && (rc = front_and_compact(bi + 1)) != MVR_OK) break;
+      HIP_TRY(hipStreamWaitEvent(s, ws.hs_staged[slot], 0));
+      HIP_TRY(rec(bi, 8));
+      const PassIn &pi = pin[slot];
+      uint32_t *oid = on_device ? out_ids + (size_t)pi.b0 * k : ws.out_ids.as<uint32_t>();
+      float *os = on_device ? out_scores + (size_t)pi.b0 * k : ws.out_s.as<float>();
+      if ((rc = run_back(s, pi.Qb, pi.bnt, pi.nb, oid, os, (int)bi, true, slot, &hps[slot])) != MVR_OK) break;
+      if (!on_device) {
+        HIP_TRY(hipMemcpyAsync(out_ids + (size_t)pi.b0 * k, oid, (size_t)pi.nb * k * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(out_scores + (size_t)pi.b0 * k, os, (size_t)pi.nb * k * 4, hipMemcpyDeviceToHost, s));
+      }
+      if (timings) {
+        const Sel sl = sel(slot);
+        HIP_TRY(launch_count_work(sl.ids, sl.n, R, pi.nb, d, sl.tot, pi.bnt, ws.counters.as<unsigned long long>(), s));
+      }
+      if (bi + 1 < n_pass) rc = stage(bi + 1);
     }
-    if (!on_device) {
-      HIP_TRY(hipMemcpyAsync(out_ids + (size_t)b0 * k, oid, (size_t)nb * k * 4, hipMemcpyDeviceToHost, s));
-      HIP_TRY(hipMemcpyAsync(out_scores + (size_t)b0 * k, os, (size_t)nb * k * 4, hipMemcpyDeviceToHost, s));
+    if (rc != MVR_OK) return rc;
+  }
+
+  if (!pipelined) {
+    int bi = 0;
+    for (uint32_t b0 = 0; b0 < n_q; b0 += B, ++bi) {
+      const uint32_t nb = std::min(B, n_q - b0);
+      const float *Qb;
+      uint32_t *oid;
+      float *os;
+      if (on_device && !use_graph) {
+        Qb = Q + (size_t)b0 * t_max * kDim;
+        oid = out_ids + (size_t)b0 * k;
+        os = out_scores + (size_t)b0 * k;
+      } else {
+        HIP_TRY(hipMemcpyAsync(ws.Q.p, Q + (size_t)b0 * t_max * kDim, (size_t)nb * t_max * kDim * 4,
+                               on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+        Qb = ws.Q.as<float>();
+        oid = ws.out_ids.as<uint32_t>();
+        os = ws.out_s.as<float>();
+      }
+      const uint32_t *rm = uniform ? nullptr : d_rmap + rows_begin[bi];
+      uint32_t n_rows = 0;
+      if (uniform) n_rows = nb * t_max;
+      else for (uint32_t i = 0; i < nb; ++i) n_rows += nt[b0 + i];
+      const uint32_t *bnt = d_nt + b0;
+      const uint32_t *bnp = d_np + b0;
+
+      if (persistent && !ix->tuned[prm->precision]) {
+        // autotune on this pass: every setting once to warm up, then timed;
+        // results are identical for every setting, so the pass stays valid
+        hipEvent_t e0, e1;
+        HIP_TRY(hipEventCreate(&e0));
+        HIP_TRY(hipEventCreate(&e1));
+        float best = 1e30f;
+        PersistCfg best_c;
+        for (int rep = 0; rep < 2; ++rep)
+          for (uint32_t tps = 1; tps <= 2; ++tps)
+            for (uint32_t wpc = 1; wpc <= 2; ++wpc) {
+                PersistCfg c;
+                c.n_wg = wpc * d.n_cu;
+                c.tps = tps;
+                ix->tune[prm->precision] = c;
+                HIP_TRY(hipEventRecord(e0, s));
+                const int rc = run_pass(s, Qb, rm, n_rows, bnt, bnp, nb, oid, os, bi, false);
+                if (rc != MVR_OK) return rc;
+                HIP_TRY(hipEventRecord(e1, s));
+                HIP_TRY(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+                if (rep == 1 && ms < best) { best = ms; best_c = c; }
+              }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        ix->tune[prm->precision] = best_c;
+        ix->tuned[prm->precision] = true;
+      }
+
+      if (use_graph) {
+        // inputs of the pass into fixed buffers, then replay (or capture) the pass
+        HIP_TRY(hipMemcpyAsync(ws.pass_nt.p, bnt, (size_t)nb * 4, hipMemcpyDeviceToDevice, s));
+        HIP_TRY(hipMemcpyAsync(ws.pass_np.p, bnp, (size_t)nb * 4, hipMemcpyDeviceToDevice, s));
+        if (rm) HIP_TRY(hipMemcpyAsync(ws.pass_rm.p, rm, (size_t)n_rows * 4, hipMemcpyDeviceToDevice, s));
+        const PersistCfg &tc = ix->tune[prm->precision];
+        const std::vector<uint64_t> key = {nb, n_rows, t_max, P, R, R2, k, prm->precision, prm->clip_zero,
+                                           uniform ? 1u : 0u, persistent ? 1u : 0u, fast_ct ? 1u : 0u,
+                                           tc.n_wg, tc.tps, sliced ? 1u : 0u};
+        hipGraphExec_t exec = nullptr;
+        for (size_t gi = 0; gi < ws.graphs.size(); ++gi)
+          if (ws.graphs[gi].key == key) {
+            exec = ws.graphs[gi].exec;
+            std::rotate(ws.graphs.begin() + gi, ws.graphs.begin() + gi + 1, ws.graphs.end());
+            break;
+          }
+        if (!exec) {
+          hipGraph_t graph = nullptr;
+          HIP_TRY(hipStreamBeginCapture(ws.cap_stream, hipStreamCaptureModeRelaxed));
+          const int rc = run_pass(ws.cap_stream, ws.Q.as<float>(), rm ? ws.pass_rm.as<uint32_t>() : nullptr,
+                                  n_rows, ws.pass_nt.as<uint32_t>(), ws.pass_np.as<uint32_t>(), nb,
+                                  ws.out_ids.as<uint32_t>(), ws.out_s.as<float>(), bi, false);
+          const hipError_t ce = hipStreamEndCapture(ws.cap_stream, &graph);
+          if (rc != MVR_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return rc;
+          }
+          HIP_TRY(ce);
+          const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+          (void)hipGraphDestroy(graph);
+          HIP_TRY(ie);
+          if (ws.graphs.size() >= 64) {
+            (void)hipGraphExecDestroy(ws.graphs.front().exec);
+            ws.graphs.erase(ws.graphs.begin());
+          }
+          ws.graphs.push_back(PassGraph{key, exec});
+        }
+        HIP_TRY(rec(bi, 0));
+        HIP_TRY(hipGraphLaunch(exec, s));
+        for (int i = 1; i < kNev; ++i) HIP_TRY(rec(bi, i));
+        pass_graphed[bi] = 1;
+        if (on_device) {
+          HIP_TRY(hipMemcpyAsync(out_ids + (size_t)b0 * k, oid, (size_t)nb * k * 4, hipMemcpyDeviceToDevice, s));
+          HIP_TRY(hipMemcpyAsync(out_scores + (size_t)b0 * k, os, (size_t)nb * k * 4, hipMemcpyDeviceToDevice, s));
+        }
+      } else {
+        const int rc = run_pass(s, Qb, rm, n_rows, bnt, bnp, nb, oid, os, bi, true);
+        if (rc != MVR_OK) return rc;
+      }
+      if (!on_device) {
+        HIP_TRY(hipMemcpyAsync(out_ids + (size_t)b0 * k, oid, (size_t)nb * k * 4, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipMemcpyAsync(out_scores + (size_t)b0 * k, os, (size_t)nb * k * 4, hipMemcpyDeviceToHost, s));
+      }
+      if (stages) {
+        HIP_TRY(hipStreamSynchronize(s));
+        if (stages->top_ids)
+          HIP_TRY(hipMemcpy(stages->top_ids + (size_t)b0 * t_max * P, ws.top_ids.p, (size_t)nb * t_max * P * 4, hipMemcpyDeviceToHost));
+        if (stages->top_s)
+          HIP_TRY(hipMemcpy(stages->top_s + (size_t)b0 * t_max * P, ws.top_s.p, (size_t)nb * t_max * P * 4, hipMemcpyDeviceToHost));
+        if (stages->cand_n)
+          HIP_TRY(hipMemcpy(stages->cand_n + b0, ws.cand_tot.p, (size_t)nb * 4, hipMemcpyDeviceToHost));
+        if (stages->sel_ids)
+          HIP_TRY(hipMemcpy(stages->sel_ids + (size_t)b0 * R, ws.sel_ids.p, (size_t)nb * R * 4, hipMemcpyDeviceToHost));
+        if (stages->sel_F)
+          HIP_TRY(hipMemcpy(stages->sel_F + (size_t)b0 * R, ws.sel_F.p, (size_t)nb * R * 4, hipMemcpyDeviceToHost));
+        if (stages->sel_score)
+          HIP_TRY(hipMemcpy(stages->sel_score + (size_t)b0 * R, ws.sel_score.p, (size_t)nb * R * 4, hipMemcpyDeviceToHost));
+        if (stages->nprobe)
+          HIP_TRY(hipMemcpy(stages->nprobe + b0, learned ? ws.dyn_np.p : (const void *)(d_np + b0), (size_t)nb * 4,
+                            hipMemcpyDeviceToHost));
+        if (stages->rerank_n)
+          HIP_TRY(hipMemcpy(stages->rerank_n + b0, ws.sel_n.p, (size_t)nb * 4, hipMemcpyDeviceToHost));
+      }
+      if (timings)
+        HIP_TRY(launch_count_work(ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R, nb, d,
+                                  ws.cand_tot.as<uint32_t>(), d_nt + b0, ws.counters.as<unsigned long long>(), s));
     }
-    if (stages) {
-      HIP_TRY(hipStreamSynchronize(s));
-      if (stages->top_ids)
-        HIP_TRY(hipMemcpy(stages->top_ids + (size_t)b0 * t_max * P, ws.top_ids.p, (size_t)nb * t_max * P * 4, hipMemcpyDeviceToHost));
-      if (stages->top_s)
-        HIP_TRY(hipMemcpy(stages->top_s + (size_t)b0 * t_max * P, ws.top_s.p, (size_t)nb * t_max * P * 4, hipMemcpyDeviceToHost));
-      if (stages->cand_n)
-        HIP_TRY(hipMemcpy(stages->cand_n + b0, ws.cand_tot.p, (size_t)nb * 4, hipMemcpyDeviceToHost));
-      if (stages->sel_ids)
-        HIP_TRY(hipMemcpy(stages->sel_ids + (size_t)b0 * R, ws.sel_ids.p, (size_t)nb * R * 4, hipMemcpyDeviceToHost));
-      if (stages->sel_F)
-        HIP_TRY(hipMemcpy(stages->sel_F + (size_t)b0 * R, ws.sel_F.p, (size_t)nb * R * 4, hipMemcpyDeviceToHost));
-      if (stages->sel_score)
-        HIP_TRY(hipMemcpy(stages->sel_score + (size_t)b0 * R, ws.sel_score.p, (size_t)nb * R * 4, hipMemcpyDeviceToHost));
-      if (stages->nprobe)
-        HIP_TRY(hipMemcpy(stages->nprobe + b0, learned ? ws.dyn_np.p : (const void *)(d_np + b0), (size_t)nb * 4,
-                          hipMemcpyDeviceToHost));
-      if (stages->rerank_n)
-        HIP_TRY(hipMemcpy(stages->rerank_n + b0, ws.sel_n.p, (size_t)nb * 4, hipMemcpyDeviceToHost));
-    }
-    if (timings)
-      HIP_TRY(launch_count_work(ws.sel_ids.as<uint32_t>(), ws.sel_n.as<uint32_t>(), R, nb, d,
-                                ws.cand_tot.as<uint32_t>(), d_nt + b0, ws.counters.as<unsigned long long>(), s));
   }
   if (timings) {
-    HIP_TRY(hipEventSynchronize(ev.back()));
+    HIP_TRY(hipEventSynchronize(ev[(size_t)(n_pass - 1) * kNev + 6]));
     for (uint32_t b = 0; b < n_pass; ++b) {
-      float ms[kNev - 1];
-      for (int i = 0; i < kNev - 1; ++i)
+      float ms[6];
+      for (int i = 0; i < 6; ++i)
         HIP_TRY(hipEventElapsedTime(&ms[i], ev[b * kNev + i], ev[b * kNev + i + 1]));
-      timings->total_ms += ms[0] + ms[1] + ms[2] + ms[3] + ms[4] + ms[5];
+      // pipelined passes overlap: the call's span instead of the sum
+      if (!pipelined) timings->total_ms += ms[0] + ms[1] + ms[2] + ms[3] + ms[4] + ms[5];
       if (pass_graphed[b]) continue;  // a replayed graph has no stage boundaries
       timings->centroid_ms += ms[0];
       timings->filter_ms += ms[1];
@@ -859,6 +1143,19 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       timings->rerank_ms += ms[3];
       timings->rescore_ms += ms[4];
       timings->topk_ms += ms[5];
+      if (ix->res_host) {
+        // the MaxSim alone (from event 8) and the staging (selection end to staged docs)
+        float rm = 0.0f, sm = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&rm, ev[b * kNev + 8], ev[b * kNev + 4]));
+        HIP_TRY(hipEventElapsedTime(&sm, ev[b * kNev + 3], ev[b * kNev + 7]));
+        timings->rerank_ms += rm - ms[3];
+        timings->staging_ms += sm;
+      }
+    }
+    if (pipelined) {
+      float span = 0.0f;
+      HIP_TRY(hipEventElapsedTime(&span, ev[0], ev[(size_t)(n_pass - 1) * kNev + 6]));
+      timings->total_ms = span;
     }
     timings->n_pass = n_pass;
     unsigned long long cnt[3] = {0, 0, 0};
@@ -898,6 +1195,8 @@ int mvr_rerank_docs(mvr_index *ix, const float *Q_dev, uint32_t n_q, uint32_t t_
                     const uint32_t *q_ntok, const uint32_t *doc_ids_dev, const uint32_t *n_docs,
                     uint32_t R, uint32_t precision, float *scores_dev, void *stream) {
   if (!ix || !Q_dev || !doc_ids_dev || !scores_dev) return fail(MVR_ERR_INVALID, "NULL argument");
+  if (ix->res_host)
+    return fail(MVR_ERR_UNSUPPORTED, "mvr_rerank_docs needs a device-resident index (residuals are on the host)");
   if (n_q == 0 || R == 0) return MVR_OK;
   if (t_max == 0 || t_max > kMaxQTok) return fail(MVR_ERR_INVALID, "t_max %u not in [1, 32]", t_max);
   if (precision > MVR_FP32_MFMA) return fail(MVR_ERR_INVALID, "unknown precision %u", precision);
@@ -937,6 +1236,8 @@ int mvr_index_set_probe(mvr_index *ix, mvr_mlp *mlp, float conf, uint32_t fallba
 
 int mvr_decompress(mvr_index *ix, const uint32_t *doc_ids, uint32_t n, float *out) {
   if (!ix || (!doc_ids && n) || (!out && n)) return fail(MVR_ERR_INVALID, "NULL argument");
+  if (ix->res_host)
+    return fail(MVR_ERR_UNSUPPORTED, "mvr_decompress needs a device-resident index (residuals are on the host)");
   if (n == 0) return MVR_OK;
   std::vector<uint64_t> toks;
   uint64_t acc = 0;

@@ -116,6 +116,29 @@ hipError_t launch_rerank(const float *Q, uint32_t t_max, const uint32_t *q_ntok,
                          const DevIndex &ix, int precision, int clip_zero, float *sel_score,
                          hipStream_t s);
 
+// Per-pass compaction buffers of a host-resident-residual index (device).
+struct HostStage {
+  uint32_t *bm = nullptr;           // [ceil(n_doc/32)] selected-doc bitmap
+  uint32_t *b_n = nullptr;          // [blocks of 256 words] docs per block, then their exclusive prefix
+  unsigned long long *b_t = nullptr;// [blocks] tokens per block, then prefix
+  uint32_t *wbase = nullptr;        // [ceil(n_doc/32)] compact id of the word's first doc
+  uint32_t *docs = nullptr;         // [U] compact id -> doc id (ascending)
+  uint32_t *lens = nullptr;         // [U]
+  uint64_t *offs = nullptr;         // [U] token offset in the pass buffers
+  uint32_t *cids = nullptr;         // [nb][R] compact id of each selected doc
+  unsigned long long *cnt = nullptr;// [2] U, T
+};
+
+// marks the selected docs and writes the compact lists, cids and cnt
+hipError_t launch_hs_compact(const uint32_t *sel_ids, const uint32_t *sel_n, uint32_t R, uint32_t nb,
+                             const uint32_t *doclens, uint32_t n_doc, const HostStage &hs, hipStream_t s);
+// copies the U compact docs' codes and residual rows from host-mapped memory
+hipError_t launch_hs_gather(const HostStage &hs, uint32_t U, const uint64_t *docoff, const uint32_t *h_codes,
+                            const uint8_t *h_res, uint32_t *codes, uint8_t *res, hipStream_t s);
+
+// ids[i] = map[ids[i]] for ids[i] < n_map, other ids unchanged
+hipError_t launch_map_ids(uint32_t *ids, uint32_t n, const uint32_t *map, uint32_t n_map, hipStream_t s);
+
 // best k of n_in[q] (score, id) pairs; out_n (nullable) receives min(n_in, k)
 hipError_t launch_topk(const uint32_t *ids, const float *score, const uint32_t *n_in, uint32_t R,
                        uint32_t n_q, uint32_t k, uint32_t *out_ids, float *out_s, uint32_t *out_n,

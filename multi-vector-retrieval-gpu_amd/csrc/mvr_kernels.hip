@@ -18,6 +18,7 @@
 #include "mvr_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cfloat>
 #include <cmath>
 
@@ -2207,6 +2208,256 @@ hipError_t launch_exhaustive_f64(const float *Q, uint32_t n_q, uint32_t t_max,
   const uint32_t n_slice = std::max(1u, std::min((n_doc + 3) / 4, 8192u / n_q));
   dim3 grid(n_slice, n_q);
   k_exhaustive_f64<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, X, doclens, docoff, n_doc, n_slice, out);
+  return hipGetLastError();
+}
+
+// ---- host-resident residuals: per-pass compaction of the selected docs ----
+// The docs selected by a pass (over all its queries) are marked in a bitmap;
+// compact id = rank of the doc among the marked ones, so compact ids follow
+// doc ids.  Words of the bitmap go 256 to a block; per block the marked docs
+// and their tokens are counted, one block scans those counts, and the list
+// kernel writes (doc, length, token offset) per compact id.
+
+__global__ void k_hs_mark(const uint32_t *__restrict__ sel_ids, const uint32_t *__restrict__ sel_n, uint32_t R,
+                          uint32_t *__restrict__ bm) {
+  const uint32_t q = blockIdx.x, n = min(sel_n[q], R);
+  for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) {
+    const uint32_t doc = sel_ids[(size_t)q * R + j];
+    atomicOr(&bm[doc >> 5], 1u << (doc & 31u));
+  }
+}
+
+// marked docs and their token count in word w
+__device__ inline void hs_word(const uint32_t *__restrict__ bm, const uint32_t *__restrict__ doclens,
+                               uint32_t n_words, uint32_t w, uint32_t &word, uint32_t &n,
+                               unsigned long long &t) {
+  word = w < n_words ? bm[w] : 0u;
+  n = __popc(word);
+  t = 0;
+  for (uint32_t m = word; m; m &= m - 1) t += doclens[(size_t)w * 32 + (__ffs(m) - 1)];
+}
+
+// exclusive scan of (n, t) over a 256-thread block, plus the block totals
+__device__ inline void hs_scan256(uint32_t n, unsigned long long t, uint32_t &ex_n, unsigned long long &ex_t,
+                                  uint32_t &tot_n, unsigned long long &tot_t) {
+  __shared__ uint32_t sn[4];
+  __shared__ unsigned long long st[4];
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  uint32_t in = n;
+  unsigned long long it = t;
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t a = __shfl_up(in, o);
+    const unsigned long long b = __shfl_up(it, o);
+    if (lane >= o) {
+      in += a;
+      it += b;
+    }
+  }
+  if (lane == 63) {
+    sn[wv] = in;
+    st[wv] = it;
+  }
+  __syncthreads();
+  uint32_t bn = 0;
+  unsigned long long bt = 0;
+  tot_n = 0;
+  tot_t = 0;
+  for (uint32_t i = 0; i < 4; ++i) {
+    if (i < wv) {
+      bn += sn[i];
+      bt += st[i];
+    }
+    tot_n += sn[i];
+    tot_t += st[i];
+  }
+  ex_n = bn + in - n;
+  ex_t = bt + it - t;
+}
+
+__global__ __launch_bounds__(256) void k_hs_wsum(const uint32_t *__restrict__ bm, const uint32_t *__restrict__ doclens,
+                                                 uint32_t n_words, uint32_t *__restrict__ b_n,
+                                                 unsigned long long *__restrict__ b_t) {
+  uint32_t word, n;
+  unsigned long long t;
+  hs_word(bm, doclens, n_words, blockIdx.x * 256 + threadIdx.x, word, n, t);
+  uint32_t ex_n, tot_n;
+  unsigned long long ex_t, tot_t;
+  hs_scan256(n, t, ex_n, ex_t, tot_n, tot_t);
+  if (threadIdx.x == 0) {
+    b_n[blockIdx.x] = tot_n;
+    b_t[blockIdx.x] = tot_t;
+  }
+}
+
+// in-place exclusive scan of the block counts; cnt = (docs, tokens) in total
+__global__ __launch_bounds__(1024) void k_hs_bscan(uint32_t *__restrict__ b_n, unsigned long long *__restrict__ b_t,
+                                                   uint32_t nblk, unsigned long long *__restrict__ cnt) {
+  __shared__ uint32_t sn[16];
+  __shared__ unsigned long long st[16];
+  const uint32_t per = (nblk + 1023) / 1024;
+  const uint32_t i0 = min(nblk, threadIdx.x * per), i1 = min(nblk, i0 + per);
+  uint32_t n = 0;
+  unsigned long long t = 0;
+  for (uint32_t i = i0; i < i1; ++i) {
+    n += b_n[i];
+    t += b_t[i];
+  }
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  uint32_t in = n;
+  unsigned long long it = t;
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t a = __shfl_up(in, o);
+    const unsigned long long b = __shfl_up(it, o);
+    if (lane >= o) {
+      in += a;
+      it += b;
+    }
+  }
+  if (lane == 63) {
+    sn[wv] = in;
+    st[wv] = it;
+  }
+  __syncthreads();
+  uint32_t bn = 0, tn = 0;
+  unsigned long long bt = 0, tt = 0;
+  for (uint32_t i = 0; i < 16; ++i) {
+    if (i < wv) {
+      bn += sn[i];
+      bt += st[i];
+    }
+    tn += sn[i];
+    tt += st[i];
+  }
+  uint32_t base_n = bn + in - n;
+  unsigned long long base_t = bt + it - t;
+  for (uint32_t i = i0; i < i1; ++i) {
+    const uint32_t a = b_n[i];
+    const unsigned long long b = b_t[i];
+    b_n[i] = base_n;
+    b_t[i] = base_t;
+    base_n += a;
+    base_t += b;
+  }
+  if (threadIdx.x == 0) {
+    cnt[0] = tn;
+    cnt[1] = tt;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_hs_list(const uint32_t *__restrict__ bm, const uint32_t *__restrict__ doclens,
+                                                 uint32_t n_words, const uint32_t *__restrict__ b_n,
+                                                 const unsigned long long *__restrict__ b_t,
+                                                 uint32_t *__restrict__ wbase, uint32_t *__restrict__ docs,
+                                                 uint32_t *__restrict__ lens, uint64_t *__restrict__ offs) {
+  const uint32_t w = blockIdx.x * 256 + threadIdx.x;
+  uint32_t word, n;
+  unsigned long long t;
+  hs_word(bm, doclens, n_words, w, word, n, t);
+  uint32_t ex_n, tot_n;
+  unsigned long long ex_t, tot_t;
+  hs_scan256(n, t, ex_n, ex_t, tot_n, tot_t);
+  uint32_t c = b_n[blockIdx.x] + ex_n;
+  unsigned long long tb = b_t[blockIdx.x] + ex_t;
+  if (w < n_words) wbase[w] = c;
+  for (uint32_t m = word; m; m &= m - 1) {
+    const uint32_t doc = w * 32 + (__ffs(m) - 1);
+    const uint32_t len = doclens[doc];
+    docs[c] = doc;
+    lens[c] = len;
+    offs[c] = tb;
+    ++c;
+    tb += len;
+  }
+}
+
+__global__ void k_hs_remap(const uint32_t *__restrict__ sel_ids, const uint32_t *__restrict__ sel_n, uint32_t R,
+                           const uint32_t *__restrict__ bm, const uint32_t *__restrict__ wbase,
+                           uint32_t *__restrict__ cids) {
+  const uint32_t q = blockIdx.x, n = min(sel_n[q], R);
+  for (uint32_t j = threadIdx.x; j < R; j += blockDim.x) {
+    uint32_t c = 0;
+    if (j < n) {
+      const uint32_t doc = sel_ids[(size_t)q * R + j], w = doc >> 5;
+      c = wbase[w] + __popc(bm[w] & ((1u << (doc & 31u)) - 1u));
+    }
+    cids[(size_t)q * R + j] = c;
+  }
+}
+
+// One wave per compact doc: its codes and residual rows read from host
+// memory mapped into the device's address space, written to the pass buffers.
+__global__ __launch_bounds__(256) void k_hs_gather(const uint32_t *__restrict__ docs, const uint32_t *__restrict__ lens,
+                                                   const uint64_t *__restrict__ offs, const uint64_t *__restrict__ docoff,
+                                                   uint32_t U, const uint32_t *__restrict__ h_codes,
+                                                   const uint4 *__restrict__ h_res, uint32_t *__restrict__ codes,
+                                                   uint4 *__restrict__ res) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t n_waves = gridDim.x * 4;
+  for (uint32_t c = blockIdx.x * 4 + (threadIdx.x >> 6); c < U; c += n_waves) {
+    const uint64_t src = docoff[docs[c]], dst = offs[c];
+    const uint32_t len = lens[c];
+    for (uint32_t i = lane; i < len; i += 64) codes[dst + i] = h_codes[src + i];
+    // 32 bytes per token = two uint4; four loads in flight per lane
+    const uint32_t n4 = len * 2;
+    const uint4 *sp = h_res + src * 2;
+    uint4 *dp = res + dst * 2;
+    for (uint32_t i = lane; i < n4; i += 256) {
+      uint4 v[4];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u)
+        if (i + u * 64 < n4) v[u] = sp[i + u * 64];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u)
+        if (i + u * 64 < n4) dp[i + u * 64] = v[u];
+    }
+  }
+}
+
+hipError_t launch_hs_compact(const uint32_t *sel_ids, const uint32_t *sel_n, uint32_t R, uint32_t nb,
+                             const uint32_t *doclens, uint32_t n_doc, const HostStage &hs, hipStream_t s) {
+  const uint32_t n_words = (n_doc + 31) / 32, nblk = (n_words + 255) / 256;
+  hipError_t e = launch_fill_u32(hs.bm, 0u, n_words, s);
+  if (e != hipSuccess || nb == 0) return e;
+  k_hs_mark<<<nb, 256, 0, s>>>(sel_ids, sel_n, R, hs.bm);
+  k_hs_wsum<<<nblk, 256, 0, s>>>(hs.bm, doclens, n_words, hs.b_n, hs.b_t);
+  k_hs_bscan<<<1, 1024, 0, s>>>(hs.b_n, hs.b_t, nblk, hs.cnt);
+  k_hs_list<<<nblk, 256, 0, s>>>(hs.bm, doclens, n_words, hs.b_n, hs.b_t, hs.wbase, hs.docs, hs.lens, hs.offs);
+  k_hs_remap<<<nb, 256, 0, s>>>(sel_ids, sel_n, R, hs.bm, hs.wbase, hs.cids);
+  return hipGetLastError();
+}
+
+// The gather is bound by the host link, not by the CUs: a few blocks keep
+// enough reads in flight, and the rest of the chip stays free for the MaxSim
+// of the other pass in flight (MVR_HS_GATHER_BLOCKS overrides the count).
+hipError_t launch_hs_gather(const HostStage &hs, uint32_t U, const uint64_t *docoff, const uint32_t *h_codes,
+                            const uint8_t *h_res, uint32_t *codes, uint8_t *res, hipStream_t s) {
+  if (U == 0) return hipSuccess;
+  static const uint32_t max_blocks = [] {
+    const char *e = std::getenv("MVR_HS_GATHER_BLOCKS");
+    const long v = e ? std::strtol(e, nullptr, 10) : 0;
+    return v > 0 && v <= 65536 ? (uint32_t)v : 128u;
+  }();
+  const uint32_t grid = std::min<uint32_t>((U + 3) / 4, max_blocks);
+  k_hs_gather<<<grid, 256, 0, s>>>(hs.docs, hs.lens, hs.offs, docoff, U, h_codes,
+                                   reinterpret_cast<const uint4 *>(h_res), codes, reinterpret_cast<uint4 *>(res));
+  return hipGetLastError();
+}
+
+// ids[i] = map[ids[i]] where ids[i] < n_map (compact ids of a host-staged pass
+// back to doc ids; the 0xFFFFFFFF fill of short result lists stays)
+__global__ void k_map_ids(uint32_t *ids, uint32_t n, const uint32_t *map, uint32_t n_map) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    const uint32_t v = ids[i];
+    if (v < n_map) ids[i] = map[v];
+  }
+}
+
+hipError_t launch_map_ids(uint32_t *ids, uint32_t n, const uint32_t *map, uint32_t n_map, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_map_ids<<<(n + 255) / 256, 256, 0, s>>>(ids, n, map, n_map);
   return hipGetLastError();
 }
 

@@ -54,8 +54,13 @@ def _ivf_from_codes(codes, doclens, n_c):
     return off, sizes, ids
 
 
+def _flags(residual_on_host: bool) -> int:
+    return _capi.INDEX_RESIDUAL_ON_HOST if residual_on_host else 0
+
+
 class Index:
-    """Device-resident IGP index (centroids, codes, residuals, IVF)."""
+    """IGP index on one device (centroids, codes, residuals, IVF); with
+    residual_on_host the codes and residuals stay in host memory."""
 
     def __init__(self, handle: C.c_void_p, device: int = 0):
         self._h = handle
@@ -67,7 +72,10 @@ class Index:
 
     # ---------------------------------------------------------------- create
     @classmethod
-    def from_arrays(cls, doclens, centroids, codes, residual, weights, ivf=None, device: int = 0):
+    def from_arrays(cls, doclens, centroids, codes, residual, weights, ivf=None, device: int = 0,
+                    residual_on_host: bool = False):
+        """residual_on_host keeps codes and residual bytes in (pinned) host
+        memory; each device pass then stages its selected docs."""
         doclens = _c(doclens, np.uint32)
         centroids = _c(centroids, np.float32)
         codes = _c(codes, np.uint32)
@@ -87,11 +95,12 @@ class Index:
         d.ivf_n_ele = ids.size
         d.on_device = 0
         h = C.c_void_p()
-        check(lib().mvr_index_create(C.byref(d), device, C.byref(h)))
+        check(lib().mvr_index_create_ex(C.byref(d), device, _flags(residual_on_host), C.byref(h)))
         return cls(h, device).set_doclens(doclens)
 
     @classmethod
-    def from_device(cls, doclens, centroids, codes, residual, weights, ivf, device: int = 0):
+    def from_device(cls, doclens, centroids, codes, residual, weights, ivf, device: int = 0,
+                    residual_on_host: bool = False):
         """Arguments are torch tensors on `device` (copied by the library)."""
         d = _capi.IndexDesc()
         d.n_doc, d.n_tok = doclens.numel(), codes.numel()
@@ -104,13 +113,13 @@ class Index:
         d.ivf_n_ele = ivf[2].numel()
         d.on_device = 1
         h = C.c_void_p()
-        check(lib().mvr_index_create(C.byref(d), device, C.byref(h)))
+        check(lib().mvr_index_create_ex(C.byref(d), device, _flags(residual_on_host), C.byref(h)))
         return cls(h, device).set_doclens(doclens.cpu().numpy())
 
     @classmethod
-    def load(cls, index_dir: str, device: int = 0):
+    def load(cls, index_dir: str, device: int = 0, residual_on_host: bool = False):
         h = C.c_void_p()
-        check(lib().mvr_index_load(index_dir.encode(), device, C.byref(h)))
+        check(lib().mvr_index_load_ex(index_dir.encode(), device, _flags(residual_on_host), C.byref(h)))
         return cls(h, device).set_doclens(np.load(os.path.join(index_dir, "item_n_vec_l.npy")))
 
     def close(self):
@@ -240,10 +249,12 @@ class IGPGPUPP:
     """Method object with the reference's call sequence:
     IGPGPUPP(item_n_vec_l_fname, n_thread) -> buildIndex(...) -> search(...)."""
 
-    def __init__(self, item_n_vec_l_fname: str, n_cpu_thread: int = 1, device: int = 0):
+    def __init__(self, item_n_vec_l_fname: str, n_cpu_thread: int = 1, device: int = 0,
+                 residual_on_host: bool = False):
         self._item_n_vec_l = np.load(item_n_vec_l_fname).astype(np.uint32)
         self.n_cpu_thread = n_cpu_thread
         self.device = device
+        self.residual_on_host = residual_on_host
         self.index: Index | None = None
 
     def buildIndex(self, centroid_l_fname, vq_code_l_fname, ivf_index_fname, weight_l_fname,
@@ -252,7 +263,7 @@ class IGPGPUPP:
                 (centroid_l_fname, vq_code_l_fname, ivf_index_fname, weight_l_fname, residual_code_l_fname)}
         if len(dirs) != 1:
             raise MvrError(1, "index files must share one directory")
-        self.index = Index.load(dirs.pop(), self.device).set_doclens(self._item_n_vec_l)
+        self.index = Index.load(dirs.pop(), self.device, self.residual_on_host).set_doclens(self._item_n_vec_l)
         if self.index.n_centroid != n_centroid or n_bit != 2:
             raise MvrError(1, f"index has {self.index.n_centroid} centroids/2 bits, asked {n_centroid}/{n_bit}")
         return True

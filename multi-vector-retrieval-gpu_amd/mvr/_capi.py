@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(LIB_DIR, "libmvr.so")
 
 MVR_OK = 0
 MVR_FP32, MVR_BF16, MVR_FP8, MVR_FP32_MFMA = 0, 1, 2, 3
+INDEX_RESIDUAL_ON_HOST = 1  # MVR_INDEX_RESIDUAL_ON_HOST
 
 
 class MvrError(RuntimeError):
@@ -45,7 +46,8 @@ class Timings(C.Structure):
     _fields_ = [("centroid_ms", C.c_double), ("filter_ms", C.c_double), ("select_ms", C.c_double),
                 ("rerank_ms", C.c_double), ("rescore_ms", C.c_double), ("topk_ms", C.c_double),
                 ("total_ms", C.c_double), ("rerank_tokens", C.c_uint64), ("candidates", C.c_uint64),
-                ("n_pass", C.c_uint32), ("rerank_pairs", C.c_uint64)]
+                ("n_pass", C.c_uint32), ("rerank_pairs", C.c_uint64), ("staging_ms", C.c_double),
+                ("staged_docs", C.c_uint64), ("staged_tokens", C.c_uint64)]
 
 
 class PlaidIndexDesc(C.Structure):
@@ -75,6 +77,8 @@ SIGNATURES = {
     "mvr_last_error": (C.c_char_p, []),
     "mvr_index_create": (C.c_int, [C.POINTER(IndexDesc), C.c_int, C.POINTER(C.c_void_p)]),
     "mvr_index_load": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]),
+    "mvr_index_create_ex": (C.c_int, [C.POINTER(IndexDesc), C.c_int, C.c_uint32, C.POINTER(C.c_void_p)]),
+    "mvr_index_load_ex": (C.c_int, [C.c_char_p, C.c_int, C.c_uint32, C.POINTER(C.c_void_p)]),
     "mvr_index_free": (None, [C.c_void_p]),
     "mvr_index_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
                                  C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),

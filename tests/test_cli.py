@@ -78,15 +78,15 @@ def write_dataset(root, ix, Q, qids, n_c):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("exe,clip", [(EXE, False), (EXE_BASIC, True)])
-def test_cli_answer_matches_oracle(tmp_path, small, exe, clip):
+@pytest.mark.parametrize("exe,clip,host", [(EXE, False, False), (EXE_BASIC, True, False), (EXE, False, True)])
+def test_cli_answer_matches_oracle(tmp_path, small, exe, clip, host):
     corpus, ix, Q = small
     qids = [100 + 3 * i for i in range(Q.shape[0])]
     write_dataset(str(tmp_path), ix, Q, qids, ix.n_c)
     env = dict(os.environ, MVR_DATASET_ROOT=str(tmp_path))
     args = FLAGS if exe == EXE else [a for i, a in enumerate(FLAGS)
                                      if a != "-n_thread" and (i == 0 or FLAGS[i - 1] != "-n_thread")]
-    r = _run(exe, args + ["-batch", "5"], env)
+    r = _run(exe, args + ["-batch", "5"] + (["-residual_on_host", "true"] if host else []), env)
     assert r.returncode == 0, r.stderr
     ans = os.path.join(tmp_path, "Result", "answer")
     suffix = "nprobe_4-probe_topk_64" + ("-n_thread_1" if exe == EXE else "")
