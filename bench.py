@@ -8,7 +8,7 @@ built on the device (synthetic data, seeded); a step is one search of the
 4,096-query batch with inputs resident in HBM.
 
 Prints one JSON line: queries/sec (value), ms per step, p50/p99 latency of a
-1,024-query device pass, recall@100 against FP64 exhaustive MaxSim over the
+device pass (--batch queries, default the whole 4,096-query batch), recall@100 against FP64 exhaustive MaxSim over the
 original embeddings, the rerank kernel's roofline figures, and a CPU baseline
 (the C oracle of the same path on the host cores, on a bounded query sample).
 """
@@ -75,7 +75,7 @@ def main():
     ap.add_argument("--persistent", action="store_true",
                     help="MaxSim + top-k as one persistent work-queue kernel per pass")
     ap.add_argument("--graph", action="store_true", help="replay each device pass from a hipGraph")
-    ap.add_argument("--batch", type=int, default=1024, help="queries per device pass")
+    ap.add_argument("--batch", type=int, default=4096, help="queries per device pass")
     ap.add_argument("--residual-on-host", action="store_true",
                     help="codes and residuals in pinned host memory, staged per device pass")
     ap.add_argument("--epochs", type=int, default=10, help="training epochs of the learned-probe configs")
@@ -148,7 +148,7 @@ def main():
         step(timings=True)
     torch.cuda.synchronize()
     # Timed region: K whole-batch searches.  Each call records HIP events
-    # around every stage of every 1,024-query pass on the search stream (read
+    # around every stage of every device pass on the search stream (read
     # back once per call), so the rerank kernel's launch durations come from
     # the timed work itself.
     stage_keys = ("centroid_ms", "filter_ms", "select_ms", "rerank_ms", "rescore_ms", "topk_ms", "total_ms")
@@ -174,7 +174,7 @@ def main():
             f"over the staging time)")
     log(f"[bench] stages (ms/step): " + ", ".join(f"{f}={stages[f]:.2f}" for f in stage_keys))
 
-    # per-pass latency: single 1,024-query passes bracketed by events on the search stream
+    # per-pass latency: single device passes bracketed by events on the search stream
     ev = []
     for _ in range(max(2, min(args.steps, 8))):
         for b0 in range(0, n_q, batch):

@@ -162,6 +162,7 @@ struct mvr_index {
   bool xl_failed = false;
   // MVR_INDEX_RESIDUAL_ON_HOST: residual bytes and a copy of the codes stay in
   // host memory (residual pinned); each pass stages its selected docs
+  uint64_t pool_budget = kPoolBudget;  // candidate pool bytes per pass (from free HBM at creation)
   bool res_host = false;
   uint8_t *h_res = nullptr;          // [n_tok][32], pinned and mapped
   uint32_t *h_codes = nullptr;       // [n_tok], pinned and mapped
@@ -379,6 +380,12 @@ int mvr_index_create_ex(const mvr_index_desc *desc, int device, uint32_t flags, 
   if (e == hipSuccess) e = launch_convert_centroids(d, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) return bail(fail(MVR_ERR_DEVICE, "index setup failed: %s", hipGetErrorString(e)));
+  {
+    // a quarter of the HBM left free by the index, between 2 and 64 GiB
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+      ix->pool_budget = std::min<uint64_t>(kPoolBudgetMax, std::max<uint64_t>(kPoolBudgetMin, free_b / 4));
+  }
   *out = ix;
   return MVR_OK;
 }
@@ -636,7 +643,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   // per (query, doc segment), so a pass is also capped by the pool budget
   const uint64_t pool_per_q = (uint64_t)d.n_seg * kSeg * 8;
   const uint32_t B = (uint32_t)std::max<uint64_t>(
-      1, std::min<uint64_t>(prm->batch ? prm->batch : 1024, kPoolBudget / pool_per_q));
+      1, std::min<uint64_t>(prm->batch ? prm->batch : 1024, ix->pool_budget / pool_per_q));
   DeviceGuard g(ix->device);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   Workspace &ws = ix->ws;

@@ -13,7 +13,8 @@ constexpr uint32_t kResBytes = 32;         // packed 2-bit residual bytes per to
 constexpr uint32_t kMaxQTok = 32;          // query tokens (bit mask width in the filter)
 constexpr uint32_t kMaxProbe = 32;         // largest nprobe
 constexpr uint32_t kSeg = 4096;            // docs per filter segment (LDS state)
-constexpr uint64_t kPoolBudget = 16ull << 30;  // candidate pool bytes per device pass
+constexpr uint64_t kPoolBudget = 16ull << 30;  // candidate pool bytes per device pass (default)
+constexpr uint64_t kPoolBudgetMin = 2ull << 30, kPoolBudgetMax = 64ull << 30;
 constexpr uint32_t kMaxSelect = 4096;      // largest probe_topk
 constexpr double kFxScale = 4294967296.0;  // filter fixed point: 2^32
 

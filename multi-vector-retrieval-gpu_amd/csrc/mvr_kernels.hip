@@ -878,6 +878,21 @@ __device__ __forceinline__ void load_q_f32(const float *__restrict__ Q, uint32_t
   }
 }
 
+// FP32 MFMA MaxSim kernel.  The query tile sits in LDS instead of 64
+// registers per lane: ql[col * kQlStride + d] = q[col][d] (zero for col >= T),
+// read as float4 per four MFMAs (rows 528 B apart, so the 16 lanes of a
+// 128-bit read phase hit 64 distinct banks), which leaves room for 4 waves per
+// SIMD.  Each wave streams its docs (slots slice*4 + wave, + 4*n_slice, ...)
+// as chunks of 8 tokens, four chunks to a 32-row tile, so a tile can span doc
+// boundaries and only the last chunk of a doc is padded (95% of the MFMA rows
+// carry tokens on a lognormal length mix around 67 tokens, against 81% with a
+// tile per 32 tokens of one doc).  Row m of a tile is token 8*(m>>3) + (m&7)'s
+// slot: lane (col, h) holds rows 8g + 4h + e (g = j>>2, e = j&3) of acc, so
+// the max over a chunk is over j in [4g, 4g+4) plus one cross-half shuffle.
+// Operands and their order are those of maxsim_doc_f32mfma per token, so the
+// scores are bit-identical to it.
+constexpr uint32_t kQlStride = kDim + 4;
+
 __global__ __launch_bounds__(256) void k_rerank_f32mfma(
     const float *__restrict__ Q, uint32_t t_max, const uint32_t *__restrict__ q_ntok,
     const uint32_t *__restrict__ sel_ids, const uint32_t *__restrict__ sel_n, uint32_t R,
@@ -887,29 +902,128 @@ __global__ __launch_bounds__(256) void k_rerank_f32mfma(
     float *__restrict__ sel_score) {
   __shared__ float wl[kDim * 4];
   __shared__ float Ms[4][kMaxQTok];
-  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  __shared__ __align__(16) float ql[kMaxQTok * kQlStride];
+  const uint32_t tid = threadIdx.x, lane = tid & 63;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const uint32_t h = lane >> 5, col = lane & 31;
   const uint32_t slice = blockIdx.x, q = blockIdx.y;
   const uint32_t T = q_ntok ? q_ntok[q] : t_max;
   const uint32_t n = sel_n[q];
 
   for (uint32_t i = tid; i < kDim * 4; i += 256) wl[i] = W[i];
-  float qf[64];
-  load_q_f32(Q, q, t_max, T, h, col, qf);
+  for (uint32_t i = tid; i < kMaxQTok * kDim / 4; i += 256) {
+    const uint32_t c = i / (kDim / 4), d4 = i % (kDim / 4);
+    const float4 v = c < T ? reinterpret_cast<const float4 *>(Q + ((size_t)q * t_max + c) * kDim)[d4]
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+    *reinterpret_cast<float4 *>(ql + c * kQlStride + 4 * d4) = v;
+  }
   __syncthreads();
 
-  for (uint32_t r = slice * 4 + wave; r < n; r += n_slice * 4) {
-    const uint32_t doc = sel_ids[(size_t)q * R + r];
-    const float M = maxsim_doc_f32mfma<1>(doclens[doc], docoff[doc], h, col, codes, residual, C, wl, qf,
-                                          clip_zero);
+  const float init = clip_zero ? 0.0f : -FLT_MAX;
+  const uint32_t stride = n_slice * 4;
+  const uint32_t *ids = sel_ids + (size_t)q * R;
+  // score of slot rr from this lane's running max: halves combined, then the
+  // left-to-right sum over query tokens
+  auto finish = [&](uint32_t rr, float M) {
+    M = fmaxf(M, __shfl_xor(M, 32));
     if (h == 0) Ms[wave][col] = M;
     wave_lds_sync();
     if (lane == 0) {
       float sc = 0.0f;
       for (uint32_t tt = 0; tt < T; ++tt) sc += Ms[wave][tt];
-      sel_score[(size_t)q * R + r] = sc;
+      sel_score[(size_t)q * R + rr] = sc;
     }
     wave_lds_sync();
+  };
+
+  // the open doc: slot r, length L, first token off, next chunk at pos
+  uint32_t r = slice * 4 + wave, L = 0, pos = 0;
+  uint64_t off = 0;
+  auto open = [&]() {
+    // empty docs have no chunk: their score comes from the initial maxima
+    while (r < n) {
+      const uint32_t doc = ids[r];
+      L = doclens[doc];
+      off = docoff[doc];
+      pos = 0;
+      if (L > 0) break;
+      finish(r, init);
+      r += stride;
+    }
+  };
+  open();
+  float Mrun = init;
+  const uint32_t qoff0 = col * kQlStride + 64 * h;
+  while (r < n) {
+    // the tile's four chunks (wave-uniform): slot, first token, length, last chunk of its doc
+    uint32_t cr[4], cp[4], cl[4];
+    uint64_t co[4];
+    bool cv_[4], last[4];
+#pragma unroll
+    for (uint32_t g = 0; g < 4; ++g) {
+      cv_[g] = r < n;
+      cr[g] = r;
+      cp[g] = pos;
+      cl[g] = L;
+      co[g] = off;
+      last[g] = cv_[g] && pos + 8 >= L;
+      if (cv_[g]) {
+        pos += 8;
+        if (last[g]) {
+          r += stride;
+          open();
+        }
+      }
+    }
+    // this lane's row: chunk col >> 3, token (col & 7) of it
+    const uint32_t g = col >> 3;
+    const uint32_t tok = (g == 0 ? cp[0] : g == 1 ? cp[1] : g == 2 ? cp[2] : cp[3]) + (col & 7);
+    const uint32_t len = g == 0 ? cl[0] : g == 1 ? cl[1] : g == 2 ? cl[2] : cl[3];
+    const uint64_t base = g == 0 ? co[0] : g == 1 ? co[1] : g == 2 ? co[2] : co[3];
+    const bool ok = (g == 0 ? cv_[0] : g == 1 ? cv_[1] : g == 2 ? cv_[2] : cv_[3]) && tok < len;
+    float4 cv[16];
+    uint4 rb = make_uint4(0u, 0u, 0u, 0u);
+    if (ok) {
+      const uint64_t gt = base + tok;
+      const uint32_t code = codes[gt];
+      rb = *reinterpret_cast<const uint4 *>(residual + gt * kResBytes + 16 * h);
+      const float4 *cptr = reinterpret_cast<const float4 *>(C + (size_t)code * kDim + 64 * h);
+#pragma unroll
+      for (uint32_t s2 = 0; s2 < 16; ++s2) cv[s2] = cptr[s2];
+    } else {
+#pragma unroll
+      for (uint32_t s2 = 0; s2 < 16; ++s2) cv[s2] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    // keep the query reads inside the loop (not hoisted into 64 registers)
+    uint32_t qoff = qoff0;
+    asm volatile("" : "+v"(qoff));
+    const float4 *qr = reinterpret_cast<const float4 *>(ql + qoff);
+    floatx16_t acc = floatx16_t{};
+    const uint32_t rw[4] = {rb.x, rb.y, rb.z, rb.w};
+#pragma unroll
+    for (uint32_t s2 = 0; s2 < 16; ++s2) {
+      const uint32_t byte = (rw[s2 >> 2] >> (8 * (s2 & 3))) & 0xFFu;
+      const float *wd = wl + 4 * (64 * h + 4 * s2);
+      const float x0 = cv[s2].x + wd[(byte >> 6) & 3u];
+      const float x1 = cv[s2].y + wd[4 + ((byte >> 4) & 3u)];
+      const float x2 = cv[s2].z + wd[8 + ((byte >> 2) & 3u)];
+      const float x3 = cv[s2].w + wd[12 + (byte & 3u)];
+      const float4 qv = qr[s2];
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x0, qv.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x1, qv.y, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x2, qv.z, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x3, qv.w, acc, 0, 0, 0);
+    }
+    // per chunk: max over its valid rows, folded into the open doc's max
+#pragma unroll
+    for (uint32_t gg = 0; gg < 4; ++gg) {
+      if (!cv_[gg]) break;
+      if (cp[gg] == 0) Mrun = init;
+#pragma unroll
+      for (uint32_t e = 0; e < 4; ++e)
+        if (cp[gg] + 4 * h + e < cl[gg]) Mrun = fmaxf(Mrun, acc[4 * gg + e]);
+      if (last[gg]) finish(cr[gg], Mrun);
+    }
   }
 }
 
