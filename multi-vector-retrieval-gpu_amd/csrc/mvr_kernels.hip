@@ -900,8 +900,10 @@ __global__ __launch_bounds__(256) void k_rerank_f32mfma(
     const uint32_t *__restrict__ codes, const uint8_t *__restrict__ residual,
     const float *__restrict__ C, const float *__restrict__ W, int clip_zero,
     float *__restrict__ sel_score) {
-  __shared__ float wl[kDim * 4];
-  __shared__ float Ms[4][kMaxQTok];
+  // decode pairs: tab2[p * 16 + v] = (W[2p][v >> 2], W[2p + 1][v & 3]), the
+  // two weights that a 4-bit nibble of residual bits selects for dims 2p, 2p+1
+  __shared__ float2 tab2[kDim / 2 * 16];
+  __shared__ float Ms[4][4][kMaxQTok + 1];  // [wave][doc finished in the tile][query token]
   __shared__ __align__(16) float ql[kMaxQTok * kQlStride];
   const uint32_t tid = threadIdx.x, lane = tid & 63;
   const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -910,7 +912,10 @@ __global__ __launch_bounds__(256) void k_rerank_f32mfma(
   const uint32_t T = q_ntok ? q_ntok[q] : t_max;
   const uint32_t n = sel_n[q];
 
-  for (uint32_t i = tid; i < kDim * 4; i += 256) wl[i] = W[i];
+  for (uint32_t i = tid; i < kDim / 2 * 16; i += 256) {
+    const uint32_t pp = i >> 4, v = i & 15u;
+    tab2[i] = make_float2(W[(2 * pp) * 4 + (v >> 2)], W[(2 * pp + 1) * 4 + (v & 3u)]);
+  }
   for (uint32_t i = tid; i < kMaxQTok * kDim / 4; i += 256) {
     const uint32_t c = i / (kDim / 4), d4 = i % (kDim / 4);
     const float4 v = c < T ? reinterpret_cast<const float4 *>(Q + ((size_t)q * t_max + c) * kDim)[d4]
@@ -922,33 +927,18 @@ __global__ __launch_bounds__(256) void k_rerank_f32mfma(
   const float init = clip_zero ? 0.0f : -FLT_MAX;
   const uint32_t stride = n_slice * 4;
   const uint32_t *ids = sel_ids + (size_t)q * R;
-  // score of slot rr from this lane's running max: halves combined, then the
-  // left-to-right sum over query tokens
-  auto finish = [&](uint32_t rr, float M) {
-    M = fmaxf(M, __shfl_xor(M, 32));
-    if (h == 0) Ms[wave][col] = M;
-    wave_lds_sync();
-    if (lane == 0) {
-      float sc = 0.0f;
-      for (uint32_t tt = 0; tt < T; ++tt) sc += Ms[wave][tt];
-      sel_score[(size_t)q * R + rr] = sc;
-    }
-    wave_lds_sync();
-  };
 
-  // the open doc: slot r, length L, first token off, next chunk at pos
+  // the open doc: slot r, length L, first token off, next chunk at pos (an
+  // empty doc is one chunk without tokens, so its score is the sum of the
+  // initial maxima)
   uint32_t r = slice * 4 + wave, L = 0, pos = 0;
   uint64_t off = 0;
   auto open = [&]() {
-    // empty docs have no chunk: their score comes from the initial maxima
-    while (r < n) {
+    pos = 0;
+    if (r < n) {
       const uint32_t doc = ids[r];
       L = doclens[doc];
       off = docoff[doc];
-      pos = 0;
-      if (L > 0) break;
-      finish(r, init);
-      r += stride;
     }
   };
   open();
@@ -981,48 +971,84 @@ __global__ __launch_bounds__(256) void k_rerank_f32mfma(
     const uint32_t len = g == 0 ? cl[0] : g == 1 ? cl[1] : g == 2 ? cl[2] : cl[3];
     const uint64_t base = g == 0 ? co[0] : g == 1 ? co[1] : g == 2 ? co[2] : co[3];
     const bool ok = (g == 0 ? cv_[0] : g == 1 ? cv_[1] : g == 2 ? cv_[2] : cv_[3]) && tok < len;
-    float4 cv[16];
+    // rows without a token decode centroid 0 with zero residual bits (any
+    // finite values: the max skips those rows), so no register is zeroed
+    uint32_t code = 0;
     uint4 rb = make_uint4(0u, 0u, 0u, 0u);
     if (ok) {
       const uint64_t gt = base + tok;
-      const uint32_t code = codes[gt];
+      code = codes[gt];
       rb = *reinterpret_cast<const uint4 *>(residual + gt * kResBytes + 16 * h);
-      const float4 *cptr = reinterpret_cast<const float4 *>(C + (size_t)code * kDim + 64 * h);
-#pragma unroll
-      for (uint32_t s2 = 0; s2 < 16; ++s2) cv[s2] = cptr[s2];
-    } else {
-#pragma unroll
-      for (uint32_t s2 = 0; s2 < 16; ++s2) cv[s2] = make_float4(0.f, 0.f, 0.f, 0.f);
     }
+    float4 cv[16];
+    const float4 *cptr = reinterpret_cast<const float4 *>(C + (size_t)code * kDim + 64 * h);
+#pragma unroll
+    for (uint32_t s2 = 0; s2 < 16; ++s2) cv[s2] = cptr[s2];
     // keep the query reads inside the loop (not hoisted into 64 registers)
     uint32_t qoff = qoff0;
     asm volatile("" : "+v"(qoff));
     const float4 *qr = reinterpret_cast<const float4 *>(ql + qoff);
-    floatx16_t acc = floatx16_t{};
+    // decode the whole tile first (the 32 data-dependent weight-pair reads
+    // in flight together, added into the centroid values in place), then the
+    // MFMA chain with no LDS wait between instructions but the query reads.
+    // Residual byte 16h + s2 holds dims 64h + 4 s2 .. +3, most significant
+    // bits first: its high nibble selects pair 32h + 2 s2, the low one the next.
     const uint32_t rw[4] = {rb.x, rb.y, rb.z, rb.w};
+    const float2 *tp = tab2 + 32 * 16 * h;
 #pragma unroll
     for (uint32_t s2 = 0; s2 < 16; ++s2) {
-      const uint32_t byte = (rw[s2 >> 2] >> (8 * (s2 & 3))) & 0xFFu;
-      const float *wd = wl + 4 * (64 * h + 4 * s2);
-      const float x0 = cv[s2].x + wd[(byte >> 6) & 3u];
-      const float x1 = cv[s2].y + wd[4 + ((byte >> 4) & 3u)];
-      const float x2 = cv[s2].z + wd[8 + ((byte >> 2) & 3u)];
-      const float x3 = cv[s2].w + wd[12 + (byte & 3u)];
-      const float4 qv = qr[s2];
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x0, qv.x, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x1, qv.y, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x2, qv.z, acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(x3, qv.w, acc, 0, 0, 0);
+      const uint32_t sh = 8 * (s2 & 3);
+      const float2 a2 = tp[(2 * s2) * 16 + ((rw[s2 >> 2] >> (sh + 4)) & 15u)];
+      const float2 b2 = tp[(2 * s2 + 1) * 16 + ((rw[s2 >> 2] >> sh) & 15u)];
+      cv[s2].x += a2.x;
+      cv[s2].y += a2.y;
+      cv[s2].z += b2.x;
+      cv[s2].w += b2.y;
     }
-    // per chunk: max over its valid rows, folded into the open doc's max
+    __builtin_amdgcn_sched_barrier(0);
+    floatx16_t acc = floatx16_t{};
+#pragma unroll
+    for (uint32_t s2 = 0; s2 < 16; ++s2) {
+      const float4 qv = qr[s2];
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[s2].x, qv.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[s2].y, qv.y, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[s2].z, qv.z, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[s2].w, qv.w, acc, 0, 0, 0);
+    }
+    // per chunk: max over its valid rows, folded into the open doc's max;
+    // a doc's last chunk parks its per-token maxima (halves combined) in LDS
+    uint32_t nfin = 0, fin_r[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
     for (uint32_t gg = 0; gg < 4; ++gg) {
-      if (!cv_[gg]) break;
-      if (cp[gg] == 0) Mrun = init;
+      if (cv_[gg]) {
+        if (cp[gg] == 0) Mrun = init;
+        if (cp[gg] + 8 <= cl[gg]) {
 #pragma unroll
-      for (uint32_t e = 0; e < 4; ++e)
-        if (cp[gg] + 4 * h + e < cl[gg]) Mrun = fmaxf(Mrun, acc[4 * gg + e]);
-      if (last[gg]) finish(cr[gg], Mrun);
+          for (uint32_t e = 0; e < 4; ++e) Mrun = fmaxf(Mrun, acc[4 * gg + e]);
+        } else {
+#pragma unroll
+          for (uint32_t e = 0; e < 4; ++e)
+            if (cp[gg] + 4 * h + e < cl[gg]) Mrun = fmaxf(Mrun, acc[4 * gg + e]);
+        }
+        if (last[gg]) {
+          const float M2 = fmaxf(Mrun, __shfl_xor(Mrun, 32));
+          if (h == 0) Ms[wave][nfin][col] = M2;
+          fin_r[nfin] = cr[gg];
+          ++nfin;
+        }
+      }
+    }
+    // scores of the docs finished in this tile, one lane each: the
+    // left-to-right sum over query tokens
+    if (nfin) {
+      wave_lds_sync();
+      if (lane < nfin) {
+        const uint32_t rr = lane == 0 ? fin_r[0] : lane == 1 ? fin_r[1] : lane == 2 ? fin_r[2] : fin_r[3];
+        float sc = 0.0f;
+        for (uint32_t tt = 0; tt < T; ++tt) sc += Ms[wave][lane][tt];
+        sel_score[(size_t)q * R + rr] = sc;
+      }
+      wave_lds_sync();
     }
   }
 }
