@@ -380,6 +380,187 @@ __global__ __launch_bounds__(256) void k_filter_seg(
 }
 
 // ---------------------------------------------------------------------------
+// The same filter with kFiltGroup consecutive segments of one query per
+// workgroup, for nprobe <= kFiltRounds (every (rank, token) slot resolved by
+// one thread).  The query's probe lists are resolved once (centroid, list
+// start, fixed-point score); per segment the slot's [lo, hi) sub-range comes
+// from seg_off.  Loads run ahead of use: the sub-ranges two segments ahead and
+// the first list entries one segment ahead, so a segment's accumulation and
+// emission overlap the next segments' global reads instead of waiting on a
+// chain of dependent loads per workgroup.  The emission clears the LDS state
+// it reads, ready for the next segment.  Same arithmetic and output layout as
+// k_filter_seg.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kFiltGroup = 8;  // segments per workgroup
+
+__global__ __launch_bounds__(256) void k_filter_grp(
+    const uint32_t *__restrict__ top_ids, const float *__restrict__ top_s, uint32_t t_max,
+    uint32_t P, const uint32_t *__restrict__ q_ntok, const uint32_t *__restrict__ q_np,
+    const uint64_t *__restrict__ ivf_off, const uint32_t *__restrict__ ivf_ids,
+    const uint32_t *__restrict__ seg_off, uint32_t n_seg, uint32_t n_doc, uint32_t n_q,
+    uint32_t *__restrict__ seg_cnt, uint64_t *__restrict__ cand_keys) {
+  __shared__ uint32_t mask[kSeg];
+  __shared__ unsigned long long acc[kSeg];
+  __shared__ uint64_t pstart[2][kFiltRounds * kMaxQTok];
+  __shared__ uint32_t plen[2][kFiltRounds * kMaxQTok];
+  __shared__ long long pfx[kFiltRounds * kMaxQTok];
+  __shared__ uint32_t wsum[4];
+
+  const uint32_t n_grp = (n_seg + kFiltGroup - 1) / kFiltGroup;
+  const uint32_t n_wg = n_grp * n_q;
+  const uint32_t per_xcd = (n_wg + 7) / 8;
+  const uint32_t L = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+  if (L >= n_wg) return;  // uniform for the whole workgroup
+  const uint32_t q = L / n_grp, grp = L - q * n_grp;
+  const uint32_t sg0 = grp * kFiltGroup, sg1 = min(n_seg, sg0 + kFiltGroup);
+  const uint32_t T = q_ntok ? q_ntok[q] : t_max;
+  const uint32_t Pq = q_np ? q_np[q] : P;
+  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
+  const uint32_t pr = tid >> 3, sub = tid & 7u;
+
+  // my slot: rank j = tid / 32, token t = tid % 32
+  const uint32_t *so = nullptr;
+  uint64_t base = 0;
+  bool valid = false;
+  {
+    const uint32_t j = tid >> 5, t = tid & 31u;
+    long long fx = 0;
+    if (j < Pq && t < T) {
+      const size_t o = ((size_t)q * t_max + t) * P + j;
+      const uint32_t c = top_ids[o];
+      if (c != 0xFFFFFFFFu) {
+        valid = true;
+        so = seg_off + (size_t)c * (n_seg + 1);
+        base = ivf_off[c];
+        fx = (long long)((double)top_s[o] * kFxScale);
+      }
+    }
+    pfx[tid] = fx;
+  }
+  uint32_t lo = 0, hi = 0, nlo = 0, nhi = 0;
+  if (valid) {
+    lo = so[sg0];
+    hi = so[sg0 + 1];
+    if (sg0 + 1 < sg1) {
+      nlo = hi;
+      nhi = so[sg0 + 2];
+    }
+  }
+  for (uint32_t i = tid; i < kSeg; i += 256) { mask[i] = 0u; acc[i] = 0ull; }
+  pstart[0][tid] = base + lo;
+  plen[0][tid] = hi - lo;
+  __syncthreads();
+  const uint32_t nr = min(kFiltRounds, Pq);
+  uint32_t ent[kFiltRounds][kFiltHold];
+#pragma unroll
+  for (uint32_t r = 0; r < kFiltRounds; ++r) {
+    const uint32_t slot = r * kMaxQTok + pr;
+    const uint32_t len = (r < nr) ? plen[0][slot] : 0u;
+    const uint64_t st = pstart[0][slot];
+#pragma unroll
+    for (uint32_t k = 0; k < kFiltHold; ++k) {
+      const uint32_t i = sub + 8 * k;
+      ent[r][k] = (i < len) ? ivf_ids[st + i] : 0xFFFFFFFFu;
+    }
+  }
+
+  for (uint32_t sg = sg0; sg < sg1; ++sg) {
+    const uint32_t b = (sg - sg0) & 1u;
+    const uint32_t seg_lo = sg * kSeg;
+    // the next segment's sub-ranges (loaded one segment ago) into the other
+    // buffer; the loads for the segment after it
+    if (sg + 1 < sg1) {
+      pstart[b ^ 1u][tid] = base + nlo;
+      plen[b ^ 1u][tid] = nhi - nlo;
+      if (valid && sg + 2 < sg1) {
+        nlo = nhi;
+        nhi = so[sg + 3];
+      }
+    }
+    __syncthreads();
+    // the next segment's first entries, in flight while this one accumulates
+    uint32_t ent_n[kFiltRounds][kFiltHold];
+#pragma unroll
+    for (uint32_t r = 0; r < kFiltRounds; ++r) {
+      const uint32_t slot = r * kMaxQTok + pr;
+      const uint32_t len = (sg + 1 < sg1 && r < nr) ? plen[b ^ 1u][slot] : 0u;
+      const uint64_t st = pstart[b ^ 1u][slot];
+#pragma unroll
+      for (uint32_t k = 0; k < kFiltHold; ++k) {
+        const uint32_t i = sub + 8 * k;
+        ent_n[r][k] = (i < len) ? ivf_ids[st + i] : 0xFFFFFFFFu;
+      }
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kFiltRounds; ++r) {
+      if (r < nr) {
+        const uint32_t slot = r * kMaxQTok + pr;
+        const uint32_t bit = 1u << pr;
+        const unsigned long long add = (unsigned long long)pfx[slot];
+#pragma unroll
+        for (uint32_t k = 0; k < kFiltHold; ++k) {
+          if (ent[r][k] != 0xFFFFFFFFu) {
+            const uint32_t dl = (ent[r][k] - seg_lo) & (kSeg - 1);
+            const uint32_t old = atomicOr(&mask[dl], bit);
+            if (!(old & bit)) atomicAdd(&acc[dl], add);
+          }
+        }
+        // long runs: the rest of the entries, loaded here
+        const uint32_t len = plen[b][slot];
+        const uint64_t st = pstart[b][slot];
+        for (uint32_t i = sub + 8 * kFiltHold; i < len; i += 8) {
+          const uint32_t dl = (ivf_ids[st + i] - seg_lo) & (kSeg - 1);
+          const uint32_t old = atomicOr(&mask[dl], bit);
+          if (!(old & bit)) atomicAdd(&acc[dl], add);
+        }
+        // no barrier between ranks: a token's runs belong to 8 lanes of one
+        // wave, whose LDS operations execute in program order, so rank r's
+        // bits are set before rank r + 1 tests them
+      }
+    }
+    __syncthreads();
+
+    // emission (as in k_filter_seg), clearing what it reads
+    uint32_t hasbits = 0, myc = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < kSeg / 256; ++k) {
+      const uint32_t dl = lane + kWave * (wv + 4 * k);
+      if (mask[dl] != 0u) {
+        mask[dl] = 0u;
+        if (seg_lo + dl < n_doc) { hasbits |= 1u << k; ++myc; }
+      }
+    }
+    uint32_t wcount = myc;
+#pragma unroll
+    for (uint32_t off = 1; off < kWave; off <<= 1) wcount += __shfl_xor(wcount, off, kWave);
+    if (lane == 0) wsum[wv] = wcount;
+    __syncthreads();
+    uint32_t pos = 0;
+    for (uint32_t w = 0; w < wv; ++w) pos += wsum[w];
+    if (tid == 0) seg_cnt[(size_t)q * n_seg + sg] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    uint64_t *out = cand_keys + ((size_t)q * n_seg + sg) * kSeg;
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+#pragma unroll
+    for (uint32_t k = 0; k < kSeg / 256; ++k) {
+      const bool hit = (hasbits >> k) & 1u;
+      const uint64_t bb = __ballot(hit);
+      const uint32_t dl = lane + kWave * (wv + 4 * k);
+      if (hit) {
+        const float F = (float)((double)(long long)acc[dl] * (1.0 / kFxScale));
+        out[pos + __popcll(bb & lt_mask)] = make_key(F, seg_lo + dl);
+      }
+      acc[dl] = 0ull;
+      pos += __popcll(bb);
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < kFiltRounds; ++r)
+#pragma unroll
+      for (uint32_t k = 0; k < kFiltHold; ++k) ent[r][k] = ent_n[r][k];
+    __syncthreads();  // wsum and the cleared state before the next segment
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Bitonic sort (descending) of n (power of two) 64-bit keys in LDS.
 // ---------------------------------------------------------------------------
 __device__ void bitonic_desc(unsigned long long *s, uint32_t n) {
@@ -2228,6 +2409,13 @@ hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t
   if (n_q == 0) return hipSuccess;
   const uint64_t n_wg = (uint64_t)ix.n_seg * n_q;
   if (n_wg > 0x7FFFFFF8ull) return hipErrorInvalidValue;
+  if (P <= kFiltRounds) {
+    const uint64_t n_grp = ((uint64_t)ix.n_seg + kFiltGroup - 1) / kFiltGroup * n_q;
+    const uint32_t grid = (uint32_t)((n_grp + 7) / 8 * 8);  // the kernel's XCD remap needs a multiple of 8
+    k_filter_grp<<<grid, 256, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np, ix.ivf_off, ix.ivf_ids,
+                                      ix.seg_off, ix.n_seg, ix.n_doc, n_q, seg_cnt, cand_keys);
+    return hipGetLastError();
+  }
   const uint32_t grid = (uint32_t)((n_wg + 7) / 8 * 8);  // the kernel's XCD remap needs a multiple of 8
   k_filter_seg<<<grid, 256, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np, ix.ivf_off,
                                     ix.ivf_ids, ix.seg_off, ix.n_seg, ix.n_doc, n_q, seg_cnt,
