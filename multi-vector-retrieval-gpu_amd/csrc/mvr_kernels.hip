@@ -593,6 +593,8 @@ __device__ __forceinline__ uint32_t pow2_ceil(uint32_t n) {
 constexpr uint32_t kSelThreads = 512;
 constexpr uint32_t kSelScratch = 2048;
 
+constexpr uint32_t kSelMaxSeg = 4096;  // segments indexed in LDS (16.7M docs); more: segment-by-segment walk
+
 __global__ __launch_bounds__(kSelThreads) void k_select_topr(
     const uint64_t *__restrict__ cand_keys, const uint32_t *__restrict__ seg_cnt, uint32_t n_seg,
     const uint32_t *__restrict__ q_r, uint32_t R_out, uint32_t *__restrict__ sel_ids,
@@ -600,16 +602,44 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
   __shared__ unsigned long long outk[kMaxSelect];
   __shared__ unsigned long long scr[kSelScratch];
   __shared__ uint32_t hist[2048];
+  __shared__ uint32_t spfx[kSelMaxSeg + 1];  // exclusive prefix of the segment counts
   __shared__ uint32_t s_out_n, s_scr_n, s_digit, s_gt, s_eq;
-  __shared__ uint32_t s_n;
+  __shared__ uint32_t s_n, s_wsum[kSelThreads / kWave];
 
   const uint32_t q = blockIdx.x, tid = threadIdx.x;
   const uint32_t *cnt = seg_cnt + (size_t)q * n_seg;
   const uint64_t *keys = cand_keys + (size_t)q * n_seg * kSeg;
+  const bool flat = n_seg <= kSelMaxSeg;
   // the query's candidates: cnt[sg] keys at keys + sg * kSeg for every segment
-  if (tid == 0) s_n = 0;
-  __syncthreads();
-  {
+  if (flat) {
+    // prefix sum of the counts: each thread sums a run of segments, then a
+    // scan over the threads' run totals
+    const uint32_t per = (n_seg + kSelThreads - 1) / kSelThreads;
+    const uint32_t s0 = min(n_seg, tid * per), s1 = min(n_seg, s0 + per);
+    uint32_t part = 0;
+    for (uint32_t sg = s0; sg < s1; ++sg) part += cnt[sg];
+    const uint32_t lane = tid & (kWave - 1), wv = tid / kWave;
+    uint32_t incl = part;
+#pragma unroll
+    for (uint32_t o = 1; o < kWave; o <<= 1) {
+      const uint32_t u = __shfl_up(incl, o);
+      if (lane >= o) incl += u;
+    }
+    if (lane == kWave - 1) s_wsum[wv] = incl;
+    __syncthreads();
+    uint32_t run = incl - part;
+    for (uint32_t w = 0; w < wv; ++w) run += s_wsum[w];
+    for (uint32_t sg = s0; sg < s1; ++sg) {
+      spfx[sg] = run;
+      run += cnt[sg];
+    }
+    if (tid == kSelThreads - 1) {
+      spfx[n_seg] = run;
+      s_n = run;
+    }
+  } else {
+    if (tid == 0) s_n = 0;
+    __syncthreads();
     uint32_t part = 0;
     for (uint32_t sg = tid; sg < n_seg; sg += kSelThreads) part += cnt[sg];
     if (part) atomicAdd(&s_n, part);
@@ -617,12 +647,34 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
   __syncthreads();
   const uint32_t n = s_n;
   if (tid == 0 && cand_tot) cand_tot[q] = n;
-  // visit every candidate key (segment by segment)
+  // visit every candidate key.  Indexed: key i of the query's list lies in
+  // the segment sg with spfx[sg] <= i < spfx[sg + 1]; thread tid takes
+  // i = tid, tid + kSelThreads, ... with a segment cursor that only moves
+  // forward, four loads in flight.  Otherwise segment by segment.
   auto for_each_key = [&](auto &&body) {
-    for (uint32_t sg = 0; sg < n_seg; ++sg) {
-      const uint32_t c = cnt[sg];
-      const uint64_t *kk = keys + (size_t)sg * kSeg;
-      for (uint32_t i = tid; i < c; i += kSelThreads) body(kk[i]);
+    if (flat) {
+      constexpr uint32_t U = 4;
+      uint32_t sg = 0;
+      for (uint32_t i0 = tid; i0 < n; i0 += kSelThreads * U) {
+        uint64_t kv[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+          const uint32_t i = i0 + u * kSelThreads;
+          if (i < n) {
+            while (spfx[sg + 1] <= i) ++sg;
+            kv[u] = keys[(size_t)sg * kSeg + (i - spfx[sg])];
+          }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+          if (i0 + u * kSelThreads < n) body(kv[u]);
+      }
+    } else {
+      for (uint32_t sg = 0; sg < n_seg; ++sg) {
+        const uint32_t c = cnt[sg];
+        const uint64_t *kk = keys + (size_t)sg * kSeg;
+        for (uint32_t i = tid; i < c; i += kSelThreads) body(kk[i]);
+      }
     }
   };
   // R_out is the row stride of the outputs; q_r (nullable) lowers R per query
@@ -634,12 +686,20 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
 
   if (n <= kMaxSelect) {
     const uint32_t np = pow2_ceil(max(n, 2u));
-    uint32_t base = 0;
-    for (uint32_t sg = 0; sg < n_seg; ++sg) {
-      const uint32_t c = cnt[sg];
-      const uint64_t *kk = keys + (size_t)sg * kSeg;
-      for (uint32_t i = tid; i < c; i += kSelThreads) outk[base + i] = kk[i];
-      base += c;
+    if (flat) {
+      uint32_t sg = 0;
+      for (uint32_t i = tid; i < n; i += kSelThreads) {
+        while (spfx[sg + 1] <= i) ++sg;
+        outk[i] = keys[(size_t)sg * kSeg + (i - spfx[sg])];
+      }
+    } else {
+      uint32_t base = 0;
+      for (uint32_t sg = 0; sg < n_seg; ++sg) {
+        const uint32_t c = cnt[sg];
+        const uint64_t *kk = keys + (size_t)sg * kSeg;
+        for (uint32_t i = tid; i < c; i += kSelThreads) outk[base + i] = kk[i];
+        base += c;
+      }
     }
     for (uint32_t i = n + tid; i < np; i += kSelThreads) outk[i] = 0ull;
     __syncthreads();
