@@ -313,14 +313,14 @@ def bench_plaid(torch, args, cfg):
     log(f"[bench] plaid setup {time.time() - t0:.1f}s: {int(doclens.sum())} tokens, ncells {ncells}, "
         f"threshold {thr}, ndocs {ndocs}")
     for _ in range(args.warmup):
-        g.search(Q, k=k)
+        g.search(Q, k=k, batch=args.batch)
     torch.cuda.synchronize()
     keys = ("candidates_ms", "filter_ms", "refine_ms", "total_ms", "n_refine_ivf", "n_refine_filter",
             "n_vec_score_refine", "scores_ms", "n_pass")
     acc = {f: 0.0 for f in keys}
     t1 = time.perf_counter()
     for _ in range(args.steps):
-        ids, sc, tm = g.search(Q, k=k, timings=True)
+        ids, sc, tm = g.search(Q, k=k, timings=True, batch=args.batch)
         for f in keys:
             acc[f] += tm[f]
     torch.cuda.synchronize()

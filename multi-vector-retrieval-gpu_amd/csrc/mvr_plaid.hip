@@ -1042,6 +1042,7 @@ using namespace mvr;
 // ---------------------------------------------------------------------------
 struct mvr_plaid_index {
   int device = 0;
+  uint64_t pool_budget = kPoolBudget;  // device bytes for a pass's per-query workspace
   uint32_t n_doc = 0, n_c = 0;
   uint64_t n_tok = 0, n_ivf = 0;
   uint32_t bm_words = 0;
@@ -1318,6 +1319,12 @@ int mvr_plaid_index_create(const mvr_plaid_index_desc *d, int device, mvr_plaid_
   for (auto &ev : ix->ev)
     if ((e = hipEventCreate(&ev)) != hipSuccess)
       return bail(pl_fail(MVR_ERR_DEVICE, "event create failed: %s", hipGetErrorString(e)));
+  {
+    // pass workspace: a quarter of the HBM left free by the index, between 2 and 64 GiB
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
+      ix->pool_budget = std::min<uint64_t>(kPoolBudgetMax, std::max<uint64_t>(kPoolBudgetMin, free_b / 4));
+  }
   *out = ix;
   return MVR_OK;
 }
@@ -1366,7 +1373,7 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
                        8 * (size_t)k + 16;
   uint32_t B = p->batch ? p->batch : 1024;
   B = std::min<uint32_t>(B, n_q);
-  B = (uint32_t)std::max<size_t>(1, std::min<size_t>(B, kPoolBudget / per_q));
+  B = (uint32_t)std::max<size_t>(1, std::min<size_t>(B, ix->pool_budget / per_q));
   if (ix->Q.cap < sizeof(float) * (size_t)B * kMaxQTok * kDim) {
     PL_TRY(ix->Q.ensure(sizeof(float) * (size_t)B * kMaxQTok * kDim));
     PL_TRY(launch_fill_u32(ix->Q.as<uint32_t>(), 0, ix->Q.cap / 4, s));
