@@ -228,11 +228,94 @@ int mvr_device_count(int *count) {
   return MVR_OK;
 }
 
+namespace {
+
+// Codes and residual bytes read from the data sections of vq_code_l.npy and
+// residual_code_l.npy (files open at their first data byte) instead of from
+// a descriptor's arrays.
+struct CodeFiles {
+  FILE *codes = nullptr, *res = nullptr;
+  ~CodeFiles() {
+    if (codes) std::fclose(codes);
+    if (res) std::fclose(res);
+  }
+};
+
+// Bytes per read of the streaming loader (MVR_LOAD_CHUNK_BYTES overrides:
+// at least 4 KiB, rounded down to a multiple of 4 so code chunks hold whole
+// codes).
+size_t load_chunk_bytes() {
+  const char *e = std::getenv("MVR_LOAD_CHUNK_BYTES");
+  const long long v = e ? std::atoll(e) : 0;
+  if (v >= 4096) return (size_t)v & ~(size_t)3;
+  return (size_t)64 << 20;
+}
+
+// codes in [0, n_c) (a code is a centroid row index in every kernel)
+int check_codes(const uint32_t *c, size_t n, uint32_t n_c, uint64_t first) {
+  for (size_t i = 0; i < n; ++i)
+    if (c[i] >= n_c)
+      return fail(MVR_ERR_IO, "vq code %u at %llu >= n_centroid %u", c[i], (unsigned long long)(first + i), n_c);
+  return MVR_OK;
+}
+
+// Streams `bytes` from f into a new device buffer *dst: chunks are read
+// into two pinned buffers in turn and copied with hipMemcpyAsync, so the
+// read of one chunk overlaps the copy of the previous.  n_c > 0: the data are
+// u32 codes, checked per chunk before they are copied.
+int stream_to_device(FILE *f, uint8_t **dst, size_t bytes, uint32_t n_c) {
+  struct Res {
+    void *buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipStream_t st = nullptr;
+    ~Res() {
+      if (st) (void)hipStreamSynchronize(st);
+      for (int i = 0; i < 2; ++i) {
+        if (buf[i]) (void)hipHostFree(buf[i]);
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+      }
+      if (st) (void)hipStreamDestroy(st);
+    }
+  } r;
+  HIP_TRY(hipMalloc(reinterpret_cast<void **>(dst), std::max<size_t>(bytes, 256)));
+  const size_t chunk = std::min(load_chunk_bytes(), std::max<size_t>(bytes, 4));
+  for (int i = 0; i < 2; ++i) {
+    HIP_TRY(hipHostMalloc(&r.buf[i], chunk, hipHostMallocDefault));
+    HIP_TRY(hipEventCreateWithFlags(&r.ev[i], hipEventDisableTiming));
+  }
+  HIP_TRY(hipStreamCreateWithFlags(&r.st, hipStreamNonBlocking));
+  uint64_t i = 0;
+  for (size_t off = 0; off < bytes; off += chunk, ++i) {
+    const int b = (int)(i & 1);
+    if (i >= 2) HIP_TRY(hipEventSynchronize(r.ev[b]));  // its previous copy is done
+    const size_t n = std::min(chunk, bytes - off);
+    if (std::fread(r.buf[b], 1, n, f) != n) return fail(MVR_ERR_IO, "npy data truncated at byte %zu", off);
+    if (n_c) {
+      const int rc = check_codes(static_cast<const uint32_t *>(r.buf[b]), n / 4, n_c, off / 4);
+      if (rc != MVR_OK) return rc;
+    }
+    HIP_TRY(hipMemcpyAsync(*dst + off, r.buf[b], n, hipMemcpyHostToDevice, r.st));
+    HIP_TRY(hipEventRecord(r.ev[b], r.st));
+  }
+  HIP_TRY(hipStreamSynchronize(r.st));
+  return MVR_OK;
+}
+
+int index_create(const mvr_index_desc *desc, int device, uint32_t flags, const CodeFiles *cf, mvr_index **out);
+
+}  // namespace
+
 int mvr_index_create(const mvr_index_desc *desc, int device, mvr_index **out) {
   return mvr_index_create_ex(desc, device, 0u, out);
 }
 
 int mvr_index_create_ex(const mvr_index_desc *desc, int device, uint32_t flags, mvr_index **out) {
+  return index_create(desc, device, flags, nullptr, out);
+}
+
+namespace {
+
+int index_create(const mvr_index_desc *desc, int device, uint32_t flags, const CodeFiles *cf, mvr_index **out) {
   if (!desc || !out) return fail(MVR_ERR_INVALID, "NULL argument");
   *out = nullptr;
   if (flags & ~MVR_INDEX_RESIDUAL_ON_HOST) return fail(MVR_ERR_INVALID, "unknown index flags 0x%x", flags);
@@ -240,7 +323,7 @@ int mvr_index_create_ex(const mvr_index_desc *desc, int device, uint32_t flags, 
   if (desc->n_bit != 2) return fail(MVR_ERR_UNSUPPORTED, "n_bit %u unsupported (kernels need 2)", desc->n_bit);
   if (desc->n_doc == 0 || desc->n_centroid == 0)
     return fail(MVR_ERR_INVALID, "empty index (n_doc %u, n_centroid %u)", desc->n_doc, desc->n_centroid);
-  if (!desc->doclens || !desc->centroids || !desc->codes || !desc->residual || !desc->weights ||
+  if (!desc->doclens || !desc->centroids || (!cf && (!desc->codes || !desc->residual)) || !desc->weights ||
       !desc->ivf_offsets || !desc->ivf_sizes || (!desc->ivf_ids && desc->ivf_n_ele))
     return fail(MVR_ERR_INVALID, "NULL index array");
   int n_dev = 0;
@@ -324,8 +407,16 @@ int mvr_index_create_ex(const mvr_index_desc *desc, int device, uint32_t flags, 
     if (e0 == hipSuccess)
       e0 = hipHostMalloc(reinterpret_cast<void **>(&ix->h_codes), std::max<size_t>(d.n_tok * 4, 256),
                          hipHostMallocMapped);
-    if (e0 == hipSuccess && d.n_tok) e0 = hipMemcpy(ix->h_res, desc->residual, d.n_tok * kResBytes, kind);
-    if (e0 == hipSuccess && d.n_tok) e0 = hipMemcpy(ix->h_codes, desc->codes, d.n_tok * 4, kind);
+    if (cf) {
+      // straight from the files into the pinned arrays
+      if (e0 == hipSuccess && d.n_tok &&
+          (std::fread(ix->h_res, 1, d.n_tok * kResBytes, cf->res) != d.n_tok * kResBytes ||
+           std::fread(ix->h_codes, 4, d.n_tok, cf->codes) != d.n_tok))
+        return bail(fail(MVR_ERR_IO, "npy data truncated"));
+    } else {
+      if (e0 == hipSuccess && d.n_tok) e0 = hipMemcpy(ix->h_res, desc->residual, d.n_tok * kResBytes, kind);
+      if (e0 == hipSuccess && d.n_tok) e0 = hipMemcpy(ix->h_codes, desc->codes, d.n_tok * 4, kind);
+    }
     void *pr = nullptr, *pc = nullptr;
     if (e0 == hipSuccess) e0 = hipHostGetDevicePointer(&pr, ix->h_res, 0);
     if (e0 == hipSuccess) e0 = hipHostGetDevicePointer(&pc, ix->h_codes, 0);
@@ -338,6 +429,10 @@ int mvr_index_create_ex(const mvr_index_desc *desc, int device, uint32_t flags, 
       for (uint64_t i = 0; i < d.n_tok; ++i)
         if (ix->h_codes[i] >= d.n_c) return bail(fail(MVR_ERR_INVALID, "code %u out of range", ix->h_codes[i]));
     }
+  } else if (cf) {
+    int rc = stream_to_device(cf->codes, reinterpret_cast<uint8_t **>(&d.codes), d.n_tok * 4, d.n_c);
+    if (rc == MVR_OK) rc = stream_to_device(cf->res, &d.residual, d.n_tok * kResBytes, 0);
+    if (rc != MVR_OK) return bail(rc);
   } else {
     UP(d.codes, desc->codes, d.n_tok);
     UP(d.residual, desc->residual, d.n_tok * kResBytes);
@@ -390,6 +485,8 @@ int mvr_index_create_ex(const mvr_index_desc *desc, int device, uint32_t flags, 
   return MVR_OK;
 }
 
+}  // namespace
+
 int mvr_index_load(const char *index_dir, int device, mvr_index **out) {
   return mvr_index_load_ex(index_dir, device, 0u, out);
 }
@@ -399,10 +496,14 @@ int mvr_index_load_ex(const char *index_dir, int device, uint32_t flags, mvr_ind
   const std::string dir(index_dir);
   NpyArray cent, codes, weight, res, lens;
   std::string err;
+  // vq_code_l.npy and residual_code_l.npy: headers only here; their data are
+  // streamed by the index creation (never held whole in host memory, unless
+  // the residuals are to stay on the host)
+  CodeFiles cf;
   if (!(err = npy_load(dir + "/centroid_l.npy", cent)).empty()) return fail(MVR_ERR_IO, "%s", err.c_str());
-  if (!(err = npy_load(dir + "/vq_code_l.npy", codes)).empty()) return fail(MVR_ERR_IO, "%s", err.c_str());
+  if (!(err = npy_open(dir + "/vq_code_l.npy", codes, &cf.codes)).empty()) return fail(MVR_ERR_IO, "%s", err.c_str());
   if (!(err = npy_load(dir + "/weight_l.npy", weight)).empty()) return fail(MVR_ERR_IO, "%s", err.c_str());
-  if (!(err = npy_load(dir + "/residual_code_l.npy", res)).empty()) return fail(MVR_ERR_IO, "%s", err.c_str());
+  if (!(err = npy_open(dir + "/residual_code_l.npy", res, &cf.res)).empty()) return fail(MVR_ERR_IO, "%s", err.c_str());
   if (!(err = npy_load(dir + "/item_n_vec_l.npy", lens)).empty()) return fail(MVR_ERR_IO, "%s", err.c_str());
   if (cent.word_size != 4 || cent.descr[1] != 'f' || cent.shape.size() != 2)
     return fail(MVR_ERR_IO, "centroid_l.npy must be float32 [n_centroid, dim]");
@@ -419,9 +520,6 @@ int mvr_index_load_ex(const char *index_dir, int device, uint32_t flags, mvr_ind
   if (res.shape[0] != n_tok * (dim / 4))
     return fail(MVR_ERR_IO, "residual_code_l.npy has %zu bytes, expected %llu", res.shape[0],
                 (unsigned long long)(n_tok * (dim / 4)));
-  const uint32_t *cd = codes.as<uint32_t>();
-  for (uint64_t i = 0; i < n_tok; ++i)
-    if (cd[i] >= n_c) return fail(MVR_ERR_IO, "vq code %u at %llu >= n_centroid %u", cd[i], (unsigned long long)i, n_c);
 
   // ivf.index: u32 n_centroid, u64 n_ele, u64[n_c] offsets, u32[n_c] sizes,
   // u32[n_ele] ids (src/include/struct/IVFIndex.hpp:168-183)
@@ -447,8 +545,8 @@ int mvr_index_load_ex(const char *index_dir, int device, uint32_t flags, mvr_ind
   desc.n_bit = 2;
   desc.doclens = lens.as<uint32_t>();
   desc.centroids = cent.as<float>();
-  desc.codes = cd;
-  desc.residual = res.as<uint8_t>();
+  desc.codes = nullptr;  // from cf
+  desc.residual = nullptr;
   desc.weights = weight.as<float>();
   desc.weights_per_dim = weight.shape.size() == 2 ? 1u : 0u;
   if (desc.weights_per_dim ? (weight.shape[0] != dim || weight.shape[1] != 4) : weight.shape[0] != 4)
@@ -458,7 +556,8 @@ int mvr_index_load_ex(const char *index_dir, int device, uint32_t flags, mvr_ind
   desc.ivf_ids = ids.data();
   desc.ivf_n_ele = n_ele;
   desc.on_device = 0;
-  return mvr_index_create_ex(&desc, device, flags, out);
+  if (flags & ~MVR_INDEX_RESIDUAL_ON_HOST) return fail(MVR_ERR_INVALID, "unknown index flags 0x%x", flags);
+  return index_create(&desc, device, flags, &cf, out);
 }
 
 void mvr_index_free(mvr_index *ix) {

@@ -24,8 +24,11 @@ struct NpyArray {
   template <class T> const T *as() const { return reinterpret_cast<const T *>(data.data()); }
 };
 
-// Returns an empty string on success, else an error message.
-inline std::string npy_load(const std::string &path, NpyArray &out) {
+// Parses the header of path into out (no data read) and leaves *fp open at
+// the first data byte.  Returns an empty string on success, else an error
+// message (and *fp closed).
+inline std::string npy_open(const std::string &path, NpyArray &out, FILE **fp) {
+  *fp = nullptr;
   FILE *f = std::fopen(path.c_str(), "rb");
   if (!f) return "cannot open " + path;
   unsigned char magic[8];
@@ -79,6 +82,15 @@ inline std::string npy_load(const std::string &path, NpyArray &out) {
     return "unsupported dtype " + out.descr + ": " + path;
   }
   out.word_size = std::stoul(out.descr.substr(2));
+  *fp = f;
+  return "";
+}
+
+// Returns an empty string on success, else an error message.
+inline std::string npy_load(const std::string &path, NpyArray &out) {
+  FILE *f = nullptr;
+  std::string err = npy_open(path, out, &f);
+  if (!err.empty()) return err;
   const size_t bytes = out.numel() * out.word_size;
   out.data.resize(bytes);
   if (bytes && std::fread(out.data.data(), 1, bytes, f) != bytes) {
