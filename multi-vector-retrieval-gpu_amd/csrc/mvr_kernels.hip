@@ -1974,11 +1974,12 @@ __global__ __launch_bounds__(256) void k_rerank_persist(
 // so pass 2 only rescans those tiles.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kBmWaves = 8;
+constexpr uint32_t kBmStage = 8;  // tiles per LDS stage
 
 __global__ __launch_bounds__(512) void k_ct_bf16_blockmax(
     const float *__restrict__ Q, uint32_t t_max, uint32_t n_q, const uint16_t *__restrict__ Cb,
     uint32_t n_c, uint32_t n_tiles, uint32_t tiles_per_split, float *__restrict__ bmax) {
-  __shared__ uint4 As[32][17];
+  __shared__ uint4 As[kBmStage][32][17];
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const uint32_t h = lane >> 5, col = lane & 31;
   const uint32_t q = blockIdx.x * kBmWaves + wave;
@@ -1998,25 +1999,38 @@ __global__ __launch_bounds__(512) void k_ct_bf16_blockmax(
   const uint32_t tb = blockIdx.y * tiles_per_split;
   const uint32_t te = min(n_tiles, tb + tiles_per_split);
   const uint4 *C4 = reinterpret_cast<const uint4 *>(Cb);
-  // maxima of 8 consecutive tiles are kept in registers and stored together,
-  // so each row's run of tile maxima is written as whole 32-byte pieces
-  for (uint32_t t0 = tb; t0 < te; t0 += 8) {
-    float mxb[8];
+  // kBmStage tiles share one LDS stage (two barriers per stage, not per
+  // tile); the next stage's rows are loaded into registers while the waves
+  // run the current stage's MFMAs.  Thread tid stages 16-byte piece tid & 15
+  // of row tid >> 4 of every tile of the stage.
+  const uint32_t sr = tid >> 4, sk = tid & 15;
+  uint4 pf[kBmStage];
+  auto load_stage = [&](uint32_t t0) {
 #pragma unroll
-    for (uint32_t u = 0; u < 8; ++u) {
+    for (uint32_t u = 0; u < kBmStage; ++u) {
+      const uint32_t tile = t0 + u, c = tile * 32 + sr;
+      pf[u] = (tile < te && c < n_c) ? C4[(size_t)c * 16 + sk] : make_uint4(0u, 0u, 0u, 0u);
+    }
+  };
+  load_stage(tb);
+  // maxima of the stage's tiles are kept in registers and stored together,
+  // so each row's run of tile maxima is written as whole 32-byte pieces
+  for (uint32_t t0 = tb; t0 < te; t0 += kBmStage) {
+    __syncthreads();  // the previous stage's readers are done
+#pragma unroll
+    for (uint32_t u = 0; u < kBmStage; ++u) As[u][sr][sk] = pf[u];
+    __syncthreads();
+    if (t0 + kBmStage < te) load_stage(t0 + kBmStage);
+    float mxb[kBmStage];
+#pragma unroll
+    for (uint32_t u = 0; u < kBmStage; ++u) {
       const uint32_t tile = t0 + u;
       mxb[u] = -INFINITY;
       if (tile < te) {  // uniform across the workgroup
-        __syncthreads();
-        {
-          const uint32_t r = tid >> 4, k = tid & 15, c = tile * 32 + r;
-          As[r][k] = c < n_c ? C4[(size_t)c * 16 + k] : make_uint4(0u, 0u, 0u, 0u);
-        }
-        __syncthreads();
         floatx16_t acc = {};
 #pragma unroll
         for (uint32_t s = 0; s < 8; ++s)
-          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(As[col][8 * h + s]), qf[s], acc, 0, 0, 0);
+          acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(As[u][col][8 * h + s]), qf[s], acc, 0, 0, 0);
         float mx = -INFINITY;
 #pragma unroll
         for (uint32_t j = 0; j < 16; ++j) {
@@ -2029,7 +2043,7 @@ __global__ __launch_bounds__(512) void k_ct_bf16_blockmax(
     if (h == 0 && vq && col < t_max) {
       float *row = bmax + ((size_t)q * t_max + col) * n_tiles;
 #pragma unroll
-      for (uint32_t u = 0; u < 8; ++u)
+      for (uint32_t u = 0; u < kBmStage; ++u)
         if (t0 + u < te) row[t0 + u] = mxb[u];
     }
   }
