@@ -2090,6 +2090,7 @@ __global__ __launch_bounds__(256) void k_ct_select(
     const float *__restrict__ bmax, uint32_t P, float margin, float c_norm_max,
     uint32_t *__restrict__ top_ids, float *__restrict__ top_s) {
   __shared__ float qs[4][kDim];
+  __shared__ float qsb[4][kDim];  // bf16-rounded query (FP32 mode's pre-check)
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const uint32_t row = blockIdx.x * 4 + wave;
   if (row >= n_rows) return;  // whole wave leaves; no block barrier below
@@ -2098,6 +2099,7 @@ __global__ __launch_bounds__(256) void k_ct_select(
   for (uint32_t d = lane; d < kDim; d += 64) {
     const float v = kF32 ? Q[qrow * kDim + d] : (float)(__bf16)Q[qrow * kDim + d];
     qs[wave][d] = v;
+    qsb[wave][d] = (float)(__bf16)v;
     nn += v * v;
   }
 #pragma unroll
@@ -2130,7 +2132,28 @@ __global__ __launch_bounds__(256) void k_ct_select(
       const uint32_t c = tile * 32 + (lane & 31);
       const bool v = tile != 0xFFFFFFFFu && c < n_c;
       float a = 0.0f;
-      if (v) {
+      bool keep = v;
+      if (kF32 && v && P1 == P) {
+        // bf16 pre-check from the 256-byte bf16 row: this approximation is
+        // within the same eps of the exact score as the tile maxima, so a
+        // centroid of the exact top-P has it >= thr (exact >= tau >= A_P -
+        // eps, which needs P tiles, i.e. P <= n_tiles); only those read the
+        // 512-byte FP32 row
+        const uint4 *cb = reinterpret_cast<const uint4 *>(Cb + (size_t)c * kDim);
+        float ap = 0.0f;
+#pragma unroll 4
+        for (uint32_t k2 = 0; k2 < 16; ++k2) {
+          const uint4 w = cb[k2];
+          const uint32_t ww[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+          for (uint32_t e = 0; e < 4; ++e) {
+            ap = __builtin_fmaf(qsb[wave][8 * k2 + 2 * e], __uint_as_float(ww[e] << 16), ap);
+            ap = __builtin_fmaf(qsb[wave][8 * k2 + 2 * e + 1], __uint_as_float(ww[e] & 0xFFFF0000u), ap);
+          }
+        }
+        keep = ap >= thr;
+      }
+      if (keep) {
         if (kF32) {
           const float4 *cr = reinterpret_cast<const float4 *>(C + (size_t)c * kDim);
 #pragma unroll 8
@@ -2155,7 +2178,7 @@ __global__ __launch_bounds__(256) void k_ct_select(
           }
         }
       }
-      wave_list_insert(fs, fi, lane, P, a + 1.0f, c, v);
+      wave_list_insert(fs, fi, lane, P, a + 1.0f, c, keep);
     }
   }
   if (lane < P) {
