@@ -391,9 +391,11 @@ __global__ __launch_bounds__(256) void k_filter_seg(
 // it reads, ready for the next segment.  Same arithmetic and output layout as
 // k_filter_seg.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kFiltGroup = 8;  // segments per workgroup
+constexpr uint32_t kFiltGroup = 8;     // segments per workgroup
+constexpr uint32_t kFgThreads = 512;   // 16 lanes per query token, 8 waves
+constexpr uint32_t kFgLanes = kFgThreads / kMaxQTok;
 
-__global__ __launch_bounds__(256) void k_filter_grp(
+__global__ __launch_bounds__(kFgThreads) void k_filter_grp(
     const uint32_t *__restrict__ top_ids, const float *__restrict__ top_s, uint32_t t_max,
     uint32_t P, const uint32_t *__restrict__ q_ntok, const uint32_t *__restrict__ q_np,
     const uint64_t *__restrict__ ivf_off, const uint32_t *__restrict__ ivf_ids,
@@ -404,7 +406,7 @@ __global__ __launch_bounds__(256) void k_filter_grp(
   __shared__ uint64_t pstart[2][kFiltRounds * kMaxQTok];
   __shared__ uint32_t plen[2][kFiltRounds * kMaxQTok];
   __shared__ long long pfx[kFiltRounds * kMaxQTok];
-  __shared__ uint32_t wsum[4];
+  __shared__ uint32_t wsum[kFgThreads / kWave];
 
   const uint32_t n_grp = (n_seg + kFiltGroup - 1) / kFiltGroup;
   const uint32_t n_wg = n_grp * n_q;
@@ -416,16 +418,18 @@ __global__ __launch_bounds__(256) void k_filter_grp(
   const uint32_t T = q_ntok ? q_ntok[q] : t_max;
   const uint32_t Pq = q_np ? q_np[q] : P;
   const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
-  const uint32_t pr = tid >> 3, sub = tid & 7u;
+  // a token's runs: kFgLanes consecutive lanes of one wave, entry i to lane i % kFgLanes
+  const uint32_t pr = tid / kFgLanes, sub = tid % kFgLanes;
+  const bool resolver = tid < kFiltRounds * kMaxQTok;
 
-  // my slot: rank j = tid / 32, token t = tid % 32
+  // my slot (threads below 256): rank j = tid / 32, token t = tid % 32
   const uint32_t *so = nullptr;
   uint64_t base = 0;
   bool valid = false;
   {
     const uint32_t j = tid >> 5, t = tid & 31u;
     long long fx = 0;
-    if (j < Pq && t < T) {
+    if (resolver && j < Pq && t < T) {
       const size_t o = ((size_t)q * t_max + t) * P + j;
       const uint32_t c = top_ids[o];
       if (c != 0xFFFFFFFFu) {
@@ -435,7 +439,7 @@ __global__ __launch_bounds__(256) void k_filter_grp(
         fx = (long long)((double)top_s[o] * kFxScale);
       }
     }
-    pfx[tid] = fx;
+    if (resolver) pfx[tid] = fx;
   }
   uint32_t lo = 0, hi = 0, nlo = 0, nhi = 0;
   if (valid) {
@@ -446,9 +450,11 @@ __global__ __launch_bounds__(256) void k_filter_grp(
       nhi = so[sg0 + 2];
     }
   }
-  for (uint32_t i = tid; i < kSeg; i += 256) { mask[i] = 0u; acc[i] = 0ull; }
-  pstart[0][tid] = base + lo;
-  plen[0][tid] = hi - lo;
+  for (uint32_t i = tid; i < kSeg; i += kFgThreads) { mask[i] = 0u; acc[i] = 0ull; }
+  if (resolver) {
+    pstart[0][tid] = base + lo;
+    plen[0][tid] = hi - lo;
+  }
   __syncthreads();
   const uint32_t nr = min(kFiltRounds, Pq);
   uint32_t ent[kFiltRounds][kFiltHold];
@@ -459,7 +465,7 @@ __global__ __launch_bounds__(256) void k_filter_grp(
     const uint64_t st = pstart[0][slot];
 #pragma unroll
     for (uint32_t k = 0; k < kFiltHold; ++k) {
-      const uint32_t i = sub + 8 * k;
+      const uint32_t i = sub + kFgLanes * k;
       ent[r][k] = (i < len) ? ivf_ids[st + i] : 0xFFFFFFFFu;
     }
   }
@@ -469,7 +475,7 @@ __global__ __launch_bounds__(256) void k_filter_grp(
     const uint32_t seg_lo = sg * kSeg;
     // the next segment's sub-ranges (loaded one segment ago) into the other
     // buffer; the loads for the segment after it
-    if (sg + 1 < sg1) {
+    if (resolver && sg + 1 < sg1) {
       pstart[b ^ 1u][tid] = base + nlo;
       plen[b ^ 1u][tid] = nhi - nlo;
       if (valid && sg + 2 < sg1) {
@@ -487,7 +493,7 @@ __global__ __launch_bounds__(256) void k_filter_grp(
       const uint64_t st = pstart[b ^ 1u][slot];
 #pragma unroll
       for (uint32_t k = 0; k < kFiltHold; ++k) {
-        const uint32_t i = sub + 8 * k;
+        const uint32_t i = sub + kFgLanes * k;
         ent_n[r][k] = (i < len) ? ivf_ids[st + i] : 0xFFFFFFFFu;
       }
     }
@@ -508,14 +514,14 @@ __global__ __launch_bounds__(256) void k_filter_grp(
         // long runs: the rest of the entries, loaded here
         const uint32_t len = plen[b][slot];
         const uint64_t st = pstart[b][slot];
-        for (uint32_t i = sub + 8 * kFiltHold; i < len; i += 8) {
+        for (uint32_t i = sub + kFgLanes * kFiltHold; i < len; i += kFgLanes) {
           const uint32_t dl = (ivf_ids[st + i] - seg_lo) & (kSeg - 1);
           const uint32_t old = atomicOr(&mask[dl], bit);
           if (!(old & bit)) atomicAdd(&acc[dl], add);
         }
-        // no barrier between ranks: a token's runs belong to 8 lanes of one
-        // wave, whose LDS operations execute in program order, so rank r's
-        // bits are set before rank r + 1 tests them
+        // no barrier between ranks: a token's runs belong to kFgLanes lanes of
+        // one wave, whose LDS operations execute in program order, so rank
+        // r's bits are set before rank r + 1 tests them
       }
     }
     __syncthreads();
@@ -523,8 +529,8 @@ __global__ __launch_bounds__(256) void k_filter_grp(
     // emission (as in k_filter_seg), clearing what it reads
     uint32_t hasbits = 0, myc = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < kSeg / 256; ++k) {
-      const uint32_t dl = lane + kWave * (wv + 4 * k);
+    for (uint32_t k = 0; k < kSeg / kFgThreads; ++k) {
+      const uint32_t dl = lane + kWave * (wv + (kFgThreads / kWave) * k);
       if (mask[dl] != 0u) {
         mask[dl] = 0u;
         if (seg_lo + dl < n_doc) { hasbits |= 1u << k; ++myc; }
@@ -537,14 +543,18 @@ __global__ __launch_bounds__(256) void k_filter_grp(
     __syncthreads();
     uint32_t pos = 0;
     for (uint32_t w = 0; w < wv; ++w) pos += wsum[w];
-    if (tid == 0) seg_cnt[(size_t)q * n_seg + sg] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+    if (tid == 0) {
+      uint32_t tot = 0;
+      for (uint32_t w = 0; w < kFgThreads / kWave; ++w) tot += wsum[w];
+      seg_cnt[(size_t)q * n_seg + sg] = tot;
+    }
     uint64_t *out = cand_keys + ((size_t)q * n_seg + sg) * kSeg;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
 #pragma unroll
-    for (uint32_t k = 0; k < kSeg / 256; ++k) {
+    for (uint32_t k = 0; k < kSeg / kFgThreads; ++k) {
       const bool hit = (hasbits >> k) & 1u;
       const uint64_t bb = __ballot(hit);
-      const uint32_t dl = lane + kWave * (wv + 4 * k);
+      const uint32_t dl = lane + kWave * (wv + (kFgThreads / kWave) * k);
       if (hit) {
         const float F = (float)((double)(long long)acc[dl] * (1.0 / kFxScale));
         out[pos + __popcll(bb & lt_mask)] = make_key(F, seg_lo + dl);
@@ -2509,7 +2519,7 @@ hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t
   if (P <= kFiltRounds) {
     const uint64_t n_grp = ((uint64_t)ix.n_seg + kFiltGroup - 1) / kFiltGroup * n_q;
     const uint32_t grid = (uint32_t)((n_grp + 7) / 8 * 8);  // the kernel's XCD remap needs a multiple of 8
-    k_filter_grp<<<grid, 256, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np, ix.ivf_off, ix.ivf_ids,
+    k_filter_grp<<<grid, kFgThreads, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np, ix.ivf_off, ix.ivf_ids,
                                       ix.seg_off, ix.n_seg, ix.n_doc, n_q, seg_cnt, cand_keys);
     return hipGetLastError();
   }
