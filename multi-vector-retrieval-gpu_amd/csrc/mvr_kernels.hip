@@ -660,10 +660,10 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
   // visit every candidate key.  Indexed: key i of the query's list lies in
   // the segment sg with spfx[sg] <= i < spfx[sg + 1]; thread tid takes
   // i = tid, tid + kSelThreads, ... with a segment cursor that only moves
-  // forward, four loads in flight.  Otherwise segment by segment.
+  // forward, eight loads in flight.  Otherwise segment by segment.
   auto for_each_key = [&](auto &&body) {
     if (flat) {
-      constexpr uint32_t U = 4;
+      constexpr uint32_t U = 8;
       uint32_t sg = 0;
       for (uint32_t i0 = tid; i0 < n; i0 += kSelThreads * U) {
         uint64_t kv[U];
