@@ -27,6 +27,7 @@
 #include "mvr_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cfloat>
 #include <chrono>
 #include <cmath>
@@ -543,6 +544,74 @@ __global__ __launch_bounds__(1024) void k_pl_compact(uint32_t *__restrict__ pid_
   if (tid == 0) cand_n[q] = min(total, cand_cap);
 }
 
+// k_pl_mark + k_pl_compact in one workgroup per query when the pid bitmap
+// fits in LDS: the IVF lists of the query's cells are or'ed into an LDS
+// bitmap (list entries spread over the 1024 threads by a prefix of the list
+// lengths, a forward list cursor per thread, eight loads in flight), then the
+// bitmap is compacted to the same descending pid list.  No global bitmap.
+constexpr uint32_t kMcCells = kMaxQTok * 8;  // cells per query: nq x ncells (ncells <= 8)
+
+__global__ __launch_bounds__(1024) void k_pl_markc(const uint32_t *__restrict__ cells, uint32_t cell_cap,
+                                                   const uint32_t *__restrict__ cell_n,
+                                                   const int32_t *__restrict__ ivf,
+                                                   const int64_t *__restrict__ ivf_off,
+                                                   const int64_t *__restrict__ ivf_len, uint32_t bm_words,
+                                                   uint32_t *__restrict__ cand, uint32_t cand_cap,
+                                                   uint32_t *__restrict__ cand_n) {
+  extern __shared__ uint32_t bm[];  // [bm_words]
+  __shared__ uint32_t cpfx[kMcCells + 1];
+  __shared__ int64_t coff[kMcCells];
+  __shared__ uint32_t warp_sums[16];
+  const uint32_t tid = threadIdx.x, q = blockIdx.x;
+  const uint32_t n = min(cell_n[q], kMcCells);
+  for (uint32_t i = tid; i < bm_words; i += 1024) bm[i] = 0u;
+  uint32_t len = 0;
+  if (tid < n) {
+    const uint32_t c = cells[(size_t)q * cell_cap + tid];
+    len = (uint32_t)ivf_len[c];
+    coff[tid] = ivf_off[c];
+  }
+  uint32_t total = 0;
+  const uint32_t ex = pl_block_scan(len, warp_sums, &total);
+  if (tid < n) cpfx[tid] = ex;
+  if (tid == 0) cpfx[n] = total;
+  __syncthreads();
+  constexpr uint32_t U = 8;
+  uint32_t ci = 0;
+  for (uint32_t e0 = tid; e0 < total; e0 += 1024 * U) {
+    uint32_t pid[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t e = e0 + u * 1024;
+      if (e < total) {
+        while (cpfx[ci + 1] <= e) ++ci;
+        pid[u] = (uint32_t)ivf[coff[ci] + (e - cpfx[ci])];
+      }
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u)
+      if (e0 + u * 1024 < total) atomicOr(&bm[pid[u] >> 5], 1u << (pid[u] & 31));
+  }
+  __syncthreads();
+  uint32_t *cq = cand + (size_t)q * cand_cap;
+  uint32_t cnt = 0;
+  for (uint32_t base = 0; base < bm_words; base += 1024) {
+    const uint32_t k = base + tid;
+    const uint32_t w = bm_words - 1 - k;  // descending words
+    uint32_t word = k < bm_words ? bm[w] : 0u;
+    uint32_t sum;
+    uint32_t pos = cnt + pl_block_scan((uint32_t)__popc(word), warp_sums, &sum);
+    while (word) {
+      const uint32_t b = 31u - (uint32_t)__clz(word);
+      word &= ~(1u << b);
+      if (pos < cand_cap) cq[pos] = w * 32 + b;
+      ++pos;
+    }
+    cnt += sum;
+  }
+  if (tid == 0) cand_n[q] = min(cnt, cand_cap);
+}
+
 // seg_cnt[q][sg] = number of keys in segment sg when the query has n keys
 // laid out flat (segment sg = keys [sg*kSeg, (sg+1)*kSeg)).
 __global__ void k_pl_segcnt(const uint32_t *__restrict__ n_in, uint32_t n_fixed, uint32_t n_seg,
@@ -1043,6 +1112,7 @@ using namespace mvr;
 struct mvr_plaid_index {
   int device = 0;
   uint64_t pool_budget = kPoolBudget;  // device bytes for a pass's per-query workspace
+  bool markc = false;  // the pid bitmap fits one workgroup's LDS: k_pl_markc
   uint32_t n_doc = 0, n_c = 0;
   uint64_t n_tok = 0, n_ivf = 0;
   uint32_t bm_words = 0;
@@ -1320,6 +1390,22 @@ int mvr_plaid_index_create(const mvr_plaid_index_desc *d, int device, mvr_plaid_
     if ((e = hipEventCreate(&ev)) != hipSuccess)
       return bail(pl_fail(MVR_ERR_DEVICE, "event create failed: %s", hipGetErrorString(e)));
   {
+    // fused candidate marking when the bitmap (plus the kernel's static
+    // LDS) fits the device's per-workgroup LDS
+    // (MVR_PLAID_FUSED_MARK=0 keeps the global-bitmap kernels)
+    const char *fm = std::getenv("MVR_PLAID_FUSED_MARK");
+    int max_lds = 0;
+    const size_t dyn = sizeof(uint32_t) * (size_t)ix->bm_words;
+    const size_t stat = sizeof(uint32_t) * (kMcCells + 1) + sizeof(int64_t) * kMcCells + 64 + 256;
+    if (!(fm && fm[0] == '0') &&
+        hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess &&
+        dyn + stat <= (size_t)max_lds &&
+        hipFuncSetAttribute(reinterpret_cast<const void *>(k_pl_markc), hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)dyn) == hipSuccess)
+      ix->markc = true;
+    (void)hipGetLastError();
+  }
+  {
     // pass workspace: a quarter of the HBM left free by the index, between 2 and 64 GiB
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
@@ -1471,7 +1557,12 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
                                                                    ix->cell_n.as<uint32_t>());
     PL_TRY(hipGetLastError());
     PL_TRY(launch_fill_u32(ix->nvec.as<uint32_t>(), 0, 2 * (size_t)b, s));
-    {
+    if (ix->markc && cell_cap <= kMcCells) {
+      k_pl_markc<<<b, 1024, sizeof(uint32_t) * (size_t)ix->bm_words, s>>>(
+          ix->cells.as<uint32_t>(), cell_cap, ix->cell_n.as<uint32_t>(), ix->ivf, ix->ivf_off, ix->ivf_len,
+          ix->bm_words, ix->cand.as<uint32_t>(), n_doc, ix->cand_n.as<uint32_t>());
+      PL_TRY(hipGetLastError());
+    } else {
       const uint32_t gx = std::max(1u, std::min(16u, 2048u / b));
       k_pl_mark<<<dim3(gx, b), 256, 0, s>>>(ix->cells.as<uint32_t>(), cell_cap, ix->cell_n.as<uint32_t>(), ix->ivf,
                                             ix->ivf_off, ix->ivf_len, ix->pid_bm.as<uint32_t>(), ix->bm_words);

@@ -233,3 +233,29 @@ def test_rank_single_query_matches_search(fx, gpix):
     ids, sc = gpix.search(Q, k=10)
     assert pids == ids[0].tolist() and scores == sc[0].tolist()
     assert len(pids) == 10 and scores == sorted(scores, reverse=True)
+
+
+def test_candidate_marking_paths_agree(fx, opix, monkeypatch):
+    """Candidate generation in one workgroup per query with the pid bitmap in
+    LDS (used whenever it fits) and the global-bitmap kernels
+    (MVR_PLAID_FUSED_MARK=0, the path for corpora whose bitmap does not fit)
+    give the oracle's answers, ncells 1 to 8."""
+    from mvr import plaid
+    from oracle import plaid as P
+    Q = fx["queries"]
+    idx = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("MVR_PLAID_FUSED_MARK", mode)
+        idx[mode] = plaid.PlaidIndex(fx["doclens"], fx["codes"], fx["residuals"],
+                                     fx["centroids"].astype(np.float32), fx["bucket_weights"].astype(np.float32),
+                                     fx["ivf"], fx["ivf_lengths"])
+    for ncells, thr, ndocs in ((1, 0.5, 256), (8, 0.3, 1024)):
+        rid, rsc, rst = opix.search(Q, 10, ncells, thr, ndocs)
+        for mode, g in idx.items():
+            ids, sc, st = g.search(Q, k=10, ncells=ncells, centroid_score_threshold=thr, ndocs=ndocs,
+                                   return_stats=True)
+            np.testing.assert_array_equal(ids, rid, err_msg=mode)
+            np.testing.assert_array_equal(sc, rsc, err_msg=mode)
+            np.testing.assert_array_equal(st, rst, err_msg=mode)
+    for g in idx.values():
+        g.close()
