@@ -123,3 +123,23 @@ def test_load_ex_residual_on_host(tmp_path, small):
     oi, os_ = ix.search(Q, 4, 64, 10)
     np.testing.assert_array_equal(ids, oi)
     np.testing.assert_array_equal(sc, os_)
+
+
+def test_host_residual_with_learned_probing(c1):
+    """The probe model's per-query nprobe and rerank depth are decided in the
+    pass's front half; with host-resident residuals (pipelined passes) the
+    answers equal the device-resident index's."""
+    from mvr.probe import ProbeModel
+    corpus, ix, Q = c1
+    dev, host = _index(ix, False), _index(ix, True)
+    models = [ProbeModel(seed=6), ProbeModel(seed=6)]  # attached models must outlive the searches
+    for m, g in zip(models, (dev, host)):
+        m.attach(g, conf=0.6, fallback=16, r_min=32)
+    kw = dict(precision="fp32_mfma", learned=True, batch=16)
+    ref_ids, ref_sc = dev.search(Q, 4, 256, 10, **kw)
+    ids, sc = host.search(Q, 4, 256, 10, **kw)
+    np.testing.assert_array_equal(ids, ref_ids)
+    np.testing.assert_array_equal(sc, ref_sc)
+    dev.close()
+    host.close()
+    del models
