@@ -3,7 +3,7 @@
 // Stage map (reference file:line of the computation each kernel performs):
 //   k_centroid_topk_f32  S'[t][c] = Q_t.C_c + 1 and per-row top-P
 //                        (GenerateVQScoreCandidatePP.hpp:137-174)
-//   k_filter_seg         IVF probe, per-(doc, token) max and per-doc sum
+//   k_filter_tok         IVF probe, per-(doc, token) max and per-doc sum
 //                        (FilterPP.hpp:99-142, 366-379)
 //   k_select_topr        top-probe_topk by filter score (FilterPP.hpp:386-394)
 //   k_rerank_f32         fused 2-bit decompress + MaxSim + sum over tokens
@@ -226,187 +226,65 @@ __global__ __launch_bounds__(256) void k_topk_merge(const uint32_t *__restrict__
 }
 
 // ---------------------------------------------------------------------------
-// Filter.  Workgroup (segment, query) owns docs [seg*4096, seg*4096+4096) in
-// LDS: a 32-bit mask of query tokens already counted and an int64 fixed-point
-// accumulator.  Probe ranks are applied in order j = 0..P-1 with a barrier in
-// between; because each token's probe list is sorted by score, the first rank
-// whose IVF list holds the doc carries that token's maximum score, so the
-// atomicOr on the mask decides which hit adds (max over probes, then an exact
-// sum over tokens).  Each list's sub-range inside the segment comes from the
-// precomputed seg_off table.
+// Filter.  A workgroup owns one query and kFiltGroup consecutive 4,096-doc
+// segments; per segment it keeps, in LDS, a 32-bit mask of the query tokens
+// already counted per doc and an int64 fixed-point accumulator.  Probe ranks
+// are applied in order j = 0..P-1 for each token; because each token's probe
+// list is sorted by score, the first rank whose IVF list holds the doc carries
+// that token's maximum score, so the atomicOr on the mask decides which hit
+// adds (max over probes, then an exact sum over tokens).  Each list's
+// sub-range inside a segment comes from the precomputed seg_off table.
 //
-// Work split: up to 8 ranks x 32 token slots are resolved at once (one
-// thread per (rank, token): centroid id, sub-range, fixed-point score).  In a
-// rank, 8 threads share a token's run, thread `sub` taking entries sub,
-// sub + 8, ...; the first two entries of every run of the chunk are loaded
-// into registers before any rank is applied, so a workgroup waits on global
-// memory about three times per chunk, and no thread searches for its run.
+// Token-major work split: the 16 lanes of query token t -- four tokens to a
+// wave -- resolve the token's probe ranks j = sub + 16 s once (centroid, list
+// start, fixed-point score) and, per segment, write the run [start, len) of
+// each of their ranks into an LDS table that only the same 16 lanes read, so
+// the tables need no barrier, and neither do the ranks: a token's runs belong
+// to lanes of one wave, whose LDS operations execute in program order, so
+// rank r's mask bits are set before rank r + 1 tests them.  A segment's ranks
+// are applied in batches of eight; the first kFiltHold entries of every run of
+// the next batch (of this segment, or the first batch of the next one) are
+// loaded into registers while the current batch is applied, and the rest of a
+// long run is loaded as it is applied.  The run tables are double-buffered
+// across segments, their columns swizzled (token ^ rank) so the per-segment
+// writes spread over the banks; the sub-range bounds are loaded one segment
+// before their table is written.
 //
 // XCD-aware order: consecutive workgroup ids go to the 8 XCDs in turn, so the
-// 1-D id is remapped to give each XCD a contiguous run of (query, segment)
-// tiles; neighbouring segments of one query then share the L2 lines of the
-// seg_off rows and of the query's probe lists.
+// 1-D id is remapped to give each XCD a contiguous run of (query, segment
+// group) tiles; neighbouring tiles of one query then share the L2 lines of
+// the seg_off rows and of the query's probe lists.
 //
-// Candidates go to the workgroup's own 4,096-slot region of the candidate
-// pool (query-major, then segment) as 64-bit selection keys, with the count
-// in seg_cnt[q][seg]: no atomics on shared counters, and each wave writes its
-// candidates as contiguous runs.
+// Candidates go to the segment's own 4,096-slot region of the candidate pool
+// (query-major, then segment) as 64-bit selection keys, with the count in
+// seg_cnt[q][seg]: no atomics on shared counters, and each wave writes its
+// candidates as contiguous runs.  The emission clears the LDS state it reads,
+// ready for the next segment.
 // ---------------------------------------------------------------------------
-constexpr uint32_t kFiltRounds = 8;   // probe ranks resolved per chunk (8 x 32 token slots)
-constexpr uint32_t kFiltHold = 2;     // list entries per thread and rank loaded up front
+constexpr uint32_t kFiltGroup = 8;                    // segments per workgroup
+constexpr uint32_t kFiltHold = 2;                     // list entries per lane and rank loaded ahead
+constexpr uint32_t kFtLanes = 16;                     // lanes per query token
+constexpr uint32_t kFtThreads = kMaxQTok * kFtLanes;  // 512
+constexpr uint32_t kFtRanks = kMaxProbe;              // probe ranks per query at most
+constexpr uint32_t kFtBatch = 8;                      // ranks per load batch
 
-__global__ __launch_bounds__(256) void k_filter_seg(
-    const uint32_t *__restrict__ top_ids, const float *__restrict__ top_s, uint32_t t_max,
-    uint32_t P, const uint32_t *__restrict__ q_ntok, const uint32_t *__restrict__ q_np,
-    const uint64_t *__restrict__ ivf_off, const uint32_t *__restrict__ ivf_ids,
-    const uint32_t *__restrict__ seg_off, uint32_t n_seg, uint32_t n_doc, uint32_t n_q,
-    uint32_t *__restrict__ seg_cnt, uint64_t *__restrict__ cand_keys) {
-  __shared__ uint32_t mask[kSeg];
-  __shared__ unsigned long long acc[kSeg];
-  __shared__ uint64_t pstart[kFiltRounds * kMaxQTok];
-  __shared__ uint32_t plen[kFiltRounds * kMaxQTok];
-  __shared__ long long pfx[kFiltRounds * kMaxQTok];
-
-  const uint32_t n_wg = n_seg * n_q;
-  const uint32_t per_xcd = (n_wg + 7) / 8;
-  const uint32_t L = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
-  if (L >= n_wg) return;  // uniform for the whole workgroup
-  const uint32_t q = L / n_seg, seg = L - q * n_seg;
-  const uint32_t T = q_ntok ? q_ntok[q] : t_max;
-  const uint32_t Pq = q_np ? q_np[q] : P;
-  const uint32_t seg_lo = seg * kSeg;
-  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
-  // in a rank, thread (pr, sub) takes entries sub, sub + 8, ... of token pr's list
-  const uint32_t pr = tid >> 3, sub = tid & 7u;
-
-  for (uint32_t i = tid; i < kSeg; i += 256) { mask[i] = 0u; acc[i] = 0ull; }
-
-  for (uint32_t j0 = 0; j0 < Pq; j0 += kFiltRounds) {
-    // resolve slot tid = (rank j0 + tid / 32, token tid % 32)
-    {
-      const uint32_t j = j0 + (tid >> 5), t = tid & 31u;
-      uint32_t len = 0;
-      uint64_t start = 0;
-      long long fx = 0;
-      if (j < Pq && t < T) {
-        const size_t o = ((size_t)q * t_max + t) * P + j;
-        const uint32_t c = top_ids[o];
-        if (c != 0xFFFFFFFFu) {
-          const uint32_t *so = seg_off + (size_t)c * (n_seg + 1);
-          const uint32_t lo = so[seg], hi = so[seg + 1];
-          len = hi - lo;
-          start = ivf_off[c] + lo;
-          fx = (long long)((double)top_s[o] * kFxScale);
-        }
-      }
-      __syncthreads();  // the previous chunk's readers are done
-      pstart[tid] = start;
-      plen[tid] = len;
-      pfx[tid] = fx;
-      __syncthreads();
-    }
-    const uint32_t nr = min(kFiltRounds, Pq - j0);
-    // load the first kFiltHold entries of every (rank, token) run of this chunk
-    uint32_t ent[kFiltRounds][kFiltHold];
-#pragma unroll
-    for (uint32_t r = 0; r < kFiltRounds; ++r) {
-      const uint32_t slot = r * kMaxQTok + pr;
-      const uint32_t len = (r < nr) ? plen[slot] : 0u;
-      const uint64_t st = pstart[slot];
-#pragma unroll
-      for (uint32_t h = 0; h < kFiltHold; ++h) {
-        const uint32_t i = sub + 8 * h;
-        ent[r][h] = (i < len) ? ivf_ids[st + i] : 0xFFFFFFFFu;
-      }
-    }
-#pragma unroll
-    for (uint32_t r = 0; r < kFiltRounds; ++r) {
-      if (r < nr) {
-        const uint32_t slot = r * kMaxQTok + pr;
-        const uint32_t bit = 1u << pr;
-        const unsigned long long add = (unsigned long long)pfx[slot];
-#pragma unroll
-        for (uint32_t h = 0; h < kFiltHold; ++h) {
-          if (ent[r][h] != 0xFFFFFFFFu) {
-            const uint32_t dl = (ent[r][h] - seg_lo) & (kSeg - 1);
-            const uint32_t old = atomicOr(&mask[dl], bit);
-            if (!(old & bit)) atomicAdd(&acc[dl], add);
-          }
-        }
-        // long runs: the rest of the entries, loaded here
-        const uint32_t len = plen[slot];
-        const uint64_t st = pstart[slot];
-        for (uint32_t i = sub + 8 * kFiltHold; i < len; i += 8) {
-          const uint32_t dl = (ivf_ids[st + i] - seg_lo) & (kSeg - 1);
-          const uint32_t old = atomicOr(&mask[dl], bit);
-          if (!(old & bit)) atomicAdd(&acc[dl], add);
-        }
-        __syncthreads();
-      }
-    }
-  }
-  __syncthreads();
-
-  // Emission: slot k of every lane in turn; a wave's hits for one k are
-  // written as one contiguous run (ballot + popcount), waves in order.
-  uint32_t hasbits = 0, myc = 0;
-#pragma unroll
-  for (uint32_t k = 0; k < kSeg / 256; ++k) {
-    const uint32_t dl = lane + kWave * (wv + 4 * k);
-    if (seg_lo + dl < n_doc && mask[dl] != 0u) { hasbits |= 1u << k; ++myc; }
-  }
-  uint32_t wcount = myc;
-#pragma unroll
-  for (uint32_t off = 1; off < kWave; off <<= 1) wcount += __shfl_xor(wcount, off, kWave);
-  __shared__ uint32_t wsum[4];
-  if (lane == 0) wsum[wv] = wcount;
-  __syncthreads();
-  uint32_t pos = 0;
-  for (uint32_t w = 0; w < wv; ++w) pos += wsum[w];
-  if (tid == 0) seg_cnt[(size_t)q * n_seg + seg] = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-  uint64_t *out = cand_keys + ((size_t)q * n_seg + seg) * kSeg;
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
-#pragma unroll
-  for (uint32_t k = 0; k < kSeg / 256; ++k) {
-    const bool hit = (hasbits >> k) & 1u;
-    const uint64_t b = __ballot(hit);
-    if (hit) {
-      const uint32_t dl = lane + kWave * (wv + 4 * k);
-      const float F = (float)((double)(long long)acc[dl] * (1.0 / kFxScale));
-      out[pos + __popcll(b & lt_mask)] = make_key(F, seg_lo + dl);
-    }
-    pos += __popcll(b);
-  }
+__device__ __forceinline__ uint32_t ft_slot(uint32_t r, uint32_t t) {
+  return r * kMaxQTok + (t ^ r);  // r < 32, t < 32
 }
 
-// ---------------------------------------------------------------------------
-// The same filter with kFiltGroup consecutive segments of one query per
-// workgroup, for nprobe <= kFiltRounds (every (rank, token) slot resolved by
-// one thread).  The query's probe lists are resolved once (centroid, list
-// start, fixed-point score); per segment the slot's [lo, hi) sub-range comes
-// from seg_off.  Loads run ahead of use: the sub-ranges two segments ahead and
-// the first list entries one segment ahead, so a segment's accumulation and
-// emission overlap the next segments' global reads instead of waiting on a
-// chain of dependent loads per workgroup.  The emission clears the LDS state
-// it reads, ready for the next segment.  Same arithmetic and output layout as
-// k_filter_seg.
-// ---------------------------------------------------------------------------
-constexpr uint32_t kFiltGroup = 8;     // segments per workgroup
-constexpr uint32_t kFgThreads = 512;   // 16 lanes per query token, 8 waves
-constexpr uint32_t kFgLanes = kFgThreads / kMaxQTok;
-
-__global__ __launch_bounds__(kFgThreads) void k_filter_grp(
+template <uint32_t NS>
+__global__ __launch_bounds__(kFtThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_filter_tok(
     const uint32_t *__restrict__ top_ids, const float *__restrict__ top_s, uint32_t t_max,
     uint32_t P, const uint32_t *__restrict__ q_ntok, const uint32_t *__restrict__ q_np,
     const uint64_t *__restrict__ ivf_off, const uint32_t *__restrict__ ivf_ids,
     const uint32_t *__restrict__ seg_off, uint32_t n_seg, uint32_t n_doc, uint32_t n_q,
     uint32_t *__restrict__ seg_cnt, uint64_t *__restrict__ cand_keys) {
+  static_assert(NS * kFtLanes <= kFtRanks, "ranks per lane");
   __shared__ uint32_t mask[kSeg];
   __shared__ unsigned long long acc[kSeg];
-  __shared__ uint64_t pstart[2][kFiltRounds * kMaxQTok];
-  __shared__ uint32_t plen[2][kFiltRounds * kMaxQTok];
-  __shared__ long long pfx[kFiltRounds * kMaxQTok];
-  __shared__ uint32_t wsum[kFgThreads / kWave];
+  __shared__ uint64_t tb_st[2][kFtRanks * kMaxQTok];
+  __shared__ uint32_t tb_len[2][kFtRanks * kMaxQTok];
+  __shared__ long long tb_fx[kFtRanks * kMaxQTok];
 
   const uint32_t n_grp = (n_seg + kFiltGroup - 1) / kFiltGroup;
   const uint32_t n_wg = n_grp * n_q;
@@ -418,119 +296,138 @@ __global__ __launch_bounds__(kFgThreads) void k_filter_grp(
   const uint32_t T = q_ntok ? q_ntok[q] : t_max;
   const uint32_t Pq = q_np ? q_np[q] : P;
   const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
-  // a token's runs: kFgLanes consecutive lanes of one wave, entry i to lane i % kFgLanes
-  const uint32_t pr = tid / kFgLanes, sub = tid % kFgLanes;
-  const bool resolver = tid < kFiltRounds * kMaxQTok;
+  const uint32_t t = tid / kFtLanes, sub = tid % kFtLanes;
+  const uint32_t bit = 1u << t;
+  const uint32_t nr = min(Pq, kFtLanes * NS);
+  const uint32_t nb = (nr + kFtBatch - 1) / kFtBatch;
 
-  // my slot (threads below 256): rank j = tid / 32, token t = tid % 32
-  const uint32_t *so = nullptr;
-  uint64_t base = 0;
-  bool valid = false;
-  {
-    const uint32_t j = tid >> 5, t = tid & 31u;
+  // this lane's ranks j = sub + 16 s: list, score, and the run bounds of the
+  // next segment (hi) and the one after it (nhi)
+  const uint32_t *so[NS];
+  uint64_t base[NS];
+  uint32_t hi[NS], nhi[NS];
+#pragma unroll
+  for (uint32_t s = 0; s < NS; ++s) {
+    const uint32_t j = sub + kFtLanes * s;
+    so[s] = nullptr;
+    base[s] = 0;
     long long fx = 0;
-    if (resolver && j < Pq && t < T) {
+    if (j < Pq && t < T) {
       const size_t o = ((size_t)q * t_max + t) * P + j;
       const uint32_t c = top_ids[o];
       if (c != 0xFFFFFFFFu) {
-        valid = true;
-        so = seg_off + (size_t)c * (n_seg + 1);
-        base = ivf_off[c];
+        so[s] = seg_off + (size_t)c * (n_seg + 1);
+        base[s] = ivf_off[c];
         fx = (long long)((double)top_s[o] * kFxScale);
       }
     }
-    if (resolver) pfx[tid] = fx;
-  }
-  uint32_t lo = 0, hi = 0, nlo = 0, nhi = 0;
-  if (valid) {
-    lo = so[sg0];
-    hi = so[sg0 + 1];
-    if (sg0 + 1 < sg1) {
-      nlo = hi;
-      nhi = so[sg0 + 2];
+    tb_fx[ft_slot(j, t)] = fx;
+    uint32_t lo = 0;
+    hi[s] = nhi[s] = 0;
+    if (so[s]) {
+      lo = so[s][sg0];
+      hi[s] = so[s][sg0 + 1];
+      nhi[s] = sg0 + 1 < sg1 ? so[s][sg0 + 2] : hi[s];
     }
+    tb_st[0][ft_slot(j, t)] = base[s] + lo;
+    tb_len[0][ft_slot(j, t)] = hi[s] - lo;
   }
-  for (uint32_t i = tid; i < kSeg; i += kFgThreads) { mask[i] = 0u; acc[i] = 0ull; }
-  if (resolver) {
-    pstart[0][tid] = base + lo;
-    plen[0][tid] = hi - lo;
-  }
+
+  // first kFiltHold entries of the runs of ranks [rb, rb + 8) from table buffer tb
+  auto load_batch = [&](uint32_t tb, uint32_t rb, uint32_t (&ent)[kFtBatch][kFiltHold],
+                        uint32_t (&len)[kFtBatch]) {
+#pragma unroll
+    for (uint32_t u = 0; u < kFtBatch; ++u) {
+      const uint32_t r = rb + u;
+      uint32_t l = 0;
+      uint64_t st = 0;
+      if (r < nr) {
+        l = tb_len[tb][ft_slot(r, t)];
+        st = tb_st[tb][ft_slot(r, t)];
+      }
+      len[u] = l;
+#pragma unroll
+      for (uint32_t k = 0; k < kFiltHold; ++k) {
+        const uint32_t i = sub + kFtLanes * k;
+        ent[u][k] = (i < l) ? ivf_ids[st + i] : 0xFFFFFFFFu;
+      }
+    }
+  };
+
+  uint32_t ent[kFtBatch][kFiltHold], len[kFtBatch];
+  load_batch(0, 0, ent, len);
+  for (uint32_t i = tid; i < kSeg; i += kFtThreads) { mask[i] = 0u; acc[i] = 0ull; }
   __syncthreads();
-  const uint32_t nr = min(kFiltRounds, Pq);
-  uint32_t ent[kFiltRounds][kFiltHold];
-#pragma unroll
-  for (uint32_t r = 0; r < kFiltRounds; ++r) {
-    const uint32_t slot = r * kMaxQTok + pr;
-    const uint32_t len = (r < nr) ? plen[0][slot] : 0u;
-    const uint64_t st = pstart[0][slot];
-#pragma unroll
-    for (uint32_t k = 0; k < kFiltHold; ++k) {
-      const uint32_t i = sub + kFgLanes * k;
-      ent[r][k] = (i < len) ? ivf_ids[st + i] : 0xFFFFFFFFu;
-    }
-  }
 
   for (uint32_t sg = sg0; sg < sg1; ++sg) {
     const uint32_t b = (sg - sg0) & 1u;
     const uint32_t seg_lo = sg * kSeg;
-    // the next segment's sub-ranges (loaded one segment ago) into the other
-    // buffer; the loads for the segment after it
-    if (resolver && sg + 1 < sg1) {
-      pstart[b ^ 1u][tid] = base + nlo;
-      plen[b ^ 1u][tid] = nhi - nlo;
-      if (valid && sg + 2 < sg1) {
-        nlo = nhi;
-        nhi = so[sg + 3];
+    for (uint32_t ib = 0; ib < nb; ++ib) {
+      // the next batch's first entries, in flight while this batch is applied
+      uint32_t ent_n[kFtBatch][kFiltHold], len_n[kFtBatch];
+      if (ib + 1 < nb) {
+        load_batch(b, (ib + 1) * kFtBatch, ent_n, len_n);
+      } else if (sg + 1 < sg1) {
+        // the next segment's runs into the other table buffer
+#pragma unroll
+        for (uint32_t s = 0; s < NS; ++s) {
+          const uint32_t j = sub + kFtLanes * s;
+          tb_st[b ^ 1u][ft_slot(j, t)] = base[s] + hi[s];
+          tb_len[b ^ 1u][ft_slot(j, t)] = nhi[s] - hi[s];
+          hi[s] = nhi[s];
+          if (so[s] && sg + 2 < sg1) nhi[s] = so[s][sg + 3];
+        }
+        load_batch(b ^ 1u, 0, ent_n, len_n);
+      } else {
+#pragma unroll
+        for (uint32_t u = 0; u < kFtBatch; ++u) {
+          len_n[u] = 0;
+#pragma unroll
+          for (uint32_t k = 0; k < kFiltHold; ++k) ent_n[u][k] = 0xFFFFFFFFu;
+        }
       }
-    }
-    __syncthreads();
-    // the next segment's first entries, in flight while this one accumulates
-    uint32_t ent_n[kFiltRounds][kFiltHold];
 #pragma unroll
-    for (uint32_t r = 0; r < kFiltRounds; ++r) {
-      const uint32_t slot = r * kMaxQTok + pr;
-      const uint32_t len = (sg + 1 < sg1 && r < nr) ? plen[b ^ 1u][slot] : 0u;
-      const uint64_t st = pstart[b ^ 1u][slot];
+      for (uint32_t u = 0; u < kFtBatch; ++u) {
+        const uint32_t r = ib * kFtBatch + u;
+        if (r < nr) {
+          const unsigned long long add = (unsigned long long)tb_fx[ft_slot(r, t)];
 #pragma unroll
-      for (uint32_t k = 0; k < kFiltHold; ++k) {
-        const uint32_t i = sub + kFgLanes * k;
-        ent_n[r][k] = (i < len) ? ivf_ids[st + i] : 0xFFFFFFFFu;
-      }
-    }
-#pragma unroll
-    for (uint32_t r = 0; r < kFiltRounds; ++r) {
-      if (r < nr) {
-        const uint32_t slot = r * kMaxQTok + pr;
-        const uint32_t bit = 1u << pr;
-        const unsigned long long add = (unsigned long long)pfx[slot];
-#pragma unroll
-        for (uint32_t k = 0; k < kFiltHold; ++k) {
-          if (ent[r][k] != 0xFFFFFFFFu) {
-            const uint32_t dl = (ent[r][k] - seg_lo) & (kSeg - 1);
-            const uint32_t old = atomicOr(&mask[dl], bit);
-            if (!(old & bit)) atomicAdd(&acc[dl], add);
+          for (uint32_t k = 0; k < kFiltHold; ++k) {
+            if (ent[u][k] != 0xFFFFFFFFu) {
+              const uint32_t dl = (ent[u][k] - seg_lo) & (kSeg - 1);
+              const uint32_t old = atomicOr(&mask[dl], bit);
+              if (!(old & bit)) atomicAdd(&acc[dl], add);
+            }
+          }
+          // long runs: the rest of the entries, loaded here
+          if (len[u] > kFtLanes * kFiltHold) {
+            const uint64_t st = tb_st[b][ft_slot(r, t)];
+            for (uint32_t i = sub + kFtLanes * kFiltHold; i < len[u]; i += kFtLanes) {
+              const uint32_t dl = (ivf_ids[st + i] - seg_lo) & (kSeg - 1);
+              const uint32_t old = atomicOr(&mask[dl], bit);
+              if (!(old & bit)) atomicAdd(&acc[dl], add);
+            }
           }
         }
-        // long runs: the rest of the entries, loaded here
-        const uint32_t len = plen[b][slot];
-        const uint64_t st = pstart[b][slot];
-        for (uint32_t i = sub + kFgLanes * kFiltHold; i < len; i += kFgLanes) {
-          const uint32_t dl = (ivf_ids[st + i] - seg_lo) & (kSeg - 1);
-          const uint32_t old = atomicOr(&mask[dl], bit);
-          if (!(old & bit)) atomicAdd(&acc[dl], add);
-        }
-        // no barrier between ranks: a token's runs belong to kFgLanes lanes of
-        // one wave, whose LDS operations execute in program order, so rank
-        // r's bits are set before rank r + 1 tests them
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < kFtBatch; ++u) {
+        len[u] = len_n[u];
+#pragma unroll
+        for (uint32_t k = 0; k < kFiltHold; ++k) ent[u][k] = ent_n[u][k];
       }
     }
     __syncthreads();
 
-    // emission (as in k_filter_seg), clearing what it reads
+    // Emission: slot k of every lane in turn; a wave's hits for one k are
+    // written as one contiguous run (ballot + popcount), waves in order.  The
+    // per-wave counts go to this segment's run-length table, whose last reader
+    // was the loop above and whose next writer runs after the closing barrier.
+    uint32_t *wsum = tb_len[b];
     uint32_t hasbits = 0, myc = 0;
 #pragma unroll
-    for (uint32_t k = 0; k < kSeg / kFgThreads; ++k) {
-      const uint32_t dl = lane + kWave * (wv + (kFgThreads / kWave) * k);
+    for (uint32_t k = 0; k < kSeg / kFtThreads; ++k) {
+      const uint32_t dl = lane + kWave * (wv + (kFtThreads / kWave) * k);
       if (mask[dl] != 0u) {
         mask[dl] = 0u;
         if (seg_lo + dl < n_doc) { hasbits |= 1u << k; ++myc; }
@@ -545,16 +442,16 @@ __global__ __launch_bounds__(kFgThreads) void k_filter_grp(
     for (uint32_t w = 0; w < wv; ++w) pos += wsum[w];
     if (tid == 0) {
       uint32_t tot = 0;
-      for (uint32_t w = 0; w < kFgThreads / kWave; ++w) tot += wsum[w];
+      for (uint32_t w = 0; w < kFtThreads / kWave; ++w) tot += wsum[w];
       seg_cnt[(size_t)q * n_seg + sg] = tot;
     }
     uint64_t *out = cand_keys + ((size_t)q * n_seg + sg) * kSeg;
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
 #pragma unroll
-    for (uint32_t k = 0; k < kSeg / kFgThreads; ++k) {
+    for (uint32_t k = 0; k < kSeg / kFtThreads; ++k) {
       const bool hit = (hasbits >> k) & 1u;
       const uint64_t bb = __ballot(hit);
-      const uint32_t dl = lane + kWave * (wv + (kFgThreads / kWave) * k);
+      const uint32_t dl = lane + kWave * (wv + (kFtThreads / kWave) * k);
       if (hit) {
         const float F = (float)((double)(long long)acc[dl] * (1.0 / kFxScale));
         out[pos + __popcll(bb & lt_mask)] = make_key(F, seg_lo + dl);
@@ -562,10 +459,6 @@ __global__ __launch_bounds__(kFgThreads) void k_filter_grp(
       acc[dl] = 0ull;
       pos += __popcll(bb);
     }
-#pragma unroll
-    for (uint32_t r = 0; r < kFiltRounds; ++r)
-#pragma unroll
-      for (uint32_t k = 0; k < kFiltHold; ++k) ent[r][k] = ent_n[r][k];
     __syncthreads();  // wsum and the cleared state before the next segment
   }
 }
@@ -2515,18 +2408,15 @@ hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t
                          const DevIndex &ix, uint32_t *seg_cnt, uint64_t *cand_keys, hipStream_t s) {
   if (n_q == 0) return hipSuccess;
   const uint64_t n_wg = (uint64_t)ix.n_seg * n_q;
-  if (n_wg > 0x7FFFFFF8ull) return hipErrorInvalidValue;
-  if (P <= kFiltRounds) {
-    const uint64_t n_grp = ((uint64_t)ix.n_seg + kFiltGroup - 1) / kFiltGroup * n_q;
-    const uint32_t grid = (uint32_t)((n_grp + 7) / 8 * 8);  // the kernel's XCD remap needs a multiple of 8
-    k_filter_grp<<<grid, kFgThreads, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np, ix.ivf_off, ix.ivf_ids,
-                                      ix.seg_off, ix.n_seg, ix.n_doc, n_q, seg_cnt, cand_keys);
-    return hipGetLastError();
-  }
-  const uint32_t grid = (uint32_t)((n_wg + 7) / 8 * 8);  // the kernel's XCD remap needs a multiple of 8
-  k_filter_seg<<<grid, 256, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np, ix.ivf_off,
-                                    ix.ivf_ids, ix.seg_off, ix.n_seg, ix.n_doc, n_q, seg_cnt,
-                                    cand_keys);
+  if (n_wg > 0x7FFFFFF8ull || P == 0 || P > kFtRanks) return hipErrorInvalidValue;
+  const uint64_t n_grp = ((uint64_t)ix.n_seg + kFiltGroup - 1) / kFiltGroup * n_q;
+  const uint32_t grid = (uint32_t)((n_grp + 7) / 8 * 8);  // the kernel's XCD remap needs a multiple of 8
+  if (P <= kFtLanes)
+    k_filter_tok<1><<<grid, kFtThreads, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np, ix.ivf_off, ix.ivf_ids,
+                                                 ix.seg_off, ix.n_seg, ix.n_doc, n_q, seg_cnt, cand_keys);
+  else
+    k_filter_tok<2><<<grid, kFtThreads, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np, ix.ivf_off, ix.ivf_ids,
+                                                 ix.seg_off, ix.n_seg, ix.n_doc, n_q, seg_cnt, cand_keys);
   return hipGetLastError();
 }
 
