@@ -2219,15 +2219,26 @@ __global__ __launch_bounds__(256) void k_count_work(const uint32_t *__restrict__
                                                     const uint32_t *__restrict__ cand_cnt,
                                                     const uint32_t *__restrict__ q_ntok,
                                                     unsigned long long *__restrict__ counters) {
+  __shared__ unsigned long long wsum[256 / kWave];
   const uint32_t q = blockIdx.x;
   const uint32_t n = sel_n[q];
   unsigned long long t = 0;
   for (uint32_t r = threadIdx.x; r < n; r += 256) t += doclens[sel_ids[(size_t)q * R + r]];
-  if (t) {
-    atomicAdd(&counters[0], t);
-    atomicAdd(&counters[2], t * q_ntok[q]);
+  // one atomic per counter and query: contended atomics from every thread
+  // would serialise on the three counters
+#pragma unroll
+  for (uint32_t off = 1; off < kWave; off <<= 1) t += __shfl_xor(t, off, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) wsum[threadIdx.x / kWave] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long tot = 0;
+    for (uint32_t w = 0; w < 256 / kWave; ++w) tot += wsum[w];
+    if (tot) {
+      atomicAdd(&counters[0], tot);
+      atomicAdd(&counters[2], tot * q_ntok[q]);
+    }
+    atomicAdd(&counters[1], (unsigned long long)cand_cnt[q]);
   }
-  if (threadIdx.x == 0) atomicAdd(&counters[1], (unsigned long long)cand_cnt[q]);
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {

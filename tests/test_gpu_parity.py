@@ -365,3 +365,20 @@ def test_centroid_topk_partial_tiles_and_tiny_nprobe():
         ti, ts = O.centroid_topk(O.bf16_round(Q), O.bf16_round(ix.C), P)
         np.testing.assert_array_equal(st["top_ids"], ti.reshape(Q.shape[0], Q.shape[1], P))
         np.testing.assert_array_equal(st["top_s"], ts.reshape(Q.shape[0], Q.shape[1], P))
+
+
+def test_timings_work_counts_exact(c1):
+    """The work counters of a timed search equal sums over its stage outputs:
+    reranked doc tokens, filter candidates, and query tokens x reranked doc
+    tokens (ragged token counts, several passes)."""
+    corpus, ix, Q = c1
+    g = _index(ix)
+    rng = np.random.default_rng(21)
+    qt = rng.integers(1, 33, Q.shape[0]).astype(np.uint32)
+    ids, sc, st, tm = g.search(Q, 8, 300, 10, q_ntok=qt, batch=24, stages=True, timings=True)
+    toks = np.array([int(ix.doclens[st["sel_ids"][q, :st["rerank_n"][q]]].astype(np.uint64).sum())
+                     for q in range(Q.shape[0])], np.uint64)
+    assert tm["rerank_tokens"] == int(toks.sum())
+    assert tm["candidates"] == int(st["cand_n"].astype(np.uint64).sum())
+    assert tm["rerank_pairs"] == int((toks * qt.astype(np.uint64)).sum())
+    assert tm["n_pass"] == 3
