@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256) void k_topk_merge(const uint32_t *__restrict__
 // ready for the next segment.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kFiltGroup = 8;                    // segments per workgroup
-constexpr uint32_t kFiltHold = 2;                     // list entries per lane and rank loaded ahead
+constexpr uint32_t kFiltHold = 1;                     // list entries per lane and rank loaded ahead
 constexpr uint32_t kFtLanes = 16;                     // lanes per query token
 constexpr uint32_t kFtThreads = kMaxQTok * kFtLanes;  // 512
 constexpr uint32_t kFtRanks = kMaxProbe;              // probe ranks per query at most
