@@ -57,16 +57,18 @@ __global__ __launch_bounds__(256) void k_gemm(const float *__restrict__ A, size_
       Bs[kk][c] = (j < N && k < K) ? B[k * b_k + j * b_j] : 0.0f;
     }
     __syncthreads();
+    const uint32_t kn = min(kTk, K - k0);  // steps of this chunk inside K (uniform)
 #pragma unroll
     for (uint32_t kk = 0; kk < kTk; ++kk) {
-      if (k0 + kk >= K) break;
-      float a[4], b[4];
+      if (kk < kn) {
+        float a[4], b[4];
 #pragma unroll
-      for (uint32_t u = 0; u < 4; ++u) { a[u] = As[kk][ty * 4 + u]; b[u] = Bs[kk][tx * 4 + u]; }
+        for (uint32_t u = 0; u < 4; ++u) { a[u] = As[kk][ty * 4 + u]; b[u] = Bs[kk][tx * 4 + u]; }
 #pragma unroll
-      for (uint32_t u = 0; u < 4; ++u)
+        for (uint32_t u = 0; u < 4; ++u)
 #pragma unroll
-        for (uint32_t v = 0; v < 4; ++v) acc[u][v] = __builtin_fmaf(a[u], b[v], acc[u][v]);
+          for (uint32_t v = 0; v < 4; ++v) acc[u][v] = __builtin_fmaf(a[u], b[v], acc[u][v]);
+      }
     }
     __syncthreads();
   }

@@ -28,33 +28,54 @@ __global__ void k_probe_cvt_fp8(const float *x, uint32_t *out, uint32_t n_pairs)
   out[i] = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(x[2 * i], x[2 * i + 1], 0, false) & 0xFFFFu;
 }
 
+// The first failing status of a sequence of HIP calls; later calls are
+// skipped once one has failed.
+struct FirstError {
+  hipError_t e = hipSuccess;
+  bool ok() const { return e == hipSuccess; }
+  void operator()(hipError_t x) {
+    if (e == hipSuccess) e = x;
+  }
+};
+
 extern "C" int probe_mfma(int kind, const void *a, const void *b, float *d) {
   const size_t op = kind == 0 ? 16 : 8;
-  void *da, *db, *dd;
-  if (hipMalloc(&da, 64 * op) || hipMalloc(&db, 64 * op) || hipMalloc(&dd, 64 * 64)) return 1;
-  hipMemcpy(da, a, 64 * op, hipMemcpyHostToDevice);
-  hipMemcpy(db, b, 64 * op, hipMemcpyHostToDevice);
-  if (kind == 0)
-    k_probe_bf16<<<1, 64>>>((const bf16x8 *)da, (const bf16x8 *)db, (floatx16 *)dd);
-  else
-    k_probe_fp8<<<1, 64>>>((const long *)da, (const long *)db, (floatx16 *)dd);
-  hipError_t e = hipMemcpy(d, dd, 64 * 64, hipMemcpyDeviceToHost);
-  hipFree(da);
-  hipFree(db);
-  hipFree(dd);
-  return e != hipSuccess;
+  void *da = nullptr, *db = nullptr, *dd = nullptr;
+  FirstError err;
+  err(hipMalloc(&da, 64 * op));
+  if (err.ok()) err(hipMalloc(&db, 64 * op));
+  if (err.ok()) err(hipMalloc(&dd, 64 * 64));
+  if (err.ok()) err(hipMemcpy(da, a, 64 * op, hipMemcpyHostToDevice));
+  if (err.ok()) err(hipMemcpy(db, b, 64 * op, hipMemcpyHostToDevice));
+  if (err.ok()) {
+    if (kind == 0)
+      k_probe_bf16<<<1, 64>>>((const bf16x8 *)da, (const bf16x8 *)db, (floatx16 *)dd);
+    else
+      k_probe_fp8<<<1, 64>>>((const long *)da, (const long *)db, (floatx16 *)dd);
+    err(hipGetLastError());
+  }
+  if (err.ok()) err(hipMemcpy(d, dd, 64 * 64, hipMemcpyDeviceToHost));
+  (void)hipFree(da);
+  (void)hipFree(db);
+  (void)hipFree(dd);
+  return !err.ok();
 }
 
 extern "C" int probe_cvt_fp8(const float *x, uint32_t *out, uint32_t n_pairs) {
-  float *dx;
-  uint32_t *dout;
-  if (hipMalloc((void **)&dx, n_pairs * 8) || hipMalloc((void **)&dout, n_pairs * 4)) return 1;
-  hipMemcpy(dx, x, n_pairs * 8, hipMemcpyHostToDevice);
-  k_probe_cvt_fp8<<<(n_pairs + 63) / 64, 64>>>(dx, dout, n_pairs);
-  hipError_t e = hipMemcpy(out, dout, n_pairs * 4, hipMemcpyDeviceToHost);
-  hipFree(dx);
-  hipFree(dout);
-  return e != hipSuccess;
+  float *dx = nullptr;
+  uint32_t *dout = nullptr;
+  FirstError err;
+  err(hipMalloc((void **)&dx, n_pairs * 8));
+  if (err.ok()) err(hipMalloc((void **)&dout, n_pairs * 4));
+  if (err.ok()) err(hipMemcpy(dx, x, n_pairs * 8, hipMemcpyHostToDevice));
+  if (err.ok()) {
+    k_probe_cvt_fp8<<<(n_pairs + 63) / 64, 64>>>(dx, dout, n_pairs);
+    err(hipGetLastError());
+  }
+  if (err.ok()) err(hipMemcpy(out, dout, n_pairs * 4, hipMemcpyDeviceToHost));
+  (void)hipFree(dx);
+  (void)hipFree(dout);
+  return !err.ok();
 }
 
 // Single-precision square root and division: the default operators (what the
@@ -69,15 +90,20 @@ __global__ void k_probe_divsqrt(const float *a, const float *b, float *out, uint
 }
 
 extern "C" int probe_divsqrt(const float *a, const float *b, float *out, uint32_t n) {
-  float *da, *db, *dout;
-  if (hipMalloc((void **)&da, n * 4) || hipMalloc((void **)&db, n * 4) || hipMalloc((void **)&dout, n * 16))
-    return 1;
-  hipMemcpy(da, a, n * 4, hipMemcpyHostToDevice);
-  hipMemcpy(db, b, n * 4, hipMemcpyHostToDevice);
-  k_probe_divsqrt<<<(n + 255) / 256, 256>>>(da, db, dout, n);
-  hipError_t e = hipMemcpy(out, dout, (size_t)n * 16, hipMemcpyDeviceToHost);
-  hipFree(da);
-  hipFree(db);
-  hipFree(dout);
-  return e != hipSuccess;
+  float *da = nullptr, *db = nullptr, *dout = nullptr;
+  FirstError err;
+  err(hipMalloc((void **)&da, n * 4));
+  if (err.ok()) err(hipMalloc((void **)&db, n * 4));
+  if (err.ok()) err(hipMalloc((void **)&dout, n * 16));
+  if (err.ok()) err(hipMemcpy(da, a, n * 4, hipMemcpyHostToDevice));
+  if (err.ok()) err(hipMemcpy(db, b, n * 4, hipMemcpyHostToDevice));
+  if (err.ok()) {
+    k_probe_divsqrt<<<(n + 255) / 256, 256>>>(da, db, dout, n);
+    err(hipGetLastError());
+  }
+  if (err.ok()) err(hipMemcpy(out, dout, (size_t)n * 16, hipMemcpyDeviceToHost));
+  (void)hipFree(da);
+  (void)hipFree(db);
+  (void)hipFree(dout);
+  return !err.ok();
 }
