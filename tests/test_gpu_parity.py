@@ -382,3 +382,30 @@ def test_timings_work_counts_exact(c1):
     assert tm["candidates"] == int(st["cand_n"].astype(np.uint64).sum())
     assert tm["rerank_pairs"] == int((toks * qt.astype(np.uint64)).sum())
     assert tm["n_pass"] == 3
+
+
+@pytest.mark.parametrize("R,k", [(4096, 4096), (4096, 100), (1, 1)])
+def test_largest_and_smallest_result_sizes(c1, R, k):
+    """probe_topk at its maximum (4,096: the radix select with a 4,096-key
+    output, the final top-k over 4,096 scores) and at its minimum, nprobe 32
+    so that the candidates exceed 4,096; bit-exact against the oracle."""
+    corpus, ix, Q = c1
+    g = _index(ix)
+    Qs = Q[:4]
+    ids, sc, st = g.search(Qs, 32, R, k, stages=True)
+    assert (st["cand_n"] > 4096).all()
+    _check(st, ids, sc, _oracle_stages(ix, Qs, 32, R, k))
+
+
+def test_zero_queries(small):
+    """An empty batch returns without work, for host and device buffers."""
+    import torch
+    corpus, ix, Q = small
+    g = _index(ix)
+    ids, sc = g.search(Q[:0], 4, 64, 10)
+    assert ids.shape == (0, 10) and sc.shape == (0, 10)
+    Qd = torch.zeros((0, Q.shape[1], Q.shape[2]), dtype=torch.float32, device="cuda")
+    oi = torch.empty((0, 10), dtype=torch.int32, device="cuda")
+    os_ = torch.empty((0, 10), dtype=torch.float32, device="cuda")
+    g.search_device(Qd, oi, os_, 4, 64, 10)
+    torch.cuda.synchronize()
