@@ -419,23 +419,30 @@ __global__ __launch_bounds__(kFtThreads) __attribute__((amdgpu_waves_per_eu(4)))
     }
     __syncthreads();
 
-    // Emission: slot k of every lane in turn; a wave's hits for one k are
-    // written as one contiguous run (ballot + popcount), waves in order.  The
-    // per-wave counts go to this segment's run-length table, whose last reader
-    // was the loop above and whose next writer runs after the closing barrier.
+    // Emission: slot k of every lane in turn; a wave's hits (slot-major, then
+    // lane) are compacted into a list of doc offsets, so the conversion and
+    // key packing run once per hit rather than once per slot, and written as
+    // one contiguous run, waves in order.  The per-wave counts and lists go
+    // to this segment's run tables, whose last reader was the loop above and
+    // whose next writer runs after the closing barrier.
     uint32_t *wsum = tb_len[b];
-    uint32_t hasbits = 0, myc = 0;
+    uint16_t *hl = reinterpret_cast<uint16_t *>(tb_st[b]) + wv * (kSeg / (kFtThreads / kWave));
+    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+    const bool tail = seg_lo + kSeg > n_doc;  // the last segment, partly past the corpus
+    uint32_t wcount = 0;  // hits of this wave so far (uniform)
 #pragma unroll
     for (uint32_t k = 0; k < kSeg / kFtThreads; ++k) {
       const uint32_t dl = lane + kWave * (wv + (kFtThreads / kWave) * k);
+      bool hit = false;
       if (mask[dl] != 0u) {
         mask[dl] = 0u;
-        if (seg_lo + dl < n_doc) { hasbits |= 1u << k; ++myc; }
+        hit = !tail || seg_lo + dl < n_doc;
+        if (!hit) acc[dl] = 0ull;
       }
+      const uint64_t bb = __ballot(hit);
+      if (hit) hl[wcount + __popcll(bb & lt_mask)] = (uint16_t)dl;
+      wcount += (uint32_t)__popcll(bb);
     }
-    uint32_t wcount = myc;
-#pragma unroll
-    for (uint32_t off = 1; off < kWave; off <<= 1) wcount += __shfl_xor(wcount, off, kWave);
     if (lane == 0) wsum[wv] = wcount;
     __syncthreads();
     uint32_t pos = 0;
@@ -445,19 +452,12 @@ __global__ __launch_bounds__(kFtThreads) __attribute__((amdgpu_waves_per_eu(4)))
       for (uint32_t w = 0; w < kFtThreads / kWave; ++w) tot += wsum[w];
       seg_cnt[(size_t)q * n_seg + sg] = tot;
     }
-    uint64_t *out = cand_keys + ((size_t)q * n_seg + sg) * kSeg;
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
-#pragma unroll
-    for (uint32_t k = 0; k < kSeg / kFtThreads; ++k) {
-      const bool hit = (hasbits >> k) & 1u;
-      const uint64_t bb = __ballot(hit);
-      const uint32_t dl = lane + kWave * (wv + (kFtThreads / kWave) * k);
-      if (hit) {
-        const float F = (float)((double)(long long)acc[dl] * (1.0 / kFxScale));
-        out[pos + __popcll(bb & lt_mask)] = make_key(F, seg_lo + dl);
-      }
+    uint64_t *out = cand_keys + ((size_t)q * n_seg + sg) * kSeg + pos;
+    for (uint32_t i = lane; i < wcount; i += kWave) {
+      const uint32_t dl = hl[i];
+      const float F = (float)((double)(long long)acc[dl] * (1.0 / kFxScale));
       acc[dl] = 0ull;
-      pos += __popcll(bb);
+      out[i] = make_key(F, seg_lo + dl);
     }
     __syncthreads();  // wsum and the cleared state before the next segment
   }
