@@ -264,7 +264,7 @@ __global__ __launch_bounds__(256) void k_topk_merge(const uint32_t *__restrict__
 constexpr uint32_t kFiltGroup = 8;                    // segments per workgroup
 constexpr uint32_t kFiltHold = 1;                     // list entries per lane and rank loaded ahead
 constexpr uint32_t kFtLanes = 16;                     // lanes per query token
-constexpr uint32_t kFtThreads = kMaxQTok * kFtLanes;  // 512
+constexpr uint32_t kFtThreads = kMaxQTok * kFtLanes;  // threads per workgroup
 constexpr uint32_t kFtRanks = kMaxProbe;              // probe ranks per query at most
 constexpr uint32_t kFtBatch = 8;                      // ranks per load batch
 
@@ -301,7 +301,7 @@ __global__ __launch_bounds__(kFtThreads) __attribute__((amdgpu_waves_per_eu(4)))
   const uint32_t nr = min(Pq, kFtLanes * NS);
   const uint32_t nb = (nr + kFtBatch - 1) / kFtBatch;
 
-  // this lane's ranks j = sub + 16 s: list, score, and the run bounds of the
+  // this lane's ranks j = sub + kFtLanes s: list, score, and the run bounds of the
   // next segment (hi) and the one after it (nhi)
   const uint32_t *so[NS];
   uint64_t base[NS];
@@ -394,7 +394,7 @@ __global__ __launch_bounds__(kFtThreads) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
           for (uint32_t k = 0; k < kFiltHold; ++k) {
             if (ent[u][k] != 0xFFFFFFFFu) {
-              const uint32_t dl = (ent[u][k] - seg_lo) & (kSeg - 1);
+              const uint32_t dl = ent[u][k] & (kSeg - 1);  // seg_lo is a multiple of kSeg
               const uint32_t old = atomicOr(&mask[dl], bit);
               if (!(old & bit)) atomicAdd(&acc[dl], add);
             }
@@ -403,7 +403,7 @@ __global__ __launch_bounds__(kFtThreads) __attribute__((amdgpu_waves_per_eu(4)))
           if (len[u] > kFtLanes * kFiltHold) {
             const uint64_t st = tb_st[b][ft_slot(r, t)];
             for (uint32_t i = sub + kFtLanes * kFiltHold; i < len[u]; i += kFtLanes) {
-              const uint32_t dl = (ivf_ids[st + i] - seg_lo) & (kSeg - 1);
+              const uint32_t dl = ivf_ids[st + i] & (kSeg - 1);
               const uint32_t old = atomicOr(&mask[dl], bit);
               if (!(old & bit)) atomicAdd(&acc[dl], add);
             }
@@ -2422,12 +2422,17 @@ hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t
   if (n_wg > 0x7FFFFFF8ull || P == 0 || P > kFtRanks) return hipErrorInvalidValue;
   const uint64_t n_grp = ((uint64_t)ix.n_seg + kFiltGroup - 1) / kFiltGroup * n_q;
   const uint32_t grid = (uint32_t)((n_grp + 7) / 8 * 8);  // the kernel's XCD remap needs a multiple of 8
+  // probe ranks per lane: the fewest that hold P
   if (P <= kFtLanes)
     k_filter_tok<1><<<grid, kFtThreads, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np, ix.ivf_off, ix.ivf_ids,
                                                  ix.seg_off, ix.n_seg, ix.n_doc, n_q, seg_cnt, cand_keys);
-  else
+  else if (P <= 2 * kFtLanes)
     k_filter_tok<2><<<grid, kFtThreads, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np, ix.ivf_off, ix.ivf_ids,
                                                  ix.seg_off, ix.n_seg, ix.n_doc, n_q, seg_cnt, cand_keys);
+  else
+    k_filter_tok<kFtRanks / kFtLanes><<<grid, kFtThreads, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np,
+                                                                   ix.ivf_off, ix.ivf_ids, ix.seg_off, ix.n_seg,
+                                                                   ix.n_doc, n_q, seg_cnt, cand_keys);
   return hipGetLastError();
 }
 
