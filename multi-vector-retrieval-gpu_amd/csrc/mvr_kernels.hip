@@ -510,6 +510,8 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
   __shared__ uint32_t s_n, s_wsum[kSelThreads / kWave];
 
   const uint32_t q = blockIdx.x, tid = threadIdx.x;
+  const uint32_t sel_lane = tid & (kWave - 1);
+  const uint64_t sel_lt_mask = sel_lane == 0 ? 0ull : (~0ull >> (kWave - sel_lane));
   const uint32_t *cnt = seg_cnt + (size_t)q * n_seg;
   const uint64_t *keys = cand_keys + (size_t)q * n_seg * kSeg;
   const bool flat = n_seg <= kSelMaxSeg;
@@ -620,10 +622,25 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
       const uint32_t nb = 1u << bits;
       for (uint32_t b = tid; b < nb; b += kSelThreads) hist[b] = 0;
       __syncthreads();
-      for_each_key([&](uint64_t k) {
-        if (hi_shift == 64 || (k >> hi_shift) == prefix)
-          atomicAdd(&hist[(uint32_t)(k >> shift) & (nb - 1)], 1u);
-      });
+      if (hi_shift == 64) {
+        // the first digit (sign, exponent and two mantissa bits of the
+        // score) crowds into a few bins: the lanes of a wave holding the same
+        // digit add their count once
+        for_each_key([&](uint64_t k) {
+          const uint32_t dg = (uint32_t)(k >> shift) & (nb - 1);
+          uint64_t m = __ballot(true);
+          while (m) {
+            const uint32_t d0 = (uint32_t)__shfl((int)dg, (int)__builtin_ctzll(m));
+            const uint64_t same = __ballot(dg == d0);
+            if (dg == d0 && (same & sel_lt_mask) == 0) atomicAdd(&hist[d0], (uint32_t)__popcll(same));
+            m &= ~same;
+          }
+        });
+      } else {
+        for_each_key([&](uint64_t k) {
+          if ((k >> hi_shift) == prefix) atomicAdd(&hist[(uint32_t)(k >> shift) & (nb - 1)], 1u);
+        });
+      }
       __syncthreads();
       if (tid == 0) {
         uint32_t cum = 0, b = nb;
