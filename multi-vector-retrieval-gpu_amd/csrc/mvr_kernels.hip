@@ -1952,10 +1952,18 @@ __global__ __launch_bounds__(512) void k_ct_bf16_blockmax(
         for (uint32_t s = 0; s < 8; ++s)
           acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(As[u][col][8 * h + s]), qf[s], acc, 0, 0, 0);
         float mx = -INFINITY;
+        if (tile * 32 + 32 <= n_c) {  // a full tile (uniform): no row checks
+          float t4[4];
 #pragma unroll
-        for (uint32_t j = 0; j < 16; ++j) {
-          const uint32_t m = 8 * (j >> 2) + 4 * h + (j & 3);
-          if (tile * 32 + m < n_c) mx = fmaxf(mx, acc[j]);
+          for (uint32_t g = 0; g < 4; ++g)
+            t4[g] = fmaxf(fmaxf(acc[4 * g], acc[4 * g + 1]), fmaxf(acc[4 * g + 2], acc[4 * g + 3]));
+          mx = fmaxf(fmaxf(t4[0], t4[1]), fmaxf(t4[2], t4[3]));
+        } else {
+#pragma unroll
+          for (uint32_t j = 0; j < 16; ++j) {
+            const uint32_t m = 8 * (j >> 2) + 4 * h + (j & 3);
+            if (tile * 32 + m < n_c) mx = fmaxf(mx, acc[j]);
+          }
         }
         mxb[u] = fmaxf(mx, __shfl_xor(mx, 32));
       }
