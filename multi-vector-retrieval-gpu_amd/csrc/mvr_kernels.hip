@@ -1906,13 +1906,23 @@ __global__ __launch_bounds__(512) void k_ct_bf16_blockmax(
   const bool vq = q < n_q;
   bf16x8_t qf[8];
   {
+    // the row pointer is clamped into the query array, so all 16 loads are
+    // issued unconditionally (in flight together) and rows past the query's
+    // tokens are zeroed afterwards
     const bool vr = vq && col < t_max;
-    const float *qr = Q + ((size_t)(vq ? q : 0) * t_max + (vr ? col : 0)) * kDim + 64 * h;
+    const float4 *qr =
+        reinterpret_cast<const float4 *>(Q + ((size_t)(vq ? q : 0) * t_max + (vr ? col : 0)) * kDim + 64 * h);
+    float4 x[16];
+#pragma unroll
+    for (uint32_t i = 0; i < 16; ++i) x[i] = qr[i];
 #pragma unroll
     for (uint32_t s = 0; s < 8; ++s) {
+      const float4 a = x[2 * s], b = x[2 * s + 1];
       bf16x8_t v;
-#pragma unroll
-      for (uint32_t e = 0; e < 8; ++e) v[e] = (__bf16)(vr ? qr[8 * s + e] : 0.0f);
+      v[0] = (__bf16)(vr ? a.x : 0.0f); v[1] = (__bf16)(vr ? a.y : 0.0f);
+      v[2] = (__bf16)(vr ? a.z : 0.0f); v[3] = (__bf16)(vr ? a.w : 0.0f);
+      v[4] = (__bf16)(vr ? b.x : 0.0f); v[5] = (__bf16)(vr ? b.y : 0.0f);
+      v[6] = (__bf16)(vr ? b.z : 0.0f); v[7] = (__bf16)(vr ? b.w : 0.0f);
       qf[s] = v;
     }
   }
