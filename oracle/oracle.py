@@ -318,10 +318,16 @@ def exhaustive_f64(Q, X, doclens, k: int, q_ntok=None, n_threads=0):
 
 # ----------------------------------------------------- synthetic workloads
 def bf16_round(x: np.ndarray) -> np.ndarray:
-    """Round float32 to bfloat16 (nearest even), returned as float32."""
-    b = np.ascontiguousarray(x, np.float32).view(np.uint32).astype(np.uint64)
-    b = (b + 0x7FFF + ((b >> 16) & 1)) >> 16 << 16
-    return b.astype(np.uint32).view(np.float32)
+    """Round float32 to bfloat16 (nearest even), returned as float32.  NaN
+    stays NaN (quietened); finite values past the largest bfloat16 round to
+    +-inf, as the hardware conversion does."""
+    x32 = np.ascontiguousarray(x, np.float32)
+    b = x32.view(np.uint32).astype(np.uint64)
+    b = ((b + 0x7FFF + ((b >> 16) & 1)) >> 16 << 16) & 0xFFFFFFFF
+    out = b.astype(np.uint32)
+    nan = np.isnan(x32)
+    out[nan] = (x32.view(np.uint32)[nan] & 0xFFFF0000) | 0x00400000
+    return out.view(np.float32)
 
 
 def e4m3_round(x: np.ndarray) -> np.ndarray:
