@@ -15,7 +15,7 @@
  *   mvr_search
  *       IGPGPUPP::search  src/impl/IGPGPUPP.hpp:253-332, i.e. per query
  *       GPGPURetrievalPP::search  src/include/alg/igp_gpu_pp/IGPGPURetrievalPP.hpp:114-176
- *       (centroid scoring GenerateVQScoreCandidatePP.hpp:305-366, filter
+ *       (centroid scoring GenerateVQScoreCandidatePP.hpp:118-179, filter
  *        FilterPP.hpp:267-398, rerank RerankPP.hpp:271-475, host top-k
  *        IGPGPURetrievalPP.hpp:153-161).  With clip_zero = 1 it is the basic
  *       variant's search, src/impl/IGPGPUBasic.hpp:190-310.
@@ -76,12 +76,17 @@ typedef struct {
 } mvr_index_desc;
 
 typedef struct {
-  uint32_t nprobe;      /* centroids probed per query token (-nprobe) */
+  uint32_t nprobe;      /* centroids probed per query token (-nprobe), at most
+                           min(n_centroid, 256); larger is MVR_ERR_UNSUPPORTED */
   uint32_t probe_topk;  /* docs kept by the filter and reranked (-probe_topk) */
   uint32_t topk;        /* results per query (-topk) */
   uint32_t precision;   /* mvr_precision of centroid scoring and MaxSim */
   uint32_t rescore;     /* BF16/FP8: FP32 rescoring of the best `rescore` docs (0 = off) */
-  uint32_t clip_zero;   /* 1: MaxSim per token is max(0, .) (basic variant) */
+  uint32_t clip_zero;   /* 0: IGP-PP semantics: the filter's per-token maxima are
+                           max(0, .) (FilterPP.hpp:136-141,257-259), MaxSim per token
+                           has identity -FLT_MAX.  1: the basic variant: the filter
+                           adds each token's first hit as it is (igp_gpu_basic/
+                           Filter.hpp:44-59), MaxSim per token is max(0, .) */
   uint32_t batch;       /* queries per device pass (0 = 1024) */
   uint32_t learned;     /* 1: per-query nprobe/probe_topk from the attached probe model */
   uint32_t persistent;  /* 1: MaxSim (and the rescore) + top-k as one persistent work-queue
@@ -193,7 +198,7 @@ int mvr_exhaustive_f64(const float *Q_dev, uint32_t n_q, uint32_t t_max, const u
                        double *out_dev, void *stream);
 
 /* ---- learned probe (no counterpart in the reference, which probes a fixed
- * nprobe: GenerateVQScoreCandidatePP.hpp:353-361) ----
+ * nprobe: GenerateVQScoreCandidatePP.hpp:166-174) ----
  * A 3-layer MLP (dims {1024, d1, d2, 6}, GELU) reads a query's top-32
  * centroid scores per token and predicts its nprobe class over {2,4,8,16,32}
  * (outputs 0..4) and log(probe_topk) (output 5).  Trained with Adam on the

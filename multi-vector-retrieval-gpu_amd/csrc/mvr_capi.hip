@@ -471,6 +471,25 @@ int index_create(const mvr_index_desc *desc, int device, uint32_t flags, const C
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d.lut_fp8), (size_t)kResBytes * 256 * 4);
   if (e == hipSuccess)
     e = hipMalloc(reinterpret_cast<void **>(&d.seg_off), (size_t)d.n_c * (d.n_seg + 1) * sizeof(uint32_t));
+  {
+    // codes in range and IVF lists ascending within [0, n_doc): checked on the
+    // device for every descriptor (host or device arrays); the codes of a
+    // host-resident index were checked on the host above
+    unsigned long long *bad = nullptr, bad_h[2] = {0, 0};
+    if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&bad), 16);
+    if (e == hipSuccess) {
+      DevIndex dv = d;
+      if (ix->res_host) dv.codes = nullptr;
+      e = launch_validate_index(dv, bad, nullptr);
+    }
+    if (e == hipSuccess) e = hipMemcpy(bad_h, bad, 16, hipMemcpyDeviceToHost);
+    if (bad) (void)hipFree(bad);
+    if (e == hipSuccess && bad_h[0])
+      return bail(fail(MVR_ERR_INVALID, "%llu codes are >= n_centroid %u", bad_h[0], d.n_c));
+    if (e == hipSuccess && bad_h[1])
+      return bail(fail(MVR_ERR_INVALID, "%llu IVF entries are >= n_doc %u or not strictly ascending in their list",
+                       bad_h[1], d.n_doc));
+  }
   if (e == hipSuccess) e = launch_seg_offsets(d, nullptr);
   if (e == hipSuccess) e = launch_convert_centroids(d, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
@@ -712,7 +731,6 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     const char *e = std::getenv("MVR_CT_SCAN");
     return e && e[0] == '1';
   }();
-  const bool fast_ct = !force_scan;
   // BF16/FP8: the best `rescore` docs by the low-precision MaxSim are scored
   // again in FP32 (on MFMA) and the final top-k comes from those FP32 scores
   const uint32_t R2 = (!fp32_mode && prm->rescore) ? std::min(prm->rescore, R) : 0;
@@ -723,7 +741,8 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   for (uint32_t q = 0; q < n_q; ++q) {
     np[q] = q_nprobe ? q_nprobe[q] : prm->nprobe;
     nt[q] = q_ntok ? q_ntok[q] : t_max;
-    if (np[q] == 0 || np[q] > kMaxProbe) return fail(MVR_ERR_INVALID, "nprobe %u not in [1, 32]", np[q]);
+    if (np[q] == 0) return fail(MVR_ERR_INVALID, "nprobe must be positive");
+    if (np[q] > kMaxProbe) return fail(MVR_ERR_UNSUPPORTED, "nprobe %u > %u", np[q], kMaxProbe);
     if (np[q] > d.n_c) return fail(MVR_ERR_INVALID, "nprobe %u is larger than n_centroid %u", np[q], d.n_c);
     if (nt[q] == 0 || nt[q] > t_max) return fail(MVR_ERR_INVALID, "query %u has %u tokens (t_max %u)", q, nt[q], t_max);
     P = std::max(P, np[q]);
@@ -733,11 +752,12 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     if (!ix->probe) return fail(MVR_ERR_INVALID, "learned probing needs a model (mvr_index_set_probe)");
     if (q_nprobe) return fail(MVR_ERR_INVALID, "learned probing chooses nprobe; pass q_nprobe = NULL");
     // the model reads the top-32 scores of every token and may pick up to 32 probes
-    P = kMaxProbe;
-    for (uint32_t q = 0; q < n_q; ++q) np[q] = kMaxProbe;
+    P = kProbeFeat;
+    for (uint32_t q = 0; q < n_q; ++q) np[q] = kProbeFeat;
   }
   P = std::min(P, d.n_c);
-  if (learned && P < kMaxProbe) return fail(MVR_ERR_INVALID, "learned probing needs >= 32 centroids");
+  if (learned && P < kProbeFeat) return fail(MVR_ERR_INVALID, "learned probing needs >= 32 centroids");
+  const bool fast_ct = !force_scan || P > 64;  // the all-FP32 scan keeps 64 per row
   // queries per device pass; the candidate pool holds a 4,096-slot region
   // per (query, doc segment), so a pass is also capped by the pool budget
   const uint64_t pool_per_q = (uint64_t)d.n_seg * kSeg * 8;
@@ -912,7 +932,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       q_r = ws.dyn_r.as<uint32_t>();
     }
     HIP_TRY(launch_filter(ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), t_max, P, bnt, bnp, nb,
-                          d, ws.cand_cnt.as<uint32_t>(), ws.cand_keys.as<uint64_t>(), st));
+                          d, prm->clip_zero ? 0 : 1, ws.cand_cnt.as<uint32_t>(), ws.cand_keys.as<uint64_t>(), st));
     HIP_TRY(mark(2));
     HIP_TRY(launch_select(ws.cand_keys.as<uint64_t>(), ws.cand_cnt.as<uint32_t>(), d.n_seg, q_r, nb, R,
                           sl.ids, sl.F, sl.n, sl.tot, st));
@@ -1331,7 +1351,7 @@ int mvr_rerank_docs(mvr_index *ix, const float *Q_dev, uint32_t n_q, uint32_t t_
 int mvr_index_set_probe(mvr_index *ix, mvr_mlp *mlp, float conf, uint32_t fallback_nprobe,
                         uint32_t r_min) {
   if (!ix) return fail(MVR_ERR_INVALID, "NULL index");
-  if (mlp && (fallback_nprobe == 0 || fallback_nprobe > kMaxProbe))
+  if (mlp && (fallback_nprobe == 0 || fallback_nprobe > kProbeFeat))
     return fail(MVR_ERR_INVALID, "fallback nprobe %u not in [1, 32]", fallback_nprobe);
   ix->probe = mlp;
   ix->probe_conf = conf;

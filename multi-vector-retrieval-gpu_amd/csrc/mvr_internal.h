@@ -11,7 +11,8 @@ namespace mvr {
 constexpr uint32_t kDim = 128;             // embedding width
 constexpr uint32_t kResBytes = 32;         // packed 2-bit residual bytes per token
 constexpr uint32_t kMaxQTok = 32;          // query tokens (bit mask width in the filter)
-constexpr uint32_t kMaxProbe = 32;         // largest nprobe
+constexpr uint32_t kMaxProbe = 256;        // largest nprobe
+constexpr uint32_t kProbeFeat = 32;        // centroid scores per token read by the learned probe (its largest nprobe)
 constexpr uint32_t kSeg = 4096;            // docs per filter segment (LDS state)
 constexpr uint64_t kPoolBudget = 16ull << 30;  // candidate pool bytes per device pass (default)
 constexpr uint64_t kPoolBudgetMin = 2ull << 30, kPoolBudgetMax = 64ull << 30;
@@ -44,6 +45,9 @@ struct DevIndex {
 
 // ---- launchers (return hipError_t of the launch) ----
 hipError_t launch_seg_offsets(const DevIndex &ix, hipStream_t s);
+// bad[0] = codes >= n_c (ix.codes may be null: skipped), bad[1] = IVF entries
+// >= n_doc or not strictly ascending in their list
+hipError_t launch_validate_index(const DevIndex &ix, unsigned long long *bad, hipStream_t s);
 hipError_t launch_convert_centroids(const DevIndex &ix, hipStream_t s);
 
 // centroid scoring + top-P.  rows: n_rows query-token rows; row_map (nullable)
@@ -65,9 +69,12 @@ hipError_t launch_centroid_topk_bf16(const float *Q, uint32_t n_q, uint32_t t_ma
 
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 
+// clamp0: per-(doc, token) maxima are max(0, .) (the PP filter); else the
+// first hit's score as it is (the basic variant)
 hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t_max, uint32_t P,
                          const uint32_t *q_ntok, const uint32_t *q_np, uint32_t n_q,
-                         const DevIndex &ix, uint32_t *seg_cnt, uint64_t *cand_keys, hipStream_t s);
+                         const DevIndex &ix, int clamp0, uint32_t *seg_cnt, uint64_t *cand_keys,
+                         hipStream_t s);
 
 // top-R per query (q_r, nullable: per-query R <= R; R is the output row stride)
 // over the per-(query, segment) candidate regions; cand_tot[q] receives the

@@ -265,37 +265,118 @@ constexpr uint32_t kFiltGroup = 8;                    // segments per workgroup
 constexpr uint32_t kFiltHold = 1;                     // list entries per lane and rank loaded ahead
 constexpr uint32_t kFtLanes = 16;                     // lanes per query token
 constexpr uint32_t kFtThreads = kMaxQTok * kFtLanes;  // threads per workgroup
-constexpr uint32_t kFtRanks = kMaxProbe;              // probe ranks per query at most
 constexpr uint32_t kFtBatch = 8;                      // ranks per load batch
+constexpr uint32_t kFtTokMax = 64;                    // most ranks of the token-major kernel
 
+// Run-table slot of (rank r, token t): rows of 32 tokens, columns swizzled by
+// the rank so one rank's writes spread over the banks.
 __device__ __forceinline__ uint32_t ft_slot(uint32_t r, uint32_t t) {
-  return r * kMaxQTok + (t ^ r);  // r < 32, t < 32
+  return r * kMaxQTok + (t ^ (r & 31u));
 }
 
+// Rank order inside a wave: a token's ranks are applied by lanes of one wave,
+// one rank after the other, and the first rank whose list holds a doc must
+// set the doc's token bit before the next rank tests it.  The LDS executes a
+// wave's operations in issue order; this fence keeps the compiler from
+// moving one rank's LDS atomics past the next rank's.
+__device__ __forceinline__ void rank_order_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+}
+
+// Fixed-point score of a probed list: S' * 2^32.  The PP filter (clamp0)
+// takes max(0, .) per (doc, token), as the reference's zero-filled maxima
+// with atomicMax do (FilterPP.hpp:136-141, memset at :257-259); the basic
+// variant adds the first hit as it is (igp_gpu_basic/Filter.hpp:44-59).
+__device__ __forceinline__ long long filter_fx(float s, int clamp0) {
+  const long long fx = (long long)((double)s * kFxScale);
+  return (clamp0 && fx < 0) ? 0 : fx;
+}
+
+// Emission of one segment: slot k of every lane in turn; a wave's hits
+// (slot-major, then lane) are compacted into a list of doc offsets (hl, kSeg
+// 16-bit entries per workgroup), so the conversion and key packing run once
+// per hit rather than once per slot, and written as one contiguous run, waves
+// in order.  wsum holds the per-wave counts.  The caller's barrier separates
+// the last LDS update of the segment from this; the emission ends with a
+// barrier and leaves mask and acc cleared.
+__device__ __forceinline__ void filter_emit(uint32_t *mask, unsigned long long *acc, uint32_t *wsum,
+                                            uint16_t *hl_all, uint32_t q, uint32_t sg, uint32_t n_seg,
+                                            uint32_t n_doc, uint32_t *__restrict__ seg_cnt,
+                                            uint64_t *__restrict__ cand_keys) {
+  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
+  const uint32_t seg_lo = sg * kSeg;
+  uint16_t *hl = hl_all + wv * (kSeg / (kFtThreads / kWave));
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+  const bool tail = seg_lo + kSeg > n_doc;  // the last segment, partly past the corpus
+  uint32_t wcount = 0;  // hits of this wave so far (uniform)
+#pragma unroll
+  for (uint32_t k = 0; k < kSeg / kFtThreads; ++k) {
+    const uint32_t dl = lane + kWave * (wv + (kFtThreads / kWave) * k);
+    bool hit = false;
+    if (mask[dl] != 0u) {
+      mask[dl] = 0u;
+      hit = !tail || seg_lo + dl < n_doc;
+      if (!hit) acc[dl] = 0ull;
+    }
+    const uint64_t bb = __ballot(hit);
+    if (hit) hl[wcount + __popcll(bb & lt_mask)] = (uint16_t)dl;
+    wcount += (uint32_t)__popcll(bb);
+  }
+  if (lane == 0) wsum[wv] = wcount;
+  __syncthreads();
+  uint32_t pos = 0;
+  for (uint32_t w = 0; w < wv; ++w) pos += wsum[w];
+  if (tid == 0) {
+    uint32_t tot = 0;
+    for (uint32_t w = 0; w < kFtThreads / kWave; ++w) tot += wsum[w];
+    seg_cnt[(size_t)q * n_seg + sg] = tot;
+  }
+  uint64_t *out = cand_keys + ((size_t)q * n_seg + sg) * kSeg + pos;
+  for (uint32_t i = lane; i < wcount; i += kWave) {
+    const uint32_t dl = hl[i];
+    const float F = (float)((double)(long long)acc[dl] * (1.0 / kFxScale));
+    acc[dl] = 0ull;
+    out[i] = make_key(F, seg_lo + dl);
+  }
+  __syncthreads();  // wsum and the cleared state before the next segment
+}
+
+// The XCD-aware workgroup order of the filter kernels: consecutive ids go to
+// the 8 XCDs in turn, so id b is remapped to give each XCD a contiguous run of
+// (query, segment group) tiles.
+__device__ __forceinline__ uint32_t filter_tile(uint32_t n_wg) {
+  const uint32_t per_xcd = (n_wg + 7) / 8;
+  return (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+}
+
+// Occupancy: two workgroups (4 waves per SIMD) with 32-rank tables; the
+// 64-rank tables leave room for one.
 template <uint32_t NS>
-__global__ __launch_bounds__(kFtThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_filter_tok(
+__global__ __launch_bounds__(kFtThreads) __attribute__((amdgpu_waves_per_eu(NS <= 2 ? 4 : 2))) void k_filter_tok(
     const uint32_t *__restrict__ top_ids, const float *__restrict__ top_s, uint32_t t_max,
     uint32_t P, const uint32_t *__restrict__ q_ntok, const uint32_t *__restrict__ q_np,
     const uint64_t *__restrict__ ivf_off, const uint32_t *__restrict__ ivf_ids,
-    const uint32_t *__restrict__ seg_off, uint32_t n_seg, uint32_t n_doc, uint32_t n_q,
+    const uint32_t *__restrict__ seg_off, uint32_t n_seg, uint32_t n_doc, uint32_t n_q, int clamp0,
     uint32_t *__restrict__ seg_cnt, uint64_t *__restrict__ cand_keys) {
-  static_assert(NS * kFtLanes <= kFtRanks, "ranks per lane");
+  // run tables for kR ranks (32, or 64 when a lane holds four ranks)
+  constexpr uint32_t kR = NS * kFtLanes > 32 ? NS * kFtLanes : 32;
+  static_assert(NS * kFtLanes <= kFtTokMax, "ranks per lane");
+  static_assert(kR * kMaxQTok * 8 >= kSeg * 2, "the emission's hit list lives in a run table");
   __shared__ uint32_t mask[kSeg];
   __shared__ unsigned long long acc[kSeg];
-  __shared__ uint64_t tb_st[2][kFtRanks * kMaxQTok];
-  __shared__ uint32_t tb_len[2][kFtRanks * kMaxQTok];
-  __shared__ long long tb_fx[kFtRanks * kMaxQTok];
+  __shared__ uint64_t tb_st[2][kR * kMaxQTok];
+  __shared__ uint32_t tb_len[2][kR * kMaxQTok];
+  __shared__ long long tb_fx[kR * kMaxQTok];
 
   const uint32_t n_grp = (n_seg + kFiltGroup - 1) / kFiltGroup;
   const uint32_t n_wg = n_grp * n_q;
-  const uint32_t per_xcd = (n_wg + 7) / 8;
-  const uint32_t L = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+  const uint32_t L = filter_tile(n_wg);
   if (L >= n_wg) return;  // uniform for the whole workgroup
   const uint32_t q = L / n_grp, grp = L - q * n_grp;
   const uint32_t sg0 = grp * kFiltGroup, sg1 = min(n_seg, sg0 + kFiltGroup);
   const uint32_t T = q_ntok ? q_ntok[q] : t_max;
   const uint32_t Pq = q_np ? q_np[q] : P;
-  const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
+  const uint32_t tid = threadIdx.x;
   const uint32_t t = tid / kFtLanes, sub = tid % kFtLanes;
   const uint32_t bit = 1u << t;
   const uint32_t nr = min(Pq, kFtLanes * NS);
@@ -318,7 +399,7 @@ __global__ __launch_bounds__(kFtThreads) __attribute__((amdgpu_waves_per_eu(4)))
       if (c != 0xFFFFFFFFu) {
         so[s] = seg_off + (size_t)c * (n_seg + 1);
         base[s] = ivf_off[c];
-        fx = (long long)((double)top_s[o] * kFxScale);
+        fx = filter_fx(top_s[o], clamp0);
       }
     }
     tb_fx[ft_slot(j, t)] = fx;
@@ -361,7 +442,6 @@ __global__ __launch_bounds__(kFtThreads) __attribute__((amdgpu_waves_per_eu(4)))
 
   for (uint32_t sg = sg0; sg < sg1; ++sg) {
     const uint32_t b = (sg - sg0) & 1u;
-    const uint32_t seg_lo = sg * kSeg;
     for (uint32_t ib = 0; ib < nb; ++ib) {
       // the next batch's first entries, in flight while this batch is applied
       uint32_t ent_n[kFtBatch][kFiltHold], len_n[kFtBatch];
@@ -394,7 +474,7 @@ __global__ __launch_bounds__(kFtThreads) __attribute__((amdgpu_waves_per_eu(4)))
 #pragma unroll
           for (uint32_t k = 0; k < kFiltHold; ++k) {
             if (ent[u][k] != 0xFFFFFFFFu) {
-              const uint32_t dl = ent[u][k] & (kSeg - 1);  // seg_lo is a multiple of kSeg
+              const uint32_t dl = ent[u][k] & (kSeg - 1);  // segments start at multiples of kSeg
               const uint32_t old = atomicOr(&mask[dl], bit);
               if (!(old & bit)) atomicAdd(&acc[dl], add);
             }
@@ -409,6 +489,7 @@ __global__ __launch_bounds__(kFtThreads) __attribute__((amdgpu_waves_per_eu(4)))
             }
           }
         }
+        rank_order_fence();
       }
 #pragma unroll
       for (uint32_t u = 0; u < kFtBatch; ++u) {
@@ -418,48 +499,91 @@ __global__ __launch_bounds__(kFtThreads) __attribute__((amdgpu_waves_per_eu(4)))
       }
     }
     __syncthreads();
+    // the emission's per-wave counts and hit lists go to this segment's run
+    // tables, whose last reader was the loop above and whose next writer runs
+    // after the emission's closing barrier
+    filter_emit(mask, acc, tb_len[b], reinterpret_cast<uint16_t *>(tb_st[b]), q, sg, n_seg, n_doc, seg_cnt,
+                cand_keys);
+  }
+}
 
-    // Emission: slot k of every lane in turn; a wave's hits (slot-major, then
-    // lane) are compacted into a list of doc offsets, so the conversion and
-    // key packing run once per hit rather than once per slot, and written as
-    // one contiguous run, waves in order.  The per-wave counts and lists go
-    // to this segment's run tables, whose last reader was the loop above and
-    // whose next writer runs after the closing barrier.
-    uint32_t *wsum = tb_len[b];
-    uint16_t *hl = reinterpret_cast<uint16_t *>(tb_st[b]) + wv * (kSeg / (kFtThreads / kWave));
-    const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
-    const bool tail = seg_lo + kSeg > n_doc;  // the last segment, partly past the corpus
-    uint32_t wcount = 0;  // hits of this wave so far (uniform)
+// Filter for nprobe above the token-major kernel's 64 ranks.  Same LDS state,
+// grid and emission; the ranks go in groups of 32: per segment and group, the
+// 16 lanes of token t resolve two ranks each (centroid, score, the list's run
+// in the segment from seg_off) into a run table, then apply the group's ranks
+// one after the other, the lanes striding over each run.  Groups and ranks
+// keep their order, so the first hit of a doc is still the token's maximum.
+__global__ __launch_bounds__(kFtThreads) void k_filter_wide(
+    const uint32_t *__restrict__ top_ids, const float *__restrict__ top_s, uint32_t t_max,
+    uint32_t P, const uint32_t *__restrict__ q_ntok, const uint32_t *__restrict__ q_np,
+    const uint64_t *__restrict__ ivf_off, const uint32_t *__restrict__ ivf_ids,
+    const uint32_t *__restrict__ seg_off, uint32_t n_seg, uint32_t n_doc, uint32_t n_q, int clamp0,
+    uint32_t *__restrict__ seg_cnt, uint64_t *__restrict__ cand_keys) {
+  constexpr uint32_t kG = 2 * kFtLanes;  // ranks per group
+  __shared__ uint32_t mask[kSeg];
+  __shared__ unsigned long long acc[kSeg];
+  __shared__ uint64_t tb_st[kG * kMaxQTok];  // also the emission's hit list
+  __shared__ uint32_t tb_len[kG * kMaxQTok];
+  __shared__ long long tb_fx[kG * kMaxQTok];
+  __shared__ uint32_t wsum[kFtThreads / kWave];
+  static_assert(kG * kMaxQTok * 8 >= kSeg * 2, "hit list size");
+
+  const uint32_t n_grp = (n_seg + kFiltGroup - 1) / kFiltGroup;
+  const uint32_t n_wg = n_grp * n_q;
+  const uint32_t L = filter_tile(n_wg);
+  if (L >= n_wg) return;
+  const uint32_t q = L / n_grp, grp = L - q * n_grp;
+  const uint32_t sg0 = grp * kFiltGroup, sg1 = min(n_seg, sg0 + kFiltGroup);
+  const uint32_t T = q_ntok ? q_ntok[q] : t_max;
+  const uint32_t nr = min(q_np ? q_np[q] : P, P);
+  const uint32_t tid = threadIdx.x;
+  const uint32_t t = tid / kFtLanes, sub = tid % kFtLanes;
+  const uint32_t bit = 1u << t;
+
+  for (uint32_t i = tid; i < kSeg; i += kFtThreads) { mask[i] = 0u; acc[i] = 0ull; }
+  __syncthreads();
+  for (uint32_t sg = sg0; sg < sg1; ++sg) {
+    for (uint32_t g0 = 0; g0 < nr; g0 += kG) {
 #pragma unroll
-    for (uint32_t k = 0; k < kSeg / kFtThreads; ++k) {
-      const uint32_t dl = lane + kWave * (wv + (kFtThreads / kWave) * k);
-      bool hit = false;
-      if (mask[dl] != 0u) {
-        mask[dl] = 0u;
-        hit = !tail || seg_lo + dl < n_doc;
-        if (!hit) acc[dl] = 0ull;
+      for (uint32_t s = 0; s < 2; ++s) {
+        const uint32_t r = sub + kFtLanes * s, j = g0 + r;
+        uint64_t st = 0;
+        uint32_t len = 0;
+        long long fx = 0;
+        if (j < nr && t < T) {
+          const size_t o = ((size_t)q * t_max + t) * P + j;
+          const uint32_t c = top_ids[o];
+          if (c != 0xFFFFFFFFu) {
+            const uint32_t *so = seg_off + (size_t)c * (n_seg + 1);
+            const uint32_t lo = so[sg];
+            st = ivf_off[c] + lo;
+            len = so[sg + 1] - lo;
+            fx = filter_fx(top_s[o], clamp0);
+          }
+        }
+        tb_st[ft_slot(r, t)] = st;
+        tb_len[ft_slot(r, t)] = len;
+        tb_fx[ft_slot(r, t)] = fx;
       }
-      const uint64_t bb = __ballot(hit);
-      if (hit) hl[wcount + __popcll(bb & lt_mask)] = (uint16_t)dl;
-      wcount += (uint32_t)__popcll(bb);
+      wave_lds_sync();  // the token's 16 lanes read each other's table entries
+      const uint32_t ng = min(kG, nr - g0);
+      for (uint32_t r = 0; r < ng; ++r) {
+        const uint32_t l = tb_len[ft_slot(r, t)];
+        if (l > sub) {
+          const uint64_t st = tb_st[ft_slot(r, t)];
+          const unsigned long long add = (unsigned long long)tb_fx[ft_slot(r, t)];
+          for (uint32_t i = sub; i < l; i += kFtLanes) {
+            const uint32_t dl = ivf_ids[st + i] & (kSeg - 1);
+            const uint32_t old = atomicOr(&mask[dl], bit);
+            if (!(old & bit)) atomicAdd(&acc[dl], add);
+          }
+        }
+        rank_order_fence();
+      }
+      wave_lds_sync();  // table reads done before the next group rewrites them
     }
-    if (lane == 0) wsum[wv] = wcount;
     __syncthreads();
-    uint32_t pos = 0;
-    for (uint32_t w = 0; w < wv; ++w) pos += wsum[w];
-    if (tid == 0) {
-      uint32_t tot = 0;
-      for (uint32_t w = 0; w < kFtThreads / kWave; ++w) tot += wsum[w];
-      seg_cnt[(size_t)q * n_seg + sg] = tot;
-    }
-    uint64_t *out = cand_keys + ((size_t)q * n_seg + sg) * kSeg + pos;
-    for (uint32_t i = lane; i < wcount; i += kWave) {
-      const uint32_t dl = hl[i];
-      const float F = (float)((double)(long long)acc[dl] * (1.0 / kFxScale));
-      acc[dl] = 0ull;
-      out[i] = make_key(F, seg_lo + dl);
-    }
-    __syncthreads();  // wsum and the cleared state before the next segment
+    filter_emit(mask, acc, wsum, reinterpret_cast<uint16_t *>(tb_st), q, sg, n_seg, n_doc, seg_cnt, cand_keys);
   }
 }
 
@@ -2000,28 +2124,68 @@ __global__ __launch_bounds__(512) void k_ct_bf16_blockmax(
 // scored exactly -- one sequential FP32 FMA chain over d, on FP32 inputs
 // (exact_f32) or bf16-rounded inputs (the BF16 mode's definition) -- and the
 // top-P by (score desc, id asc) is kept.  Output S' = score + 1.
-__device__ __forceinline__ void wave_list_insert(float &ls, uint32_t &li, uint32_t lane, uint32_t P,
+// A wave's sorted list of up to 64 J (score, id) pairs, best first by (score
+// desc, id asc): element e = 64 j + lane sits in register j of lane `lane`.
+// Only elements e < P are meaningful.
+template <int J>
+__device__ __forceinline__ void wave_list_tail(const float (&ls)[J], const uint32_t (&li)[J], uint32_t P,
+                                               float &ts, uint32_t &ti) {
+  const uint32_t jp = (P - 1) >> 6, lp = (P - 1) & 63u;
+  ts = -INFINITY;
+  ti = 0xFFFFFFFFu;
+#pragma unroll
+  for (int j = 0; j < J; ++j)
+    if ((uint32_t)j == jp) {
+      ts = readlane_f(ls[j], lp);
+      ti = readlane_u(li[j], lp);
+    }
+}
+
+// Inserts every lane's valid candidate (s, id) that beats element P-1, one
+// at a time (lowest lane first): its position is the number of better
+// elements, and the elements from there on move up by one (lane 0 of
+// register j takes lane 63 of register j-1).
+template <int J>
+__device__ __forceinline__ void wave_list_insert(float (&ls)[J], uint32_t (&li)[J], uint32_t lane, uint32_t P,
                                                  float s, uint32_t id, bool valid) {
-  float ts = readlane_f(ls, P - 1);
-  uint32_t ti = readlane_u(li, P - 1);
+  float ts;
+  uint32_t ti;
+  wave_list_tail<J>(ls, li, P, ts, ti);
   uint64_t bal = __ballot(valid && better(s, id, ts, ti));
   while (bal) {
     const uint32_t l = (uint32_t)__builtin_ctzll(bal);
     const float cs = readlane_f(s, l);
     const uint32_t ci = readlane_u(id, l);
-    const uint32_t pos = (uint32_t)__popcll(__ballot(lane < P && better(ls, li, cs, ci)));
-    const float us = __shfl_up(ls, 1);
-    const uint32_t ui = __shfl_up(li, 1);
-    if (lane > pos) { ls = us; li = ui; }
-    else if (lane == pos) { ls = cs; li = ci; }
+    uint32_t pos = 0;
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+      pos += (uint32_t)__popcll(__ballot(64u * j + lane < P && better(ls[j], li[j], cs, ci)));
+    float carry_s = -INFINITY;
+    uint32_t carry_i = 0xFFFFFFFFu;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const float us = __shfl_up(ls[j], 1);
+      const uint32_t ui = __shfl_up(li[j], 1);
+      const float top_s = readlane_f(ls[j], 63);
+      const uint32_t top_i = readlane_u(li[j], 63);
+      const uint32_t e = 64u * j + lane;
+      if (e > pos) {
+        ls[j] = lane == 0 ? carry_s : us;
+        li[j] = lane == 0 ? carry_i : ui;
+      } else if (e == pos) {
+        ls[j] = cs;
+        li[j] = ci;
+      }
+      carry_s = top_s;
+      carry_i = top_i;
+    }
     bal &= bal - 1;
-    ts = readlane_f(ls, P - 1);
-    ti = readlane_u(li, P - 1);
+    wave_list_tail<J>(ls, li, P, ts, ti);
     bal &= __ballot(valid && better(s, id, ts, ti));
   }
 }
 
-template <bool kF32>
+template <bool kF32, int J>
 __global__ __launch_bounds__(256) void k_ct_select(
     const float *__restrict__ Q, const uint32_t *__restrict__ row_map, uint32_t n_rows,
     const float *__restrict__ C, const uint16_t *__restrict__ Cb, uint32_t n_c, uint32_t n_tiles,
@@ -2045,19 +2209,26 @@ __global__ __launch_bounds__(256) void k_ct_select(
   wave_lds_sync();
   // A_P: the P-th largest approximate tile maximum
   const uint32_t P1 = min(P, n_tiles);
-  float ls = -INFINITY;
-  uint32_t li = 0xFFFFFFFFu;
+  float ls[J];
+  uint32_t li[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) { ls[j] = -INFINITY; li[j] = 0xFFFFFFFFu; }
   const float *bm = bmax + qrow * n_tiles;
   for (uint32_t base = 0; base < n_tiles; base += 64) {
     const uint32_t tl = base + lane;
     const bool v = tl < n_tiles;
-    wave_list_insert(ls, li, lane, P1, v ? bm[tl] : -INFINITY, tl, v);
+    wave_list_insert<J>(ls, li, lane, P1, v ? bm[tl] : -INFINITY, tl, v);
   }
+  float a_p;
+  uint32_t a_pi;
+  wave_list_tail<J>(ls, li, P1, a_p, a_pi);
   // 1.01: slack for rounding in the bound itself
-  const float thr = readlane_f(ls, P1 - 1) - 2.02f * margin * sqrtf(nn) * c_norm_max;
+  const float thr = a_p - 2.02f * margin * sqrtf(nn) * c_norm_max;
   // exact scores of every centroid in a tile reaching thr, two tiles per round
-  float fs = -INFINITY;
-  uint32_t fi = 0xFFFFFFFFu;
+  float fs[J];
+  uint32_t fi[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) { fs[j] = -INFINITY; fi[j] = 0xFFFFFFFFu; }
   for (uint32_t base = 0; base < n_tiles; base += 64) {
     const uint32_t tl = base + lane;
     uint64_t bal = __ballot(tl < n_tiles && bm[tl] >= thr);
@@ -2116,12 +2287,16 @@ __global__ __launch_bounds__(256) void k_ct_select(
           }
         }
       }
-      wave_list_insert(fs, fi, lane, P, a + 1.0f, c, keep);
+      wave_list_insert<J>(fs, fi, lane, P, a + 1.0f, c, keep);
     }
   }
-  if (lane < P) {
-    top_ids[qrow * P + lane] = fi;
-    top_s[qrow * P + lane] = fs;
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    const uint32_t e = 64u * j + lane;
+    if (e < P) {
+      top_ids[qrow * P + e] = fi[j];
+      top_s[qrow * P + e] = fs[j];
+    }
   }
 }
 
@@ -2245,6 +2420,41 @@ __global__ void k_seg_offsets(const uint64_t *__restrict__ ivf_off,
     if ((uint64_t)lst[mid] < key) lo = mid + 1; else hi = mid;
   }
   seg_off[g] = lo;
+}
+
+// Index checks every kernel relies on: bad[0] counts codes >= n_c (a code is
+// a centroid row index), bad[1] the IVF entries that are >= n_doc or not above
+// their list predecessor (the filter's segment runs and the seg_off binary
+// search need ascending lists).  One wave per list; codes grid-strided.
+__global__ __launch_bounds__(256) void k_validate_index(const uint32_t *__restrict__ codes, uint64_t n_tok,
+                                                        uint32_t n_c, const uint64_t *__restrict__ ivf_off,
+                                                        const uint32_t *__restrict__ ivf_size,
+                                                        const uint32_t *__restrict__ ivf_ids, uint32_t n_doc,
+                                                        unsigned long long *__restrict__ bad) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t nthr = (uint64_t)gridDim.x * blockDim.x;
+  uint32_t nb_code = 0, nb_ivf = 0;
+  if (codes)
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_tok; i += nthr)
+      nb_code += codes[i] >= n_c;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, n_wave = (uint32_t)(nthr >> 6);
+  for (uint32_t c = wave; c < n_c; c += n_wave) {
+    const uint32_t *lst = ivf_ids + ivf_off[c];
+    const uint32_t sz = ivf_size[c];
+    for (uint32_t i = lane; i < sz; i += 64) {
+      const uint32_t d = lst[i];
+      nb_ivf += (d >= n_doc) || (i > 0 && lst[i - 1] >= d);
+    }
+  }
+#pragma unroll
+  for (uint32_t off = 32; off > 0; off >>= 1) {
+    nb_code += __shfl_xor(nb_code, off);
+    nb_ivf += __shfl_xor(nb_ivf, off);
+  }
+  if (lane == 0 && (nb_code | nb_ivf)) {
+    atomicAdd(&bad[0], (unsigned long long)nb_code);
+    atomicAdd(&bad[1], (unsigned long long)nb_ivf);
+  }
 }
 
 // counters[0] += sum of the selected docs' lengths, counters[1] += candidates.
@@ -2390,6 +2600,13 @@ hipError_t launch_count_work(const uint32_t *sel_ids, const uint32_t *sel_n, uin
   return hipGetLastError();
 }
 
+hipError_t launch_validate_index(const DevIndex &ix, unsigned long long *bad, hipStream_t s) {
+  k_fill_u32<<<1, 64, 0, s>>>(reinterpret_cast<uint32_t *>(bad), 0u, 4);
+  k_validate_index<<<2048, 256, 0, s>>>(ix.codes, ix.n_tok, ix.n_c, ix.ivf_off, ix.ivf_size, ix.ivf_ids, ix.n_doc,
+                                        bad);
+  return hipGetLastError();
+}
+
 hipError_t launch_seg_offsets(const DevIndex &ix, hipStream_t s) {
   const uint64_t total = (uint64_t)ix.n_c * (ix.n_seg + 1);
   k_seg_offsets<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(ix.ivf_off, ix.ivf_size,
@@ -2425,7 +2642,7 @@ hipError_t launch_centroid_topk_bf16(const float *Q, uint32_t n_q, uint32_t t_ma
                                      uint32_t P, float *bmax, uint32_t *top_ids, float *top_s,
                                      bool exact_f32, hipStream_t s) {
   if (n_q == 0 || n_rows == 0) return hipSuccess;
-  if (!ix.C_bf16 || P > 64) return hipErrorInvalidValue;
+  if (!ix.C_bf16 || P == 0 || P > kMaxProbe) return hipErrorInvalidValue;
   const uint32_t n_tiles = (ix.n_c + 31) / 32;
   const uint32_t qblocks = (n_q + kBmWaves - 1) / kBmWaves;
   const uint32_t n_split = std::max(1u, std::min(n_tiles, (1024u + qblocks - 1) / qblocks));
@@ -2438,36 +2655,44 @@ hipError_t launch_centroid_topk_bf16(const float *Q, uint32_t n_q, uint32_t t_ma
   // (2u + u^2, u = 2^-8) plus two 128-term FP32 accumulations; bf16 inputs =
   // accumulation order only (products of bf16 values are exact in FP32), with
   // a wide allowance for the matrix unit's internal summation
-  if (exact_f32)
-    k_ct_select<true><<<(n_rows + 3) / 4, 256, 0, s>>>(Q, row_map, n_rows, ix.C, ix.C_bf16, ix.n_c,
-                                                       n_tiles, bmax, P, 0.008f, ix.c_norm_max,
-                                                       top_ids, top_s);
-  else
-    k_ct_select<false><<<(n_rows + 3) / 4, 256, 0, s>>>(Q, row_map, n_rows, ix.C, ix.C_bf16, ix.n_c,
-                                                        n_tiles, bmax, P, 1e-4f, ix.c_norm_max * 1.004f,
-                                                        top_ids, top_s);
+  const float margin = exact_f32 ? 0.008f : 1e-4f;
+  const float cmax = exact_f32 ? ix.c_norm_max : ix.c_norm_max * 1.004f;
+  const dim3 g((n_rows + 3) / 4);
+  // list registers per lane: 64 elements each
+#define MVR_CT_SELECT(F32, J)                                                                            \
+  k_ct_select<F32, J><<<g, 256, 0, s>>>(Q, row_map, n_rows, ix.C, ix.C_bf16, ix.n_c, n_tiles, bmax, P,  \
+                                        margin, cmax, top_ids, top_s)
+  if (exact_f32) {
+    if (P <= 64) MVR_CT_SELECT(true, 1);
+    else if (P <= 128) MVR_CT_SELECT(true, 2);
+    else MVR_CT_SELECT(true, 4);
+  } else {
+    if (P <= 64) MVR_CT_SELECT(false, 1);
+    else if (P <= 128) MVR_CT_SELECT(false, 2);
+    else MVR_CT_SELECT(false, 4);
+  }
+#undef MVR_CT_SELECT
   return hipGetLastError();
 }
 
 hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t_max, uint32_t P,
                          const uint32_t *q_ntok, const uint32_t *q_np, uint32_t n_q,
-                         const DevIndex &ix, uint32_t *seg_cnt, uint64_t *cand_keys, hipStream_t s) {
+                         const DevIndex &ix, int clamp0, uint32_t *seg_cnt, uint64_t *cand_keys,
+                         hipStream_t s) {
   if (n_q == 0) return hipSuccess;
   const uint64_t n_wg = (uint64_t)ix.n_seg * n_q;
-  if (n_wg > 0x7FFFFFF8ull || P == 0 || P > kFtRanks) return hipErrorInvalidValue;
+  if (n_wg > 0x7FFFFFF8ull || P == 0 || P > kMaxProbe) return hipErrorInvalidValue;
   const uint64_t n_grp = ((uint64_t)ix.n_seg + kFiltGroup - 1) / kFiltGroup * n_q;
   const uint32_t grid = (uint32_t)((n_grp + 7) / 8 * 8);  // the kernel's XCD remap needs a multiple of 8
+#define MVR_FILTER(KERNEL)                                                                                  \
+  KERNEL<<<grid, kFtThreads, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np, ix.ivf_off, ix.ivf_ids, ix.seg_off, \
+                                     ix.n_seg, ix.n_doc, n_q, clamp0, seg_cnt, cand_keys)
   // probe ranks per lane: the fewest that hold P
-  if (P <= kFtLanes)
-    k_filter_tok<1><<<grid, kFtThreads, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np, ix.ivf_off, ix.ivf_ids,
-                                                 ix.seg_off, ix.n_seg, ix.n_doc, n_q, seg_cnt, cand_keys);
-  else if (P <= 2 * kFtLanes)
-    k_filter_tok<2><<<grid, kFtThreads, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np, ix.ivf_off, ix.ivf_ids,
-                                                 ix.seg_off, ix.n_seg, ix.n_doc, n_q, seg_cnt, cand_keys);
-  else
-    k_filter_tok<kFtRanks / kFtLanes><<<grid, kFtThreads, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np,
-                                                                   ix.ivf_off, ix.ivf_ids, ix.seg_off, ix.n_seg,
-                                                                   ix.n_doc, n_q, seg_cnt, cand_keys);
+  if (P <= kFtLanes) MVR_FILTER(k_filter_tok<1>);
+  else if (P <= 2 * kFtLanes) MVR_FILTER(k_filter_tok<2>);
+  else if (P <= 4 * kFtLanes) MVR_FILTER(k_filter_tok<4>);
+  else MVR_FILTER(k_filter_wide);
+#undef MVR_FILTER
   return hipGetLastError();
 }
 

@@ -6,7 +6,7 @@
 // mean cross-entropy + mean squared error; Adam (0.9, 0.999, 1e-8, bias
 // corrected).  Every reduction has a fixed order, so a training step is
 // deterministic.  The reference has no learned component (it probes a fixed
-// nprobe, src/include/alg/igp_gpu_pp/GenerateVQScoreCandidatePP.hpp:353-361);
+// nprobe, src/include/alg/igp_gpu_pp/GenerateVQScoreCandidatePP.hpp:166-174);
 // this model chooses nprobe per query before the filter runs.
 #include "../../include/mvr.h"
 #include "mvr_internal.h"

@@ -573,15 +573,20 @@ static inline int64_t orc_fx(float s) { return (int64_t)((double)s * ORC_FX_SCAL
 
 /* Filter of one query (FilterPP.hpp:267-398):
  *   candidates = union over t < T, j < P of IVF[top_ids[t][j]];
- *   m[d][t]    = max_j { top_s[t][j] : d in IVF[top_ids[t][j]] }, 0 if none
- *                (max_qvec_doc_score :136-141; memset 0 at :257-259);
+ *   m[d][t]    = max(0, max_j { top_s[t][j] : d in IVF[top_ids[t][j]] }), 0 if
+ *                none: the maxima start zero-filled (memset at :257-259) and
+ *                take atomicMax of every hit (max_qvec_doc_score :136-141);
  *   F[d]       = sum_t m[d][t] (reduce :366-379), here as the rounded exact
  *                fixed-point sum (see file header).
+ * basic != 0 restates the basic variant instead (igp_gpu_basic/Filter.hpp:44-59):
+ * m[d][t] is the score of the first probe rank whose list holds d, as it is
+ * (it may be negative), and 0 if none.
  * Writes every candidate (doc ascending) to cand_ids/cand_F and returns the
  * count.  top_ids/top_s rows are Pmax wide and sorted (score desc). */
 uint32_t orc_filter_all(const uint32_t *top_ids, const float *top_s, uint32_t T, uint32_t Pmax,
                         uint32_t P, const uint64_t *ivf_off, const uint32_t *ivf_size,
-                        const uint32_t *ivf_ids, uint32_t n_doc, uint32_t *cand_ids, float *cand_F) {
+                        const uint32_t *ivf_ids, uint32_t n_doc, int basic, uint32_t *cand_ids,
+                        float *cand_F) {
   uint32_t *mask = (uint32_t *)calloc(n_doc, sizeof(uint32_t));
   int64_t *acc = (int64_t *)calloc(n_doc, sizeof(int64_t));
   uint8_t *seen = (uint8_t *)calloc(n_doc, 1);
@@ -592,6 +597,7 @@ uint32_t orc_filter_all(const uint32_t *top_ids, const float *top_s, uint32_t T,
       const uint32_t c = top_ids[(size_t)t * Pmax + j];
       if (c == UINT32_MAX) continue;
       const float s = top_s[(size_t)t * Pmax + j];
+      const int64_t fx = (!basic && orc_fx(s) < 0) ? 0 : orc_fx(s);
       const uint32_t *lst = ivf_ids + ivf_off[c];
       for (uint32_t e = 0; e < ivf_size[c]; e++) {
         const uint32_t d = lst[e];
@@ -599,7 +605,7 @@ uint32_t orc_filter_all(const uint32_t *top_ids, const float *top_s, uint32_t T,
         const uint32_t bit = 1u << t;
         if (mask[d] & bit) continue;
         mask[d] |= bit;
-        acc[d] += orc_fx(s);
+        acc[d] += fx;
       }
     }
   uint32_t n = 0;
@@ -682,7 +688,7 @@ static void orc_search_one(const orc_index *ix, const float *Q, uint32_t T, uint
                            uint32_t *sel_ids, float *sel_F, float *sel_score) {
   orc_centroid_topk(Q, T, ix->C, ix->n_c, ix->dim, P, top_ids, top_s);
   const uint32_t nc = orc_filter_all(top_ids, top_s, T, P, P, ix->ivf_off, ix->ivf_size, ix->ivf_ids,
-                                     ix->n_doc, cand_buf, candF_buf);
+                                     ix->n_doc, clip_zero, cand_buf, candF_buf);
   const uint32_t nr = orc_select_top(cand_buf, candF_buf, nc, R, sel_ids, sel_F);
   orc_rerank(Q, T, ix->dim, sel_ids, nr, ix->doclens, ix->docoff, ix->codes, ix->residual, ix->C,
              ix->W, ix->n_bit, clip_zero, sel_score);

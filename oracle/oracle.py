@@ -66,7 +66,7 @@ def lib():
         "orc_centroid_topk": (None, [_f32p, C.c_uint64, _f32p, C.c_uint32, C.c_uint32, C.c_uint32,
                                      _u32p, _f32p]),
         "orc_filter_all": (C.c_uint32, [_u32p, _f32p, C.c_uint32, C.c_uint32, C.c_uint32, _u64p, _u32p,
-                                        _u32p, C.c_uint32, _u32p, _f32p]),
+                                        _u32p, C.c_uint32, C.c_int, _u32p, _f32p]),
         "orc_select_top": (C.c_uint32, [_u32p, _f32p, C.c_uint32, C.c_uint32, _u32p, _f32p]),
         "orc_rerank": (None, [_f32p, C.c_uint32, C.c_uint32, _u32p, C.c_uint32, _u32p, _u64p, _u32p,
                               _u8p, _f32p, _f32p, C.c_uint32, C.c_int, _f32p]),
@@ -224,14 +224,16 @@ def centroid_topk(Q: np.ndarray, C_: np.ndarray, P: int):
     return ids, s
 
 
-def filter_all(top_ids, top_s, P: int, ivf, n_doc: int):
+def filter_all(top_ids, top_s, P: int, ivf, n_doc: int, basic: bool = False):
+    """PP filter (per-token maxima max(0, .)); basic=True: the basic
+    variant's first-hit scores."""
     top_ids, top_s = _c(top_ids, np.uint32), _c(top_s, np.float32)
     T, Pmax = top_ids.shape
     off, size, ids = ivf
     cand = np.empty(n_doc, np.uint32)
     F = np.empty(n_doc, np.float32)
     n = lib().orc_filter_all(top_ids, top_s, T, Pmax, P, _c(off, np.uint64), _c(size, np.uint32),
-                             _c(ids, np.uint32), n_doc, cand, F)
+                             _c(ids, np.uint32), n_doc, int(basic), cand, F)
     return cand[:n].copy(), F[:n].copy()
 
 

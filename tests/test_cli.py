@@ -78,21 +78,26 @@ def write_dataset(root, ix, Q, qids, n_c):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("exe,clip,host", [(EXE, False, False), (EXE_BASIC, True, False), (EXE, False, True)])
-def test_cli_answer_matches_oracle(tmp_path, small, exe, clip, host):
+@pytest.mark.parametrize("exe,clip,host,nprobe", [(EXE, False, False, 4), (EXE_BASIC, True, False, 4),
+                                                  (EXE, False, True, 4), (EXE, False, False, 64),
+                                                  (EXE_BASIC, True, False, 64)])
+def test_cli_answer_matches_oracle(tmp_path, small, exe, clip, host, nprobe):
+    """nprobe 64 is what eval_igp_gpu_pp.py:243,266,283 pass."""
     corpus, ix, Q = small
     qids = [100 + 3 * i for i in range(Q.shape[0])]
     write_dataset(str(tmp_path), ix, Q, qids, ix.n_c)
     env = dict(os.environ, MVR_DATASET_ROOT=str(tmp_path))
-    args = FLAGS if exe == EXE else [a for i, a in enumerate(FLAGS)
-                                     if a != "-n_thread" and (i == 0 or FLAGS[i - 1] != "-n_thread")]
+    flags = list(FLAGS)
+    flags[flags.index("-nprobe") + 1] = str(nprobe)
+    args = flags if exe == EXE else [a for i, a in enumerate(flags)
+                                     if a != "-n_thread" and (i == 0 or flags[i - 1] != "-n_thread")]
     r = _run(exe, args + ["-batch", "5"] + (["-residual_on_host", "true"] if host else []), env)
     assert r.returncode == 0, r.stderr
     ans = os.path.join(tmp_path, "Result", "answer")
-    suffix = "nprobe_4-probe_topk_64" + ("-n_thread_1" if exe == EXE else "")
+    suffix = f"nprobe_{nprobe}-probe_topk_64" + ("-n_thread_1" if exe == EXE else "")
     rows = [l.split("\t") for l in open(os.path.join(
         ans, f"ds-igp-top10-n_centroid_{ix.n_c}-n_bit_2-{suffix}.tsv")).read().splitlines()]
-    oi, os_ = ix.search(Q, 4, 64, 10, clip_zero=clip)
+    oi, os_ = ix.search(Q, nprobe, 64, 10, clip_zero=clip)
     assert len(rows) == Q.shape[0] * 10
     for n, (qid, doc, rank, score) in enumerate(rows):
         q, k = divmod(n, 10)

@@ -27,7 +27,7 @@ def _oracle_stages(ix, Q, P, R, k, q_ntok=None, q_nprobe=None, clip_zero=False):
         T = t_max if q_ntok is None else int(q_ntok[q])
         Pq = P if q_nprobe is None else int(q_nprobe[q])
         top_ids, top_s = O.centroid_topk(Q[q, :T], ix.C, Pq)
-        cand, F = O.filter_all(top_ids, top_s, Pq, ix.ivf, ix.n_doc)
+        cand, F = O.filter_all(top_ids, top_s, Pq, ix.ivf, ix.n_doc, basic=clip_zero)
         sel, selF = O.select_top(cand, F, R)
         sc = ix.rerank(Q[q, :T], sel, clip_zero=clip_zero)
         fid, fs = O.select_top(sel, sc, k)
@@ -409,3 +409,28 @@ def test_zero_queries(small):
     os_ = torch.empty((0, 10), dtype=torch.float32, device="cuda")
     g.search_device(Qd, oi, os_, 4, 64, 10)
     torch.cuda.synchronize()
+
+
+def test_index_create_rejects_bad_codes_and_lists(small):
+    """Codes must index a centroid and IVF lists must be strictly ascending
+    doc ids below n_doc (the kernels index centroid rows by code and the
+    filter's segment runs assume sorted lists); both are checked at creation."""
+    import mvr
+    corpus, ix, Q = small
+    codes = ix.codes.copy()
+    codes[17] = ix.n_c
+    with pytest.raises(mvr.MvrError, match="codes are >= n_centroid"):
+        mvr.Index.from_arrays(ix.doclens, ix.C, codes, ix.residual, ix.weights, ix.ivf)
+    off, size, ids = ix.ivf
+    c = int(np.argmax(size))
+    for bad in ("swap", "range"):
+        ids2 = ids.copy()
+        o = int(off[c])
+        if bad == "swap":
+            ids2[o], ids2[o + 1] = ids2[o + 1], ids2[o]
+        else:
+            ids2[o + int(size[c]) - 1] = ix.n_doc
+        with pytest.raises(mvr.MvrError, match="IVF entries"):
+            mvr.Index.from_arrays(ix.doclens, ix.C, ix.codes, ix.residual, ix.weights, (off, size, ids2))
+    g = _index(ix)  # the unmodified index is accepted
+    g.close()

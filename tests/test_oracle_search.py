@@ -93,6 +93,26 @@ def test_filter_respects_probe_count_and_fixed_point():
     assert F[0] == np.float32(fx)
 
 
+def test_filter_pp_clamps_token_maxima_at_zero_basic_keeps_first_hit():
+    """PP: maxima start at 0 and take atomicMax of every hit, so a token whose
+    probed scores are all negative adds 0 (FilterPP.hpp:136-141, :257-259);
+    the basic variant adds the first hit's score as it is (Filter.hpp:44-59).
+    Candidates are the same either way."""
+    lists = {0: [1, 2], 1: [2, 3], 2: [3]}
+    ivf = _tiny_ivf(lists, 3)
+    # token 0: both probes negative; token 1: a positive and a negative probe
+    top_ids = np.array([[0, 1], [2, 0]], np.uint32)
+    top_s = np.array([[-0.5, -0.75], [0.25, -0.125]], np.float32)
+    cand, F = O.filter_all(top_ids, top_s, 2, ivf, 4)
+    assert cand.tolist() == [1, 2, 3]
+    # doc1: t0 max(0,-0.5)=0, t1 max(0,-0.125)=0; doc2: 0 + 0; doc3: 0 + 0.25
+    np.testing.assert_array_equal(F, np.array([0.0, 0.0, 0.25], np.float32))
+    cb, Fb = O.filter_all(top_ids, top_s, 2, ivf, 4, basic=True)
+    assert cb.tolist() == [1, 2, 3]
+    # doc1: -0.5 + -0.125; doc2: -0.5 (rank 0 of t0) + -0.125; doc3: -0.75 + 0.25
+    np.testing.assert_array_equal(Fb, np.array([-0.625, -0.625, -0.5], np.float32))
+
+
 def test_select_top_orders_by_score_then_id():
     ids = np.array([7, 3, 9, 1, 4], np.uint32)
     s = np.array([0.5, 0.9, 0.5, 0.9, 0.1], np.float32)
