@@ -174,6 +174,19 @@ int mvr_rerank_docs(mvr_index *ix, const float *Q_dev, uint32_t n_q, uint32_t t_
                     const uint32_t *q_ntok, const uint32_t *doc_ids_dev, const uint32_t *n_docs,
                     uint32_t R, uint32_t precision, float *scores_dev, void *stream);
 
+/* mvr_rerank_docs without allocating.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under hipGraph capture,
+ * serving loops and harnesses that forbid hipMalloc).  Same arguments, kernels
+ * and results as mvr_rerank_docs; the per-query token and doc counts are
+ * copied from the caller's host arrays into a static device array of the
+ * library (2 x 16,384 u32 per device, held until the process ends) instead of
+ * a workspace the index allocates.  n_q must be <= 16,384 (MVR_ERR_INVALID
+ * beyond).  Every call on one device shares that array: calls that overlap on
+ * one device (e.g. queued on two streams) race, so the caller serialises them. */
+int mvr_rerank_docs_noalloc(mvr_index *ix, const float *Q_dev, uint32_t n_q, uint32_t t_max,
+                            const uint32_t *q_ntok, const uint32_t *doc_ids_dev, const uint32_t *n_docs,
+                            uint32_t R, uint32_t precision, float *scores_dev, void *stream);
+
 /* Work-queue MaxSim tuning per precision (no counterpart in the reference,
  * which launches one kernel per query): workgroups per CU of the persistent
  * grid and 32-token tiles in flight per wave.  The first persistent search in

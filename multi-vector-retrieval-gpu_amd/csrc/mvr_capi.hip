@@ -21,10 +21,15 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <string>
 #include <vector>
 
 using namespace mvr;
+
+// The no-alloc entry point's scratch (see kNoAllocMaxQ below): one copy per
+// device, held until the process ends.
+static __device__ __align__(256) uint32_t scratch[2 * 16384];
 
 namespace {
 
@@ -47,6 +52,24 @@ int fail(int code, const char *fmt, ...) {
       return fail(MVR_ERR_DEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), \
                   __FILE__, __LINE__);                                                   \
   } while (0)
+
+// Scratch of mvr_rerank_docs_noalloc: the per-query token counts [0, n_q)
+// and doc counts [n_q, 2 n_q) of one call.  Host code never uses the array's
+// name as an address: each device's copy is reached through the address
+// hipGetSymbolAddress returns for that device (scratch_on).
+constexpr uint32_t kNoAllocMaxQ = 16384;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t *g_scratch_dev[kMaxDevices] = {};
+
+// Writes defaults for absent host arrays and clamps doc counts to R.
+__global__ void k_noalloc_meta(uint32_t *meta, uint32_t n_q, uint32_t t_max, uint32_t R, int have_nt,
+                               int have_nd) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n_q) return;
+  if (!have_nt) meta[q] = t_max;
+  meta[n_q + q] = have_nd ? min(meta[n_q + q], R) : R;
+}
 
 // Growable device buffer.
 struct DBuf {
@@ -182,6 +205,24 @@ struct DeviceGuard {
     if (prev >= 0) (void)hipSetDevice(prev);
   }
 };
+
+// Device address of `scratch` on `device` (that device's copy), looked up once
+// per device with the device current and kept.
+int scratch_on(int device, uint32_t **out) {
+  static_assert(sizeof(scratch) == 2 * kNoAllocMaxQ * sizeof(uint32_t), "scratch size");
+  if (device < 0 || device >= kMaxDevices) return fail(MVR_ERR_UNSUPPORTED, "device %d: no scratch slot", device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    DeviceGuard g(device);
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(MVR_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d failed: %s", device, hipGetErrorString(e));
+    g_scratch_dev[device] = static_cast<uint32_t *>(p);
+  }
+  *out = g_scratch_dev[device];
+  return MVR_OK;
+}
 
 // float -> bfloat16 bits, round to nearest even (finite inputs)
 uint16_t f32_to_bf16(float f) {
@@ -499,6 +540,12 @@ int index_create(const mvr_index_desc *desc, int device, uint32_t flags, const C
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess)
       ix->pool_budget = std::min<uint64_t>(kPoolBudgetMax, std::max<uint64_t>(kPoolBudgetMin, free_b / 4));
+  }
+  {
+    // the no-alloc entry point's scratch address on this device (set-up, once per device)
+    uint32_t *scr = nullptr;
+    const int rc = scratch_on(device, &scr);
+    if (rc != MVR_OK) return bail(rc);
   }
   *out = ix;
   return MVR_OK;
@@ -1344,6 +1391,35 @@ int mvr_rerank_docs(mvr_index *ix, const float *Q_dev, uint32_t n_q, uint32_t t_
   HIP_TRY(hipEventRecord(ws.staging_done, s));
   const uint32_t *d_nt = ws.rr_meta.as<uint32_t>(), *d_n = d_nt + n_q;
   HIP_TRY(launch_rerank(Q_dev, t_max, d_nt, n_q, doc_ids_dev, d_n, R, ix->d, (int)precision, 0,
+                        scores_dev, s));
+  return MVR_OK;
+}
+
+int mvr_rerank_docs_noalloc(mvr_index *ix, const float *Q_dev, uint32_t n_q, uint32_t t_max,
+                            const uint32_t *q_ntok, const uint32_t *doc_ids_dev, const uint32_t *n_docs,
+                            uint32_t R, uint32_t precision, float *scores_dev, void *stream) {
+  if (!ix || !Q_dev || !doc_ids_dev || !scores_dev) return fail(MVR_ERR_INVALID, "NULL argument");
+  if (ix->res_host)
+    return fail(MVR_ERR_UNSUPPORTED, "mvr_rerank_docs_noalloc needs a device-resident index (residuals are on the host)");
+  if (n_q == 0 || R == 0) return MVR_OK;
+  if (n_q > kNoAllocMaxQ) return fail(MVR_ERR_INVALID, "n_q %u > %u (the no-alloc scratch holds %u queries)", n_q,
+                                      kNoAllocMaxQ, kNoAllocMaxQ);
+  if (t_max == 0 || t_max > kMaxQTok) return fail(MVR_ERR_INVALID, "t_max %u not in [1, 32]", t_max);
+  if (precision > MVR_FP32_MFMA) return fail(MVR_ERR_INVALID, "unknown precision %u", precision);
+  if (q_ntok)
+    for (uint32_t q = 0; q < n_q; ++q)
+      if (q_ntok[q] == 0 || q_ntok[q] > t_max) return fail(MVR_ERR_INVALID, "query %u has %u tokens", q, q_ntok[q]);
+  uint32_t *meta = nullptr;
+  const int rc = scratch_on(ix->device, &meta);
+  if (rc != MVR_OK) return rc;
+  DeviceGuard g(ix->device);
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  // the caller's host arrays straight into the scratch, then defaults and the clamp to R
+  if (q_ntok) HIP_TRY(hipMemcpyAsync(meta, q_ntok, (size_t)n_q * 4, hipMemcpyHostToDevice, s));
+  if (n_docs) HIP_TRY(hipMemcpyAsync(meta + n_q, n_docs, (size_t)n_q * 4, hipMemcpyHostToDevice, s));
+  k_noalloc_meta<<<(n_q + 255) / 256, 256, 0, s>>>(meta, n_q, t_max, R, q_ntok ? 1 : 0, n_docs ? 1 : 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(launch_rerank(Q_dev, t_max, meta, n_q, doc_ids_dev, meta + n_q, R, ix->d, (int)precision, 0,
                         scores_dev, s));
   return MVR_OK;
 }

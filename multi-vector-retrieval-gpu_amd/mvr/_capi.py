@@ -90,6 +90,8 @@ SIGNATURES = {
     "mvr_index_get_tuning": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "mvr_rerank_docs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
+    "mvr_rerank_docs_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "mvr_exhaustive_f64": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                      C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]),
     "mvr_mlp_last_error": (C.c_char_p, []),
