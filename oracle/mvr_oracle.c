@@ -274,6 +274,14 @@ void orc_assign_ip(const float *x, uint64_t n, uint32_t dim, const float *C, uin
   orc_argmax_rows(x, n, dim, C, n_c, NULL, codes);
 }
 
+/* argmax_c (<x, C_c> - bias[c]) in FP32 (sequential FMA dot, then one
+ * subtraction), lowest index on ties: the assignment step of the Lloyd
+ * iterations below with bias = |C_c|^2 / 2 (argmin L2). */
+void orc_assign_bias(const float *x, uint64_t n, uint32_t dim, const float *C, uint32_t n_c,
+                     const float *bias, uint32_t *codes) {
+  orc_argmax_rows(x, n, dim, C, n_c, bias, codes);
+}
+
 /* Lloyd's k-means: initial centroid j is point floor(j*n/n_c); each iteration
  * assigns by argmax (<x,c> - |c|^2/2) (= argmin L2, lowest index on ties, FP32
  * sequential FMA), then recomputes means in double; an empty cluster keeps its

@@ -52,6 +52,7 @@ def lib():
                                   C.c_double, C.c_uint64, C.c_uint64, _f32p]),
         "orc_dot": (C.c_float, [_f32p, _f32p, C.c_uint32]),
         "orc_assign_ip": (None, [_f32p, C.c_uint64, C.c_uint32, _f32p, C.c_uint32, _u32p]),
+        "orc_assign_bias": (None, [_f32p, C.c_uint64, C.c_uint32, _f32p, C.c_uint32, _f32p, _u32p]),
         "orc_kmeans": (None, [_f32p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, _f32p]),
         "orc_residuals": (None, [_f32p, _u32p, C.c_uint64, C.c_uint32, _f32p, _f32p]),
         "orc_quantize_pooled": (None, [_f32p, C.c_uint64, C.c_uint32, _f32p, _f32p]),
@@ -148,6 +149,14 @@ def assign_ip(x: np.ndarray, C_: np.ndarray) -> np.ndarray:
     x, C_ = _c(x, np.float32), _c(C_, np.float32)
     out = np.empty(x.shape[0], np.uint32)
     lib().orc_assign_ip(x, x.shape[0], x.shape[1], C_, C_.shape[0], out)
+    return out
+
+
+def assign_bias(x: np.ndarray, C_: np.ndarray, bias: np.ndarray) -> np.ndarray:
+    """argmax_c (<x, C_c> - bias[c]), lowest id on ties (the Lloyd step)."""
+    x, C_ = _c(x, np.float32), _c(C_, np.float32)
+    out = np.empty(x.shape[0], np.uint32)
+    lib().orc_assign_bias(x, x.shape[0], x.shape[1], C_, C_.shape[0], _c(bias, np.float32), out)
     return out
 
 

@@ -136,3 +136,19 @@ def test_kmeans_known_answer_two_clusters_and_empty_cluster():
     y = np.array([[2.0, 0.0]] * 4, np.float32)
     got = O.kmeans(y, 3, 4)
     np.testing.assert_allclose(got, [[1.0, 0.0]] * 3, atol=1e-7)
+
+
+def test_assign_bias_matches_float64_argmin_l2():
+    """With bias |C_c|^2 / 2 the biased argmax is the L2 nearest centroid."""
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((400, 24)).astype(np.float32)
+    C_ = (rng.standard_normal((30, 24)) * rng.uniform(0.5, 2.0, (30, 1))).astype(np.float32)
+    bias = (0.5 * (C_.astype(np.float64) ** 2).sum(1)).astype(np.float32)
+    got = O.assign_bias(x, C_, bias)
+    d2 = ((x.astype(np.float64)[:, None, :] - C_.astype(np.float64)[None]) ** 2).sum(-1)
+    srt = np.sort(d2, axis=1)
+    clear = srt[:, 1] - srt[:, 0] > 1e-3
+    assert clear.mean() > 0.95
+    np.testing.assert_array_equal(got[clear], np.argmin(d2, axis=1)[clear])
+    # a zero bias is the plain inner-product assignment
+    np.testing.assert_array_equal(O.assign_bias(x, C_, np.zeros(30, np.float32)), O.assign_ip(x, C_))
