@@ -127,7 +127,7 @@ struct PassGraph {
 struct Workspace {
   DBuf Q, meta, part_ids, part_s, top_ids, top_s, cand_cnt, cand_tot, cand_keys, sel_ids, sel_F, sel_n,
       sel_score, out_ids, out_s, counters, res_ids, res_s, res_n, bmax, feat, mlp_out, dyn_np, dyn_r,
-      rr_meta, queue, pass_nt, pass_np, pass_rm, xcd_part, xcd_queue, hs_bm, hs_bn, hs_bt, hs_wbase;
+      rr_meta, queue, pass_nt, pass_np, pass_rm, hs_bm, hs_bn, hs_bt, hs_wbase;
   // host-resident residuals, per pass slot: compact lists and the staged docs
   DBuf hs_docs[2], hs_lens[2], hs_offs[2], hs_cids[2], hs_cnt[2], hs_codes[2], hs_res[2];
   HBuf staging;
@@ -142,7 +142,7 @@ struct Workspace {
     return {&Q, &meta, &part_ids, &part_s, &top_ids, &top_s, &cand_cnt, &cand_tot, &cand_keys,
             &sel_ids, &sel_F, &sel_n, &sel_score, &out_ids, &out_s, &counters, &res_ids, &res_s,
             &res_n, &bmax, &feat, &mlp_out, &dyn_np, &dyn_r, &rr_meta, &queue, &pass_nt, &pass_np,
-            &pass_rm, &xcd_part, &xcd_queue, &hs_bm, &hs_bn, &hs_bt, &hs_wbase, &hs_docs[0], &hs_docs[1],
+            &pass_rm, &hs_bm, &hs_bn, &hs_bt, &hs_wbase, &hs_docs[0], &hs_docs[1],
             &hs_lens[0], &hs_lens[1], &hs_offs[0], &hs_offs[1], &hs_cids[0], &hs_cids[1], &hs_cnt[0],
             &hs_cnt[1], &hs_codes[0], &hs_codes[1], &hs_res[0], &hs_res[1]};
   }
@@ -181,8 +181,6 @@ struct mvr_index {
   Workspace ws;
   PersistCfg tune[4];          // work-queue MaxSim settings per precision
   bool tuned[4] = {false, false, false, false};
-  XcdLayout xl;                // slice-grouped token copy (built on first use)
-  bool xl_failed = false;
   // MVR_INDEX_RESIDUAL_ON_HOST: residual bytes and a copy of the codes stay in
   // host memory (residual pinned); each pass stages its selected docs
   uint64_t pool_budget = kPoolBudget;  // candidate pool bytes per pass (from free HBM at creation)
@@ -224,19 +222,11 @@ int scratch_on(int device, uint32_t **out) {
   return MVR_OK;
 }
 
-// float -> bfloat16 bits, round to nearest even (finite inputs)
-uint16_t f32_to_bf16(float f) {
-  uint32_t b;
-  std::memcpy(&b, &f, 4);
-  b += 0x7FFFu + ((b >> 16) & 1u);
-  return (uint16_t)(b >> 16);
-}
-
 void free_devindex(DevIndex &d) {
-  for (void *p : {(void *)d.C, (void *)d.C_bf16, (void *)d.C_fp8, (void *)d.C_fp8_scale,
+  for (void *p : {(void *)d.C, (void *)d.C_bf16, (void *)d.C_bf16m, (void *)d.C_fp8, (void *)d.C_fp8_scale,
                   (void *)d.codes, (void *)d.residual, (void *)d.W, (void *)d.doclens,
                   (void *)d.docoff, (void *)d.ivf_off, (void *)d.ivf_size, (void *)d.ivf_ids,
-                  (void *)d.seg_off, (void *)d.lut_bf16, (void *)d.lut_fp8})
+                  (void *)d.seg_off, (void *)d.lut_fp8})
     if (p) (void)hipFree(p);
   d = DevIndex();
 }
@@ -483,25 +473,12 @@ int index_create(const mvr_index_desc *desc, int device, uint32_t flags, const C
   UP(d.ivf_size, desc->ivf_sizes, d.n_c);
   UP(d.ivf_ids, desc->ivf_ids, d.n_ele);
 #undef UP
-  // bf16 residual table: entry [pos][byte] holds the four decoded values of
-  // dims 4*pos .. 4*pos+3 (MSB-first buckets), two bf16 per 32-bit word
-  std::vector<uint32_t> lut(kResBytes * 256 * 2);
-  for (uint32_t pos = 0; pos < kResBytes; ++pos)
-    for (uint32_t byte = 0; byte < 256; ++byte) {
-      uint16_t v[4];
-      for (uint32_t k = 0; k < 4; ++k) {
-        const uint32_t dd = 4 * pos + k, b = (byte >> (6 - 2 * k)) & 3u;
-        v[k] = f32_to_bf16(W[dd * 4 + b]);
-      }
-      lut[(pos * 256 + byte) * 2 + 0] = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
-      lut[(pos * 256 + byte) * 2 + 1] = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
-    }
   hipError_t e = upload(&d.docoff, docoff.data(), docoff.size(), false);
   if (e == hipSuccess) e = upload(&d.W, W.data(), W.size(), false);
   if (e == hipSuccess)
-    e = upload(reinterpret_cast<uint32_t **>(&d.lut_bf16), lut.data(), lut.size(), false);
-  if (e == hipSuccess)
     e = hipMalloc(reinterpret_cast<void **>(&d.C_bf16), (size_t)d.n_c * kDim * sizeof(uint16_t));
+  if (e == hipSuccess)
+    e = hipMalloc(reinterpret_cast<void **>(&d.C_bf16m), (size_t)d.n_c * kDim * sizeof(uint16_t));
   {
     float wmax = 0.0f;
     for (float v : W) wmax = std::max(wmax, std::fabs(v));
@@ -631,8 +608,6 @@ void mvr_index_free(mvr_index *ix) {
   DeviceGuard g(ix->device);
   (void)hipDeviceSynchronize();
   ix->ws.release();
-  for (void *p : {(void *)ix->xl.codes_s, (void *)ix->xl.residual_s, (void *)ix->xl.sl_off})
-    if (p) (void)hipFree(p);
   free_devindex(ix->d);
   if (ix->h_res) (void)hipHostFree(ix->h_res);
   if (ix->h_codes) (void)hipHostFree(ix->h_codes);
@@ -640,30 +615,6 @@ void mvr_index_free(mvr_index *ix) {
 }
 
 namespace {
-
-// The slice-grouped token copy for the XCD-sliced MaxSim (built on first use;
-// false when the index cannot have one, e.g. docs longer than 65,535 tokens).
-bool ensure_xcd_layout(mvr_index *ix, hipStream_t s) {
-  if (ix->xl.codes_s) return true;
-  if (ix->xl_failed) return false;
-  const DevIndex &d = ix->d;
-  XcdLayout xl;
-  hipError_t e = d.max_doclen < 65536 ? hipSuccess : hipErrorInvalidValue;
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&xl.codes_s), d.n_tok * 4);
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&xl.residual_s), d.n_tok * kResBytes);
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&xl.sl_off), (size_t)d.n_doc * 9 * 2);
-  if (e == hipSuccess) e = launch_slice_docs(d, xl.codes_s, xl.residual_s, xl.sl_off, s);
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  if (e != hipSuccess) {
-    for (void *p : {(void *)xl.codes_s, (void *)xl.residual_s, (void *)xl.sl_off})
-      if (p) (void)hipFree(p);
-    ix->xl_failed = true;
-    (void)hipGetLastError();
-    return false;
-  }
-  ix->xl = xl;
-  return true;
-}
 
 // The compact index of one pass over a host-resident-residual index.
 struct HostPass {
@@ -879,14 +830,6 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   // graph replay: not with learned probing (the model's launches) or stage outputs
   // nor with host-resident residuals (each pass synchronises to stage its docs)
   const bool use_graph = prm->graph && !learned && !stages && !ix->res_host;
-  // XCD-sliced MaxSim for the MFMA precisions: opt in with MVR_MAXSIM_SLICED=1
-  // (it cuts the fabric traffic of the centroid gathers by more than half but
-  // is not faster than the per-doc kernels on MI355X, see the design notes)
-  const char *sliced_var = std::getenv("MVR_MAXSIM_SLICED");
-  const bool sliced_env = sliced_var && sliced_var[0] == '1';
-  const bool sliced =
-      !persistent && !ix->res_host && prm->precision != MVR_FP32 && sliced_env && ensure_xcd_layout(ix, s);
-  const uint32_t xcd_wgs = std::max(1u, d.n_cu / 8) * 2;
   const uint32_t n_pass = (n_q + B - 1) / B;
   // host-resident residuals: two passes in flight, the staging copy of one (on
   // a second stream) overlapping the device work of the other; the selection
@@ -922,10 +865,6 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     HIP_TRY(ws.out_s.ensure(Bc * k * 4));
   }
   if (persistent) HIP_TRY(ws.queue.ensure(4));
-  if (sliced) {
-    HIP_TRY(ws.xcd_queue.ensure(8 * 4));
-    HIP_TRY(ws.xcd_part.ensure(Bc * std::max<size_t>(R, R2) * kXcdParts * 4));
-  }
   if (use_graph) {
     HIP_TRY(ws.pass_nt.ensure(Bc * 4));
     HIP_TRY(ws.pass_np.ensure(Bc * 4));
@@ -1024,26 +963,15 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       HIP_TRY(mark(6));
       return MVR_OK;
     }
-    if (sliced)
-      HIP_TRY(launch_rerank_xcd(Qb, t_max, bnt, sid, sl.n, R, nb, dx,
-                                ix->xl, (int)prm->precision, clip, xcd_wgs, ws.xcd_queue.as<uint32_t>(),
-                                ws.xcd_part.as<float>(), sl.score, st));
-    else
-      HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, sid, sl.n, R,
-                            dx, prm->precision, clip, sl.score, st));
+    HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, sid, sl.n, R, dx, prm->precision, clip, sl.score, st));
     HIP_TRY(mark(4));
     if (R2) {
       // best R2 by the low-precision score, FP32 MaxSim of those, top-k of the FP32 scores
       HIP_TRY(launch_topk(sid, sl.score, sl.n,
                           R, nb, R2, ws.res_ids.as<uint32_t>(), ws.res_s.as<float>(),
                           ws.res_n.as<uint32_t>(), st));
-      if (sliced)
-        HIP_TRY(launch_rerank_xcd(Qb, t_max, bnt, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2, nb, dx,
-                                  ix->xl, MVR_FP32_MFMA, clip, xcd_wgs, ws.xcd_queue.as<uint32_t>(),
-                                  ws.xcd_part.as<float>(), ws.res_s.as<float>(), st));
-      else
-        HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2,
-                              dx, MVR_FP32_MFMA, clip, ws.res_s.as<float>(), st));
+      HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2,
+                            dx, MVR_FP32_MFMA, clip, ws.res_s.as<float>(), st));
     }
     HIP_TRY(mark(5));
     if (R2)
@@ -1231,7 +1159,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
         const PersistCfg &tc = ix->tune[prm->precision];
         const std::vector<uint64_t> key = {nb, n_rows, t_max, P, R, R2, k, prm->precision, prm->clip_zero,
                                            uniform ? 1u : 0u, persistent ? 1u : 0u, fast_ct ? 1u : 0u,
-                                           tc.n_wg, tc.tps, sliced ? 1u : 0u};
+                                           tc.n_wg, tc.tps};
         hipGraphExec_t exec = nullptr;
         for (size_t gi = 0; gi < ws.graphs.size(); ++gi)
           if (ws.graphs[gi].key == key) {

@@ -27,6 +27,7 @@ struct DevIndex {
   uint32_t n_cu = 0;           // compute units of the device
   float *C = nullptr;          // [n_c][128] f32
   uint16_t *C_bf16 = nullptr;  // [n_c][128] bf16 (RNE of C)
+  uint16_t *C_bf16m = nullptr; // [n_c][128] the same values in MaxSim order: piece 4s + g = dims 32g + 8s .. +7
   uint8_t *C_fp8 = nullptr;    // [n_c][128] e4m3 of C / scale
   float *C_fp8_scale = nullptr;// [n_c]
   uint32_t *codes = nullptr;   // [n_tok]
@@ -38,7 +39,6 @@ struct DevIndex {
   uint32_t *ivf_size = nullptr;// [n_c]
   uint32_t *ivf_ids = nullptr; // [n_ele]
   uint32_t *seg_off = nullptr; // [n_c][n_seg+1] list index where each segment starts
-  uint2 *lut_bf16 = nullptr;   // [32][256]: 4 bf16 residual values of one packed byte per position
   uint32_t *lut_fp8 = nullptr; // [32][256]: 4 e4m3 residual values / fp8_res_scale
   float fp8_res_scale = 0.0f;  // max |W| / 448
 };
@@ -95,22 +95,6 @@ hipError_t launch_persist(const float *Q, uint32_t t_max, const uint32_t *q_ntok
                           const DevIndex &ix, int precision, int clip_zero, const PersistCfg &cfg,
                           uint32_t *queue, float *score, uint32_t kk, uint32_t *out_ids, float *out_s,
                           uint32_t *out_n, uint32_t out_stride, hipStream_t s);
-
-// XCD-sliced MaxSim: tokens of every doc regrouped by centroid slice
-// (code * 8 / n_c) so each XCD scores one slice (its rows stay in that L2);
-// part [nb][R][8][32] holds per-slice maxima, score [nb][R] the sums.
-struct XcdLayout {
-  uint32_t *codes_s = nullptr;    // [n_tok] codes, each doc's tokens grouped by slice
-  uint8_t *residual_s = nullptr;  // [n_tok][32] residual bytes in the same order
-  uint16_t *sl_off = nullptr;     // [n_doc][9] slice offsets inside each doc
-};
-constexpr uint32_t kXcdParts = 8 * 32;  // floats of part per (query, candidate)
-hipError_t launch_slice_docs(const DevIndex &ix, uint32_t *codes_s, uint8_t *residual_s, uint16_t *sl_off,
-                             hipStream_t s);
-hipError_t launch_rerank_xcd(const float *Q, uint32_t t_max, const uint32_t *q_ntok, const uint32_t *in_ids,
-                             const uint32_t *in_n, uint32_t R_in, uint32_t nb, const DevIndex &ix,
-                             const XcdLayout &xl, int precision, int clip_zero, uint32_t wgs_per_xcd,
-                             uint32_t *queue, float *part, float *score, hipStream_t s);
 
 // learned probe: features [n_q][1024] from the top-32 scores, and decisions
 hipError_t launch_probe_features(const float *top_s, uint32_t t_max, uint32_t P,

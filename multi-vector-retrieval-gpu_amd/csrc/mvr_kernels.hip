@@ -21,6 +21,7 @@
 #include <cstdlib>
 #include <cfloat>
 #include <cmath>
+#include <type_traits>
 
 namespace mvr {
 
@@ -955,91 +956,142 @@ __global__ __launch_bounds__(256) void k_rerank_f32(
 // ---------------------------------------------------------------------------
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx16_t __attribute__((ext_vector_type(16)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ bf16x8_t as_bf16x8(uint4 v) { return __builtin_bit_cast(bf16x8_t, v); }
 
-// Per-query-token max over one doc's tokens on BF16 MFMA tiles; returns, in
-// lanes 0..31, the max for query token `col` (after the cross-half max).
-// TPS tiles of 32 doc tokens are in flight per step (independent
-// accumulators, so one tile's loads overlap the other's MFMAs).
-template <int TPS>
-__device__ __forceinline__ float maxsim_doc_bf16(uint32_t L, uint64_t off, uint32_t h, uint32_t col,
-                                                 const uint32_t *__restrict__ codes,
-                                                 const uint8_t *__restrict__ residual,
-                                                 const uint16_t *__restrict__ Cb, const uint2 *lut,
-                                                 const bf16x8_t (&qf)[8], int clip_zero) {
-  float M = clip_zero ? 0.0f : -FLT_MAX;
-  for (uint32_t i0 = 0; i0 < L; i0 += 32 * TPS) {
-    uint4 cv[TPS][8];
-    uint4 rb[TPS];
-#pragma unroll
-    for (int u = 0; u < TPS; ++u) {
-      const uint32_t i = i0 + 32 * u + col;
-      rb[u] = make_uint4(0u, 0u, 0u, 0u);
-      if (i < L) {
-        const uint64_t g = off + i;
-        const uint32_t code = codes[g];
-        rb[u] = *reinterpret_cast<const uint4 *>(residual + g * kResBytes + 16 * h);
-        const uint4 *cp = reinterpret_cast<const uint4 *>(Cb + (size_t)code * kDim + 64 * h);
-#pragma unroll
-        for (uint32_t s = 0; s < 8; ++s) cv[u][s] = cp[s];
-      } else {
-#pragma unroll
-        for (uint32_t s = 0; s < 8; ++s) cv[u][s] = make_uint4(0u, 0u, 0u, 0u);
-      }
-    }
-    // residual operand words for the whole tile first (16 LDS reads in
-    // flight together), then the MFMA chain
-    uint4 rv[TPS][8];
-#pragma unroll
-    for (int u = 0; u < TPS; ++u) {
-      const uint32_t rw[4] = {rb[u].x, rb[u].y, rb[u].z, rb[u].w};
-#pragma unroll
-      for (uint32_t s = 0; s < 8; ++s) {
-        const uint32_t word = rw[s >> 1];
-        const uint32_t b0 = (word >> (16 * (s & 1))) & 0xFFu;
-        const uint32_t b1 = (word >> (16 * (s & 1) + 8)) & 0xFFu;
-        const uint32_t pos = 16 * h + 2 * s;
-        const uint2 r0 = lut[pos * 256 + b0];
-        const uint2 r1 = lut[(pos + 1) * 256 + b1];
-        rv[u][s] = make_uint4(r0.x, r0.y, r1.x, r1.y);
-      }
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    floatx16_t acc[TPS];
-#pragma unroll
-    for (int u = 0; u < TPS; ++u) acc[u] = floatx16_t{};
-#pragma unroll
-    for (uint32_t s = 0; s < 8; ++s) {
-#pragma unroll
-      for (int u = 0; u < TPS; ++u) {
-        acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(cv[u][s]), qf[s], acc[u], 0, 0, 0);
-        acc[u] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(rv[u][s]), qf[s], acc[u], 0, 0, 0);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < TPS; ++u)
-#pragma unroll
-      for (uint32_t j = 0; j < 16; ++j) {
-        const uint32_t m = 32 * u + 8 * (j >> 2) + 4 * h + (j & 3);
-        if (i0 + m < L) M = fmaxf(M, acc[u][j]);
-      }
-  }
-  return fmaxf(M, __shfl_xor(M, 32));
+// ---------------------------------------------------------------------------
+// BF16 MaxSim core on v_mfma_f32_16x16x32_bf16, shared by the per-doc and the
+// persistent kernels (so their scores are bit-identical).  A tile is 16 doc
+// tokens (rows) x 128 dims against the 32 query tokens in two 16-column
+// halves qh.  Lane l = (row r = l & 15, group g = l >> 4): k-step s (s = 0..3)
+// of row r covers dims 32g + 8s .. +7.  The centroid rows are read from the
+// MaxSim-order copy C_bf16m, whose piece 4s + g holds exactly those dims, so
+// the four lanes of a row read 64 contiguous bytes per load, and group g's
+// residual bits are bytes 8g .. 8g + 7 of the token's row: one 8-byte load per
+// lane, no byte loaded twice.  Each k-step is two MFMAs per query half, the
+// centroid part and then the residual part, into one accumulator:
+// acc[qh][j] = row 4g + j, query token 16 qh + (l & 15).
+//
+// Residual operand: the nibble-pair table tab[pair_slot(p) + v] holds, as two
+// bf16, W[2p][v >> 2] and W[2p + 1][v & 3] (pair p = dims 2p, 2p + 1; the
+// nibble's high bits select dim 2p).  The four pairs 4k .. 4k+3 form a block
+// of 64 words at 64 k + 16 ((k >> 2) & 3).  A lookup instruction reads pair
+// 16g + 4s + i in group g, i.e. block 4g + s, and the blocks of the four
+// groups then start 272 words apart (16 g mod 64), in disjoint 16-bank sets of
+// a 64-bank LDS (and of each 32-lane half of a 32-bank one): no conflicts.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kPairTab = 64 * 16 + 48;  // words of the padded nibble-pair table
+
+__device__ __forceinline__ uint32_t pair_slot(uint32_t p) {
+  return 64 * (p >> 2) + 16 * ((p >> 4) & 3u) + 16 * (p & 3u);
 }
 
-// Query tokens as the B operand: lane (col, h) holds Q[col][64h .. 64h+63] in
-// bf16 (zero past the query's T tokens).
-__device__ __forceinline__ void load_q_bf16(const float *__restrict__ Q, uint32_t q, uint32_t t_max,
-                                            uint32_t T, uint32_t h, uint32_t col, bf16x8_t (&qf)[8]) {
-  const float *qr = Q + ((size_t)q * t_max + col) * kDim + 64 * h;
-#pragma unroll
-  for (uint32_t s = 0; s < 8; ++s) {
-    bf16x8_t v;
-#pragma unroll
-    for (uint32_t e = 0; e < 8; ++e) v[e] = (__bf16)(col < T ? qr[8 * s + e] : 0.0f);
-    qf[s] = v;
+// fills tab (kPairTab words of LDS) from the [128][4] weights W; nt threads
+__device__ __forceinline__ void fill_pair_tab(uint32_t *tab, const float *__restrict__ W, uint32_t tid,
+                                              uint32_t nt) {
+  for (uint32_t i = tid; i < kDim / 2 * 16; i += nt) {
+    const uint32_t pp = i >> 4, v = i & 15u;
+    const uint32_t lo = __builtin_bit_cast(uint16_t, (__bf16)W[(2 * pp) * 4 + (v >> 2)]);
+    const uint32_t hi = __builtin_bit_cast(uint16_t, (__bf16)W[(2 * pp + 1) * 4 + (v & 3u)]);
+    tab[pair_slot(pp) + v] = lo | (hi << 16);
   }
+}
+
+// The row's operands: pieces cv[s] = piece 4s + g of its MaxSim-order
+// centroid row Cm and residual bytes 8g .. 8g + 7 in rb.
+__device__ __forceinline__ void load_row16_bf16(const uint16_t *__restrict__ Cm, const uint8_t *__restrict__ residual,
+                                                uint64_t tok, uint32_t code, uint32_t g, uint4 (&cv)[4], uint2 &rb) {
+  const uint4 *cp = reinterpret_cast<const uint4 *>(Cm + (size_t)code * kDim) + g;
+#pragma unroll
+  for (uint32_t s = 0; s < 4; ++s) cv[s] = cp[4 * s];
+  rb = *reinterpret_cast<const uint2 *>(residual + tok * kResBytes + 8 * g);
+}
+
+// Residual operand of k-step s: dims 32g + 8s .. +7 from bytes 8g + 2s, +1.
+__device__ __forceinline__ uint4 res16_bf16(const uint32_t *tab, const uint2 &rb, uint32_t s, uint32_t g) {
+  const uint32_t w = ((s >> 1) ? rb.y : rb.x) >> (16 * (s & 1));
+  const uint32_t b0 = w & 0xFFu, b1 = (w >> 8) & 0xFFu;
+  const uint32_t *t = tab + 272 * g + 64 * s;  // pair_slot(16g + 4s)
+  return make_uint4(t[b0 >> 4], t[16 + (b0 & 15u)], t[32 + (b1 >> 4)], t[48 + (b1 & 15u)]);
+}
+
+// The 16 x 32 scores of one tile (see above).
+__device__ __forceinline__ void mma_tile16_bf16(const uint32_t *tab, const uint4 (&cv)[4], const uint2 &rb,
+                                                const bf16x8_t (&qv)[2][4], uint32_t g, floatx4_t (&acc)[2]) {
+  uint4 rv[4];
+#pragma unroll
+  for (uint32_t s = 0; s < 4; ++s) rv[s] = res16_bf16(tab, rb, s, g);
+  acc[0] = floatx4_t{};
+  acc[1] = floatx4_t{};
+#pragma unroll
+  for (uint32_t s = 0; s < 4; ++s) {
+#pragma unroll
+    for (uint32_t qh = 0; qh < 2; ++qh) {
+      acc[qh] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(cv[s]), qv[qh][s], acc[qh], 0, 0, 0);
+      acc[qh] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(rv[s]), qv[qh][s], acc[qh], 0, 0, 0);
+    }
+  }
+}
+
+// Query tokens as the 16x16x32 B operand: lane (col = l & 15, group g) holds
+// qv[qh][s] = Q[16 qh + col][32g + 8s .. +7] in bf16 (zero past T tokens).
+__device__ __forceinline__ void load_q16_bf16(const float *__restrict__ Q, uint32_t q, uint32_t t_max, uint32_t T,
+                                              uint32_t lane, bf16x8_t (&qv)[2][4]) {
+  const uint32_t col = lane & 15u, g = lane >> 4;
+#pragma unroll
+  for (uint32_t qh = 0; qh < 2; ++qh) {
+    const uint32_t t = 16 * qh + col;
+    const float4 *qr = reinterpret_cast<const float4 *>(Q + ((size_t)q * t_max + (t < T ? t : 0)) * kDim) + 8 * g;
+#pragma unroll
+    for (uint32_t s = 0; s < 4; ++s) {
+      const float4 a = qr[2 * s], b = qr[2 * s + 1];
+      const float x[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+      bf16x8_t v;
+#pragma unroll
+      for (uint32_t e = 0; e < 8; ++e) v[e] = (__bf16)(t < T ? x[e] : 0.0f);
+      qv[qh][s] = v;
+    }
+  }
+}
+
+// Per-query-token max over one doc's tokens; returns in lane l < 32 the max
+// for query token l.  TPS tiles of 16 tokens in flight per step; rows past
+// the doc's end load its last token and are left out of the max.
+template <int TPS>
+__device__ __forceinline__ float maxsim16_doc_bf16(uint32_t L, uint64_t off, uint32_t lane,
+                                                   const uint32_t *__restrict__ codes,
+                                                   const uint8_t *__restrict__ residual,
+                                                   const uint16_t *__restrict__ Cm, const uint32_t *tab,
+                                                   const bf16x8_t (&qv)[2][4], int clip_zero) {
+  const uint32_t r = lane & 15u, g = lane >> 4;
+  float M0 = clip_zero ? 0.0f : -FLT_MAX, M1 = M0;
+  for (uint32_t i0 = 0; i0 < L; i0 += 16 * TPS) {
+    uint4 cv[TPS][4];
+    uint2 rb[TPS];
+#pragma unroll
+    for (int u = 0; u < TPS; ++u) {
+      const uint64_t tok = off + min(i0 + 16 * u + r, L - 1);
+      load_row16_bf16(Cm, residual, tok, codes[tok], g, cv[u], rb[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < TPS; ++u) {
+      floatx4_t acc[2];
+      mma_tile16_bf16(tab, cv[u], rb[u], qv, g, acc);
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) {
+        if (i0 + 16 * u + 4 * g + j < L) {
+          M0 = fmaxf(M0, acc[0][j]);
+          M1 = fmaxf(M1, acc[1][j]);
+        }
+      }
+    }
+  }
+  M0 = fmaxf(M0, __shfl_xor(M0, 16));
+  M1 = fmaxf(M1, __shfl_xor(M1, 16));
+  M0 = fmaxf(M0, __shfl_xor(M0, 32));
+  M1 = fmaxf(M1, __shfl_xor(M1, 32));
+  return (lane & 16u) ? M1 : M0;
 }
 
 __global__ __launch_bounds__(256) void k_rerank_bf16(
@@ -1047,9 +1099,9 @@ __global__ __launch_bounds__(256) void k_rerank_bf16(
     const uint32_t *__restrict__ sel_ids, const uint32_t *__restrict__ sel_n, uint32_t R,
     uint32_t n_slice, const uint32_t *__restrict__ doclens, const uint64_t *__restrict__ docoff,
     const uint32_t *__restrict__ codes, const uint8_t *__restrict__ residual,
-    const uint16_t *__restrict__ Cb, const uint2 *__restrict__ lut_g, int clip_zero,
+    const uint16_t *__restrict__ Cm, const float *__restrict__ W, int clip_zero,
     float *__restrict__ sel_score) {
-  __shared__ uint2 lut[kResBytes * 256];
+  __shared__ uint32_t tab[kPairTab];
   __shared__ float Ms[4][kMaxQTok];
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const uint32_t h = lane >> 5, col = lane & 31;
@@ -1057,15 +1109,15 @@ __global__ __launch_bounds__(256) void k_rerank_bf16(
   const uint32_t T = q_ntok ? q_ntok[q] : t_max;
   const uint32_t n = sel_n[q];
 
-  for (uint32_t i = tid; i < kResBytes * 256; i += 256) lut[i] = lut_g[i];
-  bf16x8_t qf[8];
-  load_q_bf16(Q, q, t_max, T, h, col, qf);
+  fill_pair_tab(tab, W, tid, 256);
+  bf16x8_t qv[2][4];
+  load_q16_bf16(Q, q, t_max, T, lane, qv);
   __syncthreads();
 
   for (uint32_t r = slice * 4 + wave; r < n; r += n_slice * 4) {
     const uint32_t doc = sel_ids[(size_t)q * R + r];
-    const float M = maxsim_doc_bf16<1>(doclens[doc], docoff[doc], h, col, codes, residual, Cb, lut, qf,
-                                       clip_zero);
+    const float M = maxsim16_doc_bf16<1>(doclens[doc], docoff[doc], lane, codes, residual, Cm, tab, qv,
+                                         clip_zero);
     if (h == 0) Ms[wave][col] = M;
     wave_lds_sync();
     if (lane == 0) {
@@ -1504,334 +1556,6 @@ __global__ void k_lut_fp8(const float *__restrict__ W, float inv_sr, uint32_t *_
 }
 
 // ---------------------------------------------------------------------------
-// XCD-sliced MaxSim.  The centroid table is split into 8 slices
-// (slice = code * 8 / n_c), one per XCD, and each doc's tokens are stored
-// grouped by slice (sl_off[doc][0..8], offsets inside the doc).  Workgroup b
-// runs on XCD b % 8 (consecutive workgroups go to the XCDs in turn) and only
-// scores tokens of slice b % 8, so the centroid rows it gathers come from one
-// 1/8 of the table, which stays in that XCD's L2.  Work item = (query, chunk
-// of 32 candidates) from a per-slice queue, one wave per item: the chunk's
-// slice tokens are packed into full 32-token MFMA tiles, per-(doc, query
-// token) maxima are kept in LDS (ordered-uint atomicMax; rows of a tile are
-// sorted by doc, so each lane flushes one value per doc run), and the slice's
-// maxima go to part[q][r][slice][t].  k_maxsim_combine then sums over query
-// tokens the max over slices.  Every dot product is computed exactly as in the
-// per-doc kernels, so the scores are bit-identical to them.
-// ---------------------------------------------------------------------------
-constexpr uint32_t kXcd = 8;
-// candidates per work item (BF16 keeps its 64 KB decode table in LDS too, so
-// its per-wave maxima are halved to fit two workgroups per CU)
-template <int PREC> constexpr uint32_t xcd_docs() { return PREC == 1 ? 16 : 32; }
-
-__global__ __launch_bounds__(256) void k_slice_docs(
-    const uint32_t *__restrict__ doclens, const uint64_t *__restrict__ docoff, uint32_t n_doc,
-    uint32_t n_c, const uint32_t *__restrict__ codes, const uint8_t *__restrict__ residual,
-    uint32_t *__restrict__ codes_s, uint8_t *__restrict__ residual_s, uint16_t *__restrict__ sl_off) {
-  const uint32_t doc = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
-  if (doc >= n_doc) return;
-  const uint32_t L = doclens[doc];
-  const uint64_t off = docoff[doc];
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
-  uint32_t base = 0;
-  for (uint32_t x = 0; x < kXcd; ++x) {
-    if (lane == 0) sl_off[(size_t)doc * (kXcd + 1) + x] = (uint16_t)base;
-    for (uint32_t i0 = 0; i0 < L; i0 += kWave) {
-      const uint32_t i = i0 + lane;
-      uint32_t code = 0;
-      bool hit = false;
-      if (i < L) {
-        code = codes[off + i];
-        hit = (uint32_t)(((uint64_t)code * kXcd) / n_c) == x;
-      }
-      const uint64_t b = __ballot(hit);
-      if (hit) {
-        const uint64_t dst = off + base + (uint32_t)__popcll(b & lt);
-        codes_s[dst] = code;
-        const uint4 *src = reinterpret_cast<const uint4 *>(residual + (off + i) * kResBytes);
-        uint4 *dv = reinterpret_cast<uint4 *>(residual_s + dst * kResBytes);
-        dv[0] = src[0];
-        dv[1] = src[1];
-      }
-      base += (uint32_t)__popcll(b);
-    }
-  }
-  if (lane == 0) sl_off[(size_t)doc * (kXcd + 1) + kXcd] = (uint16_t)base;
-}
-
-__device__ __forceinline__ uint32_t ord_u(float f) { return f2ord(f); }
-
-// One lane's share of a 32-token tile: its token's centroid row half and
-// residual bytes (and, in FP8, the row scale), per precision.
-template <int PREC> struct XTile;
-template <> struct XTile<3> { float4 cv[16]; uint4 rb; float sc; };
-template <> struct XTile<1> { uint4 cv[8]; uint4 rb; float sc; };
-template <> struct XTile<2> { uint4 cv[4]; uint4 rb; float sc; };
-
-template <int PREC>
-__device__ __forceinline__ void xtile_load(XTile<PREC> &t, bool valid, uint32_t code, uint64_t g, uint32_t h,
-                                           const uint8_t *__restrict__ residual_s,
-                                           const float *__restrict__ C, const uint16_t *__restrict__ Cb,
-                                           const uint8_t *__restrict__ C8, const float *__restrict__ csc) {
-  constexpr uint32_t nv = PREC == 3 ? 16 : (PREC == 1 ? 8 : 4);
-  t.rb = make_uint4(0u, 0u, 0u, 0u);
-  t.sc = 0.0f;
-  if (valid) {
-    t.rb = *reinterpret_cast<const uint4 *>(residual_s + g * kResBytes + 16 * h);
-    if constexpr (PREC == 3) {
-      const float4 *cp = reinterpret_cast<const float4 *>(C + (size_t)code * kDim + 64 * h);
-#pragma unroll
-      for (uint32_t i = 0; i < nv; ++i) t.cv[i] = cp[i];
-    } else {
-      const uint4 *cp = reinterpret_cast<const uint4 *>(
-          PREC == 1 ? reinterpret_cast<const uint8_t *>(Cb + (size_t)code * kDim + 64 * h)
-                    : C8 + (size_t)code * kDim + 64 * h);
-#pragma unroll
-      for (uint32_t i = 0; i < nv; ++i) t.cv[i] = cp[i];
-      if constexpr (PREC == 2) t.sc = csc[code];
-    }
-  } else {
-#pragma unroll
-    for (uint32_t i = 0; i < nv; ++i) {
-      if constexpr (PREC == 3) t.cv[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-      else t.cv[i] = make_uint4(0u, 0u, 0u, 0u);
-    }
-  }
-}
-
-template <int PREC>
-__device__ __forceinline__ void xtile_mfma(const XTile<PREC> &t, uint32_t h, const float (&qf)[64],
-                                           const bf16x8_t (&qb)[8], const long (&q8)[8], const float *wl,
-                                           const uint2 *lut, const uint32_t *lut8, floatx16_t &acc,
-                                           floatx16_t &accR) {
-  const uint32_t rw[4] = {t.rb.x, t.rb.y, t.rb.z, t.rb.w};
-  if constexpr (PREC == 3) {
-    float4 xv[16];
-#pragma unroll
-    for (uint32_t sdx = 0; sdx < 16; ++sdx) {
-      const uint32_t byte = (rw[sdx >> 2] >> (8 * (sdx & 3))) & 0xFFu;
-      const float *wd = wl + 4 * (64 * h + 4 * sdx);
-      xv[sdx].x = t.cv[sdx].x + wd[(byte >> 6) & 3u];
-      xv[sdx].y = t.cv[sdx].y + wd[4 + ((byte >> 4) & 3u)];
-      xv[sdx].z = t.cv[sdx].z + wd[8 + ((byte >> 2) & 3u)];
-      xv[sdx].w = t.cv[sdx].w + wd[12 + (byte & 3u)];
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (uint32_t sdx = 0; sdx < 16; ++sdx) {
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(xv[sdx].x, qf[4 * sdx], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(xv[sdx].y, qf[4 * sdx + 1], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(xv[sdx].z, qf[4 * sdx + 2], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(xv[sdx].w, qf[4 * sdx + 3], acc, 0, 0, 0);
-    }
-  } else if constexpr (PREC == 1) {
-    uint4 rv[8];
-#pragma unroll
-    for (uint32_t sdx = 0; sdx < 8; ++sdx) {
-      const uint32_t word = rw[sdx >> 1];
-      const uint32_t b0 = (word >> (16 * (sdx & 1))) & 0xFFu;
-      const uint32_t b1 = (word >> (16 * (sdx & 1) + 8)) & 0xFFu;
-      const uint32_t pos = 16 * h + 2 * sdx;
-      const uint2 e0 = lut[pos * 256 + b0];
-      const uint2 e1 = lut[(pos + 1) * 256 + b1];
-      rv[sdx] = make_uint4(e0.x, e0.y, e1.x, e1.y);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (uint32_t sdx = 0; sdx < 8; ++sdx) {
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(t.cv[sdx]), qb[sdx], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16x8(rv[sdx]), qb[sdx], acc, 0, 0, 0);
-    }
-  } else {
-    long rv[8];
-#pragma unroll
-    for (uint32_t sdx = 0; sdx < 8; ++sdx) {
-      const uint32_t word = rw[sdx >> 1];
-      const uint32_t b0 = (word >> (16 * (sdx & 1))) & 0xFFu;
-      const uint32_t b1 = (word >> (16 * (sdx & 1) + 8)) & 0xFFu;
-      const uint32_t pos = 16 * h + 2 * sdx;
-      rv[sdx] = (long)(((uint64_t)lut8[(pos + 1) * 256 + b1] << 32) | lut8[pos * 256 + b0]);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-    for (uint32_t sdx = 0; sdx < 8; ++sdx) {
-      const uint4 c4 = t.cv[sdx >> 1];
-      const long a = (long)((sdx & 1) ? (((uint64_t)c4.w << 32) | c4.z) : (((uint64_t)c4.y << 32) | c4.x));
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a, q8[sdx], acc, 0, 0, 0);
-      accR = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(rv[sdx], q8[sdx], accR, 0, 0, 0);
-    }
-  }
-}
-
-template <int PREC>
-__global__ __launch_bounds__(256, 2) void k_rerank_xcd(
-    const float *__restrict__ Q, uint32_t t_max, const uint32_t *__restrict__ q_ntok,
-    const uint32_t *__restrict__ in_ids, const uint32_t *__restrict__ in_n, uint32_t R_in, uint32_t nb,
-    uint32_t *__restrict__ queue, const uint64_t *__restrict__ docoff,
-    const uint16_t *__restrict__ sl_off, const uint32_t *__restrict__ codes_s,
-    const uint8_t *__restrict__ residual_s, const float *__restrict__ C, const float *__restrict__ W,
-    const uint16_t *__restrict__ Cb, const uint2 *__restrict__ lut_g, const uint8_t *__restrict__ C8,
-    const float *__restrict__ csc, const uint32_t *__restrict__ lut8_g, float sr, int clip_zero,
-    float *__restrict__ part) {
-  __shared__ uint2 lut[PREC == 1 ? kResBytes * 256 : 1];
-  __shared__ uint32_t lut8[PREC == 2 ? kResBytes * 256 : 1];
-  __shared__ float wl[PREC == 3 ? kDim * 4 : 1];
-  constexpr uint32_t kXcdDocs = xcd_docs<PREC>();
-  __shared__ uint32_t Mloc[4][kXcdDocs][kMaxQTok];
-  __shared__ uint64_t dstart[4][kXcdDocs];
-  __shared__ uint32_t dpre[4][kXcdDocs + 1];
-  __shared__ uint8_t rowdoc[4][32];
-  __shared__ float rsc[4][32];
-  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-  const uint32_t h = lane >> 5, col = lane & 31;
-  const uint32_t x = blockIdx.x % kXcd;
-  if constexpr (PREC == 1)
-    for (uint32_t i = tid; i < kResBytes * 256; i += 256) lut[i] = lut_g[i];
-  else if constexpr (PREC == 2)
-    for (uint32_t i = tid; i < kResBytes * 256; i += 256) lut8[i] = lut8_g[i];
-  else
-    for (uint32_t i = tid; i < kDim * 4; i += 256) wl[i] = W[i];
-  __syncthreads();
-  const uint32_t cpq = (R_in + kXcdDocs - 1) / kXcdDocs;  // chunks per query
-  const uint32_t n_items = nb * cpq;
-  const uint32_t ident = ord_u(clip_zero ? 0.0f : -FLT_MAX);
-  bf16x8_t qb[8];
-  long q8[8];
-  float qf[64];
-  float sq = 0.0f;
-  uint32_t cur_q = 0xFFFFFFFFu;
-  for (;;) {
-    uint32_t it = 0;
-    if (lane == 0) it = atomicAdd(&queue[x], 1u);
-    it = __shfl(it, 0);
-    if (it >= n_items) break;
-    const uint32_t q = it / cpq, c = it - q * cpq;
-    const uint32_t n = min(in_n[q], R_in);
-    const uint32_t r0 = c * kXcdDocs;
-    if (r0 >= n) continue;
-    const uint32_t nd = min(kXcdDocs, n - r0);
-    const uint32_t T = q_ntok ? q_ntok[q] : t_max;
-    if (q != cur_q) {
-      if constexpr (PREC == 1) load_q_bf16(Q, q, t_max, T, h, col, qb);
-      else if constexpr (PREC == 2) load_q_fp8(Q, q, t_max, T, h, col, q8, sq);
-      else load_q_f32(Q, q, t_max, T, h, col, qf);
-      cur_q = q;
-    }
-    // this slice's token runs of the chunk's docs, packed back to back
-    uint32_t len = 0;
-    if (lane < nd) {
-      const uint32_t doc = in_ids[(size_t)q * R_in + r0 + lane];
-      const uint16_t *so = sl_off + (size_t)doc * (kXcd + 1);
-      const uint32_t a = so[x], b = so[x + 1];
-      len = b - a;
-      dstart[wave][lane] = docoff[doc] + a;
-    }
-    uint32_t incl = len;
-#pragma unroll
-    for (uint32_t o = 1; o < 32; o <<= 1) {
-      const uint32_t u = __shfl_up(incl, o, 32);
-      if (col >= o) incl += u;
-    }
-    if (lane < kXcdDocs) dpre[wave][lane + 1] = incl;
-    if (lane == 0) dpre[wave][0] = 0;
-    for (uint32_t i = lane; i < kXcdDocs * kMaxQTok; i += kWave) (&Mloc[wave][0][0])[i] = ident;
-    wave_lds_sync();
-    const uint32_t ntok = dpre[wave][nd];
-    const uint32_t ntiles = (ntok + 31) / 32;
-    // row col of tile k: flattened token k*32 + col -> (doc slot j, global token g)
-    auto locate = [&](uint32_t k, bool &valid, uint32_t &j, uint64_t &g) {
-      const uint32_t f = k * 32 + col;
-      valid = k < ntiles && f < ntok;
-      j = 0;
-      g = 0;
-      if (valid) {
-        uint32_t lo = 0, hi = nd;  // largest j with dpre[j] <= f
-        while (hi - lo > 1) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (dpre[wave][mid] <= f) lo = mid; else hi = mid;
-        }
-        j = lo;
-        g = dstart[wave][j] + (f - dpre[wave][j]);
-      }
-    };
-    // software pipeline: rows of tile k+1 and the code of tile k+2 are in
-    // flight while tile k is on the matrix unit
-    bool v0, v1;
-    uint32_t j0, j1;
-    uint64_t g0, g1;
-    locate(0, v0, j0, g0);
-    locate(1, v1, j1, g1);
-    const uint32_t code0 = v0 ? codes_s[g0] : 0u;
-    uint32_t code1 = v1 ? codes_s[g1] : 0u;
-    XTile<PREC> cur;
-    xtile_load<PREC>(cur, v0, code0, g0, h, residual_s, C, Cb, C8, csc);
-    for (uint32_t k = 0; k < ntiles; ++k) {
-      XTile<PREC> nxt;
-      xtile_load<PREC>(nxt, v1, code1, g1, h, residual_s, C, Cb, C8, csc);
-      bool v2;
-      uint32_t j2;
-      uint64_t g2;
-      locate(k + 2, v2, j2, g2);
-      const uint32_t code2 = v2 ? codes_s[g2] : 0u;
-      if (h == 0) {
-        rowdoc[wave][col] = v0 ? (uint8_t)j0 : (uint8_t)0xFF;
-        if constexpr (PREC == 2) rsc[wave][col] = cur.sc;
-      }
-      floatx16_t acc = {};
-      floatx16_t accR = {};
-      xtile_mfma<PREC>(cur, h, qf, qb, q8, wl, lut, lut8, acc, accR);
-      wave_lds_sync();  // rowdoc / rsc written
-      // per-(doc, query token) max: this lane's 16 rows are sorted by doc
-      uint32_t cur_d = 0xFFu;
-      float best = 0.0f;
-#pragma unroll
-      for (uint32_t jj = 0; jj < 16; ++jj) {
-        const uint32_t m = 8 * (jj >> 2) + 4 * h + (jj & 3);
-        const uint32_t dd = rowdoc[wave][m];
-        float v = acc[jj];
-        if constexpr (PREC == 2) v = sq * (rsc[wave][m] * acc[jj] + sr * accR[jj]);
-        if (dd != cur_d) {
-          if (cur_d != 0xFFu) atomicMax(&Mloc[wave][cur_d][col], ord_u(best));
-          cur_d = dd;
-          best = v;
-        } else {
-          best = fmaxf(best, v);
-        }
-      }
-      if (cur_d != 0xFFu) atomicMax(&Mloc[wave][cur_d][col], ord_u(best));
-      wave_lds_sync();  // before the next tile rewrites rowdoc / rsc
-      cur = nxt;
-      v0 = v1; j0 = j1;
-      v1 = v2; j1 = j2; g1 = g2; code1 = code2;
-    }
-    // the chunk's maxima for this slice: part[q][r][x][t]
-    for (uint32_t jd = h; jd < nd; jd += 2)
-      part[(((size_t)q * R_in + r0 + jd) * kXcd + x) * kMaxQTok + col] = ord2f(Mloc[wave][jd][col]);
-    wave_lds_sync();  // Mloc / dpre reused by the next item
-  }
-}
-
-// score[q][r] = sum over query tokens (left to right) of the max over slices.
-__global__ __launch_bounds__(256) void k_maxsim_combine(const float *__restrict__ part,
-                                                        const uint32_t *__restrict__ in_n, uint32_t R_in,
-                                                        uint32_t nb, const uint32_t *__restrict__ q_ntok,
-                                                        uint32_t t_max, float *__restrict__ score) {
-  const uint64_t i = (uint64_t)blockIdx.x * 8 + (threadIdx.x >> 5);   // (q, r), 32 lanes each
-  const uint32_t t = threadIdx.x & 31;
-  if (i >= (uint64_t)nb * R_in) return;
-  const uint32_t q = (uint32_t)(i / R_in), r = (uint32_t)(i - (uint64_t)q * R_in);
-  if (r >= min(in_n[q], R_in)) return;  // uniform over the 32 lanes
-  const float *p = part + i * kXcd * kMaxQTok;
-  float m = p[t];
-#pragma unroll
-  for (uint32_t x = 1; x < kXcd; ++x) m = fmaxf(m, p[x * kMaxQTok + t]);
-  const uint32_t T = q_ntok ? q_ntok[q] : t_max;
-  // left-to-right sum over t < T, gathered from the lanes of this half-wave
-  float sc = 0.0f;
-  for (uint32_t tt = 0; tt < T; ++tt) sc += __shfl(m, (int)((threadIdx.x & 32) + tt));
-  if (t == 0) score[i] = sc;
-}
-
-// ---------------------------------------------------------------------------
 // Block-wide top-kk (kk <= 256) of n unique keys make_key(sc[i], ids[i]),
 // written best first.  n <= 512: one bitonic sort in LDS.  Otherwise MSB-first
 // radix select with 8-bit digits: keys above the pivot digit go to the output,
@@ -1936,7 +1660,8 @@ __device__ void block_topk_small(const float *__restrict__ sc, const uint32_t *_
 // -- no cross-workgroup hand-off, so candidate lists of very different lengths
 // spread over the chip by the queue alone.  The decode tables are loaded into
 // LDS once per workgroup, the query operand once per query.
-// PREC 1: BF16 MFMA; PREC 3: FP32 MFMA.  TPS: 32-token tiles in flight.
+// PREC 1: BF16 MFMA; PREC 3: FP32 MFMA.  TPS: tiles in flight per step
+// (16 doc tokens each in BF16, 32 in FP32).
 // ---------------------------------------------------------------------------
 template <int PREC, int TPS>
 __global__ __launch_bounds__(256) void k_rerank_persist(
@@ -1945,11 +1670,11 @@ __global__ __launch_bounds__(256) void k_rerank_persist(
     uint32_t nb, uint32_t *__restrict__ queue, const uint32_t *__restrict__ doclens,
     const uint64_t *__restrict__ docoff, const uint32_t *__restrict__ codes,
     const uint8_t *__restrict__ residual, const float *__restrict__ C,
-    const float *__restrict__ W, const uint16_t *__restrict__ Cb, const uint2 *__restrict__ lut_g,
+    const float *__restrict__ W, const uint16_t *__restrict__ Cm,
     const uint8_t *__restrict__ C8, const float *__restrict__ csc, const uint32_t *__restrict__ lut8_g,
     float sr, int clip_zero, float *__restrict__ score, uint32_t kk, uint32_t *__restrict__ out_ids,
     float *__restrict__ out_s, uint32_t *__restrict__ out_n, uint32_t out_stride) {
-  __shared__ uint2 lut[PREC == 1 ? kResBytes * 256 : 1];
+  __shared__ uint32_t tabb[PREC == 1 ? kPairTab : 1];
   __shared__ uint32_t lut8[PREC == 2 ? kResBytes * 256 : 1];
   __shared__ float wl[PREC == 3 ? kDim * 4 : 1];
   __shared__ float Ms[4][kMaxQTok];
@@ -1959,12 +1684,12 @@ __global__ __launch_bounds__(256) void k_rerank_persist(
   const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   const uint32_t h = lane >> 5, col = lane & 31;
   if constexpr (PREC == 1)
-    for (uint32_t i = tid; i < kResBytes * 256; i += 256) lut[i] = lut_g[i];
+    fill_pair_tab(tabb, W, tid, 256);
   else if constexpr (PREC == 2)
     for (uint32_t i = tid; i < kResBytes * 256; i += 256) lut8[i] = lut8_g[i];
   else
     for (uint32_t i = tid; i < kDim * 4; i += 256) wl[i] = W[i];
-  bf16x8_t qb[PREC == 1 ? 8 : 1];
+  bf16x8_t qb[PREC == 1 ? 2 : 1][4];
   float qf[PREC == 3 ? 64 : 1];
   long q8[PREC == 2 ? 8 : 1];
   float sq = 0.0f;
@@ -1976,7 +1701,7 @@ __global__ __launch_bounds__(256) void k_rerank_persist(
     if (q >= nb) break;
     const uint32_t n = min(in_n[q], R_in);
     const uint32_t T = q_ntok ? q_ntok[q] : t_max;
-    if constexpr (PREC == 1) load_q_bf16(Q, q, t_max, T, h, col, qb);
+    if constexpr (PREC == 1) load_q16_bf16(Q, q, t_max, T, lane, qb);
     else if constexpr (PREC == 2) load_q_fp8(Q, q, t_max, T, h, col, q8, sq);
     else load_q_f32(Q, q, t_max, T, h, col, qf);
     float *qs = score + (size_t)q * R_in;
@@ -1984,8 +1709,8 @@ __global__ __launch_bounds__(256) void k_rerank_persist(
       const uint32_t doc = in_ids[(size_t)q * R_in + r];
       float M;
       if constexpr (PREC == 1)
-        M = maxsim_doc_bf16<TPS>(doclens[doc], docoff[doc], h, col, codes, residual, Cb, lut, qb,
-                                 clip_zero);
+        M = maxsim16_doc_bf16<TPS>(doclens[doc], docoff[doc], lane, codes, residual, Cm, tabb, qb,
+                                   clip_zero);
       else if constexpr (PREC == 2)
         M = maxsim_doc_fp8<TPS>(doclens[doc], docoff[doc], h, col, codes, residual, C8, csc, lut8, sr, q8,
                                 sq, rsc[wave], clip_zero);
@@ -2303,6 +2028,15 @@ __global__ __launch_bounds__(256) void k_ct_select(
 __global__ void k_to_bf16(const float *__restrict__ x, uint16_t *__restrict__ y, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = __builtin_bit_cast(uint16_t, (__bf16)x[i]);
+}
+
+// The MaxSim-order bf16 copy: y[c][8 (4s + g) + e] = bf16(x[c][32g + 8s + e]).
+__global__ void k_to_bf16_maxsim(const float *__restrict__ x, uint16_t *__restrict__ y, size_t n) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t d = (uint32_t)(i % kDim), piece = d >> 3, e = d & 7u;
+  const uint32_t s = piece >> 2, g = piece & 3u;
+  y[i] = __builtin_bit_cast(uint16_t, (__bf16)x[i - d + 32 * g + 8 * s + e]);
 }
 
 // ---------------------------------------------------------------------------
@@ -2728,10 +2462,10 @@ hipError_t launch_rerank(const float *Q, uint32_t t_max, const uint32_t *q_ntok,
                                       ix.codes, ix.residual, ix.C_fp8, ix.C_fp8_scale, ix.lut_fp8,
                                       ix.fp8_res_scale, clip_zero, sel_score);
   } else if (precision == 1) {
-    if (!ix.C_bf16 || !ix.lut_bf16) return hipErrorInvalidValue;
+    if (!ix.C_bf16m) return hipErrorInvalidValue;
     k_rerank_bf16<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, sel_ids, sel_n, R, n_slice, ix.doclens,
-                                       ix.docoff, ix.codes, ix.residual, ix.C_bf16, ix.lut_bf16,
-                                       clip_zero, sel_score);
+                                       ix.docoff, ix.codes, ix.residual, ix.C_bf16m, ix.W, clip_zero,
+                                       sel_score);
   } else {
     return hipErrorNotSupported;
   }
@@ -2745,7 +2479,7 @@ hipError_t launch_persist(const float *Q, uint32_t t_max, const uint32_t *q_ntok
                           uint32_t *out_n, uint32_t out_stride, hipStream_t s) {
   if (nb == 0) return hipSuccess;
   if (kk == 0 || kk > kTopkMax || cfg.n_wg == 0) return hipErrorInvalidValue;
-  if (precision == 1 && (!ix.C_bf16 || !ix.lut_bf16)) return hipErrorInvalidValue;
+  if (precision == 1 && !ix.C_bf16m) return hipErrorInvalidValue;
   if (precision == 2 && (!ix.C_fp8 || !ix.lut_fp8)) return hipErrorInvalidValue;
   k_fill_u32<<<1, 64, 0, s>>>(queue, 0u, 1);
   hipError_t e = hipGetLastError();
@@ -2753,7 +2487,7 @@ hipError_t launch_persist(const float *Q, uint32_t t_max, const uint32_t *q_ntok
   const uint32_t grid = std::min(cfg.n_wg, nb);
 #define MVR_PERSIST_ARGS                                                                           \
   Q, t_max, q_ntok, in_ids, in_n, R_in, nb, queue, ix.doclens, ix.docoff, ix.codes, ix.residual, \
-      ix.C, ix.W, ix.C_bf16, ix.lut_bf16, ix.C_fp8, ix.C_fp8_scale, ix.lut_fp8, ix.fp8_res_scale,   \
+      ix.C, ix.W, ix.C_bf16m, ix.C_fp8, ix.C_fp8_scale, ix.lut_fp8, ix.fp8_res_scale,   \
       clip_zero, score, kk, out_ids, out_s, out_n, out_stride
   if (precision == 1 && cfg.tps == 1) k_rerank_persist<1, 1><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
   else if (precision == 1) k_rerank_persist<1, 2><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
@@ -2763,41 +2497,6 @@ hipError_t launch_persist(const float *Q, uint32_t t_max, const uint32_t *q_ntok
   else if (precision == 3) k_rerank_persist<3, 2><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
   else return hipErrorNotSupported;
 #undef MVR_PERSIST_ARGS
-  return hipGetLastError();
-}
-
-hipError_t launch_slice_docs(const DevIndex &ix, uint32_t *codes_s, uint8_t *residual_s, uint16_t *sl_off,
-                             hipStream_t s) {
-  if (ix.n_doc == 0) return hipSuccess;
-  k_slice_docs<<<(ix.n_doc + 3) / 4, 256, 0, s>>>(ix.doclens, ix.docoff, ix.n_doc, ix.n_c, ix.codes, ix.residual,
-                                                  codes_s, residual_s, sl_off);
-  return hipGetLastError();
-}
-
-hipError_t launch_rerank_xcd(const float *Q, uint32_t t_max, const uint32_t *q_ntok, const uint32_t *in_ids,
-                             const uint32_t *in_n, uint32_t R_in, uint32_t nb, const DevIndex &ix,
-                             const XcdLayout &xl, int precision, int clip_zero, uint32_t wgs_per_xcd,
-                             uint32_t *queue, float *part, float *score, hipStream_t s) {
-  if (nb == 0) return hipSuccess;
-  if (!xl.codes_s || !xl.residual_s || !xl.sl_off || wgs_per_xcd == 0) return hipErrorInvalidValue;
-  if (precision == 1 && (!ix.C_bf16 || !ix.lut_bf16)) return hipErrorInvalidValue;
-  if (precision == 2 && (!ix.C_fp8 || !ix.lut_fp8)) return hipErrorInvalidValue;
-  k_fill_u32<<<1, 64, 0, s>>>(queue, 0u, kXcd);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  const uint32_t grid = wgs_per_xcd * kXcd;
-#define MVR_XCD_ARGS                                                                                   \
-  Q, t_max, q_ntok, in_ids, in_n, R_in, nb, queue, ix.docoff, xl.sl_off, xl.codes_s, xl.residual_s, ix.C, \
-      ix.W, ix.C_bf16, ix.lut_bf16, ix.C_fp8, ix.C_fp8_scale, ix.lut_fp8, ix.fp8_res_scale, clip_zero, part
-  if (precision == 1) k_rerank_xcd<1><<<grid, 256, 0, s>>>(MVR_XCD_ARGS);
-  else if (precision == 2) k_rerank_xcd<2><<<grid, 256, 0, s>>>(MVR_XCD_ARGS);
-  else if (precision == 3) k_rerank_xcd<3><<<grid, 256, 0, s>>>(MVR_XCD_ARGS);
-  else return hipErrorNotSupported;
-#undef MVR_XCD_ARGS
-  e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  const uint64_t pairs = (uint64_t)nb * R_in;
-  k_maxsim_combine<<<(unsigned)((pairs + 7) / 8), 256, 0, s>>>(part, in_n, R_in, nb, q_ntok, t_max, score);
   return hipGetLastError();
 }
 
@@ -3121,6 +2820,11 @@ hipError_t launch_convert_centroids(const DevIndex &ix, hipStream_t s) {
   const size_t n = (size_t)ix.n_c * kDim;
   if (ix.C_bf16) {
     k_to_bf16<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(ix.C, ix.C_bf16, n);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (ix.C_bf16m) {
+    k_to_bf16_maxsim<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(ix.C, ix.C_bf16m, n);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
   }

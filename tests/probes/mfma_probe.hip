@@ -14,6 +14,20 @@ __global__ void k_probe_bf16(const bf16x8 *a, const bf16x8 *b, floatx16 *d) {
   d[l] = acc;
 }
 
+// v_mfma_f32_16x16x32_bf16: the 4 outputs of each lane in elements 0..3
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+__global__ void k_probe_bf16_16(const bf16x8 *a, const bf16x8 *b, floatx16 *d) {
+  const uint32_t l = threadIdx.x;
+  floatx4 acc = {};
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[l], b[l], acc, 0, 0, 0);
+  floatx16 o = {};
+  o[0] = acc[0];
+  o[1] = acc[1];
+  o[2] = acc[2];
+  o[3] = acc[3];
+  d[l] = o;
+}
+
 __global__ void k_probe_fp8(const long *a, const long *b, floatx16 *d) {
   const uint32_t l = threadIdx.x;
   floatx16 acc = {};
@@ -28,6 +42,26 @@ __global__ void k_probe_cvt_fp8(const float *x, uint32_t *out, uint32_t n_pairs)
   out[i] = (uint32_t)__builtin_amdgcn_cvt_pk_fp8_f32(x[2 * i], x[2 * i + 1], 0, false) & 0xFFFFu;
 }
 
+// Direct global -> LDS loads of 16 bytes per lane (global_load_lds_dwordx4):
+// lane l's piece lands at the LDS address given as the destination plus the
+// immediate offset plus 16 l; the immediate offset is added to the global
+// address as well.  Two loads into a 192-entry buffer pre-filled with a
+// marker; every address stays inside src (128 pieces) and buf.
+typedef __attribute__((address_space(3))) void lds_void;
+typedef __attribute__((address_space(1))) void glb_void;
+__global__ void k_probe_gll(const uint4 *src, const uint32_t *perm, uint4 *out) {
+  __shared__ uint4 buf[192];
+  const uint32_t l = threadIdx.x;
+  for (uint32_t i = l; i < 192; i += 64) buf[i] = make_uint4(0xDEADu, 0xDEADu, 0xDEADu, 0xDEADu);
+  __syncthreads();
+  const uint4 *g = src + (perm[l] & 63u);
+  __builtin_amdgcn_global_load_lds((glb_void *)g, (lds_void *)(buf + 32), 16, 0, 0);
+  __builtin_amdgcn_global_load_lds((glb_void *)g, (lds_void *)(buf + 32), 16, 1024, 0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (uint32_t i = l; i < 192; i += 64) out[i] = buf[i];
+}
+
 // The first failing status of a sequence of HIP calls; later calls are
 // skipped once one has failed.
 struct FirstError {
@@ -38,8 +72,9 @@ struct FirstError {
   }
 };
 
+// kind 0: bf16 32x32x16, 1: fp8 32x32x16, 2: bf16 16x16x32
 extern "C" int probe_mfma(int kind, const void *a, const void *b, float *d) {
-  const size_t op = kind == 0 ? 16 : 8;
+  const size_t op = kind == 1 ? 8 : 16;
   void *da = nullptr, *db = nullptr, *dd = nullptr;
   FirstError err;
   err(hipMalloc(&da, 64 * op));
@@ -50,6 +85,8 @@ extern "C" int probe_mfma(int kind, const void *a, const void *b, float *d) {
   if (err.ok()) {
     if (kind == 0)
       k_probe_bf16<<<1, 64>>>((const bf16x8 *)da, (const bf16x8 *)db, (floatx16 *)dd);
+    else if (kind == 2)
+      k_probe_bf16_16<<<1, 64>>>((const bf16x8 *)da, (const bf16x8 *)db, (floatx16 *)dd);
     else
       k_probe_fp8<<<1, 64>>>((const long *)da, (const long *)db, (floatx16 *)dd);
     err(hipGetLastError());
@@ -106,4 +143,23 @@ extern "C" int probe_divsqrt(const float *a, const float *b, float *out, uint32_
   (void)hipFree(db);
   (void)hipFree(dout);
   return !err.ok();
+}
+
+extern "C" int probe_gll(const uint32_t *src, const uint32_t *perm, uint32_t *out) {
+  void *ds = nullptr, *dp = nullptr, *dout = nullptr;
+  FirstError err;
+  err(hipMalloc(&ds, 128 * 16));
+  if (err.ok()) err(hipMalloc(&dp, 64 * 4));
+  if (err.ok()) err(hipMalloc(&dout, 192 * 16));
+  if (err.ok()) err(hipMemcpy(ds, src, 128 * 16, hipMemcpyHostToDevice));
+  if (err.ok()) err(hipMemcpy(dp, perm, 64 * 4, hipMemcpyHostToDevice));
+  if (err.ok()) {
+    k_probe_gll<<<1, 64>>>((const uint4 *)ds, (const uint32_t *)dp, (uint4 *)dout);
+    err(hipGetLastError());
+  }
+  if (err.ok()) err(hipMemcpy(out, dout, 192 * 16, hipMemcpyDeviceToHost));
+  (void)hipFree(ds);
+  (void)hipFree(dp);
+  (void)hipFree(dout);
+  return err.ok() ? 0 : (int)err.e;
 }

@@ -82,24 +82,3 @@ def test_graph_on_device_buffers(c1):
         torch.cuda.synchronize()
         np.testing.assert_array_equal(oi.cpu().numpy().view(np.uint32), ref_ids)
         np.testing.assert_array_equal(os_.cpu().numpy(), ref_sc)
-
-
-@pytest.mark.parametrize("prec,rescore", [("fp32_mfma", 0), ("bf16", 64), ("fp8", 64)])
-def test_xcd_sliced_maxsim_matches_per_doc_kernels(c1, prec, rescore, monkeypatch):
-    """The XCD-sliced MaxSim (tokens regrouped by centroid slice, packed
-    tiles, per-slice maxima combined) against the per-doc kernels: every dot
-    product is computed the same way, so results are bit-identical."""
-    corpus, ix, Q = c1
-    g = _index(ix)
-    qt, qp = _ragged(Q, 3)
-    kw = dict(precision=prec, rescore=rescore, q_ntok=qt, q_nprobe=qp, batch=24)
-    ref_ids, ref_sc = g.search(Q, 8, 256, 10, **kw)
-    monkeypatch.setenv("MVR_MAXSIM_SLICED", "1")
-    for clip in (False, True):
-        ids, sc = g.search(Q, 8, 256, 10, clip_zero=clip, **kw)
-        monkeypatch.setenv("MVR_MAXSIM_SLICED", "0")
-        e_ids, e_sc = g.search(Q, 8, 256, 10, clip_zero=clip, **kw)
-        monkeypatch.setenv("MVR_MAXSIM_SLICED", "1")
-        np.testing.assert_array_equal(ids, e_ids)
-        np.testing.assert_array_equal(sc, e_sc)
-    np.testing.assert_array_equal(g.search(Q, 8, 256, 10, **kw)[0], ref_ids)

@@ -5,7 +5,14 @@ Assumed (v_mfma_f32_32x32x16_{bf16,fp8_fp8}):
   A: lane l holds A[l % 32][8*(l // 32) + e], e = 0..7
   B: lane l holds B[8*(l // 32) + e][l % 32]
   D: lane l, element j holds D[8*(j // 4) + 4*(l // 32) + j % 4][l % 32]
+Assumed (v_mfma_f32_16x16x32_bf16, the BF16 MaxSim):
+  A: lane l holds A[l % 16][8*(l // 16) + e], e = 0..7
+  B: lane l holds B[8*(l // 16) + e][l % 16]
+  D: lane l, element j holds D[4*(l // 16) + j][l % 16]
 FP8: OCP e4m3 (bias 7, 1.0 = 0x38).
+global_load_lds (16 bytes): lane l's piece lands at the LDS destination + the
+immediate offset + 16 l, the offset also being added to the global address
+(the MaxSim kernels stage centroid rows this way).
 FP32 sqrtf and '/' in device code are correctly rounded (the PLAID refine
 kernel's normalisation relies on it; __fsqrt_rn is not, on this toolchain).
 """
@@ -29,6 +36,8 @@ def probe():
     lib.probe_cvt_fp8.restype = ctypes.c_int
     lib.probe_divsqrt.argtypes = [vp, vp, vp, ctypes.c_uint32]
     lib.probe_divsqrt.restype = ctypes.c_int
+    lib.probe_gll.argtypes = [vp, vp, vp]
+    lib.probe_gll.restype = ctypes.c_int
     return lib
 
 
@@ -62,6 +71,26 @@ def test_bf16_32x32x16_layout(probe):
     d = np.zeros((64, 16), np.float32)
     assert probe.probe_mfma(0, a.ctypes.data, b.ctypes.data, d.ctypes.data) == 0
     np.testing.assert_array_equal(unpack_d(d), A @ B)
+
+
+def test_bf16_16x16x32_layout(probe):
+    rng = np.random.default_rng(2)
+    A = rng.integers(-8, 9, (16, 32)).astype(np.float32)
+    B = rng.integers(-8, 9, (32, 16)).astype(np.float32)
+    a = np.zeros((64, 8), np.uint16)
+    b = np.zeros((64, 8), np.uint16)
+    for l in range(64):
+        for e in range(8):
+            a[l, e] = bf16_bits(A[l % 16, 8 * (l // 16) + e])
+            b[l, e] = bf16_bits(B[8 * (l // 16) + e, l % 16])
+    d = np.zeros((64, 16), np.float32)
+    assert probe.probe_mfma(2, a.ctypes.data, b.ctypes.data, d.ctypes.data) == 0
+    D = np.zeros((16, 16), np.float32)
+    for l in range(64):
+        for j in range(4):
+            D[4 * (l // 16) + j, l % 16] = d[l, j]
+    np.testing.assert_array_equal(D, A @ B)
+    assert not d[:, 4:].any()
 
 
 def ocp_e4m3(v):
@@ -104,3 +133,15 @@ def test_fp32_sqrt_and_division_correctly_rounded(probe):
     assert probe.probe_divsqrt(a.ctypes.data, b.ctypes.data, out.ctypes.data, n) == 0
     np.testing.assert_array_equal(out[:, 2].view(np.uint32), np.sqrt(a).view(np.uint32))
     np.testing.assert_array_equal(out[:, 3].view(np.uint32), (a / b).view(np.uint32))
+
+
+def test_global_load_lds_lane_layout(probe):
+    rng = np.random.default_rng(3)
+    src = rng.integers(0, 2**32, (128, 4), dtype=np.uint64).astype(np.uint32)
+    perm = rng.permutation(64).astype(np.uint32)
+    out = np.zeros((192, 4), np.uint32)
+    assert probe.probe_gll(src.ctypes.data, perm.ctypes.data, out.ctypes.data) == 0
+    want = np.full((192, 4), 0xDEAD, np.uint32)
+    want[32:96] = src[perm]            # destination buf + 32 pieces, offset 0
+    want[96:160] = src[perm + 64]      # offset 1024 bytes = 64 pieces, on both addresses
+    np.testing.assert_array_equal(out, want)
