@@ -142,15 +142,20 @@ def exhaustive_topk(torch, index, Q, k, n_doc, batch=512, rescore=64):
     return out
 
 
-def make_labels(torch, index, Q, k, R_max, n_doc, target=0.99, batch=1024, log=None):
+def make_labels(torch, index, Q, k, R_max, n_doc, target=0.99, batch=1024, log=None, gnd=None):
     """Features and labels for training queries Q (torch [n, 32, 128]).
-    Returns (features [n, 1024], class index [n], log R needed [n], recall
-    table [n, 5] at R_max, exhaustive top-k ids [n, k])."""
+    The reference ranking is exhaustive MaxSim over the index, or the given
+    top-k ids gnd [n, k].  Returns (features [n, 1024], class index [n],
+    log R needed [n], recall table [n, 5] at R_max, reference top-k ids
+    [n, k])."""
     dev = Q.device
     n_q = Q.shape[0]
-    gnd = exhaustive_topk(torch, index, Q, k, n_doc)
-    if log:
-        log(f"labels: exhaustive top-{k} of {n_q} queries done")
+    if gnd is None:
+        gnd = exhaustive_topk(torch, index, Q, k, n_doc)
+        if log:
+            log(f"labels: exhaustive top-{k} of {n_q} queries done")
+    else:
+        gnd = torch.as_tensor(np.asarray(gnd, np.int64), device=dev)
     feats = np.empty((n_q, 1024), np.float32)
     rec = torch.zeros((n_q, len(CLASSES)), dtype=torch.float64, device=dev)
     r_need = torch.full((n_q, len(CLASSES)), float(R_max), dtype=torch.float64, device=dev)
