@@ -85,6 +85,8 @@ def main():
     ap.add_argument("--recall-queries", type=int, default=256)
     ap.add_argument("--cpu-queries", type=int, default=640)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-others", action="store_true",
+                    help="skip the plain FP32 pipelines and the candidate counts (profiling runs)")
     ap.add_argument("--persistent", action="store_true",
                     help="MaxSim + top-k as one persistent work-queue kernel per pass")
     ap.add_argument("--graph", action="store_true", help="replay each device pass from a hipGraph")
@@ -200,7 +202,7 @@ def main():
     # the plain FP32 pipelines on the same batch (the reference of the
     # north_star's speed-up), and their answers for the recall and CPU checks
     others = {}
-    if q_nt is None and not args.residual_on_host:
+    if q_nt is None and not args.residual_on_host and not args.no_others:
         for prec in ("fp32", "fp32_mfma"):
             if prec == args.precision:
                 continue
@@ -211,7 +213,8 @@ def main():
 
     # the exhaustive 2-bit ranking needs the device-resident index
     recall = recall_report(torch, args, cfg, corpus, corpus.index, Q, answers, k, doclens, q_nt)
-    cands = candidate_counts(ix, Q, P, R, k, q_nt, q_np, min(n_q, 1024)) if not args.residual_on_host else None
+    cands = (candidate_counts(ix, Q, P, R, k, q_nt, q_np, min(n_q, 1024))
+             if not (args.residual_on_host or args.no_others) else None)
     cpu = None
     if not args.no_cpu and args.cpu_queries > 0 and q_nt is None:
         ref = others.get("fp32") or (run if args.precision == "fp32" else None)
