@@ -78,7 +78,7 @@ typedef struct {
 typedef struct {
   uint32_t nprobe;      /* centroids probed per query token (-nprobe), at most
                            min(n_centroid, 256); larger is MVR_ERR_UNSUPPORTED */
-  uint32_t probe_topk;  /* docs kept by the filter and reranked (-probe_topk) */
+  uint32_t probe_topk;  /* docs kept by the filter and reranked (-probe_topk), <= 16384 */
   uint32_t topk;        /* results per query (-topk) */
   uint32_t precision;   /* mvr_precision of centroid scoring and MaxSim */
   uint32_t rescore;     /* BF16/FP8: FP32 rescoring of the best `rescore` docs (0 = off) */

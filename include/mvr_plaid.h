@@ -96,7 +96,7 @@ int mvr_plaid_segmented_maxsim(const float *scores, uint32_t nq, const int64_t *
  * identity -9999, summed over t) and keeps the best ndocs by (score, pid)
  * descending; stage 2 rescores those over all their codes and keeps ndocs/4,
  * written to out_pids in that order.  centroid_scores f32 [n_centroid][nq]
- * (nq <= 32), idx u8 [n_centroid], ndocs <= 4096.  *out_n (host) =
+ * (nq <= 32), idx u8 [n_centroid], ndocs <= 16384.  *out_n (host) =
  * min(ndocs/4, min(ndocs, npids)) (the reference requires npids >= ndocs);
  * *n_vec_score_refine (host, nullable) = nq x the number of distinct codes
  * visited by both stages.  Synchronises stream. */
@@ -134,7 +134,7 @@ void mvr_plaid_index_free(mvr_plaid_index *ix);
 
 /* Searcher.dense_search settings (searcher.py:291-312): ncells centroids per
  * query token, centroid_score_threshold (idx = max_t S >= threshold), ndocs
- * (<= 4096; stage 2 keeps ndocs/4), k results, queries per device pass. */
+ * (<= 16384; stage 2 keeps ndocs/4), k results, queries per device pass. */
 typedef struct {
   uint32_t k, ncells, ndocs, batch;
   float centroid_score_threshold;

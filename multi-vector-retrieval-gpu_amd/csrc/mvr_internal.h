@@ -16,7 +16,7 @@ constexpr uint32_t kProbeFeat = 32;        // centroid scores per token read by 
 constexpr uint32_t kSeg = 4096;            // docs per filter segment (LDS state)
 constexpr uint64_t kPoolBudget = 16ull << 30;  // candidate pool bytes per device pass (default)
 constexpr uint64_t kPoolBudgetMin = 2ull << 30, kPoolBudgetMax = 64ull << 30;
-constexpr uint32_t kMaxSelect = 4096;      // largest probe_topk
+constexpr uint32_t kMaxSelect = 16384;     // largest probe_topk (and PLAID ndocs)
 constexpr double kFxScale = 4294967296.0;  // filter fixed point: 2^32
 
 // Device-resident index (layout mirrors the reference files).

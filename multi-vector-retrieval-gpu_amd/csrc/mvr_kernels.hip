@@ -619,15 +619,20 @@ __device__ __forceinline__ uint32_t pow2_ceil(uint32_t n) {
 // the output; the pivot bucket is refined until it fits in LDS, then sorted.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kSelThreads = 512;
-constexpr uint32_t kSelScratch = 2048;
+constexpr uint32_t kSelSmall = 4096;  // key capacity of the common variant (R <= 4096)
 
-constexpr uint32_t kSelMaxSeg = 4096;  // segments indexed in LDS (16.7M docs); more: segment-by-segment walk
-
+// CAP: keys held in LDS (R <= CAP).  The 4,096-key variant keeps a 2,048-key
+// pivot scratch and indexes up to 4,096 segments (16.7M docs); the
+// 16,384-key one (128 KB of keys) a 1,024-key scratch and 2,048 segments,
+// walking larger lists segment by segment.
+template <uint32_t CAP>
 __global__ __launch_bounds__(kSelThreads) void k_select_topr(
     const uint64_t *__restrict__ cand_keys, const uint32_t *__restrict__ seg_cnt, uint32_t n_seg,
     const uint32_t *__restrict__ q_r, uint32_t R_out, uint32_t *__restrict__ sel_ids,
     float *__restrict__ sel_F, uint32_t *__restrict__ sel_n, uint32_t *__restrict__ cand_tot) {
-  __shared__ unsigned long long outk[kMaxSelect];
+  constexpr uint32_t kSelScratch = CAP > kSelSmall ? 1024 : 2048;
+  constexpr uint32_t kSelMaxSeg = CAP > kSelSmall ? 2048 : 4096;
+  __shared__ unsigned long long outk[CAP];
   __shared__ unsigned long long scr[kSelScratch];
   __shared__ uint32_t hist[2048];
   __shared__ uint32_t spfx[kSelMaxSeg + 1];  // exclusive prefix of the segment counts
@@ -714,7 +719,7 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
   if (tid == 0) { s_out_n = 0; s_scr_n = 0; }
   __syncthreads();
 
-  if (n <= kMaxSelect) {
+  if (n <= CAP) {
     const uint32_t np = pow2_ceil(max(n, 2u));
     if (flat) {
       uint32_t sg = 0;
@@ -735,7 +740,7 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
     __syncthreads();
     bitonic_desc(outk, np);
   } else {
-    // n > kMaxSelect >= R: the pivot search below always finds a digit.
+    // n > CAP >= R: the pivot search below always finds a digit.
     uint64_t prefix = 0;
     uint32_t shift = 64;
     uint32_t need = R;
@@ -2108,11 +2113,12 @@ __global__ __launch_bounds__(256) void k_exhaustive_f64(
 // ---------------------------------------------------------------------------
 // Final top-k per query: sort the n reranked (score, doc) keys in LDS.
 // ---------------------------------------------------------------------------
+template <uint32_t CAP>
 __global__ __launch_bounds__(kSelThreads) void k_topk_final(
     const uint32_t *__restrict__ ids, const float *__restrict__ score,
     const uint32_t *__restrict__ n_in, uint32_t R, uint32_t k, uint32_t *__restrict__ out_ids,
     float *__restrict__ out_s, uint32_t *__restrict__ out_n) {
-  __shared__ unsigned long long s[kMaxSelect];
+  __shared__ unsigned long long s[CAP];
   const uint32_t q = blockIdx.x, tid = threadIdx.x;
   const uint32_t n = n_in[q];
   const uint32_t np = pow2_ceil(max(n, 2u));
@@ -2434,8 +2440,12 @@ hipError_t launch_select(const uint64_t *cand_keys, const uint32_t *seg_cnt, uin
                          const uint32_t *q_r, uint32_t n_q, uint32_t R, uint32_t *sel_ids,
                          float *sel_F, uint32_t *sel_n, uint32_t *cand_tot, hipStream_t s) {
   if (n_q == 0) return hipSuccess;
-  k_select_topr<<<n_q, kSelThreads, 0, s>>>(cand_keys, seg_cnt, n_seg, q_r, R, sel_ids, sel_F,
-                                            sel_n, cand_tot);
+  if (R <= kSelSmall)
+    k_select_topr<kSelSmall><<<n_q, kSelThreads, 0, s>>>(cand_keys, seg_cnt, n_seg, q_r, R, sel_ids, sel_F,
+                                                         sel_n, cand_tot);
+  else
+    k_select_topr<kMaxSelect><<<n_q, kSelThreads, 0, s>>>(cand_keys, seg_cnt, n_seg, q_r, R, sel_ids, sel_F,
+                                                          sel_n, cand_tot);
   return hipGetLastError();
 }
 
@@ -2765,7 +2775,10 @@ hipError_t launch_topk(const uint32_t *ids, const float *score, const uint32_t *
                        uint32_t n_q, uint32_t k, uint32_t *out_ids, float *out_s, uint32_t *out_n,
                        hipStream_t s) {
   if (n_q == 0) return hipSuccess;
-  k_topk_final<<<n_q, kSelThreads, 0, s>>>(ids, score, n_in, R, k, out_ids, out_s, out_n);
+  if (R <= kSelSmall)
+    k_topk_final<kSelSmall><<<n_q, kSelThreads, 0, s>>>(ids, score, n_in, R, k, out_ids, out_s, out_n);
+  else
+    k_topk_final<kMaxSelect><<<n_q, kSelThreads, 0, s>>>(ids, score, n_in, R, k, out_ids, out_s, out_n);
   return hipGetLastError();
 }
 

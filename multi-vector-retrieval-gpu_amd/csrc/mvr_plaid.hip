@@ -624,19 +624,23 @@ __global__ void k_pl_segcnt(const uint32_t *__restrict__ n_in, uint32_t n_fixed,
   seg_cnt[e] = n > lo ? min(n - lo, kSeg) : 0u;
 }
 
-// Segment counts of the split layout: heavy keys in segments 0 .. n_seg-2
-// (heavy_n of them, flat), light keys in the last segment (light_n).
+// Segment counts of the split layout: heavy keys in segments 0 .. n_heavy-1
+// (heavy_n of them, flat), light keys (light_n <= ndocs of them, flat) in
+// the segments from n_heavy on.
 __global__ void k_pl_segcnt_split(const uint32_t *__restrict__ heavy_n, const uint32_t *__restrict__ light_n,
-                                  uint32_t n_seg, uint32_t n_q, uint32_t *__restrict__ seg_cnt) {
+                                  uint32_t n_heavy, uint32_t n_seg, uint32_t n_q, uint32_t *__restrict__ seg_cnt) {
   const uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (uint64_t)n_q * n_seg) return;
   const uint32_t q = (uint32_t)(e / n_seg), sg = (uint32_t)(e % n_seg);
-  if (sg + 1 == n_seg) {
-    seg_cnt[e] = light_n[q];
-  } else {
-    const uint32_t n = heavy_n[q], lo = sg * kSeg;
-    seg_cnt[e] = n > lo ? min(n - lo, kSeg) : 0u;
-  }
+  const bool light = sg >= n_heavy;
+  const uint32_t n = light ? light_n[q] : heavy_n[q], lo = (light ? sg - n_heavy : sg) * kSeg;
+  seg_cnt[e] = n > lo ? min(n - lo, kSeg) : 0u;
+}
+
+// Light-key segments of the split layout: the stage-1 top-ndocs takes at most
+// min(ndocs, n_doc) of them.
+uint32_t pl_light_segs(uint32_t ndocs, uint32_t n_doc) {
+  return std::max(1u, (std::min(ndocs, n_doc) + kSeg - 1) / kSeg);
 }
 
 // ---------------------------------------------------------------------------
@@ -1071,8 +1075,10 @@ hipError_t launch_filter_stages(const uint32_t *in_ids, const uint32_t *in_n, ui
                                 uint32_t *seg1, uint32_t *sel1_ids, float *sel1_F, uint32_t *sel1_n,
                                 uint64_t *keys2, uint32_t *sel2_ids, float *sel2_F, uint32_t *sel2_n,
                                 unsigned long long *nvec, hipStream_t s) {
-  // with slots (split layout): heavy keys flat from 0, light keys in one more segment
-  const uint32_t n_seg1 = std::max(1u, (in_cap + kSeg - 1) / kSeg) + (slots ? 1u : 0u);
+  // with slots (split layout): heavy keys flat from 0, light keys in the
+  // pl_light_segs segments after them
+  const uint32_t n_heavy = std::max(1u, (in_cap + kSeg - 1) / kSeg);
+  const uint32_t n_seg1 = n_heavy + (slots ? pl_light_segs(ndocs, in_cap) : 0u);
   const uint32_t wpb = approx_waves(n_c);
   const size_t lds = (size_t)wpb * ((n_c + 31) / 32) * 4;
   // about 16k workgroups: one wave per doc is latency-bound, so many in flight
@@ -1086,19 +1092,26 @@ hipError_t launch_filter_stages(const uint32_t *in_ids, const uint32_t *in_n, ui
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (slots)
-    k_pl_segcnt_split<<<grid_for((uint64_t)n_q * n_seg1, 256), 256, 0, s>>>(slot_n, light_n, n_seg1, n_q, seg1);
+    k_pl_segcnt_split<<<grid_for((uint64_t)n_q * n_seg1, 256), 256, 0, s>>>(slot_n, light_n, n_heavy, n_seg1, n_q,
+                                                                             seg1);
   else
     k_pl_segcnt<<<grid_for((uint64_t)n_q * n_seg1, 256), 256, 0, s>>>(in_n, n_fixed, n_seg1, n_q, seg1);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = launch_select(keys1, seg1, n_seg1, nullptr, n_q, ndocs, sel1_ids, sel1_F, sel1_n, nullptr, s)) !=
       hipSuccess)
     return e;
+  // stage 2: one key per stage-1 pid, contiguous per query over n_seg2
+  // segments (ndocs may exceed one segment); the stage-1 counts buffer
+  // (n_seg1 >= n_seg2 per query) is reused for the segment counts
+  const uint32_t n_seg2 = std::max(1u, (std::min(ndocs, in_cap) + kSeg - 1) / kSeg);
   k_pl_approx<false, true><<<grid, 256, lds, s>>>(sel1_ids, sel1_n, 0, ndocs, nullptr, nullptr, S,
                                                   (uint64_t)n_c * nq, nullptr,
                                                   0, n_c, nq, n_doc, codes, doclens, offsets, nullptr, 0, wpb,
-                                                  (unsigned long long *)keys2, kSeg, nvec);
+                                                  (unsigned long long *)keys2, (uint64_t)n_seg2 * kSeg, nvec);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  return launch_select(keys2, sel1_n, 1, nullptr, n_q, ndocs / 4, sel2_ids, sel2_F, sel2_n, nullptr, s);
+  k_pl_segcnt<<<grid_for((uint64_t)n_q * n_seg2, 256), 256, 0, s>>>(sel1_n, 0, n_seg2, n_q, seg1);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return launch_select(keys2, seg1, n_seg2, nullptr, n_q, ndocs / 4, sel2_ids, sel2_F, sel2_n, nullptr, s);
 }
 
 }  // namespace
@@ -1258,7 +1271,7 @@ int mvr_plaid_filter_pids(const int32_t *pids, uint32_t npids, const float *cent
   PL_TRY(w.sel1_ids.ensure(sizeof(uint32_t) * ndocs));
   PL_TRY(w.sel1_F.ensure(sizeof(float) * ndocs));
   PL_TRY(w.sel1_n.ensure(sizeof(uint32_t)));
-  PL_TRY(w.keys2.ensure(sizeof(uint64_t) * kSeg));
+  PL_TRY(w.keys2.ensure(sizeof(uint64_t) * (size_t)((ndocs + kSeg - 1) / kSeg) * kSeg));
   PL_TRY(w.sel2_ids.ensure(sizeof(uint32_t) * (ndocs / 4)));
   PL_TRY(w.sel2_F.ensure(sizeof(float) * (ndocs / 4)));
   PL_TRY(w.sel2_n.ensure(sizeof(uint32_t)));
@@ -1447,15 +1460,16 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
   hipStream_t s = ix->stream;
   const uint32_t n_c = ix->n_c, n_doc = ix->n_doc, ndocs = p->ndocs, R2 = ndocs / 4, k = p->k;
   const uint32_t cell_cap = nq * p->ncells;
-  const uint32_t n_seg1 = (n_doc + kSeg - 1) / kSeg;
+  const uint32_t n_seg1 = (n_doc + kSeg - 1) / kSeg, n_light = pl_light_segs(ndocs, n_doc);
   // device bytes per query in a pass
   const size_t per_q = sizeof(float) * kDim * kMaxQTok + sizeof(float) * (size_t)n_c * nq + n_c +
                        (((size_t)n_c + kCellChunk - 1) / kCellChunk + 1) * 8 * (size_t)kMaxQTok * p->ncells +
                        8 * (((size_t)n_c + kGt - 1) / kGt) * kMaxQTok +
                        sizeof(uint32_t) * ((size_t)n_doc + ix->bm_words + kPassCap + 3) +
                        sizeof(uint32_t) * (cell_cap + 1) + sizeof(uint32_t) * ix->bm_words +
-                       sizeof(uint32_t) * (n_doc + 1) + sizeof(uint64_t) * (size_t)(n_seg1 + 1) * kSeg +
-                       sizeof(uint32_t) * (n_seg1 + 2) + 12 * (size_t)ndocs + 8 * (size_t)kSeg + 16 * (size_t)R2 +
+                       sizeof(uint32_t) * (n_doc + 1) + sizeof(uint64_t) * (size_t)(n_seg1 + n_light) * kSeg +
+                       sizeof(uint32_t) * (n_seg1 + n_light + 1) + 12 * (size_t)ndocs +
+                       8 * (size_t)((ndocs + kSeg - 1) / kSeg) * kSeg + 16 * (size_t)R2 +
                        8 * (size_t)k + 16;
   uint32_t B = p->batch ? p->batch : 1024;
   B = std::min<uint32_t>(B, n_q);
@@ -1490,13 +1504,13 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
   PL_TRY(ix->heavy_n.ensure(sizeof(uint32_t) * B));
   PL_TRY(ix->cand.ensure(sizeof(uint32_t) * (size_t)B * n_doc));
   PL_TRY(ix->cand_n.ensure(sizeof(uint32_t) * B));
-  PL_TRY(ix->keys1.ensure(sizeof(uint64_t) * (size_t)B * (n_seg1 + 1) * kSeg));
-  PL_TRY(ix->seg1.ensure(sizeof(uint32_t) * (size_t)B * (n_seg1 + 1)));
+  PL_TRY(ix->keys1.ensure(sizeof(uint64_t) * (size_t)B * (n_seg1 + n_light) * kSeg));
+  PL_TRY(ix->seg1.ensure(sizeof(uint32_t) * (size_t)B * (n_seg1 + n_light)));
   PL_TRY(ix->light_n.ensure(sizeof(uint32_t) * B));
   PL_TRY(ix->sel1_ids.ensure(sizeof(uint32_t) * (size_t)B * ndocs));
   PL_TRY(ix->sel1_F.ensure(sizeof(float) * (size_t)B * ndocs));
   PL_TRY(ix->sel1_n.ensure(sizeof(uint32_t) * B));
-  PL_TRY(ix->keys2.ensure(sizeof(uint64_t) * (size_t)B * kSeg));
+  PL_TRY(ix->keys2.ensure(sizeof(uint64_t) * (size_t)B * ((ndocs + kSeg - 1) / kSeg) * kSeg));
   PL_TRY(ix->sel2_ids.ensure(sizeof(uint32_t) * (size_t)B * R2));
   PL_TRY(ix->sel2_F.ensure(sizeof(float) * (size_t)B * R2));
   PL_TRY(ix->sel2_n.ensure(sizeof(uint32_t) * B));
@@ -1578,11 +1592,12 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
                                     ix->pass_bm.as<uint32_t>(), ix->bm_words);
     PL_TRY(hipGetLastError());
     {
-      const uint32_t n_seg1 = (n_doc + kSeg - 1) / kSeg + 1;
+      // key layout of launch_filter_stages: heavy segments, then light ones
+      const uint32_t n_heavy = (n_doc + kSeg - 1) / kSeg, n_keyseg = n_heavy + pl_light_segs(ndocs, n_doc);
       k_pl_split<<<b, 1024, 0, s>>>(ix->cand.as<uint32_t>(), n_doc, ix->cand_n.as<uint32_t>(), ix->pmode.as<uint32_t>(),
                                     ix->pass_bm.as<uint32_t>(), ix->bm_words, nq, ndocs,
-                                    ix->keys1.as<unsigned long long>(), (uint64_t)n_seg1 * kSeg,
-                                    (uint64_t)(n_seg1 - 1) * kSeg, ix->heavy.as<uint32_t>(),
+                                    ix->keys1.as<unsigned long long>(), (uint64_t)n_keyseg * kSeg,
+                                    (uint64_t)n_heavy * kSeg, ix->heavy.as<uint32_t>(),
                                     ix->heavy_n.as<uint32_t>(), ix->light_n.as<uint32_t>());
       PL_TRY(hipGetLastError());
     }

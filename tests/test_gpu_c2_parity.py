@@ -72,3 +72,13 @@ def test_c2_bf16_centroid_topk_exact_on_rounded_inputs(c2):
     ti, ts = O.centroid_topk(O.bf16_round(Q[:8]), O.bf16_round(ix.C), P)
     np.testing.assert_array_equal(st["top_ids"], ti.reshape(8, Q.shape[1], P))
     np.testing.assert_array_equal(st["top_s"], ts.reshape(8, Q.shape[1], P))
+
+
+def test_c2_probe_topk_above_4096_bit_exact(c2):
+    """probe_topk 8,192 (the large select and top-k variants: 240k candidates
+    per query go through the radix select into 16,384 LDS slots): fp32 bit-exact
+    against the oracle at every stage."""
+    ix, Q, g, _ = c2
+    ref = _oracle_stages(ix, Q[:4], P, 8192, K)
+    ids, sc, st = g.search(Q[:4], P, 8192, K, stages=True, batch=4)
+    _check(st, ids, sc, ref)

@@ -108,6 +108,16 @@ def test_config1_fp32_bit_exact(c1):
     _check(st, ids, sc, _oracle_stages(ix, Q, 4, 256, 10))
 
 
+def test_config1_probe_topk_above_4096_bit_exact(c1):
+    """probe_topk 6,000 of the 8,192 docs (the 16,384-key select and top-k
+    variants, all candidates sorted in LDS), fp32 bit-exact at every stage."""
+    corpus, ix, Q = c1
+    g = _index(ix)
+    ids, sc, st = g.search(Q[:8], 16, 6000, 100, stages=True)
+    _check(st, ids, sc, _oracle_stages(ix, Q[:8], 16, 6000, 100))
+    assert (st["cand_n"] > 4096).all()
+
+
 def test_config1_search_batch_matches_oracle_batch(c1):
     corpus, ix, Q = c1
     g = _index(ix)

@@ -178,6 +178,26 @@ def test_filter_pids_ties_and_short_input():
         assert nv == nv_ref
 
 
+def test_filter_pids_ndocs_above_4096():
+    """Stage-1 selections of 8,000 and 14,000 pids (the large select
+    variant), ties included, bit-exact against the oracle."""
+    from mvr import plaid
+    from oracle import plaid as P
+    rng = np.random.default_rng(12)
+    n_c, nq, n_doc = 64, 8, 40000
+    doclens = rng.integers(0, 12, n_doc).astype(np.int64)
+    offsets = np.concatenate([[0], np.cumsum(doclens)[:-1]]).astype(np.int64)
+    codes = rng.integers(0, n_c, int(doclens.sum())).astype(np.int32)
+    S = (rng.integers(0, 5, (n_c, nq)) * 0.125).astype(np.float32)
+    idx = (rng.random(n_c) < 0.7).astype(np.uint8)
+    for npids, ndocs in ((30000, 8000), (39000, 14000), (20000, 16384), (12000, 14000)):
+        pids = np.sort(rng.choice(n_doc, npids, replace=False)).astype(np.int32)
+        got, nv = plaid.filter_pids(pids, S, codes, doclens, offsets, idx.astype(bool), ndocs)
+        ref, _, nv_ref = P.filter_pids(pids, S, codes, doclens, offsets, idx, ndocs)
+        np.testing.assert_array_equal(got.cpu().numpy(), ref, err_msg=f"{npids} {ndocs}")
+        assert nv == nv_ref
+
+
 # -------------------------------------------------------- batched search
 @pytest.mark.parametrize("k", [10, 100, 1000])
 def test_search_bitexact_on_fixture(fx, opix, gpix, k):
@@ -223,6 +243,33 @@ def test_search_bitexact_on_synthetic_index():
         np.testing.assert_array_equal(ids, rid)
         np.testing.assert_array_equal(sc, rsc)
         np.testing.assert_array_equal(st, rst)
+    g.close()
+
+
+def test_search_ndocs_of_the_reference_drivers():
+    """ndocs 8,000 and 14,000 (eval_plaid_gpu.py:79,85; searcher.py:310 sets
+    ndocs = max(4k, 4096)) on a 20,000-doc synthetic index: the stage-1
+    selection keeps up to 16,384 pids (the large select variant), stage 2
+    ndocs/4; bit-exact against the oracle."""
+    from mvr import plaid
+    from oracle import plaid as P
+    d = _synthetic_plaid(7, 20000, 256, lo=4, hi=40)
+    X = d.pop("X")
+    o = P.PlaidIndex(**d)
+    g = plaid.PlaidIndex(**d)
+    rng = np.random.default_rng(17)
+    Q = X[rng.choice(X.shape[0], 12 * 32)].reshape(12, 32, 128)
+    Q = Q + 0.5 * rng.standard_normal(Q.shape).astype(np.float32)
+    Q = (Q / np.linalg.norm(Q, axis=2, keepdims=True)).astype(np.float32)
+    for k, ncells, thr, ndocs in ((100, 8, 0.2, 8000), (1000, 8, 0.1, 14000), (10, 4, 0.3, 16384)):
+        ids, sc, st = g.search(Q, k=k, ncells=ncells, centroid_score_threshold=thr, ndocs=ndocs, batch=6,
+                               return_stats=True)
+        rid, rsc, rst = o.search(Q, k, ncells, thr, ndocs)
+        np.testing.assert_array_equal(ids, rid, err_msg=f"ndocs {ndocs}")
+        np.testing.assert_array_equal(sc, rsc, err_msg=f"ndocs {ndocs}")
+        np.testing.assert_array_equal(st, rst, err_msg=f"ndocs {ndocs}")
+        # stage 1 keeps min(ndocs, candidates) pids: more than 4,096 here
+        assert (st[:, 0] > 4096).all(), "stage 1 should keep more than 4,096 pids"
     g.close()
 
 
