@@ -90,6 +90,18 @@ int mvr_plaid_decompress_pids(const int32_t *pids, uint32_t npids, const int64_t
 int mvr_plaid_segmented_maxsim(const float *scores, uint32_t nq, const int64_t *lengths,
                                uint32_t ndocs, float *out, void *stream);
 
+/* segmented_lookup_cpp (baseline/ColBERT/colbert/search/segmented_lookup.cpp:52-148,
+ * bound at strided_tensor.py:20-38 and called at :93 for the IVF lookups of
+ * candidate_generation.py:26,36): out = the rows input[offsets[i] ..
+ * offsets[i] + lengths[i]) for i = 0 .. n-1, concatenated.  A row is
+ * row_bytes bytes (dim x the element size; the reference's u8, i32, i64, f32
+ * and f16 inputs are all byte copies).  input holds n_rows rows; out must hold
+ * out_rows = sum(lengths) rows (checked).  Device pointers; returns after the
+ * copy on `stream`.  MVR_ERR_INVALID if a segment lies outside the input (the
+ * reference copies out of bounds). */
+int mvr_plaid_segmented_lookup(const void *input, uint64_t n_rows, uint32_t row_bytes, const int64_t *lengths,
+                               const int64_t *offsets, uint32_t n, void *out, uint64_t out_rows, void *stream);
+
 /* filter_pids_cpp (baseline/ColBERT/colbert/search/filter_pids.cpp:147-193;
  * bound at index_storage.py:46, called at :154-159): stage 1 scores every pid
  * over its codes c with idx[c] (per token the max of centroid_scores[c][t],

@@ -162,6 +162,30 @@ __global__ __launch_bounds__(1024) void k_pl_cumsum(const int32_t *__restrict__ 
   }
 }
 
+// segmented_lookup.cpp:25-49: segment i's rows, input[offsets[i] ..
+// offsets[i] + lengths[i]), copied to out[cum[i] ..] in words of W bytes
+// (row_words per row); one workgroup per segment, grid-stride over segments.
+// A segment outside the input (negative offset or length, or past n_rows)
+// is skipped and flagged in *bad.
+template <typename W>
+__global__ __launch_bounds__(256) void k_pl_seglookup(const W *__restrict__ in, uint64_t n_rows, uint32_t row_words,
+                                                      const int64_t *__restrict__ lengths,
+                                                      const int64_t *__restrict__ offsets,
+                                                      const int64_t *__restrict__ cum, uint32_t n,
+                                                      W *__restrict__ out, uint32_t *__restrict__ bad) {
+  for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
+    const int64_t len = lengths[i], off = offsets[i];
+    if (len < 0 || off < 0 || (uint64_t)off + (uint64_t)len > n_rows) {
+      if (threadIdx.x == 0) atomicOr(bad, 1u);
+      continue;
+    }
+    const uint64_t nw = (uint64_t)len * row_words;
+    const W *src = in + (uint64_t)off * row_words;
+    W *dst = out + (uint64_t)cum[i] * row_words;
+    for (uint64_t w = threadIdx.x; w < nw; w += blockDim.x) dst[w] = src[w];
+  }
+}
+
 // search/decompress_residuals.cpp:36-75, one workgroup per pid.
 __global__ __launch_bounds__(256) void k_pl_decompress_pids(
     const int32_t *__restrict__ pids, const int64_t *__restrict__ cum, const int64_t *__restrict__ lengths,
@@ -1044,14 +1068,14 @@ inline uint32_t approx_waves(uint32_t n_c) {
 // Scratch of the single ops (per host thread, reallocated on a device change).
 struct OpScratch {
   int device = -1;
-  PBuf cum, keys, seg, sel1_ids, sel1_F, sel1_n, keys2, sel2_ids, sel2_F, sel2_n, nvec;
+  PBuf cum, keys, seg, sel1_ids, sel1_F, sel1_n, keys2, sel2_ids, sel2_F, sel2_n, nvec, flag;
   hipError_t bind() {
     int dev = 0;
     hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     if (dev != device) {
       for (PBuf *b : {&cum, &keys, &seg, &sel1_ids, &sel1_F, &sel1_n, &keys2, &sel2_ids, &sel2_F, &sel2_n,
-                      &nvec})
+                      &nvec, &flag})
         b->release();
       device = dev;
     }
@@ -1228,6 +1252,52 @@ int mvr_plaid_decompress_pids(const int32_t *pids, uint32_t npids, const int64_t
       pids, cum, lengths, offsets, bucket_weights, reversed_bit_map, bucket_weight_combinations, vpb,
       binary_residuals, pk, codes, centroids, 0xFFFFFFFFu, dim, out);
   PL_TRY(hipGetLastError());
+  return MVR_OK;
+}
+
+int mvr_plaid_segmented_lookup(const void *input, uint64_t n_rows, uint32_t row_bytes, const int64_t *lengths,
+                               const int64_t *offsets, uint32_t n, void *out, uint64_t out_rows, void *stream) {
+  if (n == 0) {
+    if (out_rows != 0) return pl_fail(MVR_ERR_INVALID, "out_rows %llu but no segments", (unsigned long long)out_rows);
+    return MVR_OK;
+  }
+  if (row_bytes == 0) return pl_fail(MVR_ERR_INVALID, "row_bytes is 0");
+  if ((!input && n_rows) || !lengths || !offsets || (!out && out_rows)) return pl_fail(MVR_ERR_INVALID, "NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  OpScratch &w = g_ops;
+  PL_TRY(w.bind());
+  PL_TRY(w.cum.ensure(sizeof(int64_t) * ((size_t)n + 1)));
+  PL_TRY(w.flag.ensure(sizeof(uint32_t)));
+  PL_TRY(launch_fill_u32(w.flag.as<uint32_t>(), 0, 1, s));
+  k_pl_cumsum<<<1, 1024, 0, s>>>(nullptr, n, lengths, w.cum.as<int64_t>());
+  PL_TRY(hipGetLastError());
+  int64_t total = 0;
+  PL_TRY(hipMemcpyAsync(&total, w.cum.as<int64_t>() + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+  PL_TRY(hipStreamSynchronize(s));
+  if (total < 0 || (uint64_t)total != out_rows)
+    return pl_fail(MVR_ERR_INVALID, "sum(lengths) = %lld rows, out holds %llu", (long long)total,
+                   (unsigned long long)out_rows);
+  // widest word that divides the row and both base addresses
+  const uintptr_t al = reinterpret_cast<uintptr_t>(input) | reinterpret_cast<uintptr_t>(out);
+  uint32_t wb = 16;
+  while (wb > 1 && ((row_bytes % wb) != 0 || (al % wb) != 0)) wb >>= 1;
+  const uint32_t rw = row_bytes / wb, grid = std::min(n, 16384u);
+  uint32_t *bad = w.flag.as<uint32_t>();
+  const int64_t *cum = w.cum.as<int64_t>();
+#define MVR_SEGL(T)                                                                                              \
+  k_pl_seglookup<T><<<grid, 256, 0, s>>>(static_cast<const T *>(input), n_rows, rw, lengths, offsets, cum, n, \
+                                         static_cast<T *>(out), bad)
+  if (wb == 16) MVR_SEGL(uint4);
+  else if (wb == 8) MVR_SEGL(uint2);
+  else if (wb == 4) MVR_SEGL(uint32_t);
+  else if (wb == 2) MVR_SEGL(uint16_t);
+  else MVR_SEGL(uint8_t);
+#undef MVR_SEGL
+  PL_TRY(hipGetLastError());
+  uint32_t h_bad = 0;
+  PL_TRY(hipMemcpyAsync(&h_bad, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  PL_TRY(hipStreamSynchronize(s));
+  if (h_bad) return pl_fail(MVR_ERR_INVALID, "a segment lies outside the input's %llu rows", (unsigned long long)n_rows);
   return MVR_OK;
 }
 

@@ -129,6 +129,7 @@ SIGNATURES = {
     "mvr_plaid_decompress_pids": (C.c_int, [_vp, _u32, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _u32,
                                             _vp, _vp]),
     "mvr_plaid_segmented_maxsim": (C.c_int, [_vp, _u32, _vp, _u32, _vp, _vp]),
+    "mvr_plaid_segmented_lookup": (C.c_int, [_vp, C.c_uint64, _u32, _vp, _vp, _u32, _vp, C.c_uint64, _vp]),
     "mvr_plaid_filter_pids": (C.c_int, [_vp, _u32, _vp, _u32, _u32, _vp, _vp, _vp, _vp, _u32, _vp,
                                         C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), _vp]),
     "mvr_plaid_index_create": (C.c_int, [C.POINTER(PlaidIndexDesc), C.c_int, C.POINTER(C.c_void_p)]),

@@ -138,6 +138,31 @@ def segmented_maxsim(scores, lengths):
     return _back(out, scores if torch.is_tensor(scores) else s)
 
 
+def segmented_lookup(input, pids, lengths, offsets):
+    """segmented_lookup_cpp (strided_tensor.py:93, segmented_lookup.cpp:52-148):
+    the rows input[offsets[i] : offsets[i] + lengths[i]] for i < len(pids),
+    concatenated; input 1-D or 2-D, uint8 / int32 / int64 / float32 / float16.
+    Runs on the device; the result is on the device when input is."""
+    torch = _torch()
+    src = input if torch.is_tensor(input) else torch.as_tensor(np.asarray(input))
+    if src.dtype not in (torch.uint8, torch.int32, torch.int64, torch.float32, torch.float16):
+        raise _c.MvrError(1, f"segmented_lookup: dtype {src.dtype} unsupported")
+    if src.dim() not in (1, 2):
+        raise _c.MvrError(1, "segmented_lookup: input must be 1-D or 2-D")
+    x = src.to("cuda").contiguous()
+    n = int(pids.shape[0]) if hasattr(pids, "shape") else len(pids)
+    ln = _dev(lengths, torch.int64)[:n].contiguous()
+    of = _dev(offsets, torch.int64)[:n].contiguous()
+    if ln.numel() < n or of.numel() < n:
+        raise _c.MvrError(1, "segmented_lookup: fewer lengths/offsets than pids")
+    total = int(ln.sum().item()) if n else 0
+    dim = 1 if x.dim() == 1 else int(x.shape[1])
+    out = torch.empty((total,) if x.dim() == 1 else (total, dim), dtype=x.dtype, device="cuda")
+    _c.check_plaid(_lib().mvr_plaid_segmented_lookup(_ptr(x), x.shape[0], dim * x.element_size(), _ptr(ln),
+                                                     _ptr(of), n, _ptr(out), total, _stream(torch)))
+    return _back(out, src)
+
+
 def filter_pids(pids, centroid_scores, codes, doclens, offsets, idx, nfiltered_docs: int):
     """(int32 pids [<= nfiltered_docs // 4] by (score, pid) descending,
     n_vec_score_refine): the two-stage centroid filter."""

@@ -154,6 +154,21 @@ def segmented_maxsim(scores, lengths) -> np.ndarray:
     return out
 
 
+def segmented_lookup(input, pids, lengths, offsets) -> np.ndarray:
+    """segmented_lookup.cpp:52-108 (the lookup workers, :25-49): for
+    i < len(pids), the rows input[offsets[i] : offsets[i] + lengths[i]],
+    concatenated in order (the output starts zero-filled and every row is
+    written, so only the copies matter); 1-D input is rows of one element."""
+    x = np.asarray(input)
+    n = len(pids)
+    ln = np.asarray(lengths, np.int64)[:n]
+    of = np.asarray(offsets, np.int64)[:n]
+    parts = [x[int(o):int(o) + int(l)] for o, l in zip(of, ln)]
+    if not parts:
+        return np.zeros((0,) + x.shape[1:], x.dtype)
+    return np.concatenate(parts, axis=0)
+
+
 def filter_pids(pids, centroid_scores, codes, doclens, offsets, idx, ndocs: int):
     """filter_pids.cpp:147-193.  Returns (pids int32 [min(ndocs//4, ...)],
     stage-2 scores, n_vec_score_refine)."""

@@ -145,6 +145,55 @@ def test_segmented_maxsim_bitexact(nq):
     np.testing.assert_array_equal(got, P.segmented_maxsim(s, lengths))
 
 
+# ------------------------------------------------------- segmented_lookup
+@pytest.mark.parametrize("dtype,dim", [(np.uint8, 1), (np.uint8, 32), (np.int32, 1), (np.int64, 3),
+                                       (np.float32, 128), (np.float16, 5), (np.float16, 128)])
+def test_segmented_lookup_matches_oracle(dtype, dim):
+    """IVF-style lookups (segments of any length, zero-length ones,
+    overlapping and repeated segments, the last rows) for every dtype the
+    reference dispatches, 1-D and 2-D, host and device inputs."""
+    import torch
+    from mvr import plaid
+    from oracle import plaid as P
+    rng = np.random.default_rng(dim * 7 + np.dtype(dtype).itemsize)
+    n_rows = 5000
+    shape = (n_rows,) if dim == 1 else (n_rows, dim)
+    x = rng.integers(0, 250, shape).astype(dtype) if np.dtype(dtype).kind in "iu" else \
+        rng.standard_normal(shape).astype(dtype)
+    n = 700
+    lengths = rng.integers(0, 40, n).astype(np.int64)
+    lengths[:5] = [0, 0, 1, 39, 0]
+    offsets = (rng.random(n) * (n_rows - lengths)).astype(np.int64)
+    offsets[5], lengths[5] = n_rows - 7, 7            # the last rows
+    offsets[6:8], lengths[6:8] = offsets[3], lengths[3]  # a repeated segment
+    pids = np.arange(n, dtype=np.int64)
+    ref = P.segmented_lookup(x, pids, lengths, offsets)
+    got = plaid.segmented_lookup(x, pids, lengths, offsets)
+    assert not torch.is_tensor(got) or got.device.type == "cpu"
+    got = got.numpy() if torch.is_tensor(got) else got
+    assert got.dtype == ref.dtype and got.shape == ref.shape
+    np.testing.assert_array_equal(got.view(np.uint8), ref.view(np.uint8))
+    # device input gives a device result; pids shorter than lengths read only the first entries
+    xd = torch.from_numpy(x).cuda()
+    gd = plaid.segmented_lookup(xd, torch.from_numpy(pids[:300]), torch.from_numpy(lengths),
+                                torch.from_numpy(offsets))
+    assert gd.is_cuda
+    np.testing.assert_array_equal(gd.cpu().numpy().view(np.uint8),
+                                  P.segmented_lookup(x, pids[:300], lengths, offsets).view(np.uint8))
+
+
+def test_segmented_lookup_empty_and_out_of_range():
+    import torch
+    from mvr import plaid, _capi
+    x = np.arange(100, dtype=np.int32)
+    assert plaid.segmented_lookup(x, np.zeros(0), np.zeros(0, np.int64), np.zeros(0, np.int64)).shape == (0,)
+    assert plaid.segmented_lookup(x, [1, 2], [0, 0], [3, 99]).shape == (0,)
+    with pytest.raises(_capi.MvrError):
+        plaid.segmented_lookup(x, [1], [5], [98])       # past the end: the reference reads out of bounds
+    with pytest.raises(_capi.MvrError):
+        plaid.segmented_lookup(x, [1], [2], [-1])
+
+
 # ------------------------------------------------------------ filter_pids
 def test_filter_pids_bitexact_on_fixture_queries(fx, opix):
     from mvr import plaid

@@ -261,3 +261,17 @@ def test_search_recall_against_fixture_groundtruth(fx, pix):
     ids, _, _ = pix.search(fx["queries"], 10, *P.searcher_defaults(10))
     hits = np.mean([len(set(a.tolist()) & set(b.tolist())) / 10 for a, b in zip(ids, fx["gnd_ids"])])
     assert hits >= 0.6, hits
+
+
+def test_segmented_lookup_known_answers():
+    from oracle import plaid as P
+    x = np.arange(10, dtype=np.int32)
+    got = P.segmented_lookup(x, [7, 8, 9], [2, 0, 3], [1, 5, 7])
+    np.testing.assert_array_equal(got, [1, 2, 7, 8, 9])
+    assert got.dtype == np.int32
+    X = np.arange(24, dtype=np.float32).reshape(8, 3)
+    got = P.segmented_lookup(X, np.zeros(2), [1, 2], [6, 0])
+    np.testing.assert_array_equal(got, [[18, 19, 20], [0, 1, 2], [3, 4, 5]])
+    # only the first len(pids) lengths/offsets are read, like ndocs = pids.size(0)
+    np.testing.assert_array_equal(P.segmented_lookup(x, [0], [1, 5], [9, 0]), [9])
+    assert P.segmented_lookup(X, [], [], []).shape == (0, 3)
