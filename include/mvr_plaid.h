@@ -18,6 +18,14 @@
  * refine score is the left-to-right FP32 sum of max(0, max_i s) over query
  * tokens; the filter keeps pids by (score, pid) descending and the final
  * ranking orders by (score desc, pid asc).
+ *
+ * The GPU branch of the reference (index_storage.py:112-153,160-161,185-187,
+ * the path eval_plaid_gpu.py runs) computes in float16: halves for
+ * centroids, bucket weights and queries; every matmul output and every
+ * reduction result rounded to half once, with FP32 accumulation in the
+ * orders above; the padding value of colbert_score_reduce is half(-9999) =
+ * -10000, and MaxSim has no zero clip.  mvr_plaid_search with gpu_branch = 1
+ * and the *_f16 ops below follow it.
  */
 #ifndef MVR_PLAID_H
 #define MVR_PLAID_H
@@ -90,6 +98,27 @@ int mvr_plaid_decompress_pids(const int32_t *pids, uint32_t npids, const int64_t
 int mvr_plaid_segmented_maxsim(const float *scores, uint32_t nq, const int64_t *lengths,
                                uint32_t ndocs, float *out, void *stream);
 
+/* colbert_score_packed, GPU branch (baseline/ColBERT/colbert/modeling/colbert.py:181-205,
+ * called at index_storage.py:185-187 after lookup_pids): one query's half
+ * rows Q16 [nq][128] (nq <= 32) against the half rows D_packed
+ * [D_rows][128] (D_rows >= sum(D_lengths), lengths >= 0, else
+ * MVR_ERR_INVALID): s = half(FMA chain of D . Q_t), per doc M_t = max
+ * over its rows with the padding value -10000 as identity (exact whenever
+ * the scores are >= -10000, as dot products of normalised rows are), out[d] =
+ * half(left-to-right FP32 sum over t).  Device pointers, halves as uint16
+ * bits; synchronises `stream` (row offsets). */
+int mvr_plaid_colbert_score_packed_f16(const uint16_t *Q16, uint32_t nq, const uint16_t *D_packed, uint64_t D_rows,
+                                       const int64_t *D_lengths, uint32_t ndocs, uint16_t *out, void *stream);
+
+/* colbert_score_reduce, GPU branch, on its padded input
+ * (baseline/ColBERT/colbert/modeling/colbert.py:133-155; called at
+ * index_storage.py:138,149 with the centroid scores of each doc's codes):
+ * scores_padded f16 [n][L][nq] (nq <= 64), D_mask u8 [n][L]; the entries
+ * whose mask is 0 read as -10000 (half(-9999)), out[i] = half(left-to-right
+ * FP32 sum over t of the max over the L positions).  L >= 1. */
+int mvr_plaid_colbert_score_reduce_f16(const uint16_t *scores_padded, const uint8_t *D_mask, uint32_t n, uint32_t L,
+                                       uint32_t nq, uint16_t *out, void *stream);
+
 /* segmented_lookup_cpp (baseline/ColBERT/colbert/search/segmented_lookup.cpp:52-148,
  * bound at strided_tensor.py:20-38 and called at :93 for the IVF lookups of
  * candidate_generation.py:26,36): out = the rows input[offsets[i] ..
@@ -146,10 +175,16 @@ void mvr_plaid_index_free(mvr_plaid_index *ix);
 
 /* Searcher.dense_search settings (searcher.py:291-312): ncells centroids per
  * query token, centroid_score_threshold (idx = max_t S >= threshold), ndocs
- * (<= 16384; stage 2 keeps ndocs/4), k results, queries per device pass. */
+ * (<= 16384; stage 2 keeps ndocs/4), k results, queries per device pass.
+ * gpu_branch: 0 = the CPU branch's semantics (FP32, filter_pids,
+ * zero-initialised MaxSim), 1 = the GPU branch's (float16 centroid scores,
+ * idx against half(threshold), both filter stages through
+ * colbert_score_reduce with -10000 padding, half decompression and
+ * normalisation, colbert_score_packed without zero clip; scores are halves). */
 typedef struct {
   uint32_t k, ncells, ndocs, batch;
   float centroid_score_threshold;
+  uint32_t gpu_branch;
 } mvr_plaid_params;
 
 typedef struct {
@@ -159,7 +194,8 @@ typedef struct {
   uint32_t n_pass;   /* device passes */
 } mvr_plaid_timings;
 
-/* IndexScorer.rank (index_storage.py:82-198, CPU branch) for n_q queries of
+/* IndexScorer.rank (index_storage.py:82-198, CPU or GPU branch by
+ * params->gpu_branch) for n_q queries of
  * nq tokens (nq <= 32): Q f32 [n_q][nq][dim], host memory unless
  * q_on_device.  Writes out_ids (int32, -1 when fewer than k) and out_scores
  * [n_q][k]; stats (nullable, host) [n_q][3] = (pids after candidate
@@ -167,6 +203,17 @@ typedef struct {
 int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t nq,
                      const mvr_plaid_params *params, int32_t *out_ids, float *out_scores,
                      int q_on_device, uint64_t *stats, mvr_plaid_timings *timings);
+
+/* ResidualEmbeddingsStrided.lookup_pids + ResidualCodec.decompress, GPU
+ * branch (baseline/ColBERT/colbert/indexing/codecs/residual_embeddings_strided.py:30-37,
+ * residual.py:243-279; called at index_storage.py:160-161): the tokens of
+ * pids[0..npids) (device int32) in order as normalised half rows, out f16
+ * [out_rows][128]: x_d = half(w + c) with half bucket weights and centroids,
+ * y_d = half(x_d / half(sqrtf(FP32 sum of x_d^2))).  *n_rows (host) =
+ * sum of the pids' doclens; MVR_ERR_INVALID when it exceeds out_rows or a
+ * pid lies outside the index.  Synchronises `stream`. */
+int mvr_plaid_lookup_pids_f16(mvr_plaid_index *ix, const int32_t *pids, uint32_t npids, uint16_t *out,
+                              uint64_t out_rows, uint64_t *n_rows, void *stream);
 
 #ifdef __cplusplus
 }

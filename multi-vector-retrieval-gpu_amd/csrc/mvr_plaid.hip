@@ -54,6 +54,14 @@ __device__ __forceinline__ uint32_t pl_f2ord(float f) {
   return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
+// Selection key of (score, centroid): score order bits above ~centroid, so
+// that larger keys come first and equal scores put the lower centroid first;
+// -0 and +0 share a key (they compare equal).  Keys of distinct centroids
+// differ, so the top-NC lists below need no tie rule of their own.
+__device__ __forceinline__ unsigned long long pl_cell_key(float s, uint32_t c) {
+  return ((unsigned long long)pl_f2ord(s == 0.0f ? 0.0f : s) << 32) | (uint32_t)~c;
+}
+
 __device__ __forceinline__ float pl_readlane(float v, uint32_t l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
@@ -64,9 +72,24 @@ __device__ __forceinline__ float pl_max(float a, float b) { return (a < b) ? b :
 __device__ __forceinline__ float h2f(uint16_t h) {
   return (float)__builtin_bit_cast(_Float16, h);
 }
-__device__ __forceinline__ uint16_t f2h(float f) {  // round to nearest even
+// FP32 -> half, round to nearest even.  The empty asm makes the FP32 value
+// opaque: without it the compiler may fold the conversion into the FMA that
+// produced f (v_fma_mixlo_f16, one rounding of the exact result to half),
+// while the reference rounds the FP32 result of its matmul or sum to half
+// (two roundings); the two differ when the FP32 value is a half midpoint.
+__device__ __forceinline__ uint16_t f2h(float f) {
+  asm volatile("" : "+v"(f));
   return __builtin_bit_cast(uint16_t, (_Float16)f);
 }
+// f rounded to the nearest half (ties to even), as a float
+__device__ __forceinline__ float r16(float f) {
+  asm volatile("" : "+v"(f));
+  return (float)(_Float16)f;
+}
+
+// colbert_score_reduce writes -9999 into the padding of a half tensor
+// (colbert.py:135): half(-9999) = -10000
+constexpr float kPad16 = -10000.0f;
 
 // ---------------------------------------------------------------------------
 // Codec ops
@@ -255,7 +278,7 @@ constexpr uint32_t kGp = kGt + 4;   // padded LDS row
 __global__ __launch_bounds__(256) void k_pl_sgemm(const float *__restrict__ Q, uint32_t n_q, uint32_t nq,
                                                   const float *__restrict__ C, uint32_t n_c, float thr,
                                                   float *__restrict__ S, uint8_t *__restrict__ idx,
-                                                  unsigned long long *__restrict__ top1) {
+                                                  unsigned long long *__restrict__ top1, int f16) {
   __shared__ __align__(16) float As[kGk][kGp];
   __shared__ __align__(16) float Bs[kGk][kGp];
   const uint32_t tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
@@ -297,6 +320,12 @@ __global__ __launch_bounds__(256) void k_pl_sgemm(const float *__restrict__ Q, u
   }
   const uint32_t qq = ty >> 2, t0 = (ty & 3) * 8, q = q0 + qq;
   const bool qv = q < n_q;
+  if (f16) {  // GPU branch: the half matmul's output (FP32 chain rounded once)
+#pragma unroll
+    for (uint32_t i = 0; i < 8; ++i)
+#pragma unroll
+      for (uint32_t j = 0; j < 8; ++j) acc[i][j] = r16(acc[i][j]);
+  }
 #pragma unroll
   for (uint32_t j = 0; j < 8; ++j) {
     const uint32_t c = c0 + (j < 4 ? tx * 4 + j : 64 + tx * 4 + (j - 4));
@@ -327,7 +356,7 @@ __global__ __launch_bounds__(256) void k_pl_sgemm(const float *__restrict__ Q, u
 #pragma unroll
       for (uint32_t j = 0; j < 8; ++j) {
         const uint32_t c = c0 + (j < 4 ? tx * 4 + j : 64 + tx * 4 + (j - 4));
-        const unsigned long long key = ((unsigned long long)pl_f2ord(acc[i][j]) << 32) | (uint32_t)~c;
+        const unsigned long long key = pl_cell_key(acc[i][j], c);
         if (c < n_c && key > best) best = key;
       }
 #pragma unroll
@@ -355,103 +384,79 @@ __global__ __launch_bounds__(256) void k_pl_top1_merge(const unsigned long long 
   top_ids[(size_t)q * kMaxQTok + t] = best ? ~(uint32_t)(best & 0xFFFFFFFFull) : 0xFFFFFFFFu;
 }
 
-__device__ __forceinline__ bool pl_better(float s1, uint32_t i1, float s2, uint32_t i2) {
-  return (s1 > s2) || (s1 == s2 && i1 < i2);
+// Inserts key k into the descending list b[0..NC) (the smallest drops out).
+template <uint32_t NC>
+__device__ __forceinline__ void pl_insert(unsigned long long (&b)[NC], unsigned long long k) {
+#pragma unroll
+  for (uint32_t j = 0; j < NC; ++j) {
+    const unsigned long long hi = k > b[j] ? k : b[j];
+    k = k > b[j] ? b[j] : k;
+    b[j] = hi;
+  }
 }
 
 // Per query token the best NC centroids of one chunk of centroids by (score
 // desc, centroid asc): one wave per (query, chunk), lane (t, h) scanning the
 // chunk's centroids c = h (mod 2) for token t, then the two halves merge.
-// Lists go to part[q][chunk][t][NC].
+// Key lists go to part[q][chunk][t][NC] (0 = no entry).
 constexpr uint32_t kCellChunk = 1024;
 
 template <uint32_t NC>
 __global__ __launch_bounds__(256) void k_pl_cell_part(const float *__restrict__ S, uint32_t n_c, uint32_t nq,
-                                                      uint32_t n_chunk, uint32_t *__restrict__ part_ids,
-                                                      float *__restrict__ part_s) {
+                                                      uint32_t n_chunk, unsigned long long *__restrict__ part) {
   const uint32_t lane = threadIdx.x & 63, t = lane & 31, h = lane >> 5;
   const uint32_t ch = blockIdx.x * 4 + (threadIdx.x >> 6), q = blockIdx.y;
   if (ch >= n_chunk) return;
   const uint32_t c_begin = ch * kCellChunk, c_end = min(n_c, c_begin + kCellChunk);
-  float bs[NC];
-  uint32_t bi[NC];
+  unsigned long long b[NC];
 #pragma unroll
-  for (uint32_t j = 0; j < NC; ++j) { bs[j] = -INFINITY; bi[j] = 0xFFFFFFFFu; }
+  for (uint32_t j = 0; j < NC; ++j) b[j] = 0ull;
   if (t < nq) {
     const float *Sq = S + (size_t)q * n_c * nq + t;
 #pragma unroll 8
     for (uint32_t c = c_begin + h; c < c_end; c += 2) {
-      float cv = Sq[(size_t)c * nq];
-      uint32_t ci = c;
-      if (pl_better(cv, ci, bs[NC - 1], bi[NC - 1])) {
-#pragma unroll
-        for (uint32_t j = 0; j < NC; ++j) {
-          if (pl_better(cv, ci, bs[j], bi[j])) {
-            const float ts = bs[j]; const uint32_t ti = bi[j];
-            bs[j] = cv; bi[j] = ci; cv = ts; ci = ti;
-          }
-        }
-      }
+      const unsigned long long k = pl_cell_key(Sq[(size_t)c * nq], c);
+      if (k > b[NC - 1]) pl_insert<NC>(b, k);
     }
   }
-  // read the whole partner list before any insert changes the lists
-  float os[NC];
-  uint32_t oi[NC];
+  // the partner half's list, read whole before any insert
+  unsigned long long o[NC];
 #pragma unroll
-  for (uint32_t j = 0; j < NC; ++j) {
-    os[j] = __shfl(bs[j], (int)(t + 32));
-    oi[j] = (uint32_t)__shfl((int)bi[j], (int)(t + 32));
-  }
+  for (uint32_t j = 0; j < NC; ++j) o[j] = __shfl(b[j], (int)((t + 32) & 63));
+  if (h == 0) {
 #pragma unroll
-  for (uint32_t j = 0; j < NC; ++j) {
-    float cv = os[j];
-    uint32_t ci = oi[j];
+    for (uint32_t j = 0; j < NC; ++j) pl_insert<NC>(b, o[j]);
+    if (t < nq) {
+      const size_t off = (((size_t)q * n_chunk + ch) * kMaxQTok + t) * NC;
 #pragma unroll
-    for (uint32_t u = 0; u < NC; ++u) {
-      if (pl_better(cv, ci, bs[u], bi[u])) {
-        const float ts = bs[u]; const uint32_t ti = bi[u];
-        bs[u] = cv; bi[u] = ci; cv = ts; ci = ti;
-      }
+      for (uint32_t j = 0; j < NC; ++j) part[off + j] = b[j];
     }
-  }
-  if (h == 0 && t < nq) {
-    const size_t o = (((size_t)q * n_chunk + ch) * kMaxQTok + t) * NC;
-#pragma unroll
-    for (uint32_t j = 0; j < NC; ++j) { part_ids[o + j] = bi[j]; part_s[o + j] = bs[j]; }
   }
 }
 
 // Merge of the chunk lists: one thread per (query, token) writes the best NC
-// to top_ids[q*32 + t][NC].
+// centroids to top_ids[q*32 + t][NC] (0xFFFFFFFF for a missing entry).
 template <uint32_t NC>
-__global__ __launch_bounds__(256) void k_pl_cell_merge(const uint32_t *__restrict__ part_ids,
-                                                       const float *__restrict__ part_s, uint32_t n_q,
+__global__ __launch_bounds__(256) void k_pl_cell_merge(const unsigned long long *__restrict__ part, uint32_t n_q,
                                                        uint32_t nq, uint32_t n_chunk,
                                                        uint32_t *__restrict__ top_ids) {
   const uint32_t e = blockIdx.x * 256 + threadIdx.x;
   const uint32_t q = e / kMaxQTok, t = e % kMaxQTok;
   if (q >= n_q || t >= nq) return;
-  float bs[NC];
-  uint32_t bi[NC];
+  unsigned long long b[NC];
 #pragma unroll
-  for (uint32_t j = 0; j < NC; ++j) { bs[j] = -INFINITY; bi[j] = 0xFFFFFFFFu; }
+  for (uint32_t j = 0; j < NC; ++j) b[j] = 0ull;
   for (uint32_t ch = 0; ch < n_chunk; ++ch) {
-    const size_t o = (((size_t)q * n_chunk + ch) * kMaxQTok + t) * NC;
+    const size_t off = (((size_t)q * n_chunk + ch) * kMaxQTok + t) * NC;
     for (uint32_t k = 0; k < NC; ++k) {
-      float cv = part_s[o + k];
-      uint32_t ci = part_ids[o + k];
-      if (!pl_better(cv, ci, bs[NC - 1], bi[NC - 1])) break;  // the chunk list is sorted
-#pragma unroll
-      for (uint32_t u = 0; u < NC; ++u) {
-        if (pl_better(cv, ci, bs[u], bi[u])) {
-          const float ts = bs[u]; const uint32_t ti = bi[u];
-          bs[u] = cv; bi[u] = ci; cv = ts; ci = ti;
-        }
-      }
+      const unsigned long long v = part[off + k];
+      if (v <= b[NC - 1]) break;  // the chunk list is sorted
+      pl_insert<NC>(b, v);
     }
   }
 #pragma unroll
-  for (uint32_t j = 0; j < NC; ++j) top_ids[((size_t)q * kMaxQTok + t) * NC + j] = bi[j];
+  for (uint32_t j = 0; j < NC; ++j)
+    top_ids[((size_t)q * kMaxQTok + t) * NC + j] = b[j] ? ~(uint32_t)(b[j] & 0xFFFFFFFFull) : 0xFFFFFFFFu;
 }
 
 // The union of the query's per-token top-ncells lists as an ascending cell list; one wave per query, an LDS bitmap of the
@@ -684,7 +689,7 @@ __global__ __launch_bounds__(256) void k_pl_approx(
     uint64_t idx_stride, uint32_t n_c, uint32_t nq, uint32_t n_doc, const int32_t *__restrict__ codes,
     const int64_t *__restrict__ doclens, const int64_t *__restrict__ offsets, uint32_t *__restrict__ pid_bm,
     uint32_t bm_words, uint32_t waves_per_block, unsigned long long *__restrict__ keys, uint64_t key_stride,
-    unsigned long long *__restrict__ nvec) {
+    unsigned long long *__restrict__ nvec, int f16, float empty_score) {
   extern __shared__ uint32_t dyn_bm[];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (wave >= waves_per_block) return;
@@ -707,7 +712,7 @@ __global__ __launch_bounds__(256) void k_pl_approx(
     if (pid_bm && ok && lane == 0) pid_bm[(size_t)q * bm_words + (pid >> 5)] = 0;
     const uint32_t L = ok ? (uint32_t)doclens[pid] : 0u;
     const int64_t off = ok ? offsets[pid] : 0;
-    float m = -9999.0f;
+    float m = f16 ? kPad16 : -9999.0f;
     uint32_t nu = 0;  // distinct passing codes (wave-uniform)
     for (uint32_t base = 0; base < L; base += 64) {
       const uint32_t j = base + lane;
@@ -754,9 +759,11 @@ __global__ __launch_bounds__(256) void k_pl_approx(
         }
       }
     }
-    // a doc without passing codes keeps every maximum at -9999: its score is
-    // the exact sum -9999 * nq (integers below 2^24), and no bit was set
-    float score = -9999.0f * (float)nq;
+    // a doc without passing codes keeps every maximum at the identity: its
+    // score is empty_score (CPU branch: the exact sum -9999 * nq; GPU branch:
+    // half of the FP32 sum of nq x -10000, -inf from nq = 7 on), and no bit
+    // was set
+    float score = empty_score;
     if (nu) {
       pl_wave_sync();
       for (uint32_t base = 0; base < L; base += 64) {
@@ -770,6 +777,12 @@ __global__ __launch_bounds__(256) void k_pl_approx(
       m = pl_max(m, __shfl_xor(m, 32));
       score = 0.0f;
       for (uint32_t tt = 0; tt < nq; ++tt) score += pl_readlane(m, tt);
+      if (f16) {
+        // the half tensor's sum (colbert.py:155); a tiny negative sum rounds
+        // to -0, which torch.topk ranks equal to +0: one key for both
+        score = r16(score);
+        score = score == 0.0f ? 0.0f : score;
+      }
     }
     if (lane == 0) {
       keys[(size_t)q * key_stride + jj] = ((unsigned long long)pl_f2ord(score) << 32) | pid;
@@ -845,19 +858,21 @@ __global__ __launch_bounds__(256) void k_pl_passclear(const uint32_t *__restrict
 // Stage-1 split over the descending-pid candidate list, one workgroup of
 // 1024 threads per query.  Candidates holding a passing code (all of them
 // when pmode is 0) are listed in heavy[q] for the wave scan.  The others keep
-// every maximum at -9999, so their score is exactly -9999 * nq and they rank
-// by pid: only the first ndocs of them (the largest pids) can reach the
-// stage-1 top-ndocs, and their keys go to keys[q][light_base + r].
+// every maximum at the identity, so their score is empty_score (below every
+// other) and they rank by pid: only the first ndocs of them (the largest
+// pids) can reach the stage-1 top-ndocs, and their keys go to
+// keys[q][light_base + r].
 __global__ __launch_bounds__(1024) void k_pl_split(
     const uint32_t *__restrict__ cand, uint64_t cand_stride, const uint32_t *__restrict__ cand_n,
     const uint32_t *__restrict__ pmode, const uint32_t *__restrict__ pass_bm, uint32_t bm_words, uint32_t nq,
     uint32_t ndocs, unsigned long long *__restrict__ keys, uint64_t key_stride, uint64_t light_base,
-    uint32_t *__restrict__ heavy, uint32_t *__restrict__ heavy_n, uint32_t *__restrict__ light_n) {
+    uint32_t *__restrict__ heavy, uint32_t *__restrict__ heavy_n, uint32_t *__restrict__ light_n,
+    float empty_score) {
   __shared__ uint32_t warp_sums[16];
   const uint32_t tid = threadIdx.x, q = blockIdx.x;
   const uint32_t n = cand_n[q];
   const bool mark = pmode[q] != 0;
-  const unsigned long long ck = (unsigned long long)pl_f2ord(-9999.0f * (float)nq) << 32;
+  const unsigned long long ck = (unsigned long long)pl_f2ord(empty_score) << 32;
   const uint32_t *bq = pass_bm + (size_t)q * bm_words;
   uint32_t h_tot = 0, l_tot = 0;
   for (uint32_t base = 0; base < n; base += 1024) {
@@ -906,7 +921,11 @@ __global__ void k_pl_unkey(const uint32_t *__restrict__ sel, const uint32_t *__r
 constexpr uint32_t kPlTok = 16;
 
 // den[tok] = max(sqrtf(FMA chain over d of x_d^2), 1e-12), x the FP32
-// decompressed token (index_storage.py:182, residual.py:273-274).
+// decompressed token (index_storage.py:182, residual.py:273-274).  kF16 (the
+// GPU branch, residual.py:255-274 on half tensors): x_d = half(w + c) from
+// half weights and centroids, den = half(sqrtf(chain)) (eps 1e-12 rounds to
+// 0 in half).
+template <bool kF16>
 __global__ __launch_bounds__(256) void k_pl_token_den(const int32_t *__restrict__ codes, const uint8_t *__restrict__ res,
                                                       uint64_t n_tok, const float *__restrict__ C,
                                                       const float4 *__restrict__ tab4, float *__restrict__ den) {
@@ -919,7 +938,10 @@ __global__ __launch_bounds__(256) void k_pl_token_den(const int32_t *__restrict_
     for (uint32_t j = 0; j < kResBytes; ++j) {
       const float4 w = tab4[rb[j]];
       const float4 cv = c4[j];
-      const float x0 = w.x + cv.x, x1 = w.y + cv.y, x2 = w.z + cv.z, x3 = w.w + cv.w;
+      float x0 = w.x + cv.x, x1 = w.y + cv.y, x2 = w.z + cv.z, x3 = w.w + cv.w;
+      if (kF16) {
+        x0 = r16(x0); x1 = r16(x1); x2 = r16(x2); x3 = r16(x3);
+      }
       n2 = __builtin_fmaf(x0, x0, n2);
       n2 = __builtin_fmaf(x1, x1, n2);
       n2 = __builtin_fmaf(x2, x2, n2);
@@ -927,10 +949,16 @@ __global__ __launch_bounds__(256) void k_pl_token_den(const int32_t *__restrict_
     }
     // sqrtf and '/' compile to correctly rounded sequences (HIP's default
     // -fhip-fp32-correctly-rounded-divide-sqrt); __fsqrt_rn does not
-    den[g] = fmaxf(sqrtf(n2), 1e-12f);
+    den[g] = kF16 ? r16(sqrtf(n2)) : fmaxf(sqrtf(n2), 1e-12f);
   }
 }
 
+// kF16: the GPU branch (index_storage.py:160-161,185-187): half tokens
+// y_d = half(half(w + c) / den) (lookup_pids, residual.py:255-274), s_t =
+// half(FMA chain of y . Q_t) with Q rounded to half (colbert.py:198), M_t =
+// max_i s_t with the padding value -10000 as identity (no zero clip,
+// colbert_score_reduce), score = half(left-to-right FP32 sum over t).
+template <bool kF16>
 __global__ __launch_bounds__(256) void k_pl_refine(
     const float *__restrict__ Q, uint32_t nq, const uint32_t *__restrict__ sel_ids,
     const uint32_t *__restrict__ sel_n, uint32_t R, uint32_t n_slice, uint32_t n_doc,
@@ -960,7 +988,7 @@ __global__ __launch_bounds__(256) void k_pl_refine(
     const uint32_t pid = ~sel_ids[(size_t)q * R + r];
     const uint32_t L = pid < n_doc ? (uint32_t)doclens[pid] : 0u;
     const int64_t off = pid < n_doc ? offsets[pid] : 0;
-    float M = 0.0f;
+    float M = kF16 ? kPad16 : 0.0f;
     for (uint32_t i = 0; i < L; i += kPlTok) {
 #pragma unroll
       for (uint32_t c = 0; c < kPlTok / 2; ++c) {
@@ -971,7 +999,11 @@ __global__ __launch_bounds__(256) void k_pl_refine(
           const float4 w = tab[res[g * kResBytes + t]];
           const float4 cv = reinterpret_cast<const float4 *>(C + (size_t)code * kDim)[t];
           const float dn = den[g];
-          xs[wave][tk][t] = make_float4((w.x + cv.x) / dn, (w.y + cv.y) / dn, (w.z + cv.z) / dn, (w.w + cv.w) / dn);
+          if (kF16)
+            xs[wave][tk][t] = make_float4(r16(r16(w.x + cv.x) / dn), r16(r16(w.y + cv.y) / dn),
+                                          r16(r16(w.z + cv.z) / dn), r16(r16(w.w + cv.w) / dn));
+          else
+            xs[wave][tk][t] = make_float4((w.x + cv.x) / dn, (w.y + cv.y) / dn, (w.z + cv.z) / dn, (w.w + cv.w) / dn);
         }
       }
       pl_wave_sync();
@@ -991,7 +1023,7 @@ __global__ __launch_bounds__(256) void k_pl_refine(
       }
 #pragma unroll
       for (uint32_t c = 0; c < kPlTok / 2; ++c)
-        if (i + 2 * c + h < L) M = pl_max(M, a[c]);
+        if (i + 2 * c + h < L) M = pl_max(M, kF16 ? r16(a[c]) : a[c]);
       pl_wave_sync();
     }
     M = pl_max(M, __shfl_xor(M, 32));
@@ -1000,8 +1032,140 @@ __global__ __launch_bounds__(256) void k_pl_refine(
     if (lane == 0) {
       float sc = 0.0f;
       for (uint32_t tt = 0; tt < nq; ++tt) sc += Ms[wave][tt];
+      // GPU branch: the half sum, with -0 reported as +0 so that the two
+      // zeros rank as the tie they are
+      if (kF16) {
+        sc = r16(sc);
+        sc = sc == 0.0f ? 0.0f : sc;
+      }
       out_score[(size_t)q * R + r] = sc;
       out_ids[(size_t)q * R + r] = pid;
+    }
+    pl_wave_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// GPU-branch single ops (float16)
+// ---------------------------------------------------------------------------
+// out[i] = in[i] rounded to half (in place allowed)
+__global__ __launch_bounds__(256) void k_pl_round16(const float *in, uint64_t n, float *out) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = r16(in[i]);
+}
+
+// *bad = 1 when some length is negative
+__global__ __launch_bounds__(256) void k_pl_check_nonneg(const int64_t *__restrict__ v, uint32_t n,
+                                                         uint32_t *__restrict__ bad) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && v[i] < 0) *bad = 1u;
+}
+
+// *bad = 1 when some id lies outside [0, n)
+__global__ __launch_bounds__(256) void k_pl_check_ids(const int32_t *__restrict__ ids, uint32_t n_ids, uint32_t n,
+                                                      uint32_t *__restrict__ bad) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n_ids && (ids[i] < 0 || (uint32_t)ids[i] >= n)) *bad = 1u;
+}
+
+// lookup_pids of the GPU branch (residual_embeddings_strided.py:30-37 +
+// ResidualCodec.decompress, residual.py:243-279): the tokens of pids in
+// order, row cum[i] + j = token j of pids[i], each y_d = half(half(w + c) /
+// den16) with half weights (tab4h), half centroids (C16) and the token's half
+// norm.  Workgroup per pid, 32 lanes per token (four dims each), eight tokens
+// per step.
+__global__ __launch_bounds__(256) void k_pl_lookup16(const int32_t *__restrict__ pids, uint32_t npids,
+                                                     const int64_t *__restrict__ cum,
+                                                     const int64_t *__restrict__ doclens,
+                                                     const int64_t *__restrict__ offsets,
+                                                     const int32_t *__restrict__ codes, const uint8_t *__restrict__ res,
+                                                     const float *__restrict__ C16, const float4 *__restrict__ tab4h,
+                                                     const float *__restrict__ den16, uint16_t *__restrict__ out) {
+  const uint32_t t = threadIdx.x & 31, slot = threadIdx.x >> 5;
+  for (uint32_t i = blockIdx.x; i < npids; i += gridDim.x) {
+    const int32_t pid = pids[i];
+    const int64_t L = doclens[pid], off = offsets[pid], row0 = cum[i];
+    for (int64_t j = slot; j < L; j += 8) {
+      const uint64_t g = (uint64_t)(off + j);
+      const float4 w = tab4h[res[g * kResBytes + t]];
+      const float4 cv = reinterpret_cast<const float4 *>(C16 + (size_t)(uint32_t)codes[g] * kDim)[t];
+      const float dn = den16[g];
+      ushort4 o;
+      o.x = f2h(r16(w.x + cv.x) / dn);
+      o.y = f2h(r16(w.y + cv.y) / dn);
+      o.z = f2h(r16(w.z + cv.z) / dn);
+      o.w = f2h(r16(w.w + cv.w) / dn);
+      reinterpret_cast<ushort4 *>(out + (size_t)(row0 + j) * kDim)[t] = o;
+    }
+  }
+}
+
+// colbert_score_packed of the GPU branch (colbert.py:181-205): one wave per
+// doc; lane (t, h) holds half query row t in registers and scores the doc's
+// rows of parity h: s = half(FMA chain over d of D . Q_t), M_t = max with
+// the padding value -10000 as identity, score = half(left-to-right FP32 sum
+// over t).  D f16 [cum[ndocs]][128], Q16 f16 [nq][128] (nq <= 32).
+__global__ __launch_bounds__(256) void k_pl_score_packed16(const uint16_t *__restrict__ Q16, uint32_t nq,
+                                                           const uint16_t *__restrict__ D,
+                                                           const int64_t *__restrict__ cum, uint32_t ndocs,
+                                                           uint16_t *__restrict__ out) {
+  __shared__ float Ms[4][kMaxQTok];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63, t = lane & 31, h = lane >> 5;
+  float qv[kDim];
+#pragma unroll
+  for (uint32_t d = 0; d < kDim; ++d) qv[d] = t < nq ? h2f(Q16[(size_t)t * kDim + d]) : 0.0f;
+  for (uint32_t doc = blockIdx.x * 4 + wave; doc < ndocs; doc += gridDim.x * 4) {
+    const int64_t r0 = cum[doc], r1 = cum[doc + 1];
+    float M = kPad16;
+    for (int64_t r = r0 + h; r < r1; r += 2) {
+      const uint4 *row = reinterpret_cast<const uint4 *>(D + (size_t)r * kDim);
+      float a = 0.0f;
+#pragma unroll
+      for (uint32_t k = 0; k < kDim / 8; ++k) {
+        const uint4 v = row[k];
+        const uint32_t w4[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (uint32_t e = 0; e < 4; ++e) {
+          a = __builtin_fmaf(h2f((uint16_t)(w4[e] & 0xFFFFu)), qv[8 * k + 2 * e], a);
+          a = __builtin_fmaf(h2f((uint16_t)(w4[e] >> 16)), qv[8 * k + 2 * e + 1], a);
+        }
+      }
+      M = pl_max(M, r16(a));
+    }
+    M = pl_max(M, __shfl_xor(M, 32));
+    if (h == 0) Ms[wave][t] = M;
+    pl_wave_sync();
+    if (lane == 0) {
+      float sc = 0.0f;
+      for (uint32_t tt = 0; tt < nq; ++tt) sc += Ms[wave][tt];
+      out[doc] = f2h(sc);
+    }
+    pl_wave_sync();
+  }
+}
+
+// colbert_score_reduce of the GPU branch on the padded form (colbert.py:
+// 133-155): scores [n][L][nq] f16, mask [n][L]; the masked-out entries read
+// as -10000, M_t = max over the L positions, out = half(left-to-right FP32
+// sum over t).  One wave per doc, lane t < nq (nq <= 64).
+__global__ __launch_bounds__(256) void k_pl_score_reduce16(const uint16_t *__restrict__ sp, const uint8_t *__restrict__ mask,
+                                                           uint32_t n, uint32_t L, uint32_t nq,
+                                                           uint16_t *__restrict__ out) {
+  __shared__ float Ms[4][64];
+  const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (uint32_t doc = blockIdx.x * 4 + wave; doc < n; doc += gridDim.x * 4) {
+    float M = -INFINITY;
+    if (lane < nq)
+      for (uint32_t j = 0; j < L; ++j) {
+        const float v = mask[(size_t)doc * L + j] ? h2f(sp[((size_t)doc * L + j) * nq + lane]) : kPad16;
+        M = j ? pl_max(M, v) : v;
+      }
+    Ms[wave][lane] = M;
+    pl_wave_sync();
+    if (lane == 0) {
+      float sc = 0.0f;
+      for (uint32_t tt = 0; tt < nq; ++tt) sc += Ms[wave][tt];
+      out[doc] = f2h(sc);
     }
     pl_wave_sync();
   }
@@ -1098,7 +1262,7 @@ hipError_t launch_filter_stages(const uint32_t *in_ids, const uint32_t *in_n, ui
                                 uint32_t bm_words, uint32_t n_q, uint32_t ndocs, uint64_t *keys1,
                                 uint32_t *seg1, uint32_t *sel1_ids, float *sel1_F, uint32_t *sel1_n,
                                 uint64_t *keys2, uint32_t *sel2_ids, float *sel2_F, uint32_t *sel2_n,
-                                unsigned long long *nvec, hipStream_t s) {
+                                unsigned long long *nvec, int f16, float empty_score, hipStream_t s) {
   // with slots (split layout): heavy keys flat from 0, light keys in the
   // pl_light_segs segments after them
   const uint32_t n_heavy = std::max(1u, (in_cap + kSeg - 1) / kSeg);
@@ -1112,7 +1276,7 @@ hipError_t launch_filter_stages(const uint32_t *in_ids, const uint32_t *in_n, ui
                                                   (uint64_t)n_c * nq,
                                                   idx, n_c, n_c, nq, n_doc, codes, doclens, offsets, pid_bm,
                                                   bm_words, wpb, (unsigned long long *)keys1,
-                                                  (uint64_t)n_seg1 * kSeg, nvec);
+                                                  (uint64_t)n_seg1 * kSeg, nvec, f16, empty_score);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (slots)
@@ -1131,7 +1295,8 @@ hipError_t launch_filter_stages(const uint32_t *in_ids, const uint32_t *in_n, ui
   k_pl_approx<false, true><<<grid, 256, lds, s>>>(sel1_ids, sel1_n, 0, ndocs, nullptr, nullptr, S,
                                                   (uint64_t)n_c * nq, nullptr,
                                                   0, n_c, nq, n_doc, codes, doclens, offsets, nullptr, 0, wpb,
-                                                  (unsigned long long *)keys2, (uint64_t)n_seg2 * kSeg, nvec);
+                                                  (unsigned long long *)keys2, (uint64_t)n_seg2 * kSeg, nvec, f16,
+                                                  empty_score);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   k_pl_segcnt<<<grid_for((uint64_t)n_q * n_seg2, 256), 256, 0, s>>>(sel1_n, 0, n_seg2, n_q, seg1);
   if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -1161,23 +1326,143 @@ struct mvr_plaid_index {
   int64_t *ivf_off = nullptr, *ivf_len = nullptr;
   float4 *tab4 = nullptr;
   float *den = nullptr;  // [n_tok] per-token normalisation denominators
+  // GPU branch (half precision), built on first use: centroids and the byte
+  // table rounded to half, per-token half norms
+  float *C16 = nullptr;
+  float4 *tab4h = nullptr;
+  float *den16 = nullptr;
   hipStream_t stream = nullptr;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   bool bm_dirty = true;  // pid bitmap must be cleared before the next pass
   uint32_t q_nq = 0;     // query rows last copied into the padded Q buffer
   float avg_len = 0.0f;  // mean doc length (tokens)
-  PBuf part_ids, part_s, top_ids, top1, pass_list, pass_cnt, pmode, pass_bm, heavy, heavy_n, light_n;
+  PBuf part_keys, top_ids, top1, pass_list, pass_cnt, pmode, pass_bm, heavy, heavy_n, light_n;
   PBuf Q, S, idx, cells, cell_n, pid_bm, cand, cand_n, keys1, seg1, sel1_ids, sel1_F, sel1_n, keys2, sel2_ids,
       sel2_F, sel2_n, ids2, score2, out_ids, out_s, nvec;
   std::vector<PBuf *> bufs() {
-    return {&part_ids, &part_s, &top_ids, &top1, &pass_list, &pass_cnt, &pmode, &pass_bm, &heavy, &heavy_n, &light_n, &Q, &S, &idx, &cells, &cell_n, &pid_bm, &cand, &cand_n, &keys1, &seg1, &sel1_ids, &sel1_F, &sel1_n,
+    return {&part_keys, &top_ids, &top1, &pass_list, &pass_cnt, &pmode, &pass_bm, &heavy, &heavy_n, &light_n, &Q, &S, &idx, &cells, &cell_n, &pid_bm, &cand, &cand_n, &keys1, &seg1, &sel1_ids, &sel1_F, &sel1_n,
             &keys2, &sel2_ids, &sel2_F, &sel2_n, &ids2, &score2, &out_ids, &out_s, &nvec};
   }
 };
 
+namespace {
+// half rounding on the host (ties to even)
+float host_r16(float f) { return (float)(_Float16)f; }
+
+// The GPU branch's half-precision arrays (ResidualCodec holds centroids and
+// bucket weights as halves on a CUDA device, residual.py:30-45): built once
+// per index on first use.
+int pl_prepare_f16(mvr_plaid_index *ix) {
+  if (ix->den16) return MVR_OK;
+  hipStream_t s = ix->stream;
+  hipError_t e;
+  if ((e = hipMalloc((void **)&ix->C16, sizeof(float) * (size_t)ix->n_c * kDim)) != hipSuccess ||
+      (e = hipMalloc((void **)&ix->tab4h, sizeof(float4) * 256)) != hipSuccess) {
+    for (void **pp : {(void **)&ix->C16, (void **)&ix->tab4h})
+      if (*pp) { (void)hipFree(*pp); *pp = nullptr; }
+    return pl_fail(MVR_ERR_DEVICE, "half arrays: %s", hipGetErrorString(e));
+  }
+  k_pl_round16<<<grid_for((uint64_t)ix->n_c * kDim, 256), 256, 0, s>>>(ix->C, (uint64_t)ix->n_c * kDim, ix->C16);
+  k_pl_round16<<<4, 256, 0, s>>>(reinterpret_cast<const float *>(ix->tab4), 1024, reinterpret_cast<float *>(ix->tab4h));
+  PL_TRY(hipGetLastError());
+  float *den16 = nullptr;
+  PL_TRY(hipMalloc((void **)&den16, sizeof(float) * std::max<uint64_t>(ix->n_tok, 1)));
+  if (ix->n_tok)
+    k_pl_token_den<true><<<grid_for(ix->n_tok, 256), 256, 0, s>>>(ix->codes, ix->res, ix->n_tok, ix->C16, ix->tab4h,
+                                                                 den16);
+  if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(s)) != hipSuccess) {
+    (void)hipFree(den16);
+    return pl_fail(MVR_ERR_DEVICE, "half token norms: %s", hipGetErrorString(e));
+  }
+  ix->den16 = den16;
+  return MVR_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char *mvr_plaid_last_error(void) { return g_pl_err.c_str(); }
+
+int mvr_plaid_lookup_pids_f16(mvr_plaid_index *ix, const int32_t *pids, uint32_t npids, uint16_t *out,
+                              uint64_t out_rows, uint64_t *n_rows, void *stream) {
+  if (!ix || !n_rows) return pl_fail(MVR_ERR_INVALID, "NULL argument");
+  *n_rows = 0;
+  if (npids == 0) return MVR_OK;
+  if (!pids) return pl_fail(MVR_ERR_INVALID, "pids is NULL");
+  PL_TRY(hipSetDevice(ix->device));
+  int rc = pl_prepare_f16(ix);
+  if (rc != MVR_OK) return rc;
+  hipStream_t s = (hipStream_t)stream;
+  OpScratch &w = g_ops;
+  PL_TRY(w.bind());
+  PL_TRY(w.flag.ensure(sizeof(uint32_t)));
+  PL_TRY(w.cum.ensure(sizeof(int64_t) * ((size_t)npids + 1)));
+  PL_TRY(launch_fill_u32(w.flag.as<uint32_t>(), 0, 1, s));
+  k_pl_check_ids<<<(npids + 255) / 256, 256, 0, s>>>(pids, npids, ix->n_doc, w.flag.as<uint32_t>());
+  PL_TRY(hipGetLastError());
+  uint32_t bad = 0;
+  PL_TRY(hipMemcpyAsync(&bad, w.flag.p, sizeof(bad), hipMemcpyDeviceToHost, s));
+  PL_TRY(hipStreamSynchronize(s));
+  if (bad) return pl_fail(MVR_ERR_INVALID, "a pid lies outside [0, %u)", ix->n_doc);
+  k_pl_cumsum<<<1, 1024, 0, s>>>(pids, npids, ix->doclens, w.cum.as<int64_t>());
+  PL_TRY(hipGetLastError());
+  int64_t total = 0;
+  PL_TRY(hipMemcpyAsync(&total, w.cum.as<int64_t>() + npids, sizeof(total), hipMemcpyDeviceToHost, s));
+  PL_TRY(hipStreamSynchronize(s));
+  *n_rows = (uint64_t)total;
+  if ((uint64_t)total > out_rows)
+    return pl_fail(MVR_ERR_INVALID, "out holds %llu rows, the pids have %lld tokens", (unsigned long long)out_rows,
+                   (long long)total);
+  if (total == 0) return MVR_OK;
+  if (!out) return pl_fail(MVR_ERR_INVALID, "out is NULL");
+  k_pl_lookup16<<<std::min<uint32_t>(npids, 65536), 256, 0, s>>>(pids, npids, w.cum.as<int64_t>(), ix->doclens,
+                                                                  ix->offsets, ix->codes, ix->res, ix->C16, ix->tab4h,
+                                                                  ix->den16, out);
+  PL_TRY(hipGetLastError());
+  return MVR_OK;
+}
+
+int mvr_plaid_colbert_score_packed_f16(const uint16_t *Q16, uint32_t nq, const uint16_t *D_packed, uint64_t D_rows,
+                                       const int64_t *D_lengths, uint32_t ndocs, uint16_t *out, void *stream) {
+  if (ndocs == 0) return MVR_OK;
+  if (nq == 0 || nq > kMaxQTok) return pl_fail(MVR_ERR_UNSUPPORTED, "nq %u must be 1..32", nq);
+  if (!Q16 || !D_lengths || !out) return pl_fail(MVR_ERR_INVALID, "NULL argument");
+  hipStream_t s = (hipStream_t)stream;
+  OpScratch &w = g_ops;
+  PL_TRY(w.bind());
+  PL_TRY(w.cum.ensure(sizeof(int64_t) * ((size_t)ndocs + 1)));
+  PL_TRY(w.flag.ensure(sizeof(uint32_t)));
+  PL_TRY(launch_fill_u32(w.flag.as<uint32_t>(), 0, 1, s));
+  k_pl_check_nonneg<<<(ndocs + 255) / 256, 256, 0, s>>>(D_lengths, ndocs, w.flag.as<uint32_t>());
+  k_pl_cumsum<<<1, 1024, 0, s>>>(nullptr, ndocs, D_lengths, w.cum.as<int64_t>());
+  PL_TRY(hipGetLastError());
+  int64_t total = 0;
+  uint32_t bad = 0;
+  PL_TRY(hipMemcpyAsync(&total, w.cum.as<int64_t>() + ndocs, sizeof(total), hipMemcpyDeviceToHost, s));
+  PL_TRY(hipMemcpyAsync(&bad, w.flag.p, sizeof(bad), hipMemcpyDeviceToHost, s));
+  PL_TRY(hipStreamSynchronize(s));
+  if (bad) return pl_fail(MVR_ERR_INVALID, "negative doc length");
+  if ((uint64_t)total > D_rows)
+    return pl_fail(MVR_ERR_INVALID, "D_lengths sum to %lld rows, D_packed holds %llu", (long long)total,
+                   (unsigned long long)D_rows);
+  if (total > 0 && !D_packed) return pl_fail(MVR_ERR_INVALID, "D_packed is NULL");
+  k_pl_score_packed16<<<std::min<uint32_t>((ndocs + 3) / 4, 16384), 256, 0, s>>>(Q16, nq, D_packed,
+                                                                                  w.cum.as<int64_t>(), ndocs, out);
+  PL_TRY(hipGetLastError());
+  return MVR_OK;
+}
+
+int mvr_plaid_colbert_score_reduce_f16(const uint16_t *scores_padded, const uint8_t *D_mask, uint32_t n, uint32_t L,
+                                       uint32_t nq, uint16_t *out, void *stream) {
+  if (n == 0) return MVR_OK;
+  if (L == 0) return pl_fail(MVR_ERR_INVALID, "no positions to reduce (L = 0)");
+  if (nq == 0 || nq > 64) return pl_fail(MVR_ERR_UNSUPPORTED, "nq %u must be 1..64", nq);
+  if (!scores_padded || !D_mask || !out) return pl_fail(MVR_ERR_INVALID, "NULL argument");
+  k_pl_score_reduce16<<<std::min<uint32_t>((n + 3) / 4, 16384), 256, 0, (hipStream_t)stream>>>(scores_padded, D_mask,
+                                                                                              n, L, nq, out);
+  PL_TRY(hipGetLastError());
+  return MVR_OK;
+}
 
 int mvr_plaid_decompress_residuals(const uint8_t *binary_residuals, uint64_t n, const uint16_t *bucket_weights,
                                    const uint8_t *reversed_bit_map, const uint8_t *bucket_weight_combinations,
@@ -1352,7 +1637,8 @@ int mvr_plaid_filter_pids(const int32_t *pids, uint32_t npids, const float *cent
                               nullptr, 0, 1, ndocs, w.keys.as<uint64_t>(), w.seg.as<uint32_t>(),
                               w.sel1_ids.as<uint32_t>(), w.sel1_F.as<float>(), w.sel1_n.as<uint32_t>(),
                               w.keys2.as<uint64_t>(), w.sel2_ids.as<uint32_t>(), w.sel2_F.as<float>(),
-                              w.sel2_n.as<uint32_t>(), w.nvec.as<unsigned long long>(), s));
+                              w.sel2_n.as<uint32_t>(), w.nvec.as<unsigned long long>(), 0,
+                              -9999.0f * (float)nq, s));
   k_pl_unkey<<<(ndocs / 4 + 255) / 256, 256, 0, s>>>(w.sel2_ids.as<uint32_t>(), w.sel2_n.as<uint32_t>(),
                                                       out_pids);
   PL_TRY(hipGetLastError());
@@ -1464,8 +1750,8 @@ int mvr_plaid_index_create(const mvr_plaid_index_desc *d, int device, mvr_plaid_
       (e = hipStreamCreateWithFlags(&ix->stream, hipStreamNonBlocking)) != hipSuccess)
     return bail(pl_fail(MVR_ERR_DEVICE, "index upload failed: %s", hipGetErrorString(e)));
   if (d->n_tok) {
-    k_pl_token_den<<<grid_for(d->n_tok, 256), 256, 0, ix->stream>>>(ix->codes, ix->res, d->n_tok, ix->C, ix->tab4,
-                                                                   ix->den);
+    k_pl_token_den<false><<<grid_for(d->n_tok, 256), 256, 0, ix->stream>>>(ix->codes, ix->res, d->n_tok, ix->C,
+                                                                          ix->tab4, ix->den);
     if ((e = hipGetLastError()) != hipSuccess || (e = hipStreamSynchronize(ix->stream)) != hipSuccess)
       return bail(pl_fail(MVR_ERR_DEVICE, "token norms failed: %s", hipGetErrorString(e)));
   }
@@ -1503,7 +1789,8 @@ void mvr_plaid_index_free(mvr_plaid_index *ix) {
   (void)hipSetDevice(ix->device);
   if (ix->stream) (void)hipStreamSynchronize(ix->stream);
   for (void *p : {(void *)ix->C, (void *)ix->codes, (void *)ix->res, (void *)ix->doclens, (void *)ix->offsets,
-                  (void *)ix->ivf, (void *)ix->ivf_off, (void *)ix->ivf_len, (void *)ix->tab4, (void *)ix->den})
+                  (void *)ix->ivf, (void *)ix->ivf_off, (void *)ix->ivf_len, (void *)ix->tab4, (void *)ix->den,
+                  (void *)ix->C16, (void *)ix->tab4h, (void *)ix->den16})
     if (p) (void)hipFree(p);
   for (PBuf *b : ix->bufs()) b->release();
   for (auto &ev : ix->ev)
@@ -1527,6 +1814,18 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
     return pl_fail(MVR_ERR_UNSUPPORTED, "ndocs %u must be 4..%u", p->ndocs, kMaxSelect);
   if (p->k > kMaxSelect) return pl_fail(MVR_ERR_UNSUPPORTED, "k %u > %u", p->k, kMaxSelect);
   PL_TRY(hipSetDevice(ix->device));
+  const bool gb = p->gpu_branch != 0;
+  if (gb) {
+    const int rc = pl_prepare_f16(ix);
+    if (rc != MVR_OK) return rc;
+  }
+  // GPU branch: idx compares the half scores with the threshold rounded to
+  // half (a half tensor against a Python float); a doc whose maxima all keep
+  // the identity scores the half of their FP32 sum
+  const float thr = gb ? host_r16(p->centroid_score_threshold) : p->centroid_score_threshold;
+  float empty_score = 0.0f;
+  for (uint32_t t = 0; t < nq; ++t) empty_score += gb ? -10000.0f : -9999.0f;
+  if (gb) empty_score = host_r16(empty_score);
   hipStream_t s = ix->stream;
   const uint32_t n_c = ix->n_c, n_doc = ix->n_doc, ndocs = p->ndocs, R2 = ndocs / 4, k = p->k;
   const uint32_t cell_cap = nq * p->ncells;
@@ -1554,8 +1853,7 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
     ix->q_nq = nq;
   }
   const uint32_t n_chunk = (n_c + kCellChunk - 1) / kCellChunk;
-  PL_TRY(ix->part_ids.ensure(sizeof(uint32_t) * (size_t)n_chunk * B * kMaxQTok * p->ncells));
-  PL_TRY(ix->part_s.ensure(sizeof(float) * (size_t)n_chunk * B * kMaxQTok * p->ncells));
+  PL_TRY(ix->part_keys.ensure(sizeof(uint64_t) * (size_t)n_chunk * B * kMaxQTok * p->ncells));
   PL_TRY(ix->top_ids.ensure(sizeof(uint32_t) * (size_t)B * kMaxQTok * p->ncells));
   PL_TRY(ix->top1.ensure(sizeof(unsigned long long) * (size_t)B * ((n_c + kGt - 1) / kGt) * kMaxQTok));
   PL_TRY(ix->S.ensure(sizeof(float) * (size_t)B * n_c * nq));
@@ -1609,10 +1907,15 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
     // a failed pass leaves pid bits set; the next call clears the bitmap
     ix->bm_dirty = true;
     PL_TRY(hipEventRecord(ix->ev[0], s));
+    if (gb) {  // queries as halves (candidate_generation.py:56, colbert.py:198)
+      k_pl_round16<<<grid_for((uint64_t)b * kMaxQTok * kDim, 256), 256, 0, s>>>(
+          ix->Q.as<float>(), (uint64_t)b * kMaxQTok * kDim, ix->Q.as<float>());
+      PL_TRY(hipGetLastError());
+    }
     const uint32_t n_tiles = (n_c + kGt - 1) / kGt;
     k_pl_sgemm<<<dim3(n_tiles, (b + 3) / 4), 256, 0, s>>>(
-        ix->Q.as<float>(), b, nq, ix->C, n_c, p->centroid_score_threshold, ix->S.as<float>(), ix->idx.as<uint8_t>(),
-        p->ncells == 1 ? ix->top1.as<unsigned long long>() : nullptr);
+        ix->Q.as<float>(), b, nq, gb ? ix->C16 : ix->C, n_c, thr, ix->S.as<float>(), ix->idx.as<uint8_t>(),
+        p->ncells == 1 ? ix->top1.as<unsigned long long>() : nullptr, gb ? 1 : 0);
     PL_TRY(hipGetLastError());
     PL_TRY(hipEventRecord(ix->ev[4], s));
     if (p->ncells == 1) {
@@ -1626,9 +1929,9 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
       switch (p->ncells) {
 #define PL_CELLS(NC)                                                                                          \
   case NC:                                                                                                    \
-    k_pl_cell_part<NC><<<gp, 256, 0, s>>>(ix->S.as<float>(), n_c, nq, n_chunk, ix->part_ids.as<uint32_t>(),   \
-                                          ix->part_s.as<float>());                                            \
-    k_pl_cell_merge<NC><<<gm, 256, 0, s>>>(ix->part_ids.as<uint32_t>(), ix->part_s.as<float>(), b, nq,        \
+    k_pl_cell_part<NC><<<gp, 256, 0, s>>>(ix->S.as<float>(), n_c, nq, n_chunk,                                \
+                                          ix->part_keys.as<unsigned long long>());                            \
+    k_pl_cell_merge<NC><<<gm, 256, 0, s>>>(ix->part_keys.as<unsigned long long>(), b, nq,                     \
                                            n_chunk, ix->top_ids.as<uint32_t>());                              \
     break;
         PL_CELLS(1) PL_CELLS(2) PL_CELLS(3) PL_CELLS(4) PL_CELLS(5) PL_CELLS(6) PL_CELLS(7) PL_CELLS(8)
@@ -1668,7 +1971,7 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
                                     ix->pass_bm.as<uint32_t>(), ix->bm_words, nq, ndocs,
                                     ix->keys1.as<unsigned long long>(), (uint64_t)n_keyseg * kSeg,
                                     (uint64_t)n_heavy * kSeg, ix->heavy.as<uint32_t>(),
-                                    ix->heavy_n.as<uint32_t>(), ix->light_n.as<uint32_t>());
+                                    ix->heavy_n.as<uint32_t>(), ix->light_n.as<uint32_t>(), empty_score);
       PL_TRY(hipGetLastError());
     }
     PL_TRY(launch_filter_stages(ix->cand.as<uint32_t>(), ix->cand_n.as<uint32_t>(), 0, n_doc, n_doc,
@@ -1678,7 +1981,7 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
                                 ix->keys1.as<uint64_t>(), ix->seg1.as<uint32_t>(), ix->sel1_ids.as<uint32_t>(),
                                 ix->sel1_F.as<float>(), ix->sel1_n.as<uint32_t>(), ix->keys2.as<uint64_t>(),
                                 ix->sel2_ids.as<uint32_t>(), ix->sel2_F.as<float>(), ix->sel2_n.as<uint32_t>(),
-                                ix->nvec.as<unsigned long long>(), s));
+                                ix->nvec.as<unsigned long long>(), gb ? 1 : 0, empty_score, s));
     k_pl_passclear<<<b, 256, 0, s>>>(ix->pass_list.as<uint32_t>(), ix->pass_cnt.as<uint32_t>(), ix->ivf,
                                      ix->ivf_off, ix->ivf_len, ix->pass_bm.as<uint32_t>(), ix->bm_words);
     PL_TRY(hipGetLastError());
@@ -1686,10 +1989,16 @@ int mvr_plaid_search(mvr_plaid_index *ix, const float *Q, uint32_t n_q, uint32_t
     {
       // about 4 docs per wave: each wave holds its query's 32 rows in registers
       const uint32_t n_slice = std::max(1u, std::min((R2 + 15) / 16, std::max(4u, 16384u / b)));
-      k_pl_refine<<<dim3(n_slice, b), 256, 0, s>>>(ix->Q.as<float>(), nq, ix->sel2_ids.as<uint32_t>(),
-                                                   ix->sel2_n.as<uint32_t>(), R2, n_slice, n_doc, ix->doclens,
-                                                   ix->offsets, ix->codes, ix->res, ix->C, ix->tab4, ix->den,
-                                                   ix->ids2.as<uint32_t>(), ix->score2.as<float>());
+      if (gb)
+        k_pl_refine<true><<<dim3(n_slice, b), 256, 0, s>>>(ix->Q.as<float>(), nq, ix->sel2_ids.as<uint32_t>(),
+                                                         ix->sel2_n.as<uint32_t>(), R2, n_slice, n_doc, ix->doclens,
+                                                         ix->offsets, ix->codes, ix->res, ix->C16, ix->tab4h,
+                                                         ix->den16, ix->ids2.as<uint32_t>(), ix->score2.as<float>());
+      else
+        k_pl_refine<false><<<dim3(n_slice, b), 256, 0, s>>>(ix->Q.as<float>(), nq, ix->sel2_ids.as<uint32_t>(),
+                                                          ix->sel2_n.as<uint32_t>(), R2, n_slice, n_doc, ix->doclens,
+                                                          ix->offsets, ix->codes, ix->res, ix->C, ix->tab4, ix->den,
+                                                          ix->ids2.as<uint32_t>(), ix->score2.as<float>());
       PL_TRY(hipGetLastError());
     }
     PL_TRY(launch_topk(ix->ids2.as<uint32_t>(), ix->score2.as<float>(), ix->sel2_n.as<uint32_t>(), R2, b, k,

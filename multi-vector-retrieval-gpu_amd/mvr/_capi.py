@@ -59,7 +59,7 @@ class PlaidIndexDesc(C.Structure):
 
 class PlaidParams(C.Structure):
     _fields_ = [("k", C.c_uint32), ("ncells", C.c_uint32), ("ndocs", C.c_uint32), ("batch", C.c_uint32),
-                ("centroid_score_threshold", C.c_float)]
+                ("centroid_score_threshold", C.c_float), ("gpu_branch", C.c_uint32)]
 
 
 class PlaidTimings(C.Structure):
@@ -130,6 +130,9 @@ SIGNATURES = {
                                             _vp, _vp]),
     "mvr_plaid_segmented_maxsim": (C.c_int, [_vp, _u32, _vp, _u32, _vp, _vp]),
     "mvr_plaid_segmented_lookup": (C.c_int, [_vp, C.c_uint64, _u32, _vp, _vp, _u32, _vp, C.c_uint64, _vp]),
+    "mvr_plaid_colbert_score_packed_f16": (C.c_int, [_vp, _u32, _vp, C.c_uint64, _vp, _u32, _vp, _vp]),
+    "mvr_plaid_colbert_score_reduce_f16": (C.c_int, [_vp, _vp, _u32, _u32, _u32, _vp, _vp]),
+    "mvr_plaid_lookup_pids_f16": (C.c_int, [_vp, _vp, _u32, _vp, C.c_uint64, C.POINTER(C.c_uint64), _vp]),
     "mvr_plaid_filter_pids": (C.c_int, [_vp, _u32, _vp, _u32, _u32, _vp, _vp, _vp, _vp, _u32, _vp,
                                         C.POINTER(C.c_uint32), C.POINTER(C.c_uint64), _vp]),
     "mvr_plaid_index_create": (C.c_int, [C.POINTER(PlaidIndexDesc), C.c_int, C.POINTER(C.c_void_p)]),

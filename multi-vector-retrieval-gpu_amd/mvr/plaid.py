@@ -12,13 +12,24 @@ HIP kernels in libmvr (include/mvr_plaid.h):
                              signature (IndexScorer.decompress_residuals)
   filter_pids                search/filter_pids.cpp:147-193 (IndexScorer.filter_pids)
   segmented_maxsim           modeling/segmented_maxsim.cpp:47-93 (ColBERT.segmented_maxsim)
+  segmented_lookup           search/segmented_lookup.cpp:52-148 (StridedTensor.segmented_lookup)
+
+the float16 functions of the reference's GPU branch:
+
+  colbert_score_packed       modeling/colbert.py:181-205 (GPU branch: half scores, no zero clip)
+  colbert_score_reduce       modeling/colbert.py:133-155 (-9999 padding in a half tensor)
+  PlaidIndex.lookup_pids     residual_embeddings_strided.py:30-37 + ResidualCodec.decompress
+                             (residual.py:243-279): half decompression and normalisation
 
 and `PlaidIndex`, the device-resident IndexScorer: `rank` for one query
-(index_storage.py:82-98) and `search` for a batch, both running the CPU
-branch's semantics (two-stage filter, FP32 decompression, L2 normalisation,
-zero-initialised MaxSim).  Inputs on the CPU are moved to the current GPU and
-results come back on the input's device.  There is no CPU fallback: without
-the library every call raises.
+(index_storage.py:82-98) and `search` for a batch, running either branch of
+score_pids: branch="cpu" (two-stage filter_pids, FP32 decompression, L2
+normalisation, zero-initialised MaxSim; index_storage.py:154-190) or
+branch="gpu" (float16 centroid scores, both filter stages through
+colbert_score_reduce, half decompression and normalisation,
+colbert_score_packed; index_storage.py:112-153,160-161,185-187).  Inputs on
+the CPU are moved to the current GPU and results come back on the input's
+device.  There is no CPU fallback: without the library every call raises.
 """
 from __future__ import annotations
 
@@ -183,6 +194,40 @@ def filter_pids(pids, centroid_scores, codes, doclens, offsets, idx, nfiltered_d
     return _back(res, pids if torch.is_tensor(pids) else p), int(nv.value)
 
 
+def colbert_score_packed(Q, D_packed, D_lengths):
+    """float16 [ndocs]: colbert_score_packed of the GPU branch for one query
+    Q [1, nq, dim] or [nq, dim] (rounded to half) against the half rows
+    D_packed [sum(D_lengths), dim]: per doc the sum over query tokens of the
+    max over its rows of half(D . Q_t)."""
+    torch = _torch()
+    Qt = Q if torch.is_tensor(Q) else torch.as_tensor(np.asarray(Q))
+    Qt = Qt.reshape(-1, Qt.shape[-1])
+    q = Qt.to(device="cuda", dtype=torch.float16).contiguous()
+    D = _dev(D_packed, torch.float16)
+    ln = _dev(D_lengths, torch.int64)
+    if q.shape[1] != 128 or (D.numel() and D.shape[-1] != 128):
+        raise _c.MvrError(4, "colbert_score_packed: dim must be 128")
+    out = torch.empty(ln.numel(), dtype=torch.float16, device="cuda")
+    _c.check_plaid(_lib().mvr_plaid_colbert_score_packed_f16(_ptr(q), q.shape[0], _ptr(D),
+                                                             D.shape[0] if D.dim() == 2 else 0, _ptr(ln),
+                                                             ln.numel(), _ptr(out), _stream(torch)))
+    return _back(out, D_packed if torch.is_tensor(D_packed) else D)
+
+
+def colbert_score_reduce(scores_padded, D_mask):
+    """float16 [n]: colbert_score_reduce of the GPU branch on its padded
+    input: scores_padded [n, L, nq] (half), D_mask [n, L] (or any shape with
+    n * L entries); masked-out entries read as half(-9999) = -10000."""
+    torch = _torch()
+    sp = _dev(scores_padded, torch.float16)
+    n, L, nq = sp.shape
+    m = _dev(D_mask, torch.bool).reshape(n, L).contiguous().view(torch.uint8)
+    out = torch.empty(n, dtype=torch.float16, device="cuda")
+    _c.check_plaid(_lib().mvr_plaid_colbert_score_reduce_f16(_ptr(sp), _ptr(m), n, L, nq, _ptr(out),
+                                                             _stream(torch)))
+    return _back(out, scores_padded if torch.is_tensor(scores_padded) else sp)
+
+
 # ------------------------------------------------------- batched search
 def searcher_defaults(k: int):
     """(ncells, centroid_score_threshold, ndocs) by k (searcher.py:291-312)."""
@@ -217,6 +262,7 @@ class PlaidIndex:
         h = C.c_void_p()
         _c.check_plaid(_lib().mvr_plaid_index_create(C.byref(desc), device, C.byref(h)))
         self._h = h
+        self._doclens_host = dl
         self._keep = None
 
     def close(self):
@@ -230,16 +276,37 @@ class PlaidIndex:
         except Exception:
             pass
 
+    def lookup_pids(self, passage_ids):
+        """(D_packed float16 [sum of doclens, dim], D_lengths int64): the GPU
+        branch's lookup_pids, normalised half rows of the pids' tokens in
+        order, on the current GPU."""
+        torch = _torch()
+        p = _dev(passage_ids, torch.int32).reshape(-1)
+        ph = p.cpu().numpy().astype(np.int64)
+        if ph.size and (ph.min() < 0 or ph.max() >= self.n_doc):
+            raise _c.MvrError(1, f"lookup_pids: a pid lies outside [0, {self.n_doc})")
+        ln = torch.as_tensor(self._doclens_host[ph] if ph.size else np.zeros(0, np.int64), device="cuda")
+        rows = int(ln.sum().item()) if p.numel() else 0
+        out = torch.empty((rows, self.dim), dtype=torch.float16, device="cuda")
+        got = C.c_uint64(0)
+        _c.check_plaid(_lib().mvr_plaid_lookup_pids_f16(self._h, _ptr(p), p.numel(), _ptr(out), rows,
+                                                        C.byref(got), _stream(torch)))
+        return out, ln
+
     def search(self, Q, k: int = 10, ncells=None, centroid_score_threshold=None, ndocs=None, batch: int = 1024,
-               return_stats: bool = False, timings: bool = False):
+               return_stats: bool = False, timings: bool = False, branch: str = "cpu"):
         """Q [n_q, nq, dim] (numpy or torch, host or device).  Returns
         (pids int32 [n_q, k], scores float32 [n_q, k]) as numpy, -1 / -inf
         where fewer than k docs survive; plus stats [n_q, 3] (candidate pids,
-        filtered pids, n_vec_score_refine) and/or a timings dict."""
+        filtered pids, n_vec_score_refine) and/or a timings dict.
+        branch: "cpu" (the reference's CPU branch, FP32) or "gpu" (its GPU
+        branch, float16; scores are half values)."""
+        if branch not in ("cpu", "gpu"):
+            raise ValueError(f"branch must be 'cpu' or 'gpu', not {branch!r}")
         d_nc, d_thr, d_nd = searcher_defaults(k)
         p = _c.PlaidParams(k=k, ncells=ncells or d_nc, ndocs=ndocs or d_nd, batch=batch,
                            centroid_score_threshold=d_thr if centroid_score_threshold is None
-                           else centroid_score_threshold)
+                           else centroid_score_threshold, gpu_branch=1 if branch == "gpu" else 0)
         on_dev = hasattr(Q, "is_cuda") and Q.is_cuda
         if on_dev:
             Qc = Q.float().contiguous()

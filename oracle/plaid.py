@@ -49,6 +49,13 @@ def lib():
         "orc_plaid_search_batch": (None, [u32, u32, u32, u32, _i64p, _i64p, _i32p, _u8p, _f32p, _f32p,
                                           _u8p, _u8p, _i32p, _i64p, _i64p, _f32p, u32, u32, u32,
                                           C.c_float, u32, u32, C.c_int, _i32p, _f32p, C.c_void_p]),
+        "orc_plaid_search_batch_gpu": (None, [u32, u32, u32, u32, _i64p, _i64p, _i32p, _u8p, _f32p, _f32p,
+                                              _u8p, _u8p, _i32p, _i64p, _i64p, _f32p, u32, u32, u32,
+                                              C.c_float, u32, u32, C.c_int, _i32p, _f32p, C.c_void_p]),
+        "orc_plaid_lookup_pids_f16": (u64, [_i32p, u32, _i64p, _i64p, _i32p, _u8p, _f32p, _f32p, _u8p, _u8p,
+                                            u32, u32, _f32p]),
+        "orc_plaid_score_reduce_f16": (None, [_f32p, _i64p, u32, u32, _f32p]),
+        "orc_plaid_score_packed_f16": (None, [_f32p, u32, _f32p, _i64p, u32, u32, _f32p]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -239,21 +246,63 @@ class PlaidIndex:
                                    self.lut.reshape(-1), self.dim, self.nbits, out)
         return out[:pids.size]
 
-    def search(self, Q, k: int, ncells: int, threshold: float, ndocs: int, n_threads: int = 0):
-        """IndexScorer.rank (CPU branch) for a batch Q [n_q][nq][dim].
-        Returns (pids int32 [n_q][k], scores [n_q][k], stats uint64 [n_q][3])."""
+    def lookup_pids_f16(self, pids):
+        """lookup_pids of the GPU branch (residual_embeddings_strided.py:30-37,
+        residual.py:243-279): (normalised float16 rows of the pids' tokens,
+        their doclens)."""
+        pids = _c(pids, np.int32)
+        ln = self.doclens[pids.astype(np.int64)] if pids.size else np.zeros(0, np.int64)
+        rows = int(ln.sum())
+        out = np.empty((max(rows, 1), self.dim), np.float32)
+        got = lib().orc_plaid_lookup_pids_f16(pids if pids.size else np.empty(1, np.int32), pids.size,
+                                              self.doclens, self.offsets, self.codes,
+                                              self.residuals.reshape(-1), self.centroids.reshape(-1),
+                                              self.bucket_weights, self.rbm, self.lut.reshape(-1), self.dim,
+                                              self.nbits, out.reshape(-1))
+        assert got == rows
+        return out[:rows].astype(np.float16), ln.astype(np.int64)
+
+    def search(self, Q, k: int, ncells: int, threshold: float, ndocs: int, n_threads: int = 0,
+               branch: str = "cpu"):
+        """IndexScorer.rank for a batch Q [n_q][nq][dim], CPU branch (FP32)
+        or GPU branch (float16, branch="gpu").  Returns (pids int32 [n_q][k],
+        scores [n_q][k], stats uint64 [n_q][3])."""
         Q = _c(Q, np.float32)
         n_q, nq, dim = Q.shape
         ids = np.empty((n_q, k), np.int32)
         sc = np.empty((n_q, k), np.float32)
         stats = np.zeros((n_q, 3), np.uint64)
-        lib().orc_plaid_search_batch(self.n_doc, self.n_c, dim, self.nbits, self.doclens, self.offsets,
+        fn = {"cpu": lib().orc_plaid_search_batch, "gpu": lib().orc_plaid_search_batch_gpu}[branch]
+        fn(self.n_doc, self.n_c, dim, self.nbits, self.doclens, self.offsets,
                                      self.codes, self.residuals.reshape(-1), self.centroids.reshape(-1),
                                      self.bucket_weights, self.rbm, self.lut.reshape(-1), self.ivf,
                                      self.ivf_off, self.ivf_lengths, Q.reshape(-1), n_q, nq, ncells,
                                      float(threshold), ndocs, k, n_threads, ids, sc,
                                      stats.ctypes.data_as(C.c_void_p))
         return ids, sc, stats
+
+
+def score_reduce_f16(scores, lengths) -> np.ndarray:
+    """colbert_score_reduce of the GPU branch (colbert.py:133-155) over a
+    packed half score matrix [sum(lengths)][nq]: float16 [ndocs]."""
+    s = _c(np.asarray(scores, np.float16).astype(np.float32), np.float32)
+    ln = _c(lengths, np.int64)
+    nq = s.shape[1] if s.ndim == 2 else 0
+    out = np.empty(max(ln.size, 1), np.float32)
+    lib().orc_plaid_score_reduce_f16(s.reshape(-1) if s.size else np.empty(1, np.float32), ln, ln.size, nq, out)
+    return out[:ln.size].astype(np.float16)
+
+
+def score_packed_f16(Q, D_packed, lengths) -> np.ndarray:
+    """colbert_score_packed of the GPU branch (colbert.py:181-205), one query
+    Q [nq][dim] against half rows D_packed [sum(lengths)][dim]: float16 [ndocs]."""
+    Qf = _c(Q, np.float32)
+    D = _c(np.asarray(D_packed, np.float16).astype(np.float32), np.float32)
+    ln = _c(lengths, np.int64)
+    out = np.empty(max(ln.size, 1), np.float32)
+    lib().orc_plaid_score_packed_f16(Qf.reshape(-1), Qf.shape[0], D.reshape(-1) if D.size else np.empty(1, np.float32),
+                                     ln, ln.size, Qf.shape[1], out)
+    return out[:ln.size].astype(np.float16)
 
 
 def searcher_defaults(k: int):

@@ -17,6 +17,9 @@
  *   candidate generation    baseline/ColBERT/colbert/search/candidate_generation.py:13-66
  *   score_pids, CPU branch  baseline/ColBERT/colbert/search/index_storage.py:101-198
  *   normalisation           residual.py:273-274 (torch.nn.functional.normalize, p=2, eps=1e-12)
+ *   score_pids, GPU branch  index_storage.py:112-153,160-161,185-187 with colbert_score_reduce /
+ *                           colbert_score_packed (modeling/colbert.py:133-155,181-205) and the
+ *                           half-precision decompress + normalise of residual.py:255-274
  *
  * Orders the reference leaves to MKL / torch reductions are fixed here, and the
  * GPU kernels follow the same definition:
@@ -31,6 +34,9 @@
  *   - the final ranking orders by (score descending, pid ascending) and
  *     top-ncells per token by (score descending, centroid ascending), which is
  *     argmax's first-maximum rule for ncells = 1.
+ * The GPU branch computes in float16: every matmul output and every reduction
+ * result is rounded to half once (FP32 accumulation inside, in the orders
+ * above), and torch.topk's open tie order is fixed as in the CPU branch.
  */
 #include <math.h>
 #include <stdint.h>
@@ -410,25 +416,205 @@ static uint32_t orc_plaid_candidates(const float *S, uint32_t n_c, uint32_t nq, 
   return n;
 }
 
-/* IndexScorer.rank, CPU branch (index_storage.py:82-198), for n_q queries of
- * nq tokens: candidates, idx = max_t S >= threshold, the two-stage filter,
- * the FP32 refine, then the best k by (score desc, pid asc).  Missing results
- * are (-1, -inf).  stats [n_q][3] (nullable): pids after candidate
- * generation, pids after the filter, n_vec_score_refine. */
-void orc_plaid_search_batch(uint32_t n_doc, uint32_t n_c, uint32_t dim, uint32_t nbits,
-                            const int64_t *doclens, const int64_t *offsets, const int32_t *codes,
-                            const uint8_t *res, const float *C, const float *bw,
-                            const uint8_t *rbm, const uint8_t *lut, const int32_t *ivf,
-                            const int64_t *ivf_off, const int64_t *ivf_len, const float *Qall,
-                            uint32_t n_q, uint32_t nq, uint32_t ncells, float threshold,
-                            uint32_t ndocs, uint32_t k, int n_threads, int32_t *out_ids,
-                            float *out_s, uint64_t *stats) {
+/* ------------------------------------------------------------------------ */
+/* GPU branch of IndexScorer.score_pids (index_storage.py:112-153, 160-161,  */
+/* 185-187) in float16, as the reference computes it on a CUDA device:       */
+/* centroids and bucket weights are halves (residual.py:30-45), so are the   */
+/* queries (candidate_generation.py:56, colbert.py:198), and so is every     */
+/* matmul output and reduction result.                                       */
+/* ------------------------------------------------------------------------ */
+
+static inline float orc_r16(float x) { return orc_f16_to_f32(orc_f32_to_f16(x)); }
+
+/* colbert_score_reduce writes -9999 into the half tensor's padding
+ * (colbert.py:135): half(-9999) = -10000 */
+#define ORC_PAD16 (-10000.0f)
+
+/* One token of ResidualCodec.decompress, GPU branch (residual.py:255-274):
+ * x_d = half(w16[bucket] + c16_d) (decompress_residuals.cu, at::Half add),
+ * then torch.nn.functional.normalize on the half row: n = half(sqrtf(sum of
+ * x_d^2 in FP32)) (the squares of halves are exact; the sum is a sequential
+ * FMA chain), eps = 1e-12 rounds to 0 in half so the denominator is n, and
+ * y_d = half(x_d / n) with the FP32 quotient.  c16, w16: half values. */
+static void orc_p_token16(const float *c16, const uint8_t *rb, const float *w16,
+                          const uint8_t *rbm, const uint8_t *lut, uint32_t dim, uint32_t nbits,
+                          float *y) {
+  const uint32_t vpb = 8 / nbits, pk = dim / vpb;
+  for (uint32_t k = 0; k < pk; k++) {
+    const uint8_t b = rbm[rb[k]];
+    for (uint32_t l = 0; l < vpb; l++)
+      y[k * vpb + l] = orc_r16(w16[lut[(uint32_t)b * vpb + l]] + c16[k * vpb + l]);
+  }
+  float n2 = 0.0f;
+  for (uint32_t d = 0; d < dim; d++) n2 = fmaf(y[d], y[d], n2);
+  const float n = orc_r16(sqrtf(n2));
+  for (uint32_t d = 0; d < dim; d++) y[d] = orc_r16(y[d] / n);
+}
+
+/* ResidualEmbeddingsStrided.lookup_pids + ResidualCodec.decompress, GPU
+ * branch (residual_embeddings_strided.py:30-37, residual.py:243-279): the
+ * normalised half rows of the tokens of pids[0..npids) in order, returned as
+ * float (half values).  C and bw are rounded to half first.  Returns the
+ * number of rows. */
+uint64_t orc_plaid_lookup_pids_f16(const int32_t *pids, uint32_t npids, const int64_t *doclens,
+                                   const int64_t *offsets, const int32_t *codes, const uint8_t *res,
+                                   const float *C, const float *bw, const uint8_t *rbm,
+                                   const uint8_t *lut, uint32_t dim, uint32_t nbits, float *out) {
+  const uint32_t pk = dim * nbits / 8;
+  float w16[256], *c16 = (float *)malloc(sizeof(float) * dim);
+  for (uint32_t b = 0; b < (1u << nbits); b++) w16[b] = orc_r16(bw[b]);
+  uint64_t row = 0;
+  for (uint32_t i = 0; i < npids; i++) {
+    const int64_t off = offsets[pids[i]];
+    for (int64_t j = 0; j < doclens[pids[i]]; j++, row++) {
+      const uint64_t tok = (uint64_t)(off + j);
+      for (uint32_t d = 0; d < dim; d++) c16[d] = orc_r16(C[(size_t)codes[tok] * dim + d]);
+      orc_p_token16(c16, res + tok * pk, w16, rbm, lut, dim, nbits, out + row * dim);
+    }
+  }
+  free(c16);
+  return row;
+}
+
+/* colbert_score_reduce, GPU branch (colbert.py:133-155) over a packed half
+ * score matrix: per doc, M_t = max over its rows of s[row][t] with the
+ * padding value -10000 as identity (a doc without rows keeps it), and
+ * score = half(left-to-right FP32 sum over t of M_t). */
+void orc_plaid_score_reduce_f16(const float *s, const int64_t *lengths, uint32_t ndocs,
+                                uint32_t nq, float *out) {
+  const float *p = s;
+  float *m = (float *)malloc(sizeof(float) * (nq ? nq : 1));
+  for (uint32_t d = 0; d < ndocs; d++) {
+    for (uint32_t t = 0; t < nq; t++) m[t] = ORC_PAD16;
+    for (int64_t j = 0; j < lengths[d]; j++, p += nq)
+      for (uint32_t t = 0; t < nq; t++) m[t] = (m[t] < p[t]) ? p[t] : m[t];
+    float sum = 0.0f;
+    for (uint32_t t = 0; t < nq; t++) sum += m[t];
+    out[d] = orc_r16(sum);
+  }
+  free(m);
+}
+
+/* colbert_score_packed, GPU branch (colbert.py:181-205), one query: D (half
+ * values) [sum(lengths)][dim] times Q^T rounded to half (Q is rounded to half
+ * first, colbert.py:198), each entry a sequential FMA chain over d rounded to
+ * half, then colbert_score_reduce. */
+void orc_plaid_score_packed_f16(const float *Q, uint32_t nq, const float *D, const int64_t *lengths,
+                                uint32_t ndocs, uint32_t dim, float *out) {
+  uint64_t rows = 0;
+  for (uint32_t d = 0; d < ndocs; d++) rows += (uint64_t)lengths[d];
+  const uint32_t tp = (nq + 7) & ~7u;
+  float *QT = (float *)calloc((size_t)dim * tp, sizeof(float));
+  float *s = (float *)malloc(sizeof(float) * (rows ? rows : 1) * (nq ? nq : 1));
+  float *sv = (float *)malloc(sizeof(float) * tp);
+  for (uint32_t t = 0; t < nq; t++)
+    for (uint32_t d = 0; d < dim; d++) QT[(size_t)d * tp + t] = orc_r16(Q[(size_t)t * dim + d]);
+  for (uint64_t r = 0; r < rows; r++) {
+    for (uint32_t t0 = 0; t0 < tp; t0 += 8) _mm256_storeu_ps(sv + t0, orc_p_dot8(D + r * dim, QT + t0, tp, dim));
+    for (uint32_t t = 0; t < nq; t++) s[r * nq + t] = orc_r16(sv[t]);
+  }
+  orc_plaid_score_reduce_f16(s, lengths, ndocs, nq, out);
+  free(QT); free(s); free(sv);
+}
+
+/* index_storage.py:120-138 (stage 1) or :143-149 (stage 2, idx NULL) for one
+ * pid: colbert_score_reduce of the S16 rows of the doc's codes that pass idx
+ * (repeated codes change no maximum, so each distinct code is visited once).
+ * *n_unique counts the distinct passing codes; *any is set when one passed. */
+static float orc_plaid_approx16(int32_t pid, const float *S, uint32_t nq, const int32_t *codes,
+                                const int64_t *doclens, const int64_t *offsets, const uint8_t *idx,
+                                uint8_t *seen, float *m, uint64_t *n_unique, int *any) {
+  for (uint32_t k = 0; k < nq; k++) m[k] = ORC_PAD16;
+  uint64_t nu = 0;
+  const int64_t off = offsets[pid];
+  for (int64_t j = 0; j < doclens[pid]; j++) {
+    const int32_t code = codes[off + j];
+    if ((!idx || idx[code]) && !seen[code]) {
+      for (uint32_t k = 0; k < nq; k++) {
+        const float v = S[(size_t)code * nq + k];
+        m[k] = (m[k] < v) ? v : m[k];
+      }
+      seen[code] = 1;
+      nu++;
+    }
+  }
+  for (int64_t j = 0; j < doclens[pid]; j++) seen[codes[off + j]] = 0;
+  float score = 0.0f;
+  for (uint32_t k = 0; k < nq; k++) score += m[k];
+  *n_unique += nu;
+  if (nu) *any = 1;
+  return orc_r16(score);
+}
+
+/* The two filter stages of the GPU branch for one query (index_storage.py:
+ * 120-153): stage 1 keeps the best ndocs of the candidates (torch.topk, only
+ * when there are more), stage 2 rescores those over all their codes and keeps
+ * ndocs / 4.  A 2^20-pid batch in which no candidate holds a passing code
+ * scores 0 throughout (index_storage.py:133-135).  Both stages keep (score,
+ * pid) descending.  Returns the number kept; *n_vec counts distinct visited
+ * codes x nq over both stages. */
+static uint32_t orc_plaid_filter16(const int32_t *pids, uint32_t npids, const float *S, uint32_t n_c,
+                                   uint32_t nq, const int32_t *codes, const int64_t *doclens,
+                                   const int64_t *offsets, const uint8_t *idx, uint32_t ndocs,
+                                   int32_t *out, uint64_t *n_vec) {
+  uint8_t *seen = (uint8_t *)calloc(n_c ? n_c : 1, 1);
+  float *m = (float *)malloc(sizeof(float) * (nq ? nq : 1));
+  orc_sp *sp = (orc_sp *)malloc(sizeof(orc_sp) * (npids ? npids : 1));
+  uint64_t nu = 0;
+  const uint32_t batch = 1u << 20;
+  for (uint32_t b0 = 0; b0 < npids; b0 += batch) {
+    const uint32_t b1 = (npids - b0 > batch) ? b0 + batch : npids;
+    int any = 0;
+    for (uint32_t i = b0; i < b1; i++) {
+      sp[i].pid = pids[i];
+      sp[i].s = orc_plaid_approx16(pids[i], S, nq, codes, doclens, offsets, idx, seen, m, &nu, &any);
+    }
+    if (!any)
+      for (uint32_t i = b0; i < b1; i++) sp[i].s = 0.0f;
+  }
+  qsort(sp, npids, sizeof(orc_sp), orc_cmp_sp_desc);
+  const uint32_t n1 = npids < ndocs ? npids : ndocs;
+  int dummy = 0;
+  for (uint32_t i = 0; i < n1; i++)
+    sp[i].s = orc_plaid_approx16(sp[i].pid, S, nq, codes, doclens, offsets, NULL, seen, m, &nu, &dummy);
+  qsort(sp, n1, sizeof(orc_sp), orc_cmp_sp_desc);
+  const uint32_t n2 = n1 < ndocs / 4 ? n1 : ndocs / 4;
+  for (uint32_t i = 0; i < n2; i++) out[i] = sp[i].pid;
+  *n_vec += nu * nq;
+  free(seen); free(m); free(sp);
+  return n2;
+}
+
+/* IndexScorer.rank (index_storage.py:82-198) for n_q queries of nq tokens,
+ * CPU branch (gpu_branch 0: FP32, filter_pids, zero-initialised MaxSim) or GPU
+ * branch (1: the float16 pipeline above, idx = max_t S16 >= half(threshold),
+ * the comparison torch makes between a half tensor and a Python float).
+ * Candidates, the two-stage filter, the refine, then the best k by (score
+ * desc, pid asc).  Missing results are (-1, -inf).  stats [n_q][3]
+ * (nullable): pids after candidate generation, pids after the filter,
+ * n_vec_score_refine (the GPU branch reports -1 in the reference; here the
+ * same count as the CPU branch). */
+static void orc_plaid_search_impl(uint32_t n_doc, uint32_t n_c, uint32_t dim, uint32_t nbits,
+                                  const int64_t *doclens, const int64_t *offsets, const int32_t *codes,
+                                  const uint8_t *res, const float *C, const float *bw,
+                                  const uint8_t *rbm, const uint8_t *lut, const int32_t *ivf,
+                                  const int64_t *ivf_off, const int64_t *ivf_len, const float *Qall,
+                                  uint32_t n_q, uint32_t nq, uint32_t ncells, float threshold,
+                                  uint32_t ndocs, uint32_t k, int n_threads, int gpu_branch,
+                                  int32_t *out_ids, float *out_s, uint64_t *stats) {
 #ifdef _OPENMP
   if (n_threads > 0) omp_set_num_threads(n_threads);
 #endif
+  float *C16 = NULL;
+  if (gpu_branch) {
+    C16 = (float *)malloc(sizeof(float) * (size_t)n_c * dim);
+    for (size_t i = 0; i < (size_t)n_c * dim; i++) C16[i] = orc_r16(C[i]);
+  }
+  const float thr = gpu_branch ? orc_r16(threshold) : threshold;
 #pragma omp parallel
   {
     float *S = (float *)malloc(sizeof(float) * (size_t)n_c * nq);
+    float *Q16 = (float *)malloc(sizeof(float) * (size_t)nq * dim);
     uint8_t *idx = (uint8_t *)malloc(n_c);
     uint8_t *cell_mark = (uint8_t *)malloc(n_c);
     uint8_t *pid_mark = (uint8_t *)malloc(n_doc ? n_doc : 1);
@@ -439,19 +625,37 @@ void orc_plaid_search_batch(uint32_t n_doc, uint32_t n_c, uint32_t dim, uint32_t
 #pragma omp for schedule(dynamic, 1)
     for (int64_t q = 0; q < (int64_t)n_q; q++) {
       const float *Q = Qall + (size_t)q * nq * dim;
-      orc_plaid_centroid_scores(C, n_c, Q, nq, dim, S);
+      if (gpu_branch) {
+        for (size_t i = 0; i < (size_t)nq * dim; i++) Q16[i] = orc_r16(Q[i]);
+        orc_plaid_centroid_scores(C16, n_c, Q16, nq, dim, S);
+        for (size_t i = 0; i < (size_t)n_c * nq; i++) S[i] = orc_r16(S[i]);
+      } else {
+        orc_plaid_centroid_scores(C, n_c, Q, nq, dim, S);
+      }
       const uint32_t np = orc_plaid_candidates(S, n_c, nq, ncells, ivf, ivf_off, ivf_len, n_doc,
                                                cell_mark, pid_mark, pids);
       for (uint32_t c = 0; c < n_c; c++) {
         float m = S[(size_t)c * nq];
         for (uint32_t t = 1; t < nq; t++) m = fmaxf(m, S[(size_t)c * nq + t]);
-        idx[c] = m >= threshold;
+        idx[c] = m >= thr;
       }
       uint64_t nv = 0;
-      const uint32_t nf = orc_plaid_filter_pids(pids, np, S, n_c, nq, codes, doclens, offsets, idx,
-                                                ndocs, fp, NULL, &nv);
-      orc_plaid_score_pids(Q, nq, fp, nf, doclens, offsets, codes, res, C, bw, rbm, lut, dim, nbits,
-                           sc);
+      uint32_t nf;
+      if (gpu_branch) {
+        nf = orc_plaid_filter16(pids, np, S, n_c, nq, codes, doclens, offsets, idx, ndocs, fp, &nv);
+        uint64_t rows = 0;
+        int64_t *ln = (int64_t *)malloc(sizeof(int64_t) * (nf ? nf : 1));
+        for (uint32_t i = 0; i < nf; i++) rows += (uint64_t)(ln[i] = doclens[fp[i]]);
+        float *D = (float *)malloc(sizeof(float) * (rows ? rows : 1) * dim);
+        orc_plaid_lookup_pids_f16(fp, nf, doclens, offsets, codes, res, C, bw, rbm, lut, dim, nbits, D);
+        orc_plaid_score_packed_f16(Q, nq, D, ln, nf, dim, sc);
+        free(D); free(ln);
+      } else {
+        nf = orc_plaid_filter_pids(pids, np, S, n_c, nq, codes, doclens, offsets, idx, ndocs, fp,
+                                   NULL, &nv);
+        orc_plaid_score_pids(Q, nq, fp, nf, doclens, offsets, codes, res, C, bw, rbm, lut, dim,
+                             nbits, sc);
+      }
       for (uint32_t i = 0; i < nf; i++) { r[i].s = sc[i]; r[i].id = fp[i]; }
       qsort(r, nf, sizeof(orc_si), orc_cmp_si);
       for (uint32_t i = 0; i < k; i++) {
@@ -464,6 +668,36 @@ void orc_plaid_search_batch(uint32_t n_doc, uint32_t n_c, uint32_t dim, uint32_t
         stats[q * 3 + 2] = nv;
       }
     }
-    free(S); free(idx); free(cell_mark); free(pid_mark); free(pids); free(fp); free(sc); free(r);
+    free(S); free(Q16); free(idx); free(cell_mark); free(pid_mark); free(pids); free(fp); free(sc);
+    free(r);
   }
+  free(C16);
+}
+
+/* IndexScorer.rank, CPU branch (index_storage.py:82-198). */
+void orc_plaid_search_batch(uint32_t n_doc, uint32_t n_c, uint32_t dim, uint32_t nbits,
+                            const int64_t *doclens, const int64_t *offsets, const int32_t *codes,
+                            const uint8_t *res, const float *C, const float *bw,
+                            const uint8_t *rbm, const uint8_t *lut, const int32_t *ivf,
+                            const int64_t *ivf_off, const int64_t *ivf_len, const float *Qall,
+                            uint32_t n_q, uint32_t nq, uint32_t ncells, float threshold,
+                            uint32_t ndocs, uint32_t k, int n_threads, int32_t *out_ids,
+                            float *out_s, uint64_t *stats) {
+  orc_plaid_search_impl(n_doc, n_c, dim, nbits, doclens, offsets, codes, res, C, bw, rbm, lut, ivf,
+                        ivf_off, ivf_len, Qall, n_q, nq, ncells, threshold, ndocs, k, n_threads, 0,
+                        out_ids, out_s, stats);
+}
+
+/* IndexScorer.rank, GPU branch (index_storage.py:82-198 with use_gpu). */
+void orc_plaid_search_batch_gpu(uint32_t n_doc, uint32_t n_c, uint32_t dim, uint32_t nbits,
+                                const int64_t *doclens, const int64_t *offsets,
+                                const int32_t *codes, const uint8_t *res, const float *C,
+                                const float *bw, const uint8_t *rbm, const uint8_t *lut,
+                                const int32_t *ivf, const int64_t *ivf_off, const int64_t *ivf_len,
+                                const float *Qall, uint32_t n_q, uint32_t nq, uint32_t ncells,
+                                float threshold, uint32_t ndocs, uint32_t k, int n_threads,
+                                int32_t *out_ids, float *out_s, uint64_t *stats) {
+  orc_plaid_search_impl(n_doc, n_c, dim, nbits, doclens, offsets, codes, res, C, bw, rbm, lut, ivf,
+                        ivf_off, ivf_len, Qall, n_q, nq, ncells, threshold, ndocs, k, n_threads, 1,
+                        out_ids, out_s, stats);
 }

@@ -355,3 +355,31 @@ def test_candidate_marking_paths_agree(fx, opix, monkeypatch):
             np.testing.assert_array_equal(st, rst, err_msg=mode)
     for g in idx.values():
         g.close()
+
+
+def test_search_with_exactly_tied_centroid_scores():
+    """Centroid rows duplicated in pairs, so every query token sees exact ties
+    in its top-ncells list (the float16 GPU branch rounds many scores equal):
+    the cell lists keep (score desc, centroid asc) without duplicates, in both
+    branches, bit-exact against the oracle."""
+    from mvr import plaid
+    from oracle import plaid as P
+    d = _synthetic_plaid(13, 2000, 256)
+    X = d.pop("X")
+    C_ = d["centroids"].copy()
+    C_[128:] = C_[:128]  # centroid c + 128 duplicates c (its IVF list stays empty)
+    d["centroids"] = C_
+    rng = np.random.default_rng(17)
+    Q = X[rng.choice(X.shape[0], 24 * 32)].reshape(24, 32, 128).astype(np.float32)
+    g = plaid.PlaidIndex(**d)
+    o = P.PlaidIndex(**d)
+    for branch in ("cpu", "gpu"):
+        for ncells in (2, 3, 4):
+            print(f"[ties] branch={branch} ncells={ncells}", flush=True)
+            ids, sc, st = g.search(Q, k=10, ncells=ncells, centroid_score_threshold=0.3, ndocs=256,
+                                   return_stats=True, branch=branch)
+            rid, rsc, rst = o.search(Q, 10, ncells, 0.3, 256, branch=branch)
+            np.testing.assert_array_equal(st, rst)
+            np.testing.assert_array_equal(ids, rid)
+            np.testing.assert_array_equal(sc, rsc)
+    g.close()
