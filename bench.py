@@ -90,6 +90,8 @@ def main():
     ap.add_argument("--persistent", action="store_true",
                     help="MaxSim + top-k as one persistent work-queue kernel per pass")
     ap.add_argument("--graph", action="store_true", help="replay each device pass from a hipGraph")
+    ap.add_argument("--maxsim-order", default="auto", choices=["auto", "query", "doc"],
+                    help="BF16 MaxSim loop order (same scores in every order)")
     ap.add_argument("--batch", type=int, default=4096, help="queries per device pass")
     ap.add_argument("--residual-on-host", action="store_true",
                     help="codes and residuals in pinned host memory, staged per device pass")
@@ -129,6 +131,7 @@ def main():
     if args.residual_on_host:
         ix = mvr.Index.from_device(torch.from_numpy(doclens.astype(np.int32)).cuda(), corpus.C, corpus.codes,
                                    corpus.residual, corpus.W, corpus.ivf, residual_on_host=True)
+    ix.set_maxsim_order(args.maxsim_order)
     n_q, P, R, k = cfg["n_q"], cfg["P"], cfg["R"], cfg["k"]
     Q = synth.make_queries(torch, args.seed, corpus.centers, n_q)
     q_nt = q_np = None
@@ -197,6 +200,10 @@ def main():
     achieved_tf = flops / (avg_launch_ms * 1e-3) / 1e12 if avg_launch_ms > 0 else None
     kname = {"fp32": "k_rerank_f32", "fp32_mfma": "k_rerank_f32mfma"}.get(args.precision,
                                                                        f"k_rerank_{args.precision}")
+    # the library's rule for the BF16 loop order (mvr_index_set_maxsim_order)
+    if args.precision == "bf16" and not args.persistent and (
+            args.maxsim_order == "doc" or (args.maxsim_order == "auto" and batch * R >= cfg["n_doc"])):
+        kname = "k_rerank_bf16_dm"
     traffic = measured_traffic(kname, args.config, args.precision)
 
     # the plain FP32 pipelines on the same batch (the reference of the

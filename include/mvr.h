@@ -148,6 +148,16 @@ int mvr_index_load(const char *index_dir, int device, mvr_index **out);
 int mvr_index_create_ex(const mvr_index_desc *desc, int device, uint32_t flags, mvr_index **out);
 int mvr_index_load_ex(const char *index_dir, int device, uint32_t flags, mvr_index **out);
 void mvr_index_free(mvr_index *ix);
+/* Loop order of the BF16 MaxSim in mvr_search (no counterpart in the
+ * reference, whose rerank is per query: RerankPP.hpp:416-471).  Both orders
+ * give bit-identical scores.  MVR_MAXSIM_QUERY: per query, its candidate docs
+ * in turn.  MVR_MAXSIM_DOC: the pass's (query, doc) pairs grouped by doc, each
+ * doc's tokens loaded once for all the queries that selected it.
+ * MVR_MAXSIM_AUTO (default): doc order when a device pass holds at least as
+ * many pairs as the index has docs.  The persistent kernel is always per
+ * query. */
+enum { MVR_MAXSIM_AUTO = 0, MVR_MAXSIM_QUERY = 1, MVR_MAXSIM_DOC = 2 };
+int mvr_index_set_maxsim_order(mvr_index *ix, uint32_t order);
 int mvr_index_info(const mvr_index *ix, uint32_t *n_doc, uint64_t *n_tok, uint32_t *n_centroid,
                    uint32_t *max_list, uint32_t *max_doclen);
 

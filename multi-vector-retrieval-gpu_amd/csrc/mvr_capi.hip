@@ -128,6 +128,8 @@ struct Workspace {
   DBuf Q, meta, part_ids, part_s, top_ids, top_s, cand_cnt, cand_tot, cand_keys, sel_ids, sel_F, sel_n,
       sel_score, out_ids, out_s, counters, res_ids, res_s, res_n, bmax, feat, mlp_out, dyn_np, dyn_r,
       rr_meta, queue, pass_nt, pass_np, pass_rm, hs_bm, hs_bn, hs_bt, hs_wbase;
+  // doc-major BF16 MaxSim (DocMajorBufs)
+  DBuf dm_cnt, dm_work, dm_plist, dm_bp, dm_bd, dm_tot, dm_Qm;
   // host-resident residuals, per pass slot: compact lists and the staged docs
   DBuf hs_docs[2], hs_lens[2], hs_offs[2], hs_cids[2], hs_cnt[2], hs_codes[2], hs_res[2];
   HBuf staging;
@@ -144,7 +146,8 @@ struct Workspace {
             &res_n, &bmax, &feat, &mlp_out, &dyn_np, &dyn_r, &rr_meta, &queue, &pass_nt, &pass_np,
             &pass_rm, &hs_bm, &hs_bn, &hs_bt, &hs_wbase, &hs_docs[0], &hs_docs[1],
             &hs_lens[0], &hs_lens[1], &hs_offs[0], &hs_offs[1], &hs_cids[0], &hs_cids[1], &hs_cnt[0],
-            &hs_cnt[1], &hs_codes[0], &hs_codes[1], &hs_res[0], &hs_res[1]};
+            &hs_cnt[1], &hs_codes[0], &hs_codes[1], &hs_res[0], &hs_res[1], &dm_cnt, &dm_work,
+            &dm_plist, &dm_bp, &dm_bd, &dm_tot, &dm_Qm};
   }
   void drop_graphs() {
     for (auto &g : graphs) (void)hipGraphExecDestroy(g.exec);
@@ -180,6 +183,7 @@ struct mvr_index {
   std::vector<uint32_t> h_doclens;
   Workspace ws;
   PersistCfg tune[4];          // work-queue MaxSim settings per precision
+  uint32_t maxsim_order = MVR_MAXSIM_AUTO;  // BF16 MaxSim loop order (mvr_index_set_maxsim_order)
   bool tuned[4] = {false, false, false, false};
   // MVR_INDEX_RESIDUAL_ON_HOST: residual bytes and a copy of the codes stay in
   // host memory (residual pinned); each pass stages its selected docs
@@ -697,6 +701,13 @@ int hs_gather(mvr_index *ix, Workspace &ws, int slot, uint32_t nb, uint32_t R, h
 
 }  // namespace
 
+int mvr_index_set_maxsim_order(mvr_index *ix, uint32_t order) {
+  if (!ix) return fail(MVR_ERR_INVALID, "NULL index");
+  if (order > MVR_MAXSIM_DOC) return fail(MVR_ERR_INVALID, "unknown MaxSim order %u", order);
+  ix->maxsim_order = order;
+  return MVR_OK;
+}
+
 int mvr_index_info(const mvr_index *ix, uint32_t *n_doc, uint64_t *n_tok, uint32_t *n_centroid,
                    uint32_t *max_list, uint32_t *max_doclen) {
   if (!ix) return fail(MVR_ERR_INVALID, "NULL index");
@@ -827,6 +838,31 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   // persistent work-queue MaxSim: BF16 / FP32-MFMA, top-k (and rescore) <= 256
   const bool persistent = prm->persistent && prm->precision != MVR_FP32 &&
                           k <= kPersistTopkMax && R2 <= kPersistTopkMax;
+  // BF16 MaxSim doc-major (each selected doc's tiles loaded once per pass for
+  // all of its queries) when a pass holds at least as many (query, doc) pairs
+  // as the index has docs, i.e. about 1.6 or more pairs per distinct doc;
+  // query-major below that, where most docs would be loaded for one query
+  const bool doc_major = prm->precision == MVR_BF16 && !persistent && (uint64_t)Bc * R <= UINT32_MAX &&
+                         (ix->maxsim_order == MVR_MAXSIM_DOC ||
+                          (ix->maxsim_order == MVR_MAXSIM_AUTO && (uint64_t)Bc * R >= d.n_doc));
+  DocMajorBufs dmb;
+  if (doc_major) {
+    const size_t nblk = (d.n_doc + kDmDocsPerBlock - 1) / kDmDocsPerBlock;
+    HIP_TRY(ws.dm_cnt.ensure((size_t)d.n_doc * 4));
+    HIP_TRY(ws.dm_work.ensure((size_t)d.n_doc * 8));
+    HIP_TRY(ws.dm_plist.ensure(Bc * R * 4));
+    HIP_TRY(ws.dm_bp.ensure(nblk * 4));
+    HIP_TRY(ws.dm_bd.ensure(nblk * 4));
+    HIP_TRY(ws.dm_tot.ensure(16));
+    HIP_TRY(ws.dm_Qm.ensure(Bc * 512 * 16));
+    dmb.cnt = ws.dm_cnt.as<uint32_t>();
+    dmb.work = ws.dm_work.as<uint2>();
+    dmb.plist = ws.dm_plist.as<uint32_t>();
+    dmb.b_p = ws.dm_bp.as<uint32_t>();
+    dmb.b_d = ws.dm_bd.as<uint32_t>();
+    dmb.tot = ws.dm_tot.as<uint32_t>();
+    dmb.Qm = ws.dm_Qm.as<uint4>();
+  }
   // graph replay: not with learned probing (the model's launches) or stage outputs
   // nor with host-resident residuals (each pass synchronises to stage its docs)
   const bool use_graph = prm->graph && !learned && !stages && !ix->res_host;
@@ -963,7 +999,10 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       HIP_TRY(mark(6));
       return MVR_OK;
     }
-    HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, sid, sl.n, R, dx, prm->precision, clip, sl.score, st));
+    if (doc_major)
+      HIP_TRY(launch_rerank_bf16_dm(Qb, t_max, bnt, nb, sid, sl.n, R, dx, clip, dmb, sl.score, st));
+    else
+      HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, sid, sl.n, R, dx, prm->precision, clip, sl.score, st));
     HIP_TRY(mark(4));
     if (R2) {
       // best R2 by the low-precision score, FP32 MaxSim of those, top-k of the FP32 scores

@@ -108,6 +108,24 @@ hipError_t launch_rerank(const float *Q, uint32_t t_max, const uint32_t *q_ntok,
                          const DevIndex &ix, int precision, int clip_zero, float *sel_score,
                          hipStream_t s);
 
+// Doc-major BF16 MaxSim: the pass's (query, slot) pairs grouped by doc, each
+// doc's tiles loaded once for all of its queries.  Buffers (device):
+struct DocMajorBufs {
+  uint32_t *cnt = nullptr;    // [n_doc] pairs per doc, then the scatter cursor, then each doc's end
+  uint2 *work = nullptr;      // [n_doc] (doc, first pair) of every doc with pairs, by doc id
+  uint32_t *plist = nullptr;  // [n_q * R] pair ids q * R + slot, grouped by doc
+  uint32_t *b_p = nullptr;    // [ceil(n_doc / 4096)] pairs per block of docs, then their prefix
+  uint32_t *b_d = nullptr;    // [ceil(n_doc / 4096)] docs with pairs per block, then their prefix
+  uint32_t *tot = nullptr;    // [3] pairs, work items, work-queue counter
+  uint4 *Qm = nullptr;        // [n_q][512] bf16 query operands in MFMA lane order
+};
+constexpr uint32_t kDmDocsPerBlock = 4096;
+// Same scores, bit for bit, as launch_rerank with precision 1 (BF16).
+hipError_t launch_rerank_bf16_dm(const float *Q, uint32_t t_max, const uint32_t *q_ntok, uint32_t n_q,
+                                 const uint32_t *sel_ids, const uint32_t *sel_n, uint32_t R,
+                                 const DevIndex &ix, int clip_zero, const DocMajorBufs &b,
+                                 float *sel_score, hipStream_t s);
+
 // Per-pass compaction buffers of a host-resident-residual index (device).
 struct HostStage {
   uint32_t *bm = nullptr;           // [ceil(n_doc/32)] selected-doc bitmap

@@ -1021,12 +1021,15 @@ __device__ __forceinline__ uint4 res16_bf16(const uint32_t *tab, const uint2 &rb
   return make_uint4(t[b0 >> 4], t[16 + (b0 & 15u)], t[32 + (b1 >> 4)], t[48 + (b1 & 15u)]);
 }
 
-// The 16 x 32 scores of one tile (see above).
-__device__ __forceinline__ void mma_tile16_bf16(const uint32_t *tab, const uint4 (&cv)[4], const uint2 &rb,
-                                                const bf16x8_t (&qv)[2][4], uint32_t g, floatx4_t (&acc)[2]) {
-  uint4 rv[4];
+// The residual operands of one tile, all four k-steps.
+__device__ __forceinline__ void dec_res16_bf16(const uint32_t *tab, const uint2 &rb, uint32_t g, uint4 (&rv)[4]) {
 #pragma unroll
   for (uint32_t s = 0; s < 4; ++s) rv[s] = res16_bf16(tab, rb, s, g);
+}
+
+// The 16 x 32 scores of one tile from its decoded operands (see above).
+__device__ __forceinline__ void mma16_bf16(const uint4 (&cv)[4], const uint4 (&rv)[4], const bf16x8_t (&qv)[2][4],
+                                           floatx4_t (&acc)[2]) {
   acc[0] = floatx4_t{};
   acc[1] = floatx4_t{};
 #pragma unroll
@@ -1037,6 +1040,14 @@ __device__ __forceinline__ void mma_tile16_bf16(const uint32_t *tab, const uint4
       acc[qh] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(as_bf16x8(rv[s]), qv[qh][s], acc[qh], 0, 0, 0);
     }
   }
+}
+
+// The 16 x 32 scores of one tile (see above).
+__device__ __forceinline__ void mma_tile16_bf16(const uint32_t *tab, const uint4 (&cv)[4], const uint2 &rb,
+                                                const bf16x8_t (&qv)[2][4], uint32_t g, floatx4_t (&acc)[2]) {
+  uint4 rv[4];
+  dec_res16_bf16(tab, rb, g, rv);
+  mma16_bf16(cv, rv, qv, acc);
 }
 
 // Query tokens as the 16x16x32 B operand: lane (col = l & 15, group g) holds
@@ -1060,6 +1071,42 @@ __device__ __forceinline__ void load_q16_bf16(const float *__restrict__ Q, uint3
   }
 }
 
+// Running max m of the values x, as a compare and a select (fmaxf quiets
+// signalling-NaN inputs first: two more VALU instructions per max); a NaN x
+// leaves m as it is.
+__device__ __forceinline__ float vmax(float m, float x) { return x > m ? x : m; }
+
+// Running per-query-token maxima over one 16-row tile's scores (lane: rows
+// 4g .. 4g+3 of query tokens col, 16 + col); only rows below `valid` (rows
+// of the tile that hold doc tokens) count, and a full tile skips the checks.
+__device__ __forceinline__ void tile_max16(const floatx4_t (&acc)[2], uint32_t g, int valid, float &M0,
+                                           float &M1) {
+  if (valid >= 16) {
+#pragma unroll
+    for (uint32_t e = 0; e < 4; ++e) {
+      M0 = vmax(M0, acc[0][e]);
+      M1 = vmax(M1, acc[1][e]);
+    }
+  } else {
+#pragma unroll
+    for (uint32_t e = 0; e < 4; ++e) {
+      if ((int)(4 * g + e) < valid) {
+        M0 = vmax(M0, acc[0][e]);
+        M1 = vmax(M1, acc[1][e]);
+      }
+    }
+  }
+}
+
+// lane l < 32 of the result: the max of query token l over the 16 row groups
+__device__ __forceinline__ float tile_max_reduce(float M0, float M1, uint32_t lane) {
+  M0 = vmax(M0, __shfl_xor(M0, 16));
+  M1 = vmax(M1, __shfl_xor(M1, 16));
+  M0 = vmax(M0, __shfl_xor(M0, 32));
+  M1 = vmax(M1, __shfl_xor(M1, 32));
+  return (lane & 16u) ? M1 : M0;
+}
+
 // Per-query-token max over one doc's tokens; returns in lane l < 32 the max
 // for query token l.  TPS tiles of 16 tokens in flight per step; rows past
 // the doc's end load its last token and are left out of the max.
@@ -1072,31 +1119,24 @@ __device__ __forceinline__ float maxsim16_doc_bf16(uint32_t L, uint64_t off, uin
   const uint32_t r = lane & 15u, g = lane >> 4;
   float M0 = clip_zero ? 0.0f : -FLT_MAX, M1 = M0;
   for (uint32_t i0 = 0; i0 < L; i0 += 16 * TPS) {
+    uint32_t cd[TPS];
+#pragma unroll
+    for (int u = 0; u < TPS; ++u) cd[u] = codes[off + min(i0 + 16 * u + r, L - 1)];
     uint4 cv[TPS][4];
     uint2 rb[TPS];
 #pragma unroll
     for (int u = 0; u < TPS; ++u) {
       const uint64_t tok = off + min(i0 + 16 * u + r, L - 1);
-      load_row16_bf16(Cm, residual, tok, codes[tok], g, cv[u], rb[u]);
+      load_row16_bf16(Cm, residual, tok, cd[u], g, cv[u], rb[u]);
     }
 #pragma unroll
     for (int u = 0; u < TPS; ++u) {
       floatx4_t acc[2];
       mma_tile16_bf16(tab, cv[u], rb[u], qv, g, acc);
-#pragma unroll
-      for (uint32_t j = 0; j < 4; ++j) {
-        if (i0 + 16 * u + 4 * g + j < L) {
-          M0 = fmaxf(M0, acc[0][j]);
-          M1 = fmaxf(M1, acc[1][j]);
-        }
-      }
+      tile_max16(acc, g, (int)L - (int)(i0 + 16 * u), M0, M1);
     }
   }
-  M0 = fmaxf(M0, __shfl_xor(M0, 16));
-  M1 = fmaxf(M1, __shfl_xor(M1, 16));
-  M0 = fmaxf(M0, __shfl_xor(M0, 32));
-  M1 = fmaxf(M1, __shfl_xor(M1, 32));
-  return (lane & 16u) ? M1 : M0;
+  return tile_max_reduce(M0, M1, lane);
 }
 
 __global__ __launch_bounds__(256) void k_rerank_bf16(
@@ -1131,6 +1171,327 @@ __global__ __launch_bounds__(256) void k_rerank_bf16(
       sel_score[(size_t)q * R + r] = sc;
     }
     wave_lds_sync();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Doc-major BF16 MaxSim.  A pass of nb queries x R candidates holds each doc
+// about nb*R/n_doc times (about four at 4,096 queries x 1,000 over 1M docs),
+// so the query-major kernel above gathers every candidate doc's centroid rows
+// once per query that chose it.  Here the (query, slot) pairs are grouped by
+// doc (counting sort: k_dm_count, k_dm_bsum, k_dm_bscan, k_dm_offsets,
+// k_dm_scatter), and a wave loads a doc's tiles once -- up to kDmTiles tiles
+// of 16 tokens in registers -- then scores every query of the doc against
+// them, reading each query's B operand as eight coalesced 1 KB loads from a
+// per-pass bf16 copy in MFMA lane order (k_q_bf16m).  The MFMA operands, their
+// order, the row mask and the left-to-right sum over query tokens are those
+// of maxsim16_doc_bf16, so every score is bit-identical to k_rerank_bf16.
+// Docs longer than kDmTiles tiles go chunk by chunk over groups of up to
+// kDmGroup of their pairs, the per-token maxima of a group kept in LDS
+// between chunks (max is exact, so the chunking changes no bit).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kDmTiles = 8;     // 16-token doc tiles held in registers (128 tokens)
+constexpr uint32_t kDmGroup = 16;    // pairs per group of a multi-chunk doc
+constexpr uint32_t kDmGrab = 16;     // work items a wave takes per queue step
+constexpr uint32_t kDmHoist = 5;     // chunks of up to this many tiles decode residuals once per chunk
+constexpr uint32_t kDmBlockDocs = kDmDocsPerBlock;  // docs per block of the offset scan (256 threads x 16)
+
+// Qm[q][qh][s][lane] = the B operand qv[qh][s] of load_q16_bf16 for lane.
+__global__ __launch_bounds__(256) void k_q_bf16m(const float *__restrict__ Q, uint32_t t_max,
+                                                 const uint32_t *__restrict__ q_ntok, uint32_t nb,
+                                                 uint4 *__restrict__ Qm) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;  // (q, qh, s, lane)
+  if (i >= nb * 512u) return;
+  const uint32_t q = i >> 9, qh = (i >> 8) & 1u, s = (i >> 6) & 3u, lane = i & 63u;
+  const uint32_t T = q_ntok ? q_ntok[q] : t_max;
+  const uint32_t t = 16 * qh + (lane & 15u), g = lane >> 4;
+  const float4 *qr = reinterpret_cast<const float4 *>(Q + ((size_t)q * t_max + (t < T ? t : 0)) * kDim) + 8 * g;
+  const float4 a = qr[2 * s], b = qr[2 * s + 1];
+  const float x[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  bf16x8_t v;
+#pragma unroll
+  for (uint32_t e = 0; e < 8; ++e) v[e] = (__bf16)(t < T ? x[e] : 0.0f);
+  Qm[i] = __builtin_bit_cast(uint4, v);
+}
+
+// cnt[doc] += 1 for every selected (query, slot)
+__global__ __launch_bounds__(256) void k_dm_count(const uint32_t *__restrict__ sel_ids,
+                                                  const uint32_t *__restrict__ sel_n, uint32_t R,
+                                                  uint32_t *__restrict__ cnt) {
+  const uint32_t q = blockIdx.y, n = min(sel_n[q], R);
+  for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256)
+    atomicAdd(&cnt[sel_ids[(size_t)q * R + j]], 1u);
+}
+
+// exclusive scan over a 256-thread block of (a, b); returns the block totals
+__device__ __forceinline__ void dm_scan256(uint32_t a, uint32_t b, uint32_t &ex_a, uint32_t &ex_b,
+                                           uint32_t &tot_a, uint32_t &tot_b) {
+  __shared__ uint32_t sa[4], sb[4];
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  uint32_t ia = a, ib = b;
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t ua = __shfl_up(ia, o), ub = __shfl_up(ib, o);
+    if (lane >= o) {
+      ia += ua;
+      ib += ub;
+    }
+  }
+  if (lane == 63) {
+    sa[wv] = ia;
+    sb[wv] = ib;
+  }
+  __syncthreads();
+  uint32_t ba = 0, bb = 0;
+  tot_a = tot_b = 0;
+  for (uint32_t i = 0; i < 4; ++i) {
+    if (i < wv) {
+      ba += sa[i];
+      bb += sb[i];
+    }
+    tot_a += sa[i];
+    tot_b += sb[i];
+  }
+  ex_a = ba + ia - a;
+  ex_b = bb + ib - b;
+}
+
+// per block of kDmBlockDocs docs: pairs and docs with pairs
+__global__ __launch_bounds__(256) void k_dm_bsum(const uint32_t *__restrict__ cnt, uint32_t n_doc,
+                                                 uint32_t *__restrict__ b_p, uint32_t *__restrict__ b_d) {
+  const size_t d0 = (size_t)blockIdx.x * kDmBlockDocs + threadIdx.x * 16;
+  uint32_t np = 0, nd = 0;
+  for (uint32_t e = 0; e < 16; ++e) {
+    const uint32_t c = d0 + e < n_doc ? cnt[d0 + e] : 0u;
+    np += c;
+    nd += c != 0;
+  }
+  uint32_t ea, eb, ta, tb;
+  dm_scan256(np, nd, ea, eb, ta, tb);
+  if (threadIdx.x == 0) {
+    b_p[blockIdx.x] = ta;
+    b_d[blockIdx.x] = tb;
+  }
+}
+
+// in-place exclusive scan of the block sums; tot = (pairs, work items);
+// resets the work-queue counter
+__global__ __launch_bounds__(1024) void k_dm_bscan(uint32_t *__restrict__ b_p, uint32_t *__restrict__ b_d,
+                                                   uint32_t nblk, uint32_t *__restrict__ tot) {
+  __shared__ uint32_t sp[16], sd[16];
+  const uint32_t per = (nblk + 1023) / 1024;
+  const uint32_t i0 = min(nblk, threadIdx.x * per), i1 = min(nblk, i0 + per);
+  uint32_t np = 0, nd = 0;
+  for (uint32_t i = i0; i < i1; ++i) {
+    np += b_p[i];
+    nd += b_d[i];
+  }
+  const uint32_t lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  uint32_t ip = np, id = nd;
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const uint32_t up = __shfl_up(ip, o), ud = __shfl_up(id, o);
+    if (lane >= o) {
+      ip += up;
+      id += ud;
+    }
+  }
+  if (lane == 63) {
+    sp[wv] = ip;
+    sd[wv] = id;
+  }
+  __syncthreads();
+  uint32_t bp = 0, bd = 0, tp = 0, td = 0;
+  for (uint32_t i = 0; i < 16; ++i) {
+    if (i < wv) {
+      bp += sp[i];
+      bd += sd[i];
+    }
+    tp += sp[i];
+    td += sd[i];
+  }
+  uint32_t base_p = bp + ip - np, base_d = bd + id - nd;
+  for (uint32_t i = i0; i < i1; ++i) {
+    const uint32_t a = b_p[i], b = b_d[i];
+    b_p[i] = base_p;
+    b_d[i] = base_d;
+    base_p += a;
+    base_d += b;
+  }
+  if (threadIdx.x == 0) {
+    tot[0] = tp;
+    tot[1] = td;
+    tot[2] = 0;  // work-queue counter
+  }
+}
+
+// pair offsets: work[i] = (doc, first pair) for the i-th doc with pairs, and
+// cnt[doc] = its first pair (the scatter's cursor)
+__global__ __launch_bounds__(256) void k_dm_offsets(uint32_t *__restrict__ cnt, uint32_t n_doc,
+                                                    const uint32_t *__restrict__ b_p,
+                                                    const uint32_t *__restrict__ b_d, uint2 *__restrict__ work) {
+  const size_t d0 = (size_t)blockIdx.x * kDmBlockDocs + threadIdx.x * 16;
+  uint32_t c[16], np = 0, nd = 0;
+#pragma unroll
+  for (uint32_t e = 0; e < 16; ++e) {
+    c[e] = d0 + e < n_doc ? cnt[d0 + e] : 0u;
+    np += c[e];
+    nd += c[e] != 0;
+  }
+  uint32_t ea, eb, ta, tb;
+  dm_scan256(np, nd, ea, eb, ta, tb);
+  uint32_t p = b_p[blockIdx.x] + ea, w = b_d[blockIdx.x] + eb;
+#pragma unroll
+  for (uint32_t e = 0; e < 16; ++e) {
+    if (c[e]) {
+      work[w++] = make_uint2((uint32_t)(d0 + e), p);
+      cnt[d0 + e] = p;
+      p += c[e];
+    }
+  }
+}
+
+// plist[cursor(doc)++] = q * R + slot; afterwards cnt[doc] = the doc's end
+__global__ __launch_bounds__(256) void k_dm_scatter(const uint32_t *__restrict__ sel_ids,
+                                                    const uint32_t *__restrict__ sel_n, uint32_t R,
+                                                    uint32_t *__restrict__ cnt, uint32_t *__restrict__ plist) {
+  const uint32_t q = blockIdx.y, n = min(sel_n[q], R);
+  for (uint32_t j = blockIdx.x * 256 + threadIdx.x; j < n; j += gridDim.x * 256) {
+    const uint32_t pos = atomicAdd(&cnt[sel_ids[(size_t)q * R + j]], 1u);
+    plist[pos] = q * R + j;
+  }
+}
+
+__device__ __forceinline__ void load_qm(const uint4 *__restrict__ Qm, uint32_t q, uint32_t lane,
+                                        bf16x8_t (&qv)[2][4]) {
+  const uint4 *p = Qm + (size_t)q * 512 + lane;
+#pragma unroll
+  for (uint32_t qh = 0; qh < 2; ++qh)
+#pragma unroll
+    for (uint32_t s = 0; s < 4; ++s) qv[qh][s] = as_bf16x8(p[64 * (4 * qh + s)]);
+}
+
+// One chunk of NT doc tiles (rows t0*16 ..) against the ng pairs pg .. of the
+// doc: per-token maxima into mg[j] (combined with earlier chunks unless t0 = 0).
+template <uint32_t NT>
+__device__ __forceinline__ void dm_chunk(const uint4 *__restrict__ Qm, uint32_t R, const uint32_t *__restrict__ plist,
+                                         uint32_t pg, uint32_t ng, uint32_t L, uint64_t off, uint32_t t0,
+                                         const uint32_t *__restrict__ codes, const uint8_t *__restrict__ residual,
+                                         const uint16_t *__restrict__ Cm, const uint32_t *tab, float init,
+                                         uint32_t lane, float (*mg)[kMaxQTok + 1]) {
+  const uint32_t r = lane & 15u, g = lane >> 4;
+  // every tile's code first, then the rows: each row load depends on its
+  // code, and waiting for one load waits for all issued before it
+  uint32_t cd[NT];
+#pragma unroll
+  for (uint32_t u = 0; u < NT; ++u) cd[u] = codes[off + min(16 * (t0 + u) + r, L - 1)];
+  uint4 cv[NT][4];
+  uint2 rb[NT];
+#pragma unroll
+  for (uint32_t u = 0; u < NT; ++u)
+    load_row16_bf16(Cm, residual, off + min(16 * (t0 + u) + r, L - 1), cd[u], g, cv[u], rb[u]);
+  // chunks of up to kDmHoist tiles decode their residual operands once for
+  // all pairs; longer ones decode per pair (registers)
+  constexpr bool kHoist = NT <= kDmHoist;
+  uint4 rh[kHoist ? NT : 1][4];
+  if (kHoist) {
+#pragma unroll
+    for (uint32_t u = 0; u < NT; ++u) dec_res16_bf16(tab, rb[u], g, rh[kHoist ? u : 0]);
+  }
+  // rows of the chunk's last tile that hold doc tokens (the others are full)
+  const int last_valid = (int)L - (int)(16 * (t0 + NT - 1));
+  for (uint32_t j = 0; j < ng; ++j) {
+    bf16x8_t qv[2][4];
+    load_qm(Qm, plist[pg + j] / R, lane, qv);
+    // all eight loads issued before the first MFMA waits on one of them
+    __builtin_amdgcn_sched_barrier(0);
+    // per-pair decode one tile ahead; the empty asm keeps the compiler from
+    // hoisting it out of the pair loop (16 more live registers per tile)
+    uint4 rv[2][4];
+    if (!kHoist) {
+      uint2 b = rb[0];
+      asm volatile("" : "+v"(b.x), "+v"(b.y));
+      dec_res16_bf16(tab, b, g, rv[0]);
+    }
+    float M0 = init, M1 = init;
+#pragma unroll
+    for (uint32_t u = 0; u < NT; ++u) {
+      if (!kHoist && u + 1 < NT) {
+        uint2 b = rb[u + 1];
+        asm volatile("" : "+v"(b.x), "+v"(b.y));
+        dec_res16_bf16(tab, b, g, rv[(u + 1) & 1]);
+      }
+      floatx4_t acc[2];
+      mma16_bf16(cv[u], kHoist ? rh[kHoist ? u : 0] : rv[u & 1], qv, acc);
+      tile_max16(acc, g, u + 1 < NT ? 16 : last_valid, M0, M1);
+    }
+    const float M = tile_max_reduce(M0, M1, lane);
+    if (lane < 32) {  // lane l: query token l; each lane rereads only its own entry
+      mg[j][lane] = t0 != 0 ? vmax(M, mg[j][lane]) : M;
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_rerank_bf16_dm(
+    const uint4 *__restrict__ Qm, uint32_t t_max, const uint32_t *__restrict__ q_ntok, uint32_t R,
+    const uint2 *__restrict__ work, uint32_t *__restrict__ tot, const uint32_t *__restrict__ pend,
+    const uint32_t *__restrict__ plist, const uint32_t *__restrict__ doclens,
+    const uint64_t *__restrict__ docoff, const uint32_t *__restrict__ codes,
+    const uint8_t *__restrict__ residual, const uint16_t *__restrict__ Cm, const float *__restrict__ W,
+    int clip_zero, float *__restrict__ sel_score) {
+  static_assert(kDmTiles == 8, "dm_chunk is dispatched for 1 .. 8 tiles");
+  __shared__ uint32_t tab[kPairTab];
+  // per-token maxima of a wave's pair group; rows padded so that the lanes
+  // summing one row each read distinct banks
+  __shared__ float Mg[4][kDmGroup][kMaxQTok + 1];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  fill_pair_tab(tab, W, tid, 256);
+  __syncthreads();
+  float(*mg)[kMaxQTok + 1] = Mg[wave];
+  const uint32_t n_work = tot[1];
+  const float init = clip_zero ? 0.0f : -FLT_MAX;
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&tot[2], kDmGrab);
+    base = __builtin_amdgcn_readfirstlane(base);
+    if (base >= n_work) break;
+    const uint32_t w_end = min(base + kDmGrab, n_work);
+    for (uint32_t wi = base; wi < w_end; ++wi) {
+      const uint2 it = work[wi];
+      const uint32_t doc = it.x, p0 = it.y, p1 = pend[doc];
+      const uint32_t L = doclens[doc];
+      const uint64_t off = docoff[doc];
+      const uint32_t n_t = (L + 15) >> 4;
+      for (uint32_t pg = p0; pg < p1; pg += kDmGroup) {
+        const uint32_t ng = min(kDmGroup, p1 - pg);
+        if (n_t == 0 && lane < 32)  // an empty doc: the identity for every token
+          for (uint32_t j = 0; j < ng; ++j) mg[j][lane] = init;
+        for (uint32_t t0 = 0; t0 < n_t; t0 += kDmTiles) {
+#define MVR_DM_CHUNK(N) dm_chunk<N>(Qm, R, plist, pg, ng, L, off, t0, codes, residual, Cm, tab, init, lane, mg)
+          switch (min(kDmTiles, n_t - t0)) {
+            case 1: MVR_DM_CHUNK(1); break;
+            case 2: MVR_DM_CHUNK(2); break;
+            case 3: MVR_DM_CHUNK(3); break;
+            case 4: MVR_DM_CHUNK(4); break;
+            case 5: MVR_DM_CHUNK(5); break;
+            case 6: MVR_DM_CHUNK(6); break;
+            case 7: MVR_DM_CHUNK(7); break;
+            default: MVR_DM_CHUNK(8); break;
+          }
+#undef MVR_DM_CHUNK
+        }
+        // scores of the group: lane j sums pair j's maxima left to right
+        wave_lds_sync();
+        if (lane < ng) {
+          const uint32_t pq = plist[pg + lane], q = pq / R;
+          const uint32_t T = q_ntok ? q_ntok[q] : t_max;
+          float sc = 0.0f;
+          for (uint32_t tt = 0; tt < T; ++tt) sc += mg[lane][tt];
+          sel_score[pq] = sc;
+        }
+        wave_lds_sync();
+      }
+    }
   }
 }
 
@@ -2479,6 +2840,33 @@ hipError_t launch_rerank(const float *Q, uint32_t t_max, const uint32_t *q_ntok,
   } else {
     return hipErrorNotSupported;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_rerank_bf16_dm(const float *Q, uint32_t t_max, const uint32_t *q_ntok, uint32_t n_q,
+                                 const uint32_t *sel_ids, const uint32_t *sel_n, uint32_t R,
+                                 const DevIndex &ix, int clip_zero, const DocMajorBufs &b,
+                                 float *sel_score, hipStream_t s) {
+  if (n_q == 0 || ix.n_doc == 0) return hipSuccess;
+  if (!ix.C_bf16m || (uint64_t)n_q * R > UINT32_MAX) return hipErrorInvalidValue;
+  const uint32_t nblk = (ix.n_doc + kDmBlockDocs - 1) / kDmBlockDocs;
+  static int occ = [] {
+    int o = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_rerank_bf16_dm, 256, 0) != hipSuccess || o < 1) o = 2;
+    return o;
+  }();
+  k_fill_u32<<<(ix.n_doc + 255) / 256, 256, 0, s>>>(b.cnt, 0u, ix.n_doc);
+  k_q_bf16m<<<(n_q * 512u + 255) / 256, 256, 0, s>>>(Q, t_max, q_ntok, n_q, b.Qm);
+  const dim3 gq(std::min((R + 255) / 256, 16u), n_q);
+  k_dm_count<<<gq, 256, 0, s>>>(sel_ids, sel_n, R, b.cnt);
+  k_dm_bsum<<<nblk, 256, 0, s>>>(b.cnt, ix.n_doc, b.b_p, b.b_d);
+  k_dm_bscan<<<1, 1024, 0, s>>>(b.b_p, b.b_d, nblk, b.tot);
+  k_dm_offsets<<<nblk, 256, 0, s>>>(b.cnt, ix.n_doc, b.b_p, b.b_d, b.work);
+  k_dm_scatter<<<gq, 256, 0, s>>>(sel_ids, sel_n, R, b.cnt, b.plist);
+  const uint32_t grid = std::max(1u, ix.n_cu) * (uint32_t)occ;
+  k_rerank_bf16_dm<<<grid, 256, 0, s>>>(b.Qm, t_max, q_ntok, R, b.work, b.tot, b.cnt, b.plist, ix.doclens,
+                                        ix.docoff, ix.codes, ix.residual, ix.C_bf16m, ix.W, clip_zero,
+                                        sel_score);
   return hipGetLastError();
 }
 

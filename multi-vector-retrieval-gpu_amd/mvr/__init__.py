@@ -153,6 +153,11 @@ class Index:
         prec = _PRECISION.get(precision, precision) if isinstance(precision, str) else precision
         check(lib().mvr_index_set_tuning(self._h, prec, wgs_per_cu, tiles))
 
+    def set_maxsim_order(self, order: str):
+        """BF16 MaxSim loop order of search(): "auto", "query" or "doc" (same
+        scores in every order)."""
+        check(lib().mvr_index_set_maxsim_order(self._h, {"auto": 0, "query": 1, "doc": 2}[order]))
+
     def get_tuning(self, precision):
         """(workgroups per CU, tiles in flight) or None before tuning."""
         prec = _PRECISION.get(precision, precision) if isinstance(precision, str) else precision

@@ -80,6 +80,7 @@ SIGNATURES = {
     "mvr_index_create_ex": (C.c_int, [C.POINTER(IndexDesc), C.c_int, C.c_uint32, C.POINTER(C.c_void_p)]),
     "mvr_index_load_ex": (C.c_int, [C.c_char_p, C.c_int, C.c_uint32, C.POINTER(C.c_void_p)]),
     "mvr_index_free": (None, [C.c_void_p]),
+    "mvr_index_set_maxsim_order": (C.c_int, [C.c_void_p, C.c_uint32]),
     "mvr_index_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_uint64),
                                  C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     "mvr_search": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
