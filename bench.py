@@ -222,6 +222,10 @@ def main():
     recall = recall_report(torch, args, cfg, corpus, corpus.index, Q, answers, k, doclens, q_nt)
     cands = (candidate_counts(ix, Q, P, R, k, q_nt, q_np, min(n_q, 1024))
              if not (args.residual_on_host or args.no_others) else None)
+    probe8 = (probe_agreement(ix, Q, P, R, k, min(n_q, 256))
+              if args.precision == "fp8" and q_nt is None and not args.residual_on_host else None)
+    if probe8:
+        log(f"[bench] fp8 probe vs exact FP32 probe: {probe8}")
     cpu = None
     if not args.no_cpu and args.cpu_queries > 0 and q_nt is None:
         ref = others.get("fp32") or (run if args.precision == "fp32" else None)
@@ -266,6 +270,7 @@ def main():
                       for p, r in others.items()},
         "stages_ms_per_step": {f: round(stages[f], 3) for f in stage_keys},
         "candidates": cands,
+        "fp8_probe": probe8,
         "host_staging": ({"ms_per_step": round(tm["staging_ms"], 3), "docs_per_step": tm["staged_docs"],
                           "tokens_per_step": tm["staged_tokens"], "bytes_per_token": 36}
                          if args.residual_on_host else None),
@@ -339,6 +344,19 @@ def candidate_counts(ix, Q, P, R, k, q_nt, q_np, n):
     return {"queries": int(n), "mean": round(float(c.mean()), 1), "p50": int(np.percentile(c, 50)),
             "p99": int(np.percentile(c, 99)), "max": int(c.max()), "cap": 8192,
             "cap_binding_frac": round(float((c > 8192).mean()), 4)}
+
+
+def probe_agreement(ix, Q, P, R, k, n):
+    """FP8 probe against the exact FP32 probe on the first n queries: the
+    fraction of query-token rows whose probed centroid set is the exact top-P
+    (ids in order), and the fraction of probed ids that belong to it."""
+    Qh = Q[:n].cpu().numpy()
+    _, _, s8 = ix.search(Qh, P, R, k, "fp8", stages=True, batch=n)
+    _, _, s32 = ix.search(Qh, P, R, k, "fp32", stages=True, batch=n)
+    a, b = s8["top_ids"].reshape(-1, P), s32["top_ids"].reshape(-1, P)
+    same = float((a == b).all(1).mean())
+    inter = float(np.mean([len(set(x.tolist()) & set(y.tolist())) / P for x, y in zip(a, b)]))
+    return {"queries": int(n), "rows_exact": round(same, 4), "ids_shared": round(inter, 4)}
 
 
 def score_errors(torch, corpus, Q, ids, sc, doclens, nr):

@@ -90,7 +90,8 @@ typedef struct {
   uint32_t batch;       /* queries per device pass (0 = 1024) */
   uint32_t learned;     /* 1: per-query nprobe/probe_topk from the attached probe model */
   uint32_t persistent;  /* 1: MaxSim (and the rescore) + top-k as one persistent work-queue
-                           kernel per pass (MVR_FP32_MFMA / MVR_BF16, topk and rescore <= 256) */
+                           kernel per pass (MVR_FP32_MFMA / MVR_BF16 / MVR_FP8, topk and
+                           rescore <= 256; other settings are MVR_ERR_UNSUPPORTED) */
   uint32_t graph;       /* 1: each device pass is replayed from a captured hipGraph
                            (cached per pass shape; not with learned probing) */
 } mvr_search_params;

@@ -835,9 +835,13 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     HIP_TRY(ws.res_s.ensure(Bc * R2 * 4));
     HIP_TRY(ws.res_n.ensure(Bc * 4));
   }
-  // persistent work-queue MaxSim: BF16 / FP32-MFMA, top-k (and rescore) <= 256
-  const bool persistent = prm->persistent && prm->precision != MVR_FP32 &&
-                          k <= kPersistTopkMax && R2 <= kPersistTopkMax;
+  // persistent work-queue MaxSim: BF16 / FP8 / FP32-MFMA, top-k (and rescore) <= 256
+  if (prm->persistent && prm->precision == MVR_FP32)
+    return fail(MVR_ERR_UNSUPPORTED, "the persistent kernel has no exact-FP32 variant (use MVR_FP32_MFMA)");
+  if (prm->persistent && (k > kPersistTopkMax || R2 > kPersistTopkMax))
+    return fail(MVR_ERR_UNSUPPORTED, "the persistent kernel selects at most %u (topk %u, rescore %u)",
+                kPersistTopkMax, k, R2);
+  const bool persistent = prm->persistent != 0;
   // BF16 MaxSim doc-major (each selected doc's tiles loaded once per pass for
   // all of its queries) when a pass holds at least as many (query, doc) pairs
   // as the index has docs, i.e. about 1.6 or more pairs per distinct doc;
@@ -937,10 +941,12 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       HIP_TRY(launch_centroid_topk(Qb, rm, n_rows, nb * t_max, d, P, MVR_FP32, n_split,
                                    ws.part_ids.as<uint32_t>(), ws.part_s.as<float>(),
                                    ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), nullptr, st));
+    else if (prm->precision == MVR_FP8)
+      HIP_TRY(launch_centroid_topk_fp8(Qb, nb, t_max, rm, n_rows, d, P, ws.bmax.as<float>(),
+                                       ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), st));
     else
       HIP_TRY(launch_centroid_topk_bf16(Qb, nb, t_max, rm, n_rows, d, P, ws.bmax.as<float>(),
-                                        ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(),
-                                        fp32_mode || prm->precision == MVR_FP8, st));
+                                        ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), fp32_mode, st));
     HIP_TRY(mark(1));
     const uint32_t *q_r = nullptr;
     if (learned) {

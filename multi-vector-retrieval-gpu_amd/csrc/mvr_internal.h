@@ -67,6 +67,14 @@ hipError_t launch_centroid_topk_bf16(const float *Q, uint32_t n_q, uint32_t t_ma
                                      uint32_t P, float *bmax, uint32_t *top_ids, float *top_s,
                                      bool exact_f32, hipStream_t s);
 
+// FP8 centroid scoring (the FP8 mode's probe, approximate): e4m3 MFMA tile
+// maxima with per-token and per-centroid scales, then the exact FP32 top-P
+// over the centroids of the kFp8ProbeTiles * P tiles with the highest maxima
+constexpr uint32_t kFp8ProbeTiles = 2;
+hipError_t launch_centroid_topk_fp8(const float *Q, uint32_t n_q, uint32_t t_max, const uint32_t *row_map,
+                                    uint32_t n_rows, const DevIndex &ix, uint32_t P, float *bmax,
+                                    uint32_t *top_ids, float *top_s, hipStream_t s);
+
 hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 
 // clamp0: per-(doc, token) maxima are max(0, .) (the PP filter); else the

@@ -2109,6 +2109,9 @@ __global__ __launch_bounds__(256) void k_rerank_persist(
 // so pass 2 only rescans those tiles.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kBmWaves = 8;
+// |bf16-input dot - exact FP32 dot| <= kBf16Margin ||q|| ||C|| for FP32 inputs:
+// input rounding 2u + u^2 (u = 2^-8) plus two 128-term FP32 accumulations
+constexpr float kBf16Margin = 0.008f;
 constexpr uint32_t kBmStage = 8;  // tiles per LDS stage
 
 __global__ __launch_bounds__(512) void k_ct_bf16_blockmax(
@@ -2202,6 +2205,123 @@ __global__ __launch_bounds__(512) void k_ct_bf16_blockmax(
   }
 }
 
+// Centroid scoring in FP8 (the FP8 mode), pass 1: tile maxima on
+// v_mfma_f32_32x32x64_f8f6f4.  Centroid rows are the e4m3 codes of C_c / s_c
+// (s_c = max_d |C_c[d]| / 448, the FP8 MaxSim's copy) and query tokens the
+// e4m3 codes of q_t / s_t (s_t = max_d |q_t[d]| / 448, the FP8 MaxSim's
+// per-token scale); score(t, c) = s_t s_c sum_d q8[d] c8[d], the products
+// exact in FP32.  Lane (row / column l & 31, h = l >> 5) holds dims
+// 64m + 32h .. +31 of its row in MFMA m = 0, 1 -- the same assignment for both
+// operands, so the two MFMAs sum over all 128 dims.  Workgroup, staging and
+// output as in k_ct_bf16_blockmax; a tile's 32 row scales are staged with its
+// rows and applied to the accumulators before the maximum (s_t after it).
+typedef int intx8_t __attribute__((ext_vector_type(8)));
+constexpr uint32_t kF8Row = 9;  // uint4 per LDS row: 128 bytes + 16 of padding (distinct banks per 16 lanes)
+
+__global__ __launch_bounds__(512) void k_ct_fp8_blockmax(
+    const float *__restrict__ Q, uint32_t t_max, uint32_t n_q, const uint8_t *__restrict__ C8,
+    const float *__restrict__ csc, uint32_t n_c, uint32_t n_tiles, uint32_t tiles_per_split,
+    float *__restrict__ bmax) {
+  __shared__ uint4 As[kBmStage][32][kF8Row];
+  __shared__ __align__(16) float Ss[kBmStage][32];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const uint32_t h = lane >> 5, col = lane & 31;
+  const uint32_t q = blockIdx.x * kBmWaves + wave;
+  const bool vq = q < n_q;
+  intx8_t qf[2];
+  float st;
+  {
+    // the row pointer is clamped into the query array so that every load is
+    // issued unconditionally; rows past the query's tokens are zeroed
+    const bool vr = vq && col < t_max;
+    const float *qr = Q + ((size_t)(vq ? q : 0) * t_max + (vr ? col : 0)) * kDim + 32 * h;
+    float4 x[16];  // dims 64m + 32h + 4i .. +3 in x[8m + i]
+#pragma unroll
+    for (uint32_t m = 0; m < 2; ++m)
+#pragma unroll
+      for (uint32_t i = 0; i < 8; ++i) x[8 * m + i] = reinterpret_cast<const float4 *>(qr + 64 * m)[i];
+    float amax = 0.0f;
+#pragma unroll
+    for (uint32_t i = 0; i < 16; ++i) {
+      if (!vr) x[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+      amax = fmaxf(amax, fmaxf(fmaxf(fabsf(x[i].x), fabsf(x[i].y)), fmaxf(fabsf(x[i].z), fabsf(x[i].w))));
+    }
+    amax = fmaxf(amax, __shfl_xor(amax, 32));  // the other half holds the other 64 dims
+    st = amax / 448.0f;
+    const float inv = amax > 0.0f ? 448.0f / amax : 0.0f;
+#pragma unroll
+    for (uint32_t m = 0; m < 2; ++m)
+#pragma unroll
+      for (uint32_t i = 0; i < 8; ++i) {
+        const float4 v = x[8 * m + i];
+        qf[m][i] = (int)(fp8x2(v.x * inv, v.y * inv) | (fp8x2(v.z * inv, v.w * inv) << 16));
+      }
+  }
+  const uint32_t tb = blockIdx.y * tiles_per_split;
+  const uint32_t te = min(n_tiles, tb + tiles_per_split);
+  const uint4 *C4 = reinterpret_cast<const uint4 *>(C8);
+  // a stage is kBmStage tiles of 32 rows x 8 pieces of 16 bytes: thread tid
+  // loads pieces tid + 512 i (i = 0..3); threads below 256 load one row scale
+  uint4 pf[4];
+  float ps = 0.0f;
+  auto load_stage = [&](uint32_t t0) {
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+      const uint32_t pc = tid + 512 * i, u = pc >> 8, r = (pc >> 3) & 31u, k = pc & 7u;
+      const uint32_t tile = t0 + u, c = tile * 32 + r;
+      pf[i] = (tile < te && c < n_c) ? C4[(size_t)c * 8 + k] : make_uint4(0u, 0u, 0u, 0u);
+    }
+    const uint32_t tile = t0 + (tid >> 5), c = tile * 32 + (tid & 31u);
+    ps = (tid < 256 && tile < te && c < n_c) ? csc[c] : 0.0f;
+  };
+  load_stage(tb);
+  for (uint32_t t0 = tb; t0 < te; t0 += kBmStage) {
+    __syncthreads();  // the previous stage's readers are done
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+      const uint32_t pc = tid + 512 * i;
+      As[pc >> 8][(pc >> 3) & 31u][pc & 7u] = pf[i];
+    }
+    if (tid < 256) Ss[tid >> 5][tid & 31u] = ps;
+    __syncthreads();
+    if (t0 + kBmStage < te) load_stage(t0 + kBmStage);
+    float mxb[kBmStage];
+#pragma unroll
+    for (uint32_t u = 0; u < kBmStage; ++u) {
+      const uint32_t tile = t0 + u;
+      mxb[u] = -INFINITY;
+      if (tile < te) {  // uniform across the workgroup
+        floatx16_t acc = {};
+#pragma unroll
+        for (uint32_t m = 0; m < 2; ++m) {
+          const uint4 a0 = As[u][col][4 * m + 2 * h], a1 = As[u][col][4 * m + 2 * h + 1];
+          const intx8_t av = {(int)a0.x, (int)a0.y, (int)a0.z, (int)a0.w,
+                              (int)a1.x, (int)a1.y, (int)a1.z, (int)a1.w};
+          acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(av, qf[m], acc, 0, 0, 0, 0, 0, 0);
+        }
+        // element 4g + e is row 8g + 4h + e: scaled by its centroid's s_c
+        const bool full = tile * 32 + 32 <= n_c;  // uniform
+        float mx = -INFINITY;
+#pragma unroll
+        for (uint32_t g = 0; g < 4; ++g) {
+          const float4 sc = *reinterpret_cast<const float4 *>(&Ss[u][8 * g + 4 * h]);
+          const float sv[4] = {sc.x, sc.y, sc.z, sc.w};
+#pragma unroll
+          for (uint32_t e = 0; e < 4; ++e)
+            if (full || tile * 32 + 8 * g + 4 * h + e < n_c) mx = vmax(mx, acc[4 * g + e] * sv[e]);
+        }
+        mxb[u] = vmax(mx, __shfl_xor(mx, 32)) * st;
+      }
+    }
+    if (h == 0 && vq && col < t_max) {
+      float *row = bmax + ((size_t)q * t_max + col) * n_tiles;
+#pragma unroll
+      for (uint32_t u = 0; u < kBmStage; ++u)
+        if (t0 + u < te) row[t0 + u] = mxb[u];
+    }
+  }
+}
+
 // Pass 2: one wave per query-token row.  The tile maxima of pass 1 are
 // approximate (bf16 inputs, FP32 accumulation in the matrix unit's order);
 // |approx - exact| <= eps for every centroid, with
@@ -2215,6 +2335,9 @@ __global__ __launch_bounds__(512) void k_ct_bf16_blockmax(
 // scored exactly -- one sequential FP32 FMA chain over d, on FP32 inputs
 // (exact_f32) or bf16-rounded inputs (the BF16 mode's definition) -- and the
 // top-P by (score desc, id asc) is kept.  Output S' = score + 1.
+// The FP8 probe calls the same kernel with margin 0 and P_thr = the fixed
+// number of tiles it rescans (every tile reaching the P_thr-th largest FP8
+// maximum), without the bf16 pre-check (precheck = 0).
 // A wave's sorted list of up to 64 J (score, id) pairs, best first by (score
 // desc, id asc): element e = 64 j + lane sits in register j of lane `lane`.
 // Only elements e < P are meaningful.
@@ -2280,8 +2403,8 @@ template <bool kF32, int J>
 __global__ __launch_bounds__(256) void k_ct_select(
     const float *__restrict__ Q, const uint32_t *__restrict__ row_map, uint32_t n_rows,
     const float *__restrict__ C, const uint16_t *__restrict__ Cb, uint32_t n_c, uint32_t n_tiles,
-    const float *__restrict__ bmax, uint32_t P, float margin, float c_norm_max,
-    uint32_t *__restrict__ top_ids, float *__restrict__ top_s) {
+    const float *__restrict__ bmax, uint32_t P, uint32_t P_thr, float margin, float c_norm_max,
+    int precheck, uint32_t *__restrict__ top_ids, float *__restrict__ top_s) {
   __shared__ float qs[4][kDim];
   __shared__ float qsb[4][kDim];  // bf16-rounded query (FP32 mode's pre-check)
   const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -2298,8 +2421,9 @@ __global__ __launch_bounds__(256) void k_ct_select(
 #pragma unroll
   for (uint32_t off = 32; off > 0; off >>= 1) nn += __shfl_xor(nn, off);
   wave_lds_sync();
-  // A_P: the P-th largest approximate tile maximum
-  const uint32_t P1 = min(P, n_tiles);
+  // A_P: the P_thr-th largest approximate tile maximum (P_thr = P with an
+  // error margin; a fixed rank P_thr > P without one)
+  const uint32_t P1 = min(P_thr, n_tiles);
   float ls[J];
   uint32_t li[J];
 #pragma unroll
@@ -2333,7 +2457,7 @@ __global__ __launch_bounds__(256) void k_ct_select(
       const bool v = tile != 0xFFFFFFFFu && c < n_c;
       float a = 0.0f;
       bool keep = v;
-      if (kF32 && v && P1 == P) {
+      if (kF32 && v && precheck && P1 == P) {
         // bf16 pre-check from the 256-byte bf16 row: this approximation is
         // within the same eps of the exact score as the tile maxima, so a
         // centroid of the exact top-P has it >= thr (exact >= tau >= A_P -
@@ -2756,13 +2880,13 @@ hipError_t launch_centroid_topk_bf16(const float *Q, uint32_t n_q, uint32_t t_ma
   // (2u + u^2, u = 2^-8) plus two 128-term FP32 accumulations; bf16 inputs =
   // accumulation order only (products of bf16 values are exact in FP32), with
   // a wide allowance for the matrix unit's internal summation
-  const float margin = exact_f32 ? 0.008f : 1e-4f;
+  const float margin = exact_f32 ? kBf16Margin : 1e-4f;
   const float cmax = exact_f32 ? ix.c_norm_max : ix.c_norm_max * 1.004f;
   const dim3 g((n_rows + 3) / 4);
   // list registers per lane: 64 elements each
 #define MVR_CT_SELECT(F32, J)                                                                            \
   k_ct_select<F32, J><<<g, 256, 0, s>>>(Q, row_map, n_rows, ix.C, ix.C_bf16, ix.n_c, n_tiles, bmax, P,  \
-                                        margin, cmax, top_ids, top_s)
+                                        P, margin, cmax, 1, top_ids, top_s)
   if (exact_f32) {
     if (P <= 64) MVR_CT_SELECT(true, 1);
     else if (P <= 128) MVR_CT_SELECT(true, 2);
@@ -2772,6 +2896,34 @@ hipError_t launch_centroid_topk_bf16(const float *Q, uint32_t n_q, uint32_t t_ma
     else if (P <= 128) MVR_CT_SELECT(false, 2);
     else MVR_CT_SELECT(false, 4);
   }
+#undef MVR_CT_SELECT
+  return hipGetLastError();
+}
+
+hipError_t launch_centroid_topk_fp8(const float *Q, uint32_t n_q, uint32_t t_max, const uint32_t *row_map,
+                                    uint32_t n_rows, const DevIndex &ix, uint32_t P, float *bmax,
+                                    uint32_t *top_ids, float *top_s, hipStream_t s) {
+  if (n_q == 0 || n_rows == 0) return hipSuccess;
+  if (!ix.C_fp8 || !ix.C_fp8_scale || P == 0 || P > kMaxProbe) return hipErrorInvalidValue;
+  const uint32_t n_tiles = (ix.n_c + 31) / 32;
+  const uint32_t qblocks = (n_q + kBmWaves - 1) / kBmWaves;
+  const uint32_t n_split = std::max(1u, std::min(n_tiles, (1024u + qblocks - 1) / qblocks));
+  const uint32_t tps = (n_tiles + n_split - 1) / n_split;
+  dim3 grid(qblocks, (n_tiles + tps - 1) / tps);
+  k_ct_fp8_blockmax<<<grid, 512, 0, s>>>(Q, t_max, n_q, ix.C_fp8, ix.C_fp8_scale, ix.n_c, n_tiles, tps, bmax);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  // exact FP32 scores of every centroid in the kFp8ProbeTiles * P tiles with
+  // the highest FP8 maxima (no error margin: the FP8 probe is approximate)
+  // (the wave lists hold at most 256 tiles)
+  const uint32_t P_thr = std::min(std::min(n_tiles, kFp8ProbeTiles * P), std::max(P, 256u));
+  const dim3 g((n_rows + 3) / 4);
+#define MVR_CT_SELECT(J)                                                                                   \
+  k_ct_select<true, J><<<g, 256, 0, s>>>(Q, row_map, n_rows, ix.C, ix.C_bf16, ix.n_c, n_tiles, bmax, P, \
+                                         P_thr, 0.0f, ix.c_norm_max, 0, top_ids, top_s)
+  if (P_thr <= 64 && P <= 64) MVR_CT_SELECT(1);
+  else if (P_thr <= 128 && P <= 128) MVR_CT_SELECT(2);
+  else MVR_CT_SELECT(4);
 #undef MVR_CT_SELECT
   return hipGetLastError();
 }

@@ -364,6 +364,52 @@ def fp8_scaled(x: np.ndarray, axis=-1):
     return e4m3_round((x * inv).astype(np.float32)), (amax / np.float32(448.0)).astype(np.float32)
 
 
+def fp8_tile_maxima(Q: np.ndarray, C_: np.ndarray) -> np.ndarray:
+    """FP8 centroid tile maxima as the FP8 mode defines them (no counterpart
+    in the reference, whose centroid scores are FP32 GEMMs): query tokens and
+    centroids encoded per row (fp8_scaled), score(t, c) = s_t s_c q8.c8 in
+    float64, maximum over each tile of 32 consecutive centroids.  Returns
+    [rows, ceil(n_c / 32)] float64."""
+    rows = np.asarray(Q, np.float32).reshape(-1, np.shape(C_)[1])
+    q8, sq = fp8_scaled(rows)
+    c8, sc = fp8_scaled(C_)
+    S = (q8.astype(np.float64) @ (c8.astype(np.float64) * sc.astype(np.float64)).T) * sq.astype(np.float64)
+    n_c = S.shape[1]
+    n_t = (n_c + 31) // 32
+    pad = np.full((S.shape[0], n_t * 32 - n_c), -np.inf)
+    return np.concatenate([S, pad], 1).reshape(S.shape[0], n_t, 32).max(2)
+
+
+def centroid_topk_fp8(Q: np.ndarray, C_: np.ndarray, P: int, probe_tiles: int = 2, rel: float = 1e-5):
+    """The FP8 mode's probe: the exact FP32 top-P (centroid_topk's chain and
+    order) among the centroids of the P' = min(n_tiles, probe_tiles P, 256)
+    tiles with the highest FP8 maxima (every tile reaching the P'-th largest).
+    Returns (ids, scores, ambiguous): ambiguous rows hold a tile maximum
+    within `rel` of the P'-th largest on either side of it, where FP32
+    summation in the matrix unit can order the tiles differently."""
+    C_ = np.asarray(C_, np.float32)
+    rows = np.asarray(Q, np.float32).reshape(-1, C_.shape[1])
+    tm = fp8_tile_maxima(rows, C_)
+    n_t = tm.shape[1]
+    Pt = min(n_t, probe_tiles * P, max(P, 256))
+    ids = np.empty((rows.shape[0], P), np.uint32)
+    s = np.empty((rows.shape[0], P), np.float32)
+    amb = np.zeros(rows.shape[0], bool)
+    for r in range(rows.shape[0]):
+        a = np.sort(tm[r])[::-1][Pt - 1]
+        sel = np.nonzero(tm[r] >= a)[0]
+        near = np.abs(tm[r] - a) <= rel * max(abs(a), 1e-30)
+        amb[r] = Pt < n_t and near.sum() > 1
+        cand = (sel[:, None] * 32 + np.arange(32)[None, :]).ravel()
+        cand = cand[cand < C_.shape[0]]
+        ci, cs = centroid_topk(rows[r:r + 1], C_[cand], min(P, cand.size))
+        ids[r, :] = 0xFFFFFFFF
+        s[r, :] = -np.inf
+        ids[r, :ci.shape[1]] = cand[ci[0]]
+        s[r, :ci.shape[1]] = cs[0]
+    return ids, s, amb
+
+
 def recall_at_k(est: np.ndarray, gnd: np.ndarray) -> float:
     """|gnd ∩ est| / k averaged over queries (script/evaluation/performance_metric.py:259-272)."""
     k = gnd.shape[1]

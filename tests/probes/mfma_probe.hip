@@ -35,6 +35,15 @@ __global__ void k_probe_fp8(const long *a, const long *b, floatx16 *d) {
   d[l] = acc;
 }
 
+// v_mfma_f32_32x32x64_f8f6f4 (e4m3 operands, 32 bytes per lane)
+typedef int intx8 __attribute__((ext_vector_type(8)));
+__global__ void k_probe_fp8_64(const intx8 *a, const intx8 *b, floatx16 *d) {
+  const uint32_t l = threadIdx.x;
+  floatx16 acc = {};
+  acc = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a[l], b[l], acc, 0, 0, 0, 0, 0, 0);
+  d[l] = acc;
+}
+
 // Packs pairs of floats to fp8 with the hardware conversion (e4m3).
 __global__ void k_probe_cvt_fp8(const float *x, uint32_t *out, uint32_t n_pairs) {
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -72,9 +81,9 @@ struct FirstError {
   }
 };
 
-// kind 0: bf16 32x32x16, 1: fp8 32x32x16, 2: bf16 16x16x32
+// kind 0: bf16 32x32x16, 1: fp8 32x32x16, 2: bf16 16x16x32, 3: fp8 32x32x64
 extern "C" int probe_mfma(int kind, const void *a, const void *b, float *d) {
-  const size_t op = kind == 1 ? 8 : 16;
+  const size_t op = kind == 1 ? 8 : kind == 3 ? 32 : 16;
   void *da = nullptr, *db = nullptr, *dd = nullptr;
   FirstError err;
   err(hipMalloc(&da, 64 * op));
@@ -87,6 +96,8 @@ extern "C" int probe_mfma(int kind, const void *a, const void *b, float *d) {
       k_probe_bf16<<<1, 64>>>((const bf16x8 *)da, (const bf16x8 *)db, (floatx16 *)dd);
     else if (kind == 2)
       k_probe_bf16_16<<<1, 64>>>((const bf16x8 *)da, (const bf16x8 *)db, (floatx16 *)dd);
+    else if (kind == 3)
+      k_probe_fp8_64<<<1, 64>>>((const intx8 *)da, (const intx8 *)db, (floatx16 *)dd);
     else
       k_probe_fp8<<<1, 64>>>((const long *)da, (const long *)db, (floatx16 *)dd);
     err(hipGetLastError());

@@ -82,3 +82,17 @@ def test_graph_on_device_buffers(c1):
         torch.cuda.synchronize()
         np.testing.assert_array_equal(oi.cpu().numpy().view(np.uint32), ref_ids)
         np.testing.assert_array_equal(os_.cpu().numpy(), ref_sc)
+
+
+def test_persistent_rejects_unsupported_settings(small):
+    """The persistent kernel has no exact-FP32 variant and selects at most 256:
+    such requests are errors, not a silent switch to per-pass launches."""
+    from mvr._capi import MvrError
+    corpus, ix, Q = small
+    g = _index(ix)
+    for kw in (dict(precision="fp32"), dict(precision="fp32_mfma", k=300),
+               dict(precision="bf16", rescore=300, k=10)):
+        k = kw.pop("k", 10)
+        with pytest.raises(MvrError):
+            g.search(Q, 4, 512, k, persistent=True, **kw)
+    g.close()

@@ -9,6 +9,9 @@ Assumed (v_mfma_f32_16x16x32_bf16, the BF16 MaxSim):
   A: lane l holds A[l % 16][8*(l // 16) + e], e = 0..7
   B: lane l holds B[8*(l // 16) + e][l % 16]
   D: lane l, element j holds D[4*(l // 16) + j][l % 16]
+Assumed (v_mfma_f32_32x32x64_f8f6f4 with e4m3 operands, the FP8 centroid
+tile maxima): lane l holds A[l % 32][32*(l // 32) + e] and
+B[32*(l // 32) + e][l % 32], e = 0..31; D as for 32x32x16.
 FP8: OCP e4m3 (bias 7, 1.0 = 0x38).
 global_load_lds (16 bytes): lane l's piece lands at the LDS destination + the
 immediate offset + 16 l, the offset also being added to the global address
@@ -145,3 +148,19 @@ def test_global_load_lds_lane_layout(probe):
     want[32:96] = src[perm]            # destination buf + 32 pieces, offset 0
     want[96:160] = src[perm + 64]      # offset 1024 bytes = 64 pieces, on both addresses
     np.testing.assert_array_equal(out, want)
+
+
+def test_fp8_32x32x64_layout(probe):
+    rng = np.random.default_rng(4)
+    A = rng.integers(-4, 5, (32, 64)).astype(np.float32)
+    B = rng.integers(-4, 5, (64, 32)).astype(np.float32)
+    enc = np.vectorize(ocp_e4m3)
+    Ae, Be = enc(A).astype(np.uint8), enc(B).astype(np.uint8)
+    a = np.zeros((64, 32), np.uint8)
+    b = np.zeros((64, 32), np.uint8)
+    for l in range(64):
+        a[l] = Ae[l % 32, 32 * (l // 32):32 * (l // 32) + 32]
+        b[l] = Be[32 * (l // 32):32 * (l // 32) + 32, l % 32]
+    d = np.zeros((64, 16), np.float32)
+    assert probe.probe_mfma(3, a.ctypes.data, b.ctypes.data, d.ctypes.data) == 0
+    np.testing.assert_array_equal(unpack_d(d), A @ B)

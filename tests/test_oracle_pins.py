@@ -8,6 +8,9 @@
   lowest id on ties, Lloyd iterations with argmin-L2 assignment, means in
   double, empty clusters kept, L2 normalisation at the end); checked against a
   float64 numpy brute force on small inputs whose assignments have clear gaps.
+* fp8_tile_maxima / centroid_topk_fp8: the FP8 probe's model; tile maxima
+  against a per-element loop over decoded e4m3 values, and the probe against
+  the exact top-P when its tiles cover every centroid.
 """
 import numpy as np
 import pytest
@@ -152,3 +155,47 @@ def test_assign_bias_matches_float64_argmin_l2():
     np.testing.assert_array_equal(got[clear], np.argmin(d2, axis=1)[clear])
     # a zero bias is the plain inner-product assignment
     np.testing.assert_array_equal(O.assign_bias(x, C_, np.zeros(30, np.float32)), O.assign_ip(x, C_))
+
+
+def test_fp8_tile_maxima_brute_force():
+    """Tile maxima of the FP8 probe against a loop over every (row, centroid)
+    with e4m3 values decoded from their bit patterns (a partial last tile)."""
+    rng = np.random.default_rng(11)
+    Q = rng.standard_normal((3, 128)).astype(np.float32)
+    C = rng.standard_normal((70, 128)).astype(np.float32)
+    tm = O.fp8_tile_maxima(Q, C)
+    assert tm.shape == (3, 3)
+
+    def enc(x):
+        amax = np.float32(np.abs(x).max())
+        inv = np.float32(448.0) / amax
+        return [float(O.e4m3_round(np.array([np.float32(v) * inv], np.float32))[0]) for v in x], \
+            float(amax / np.float32(448.0))
+    for r in range(3):
+        q8, sq = enc(Q[r])
+        best = [-np.inf] * 3
+        for c in range(70):
+            c8, sc = enc(C[c])
+            v = sq * sc * sum(a * b for a, b in zip(q8, c8))
+            best[c // 32] = max(best[c // 32], v)
+        np.testing.assert_allclose(tm[r], best, rtol=1e-12)
+
+
+def test_fp8_probe_with_every_tile_is_the_exact_topk():
+    """With P' covering every tile the FP8 probe is the exact FP32 top-P, and
+    its scores are the chain's bits; with fewer tiles each id lies in one of
+    the P' tiles of highest FP8 maximum."""
+    rng = np.random.default_rng(12)
+    C = rng.standard_normal((200, 128)).astype(np.float32)
+    C /= np.linalg.norm(C, axis=1, keepdims=True)
+    Q = rng.standard_normal((8, 128)).astype(np.float32)
+    ids, s, amb = O.centroid_topk_fp8(Q, C, 4, probe_tiles=2)      # 7 tiles <= 8: all
+    ei, es = O.centroid_topk(Q, C, 4)
+    np.testing.assert_array_equal(ids, ei)
+    np.testing.assert_array_equal(s, es)
+    assert not amb.any()
+    ids, s, _ = O.centroid_topk_fp8(Q, C, 2, probe_tiles=1)         # 2 of 7 tiles
+    tm = O.fp8_tile_maxima(Q, C)
+    for r in range(8):
+        top2 = set(np.argsort(-tm[r])[:2].tolist())
+        assert {int(i) // 32 for i in ids[r]} <= top2
