@@ -66,6 +66,9 @@ CONFIGS = {
     # a recall-QPS frontier (see bench_northstar)
     "northstar": dict(n_doc=1 << 20, n_comp=65536, n_c=65536, lognormal=True, n_q=4096, P=8, R=1000, k=100,
                       kmeans_sample=1 << 21, n_iter=4, n_train=16384),
+    # the groundtruth module (BruteForceProgressive): all-pairs MaxSim of 1,024
+    # queries over 65,536 docs of U{16..128} tokens, exact FMA chains and FP32 MFMA
+    "exhaustive": dict(n_doc=65536, n_comp=4096, n_q=1024, exhaustive=True),
 }
 
 
@@ -117,6 +120,9 @@ def main():
         return
     if cfg.get("plaid"):
         bench_plaid(torch, args, cfg)
+        return
+    if cfg.get("exhaustive"):
+        bench_exhaustive(torch, args, cfg)
         return
     t_setup = time.time()
     if cfg["lognormal"]:
@@ -537,6 +543,51 @@ def bench_learned(torch, args, cfg):
                                    f"rerank depth {cfg['R']}; MLP 1024-512-256-6, Adam lr 1e-3, batch 1024, "
                                    f"{args.epochs} epochs, fallback nprobe 16 below softmax 0.6"},
             **out}
+    print(json.dumps(line), flush=True)
+
+
+def bench_exhaustive(torch, args, cfg):
+    """All-pairs MaxSim through the groundtruth module's C ABI
+    (mvr_maxsim_scores_host_ex, host arrays as BruteForceProgressive passes
+    them): seconds per call, whole-call TFLOP/s (2 x 128 flops per query
+    token x doc token, PCIe copies included) for the exact FMA chains and for
+    FP32 MFMA, and the largest score difference between the two."""
+    from mvr import synth
+    from mvr._capi import lib
+    doclens = synth.doclens_uniform(args.seed, cfg["n_doc"], 16, 128)
+    n_tok = int(doclens.sum())
+    dev = torch.device("cuda")
+    centers = synth.gen_centers(torch, args.seed, cfg["n_comp"])
+    X = torch.empty((n_tok, 128), dtype=torch.float32, device=dev)
+    synth.synth_tokens(torch, args.seed, synth.STREAM_TOKCOMP, synth.STREAM_TOKNOISE, centers, n_tok, out=X)
+    Q = synth.make_queries(torch, args.seed, centers, cfg["n_q"])
+    Xh, Qh = X.cpu().numpy(), Q.cpu().numpy()
+    del X
+    lens = doclens.astype(np.uint32)
+    n_q, T = Qh.shape[0], Qh.shape[1]
+    flop = 2.0 * 128 * T * n_q * n_tok
+    out, res = {}, {}
+    for name, prec in (("fp32", 0), ("fp32_mfma", 3)):
+        sc = np.zeros((n_q, lens.size), np.float32)
+        args_c = (Qh.ctypes.data, n_q, T, 128, Xh.ctypes.data, lens.ctypes.data, lens.size, sc.ctypes.data, 0, prec)
+        assert lib().mvr_maxsim_scores_host_ex(*args_c) == 0, lib().mvr_maxsim_last_error()  # warm-up
+        ts = []
+        for _ in range(max(1, min(args.steps, 3))):
+            t0 = time.perf_counter()
+            assert lib().mvr_maxsim_scores_host_ex(*args_c) == 0, lib().mvr_maxsim_last_error()
+            ts.append(time.perf_counter() - t0)
+        sec = float(np.median(ts))
+        out[name] = sc
+        res[name] = {"seconds_per_call": round(sec, 4), "tflops_whole_call": round(flop / sec / 1e12, 2),
+                     "frac_of_fp32_mfma_peak": round(flop / sec / 1e12 / 157.3, 4)}
+        log(f"[bench] exhaustive {name}: {sec:.3f} s per call, {flop / sec / 1e12:.1f} TFLOP/s")
+    d = np.abs(out["fp32_mfma"] - out["fp32"]) / np.maximum(np.abs(out["fp32"]), 1e-6)
+    line = {"metric": "exhaustive MaxSim TFLOP/s (groundtruth module)", "value": res["fp32_mfma"]["tflops_whole_call"],
+            "unit": "TFLOP/s", "n_gpus": 1, "higher_is_better": True, "dtype": "fp32",
+            "data": "synthetic (seeded mixture tokens and queries)",
+            "config": {"workload": f"exhaustive: {n_q} queries x {T} tokens against {lens.size} docs / {n_tok} tokens "
+                                   f"(U{{16..128}})"},
+            "by_precision": res, "max_rel_score_diff_mfma_vs_fp32": float(d.max())}
     print(json.dumps(line), flush=True)
 
 

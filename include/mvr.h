@@ -264,6 +264,13 @@ int mvr_index_set_probe(mvr_index *ix, mvr_mlp *mlp, float conf, uint32_t fallba
  * errors are described by mvr_maxsim_last_error(). */
 int mvr_maxsim_scores_host(const float *Q, uint32_t n_q, uint32_t T, uint32_t dim, const float *X,
                            const uint32_t *lens, uint32_t n_item, float *scores, int device);
+/* The same with precision MVR_FP32 (as above) or MVR_FP32_MFMA: the dot
+ * products on FP32 MFMA, summed in the matrix unit's order (within 1e-5
+ * relative of MVR_FP32), as the reference's cuBLAS SGEMM leaves its order to
+ * the library (GPUComputeScore.hpp:196-246). */
+int mvr_maxsim_scores_host_ex(const float *Q, uint32_t n_q, uint32_t T, uint32_t dim, const float *X,
+                              const uint32_t *lens, uint32_t n_item, float *scores, int device,
+                              uint32_t precision);
 const char *mvr_maxsim_last_error(void);
 
 /* ---- synthetic corpus and index build on the device (bench and tests) ---- */

@@ -1,6 +1,7 @@
 // BruteForceProgressive: pybind11 module with the reference's Python surface
 // (src/app/BruteForceProgressive.cpp:16-33, src/impl/BruteForceProgressive.hpp):
-//   DocRetrieval(query_l [n_q, T, dim], vec_dim)
+//   DocRetrieval(query_l [n_q, T, dim], vec_dim, precision="fp32")  ("fp32_mfma": the
+//     dots on FP32 MFMA, within 1e-5 relative; not in the reference's signature)
 //   .computeScore(item_l: list of [L_i, dim], itemID_l)   all-pairs MaxSim on the GPU
 //   .searchKNN(topk) -> (scores [n_q, topk], ids [n_q, topk])
 //   .finish_compute()
@@ -36,7 +37,7 @@ namespace {
 
 struct MaxSimLib {
   int (*scores)(const float *, uint32_t, uint32_t, uint32_t, const float *, const uint32_t *, uint32_t, float *,
-                int) = nullptr;
+                int, uint32_t) = nullptr;
   const char *(*last_error)() = nullptr;
   bool tried = false;
 };
@@ -55,7 +56,7 @@ MaxSimLib &maxsim_lib() {
   dir = dir.substr(0, dir.find_last_of('/') + 1);
   void *h = dlopen((dir + "libmvr.so").c_str(), RTLD_NOW | RTLD_LOCAL);
   if (!h) return g;
-  g.scores = reinterpret_cast<decltype(g.scores)>(dlsym(h, "mvr_maxsim_scores_host"));
+  g.scores = reinterpret_cast<decltype(g.scores)>(dlsym(h, "mvr_maxsim_scores_host_ex"));
   g.last_error = reinterpret_cast<const char *(*)()>(dlsym(h, "mvr_maxsim_last_error"));
   return g;
 }
@@ -76,11 +77,15 @@ void topk_ascending(std::vector<Cand> &cand, uint32_t k, Cand *out) {
 class DocRetrieval {
  public:
   uint32_t n_query_ = 0, query_n_vecs_ = 0, vec_dim_ = 0, n_item_ = 0;
+  uint32_t precision_ = 0;  // mvr_precision: MVR_FP32 (0) or MVR_FP32_MFMA (3)
   std::vector<float> query_;
   std::vector<float> score_;        // [n_query][n_item]
   std::vector<uint32_t> external_;  // [n_item]
 
-  DocRetrieval(const arr_f32 &query_l, uint32_t vec_dim) : vec_dim_(vec_dim) {
+  DocRetrieval(const arr_f32 &query_l, uint32_t vec_dim, const std::string &precision) : vec_dim_(vec_dim) {
+    if (precision == "fp32") precision_ = 0;
+    else if (precision == "fp32_mfma") precision_ = 3;
+    else throw std::invalid_argument("precision must be \"fp32\" or \"fp32_mfma\"");
     if (query_l.ndim() != 3) throw std::runtime_error("the vector set list dimension should be 3");
     if ((uint32_t)query_l.shape(2) != vec_dim_)
       throw std::runtime_error("the embedding dimension should equal to the pre-assignment");
@@ -109,14 +114,15 @@ class DocRetrieval {
       o += a.size();
     }
     MaxSimLib &L = maxsim_lib();
-    if (!L.scores) throw std::runtime_error("libmvr.so (mvr_maxsim_scores_host) is not available next to this module");
+    if (!L.scores) throw std::runtime_error("libmvr.so (mvr_maxsim_scores_host_ex) is not available next to this module");
     n_item_ = (uint32_t)item_l.size();
     score_.assign((size_t)n_query_ * n_item_, 0.0f);
     external_ = itemID_l;
     int st;
     {
       py::gil_scoped_release nogil;
-      st = L.scores(query_.data(), n_query_, query_n_vecs_, vec_dim_, X.data(), lens.data(), n_item_, score_.data(), 0);
+      st = L.scores(query_.data(), n_query_, query_n_vecs_, vec_dim_, X.data(), lens.data(), n_item_, score_.data(), 0,
+                    precision_);
     }
     if (st != 0) throw std::runtime_error(std::string("computeScore: ") + L.last_error());
   }
@@ -189,7 +195,8 @@ class DocRetrieval {
 PYBIND11_MODULE(BruteForceProgressive, m) {
   py::class_<DocRetrieval>(m, "DocRetrieval",
                            "Exhaustive MaxSim of a query set against batches of documents (groundtruth).")
-      .def(py::init<const arr_f32 &, uint32_t>(), py::arg("query_l"), py::arg("vec_dim"))
+      .def(py::init<const arr_f32 &, uint32_t, const std::string &>(), py::arg("query_l"), py::arg("vec_dim"),
+           py::arg("precision") = "fp32")
       .def("computeScore", &DocRetrieval::computeScore, py::arg("item_l"), py::arg("itemID_l"))
       .def("searchKNN", &DocRetrieval::searchKNN, py::arg("topk"))
       .def("finish_compute", &DocRetrieval::finish_compute)

@@ -146,6 +146,128 @@ __global__ __launch_bounds__(256) void k_exh_maxsim(const float *__restrict__ Q,
   }
 }
 
+// The same scores with the dot products on FP32 MFMA (v_mfma_f32_32x32x2_f32):
+// dots summed in the matrix unit's order instead of one FMA chain (within
+// about 1e-6 relative), the max and the sum over query tokens unchanged.
+// Wave w of the workgroup owns query q0 + w: its 32 tokens are the B operand
+// (columns), held in 64 registers per lane for the whole block; the item
+// tokens are the A operand, 32 rows per MFMA tile and four tiles per 128-token
+// LDS tile.  In K-chunk c (dims 32c .. +31) MFMA j takes dim 32c + 16h + j
+// from lane half h in both operands.  Lane (query token t = l & 31, h) ends a
+// tile with the scores of item-token rows 8g + 4h + e (g, e = 0..3), folded
+// into the per-(row, item) LDS maxima by runs of the same item.
+typedef float x_floatx16 __attribute__((ext_vector_type(16)));
+// LDS row of the MFMA kernel: the two lane halves read rows 16 apart, which
+// 16 * 130 = 32 (mod 64) puts on disjoint banks (the VALU kernel's 132 would not)
+constexpr uint32_t kXpM = kXt + 2;
+
+__global__ __launch_bounds__(256) void k_exh_maxsim_mfma(const float *__restrict__ Q, uint32_t n_q, uint32_t T,
+                                                         const float *__restrict__ X,
+                                                         const uint4 *__restrict__ blocks,
+                                                         const uint64_t *__restrict__ item_off, uint32_t n_item,
+                                                         float *__restrict__ scores, uint64_t score_stride) {
+  __shared__ float Bs[kXk][kXpM];
+  __shared__ uint32_t M[kXt][kXItems];
+  __shared__ uint32_t slot[kXt];
+  __shared__ uint64_t ioff[kXItems + 1];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, t = lane & 31, h = lane >> 5;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const uint32_t q0 = blockIdx.y * 4, q = q0 + wave;
+  const uint4 blk = blocks[blockIdx.x];
+  const uint64_t tok0 = item_off[blk.z];
+  const uint32_t ntok = blk.y, item0 = blk.z, nit = blk.w;
+  for (uint32_t e = tid; e < kXt * kXItems; e += 256) (&M[0][0])[e] = 0x007FFFFFu;  // order bits of -inf
+  for (uint32_t e = tid; e <= nit; e += 256) ioff[e] = item_off[item0 + e] - tok0;
+  // the wave's query tokens (rows past n_q read as zero; their scores are not stored)
+  float qv[4][16];
+  {
+    const float *qr = Q + ((size_t)(q < n_q ? q : 0) * kMaxQTok + t) * kDim + 16 * h;
+#pragma unroll
+    for (uint32_t c = 0; c < 4; ++c)
+#pragma unroll
+      for (uint32_t j4 = 0; j4 < 4; ++j4) {
+        const float4 v = reinterpret_cast<const float4 *>(qr + 32 * c)[j4];
+        qv[c][4 * j4] = v.x; qv[c][4 * j4 + 1] = v.y; qv[c][4 * j4 + 2] = v.z; qv[c][4 * j4 + 3] = v.w;
+      }
+  }
+  __syncthreads();
+  const float4 *X4 = reinterpret_cast<const float4 *>(X);
+  // chunk (tile, c) staged through registers one chunk ahead: its global
+  // loads are in flight during the previous chunk's MFMAs
+  float4 pf[4];
+  auto load_chunk = [&](uint32_t tile0, uint32_t c) {
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) {
+      const uint32_t e = tid + 256 * u, r = e >> 3, d4 = e & 7, tk = tile0 + r;
+      pf[u] = tk < ntok ? X4[(tok0 + tk) * (kDim / 4) + 8 * c + d4] : make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+  };
+  load_chunk(0, 0);
+  for (uint32_t tile0 = 0; tile0 < ntok; tile0 += kXt) {
+    if (tid < kXt) {
+      const uint32_t tk = tile0 + tid;
+      uint32_t sl = 0xFFFFFFFFu;
+      if (tk < ntok) {
+        sl = 0;
+        while (sl + 1 < nit && ioff[sl + 1] <= tk) ++sl;
+      }
+      slot[tid] = sl;
+    }
+    x_floatx16 acc[4];
+#pragma unroll
+    for (uint32_t u = 0; u < 4; ++u) acc[u] = x_floatx16{};
+#pragma unroll
+    for (uint32_t c = 0; c < 4; ++c) {
+      __syncthreads();  // the previous chunk's readers are done
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+        const uint32_t e = tid + 256 * u, r = e >> 3, d4 = e & 7;
+        Bs[4 * d4 + 0][r] = pf[u].x; Bs[4 * d4 + 1][r] = pf[u].y; Bs[4 * d4 + 2][r] = pf[u].z; Bs[4 * d4 + 3][r] = pf[u].w;
+      }
+      __syncthreads();
+      if (c < 3) load_chunk(tile0, c + 1);
+      else if (tile0 + kXt < ntok) load_chunk(tile0 + kXt, 0);
+#pragma unroll
+      for (uint32_t j = 0; j < 16; ++j) {
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u)
+          acc[u] = __builtin_amdgcn_mfma_f32_32x32x2f32(Bs[16 * h + j][32 * u + t], qv[c][j], acc[u], 0, 0, 0);
+      }
+    }
+    // fold: runs of consecutive rows of the same item, one atomic per run
+    if (t < T) {
+      uint32_t *Mr = &M[wave * kMaxQTok + t][0];
+#pragma unroll
+      for (uint32_t u = 0; u < 4; ++u) {
+#pragma unroll
+        for (uint32_t g = 0; g < 4; ++g) {
+          uint32_t cur = 0xFFFFFFFFu;
+          float m = -INFINITY;
+#pragma unroll
+          for (uint32_t e = 0; e < 4; ++e) {
+            const uint32_t sl = slot[32 * u + 8 * g + 4 * h + e];
+            if (sl != cur) {
+              if (cur != 0xFFFFFFFFu) atomicMax(&Mr[cur], x_f2ord(m));
+              cur = sl;
+              m = -INFINITY;
+            }
+            m = fmaxf(m, acc[u][4 * g + e]);
+          }
+          if (cur != 0xFFFFFFFFu) atomicMax(&Mr[cur], x_f2ord(m));
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (uint32_t e = tid; e < 4 * nit; e += 256) {
+    const uint32_t qq = e / nit, it = e % nit, qo = q0 + qq;
+    if (qo >= n_q) continue;
+    float sc = 0.0f;
+    for (uint32_t tt = 0; tt < T; ++tt) sc += x_ord2f(M[qq * kMaxQTok + tt][it]);
+    scores[(size_t)qo * score_stride + item0 + it] = sc;
+  }
+}
+
 thread_local std::string g_x_err;
 
 int x_fail(int code, const char *fmt, ...) {
@@ -169,6 +291,14 @@ const char *mvr_maxsim_last_error(void) { return g_x_err.c_str(); }
 
 int mvr_maxsim_scores_host(const float *Q, uint32_t n_q, uint32_t T, uint32_t dim, const float *X,
                            const uint32_t *lens, uint32_t n_item, float *scores, int device) {
+  return mvr_maxsim_scores_host_ex(Q, n_q, T, dim, X, lens, n_item, scores, device, MVR_FP32);
+}
+
+int mvr_maxsim_scores_host_ex(const float *Q, uint32_t n_q, uint32_t T, uint32_t dim, const float *X,
+                              const uint32_t *lens, uint32_t n_item, float *scores, int device,
+                              uint32_t precision) {
+  if (precision != MVR_FP32 && precision != MVR_FP32_MFMA)
+    return x_fail(MVR_ERR_UNSUPPORTED, "precision %u: MVR_FP32 or MVR_FP32_MFMA", precision);
   if (dim != kDim) return x_fail(MVR_ERR_UNSUPPORTED, "dim must be 128");
   if (T == 0 || T > kMaxQTok) return x_fail(MVR_ERR_UNSUPPORTED, "query length %u must be 1..32", T);
   if (n_q == 0 || n_item == 0) return MVR_OK;
@@ -254,8 +384,12 @@ int mvr_maxsim_scores_host(const float *Q, uint32_t n_q, uint32_t T, uint32_t di
       const uint32_t gy = std::min<uint32_t>(65535, (n_q + 3) / 4 - y0);
       // query groups [y0, y0 + gy): shift Q and the score rows
       const uint32_t qb = y0 * 4, nqb = std::min<uint32_t>(n_q - qb, gy * 4);
-      k_exh_maxsim<<<dim3((uint32_t)cb.size(), gy), 256, 0, s>>>(dQ + (size_t)qb * kMaxQTok * kDim, nqb, T, dX,
-                                                                 dB, dOff, cn, dS + (size_t)qb * cn, cn);
+      if (precision == MVR_FP32_MFMA)
+        k_exh_maxsim_mfma<<<dim3((uint32_t)cb.size(), gy), 256, 0, s>>>(
+            dQ + (size_t)qb * kMaxQTok * kDim, nqb, T, dX, dB, dOff, cn, dS + (size_t)qb * cn, cn);
+      else
+        k_exh_maxsim<<<dim3((uint32_t)cb.size(), gy), 256, 0, s>>>(dQ + (size_t)qb * kMaxQTok * kDim, nqb, T, dX,
+                                                                   dB, dOff, cn, dS + (size_t)qb * cn, cn);
       if ((e = hipGetLastError()) != hipSuccess) return dev_fail("launch");
     }
     // cb / coff live until the stream drains (the copies above read them)

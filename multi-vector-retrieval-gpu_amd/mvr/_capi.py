@@ -120,6 +120,8 @@ SIGNATURES = {
                                     C.c_void_p, C.c_int, C.c_void_p, C.c_int]),
     "mvr_maxsim_scores_host": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                          C.c_uint32, C.c_void_p, C.c_int]),
+    "mvr_maxsim_scores_host_ex": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
+                                         C.c_uint32, C.c_void_p, C.c_int, C.c_uint32]),
     "mvr_maxsim_last_error": (C.c_char_p, []),
     # include/mvr_plaid.h
     "mvr_plaid_last_error": (C.c_char_p, []),
