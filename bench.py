@@ -237,6 +237,8 @@ def main():
         ref = others.get("fp32") or (run if args.precision == "fp32" else None)
         mref = others.get("fp32_mfma") or (run if args.precision == "fp32_mfma" else None)
         cpu = cpu_baseline(torch, corpus, Q, P, R, k, args.cpu_queries, ref, mref)
+        if cpu is not None and corpus.X is not None:
+            cpu["plaid_cpu_path"] = plaid_cpu_baseline(torch, corpus, Q, k, args.cpu_queries)
     rec_main = recall["by_pipeline"][args.precision]["search_vs_fp64"] if recall else None
     line = {
         "metric": BASELINE_METRIC,
@@ -350,6 +352,33 @@ def candidate_counts(ix, Q, P, R, k, q_nt, q_np, n):
     return {"queries": int(n), "mean": round(float(c.mean()), 1), "p50": int(np.percentile(c, 50)),
             "p99": int(np.percentile(c, 99)), "max": int(c.max()), "cap": 8192,
             "cap_binding_frac": round(float((c > 8192).mean()), 4)}
+
+
+def plaid_cpu_baseline(torch, corpus, Q, k, n):
+    """The reference's CPU search path (PLAID's CPU branch, the restatement in
+    oracle/plaid_oracle.c) timed on the same corpus and queries: a PLAID index
+    of the corpus (float16 centroids, binarized residuals, pid IVF), the
+    searcher's defaults for k, OpenMP over the first n queries on every host
+    core.  A baseline of the reference's CPU speed, not a check of this
+    pipeline's answers (PLAID ranks with its own codec)."""
+    from mvr import plaid, synth
+    from oracle import plaid as OP
+    pa = synth.plaid_from_corpus(torch, corpus, log=lambda m: log("[bench]", m))
+    ncells, thr, ndocs = plaid.searcher_defaults(k)
+    cores = len(os.sched_getaffinity(0))
+    oix = OP.PlaidIndex(**pa.index_kwargs())
+    Qh = Q[:n].cpu().numpy()
+    oix.search(Qh[:16], k, ncells, thr, ndocs, n_threads=cores)
+    t0 = time.time()
+    oix.search(Qh, k, ncells, thr, ndocs, n_threads=cores)
+    wall = time.time() - t0
+    log(f"[bench] plaid cpu path: {n / wall:.1f} QPS on {cores} cores (k {k}, ncells {ncells}, ndocs {ndocs})")
+    return {"value": round(n / wall, 2), "unit": "queries/s", "cores": cores, "kind": "port",
+            "sample": f"first {n} queries, PLAID index of the same corpus, searcher defaults for k={k} "
+                      f"(ncells {ncells}, threshold {thr}, ndocs {ndocs})",
+            "build": "oracle/plaid_oracle.c: the restatement of PLAID's CPU search (filter_pids, "
+                     "decompress_residuals, segmented_maxsim), gcc -O3 -mavx2 -mfma -fopenmp",
+            "seconds": round(wall, 2)}
 
 
 def probe_agreement(ix, Q, P, R, k, n):
