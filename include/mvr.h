@@ -61,7 +61,12 @@ typedef struct {
   uint64_t n_tok;
   uint32_t n_centroid;
   uint32_t dim;              /* must be 128 */
-  uint32_t n_bit;            /* must be 2 */
+  uint32_t n_bit;            /* 1, 2, 4 or 8 bits per residual value; other
+                              * than 2: device-resident, searched and reranked
+                              * with MVR_FP32 only (the MFMA, persistent and
+                              * host-staging kernels decode 2-bit codes; the
+                              * reference's decode table for 8/b values per
+                              * byte is DataStructPP.hpp:184-200) */
   const uint32_t *doclens;   /* [n_doc]                         item_n_vec_l.npy */
   const float *centroids;    /* [n_centroid][dim]               centroid_l.npy */
   const uint32_t *codes;     /* [n_tok]                         vq_code_l.npy */
@@ -131,7 +136,8 @@ const char *mvr_last_error(void);
 
 int mvr_index_create(const mvr_index_desc *desc, int device, mvr_index **out);
 /* Loads centroid_l.npy, vq_code_l.npy, weight_l.npy, residual_code_l.npy,
- * item_n_vec_l.npy and ivf.index from index_dir (IGPGPUPP.cpp:33-73). */
+ * item_n_vec_l.npy and ivf.index from index_dir (IGPGPUPP.cpp:33-73).  The
+ * bit width comes from weight_l.npy's last dimension (2^n_bit). */
 int mvr_index_load(const char *index_dir, int device, mvr_index **out);
 
 /* Index flags for the _ex variants.

@@ -355,7 +355,10 @@ int index_create(const mvr_index_desc *desc, int device, uint32_t flags, const C
   *out = nullptr;
   if (flags & ~MVR_INDEX_RESIDUAL_ON_HOST) return fail(MVR_ERR_INVALID, "unknown index flags 0x%x", flags);
   if (desc->dim != kDim) return fail(MVR_ERR_UNSUPPORTED, "dim %u unsupported (kernels need 128)", desc->dim);
-  if (desc->n_bit != 2) return fail(MVR_ERR_UNSUPPORTED, "n_bit %u unsupported (kernels need 2)", desc->n_bit);
+  if (desc->n_bit != 1 && desc->n_bit != 2 && desc->n_bit != 4 && desc->n_bit != 8)
+    return fail(MVR_ERR_INVALID, "n_bit %u must be 1, 2, 4 or 8", desc->n_bit);
+  if (desc->n_bit != 2 && (flags & MVR_INDEX_RESIDUAL_ON_HOST))
+    return fail(MVR_ERR_UNSUPPORTED, "host-resident residuals need 2-bit codes (n_bit %u)", desc->n_bit);
   if (desc->n_doc == 0 || desc->n_centroid == 0)
     return fail(MVR_ERR_INVALID, "empty index (n_doc %u, n_centroid %u)", desc->n_doc, desc->n_centroid);
   if (!desc->doclens || !desc->centroids || (!cf && (!desc->codes || !desc->residual)) || !desc->weights ||
@@ -374,6 +377,9 @@ int index_create(const mvr_index_desc *desc, int device, uint32_t flags, const C
   d.n_tok = desc->n_tok;
   d.n_c = desc->n_centroid;
   d.n_ele = desc->ivf_n_ele;
+  d.n_bit = desc->n_bit;
+  const size_t res_bytes = (size_t)kDim * d.n_bit / 8;  // packed residual bytes per token
+  const uint32_t nq = 1u << d.n_bit;
   d.n_seg = (d.n_doc + kSeg - 1) / kSeg;
   {
     int n_cu = 0;
@@ -415,12 +421,12 @@ int index_create(const mvr_index_desc *desc, int device, uint32_t flags, const C
     if (offs[c] + sizes[c] > d.n_ele)
       return bail(fail(MVR_ERR_INVALID, "IVF list %u exceeds n_ele", c));
   }
-  std::vector<float> W(kDim * 4);
+  std::vector<float> W((size_t)kDim * nq);
   {
-    const size_t nw = desc->weights_per_dim ? kDim * 4 : 4;
+    const size_t nw = desc->weights_per_dim ? (size_t)kDim * nq : nq;
     std::vector<float> w = fetch(desc->weights, nw, od);
     for (uint32_t dd = 0; dd < kDim; ++dd)
-      for (uint32_t b = 0; b < 4; ++b) W[dd * 4 + b] = w[(desc->weights_per_dim ? dd * 4 : 0) + b];
+      for (uint32_t b = 0; b < nq; ++b) W[dd * nq + b] = w[(desc->weights_per_dim ? dd * nq : 0) + b];
   }
 
   UP(d.C, desc->centroids, (size_t)d.n_c * kDim);
@@ -466,11 +472,11 @@ int index_create(const mvr_index_desc *desc, int device, uint32_t flags, const C
     }
   } else if (cf) {
     int rc = stream_to_device(cf->codes, reinterpret_cast<uint8_t **>(&d.codes), d.n_tok * 4, d.n_c);
-    if (rc == MVR_OK) rc = stream_to_device(cf->res, &d.residual, d.n_tok * kResBytes, 0);
+    if (rc == MVR_OK) rc = stream_to_device(cf->res, &d.residual, d.n_tok * res_bytes, 0);
     if (rc != MVR_OK) return bail(rc);
   } else {
     UP(d.codes, desc->codes, d.n_tok);
-    UP(d.residual, desc->residual, d.n_tok * kResBytes);
+    UP(d.residual, desc->residual, d.n_tok * res_bytes);
   }
   UP(d.doclens, desc->doclens, d.n_doc);
   UP(d.ivf_off, desc->ivf_offsets, d.n_c);
@@ -490,7 +496,9 @@ int index_create(const mvr_index_desc *desc, int device, uint32_t flags, const C
   }
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d.C_fp8), (size_t)d.n_c * kDim);
   if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d.C_fp8_scale), (size_t)d.n_c * sizeof(float));
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void **>(&d.lut_fp8), (size_t)kResBytes * 256 * 4);
+  // the FP8 residual table decodes 2-bit codes (other widths: exact FP32 path only)
+  if (e == hipSuccess && d.n_bit == 2)
+    e = hipMalloc(reinterpret_cast<void **>(&d.lut_fp8), (size_t)kResBytes * 256 * 4);
   if (e == hipSuccess)
     e = hipMalloc(reinterpret_cast<void **>(&d.seg_off), (size_t)d.n_c * (d.n_seg + 1) * sizeof(uint32_t));
   {
@@ -564,9 +572,17 @@ int mvr_index_load_ex(const char *index_dir, int device, uint32_t flags, mvr_ind
     return fail(MVR_ERR_IO, "item_n_vec_l.npy must be uint32 [n_item]");
   const uint32_t n_c = (uint32_t)cent.shape[0], dim = (uint32_t)cent.shape[1];
   const uint64_t n_tok = codes.shape[0];
-  if (res.shape[0] != n_tok * (dim / 4))
-    return fail(MVR_ERR_IO, "residual_code_l.npy has %zu bytes, expected %llu", res.shape[0],
-                (unsigned long long)(n_tok * (dim / 4)));
+  // bits per dim from the decode table's width: weight_l.npy is [2^b] or [dim, 2^b]
+  const size_t n_w = weight.shape.back();
+  uint32_t n_bit = 0;
+  for (uint32_t b : {1u, 2u, 4u, 8u})
+    if (n_w == (1u << b)) n_bit = b;
+  if (n_bit == 0) return fail(MVR_ERR_IO, "weight_l.npy holds %zu values per dim (expected 2, 4, 16 or 256)", n_w);
+  if (weight.shape.size() == 2 && weight.shape[0] != dim)
+    return fail(MVR_ERR_IO, "weight_l.npy has %zu rows, expected dim %u", weight.shape[0], dim);
+  if (res.shape[0] != n_tok * dim * n_bit / 8)
+    return fail(MVR_ERR_IO, "residual_code_l.npy has %zu bytes, expected %llu (%u-bit codes)", res.shape[0],
+                (unsigned long long)(n_tok * dim * n_bit / 8), n_bit);
 
   // ivf.index: u32 n_centroid, u64 n_ele, u64[n_c] offsets, u32[n_c] sizes,
   // u32[n_ele] ids (src/include/struct/IVFIndex.hpp:168-183)
@@ -589,15 +605,13 @@ int mvr_index_load_ex(const char *index_dir, int device, uint32_t flags, mvr_ind
   desc.n_tok = n_tok;
   desc.n_centroid = n_c;
   desc.dim = dim;
-  desc.n_bit = 2;
+  desc.n_bit = n_bit;
   desc.doclens = lens.as<uint32_t>();
   desc.centroids = cent.as<float>();
   desc.codes = nullptr;  // from cf
   desc.residual = nullptr;
   desc.weights = weight.as<float>();
   desc.weights_per_dim = weight.shape.size() == 2 ? 1u : 0u;
-  if (desc.weights_per_dim ? (weight.shape[0] != dim || weight.shape[1] != 4) : weight.shape[0] != 4)
-    return fail(MVR_ERR_UNSUPPORTED, "weight_l.npy shape does not match 2-bit codes");
   desc.ivf_offsets = off.data();
   desc.ivf_sizes = size.data();
   desc.ivf_ids = ids.data();
@@ -733,6 +747,9 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   if (k == 0) return fail(MVR_ERR_INVALID, "topk must be positive");
   if (R > kMaxSelect) return fail(MVR_ERR_UNSUPPORTED, "probe_topk %u > %u", R, kMaxSelect);
   if (prm->precision > MVR_FP32_MFMA) return fail(MVR_ERR_INVALID, "unknown precision %u", prm->precision);
+  if (d.n_bit != 2 && (prm->precision != MVR_FP32 || prm->persistent))
+    return fail(MVR_ERR_UNSUPPORTED, "%u-bit residual codes are searched with MVR_FP32 only (the MFMA and "
+                "persistent kernels decode 2-bit codes)", d.n_bit);
   const bool fp32_mode = prm->precision == MVR_FP32 || prm->precision == MVR_FP32_MFMA;
   // FP32 centroid top-P: BF16 MFMA tile maxima + exact FP32 rescan (same
   // result as the all-FP32 scan); MVR_CT_SCAN=1 forces the all-FP32 scan
@@ -1346,6 +1363,8 @@ int mvr_rerank_docs(mvr_index *ix, const float *Q_dev, uint32_t n_q, uint32_t t_
   if (n_q == 0 || R == 0) return MVR_OK;
   if (t_max == 0 || t_max > kMaxQTok) return fail(MVR_ERR_INVALID, "t_max %u not in [1, 32]", t_max);
   if (precision > MVR_FP32_MFMA) return fail(MVR_ERR_INVALID, "unknown precision %u", precision);
+  if (ix->d.n_bit != 2 && precision != MVR_FP32)
+    return fail(MVR_ERR_UNSUPPORTED, "%u-bit residual codes are reranked with MVR_FP32 only", ix->d.n_bit);
   DeviceGuard g(ix->device);
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   Workspace &ws = ix->ws;
@@ -1379,6 +1398,8 @@ int mvr_rerank_docs_noalloc(mvr_index *ix, const float *Q_dev, uint32_t n_q, uin
                                       kNoAllocMaxQ, kNoAllocMaxQ);
   if (t_max == 0 || t_max > kMaxQTok) return fail(MVR_ERR_INVALID, "t_max %u not in [1, 32]", t_max);
   if (precision > MVR_FP32_MFMA) return fail(MVR_ERR_INVALID, "unknown precision %u", precision);
+  if (ix->d.n_bit != 2 && precision != MVR_FP32)
+    return fail(MVR_ERR_UNSUPPORTED, "%u-bit residual codes are reranked with MVR_FP32 only", ix->d.n_bit);
   if (q_ntok)
     for (uint32_t q = 0; q < n_q; ++q)
       if (q_ntok[q] == 0 || q_ntok[q] > t_max) return fail(MVR_ERR_INVALID, "query %u has %u tokens", q, q_ntok[q]);

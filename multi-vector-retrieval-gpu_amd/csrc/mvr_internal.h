@@ -31,8 +31,9 @@ struct DevIndex {
   uint8_t *C_fp8 = nullptr;    // [n_c][128] e4m3 of C / scale
   float *C_fp8_scale = nullptr;// [n_c]
   uint32_t *codes = nullptr;   // [n_tok]
-  uint8_t *residual = nullptr; // [n_tok][32]
-  float *W = nullptr;          // [128][4] decode table
+  uint32_t n_bit = 2;          // residual bits per dim: 1, 2, 4 or 8 (only 2 outside the exact FP32 path)
+  uint8_t *residual = nullptr; // [n_tok][16 * n_bit], values packed MSB-first
+  float *W = nullptr;          // [128][2^n_bit] decode table
   uint32_t *doclens = nullptr; // [n_doc]
   uint64_t *docoff = nullptr;  // [n_doc]
   uint64_t *ivf_off = nullptr; // [n_c]

@@ -849,7 +849,11 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
 // independent FMA chains -- its query token against those four tokens --
 // reading the decoded tokens as LDS broadcasts.  Each dot product is still
 // one sequential chain over d, so results are bit-identical to the oracle.
-// No decompressed embedding touches HBM.
+// No decompressed embedding touches HBM.  NB = residual bits per dim: 2 keeps
+// the decode weights of the lane's four dims in VGPRs; 1, 4 and 8 read the
+// lane's NB/2 bytes and look each value up in the [128][2^NB] table in global
+// memory (the reference's decode for 8/NB values per byte,
+// DataStructPP.hpp:184-200, RerankPP.hpp:100-108).
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float pick4(uint32_t b, float w0, float w1, float w2, float w3) {
   const float lo = (b & 1u) ? w1 : w0;
@@ -860,6 +864,7 @@ __device__ __forceinline__ float pick4(uint32_t b, float w0, float w1, float w2,
 constexpr uint32_t kRrTok = 8;   // doc tokens per step
 constexpr uint32_t kRrChains = kRrTok / 2;
 
+template <uint32_t NB>
 __global__ __launch_bounds__(256) void k_rerank_f32(
     const float *__restrict__ Q, uint32_t t_max, const uint32_t *__restrict__ q_ntok,
     const uint32_t *__restrict__ sel_ids, const uint32_t *__restrict__ sel_n, uint32_t R,
@@ -884,12 +889,15 @@ __global__ __launch_bounds__(256) void k_rerank_f32(
       qv[4 * i] = v.x; qv[4 * i + 1] = v.y; qv[4 * i + 2] = v.z; qv[4 * i + 3] = v.w;
     }
   }
-  // decode weights of dims 4t..4t+3
+  // decode weights of dims 4t..4t+3 (2-bit codes)
+  constexpr uint32_t kRb = kDim * NB / 8, kNq = 1u << NB, kMask = kNq - 1;
   float w[4][4];
+  if constexpr (NB == 2) {
 #pragma unroll
-  for (uint32_t i = 0; i < 4; ++i)
+    for (uint32_t i = 0; i < 4; ++i)
 #pragma unroll
-    for (uint32_t b = 0; b < 4; ++b) w[i][b] = W[(4 * t + i) * 4 + b];
+      for (uint32_t b = 0; b < 4; ++b) w[i][b] = W[(4 * t + i) * 4 + b];
+  }
 
   for (uint32_t r = slice * 4 + wave; r < n; r += n_slice * 4) {
     const uint32_t doc = sel_ids[(size_t)q * R + r];
@@ -903,13 +911,27 @@ __global__ __launch_bounds__(256) void k_rerank_f32(
         if (i + tk < L) {
           const uint64_t g = off + i + tk;
           const uint32_t code = codes[g];
-          const uint32_t byte = residual[g * kResBytes + t];
           const float4 cv = reinterpret_cast<const float4 *>(C + (size_t)code * kDim)[t];
           float4 x;
-          x.x = cv.x + pick4((byte >> 6) & 3u, w[0][0], w[0][1], w[0][2], w[0][3]);
-          x.y = cv.y + pick4((byte >> 4) & 3u, w[1][0], w[1][1], w[1][2], w[1][3]);
-          x.z = cv.z + pick4((byte >> 2) & 3u, w[2][0], w[2][1], w[2][2], w[2][3]);
-          x.w = cv.w + pick4(byte & 3u, w[3][0], w[3][1], w[3][2], w[3][3]);
+          if constexpr (NB == 2) {
+            const uint32_t byte = residual[g * kRb + t];
+            x.x = cv.x + pick4((byte >> 6) & 3u, w[0][0], w[0][1], w[0][2], w[0][3]);
+            x.y = cv.y + pick4((byte >> 4) & 3u, w[1][0], w[1][1], w[1][2], w[1][3]);
+            x.z = cv.z + pick4((byte >> 2) & 3u, w[2][0], w[2][1], w[2][2], w[2][3]);
+            x.w = cv.w + pick4(byte & 3u, w[3][0], w[3][1], w[3][2], w[3][3]);
+          } else {
+            // the 4*NB bits of dims 4t..4t+3, MSB-first from bit 4t*NB of the row
+            uint32_t bits;
+            const uint8_t *row = residual + g * kRb;
+            if constexpr (NB == 1) bits = (row[t >> 1] >> ((t & 1u) ? 0 : 4)) & 0xFu;
+            else if constexpr (NB == 4) bits = ((uint32_t)row[2 * t] << 8) | row[2 * t + 1];
+            else bits = __builtin_bswap32(reinterpret_cast<const uint32_t *>(row)[t]);
+            const float *w0 = W + (size_t)(4 * t) * kNq;
+            x.x = cv.x + w0[(bits >> (3 * NB)) & kMask];
+            x.y = cv.y + w0[kNq + ((bits >> (2 * NB)) & kMask)];
+            x.z = cv.z + w0[2 * kNq + ((bits >> NB) & kMask)];
+            x.w = cv.w + w0[3 * kNq + (bits & kMask)];
+          }
           xs[wave][tk][t] = x;
         }
       }
@@ -2716,23 +2738,29 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
   if (i < n) p[i] = v;
 }
 
+// One thread per (token, 4 dims): x = C[code] + W[d][value], the value of
+// dim d read MSB-first from bit d*n_bit of the token's packed row.
 __global__ void k_decompress(const uint64_t *__restrict__ tok_ids, uint64_t n,
                              const uint32_t *__restrict__ codes,
                              const uint8_t *__restrict__ residual, const float *__restrict__ C,
-                             const float *__restrict__ W, float *__restrict__ out) {
+                             const float *__restrict__ W, uint32_t n_bit, float *__restrict__ out) {
   const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= n * (kDim / 4)) return;
   const uint64_t k = g / (kDim / 4);
   const uint32_t j = (uint32_t)(g % (kDim / 4));
   const uint64_t tok = tok_ids[k];
-  const uint32_t byte = residual[tok * kResBytes + j];
-  const float4 c = reinterpret_cast<const float4 *>(C + (size_t)codes[tok] * kDim)[j];
-  float4 x;
-  x.x = c.x + W[(4 * j + 0) * 4 + ((byte >> 6) & 3u)];
-  x.y = c.y + W[(4 * j + 1) * 4 + ((byte >> 4) & 3u)];
-  x.z = c.z + W[(4 * j + 2) * 4 + ((byte >> 2) & 3u)];
-  x.w = c.w + W[(4 * j + 3) * 4 + (byte & 3u)];
-  reinterpret_cast<float4 *>(out)[g] = x;
+  const uint8_t *row = residual + tok * (kDim * n_bit / 8);
+  const uint32_t nq = 1u << n_bit, mask = nq - 1;
+  const float4 cv = reinterpret_cast<const float4 *>(C + (size_t)codes[tok] * kDim)[j];
+  const float c[4] = {cv.x, cv.y, cv.z, cv.w};
+  float x[4];
+#pragma unroll
+  for (uint32_t i = 0; i < 4; ++i) {
+    const uint32_t d = 4 * j + i, bit = d * n_bit;
+    const uint32_t v = (row[bit >> 3] >> (8 - n_bit - (bit & 7u))) & mask;
+    x[i] = c[i] + W[d * nq + v];
+  }
+  reinterpret_cast<float4 *>(out)[g] = make_float4(x[0], x[1], x[2], x[3]);
 }
 
 // ---------------------------------------------------------------------------
@@ -2972,9 +3000,19 @@ hipError_t launch_rerank(const float *Q, uint32_t t_max, const uint32_t *q_ntok,
   const uint32_t n_slice = std::min((R + 63) / 64, std::max(4u, 8192u / n_q));
   dim3 grid(n_slice, n_q);
   if (precision == 0) {
-    k_rerank_f32<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, sel_ids, sel_n, R, n_slice, ix.doclens,
-                                      ix.docoff, ix.codes, ix.residual, ix.C, ix.W, clip_zero,
-                                      sel_score);
+#define MVR_RR_F32(NB)                                                                                      \
+  k_rerank_f32<NB><<<grid, 256, 0, s>>>(Q, t_max, q_ntok, sel_ids, sel_n, R, n_slice, ix.doclens, ix.docoff, \
+                                        ix.codes, ix.residual, ix.C, ix.W, clip_zero, sel_score)
+    switch (ix.n_bit) {
+      case 1: MVR_RR_F32(1); break;
+      case 2: MVR_RR_F32(2); break;
+      case 4: MVR_RR_F32(4); break;
+      case 8: MVR_RR_F32(8); break;
+      default: return hipErrorNotSupported;
+    }
+#undef MVR_RR_F32
+  } else if (ix.n_bit != 2) {
+    return hipErrorNotSupported;  // the MFMA kernels decode 2-bit codes only
   } else if (precision == 3) {
     k_rerank_f32mfma<<<grid, 256, 0, s>>>(Q, t_max, q_ntok, sel_ids, sel_n, R, n_slice, ix.doclens,
                                           ix.docoff, ix.codes, ix.residual, ix.C, ix.W, clip_zero,
@@ -3000,6 +3038,7 @@ hipError_t launch_rerank_bf16_dm(const float *Q, uint32_t t_max, const uint32_t 
                                  const DevIndex &ix, int clip_zero, const DocMajorBufs &b,
                                  float *sel_score, hipStream_t s) {
   if (n_q == 0 || ix.n_doc == 0) return hipSuccess;
+  if (ix.n_bit != 2) return hipErrorNotSupported;
   if (!ix.C_bf16m || (uint64_t)n_q * R > UINT32_MAX) return hipErrorInvalidValue;
   const uint32_t nblk = (ix.n_doc + kDmBlockDocs - 1) / kDmBlockDocs;
   static int occ = [] {
@@ -3028,6 +3067,7 @@ hipError_t launch_persist(const float *Q, uint32_t t_max, const uint32_t *q_ntok
                           uint32_t *queue, float *score, uint32_t kk, uint32_t *out_ids, float *out_s,
                           uint32_t *out_n, uint32_t out_stride, hipStream_t s) {
   if (nb == 0) return hipSuccess;
+  if (ix.n_bit != 2) return hipErrorNotSupported;
   if (kk == 0 || kk > kTopkMax || cfg.n_wg == 0) return hipErrorInvalidValue;
   if (precision == 1 && !ix.C_bf16m) return hipErrorInvalidValue;
   if (precision == 2 && (!ix.C_fp8 || !ix.lut_fp8)) return hipErrorInvalidValue;
@@ -3326,8 +3366,9 @@ hipError_t launch_decompress(const uint64_t *tok_ids, uint64_t n, const DevIndex
                              hipStream_t s) {
   if (n == 0) return hipSuccess;
   const uint64_t total = n * (kDim / 4);
+  if (ix.n_bit != 1 && ix.n_bit != 2 && ix.n_bit != 4 && ix.n_bit != 8) return hipErrorNotSupported;
   k_decompress<<<(unsigned)((total + 255) / 256), 256, 0, s>>>(tok_ids, n, ix.codes, ix.residual,
-                                                               ix.C, ix.W, out);
+                                                               ix.C, ix.W, ix.n_bit, out);
   return hipGetLastError();
 }
 

@@ -54,6 +54,15 @@ def _ivf_from_codes(codes, doclens, n_c):
     return off, sizes, ids
 
 
+def _n_bit(n_weight: int) -> int:
+    """Bits per residual value of a decode table with n_weight entries per dim
+    (weight_l is [2^n_bit] or [dim, 2^n_bit])."""
+    for b in (1, 2, 4, 8):
+        if n_weight == 1 << b:
+            return b
+    raise MvrError(1, f"weights hold {n_weight} values per dim (expected 2, 4, 16 or 256)")
+
+
 def _flags(residual_on_host: bool) -> int:
     return _capi.INDEX_RESIDUAL_ON_HOST if residual_on_host else 0
 
@@ -87,7 +96,7 @@ class Index:
         d = _capi.IndexDesc()
         d.n_doc, d.n_tok = doclens.size, codes.size
         d.n_centroid, d.dim = centroids.shape
-        d.n_bit = 2
+        d.n_bit = _n_bit(weights.shape[-1])
         d.doclens, d.centroids, d.codes = _ptr(doclens), _ptr(centroids), _ptr(codes)
         d.residual, d.weights = _ptr(residual), _ptr(weights)
         d.weights_per_dim = 1 if weights.ndim == 2 else 0
@@ -105,7 +114,7 @@ class Index:
         d = _capi.IndexDesc()
         d.n_doc, d.n_tok = doclens.numel(), codes.numel()
         d.n_centroid, d.dim = centroids.shape
-        d.n_bit = 2
+        d.n_bit = _n_bit(weights.shape[-1])
         d.doclens, d.centroids = doclens.data_ptr(), centroids.data_ptr()
         d.codes, d.residual, d.weights = codes.data_ptr(), residual.data_ptr(), weights.data_ptr()
         d.weights_per_dim = 1 if weights.dim() == 2 else 0
@@ -269,8 +278,10 @@ class IGPGPUPP:
         if len(dirs) != 1:
             raise MvrError(1, "index files must share one directory")
         self.index = Index.load(dirs.pop(), self.device, self.residual_on_host).set_doclens(self._item_n_vec_l)
-        if self.index.n_centroid != n_centroid or n_bit != 2:
-            raise MvrError(1, f"index has {self.index.n_centroid} centroids/2 bits, asked {n_centroid}/{n_bit}")
+        ix_bit = _n_bit(np.load(weight_l_fname, mmap_mode="r").shape[-1])
+        if self.index.n_centroid != n_centroid or n_bit != ix_bit:
+            raise MvrError(1, f"index has {self.index.n_centroid} centroids/{ix_bit} bits, "
+                              f"asked {n_centroid}/{n_bit}")
         return True
 
     def search(self, query_l, n_query: int, query_n_vec: int, topk: int, nprobe: int,

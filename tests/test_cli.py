@@ -52,10 +52,10 @@ def test_missing_index_is_an_error(tmp_path):
     assert r.returncode == 1 and "cannot open" in r.stderr
 
 
-def write_dataset(root, ix, Q, qids, n_c):
+def write_dataset(root, ix, Q, qids, n_c, n_bit=2):
     """Index/Embedding/RawData layout of src/app/IGPGPUPP.cpp:33-73."""
     import mvr
-    d = os.path.join(root, "Index", "ds", f"igp-n_centroid_{n_c}-n_bit_2")
+    d = os.path.join(root, "Index", "ds", f"igp-n_centroid_{n_c}-n_bit_{n_bit}")
     os.makedirs(d)
     np.save(os.path.join(d, "item_n_vec_l.npy"), ix.doclens.astype(np.uint32))
     np.save(os.path.join(d, "centroid_l.npy"), ix.C.astype(np.float32))

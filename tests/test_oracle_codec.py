@@ -123,3 +123,96 @@ def test_e4m3_round_known_values():
     np.testing.assert_array_equal(O.e4m3_round(v), v)
     mids = ((v[:-1].astype(np.float64) + v[1:]) / 2 * 0.999).astype(np.float32)
     np.testing.assert_array_equal(O.e4m3_round(mids), v[:-1])
+
+
+# ---- other widths: 1, 4 and 8 bits per value (8/b values per byte) ----
+
+def _unpack_np(res, n_bit, dim):
+    """MSB-first unpack by a different route than the oracle's shifts: the
+    row's bits in order (np.unpackbits, most significant first), n_bit at a time."""
+    bits = np.unpackbits(res, axis=1)[:, : dim * n_bit].reshape(res.shape[0], dim, n_bit).astype(np.int64)
+    return (bits << np.arange(n_bit - 1, -1, -1)).sum(-1)
+
+
+def _pack_np(b, n_bit):
+    """Buckets [n, dim] -> bytes by np.packbits over each value's n_bit bits."""
+    bits = (b[..., None] >> np.arange(n_bit - 1, -1, -1)) & 1
+    return np.packbits(bits.reshape(b.shape[0], -1).astype(np.uint8), axis=1)
+
+
+@pytest.mark.parametrize("n_bit,byte,vals", [
+    (1, 0b10110001, [1, 0, 1, 1, 0, 0, 0, 1]),
+    (2, 0b00011011, [0, 1, 2, 3]),
+    (4, 0xA5, [0xA, 0x5]),
+    (8, 0xC3, [0xC3]),
+])
+def test_unpack_known_answers_every_width(n_bit, byte, vals):
+    """Value i of a byte is (byte >> (8 - n_bit*(i+1))) & mask
+    (DataStructPP.hpp:184-200); weights 10*b make the decoded value show b."""
+    dim = 128
+    res = np.full((1, dim * n_bit // 8), byte, np.uint8)
+    W = O.decode_table(np.arange(1 << n_bit, dtype=np.float32) * 10, dim, n_bit)
+    x = O.decompress_tokens(np.array([0], np.uint64), np.array([0], np.uint32), res,
+                            np.zeros((1, dim), np.float32), W, n_bit)[0]
+    np.testing.assert_array_equal(x, np.tile(np.array(vals, np.float32) * 10, dim // len(vals)))
+    np.testing.assert_array_equal(_unpack_np(res, n_bit, dim)[0], np.tile(vals, dim // len(vals)))
+
+
+@pytest.mark.parametrize("n_bit", [1, 2, 4])
+def test_pooled_quantiles_known_answer_every_width(n_bit):
+    # 2*nq distinct values 0..2nq-1: cutoffs at ranks floor(i/nq*2nq) = 2i,
+    # weights at floor((i+.5)/nq*2nq) = 2i+1
+    nq = 1 << n_bit
+    v = np.random.default_rng(n_bit).permutation(2 * nq).astype(np.float32)
+    cut, w = O.quantize_pooled(v, n_bit)
+    np.testing.assert_array_equal(cut, 2 * np.arange(1, nq))
+    np.testing.assert_array_equal(w, 2 * np.arange(nq) + 1)
+
+
+@pytest.mark.parametrize("n_bit", [1, 4, 8])
+def test_perdim_quantiles_every_width(n_bit):
+    rng = np.random.default_rng(30 + n_bit)
+    n, dim, nq = 4001, 128, 1 << n_bit
+    e = (rng.standard_normal((n, dim)) * np.linspace(0.01, 0.2, dim)).astype(np.float32)
+    cut, w = O.quantize_perdim(e, n_bit)
+    assert cut.shape == (dim, nq - 1) and w.shape == (dim, nq)
+    for d in (0, 77, 127):
+        col = e[:, d]
+        for i in range(nq - 1):
+            assert (col < cut[d, i]).sum() == int(np.floor((i + 1) / nq * n))
+        b = np.searchsorted(cut[d], col, side="right")
+        for k in range(nq):
+            np.testing.assert_allclose(w[d, k], col[b == k].astype(np.float64).mean(), rtol=1e-5)
+
+
+@pytest.mark.parametrize("n_bit", [1, 2, 4, 8])
+@pytest.mark.parametrize("per_dim", [True, False])
+def test_codec_roundtrip_every_width(n_bit, per_dim):
+    """compress == np.packbits of the searchsorted buckets; decompress ==
+    C[code] + W[d][bucket]; the oracle's MaxSim over those tokens == float64
+    MaxSim of the decoded vectors."""
+    rng = np.random.default_rng(40 + n_bit)
+    dim, n = 128, 600
+    C = rng.standard_normal((5, dim)).astype(np.float32)
+    codes = rng.integers(0, 5, n).astype(np.uint32)
+    x = (C[codes] + rng.standard_normal((n, dim)) * 0.05).astype(np.float32)
+    e = x - C[codes]
+    cut, w = O.quantize_perdim(e, n_bit) if per_dim else O.quantize_pooled(e, n_bit)
+    res = O.compress(x, codes, C, cut, n_bit)
+    cutd = cut if per_dim else np.tile(cut, (dim, 1))
+    b = np.stack([np.searchsorted(cutd[d], e[:, d], side="right") for d in range(dim)], 1)
+    assert res.shape == (n, dim * n_bit // 8)
+    np.testing.assert_array_equal(res, _pack_np(b, n_bit))
+    np.testing.assert_array_equal(_unpack_np(res, n_bit, dim), b)
+    W = O.decode_table(w, dim, n_bit)
+    wd = w if per_dim else np.tile(w, (dim, 1))
+    np.testing.assert_array_equal(W, wd)
+    xd = O.decompress_tokens(np.arange(n, dtype=np.uint64), codes, res, C, W, n_bit)
+    np.testing.assert_array_equal(xd, (C[codes] + wd[np.arange(dim)[None, :], b]).astype(np.float32))
+    # MaxSim: 6 docs of 100 tokens
+    doclens = np.full(6, 100, np.uint32)
+    ix = O.Index(doclens, codes, res, C, w, n_bit)
+    Q = rng.standard_normal((7, dim)).astype(np.float32)
+    got = ix.rerank(Q, np.arange(6, dtype=np.uint32))
+    ref = [(Q.astype(np.float64) @ xd[100 * k:100 * (k + 1)].astype(np.float64).T).max(1).sum() for k in range(6)]
+    np.testing.assert_allclose(got, ref, rtol=1e-5)
