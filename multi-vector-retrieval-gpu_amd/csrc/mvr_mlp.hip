@@ -24,9 +24,14 @@ namespace {
 // C[M][N] = A(M x K) * B(K x N) with strided operands:
 //   A(i, k) = A[i * a_i + k * a_k],  B(k, j) = B[k * b_k + j * b_j].
 // Epilogue: + bias[j] (nullable); Z (nullable) receives the pre-activation;
-// C receives GELU(.) when gelu != 0.  64x64 tile, 256 threads, 4x4 per thread,
-// K in steps of 16 through LDS; each output is a sequential FMA chain in k.
+// C receives GELU(.) when gelu != 0.  FP32 on the matrix cores
+// (v_mfma_f32_32x32x2_f32): a 64x64 tile per workgroup, each of the four
+// waves owns a 32x32 quarter; K in chunks of 16 staged through LDS with zeros
+// past M, N and K, eight MFMAs per chunk (MFMA s reads k = 2s from the lower
+// lane half and k = 2s + 1 from the upper).  The products are summed in the
+// matrix unit's fixed order, so a step is deterministic.
 constexpr int kTm = 64, kTn = 64, kTk = 16;
+typedef float mlp_floatx16 __attribute__((ext_vector_type(16)));
 
 __device__ __forceinline__ float gelu_f(float z) { return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f)); }
 __device__ __forceinline__ float gelu_grad_f(float z) {
@@ -42,9 +47,11 @@ __global__ __launch_bounds__(256) void k_gemm(const float *__restrict__ A, size_
                                               uint32_t N, uint32_t K) {
   __shared__ float As[kTk][kTm + 4];
   __shared__ float Bs[kTk][kTn + 4];
-  const uint32_t tid = threadIdx.x, tx = tid & 15, ty = tid >> 4;
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const uint32_t t = lane & 31, h = lane >> 5;
+  const uint32_t wi = (wave >> 1) * 32, wj = (wave & 1) * 32;  // the wave's quarter of the tile
   const uint32_t i0 = blockIdx.y * kTm, j0 = blockIdx.x * kTn;
-  float acc[4][4] = {};
+  mlp_floatx16 acc = {};
   for (uint32_t k0 = 0; k0 < K; k0 += kTk) {
     for (uint32_t e = tid; e < kTm * kTk; e += 256) {
       const uint32_t r = e / kTk, kk = e % kTk;
@@ -57,31 +64,23 @@ __global__ __launch_bounds__(256) void k_gemm(const float *__restrict__ A, size_
       Bs[kk][c] = (j < N && k < K) ? B[k * b_k + j * b_j] : 0.0f;
     }
     __syncthreads();
-    const uint32_t kn = min(kTk, K - k0);  // steps of this chunk inside K (uniform)
 #pragma unroll
-    for (uint32_t kk = 0; kk < kTk; ++kk) {
-      if (kk < kn) {
-        float a[4], b[4];
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u) { a[u] = As[kk][ty * 4 + u]; b[u] = Bs[kk][tx * 4 + u]; }
-#pragma unroll
-        for (uint32_t u = 0; u < 4; ++u)
-#pragma unroll
-          for (uint32_t v = 0; v < 4; ++v) acc[u][v] = __builtin_fmaf(a[u], b[v], acc[u][v]);
-      }
-    }
+    for (uint32_t st = 0; st < kTk / 2; ++st)
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(As[2 * st + h][wi + t], Bs[2 * st + h][wj + t], acc, 0, 0, 0);
     __syncthreads();
   }
+  // acc[4g + e] holds row 8g + 4h + e, column t of the wave's quarter
+  const uint32_t j = j0 + wj + t;
+  if (j >= N) return;
+  const float bj = bias ? bias[j] : 0.0f;
 #pragma unroll
-  for (uint32_t u = 0; u < 4; ++u)
-#pragma unroll
-    for (uint32_t v = 0; v < 4; ++v) {
-      const uint32_t i = i0 + ty * 4 + u, j = j0 + tx * 4 + v;
-      if (i >= M || j >= N) continue;
-      float z = acc[u][v] + (bias ? bias[j] : 0.0f);
-      if (Z) Z[(size_t)i * N + j] = z;
-      Cm[(size_t)i * N + j] = gelu ? gelu_f(z) : z;
-    }
+  for (uint32_t r = 0; r < 16; ++r) {
+    const uint32_t i = i0 + wi + 8 * (r >> 2) + 4 * h + (r & 3);
+    if (i >= M) continue;
+    const float z = acc[r] + bj;
+    if (Z) Z[(size_t)i * N + j] = z;
+    Cm[(size_t)i * N + j] = gelu ? gelu_f(z) : z;
+  }
 }
 
 // dZ = dH * GELU'(Z), in place on dH.

@@ -42,6 +42,33 @@ def test_forward_loss_and_gradients_match_oracle(torch):
         np.testing.assert_allclose(got, exp, rtol=2e-3, atol=2e-6 * max(1.0, np.abs(exp).max()))
 
 
+def test_partial_tiles_match_oracle(torch):
+    """Widths and batch that fill no 64x64 output tile and no 16-deep K chunk
+    (1000 -> 100 -> 37 -> 6, batch 77): the matrix-core GEMMs' zero padding
+    past M, N and K, forward and every gradient."""
+    from mvr.probe import ProbeModel, split_params
+    from oracle import mlp as M
+    dims = (1000, 100, 37, 6)
+    rng = np.random.default_rng(9)
+    model = ProbeModel(dims=dims, seed=2)
+    flat = model.get_params()
+    flat[np.arange(flat.size) % 31 == 0] += 0.02
+    model.set_params(flat)
+    params = split_params(flat, dims)
+    x = (rng.standard_normal((77, 1000)) * 0.3).astype(np.float32)
+    y = rng.integers(0, 5, 77).astype(np.int32)
+    r = rng.uniform(2.0, 7.0, 77).astype(np.float32)
+    xd, yd, rd = (torch.from_numpy(a).cuda() for a in (x, y, r))
+    ref, _ = M.forward(params, x)
+    np.testing.assert_allclose(model.forward(torch, xd).cpu().numpy(), ref, rtol=1e-4, atol=1e-5)
+    loss, g = model.grads(xd, yd, rd)
+    rl, rg = M.loss_and_grads(params, x, y, r)
+    assert abs(loss - rl) <= 1e-4 * abs(rl)
+    for got, exp in zip(split_params(g, dims), rg):
+        np.testing.assert_allclose(got, exp, rtol=2e-3, atol=2e-6 * max(1.0, np.abs(exp).max()))
+    model.close()
+
+
 def test_adam_step_matches_oracle(torch):
     from mvr.probe import ProbeModel, split_params
     from oracle import mlp as M
