@@ -1184,20 +1184,25 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       const uint32_t *bnp = d_np + b0;
 
       if (persistent && !ix->tuned[prm->precision]) {
-        // autotune on this pass: every setting once to warm up, then timed;
-        // results are identical for every setting, so the pass stays valid
+        // autotune on this pass: each candidate setting runs the whole pass
+        // once to warm up and once timed (results are identical for every
+        // setting, so the pass stays valid).  Candidates: tiles in flight per
+        // wave x workgroups per CU up to the kernel's residency.  With a
+        // rescore, the FP32 phase's setting is tuned next (unless FP32 MFMA
+        // already has one), the first phase's best held fixed.
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
-        float best = 1e30f;
-        PersistCfg best_c;
-        for (int rep = 0; rep < 2; ++rep)
+        auto tune_one = [&](uint32_t prec) -> int {
+          float best = 1e30f;
+          PersistCfg best_c;
           for (uint32_t tps = 1; tps <= 2; ++tps)
-            for (uint32_t wpc = 1; wpc <= 2; ++wpc) {
+            for (uint32_t wpc = 1; wpc <= persist_max_wgs((int)prec, tps); ++wpc)
+              for (int rep = 0; rep < 2; ++rep) {
                 PersistCfg c;
                 c.n_wg = wpc * d.n_cu;
                 c.tps = tps;
-                ix->tune[prm->precision] = c;
+                ix->tune[prec] = c;
                 HIP_TRY(hipEventRecord(e0, s));
                 const int rc = run_pass(s, Qb, rm, n_rows, bnt, bnp, nb, oid, os, bi, false);
                 if (rc != MVR_OK) return rc;
@@ -1205,12 +1210,27 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
                 HIP_TRY(hipEventSynchronize(e1));
                 float ms = 0.0f;
                 HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                if (rep == 1 && ms < best) { best = ms; best_c = c; }
+                if (rep == 1 && ms < best) {
+                  best = ms;
+                  best_c = c;
+                }
               }
+          ix->tune[prec] = best_c;
+          ix->tuned[prec] = true;
+          return MVR_OK;
+        };
+        int rc = tune_one(prm->precision);
+        if (rc == MVR_OK && R2 && prm->precision != MVR_FP32_MFMA && !ix->tuned[MVR_FP32_MFMA]) {
+          ix->tuned[MVR_FP32_MFMA] = true;  // run_pass's rescore phase reads tune[MVR_FP32_MFMA]
+          rc = tune_one(MVR_FP32_MFMA);
+          if (rc != MVR_OK) ix->tuned[MVR_FP32_MFMA] = false;
+        }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        ix->tune[prm->precision] = best_c;
-        ix->tuned[prm->precision] = true;
+        if (rc != MVR_OK) {
+          ix->tuned[prm->precision] = false;
+          return rc;
+        }
       }
 
       if (use_graph) {
@@ -1336,8 +1356,10 @@ int mvr_index_set_tuning(mvr_index *ix, uint32_t precision, uint32_t wgs_per_cu,
   if (!ix) return fail(MVR_ERR_INVALID, "NULL index");
   if (precision != MVR_BF16 && precision != MVR_FP8 && precision != MVR_FP32_MFMA)
     return fail(MVR_ERR_UNSUPPORTED, "no work-queue kernel for precision %u", precision);
-  if (wgs_per_cu < 1 || wgs_per_cu > 2) return fail(MVR_ERR_INVALID, "wgs_per_cu %u not in {1,2}", wgs_per_cu);
   if (tiles < 1 || tiles > 2) return fail(MVR_ERR_INVALID, "tiles %u not in {1,2}", tiles);
+  const uint32_t wmax = persist_max_wgs((int)precision, tiles);
+  if (wgs_per_cu < 1 || wgs_per_cu > wmax)
+    return fail(MVR_ERR_INVALID, "wgs_per_cu %u not in 1..%u (residency of that kernel)", wgs_per_cu, wmax);
   PersistCfg c;
   c.n_wg = wgs_per_cu * ix->d.n_cu;
   c.tps = tiles;

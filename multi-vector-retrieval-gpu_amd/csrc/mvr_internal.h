@@ -99,6 +99,10 @@ struct PersistCfg {
   uint32_t tps = 1;    // 32-token tiles in flight per wave
 };
 constexpr uint32_t kPersistTopkMax = 256;
+// resident workgroups per CU of the persistent kernel for (precision, tiles
+// in flight): the largest useful grid is this times the CU count; 0 for a
+// combination without a kernel
+uint32_t persist_max_wgs(int precision, uint32_t tps);
 hipError_t launch_persist(const float *Q, uint32_t t_max, const uint32_t *q_ntok,
                           const uint32_t *in_ids, const uint32_t *in_n, uint32_t R_in, uint32_t nb,
                           const DevIndex &ix, int precision, int clip_zero, const PersistCfg &cfg,

@@ -3090,6 +3090,23 @@ hipError_t launch_persist(const float *Q, uint32_t t_max, const uint32_t *q_ntok
   return hipGetLastError();
 }
 
+uint32_t persist_max_wgs(int precision, uint32_t tps) {
+  // resident workgroups per CU of each persistent kernel (registers and LDS)
+  auto occ = [](const void *f) -> uint32_t {
+    int o = 0;
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, f, 256, 0) == hipSuccess && o > 0 ? (uint32_t)o : 1u;
+  };
+  static const uint32_t t[3][2] = {
+      {occ(reinterpret_cast<const void *>(k_rerank_persist<1, 1>)),
+       occ(reinterpret_cast<const void *>(k_rerank_persist<1, 2>))},
+      {occ(reinterpret_cast<const void *>(k_rerank_persist<2, 1>)),
+       occ(reinterpret_cast<const void *>(k_rerank_persist<2, 2>))},
+      {occ(reinterpret_cast<const void *>(k_rerank_persist<3, 1>)),
+       occ(reinterpret_cast<const void *>(k_rerank_persist<3, 2>))}};
+  if (precision < 1 || precision > 3 || tps < 1 || tps > 2) return 0;
+  return t[precision - 1][tps - 1];
+}
+
 hipError_t launch_exhaustive_f64(const float *Q, uint32_t n_q, uint32_t t_max,
                                  const uint32_t *q_ntok, const float *X, const uint32_t *doclens,
                                  const uint64_t *docoff, uint32_t n_doc, double *out,

@@ -11,6 +11,8 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
+from mvr._capi import MvrError  # noqa: E402
+
 
 def _index(ix):
     import mvr
@@ -31,12 +33,21 @@ def test_persistent_matches_launch_path_every_setting(c1, prec, rescore):
     qt, qp = _ragged(Q, 1)
     ref_ids, ref_sc = g.search(Q, 8, 256, 10, precision=prec, rescore=rescore, q_ntok=qt, q_nprobe=qp,
                                batch=24)
-    for wpc, tps in itertools.product((1, 2), (1, 2)):
-        g.set_tuning(prec, wpc, tps)
+    n_set = 0
+    for wpc, tps in itertools.product((1, 2, 3, 4), (1, 2)):
+        try:
+            g.set_tuning(prec, wpc, tps)
+        except MvrError:
+            assert wpc > 1, "one workgroup per CU is always resident"
+            continue  # more workgroups per CU than that kernel keeps resident
+        n_set += 1
         ids, sc = g.search(Q, 8, 256, 10, precision=prec, rescore=rescore, q_ntok=qt, q_nprobe=qp,
                            batch=24, persistent=True)
         np.testing.assert_array_equal(ids, ref_ids, err_msg=f"{prec} wpc={wpc} tps={tps}")
         np.testing.assert_array_equal(sc, ref_sc, err_msg=f"{prec} wpc={wpc} tps={tps}")
+    assert n_set >= 3
+    with pytest.raises(MvrError):
+        g.set_tuning(prec, 9, 1)
 
 
 def test_autotuner_picks_a_valid_setting(small):
@@ -46,7 +57,15 @@ def test_autotuner_picks_a_valid_setting(small):
     ref = g.search(Q, 4, 64, 10, precision="bf16")
     got = g.search(Q, 4, 64, 10, precision="bf16", persistent=True)
     wpc, tps = g.get_tuning("bf16")
-    assert wpc in (1, 2) and tps in (1, 2)
+    assert 1 <= wpc <= 8 and tps in (1, 2)
+    np.testing.assert_array_equal(got[0], ref[0])
+    np.testing.assert_array_equal(got[1], ref[1])
+    # with a rescore the FP32 phase gets a setting of its own
+    assert g.get_tuning("fp32_mfma") is None
+    g2 = _index(ix)
+    ref = g2.search(Q, 4, 64, 10, precision="bf16", rescore=32)
+    got = g2.search(Q, 4, 64, 10, precision="bf16", rescore=32, persistent=True)
+    assert g2.get_tuning("bf16") is not None and g2.get_tuning("fp32_mfma") is not None
     np.testing.assert_array_equal(got[0], ref[0])
     np.testing.assert_array_equal(got[1], ref[1])
 
