@@ -258,7 +258,9 @@ def main():
                                + ("U{8..32} tokens, nprobe U{2,4,8,16,32}" if q_nt is not None
                                   else f"32 tokens; nprobe {P}") + f", probe_topk {R}, k {k}",
                    "persistent": args.persistent, "graph": args.graph,
-                   "tuning": (ix.get_tuning(args.precision) if args.persistent else None),
+                   "tuning": ({p: ix.get_tuning(p) for p in (args.precision, "fp32_mfma")}
+                              if args.persistent else None),
+                   "maxsim_order": args.maxsim_order,
                    "batch_per_pass": batch, "precision": args.precision, "rescore": rescore,
                    "residual_on_host": args.residual_on_host},
         # the metric's recall clause, as measured: this value is the QPS at the

@@ -161,8 +161,10 @@ void mvr_index_free(mvr_index *ix);
  * in turn.  MVR_MAXSIM_DOC: the pass's (query, doc) pairs grouped by doc, each
  * doc's tokens loaded once for all the queries that selected it.
  * MVR_MAXSIM_AUTO (default): doc order when a device pass holds at least as
- * many pairs as the index has docs.  The persistent kernel is always per
- * query. */
+ * many pairs as the index has docs.  In a persistent search the doc order
+ * makes the BF16 phase a work queue of docs followed by a per-query top-R2
+ * (or top-k) launch; the query order runs the per-query persistent kernel
+ * with its fused top-k.  The FP32 rescore phase is always per query. */
 enum { MVR_MAXSIM_AUTO = 0, MVR_MAXSIM_QUERY = 1, MVR_MAXSIM_DOC = 2 };
 int mvr_index_set_maxsim_order(mvr_index *ix, uint32_t order);
 int mvr_index_info(const mvr_index *ix, uint32_t *n_doc, uint64_t *n_tok, uint32_t *n_centroid,

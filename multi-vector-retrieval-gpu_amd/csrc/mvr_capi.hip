@@ -862,8 +862,10 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   // BF16 MaxSim doc-major (each selected doc's tiles loaded once per pass for
   // all of its queries) when a pass holds at least as many (query, doc) pairs
   // as the index has docs, i.e. about 1.6 or more pairs per distinct doc;
-  // query-major below that, where most docs would be loaded for one query
-  const bool doc_major = prm->precision == MVR_BF16 && !persistent && (uint64_t)Bc * R <= UINT32_MAX &&
+  // query-major below that, where most docs would be loaded for one query.
+  // In a persistent search the doc-major kernel is the first phase's work
+  // queue (of docs), followed by the per-query top-R2 (or top-k).
+  const bool doc_major = prm->precision == MVR_BF16 && (uint64_t)Bc * R <= UINT32_MAX &&
                          (ix->maxsim_order == MVR_MAXSIM_DOC ||
                           (ix->maxsim_order == MVR_MAXSIM_AUTO && (uint64_t)Bc * R >= d.n_doc));
   DocMajorBufs dmb;
@@ -891,7 +893,11 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   // host-resident residuals: two passes in flight, the staging copy of one (on
   // a second stream) overlapping the device work of the other; the selection
   // outputs and the query buffer then have one slot per pass in flight
-  const bool pipelined = ix->res_host && !stages && n_pass > 1 && !(persistent && !ix->tuned[prm->precision]);
+  // persistent settings still to tune: the first phase's (unless it runs
+  // doc-major) and, with a rescore, the FP32 phase's
+  const bool tune_p1 = persistent && !doc_major && !ix->tuned[prm->precision];
+  const bool tune_p2 = persistent && R2 && prm->precision != MVR_FP32_MFMA && !ix->tuned[MVR_FP32_MFMA];
+  const bool pipelined = ix->res_host && !stages && n_pass > 1 && !tune_p1 && !tune_p2;
   const size_t n_slot = pipelined ? 2 : 1;
   HIP_TRY(ws.cand_tot.ensure(n_slot * Bc * 4));
   HIP_TRY(ws.sel_ids.ensure(n_slot * Bc * R * 4));
@@ -995,6 +1001,29 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     const uint32_t *sid = hp ? hp->ids : sl.ids;
     const DevIndex &dx = hp ? hp->d : d;
     const int clip = (int)prm->clip_zero;
+    if (persistent && doc_major) {
+      // phase 1: doc-major BF16 MaxSim of all candidates (a work queue of
+      // docs), then each query's top-R2 -- or its top-k without a rescore
+      HIP_TRY(launch_rerank_bf16_dm(Qb, t_max, bnt, nb, sid, sl.n, R, dx, clip, dmb, sl.score, st));
+      if (R2) {
+        HIP_TRY(launch_topk(sid, sl.score, sl.n, R, nb, R2, ws.res_ids.as<uint32_t>(), ws.res_s.as<float>(),
+                            ws.res_n.as<uint32_t>(), st));
+        HIP_TRY(mark(4));
+        // phase 2: FP32 MaxSim of those, fused top-k
+        const PersistCfg &c2 = ix->tuned[MVR_FP32_MFMA] ? ix->tune[MVR_FP32_MFMA] : ix->tune[prm->precision];
+        HIP_TRY(launch_persist(Qb, t_max, bnt, ws.res_ids.as<uint32_t>(), ws.res_n.as<uint32_t>(), R2, nb, dx,
+                               MVR_FP32_MFMA, clip, c2, ws.queue.as<uint32_t>(), ws.res_s.as<float>(), k, oid,
+                               os, nullptr, k, st));
+        HIP_TRY(mark(5));
+      } else {
+        HIP_TRY(launch_topk(sid, sl.score, sl.n, R, nb, k, oid, os, nullptr, st));
+        HIP_TRY(mark(4));
+        HIP_TRY(mark(5));
+      }
+      if (hp) HIP_TRY(launch_map_ids(oid, nb * k, hp->docs, hp->n, st));
+      HIP_TRY(mark(6));
+      return MVR_OK;
+    }
     if (persistent) {
       const PersistCfg &c1 = ix->tune[prm->precision];
       uint32_t *qu = ws.queue.as<uint32_t>();
@@ -1183,7 +1212,9 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       const uint32_t *bnt = d_nt + b0;
       const uint32_t *bnp = d_np + b0;
 
-      if (persistent && !ix->tuned[prm->precision]) {
+      const bool need_p1 = tune_p1 && !ix->tuned[prm->precision];
+      const bool need_p2 = tune_p2 && !ix->tuned[MVR_FP32_MFMA];
+      if (need_p1 || need_p2) {
         // autotune on this pass: each candidate setting runs the whole pass
         // once to warm up and once timed (results are identical for every
         // setting, so the pass stays valid).  Candidates: tiles in flight per
@@ -1219,8 +1250,8 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
           ix->tuned[prec] = true;
           return MVR_OK;
         };
-        int rc = tune_one(prm->precision);
-        if (rc == MVR_OK && R2 && prm->precision != MVR_FP32_MFMA && !ix->tuned[MVR_FP32_MFMA]) {
+        int rc = need_p1 ? tune_one(prm->precision) : MVR_OK;
+        if (rc == MVR_OK && need_p2) {
           ix->tuned[MVR_FP32_MFMA] = true;  // run_pass's rescore phase reads tune[MVR_FP32_MFMA]
           rc = tune_one(MVR_FP32_MFMA);
           if (rc != MVR_OK) ix->tuned[MVR_FP32_MFMA] = false;
@@ -1228,7 +1259,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         if (rc != MVR_OK) {
-          ix->tuned[prm->precision] = false;
+          if (need_p1) ix->tuned[prm->precision] = false;
           return rc;
         }
       }

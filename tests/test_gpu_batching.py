@@ -53,6 +53,7 @@ def test_persistent_matches_launch_path_every_setting(c1, prec, rescore):
 def test_autotuner_picks_a_valid_setting(small):
     corpus, ix, Q = small
     g = _index(ix)
+    g.set_maxsim_order("query")  # the per-query persistent kernel in the first phase
     assert g.get_tuning("bf16") is None
     ref = g.search(Q, 4, 64, 10, precision="bf16")
     got = g.search(Q, 4, 64, 10, precision="bf16", persistent=True)
@@ -63,11 +64,32 @@ def test_autotuner_picks_a_valid_setting(small):
     # with a rescore the FP32 phase gets a setting of its own
     assert g.get_tuning("fp32_mfma") is None
     g2 = _index(ix)
+    g2.set_maxsim_order("query")
     ref = g2.search(Q, 4, 64, 10, precision="bf16", rescore=32)
     got = g2.search(Q, 4, 64, 10, precision="bf16", rescore=32, persistent=True)
     assert g2.get_tuning("bf16") is not None and g2.get_tuning("fp32_mfma") is not None
     np.testing.assert_array_equal(got[0], ref[0])
     np.testing.assert_array_equal(got[1], ref[1])
+
+
+@pytest.mark.parametrize("rescore", [0, 64])
+def test_persistent_doc_major_first_phase(c1, rescore):
+    """A pass of 64 queries x 256 candidates over 8,192 docs runs the BF16
+    phase doc-major (a work queue of docs, then the per-query top-R2 or
+    top-k): same results as the launch path, eagerly and replayed from a
+    graph; only the FP32 phase has a persistent setting to tune."""
+    corpus, ix, Q = c1
+    g = _index(ix)
+    qt, qp = _ragged(Q, 3)
+    kw = dict(precision="bf16", rescore=rescore, q_ntok=qt, q_nprobe=qp)
+    ref_ids, ref_sc = g.search(Q, 8, 256, 10, **kw)
+    for graph in (False, True, True, True):
+        ids, sc = g.search(Q, 8, 256, 10, persistent=True, graph=graph, **kw)
+        np.testing.assert_array_equal(ids, ref_ids, err_msg=f"graph={graph}")
+        np.testing.assert_array_equal(sc, ref_sc, err_msg=f"graph={graph}")
+    assert g.get_tuning("bf16") is None
+    assert (g.get_tuning("fp32_mfma") is not None) == (rescore > 0)
+    g.close()
 
 
 @pytest.mark.parametrize("persistent", [False, True])
