@@ -208,11 +208,14 @@ int mvr_rerank_docs_noalloc(mvr_index *ix, const float *Q_dev, uint32_t n_q, uin
 
 /* Work-queue MaxSim tuning per precision (no counterpart in the reference,
  * which launches one kernel per query): workgroups per CU of the persistent
- * grid and 32-token tiles in flight per wave.  The first persistent search in
- * a precision times the candidate settings on its first pass and keeps the
- * fastest; set_tuning fixes them instead (wgs_per_cu in {1, 2}, tiles in
- * {1, 2}).  get_tuning writes (wgs_per_cu, tiles) and fails with
- * MVR_ERR_INVALID until a setting exists. */
+ * grid and doc tiles in flight per wave (16-token tiles in BF16 and FP8: 1 or
+ * 2; the FP32 MFMA kernel streams one packed 32-row tile: 1).  The first
+ * persistent search in a precision times every candidate setting on its first
+ * pass and keeps the fastest, and with a rescore also tunes the FP32 MFMA
+ * phase; set_tuning fixes a setting instead (wgs_per_cu from 1 to the number
+ * of that kernel's workgroups a CU holds, MVR_ERR_INVALID beyond).
+ * get_tuning writes (wgs_per_cu, tiles) and fails with MVR_ERR_INVALID until
+ * a setting exists. */
 int mvr_index_set_tuning(mvr_index *ix, uint32_t precision, uint32_t wgs_per_cu, uint32_t tiles);
 int mvr_index_get_tuning(const mvr_index *ix, uint32_t precision, uint32_t out[2]);
 

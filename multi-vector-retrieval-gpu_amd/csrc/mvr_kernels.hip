@@ -1520,8 +1520,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // ---------------------------------------------------------------------------
 // Fused decompress + MaxSim on FP32 MFMA (v_mfma_f32_32x32x2_f32).  Same tile
 // scheme as the BF16 kernel: A = 32 doc tokens (rows), B = the 32 query tokens
-// (columns, 64 FP32 values per lane held for the whole workgroup).  Lane l =
-// (row/col c = l & 31, half h = l >> 5) decodes its token's dims
+// (columns).  Lane l = (row/col c = l & 31, half h = l >> 5) decodes its token's dims
 // [64h, 64h + 64) in registers -- x = C[code] + w[d][bucket], the exact FP32
 // values of the decompressed embedding -- from one 16-byte residual load and
 // 16 float4 loads of the centroid row; instruction s, half h covers dim
@@ -1530,79 +1529,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // bit-identical); the max over tokens is local plus one cross-half shuffle
 // and the sum over query tokens runs left to right.
 // ---------------------------------------------------------------------------
-// Per-query-token max over one doc's tokens on FP32 MFMA tiles (see above);
-// wl = the [128][4] decode weights in LDS; TPS tiles in flight per step.
-template <int TPS>
-__device__ __forceinline__ float maxsim_doc_f32mfma(uint32_t L, uint64_t off, uint32_t h, uint32_t col,
-                                                    const uint32_t *__restrict__ codes,
-                                                    const uint8_t *__restrict__ residual,
-                                                    const float *__restrict__ C, const float *wl,
-                                                    const float (&qf)[64], int clip_zero) {
-  float M = clip_zero ? 0.0f : -FLT_MAX;
-  for (uint32_t i0 = 0; i0 < L; i0 += 32 * TPS) {
-    float4 cv[TPS][16];
-    uint4 rb[TPS];
-#pragma unroll
-    for (int u = 0; u < TPS; ++u) {
-      const uint32_t i = i0 + 32 * u + col;
-      rb[u] = make_uint4(0u, 0u, 0u, 0u);
-      if (i < L) {
-        const uint64_t g = off + i;
-        const uint32_t code = codes[g];
-        rb[u] = *reinterpret_cast<const uint4 *>(residual + g * kResBytes + 16 * h);
-        const float4 *cp = reinterpret_cast<const float4 *>(C + (size_t)code * kDim + 64 * h);
-#pragma unroll
-        for (uint32_t s = 0; s < 16; ++s) cv[u][s] = cp[s];
-      } else {
-#pragma unroll
-        for (uint32_t s = 0; s < 16; ++s) cv[u][s] = make_float4(0.f, 0.f, 0.f, 0.f);
-      }
-    }
-    // decode the whole tile first (64 data-dependent LDS reads in flight
-    // together, added into the centroid values in place), then run the
-    // MFMA chain without waiting on LDS between instructions
-    floatx16_t acc[TPS];
-#pragma unroll
-    for (int u = 0; u < TPS; ++u) acc[u] = floatx16_t{};
-#pragma unroll
-    for (uint32_t s = 0; s < 16; ++s) {
-#pragma unroll
-      for (int u = 0; u < TPS; ++u) {
-        const uint32_t rw[4] = {rb[u].x, rb[u].y, rb[u].z, rb[u].w};
-        // residual byte 16h + s holds dims 64h + 4s .. +3, most significant bits first
-        const uint32_t byte = (rw[s >> 2] >> (8 * (s & 3))) & 0xFFu;
-        const float *wd = wl + 4 * (64 * h + 4 * s);  // weights of dim 64h + 4s, 4 per dim
-        const float x0 = cv[u][s].x + wd[(byte >> 6) & 3u];
-        const float x1 = cv[u][s].y + wd[4 + ((byte >> 4) & 3u)];
-        const float x2 = cv[u][s].z + wd[8 + ((byte >> 2) & 3u)];
-        const float x3 = cv[u][s].w + wd[12 + (byte & 3u)];
-        acc[u] = __builtin_amdgcn_mfma_f32_32x32x2f32(x0, qf[4 * s], acc[u], 0, 0, 0);
-        acc[u] = __builtin_amdgcn_mfma_f32_32x32x2f32(x1, qf[4 * s + 1], acc[u], 0, 0, 0);
-        acc[u] = __builtin_amdgcn_mfma_f32_32x32x2f32(x2, qf[4 * s + 2], acc[u], 0, 0, 0);
-        acc[u] = __builtin_amdgcn_mfma_f32_32x32x2f32(x3, qf[4 * s + 3], acc[u], 0, 0, 0);
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < TPS; ++u)
-#pragma unroll
-      for (uint32_t j = 0; j < 16; ++j) {
-        const uint32_t m = 32 * u + 8 * (j >> 2) + 4 * h + (j & 3);
-        if (i0 + m < L) M = fmaxf(M, acc[u][j]);
-      }
-  }
-  return fmaxf(M, __shfl_xor(M, 32));
-}
-
-__device__ __forceinline__ void load_q_f32(const float *__restrict__ Q, uint32_t q, uint32_t t_max,
-                                           uint32_t T, uint32_t h, uint32_t col, float (&qf)[64]) {
-  const float4 *qr = reinterpret_cast<const float4 *>(Q + ((size_t)q * t_max + col) * kDim + 64 * h);
-#pragma unroll
-  for (uint32_t s = 0; s < 16; ++s) {
-    const float4 v = (col < T) ? qr[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-    qf[4 * s] = v.x; qf[4 * s + 1] = v.y; qf[4 * s + 2] = v.z; qf[4 * s + 3] = v.w;
-  }
-}
-
 // FP32 MFMA MaxSim kernel.  The query tile sits in LDS instead of 64
 // registers per lane: ql[col * kQlStride + d] = q[col][d] (zero for col >= T),
 // read as float4 per four MFMAs (rows 528 B apart, so the 16 lanes of a
@@ -1614,9 +1540,172 @@ __device__ __forceinline__ void load_q_f32(const float *__restrict__ Q, uint32_t
 // tile per 32 tokens of one doc).  Row m of a tile is token 8*(m>>3) + (m&7)'s
 // slot: lane (col, h) holds rows 8g + 4h + e (g = j>>2, e = j&3) of acc, so
 // the max over a chunk is over j in [4g, 4g+4) plus one cross-half shuffle.
-// Operands and their order are those of maxsim_doc_f32mfma per token, so the
-// scores are bit-identical to it.
+// The persistent kernel's FP32 phase runs the same loop (f32mfma_wave_docs,
+// kept as a copy: written inline here the launch kernel stays within 128
+// registers, 4 waves per SIMD), so their scores are bit-identical.
 constexpr uint32_t kQlStride = kDim + 4;
+
+// Decode-pair table of the FP32 MFMA MaxSim: tab2[p * 16 + v] = (W[2p][v >> 2],
+// W[2p + 1][v & 3]), the two weights that a 4-bit nibble of residual bits
+// selects for dims 2p, 2p + 1 (256 threads).
+__device__ __forceinline__ void f32mfma_fill_tab(float2 *tab2, const float *__restrict__ W, uint32_t tid) {
+  for (uint32_t i = tid; i < kDim / 2 * 16; i += 256) {
+    const uint32_t pp = i >> 4, v = i & 15u;
+    tab2[i] = make_float2(W[(2 * pp) * 4 + (v >> 2)], W[(2 * pp + 1) * 4 + (v & 3u)]);
+  }
+}
+
+// The query tile: ql[c * kQlStride + d] = Q[q][c][d], zero for c >= T (256 threads).
+__device__ __forceinline__ void f32mfma_load_query(float *ql, const float *__restrict__ Q, uint32_t q,
+                                                   uint32_t t_max, uint32_t T, uint32_t tid) {
+  for (uint32_t i = tid; i < kMaxQTok * kDim / 4; i += 256) {
+    const uint32_t c = i / (kDim / 4), d4 = i % (kDim / 4);
+    const float4 v = c < T ? reinterpret_cast<const float4 *>(Q + ((size_t)q * t_max + c) * kDim)[d4]
+                           : make_float4(0.f, 0.f, 0.f, 0.f);
+    *reinterpret_cast<float4 *>(ql + c * kQlStride + 4 * d4) = v;
+  }
+}
+
+// One wave's share of a query's candidates: slots r = r_first, r_first +
+// stride, ... < n of ids, scores to out[r].  tab2 and ql as filled above; Ms
+// is this wave's [4][kMaxQTok + 1] LDS scratch.
+__device__ __forceinline__ void f32mfma_wave_docs(const float2 *tab2, const float *ql, float (*Ms)[kMaxQTok + 1],
+                                                  uint32_t lane, uint32_t T, const uint32_t *__restrict__ ids,
+                                                  uint32_t n, uint32_t r_first, uint32_t stride,
+                                                  const uint32_t *__restrict__ doclens,
+                                                  const uint64_t *__restrict__ docoff,
+                                                  const uint32_t *__restrict__ codes,
+                                                  const uint8_t *__restrict__ residual,
+                                                  const float *__restrict__ C, int clip_zero,
+                                                  float *__restrict__ out) {
+  const uint32_t h = lane >> 5, col = lane & 31;
+  const float init = clip_zero ? 0.0f : -FLT_MAX;
+
+  // the open doc: slot r, length L, first token off, next chunk at pos (an
+  // empty doc is one chunk without tokens, so its score is the sum of the
+  // initial maxima)
+  uint32_t r = r_first, L = 0, pos = 0;
+  uint64_t off = 0;
+  auto open = [&]() {
+    pos = 0;
+    if (r < n) {
+      const uint32_t doc = ids[r];
+      L = doclens[doc];
+      off = docoff[doc];
+    }
+  };
+  open();
+  float Mrun = init;
+  const uint32_t qoff0 = col * kQlStride + 64 * h;
+  while (r < n) {
+    // the tile's four chunks (wave-uniform): slot, first token, length, last chunk of its doc
+    uint32_t cr[4], cp[4], cl[4];
+    uint64_t co[4];
+    bool cv_[4], last[4];
+#pragma unroll
+    for (uint32_t g = 0; g < 4; ++g) {
+      cv_[g] = r < n;
+      cr[g] = r;
+      cp[g] = pos;
+      cl[g] = L;
+      co[g] = off;
+      last[g] = cv_[g] && pos + 8 >= L;
+      if (cv_[g]) {
+        pos += 8;
+        if (last[g]) {
+          r += stride;
+          open();
+        }
+      }
+    }
+    // this lane's row: chunk col >> 3, token (col & 7) of it
+    const uint32_t g = col >> 3;
+    const uint32_t tok = (g == 0 ? cp[0] : g == 1 ? cp[1] : g == 2 ? cp[2] : cp[3]) + (col & 7);
+    const uint32_t len = g == 0 ? cl[0] : g == 1 ? cl[1] : g == 2 ? cl[2] : cl[3];
+    const uint64_t base = g == 0 ? co[0] : g == 1 ? co[1] : g == 2 ? co[2] : co[3];
+    const bool ok = (g == 0 ? cv_[0] : g == 1 ? cv_[1] : g == 2 ? cv_[2] : cv_[3]) && tok < len;
+    // rows without a token decode centroid 0 with zero residual bits (any
+    // finite values: the max skips those rows), so no register is zeroed
+    uint32_t code = 0;
+    uint4 rb = make_uint4(0u, 0u, 0u, 0u);
+    if (ok) {
+      const uint64_t gt = base + tok;
+      code = codes[gt];
+      rb = *reinterpret_cast<const uint4 *>(residual + gt * kResBytes + 16 * h);
+    }
+    float4 cv[16];
+    const float4 *cptr = reinterpret_cast<const float4 *>(C + (size_t)code * kDim + 64 * h);
+#pragma unroll
+    for (uint32_t s2 = 0; s2 < 16; ++s2) cv[s2] = cptr[s2];
+    // keep the query reads inside the loop (not hoisted into 64 registers)
+    uint32_t qoff = qoff0;
+    asm volatile("" : "+v"(qoff));
+    const float4 *qr = reinterpret_cast<const float4 *>(ql + qoff);
+    // decode the whole tile first (the 32 data-dependent weight-pair reads
+    // in flight together, added into the centroid values in place), then the
+    // MFMA chain with no LDS wait between instructions but the query reads.
+    // Residual byte 16h + s2 holds dims 64h + 4 s2 .. +3, most significant
+    // bits first: its high nibble selects pair 32h + 2 s2, the low one the next.
+    const uint32_t rw[4] = {rb.x, rb.y, rb.z, rb.w};
+    const float2 *tp = tab2 + 32 * 16 * h;
+#pragma unroll
+    for (uint32_t s2 = 0; s2 < 16; ++s2) {
+      const uint32_t sh = 8 * (s2 & 3);
+      const float2 a2 = tp[(2 * s2) * 16 + ((rw[s2 >> 2] >> (sh + 4)) & 15u)];
+      const float2 b2 = tp[(2 * s2 + 1) * 16 + ((rw[s2 >> 2] >> sh) & 15u)];
+      cv[s2].x += a2.x;
+      cv[s2].y += a2.y;
+      cv[s2].z += b2.x;
+      cv[s2].w += b2.y;
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    floatx16_t acc = floatx16_t{};
+#pragma unroll
+    for (uint32_t s2 = 0; s2 < 16; ++s2) {
+      const float4 qv = qr[s2];
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[s2].x, qv.x, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[s2].y, qv.y, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[s2].z, qv.z, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(cv[s2].w, qv.w, acc, 0, 0, 0);
+    }
+    // per chunk: max over its valid rows, folded into the open doc's max;
+    // a doc's last chunk parks its per-token maxima (halves combined) in LDS
+    uint32_t nfin = 0, fin_r[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (uint32_t gg = 0; gg < 4; ++gg) {
+      if (cv_[gg]) {
+        if (cp[gg] == 0) Mrun = init;
+        if (cp[gg] + 8 <= cl[gg]) {
+#pragma unroll
+          for (uint32_t e = 0; e < 4; ++e) Mrun = fmaxf(Mrun, acc[4 * gg + e]);
+        } else {
+#pragma unroll
+          for (uint32_t e = 0; e < 4; ++e)
+            if (cp[gg] + 4 * h + e < cl[gg]) Mrun = fmaxf(Mrun, acc[4 * gg + e]);
+        }
+        if (last[gg]) {
+          const float M2 = fmaxf(Mrun, __shfl_xor(Mrun, 32));
+          if (h == 0) Ms[nfin][col] = M2;
+          fin_r[nfin] = cr[gg];
+          ++nfin;
+        }
+      }
+    }
+    // scores of the docs finished in this tile, one lane each: the
+    // left-to-right sum over query tokens
+    if (nfin) {
+      wave_lds_sync();
+      if (lane < nfin) {
+        const uint32_t rr = lane == 0 ? fin_r[0] : lane == 1 ? fin_r[1] : lane == 2 ? fin_r[2] : fin_r[3];
+        float sc = 0.0f;
+        for (uint32_t tt = 0; tt < T; ++tt) sc += Ms[lane][tt];
+        out[rr] = sc;
+      }
+      wave_lds_sync();
+    }
+  }
+}
+
 
 __global__ __launch_bounds__(256) void k_rerank_f32mfma(
     const float *__restrict__ Q, uint32_t t_max, const uint32_t *__restrict__ q_ntok,
@@ -2052,7 +2141,7 @@ __device__ void block_topk_small(const float *__restrict__ sc, const uint32_t *_
 // (16 doc tokens each in BF16, 32 in FP32).
 // ---------------------------------------------------------------------------
 template <int PREC, int TPS>
-__global__ __launch_bounds__(256) void k_rerank_persist(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(PREC == 3 ? 4 : 1))) void k_rerank_persist(
     const float *__restrict__ Q, uint32_t t_max, const uint32_t *__restrict__ q_ntok,
     const uint32_t *__restrict__ in_ids, const uint32_t *__restrict__ in_n, uint32_t R_in,
     uint32_t nb, uint32_t *__restrict__ queue, const uint32_t *__restrict__ doclens,
@@ -2064,7 +2153,9 @@ __global__ __launch_bounds__(256) void k_rerank_persist(
     float *__restrict__ out_s, uint32_t *__restrict__ out_n, uint32_t out_stride) {
   __shared__ uint32_t tabb[PREC == 1 ? kPairTab : 1];
   __shared__ uint32_t lut8[PREC == 2 ? kResBytes * 256 : 1];
-  __shared__ float wl[PREC == 3 ? kDim * 4 : 1];
+  __shared__ float2 tab2[PREC == 3 ? kDim / 2 * 16 : 1];
+  __shared__ __align__(16) float ql[PREC == 3 ? kMaxQTok * kQlStride : 1];
+  __shared__ float Ms3[PREC == 3 ? 4 : 1][4][kMaxQTok + 1];
   __shared__ float Ms[4][kMaxQTok];
   __shared__ float rsc[4][PREC == 2 ? 32 * TPS : 1];
   __shared__ TopkLds tk;
@@ -2076,9 +2167,8 @@ __global__ __launch_bounds__(256) void k_rerank_persist(
   else if constexpr (PREC == 2)
     for (uint32_t i = tid; i < kResBytes * 256; i += 256) lut8[i] = lut8_g[i];
   else
-    for (uint32_t i = tid; i < kDim * 4; i += 256) wl[i] = W[i];
+    f32mfma_fill_tab(tab2, W, tid);
   bf16x8_t qb[PREC == 1 ? 2 : 1][4];
-  float qf[PREC == 3 ? 64 : 1];
   long q8[PREC == 2 ? 8 : 1];
   float sq = 0.0f;
   for (;;) {
@@ -2089,22 +2179,26 @@ __global__ __launch_bounds__(256) void k_rerank_persist(
     if (q >= nb) break;
     const uint32_t n = min(in_n[q], R_in);
     const uint32_t T = q_ntok ? q_ntok[q] : t_max;
+    float *qs = score + (size_t)q * R_in;
+    if constexpr (PREC == 3) {
+      // the launch kernel's FP32 MFMA loop: query tile in LDS, docs streamed
+      // as 8-token chunks packed four to a tile
+      f32mfma_load_query(ql, Q, q, t_max, T, tid);
+      __syncthreads();
+      f32mfma_wave_docs(tab2, ql, Ms3[wave], lane, T, in_ids + (size_t)q * R_in, n, wave, 4, doclens, docoff,
+                        codes, residual, C, clip_zero, qs);
+    }
     if constexpr (PREC == 1) load_q16_bf16(Q, q, t_max, T, lane, qb);
     else if constexpr (PREC == 2) load_q_fp8(Q, q, t_max, T, h, col, q8, sq);
-    else load_q_f32(Q, q, t_max, T, h, col, qf);
-    float *qs = score + (size_t)q * R_in;
-    for (uint32_t r = wave; r < n; r += 4) {
+    for (uint32_t r = wave; r < (PREC == 3 ? 0u : n); r += 4) {
       const uint32_t doc = in_ids[(size_t)q * R_in + r];
-      float M;
+      float M = 0.0f;
       if constexpr (PREC == 1)
         M = maxsim16_doc_bf16<TPS>(doclens[doc], docoff[doc], lane, codes, residual, Cm, tabb, qb,
                                    clip_zero);
       else if constexpr (PREC == 2)
         M = maxsim_doc_fp8<TPS>(doclens[doc], docoff[doc], h, col, codes, residual, C8, csc, lut8, sr, q8,
                                 sq, rsc[wave], clip_zero);
-      else
-        M = maxsim_doc_f32mfma<TPS>(doclens[doc], docoff[doc], h, col, codes, residual, C, wl, qf,
-                                    clip_zero);
       if (h == 0) Ms[wave][col] = M;
       wave_lds_sync();
       if (lane == 0) {
@@ -3083,8 +3177,7 @@ hipError_t launch_persist(const float *Q, uint32_t t_max, const uint32_t *q_ntok
   else if (precision == 1) k_rerank_persist<1, 2><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
   else if (precision == 2 && cfg.tps == 1) k_rerank_persist<2, 1><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
   else if (precision == 2) k_rerank_persist<2, 2><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
-  else if (precision == 3 && cfg.tps == 1) k_rerank_persist<3, 1><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
-  else if (precision == 3) k_rerank_persist<3, 2><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
+  else if (precision == 3) k_rerank_persist<3, 1><<<grid, 256, 0, s>>>(MVR_PERSIST_ARGS);
   else return hipErrorNotSupported;
 #undef MVR_PERSIST_ARGS
   return hipGetLastError();
@@ -3096,13 +3189,14 @@ uint32_t persist_max_wgs(int precision, uint32_t tps) {
     int o = 0;
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, f, 256, 0) == hipSuccess && o > 0 ? (uint32_t)o : 1u;
   };
+  // (the FP32 kernel streams one packed 32-row tile per wave step: no
+  // second tile setting)
   static const uint32_t t[3][2] = {
       {occ(reinterpret_cast<const void *>(k_rerank_persist<1, 1>)),
        occ(reinterpret_cast<const void *>(k_rerank_persist<1, 2>))},
       {occ(reinterpret_cast<const void *>(k_rerank_persist<2, 1>)),
        occ(reinterpret_cast<const void *>(k_rerank_persist<2, 2>))},
-      {occ(reinterpret_cast<const void *>(k_rerank_persist<3, 1>)),
-       occ(reinterpret_cast<const void *>(k_rerank_persist<3, 2>))}};
+      {occ(reinterpret_cast<const void *>(k_rerank_persist<3, 1>)), 0u}};
   if (precision < 1 || precision > 3 || tps < 1 || tps > 2) return 0;
   return t[precision - 1][tps - 1];
 }

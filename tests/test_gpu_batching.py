@@ -38,14 +38,15 @@ def test_persistent_matches_launch_path_every_setting(c1, prec, rescore):
         try:
             g.set_tuning(prec, wpc, tps)
         except MvrError:
-            assert wpc > 1, "one workgroup per CU is always resident"
+            # one workgroup per CU is always resident; the FP32 kernel has one tile setting
+            assert wpc > 1 or (prec == "fp32_mfma" and tps == 2)
             continue  # more workgroups per CU than that kernel keeps resident
         n_set += 1
         ids, sc = g.search(Q, 8, 256, 10, precision=prec, rescore=rescore, q_ntok=qt, q_nprobe=qp,
                            batch=24, persistent=True)
         np.testing.assert_array_equal(ids, ref_ids, err_msg=f"{prec} wpc={wpc} tps={tps}")
         np.testing.assert_array_equal(sc, ref_sc, err_msg=f"{prec} wpc={wpc} tps={tps}")
-    assert n_set >= 3
+    assert n_set >= 2
     with pytest.raises(MvrError):
         g.set_tuning(prec, 9, 1)
 
