@@ -258,7 +258,7 @@ def main():
                                + ("U{8..32} tokens, nprobe U{2,4,8,16,32}" if q_nt is not None
                                   else f"32 tokens; nprobe {P}") + f", probe_topk {R}, k {k}",
                    "persistent": args.persistent, "graph": args.graph,
-                   "tuning": ({p: ix.get_tuning(p) for p in (args.precision, "fp32_mfma")}
+                   "tuning": ({p: ix.get_tuning_ex(p) for p in (args.precision, "fp32_mfma")}
                               if args.persistent else None),
                    "maxsim_order": args.maxsim_order,
                    "batch_per_pass": batch, "precision": args.precision, "rescore": rescore,

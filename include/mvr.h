@@ -164,7 +164,9 @@ void mvr_index_free(mvr_index *ix);
  * many pairs as the index has docs.  In a persistent search the doc order
  * makes the BF16 phase a work queue of docs followed by a per-query top-R2
  * (or top-k) launch; the query order runs the per-query persistent kernel
- * with its fused top-k.  The FP32 rescore phase is always per query. */
+ * with its fused top-k; the automatic order is the autotuner's choice of the
+ * two (mvr_index_get_tuning_ex).  The FP32 rescore phase is always per
+ * query. */
 enum { MVR_MAXSIM_AUTO = 0, MVR_MAXSIM_QUERY = 1, MVR_MAXSIM_DOC = 2 };
 int mvr_index_set_maxsim_order(mvr_index *ix, uint32_t order);
 int mvr_index_info(const mvr_index *ix, uint32_t *n_doc, uint64_t *n_tok, uint32_t *n_centroid,
@@ -218,6 +220,14 @@ int mvr_rerank_docs_noalloc(mvr_index *ix, const float *Q_dev, uint32_t n_q, uin
  * a setting exists. */
 int mvr_index_set_tuning(mvr_index *ix, uint32_t precision, uint32_t wgs_per_cu, uint32_t tiles);
 int mvr_index_get_tuning(const mvr_index *ix, uint32_t precision, uint32_t out[2]);
+/* The same with the first phase's loop order: out = (MVR_MAXSIM_QUERY or
+ * MVR_MAXSIM_DOC, wgs_per_cu, tiles).  With the automatic MaxSim order
+ * (mvr_index_set_maxsim_order) the autotuner of a persistent BF16 search also
+ * times the doc-major first phase (a work queue of docs, then the per-query
+ * top-R2) against the best per-query setting and keeps the faster; wgs_per_cu
+ * and tiles are then still the best per-query setting, which searches in the
+ * query order use.  No counterpart in the reference. */
+int mvr_index_get_tuning_ex(const mvr_index *ix, uint32_t precision, uint32_t out[3]);
 
 /* FP32 decompression of whole documents: out receives sum(doclens[doc_ids])
  * rows of dim floats, documents in the given order (host buffers). */

@@ -864,12 +864,20 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   // as the index has docs, i.e. about 1.6 or more pairs per distinct doc;
   // query-major below that, where most docs would be loaded for one query.
   // In a persistent search the doc-major kernel is the first phase's work
-  // queue (of docs), followed by the per-query top-R2 (or top-k).
-  const bool doc_major = prm->precision == MVR_BF16 && (uint64_t)Bc * R <= UINT32_MAX &&
+  // queue (of docs), followed by the per-query top-R2 (or top-k); with the
+  // automatic order the autotuner times it against the per-query persistent
+  // kernel and keeps the faster (PersistCfg::dm).
+  const bool dm_ok = prm->precision == MVR_BF16 && (uint64_t)Bc * R <= UINT32_MAX;
+  const bool doc_major = dm_ok && !persistent &&
                          (ix->maxsim_order == MVR_MAXSIM_DOC ||
                           (ix->maxsim_order == MVR_MAXSIM_AUTO && (uint64_t)Bc * R >= d.n_doc));
+  const bool p_dm_fixed = persistent && dm_ok && ix->maxsim_order == MVR_MAXSIM_DOC;
+  const bool p_dm_tuned = persistent && dm_ok && ix->maxsim_order == MVR_MAXSIM_AUTO;
+  // the persistent first phase runs doc-major (read when the pass runs: the
+  // autotuner changes the setting between its timed passes)
+  auto persist_dm = [&]() { return p_dm_fixed || (p_dm_tuned && ix->tune[prm->precision].dm); };
   DocMajorBufs dmb;
-  if (doc_major) {
+  if (doc_major || p_dm_fixed || p_dm_tuned) {
     const size_t nblk = (d.n_doc + kDmDocsPerBlock - 1) / kDmDocsPerBlock;
     HIP_TRY(ws.dm_cnt.ensure((size_t)d.n_doc * 4));
     HIP_TRY(ws.dm_work.ensure((size_t)d.n_doc * 8));
@@ -893,9 +901,9 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   // host-resident residuals: two passes in flight, the staging copy of one (on
   // a second stream) overlapping the device work of the other; the selection
   // outputs and the query buffer then have one slot per pass in flight
-  // persistent settings still to tune: the first phase's (unless it runs
+  // persistent settings still to tune: the first phase's (unless it is fixed
   // doc-major) and, with a rescore, the FP32 phase's
-  const bool tune_p1 = persistent && !doc_major && !ix->tuned[prm->precision];
+  const bool tune_p1 = persistent && !p_dm_fixed && !ix->tuned[prm->precision];
   const bool tune_p2 = persistent && R2 && prm->precision != MVR_FP32_MFMA && !ix->tuned[MVR_FP32_MFMA];
   const bool pipelined = ix->res_host && !stages && n_pass > 1 && !tune_p1 && !tune_p2;
   const size_t n_slot = pipelined ? 2 : 1;
@@ -1001,7 +1009,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     const uint32_t *sid = hp ? hp->ids : sl.ids;
     const DevIndex &dx = hp ? hp->d : d;
     const int clip = (int)prm->clip_zero;
-    if (persistent && doc_major) {
+    if (persistent && persist_dm()) {
       // phase 1: doc-major BF16 MaxSim of all candidates (a work queue of
       // docs), then each query's top-R2 -- or its top-k without a rescore
       HIP_TRY(launch_rerank_bf16_dm(Qb, t_max, bnt, nb, sid, sl.n, R, dx, clip, dmb, sl.score, st));
@@ -1224,28 +1232,42 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
         hipEvent_t e0, e1;
         HIP_TRY(hipEventCreate(&e0));
         HIP_TRY(hipEventCreate(&e1));
+        auto time_pass = [&](const PersistCfg &c, uint32_t prec, float &ms) -> int {
+          ix->tune[prec] = c;
+          for (int rep = 0; rep < 2; ++rep) {
+            HIP_TRY(hipEventRecord(e0, s));
+            const int rc = run_pass(s, Qb, rm, n_rows, bnt, bnp, nb, oid, os, bi, false);
+            if (rc != MVR_OK) return rc;
+            HIP_TRY(hipEventRecord(e1, s));
+            HIP_TRY(hipEventSynchronize(e1));
+            HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
+          }
+          return MVR_OK;
+        };
         auto tune_one = [&](uint32_t prec) -> int {
-          float best = 1e30f;
+          float best = 1e30f, ms = 0.0f;
           PersistCfg best_c;
           for (uint32_t tps = 1; tps <= 2; ++tps)
-            for (uint32_t wpc = 1; wpc <= persist_max_wgs((int)prec, tps); ++wpc)
-              for (int rep = 0; rep < 2; ++rep) {
-                PersistCfg c;
-                c.n_wg = wpc * d.n_cu;
-                c.tps = tps;
-                ix->tune[prec] = c;
-                HIP_TRY(hipEventRecord(e0, s));
-                const int rc = run_pass(s, Qb, rm, n_rows, bnt, bnp, nb, oid, os, bi, false);
-                if (rc != MVR_OK) return rc;
-                HIP_TRY(hipEventRecord(e1, s));
-                HIP_TRY(hipEventSynchronize(e1));
-                float ms = 0.0f;
-                HIP_TRY(hipEventElapsedTime(&ms, e0, e1));
-                if (rep == 1 && ms < best) {
-                  best = ms;
-                  best_c = c;
-                }
+            for (uint32_t wpc = 1; wpc <= persist_max_wgs((int)prec, tps); ++wpc) {
+              PersistCfg c;
+              c.n_wg = wpc * d.n_cu;
+              c.tps = tps;
+              const int rc = time_pass(c, prec, ms);
+              if (rc != MVR_OK) return rc;
+              if (ms < best) {
+                best = ms;
+                best_c = c;
               }
+            }
+          // the doc-major first phase against the best per-query setting
+          // (which stays recorded for searches in the query order)
+          if (prec == prm->precision && p_dm_tuned) {
+            PersistCfg c = best_c;
+            c.dm = true;
+            const int rc = time_pass(c, prec, ms);
+            if (rc != MVR_OK) return rc;
+            if (ms < best) best_c = c;
+          }
           ix->tune[prec] = best_c;
           ix->tuned[prec] = true;
           return MVR_OK;
@@ -1272,7 +1294,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
         const PersistCfg &tc = ix->tune[prm->precision];
         const std::vector<uint64_t> key = {nb, n_rows, t_max, P, R, R2, k, prm->precision, prm->clip_zero,
                                            uniform ? 1u : 0u, persistent ? 1u : 0u, fast_ct ? 1u : 0u,
-                                           tc.n_wg, tc.tps};
+                                           tc.n_wg, tc.tps, tc.dm ? 1u : 0u};
         hipGraphExec_t exec = nullptr;
         for (size_t gi = 0; gi < ws.graphs.size(); ++gi)
           if (ws.graphs[gi].key == key) {
@@ -1394,8 +1416,17 @@ int mvr_index_set_tuning(mvr_index *ix, uint32_t precision, uint32_t wgs_per_cu,
   PersistCfg c;
   c.n_wg = wgs_per_cu * ix->d.n_cu;
   c.tps = tiles;
-  ix->tune[precision] = c;
+  ix->tune[precision] = c;  // the per-query persistent kernel (dm = false)
   ix->tuned[precision] = true;
+  return MVR_OK;
+}
+
+int mvr_index_get_tuning_ex(const mvr_index *ix, uint32_t precision, uint32_t out[3]) {
+  if (!ix || !out) return fail(MVR_ERR_INVALID, "NULL argument");
+  if (precision > 3 || !ix->tuned[precision]) return fail(MVR_ERR_INVALID, "precision %u not tuned", precision);
+  out[0] = ix->tune[precision].dm ? MVR_MAXSIM_DOC : MVR_MAXSIM_QUERY;
+  out[1] = ix->tune[precision].n_wg / std::max(1u, ix->d.n_cu);
+  out[2] = ix->tune[precision].tps;
   return MVR_OK;
 }
 

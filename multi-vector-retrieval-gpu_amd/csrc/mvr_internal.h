@@ -96,7 +96,8 @@ hipError_t launch_select(const uint64_t *cand_keys, const uint32_t *seg_cnt, uin
 // precision 1 (BF16) or 3 (FP32 MFMA).  queue [1] is scratch.
 struct PersistCfg {
   uint32_t n_wg = 0;   // workgroups in the grid
-  uint32_t tps = 1;    // 32-token tiles in flight per wave
+  uint32_t tps = 1;    // doc tiles in flight per wave
+  bool dm = false;     // BF16 first phase doc-major (a work queue of docs), not per query
 };
 constexpr uint32_t kPersistTopkMax = 256;
 // resident workgroups per CU of the persistent kernel for (precision, tiles

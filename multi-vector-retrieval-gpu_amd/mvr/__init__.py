@@ -175,6 +175,16 @@ class Index:
             return None
         return tuple(int(v) for v in out)
 
+    def get_tuning_ex(self, precision):
+        """{"order": "query"|"doc", "wgs_per_cu": w, "tiles": t} (the first
+        phase's loop order the autotuner chose, and the per-query kernel's
+        setting) or None before tuning."""
+        prec = _PRECISION.get(precision, precision) if isinstance(precision, str) else precision
+        out = (C.c_uint32 * 3)()
+        if lib().mvr_index_get_tuning_ex(self._h, prec, out) != 0:
+            return None
+        return {"order": {1: "query", 2: "doc"}[int(out[0])], "wgs_per_cu": int(out[1]), "tiles": int(out[2])}
+
     def search(self, Q, nprobe: int, probe_topk: int, topk: int, precision="fp32", rescore: int = 0,
                q_ntok=None, q_nprobe=None, clip_zero=False, batch: int = 0, stages=False,
                timings=False, learned=False, persistent=False, graph=False):

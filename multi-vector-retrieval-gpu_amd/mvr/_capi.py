@@ -89,6 +89,7 @@ SIGNATURES = {
     "mvr_decompress": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "mvr_index_set_tuning": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32]),
     "mvr_index_get_tuning": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "mvr_index_get_tuning_ex": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "mvr_rerank_docs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
                                   C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p]),
     "mvr_rerank_docs_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p,
