@@ -160,8 +160,8 @@ void mvr_index_free(mvr_index *ix);
  * give bit-identical scores.  MVR_MAXSIM_QUERY: per query, its candidate docs
  * in turn.  MVR_MAXSIM_DOC: the pass's (query, doc) pairs grouped by doc, each
  * doc's tokens loaded once for all the queries that selected it.
- * MVR_MAXSIM_AUTO (default): doc order when a device pass holds at least as
- * many pairs as the index has docs.  In a persistent search the doc order
+ * MVR_MAXSIM_AUTO (default): doc order when a device pass holds at least a
+ * quarter as many pairs as the index has docs.  In a persistent search the doc order
  * makes the BF16 phase a work queue of docs followed by a per-query top-R2
  * (or top-k) launch; the query order runs the per-query persistent kernel
  * with its fused top-k; the automatic order is the autotuner's choice of the
