@@ -859,7 +859,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     return fail(MVR_ERR_UNSUPPORTED, "the persistent kernel selects at most %u (topk %u, rescore %u)",
                 kPersistTopkMax, k, R2);
   const bool persistent = prm->persistent != 0;
-  // BF16 MaxSim doc-major (each selected doc's tiles loaded once per pass for
+  // BF16 and FP8 MaxSim doc-major (each selected doc's tiles loaded once per pass for
   // all of its queries) when a pass holds at least a quarter as many (query,
   // doc) pairs as the index has docs; query-major below that, where the
   // grouping's passes over every doc cost more than the sharing saves.
@@ -869,7 +869,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   // queue (of docs), followed by the per-query top-R2 (or top-k); with the
   // automatic order the autotuner times it against the per-query persistent
   // kernel and keeps the faster (PersistCfg::dm).
-  const bool dm_ok = prm->precision == MVR_BF16 && (uint64_t)Bc * R <= UINT32_MAX;
+  const bool dm_ok = (prm->precision == MVR_BF16 || prm->precision == MVR_FP8) && (uint64_t)Bc * R <= UINT32_MAX;
   const bool doc_major = dm_ok && !persistent &&
                          (ix->maxsim_order == MVR_MAXSIM_DOC ||
                           (ix->maxsim_order == MVR_MAXSIM_AUTO && (uint64_t)Bc * R * 4 >= d.n_doc));
@@ -1011,10 +1011,15 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
     const uint32_t *sid = hp ? hp->ids : sl.ids;
     const DevIndex &dx = hp ? hp->d : d;
     const int clip = (int)prm->clip_zero;
+    auto rerank_dm = [&]() -> hipError_t {
+      return prm->precision == MVR_FP8
+                 ? launch_rerank_fp8_dm(Qb, t_max, bnt, nb, sid, sl.n, R, dx, clip, dmb, sl.score, st)
+                 : launch_rerank_bf16_dm(Qb, t_max, bnt, nb, sid, sl.n, R, dx, clip, dmb, sl.score, st);
+    };
     if (persistent && persist_dm()) {
-      // phase 1: doc-major BF16 MaxSim of all candidates (a work queue of
+      // phase 1: doc-major BF16/FP8 MaxSim of all candidates (a work queue of
       // docs), then each query's top-R2 -- or its top-k without a rescore
-      HIP_TRY(launch_rerank_bf16_dm(Qb, t_max, bnt, nb, sid, sl.n, R, dx, clip, dmb, sl.score, st));
+      HIP_TRY(rerank_dm());
       if (R2) {
         HIP_TRY(launch_topk(sid, sl.score, sl.n, R, nb, R2, ws.res_ids.as<uint32_t>(), ws.res_s.as<float>(),
                             ws.res_n.as<uint32_t>(), st));
@@ -1062,7 +1067,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       return MVR_OK;
     }
     if (doc_major)
-      HIP_TRY(launch_rerank_bf16_dm(Qb, t_max, bnt, nb, sid, sl.n, R, dx, clip, dmb, sl.score, st));
+      HIP_TRY(rerank_dm());
     else
       HIP_TRY(launch_rerank(Qb, t_max, bnt, nb, sid, sl.n, R, dx, prm->precision, clip, sl.score, st));
     HIP_TRY(mark(4));

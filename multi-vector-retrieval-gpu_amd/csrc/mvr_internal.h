@@ -131,7 +131,8 @@ struct DocMajorBufs {
   uint32_t *b_p = nullptr;    // [ceil(n_doc / 4096)] pairs per block of docs, then their prefix
   uint32_t *b_d = nullptr;    // [ceil(n_doc / 4096)] docs with pairs per block, then their prefix
   uint32_t *tot = nullptr;    // [3] pairs, work items, work-queue counter
-  uint4 *Qm = nullptr;        // [n_q][512] bf16 query operands in MFMA lane order
+  uint4 *Qm = nullptr;        // [n_q][512] bf16 query operands in MFMA lane order, or the
+                              // FP8 ones ([n_q][256]) followed by their token scales
 };
 constexpr uint32_t kDmDocsPerBlock = 4096;
 // Same scores, bit for bit, as launch_rerank with precision 1 (BF16).
@@ -139,6 +140,13 @@ hipError_t launch_rerank_bf16_dm(const float *Q, uint32_t t_max, const uint32_t 
                                  const uint32_t *sel_ids, const uint32_t *sel_n, uint32_t R,
                                  const DevIndex &ix, int clip_zero, const DocMajorBufs &b,
                                  float *sel_score, hipStream_t s);
+
+// Doc-major FP8 MaxSim: same scores, bit for bit, as launch_rerank with
+// precision 2 (FP8); b.Qm must hold n_q * (4096 + 128) bytes.
+hipError_t launch_rerank_fp8_dm(const float *Q, uint32_t t_max, const uint32_t *q_ntok, uint32_t n_q,
+                                const uint32_t *sel_ids, const uint32_t *sel_n, uint32_t R,
+                                const DevIndex &ix, int clip_zero, const DocMajorBufs &b,
+                                float *sel_score, hipStream_t s);
 
 // Per-pass compaction buffers of a host-resident-residual index (device).
 struct HostStage {

@@ -2007,6 +2007,194 @@ __global__ __launch_bounds__(256) void k_rerank_fp8(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Doc-major FP8 MaxSim: the pass's pairs grouped by doc as for the doc-major
+// BF16 kernel, each doc's 32-token FP8 tiles loaded once per pass for every
+// query that selected it.  Per pass, k_q_fp8m encodes the queries once: the
+// B operand of lane (col, h) -- qf8[s] of load_q_fp8 -- in MFMA lane order
+// (four coalesced 1 KB loads per pair, half the BF16 operand) and each
+// token's scale.  A wave holds up to kDm8Tiles tiles of a doc in registers
+// (centroid bytes, residual operands decoded once per chunk through the LDS
+// table, row scales in LDS) and runs, for each pair of the doc, the
+// per-query kernel's MFMA sequence and combine s_q (s_c acc_C + s_r acc_R)
+// on the same operands: scores are bit-identical to k_rerank_fp8.  Longer
+// docs go chunk by chunk over groups of up to kDmGroup pairs, the maxima
+// combined in LDS.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kDm8Tiles = 4;  // 32-token FP8 tiles held in registers (128 tokens)
+
+// Qm8[(q * 4 + k) * 64 + lane] = (qf8[2k], qf8[2k + 1]) of lane (col, h) and
+// sqm[q * 32 + col] = s_q of token col; one wave per query.
+__global__ __launch_bounds__(256) void k_q_fp8m(const float *__restrict__ Q, uint32_t t_max,
+                                                const uint32_t *__restrict__ q_ntok, uint32_t nb,
+                                                uint4 *__restrict__ Qm8, float *__restrict__ sqm) {
+  const uint32_t q = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (q >= nb) return;  // whole waves
+  const uint32_t h = lane >> 5, col = lane & 31;
+  const uint32_t T = q_ntok ? q_ntok[q] : t_max;
+  long qf8[8];
+  float sq;
+  load_q_fp8(Q, q, t_max, T, h, col, qf8, sq);
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint64_t a = (uint64_t)qf8[2 * k], b = (uint64_t)qf8[2 * k + 1];
+    Qm8[((size_t)q * 4 + k) * 64 + lane] = make_uint4((uint32_t)a, (uint32_t)(a >> 32), (uint32_t)b,
+                                                      (uint32_t)(b >> 32));
+  }
+  if (h == 0) sqm[(size_t)q * 32 + col] = sq;
+}
+
+// One chunk of NT tiles (rows 32 t0 ..) of a doc against the ng pairs pg ..;
+// per-token maxima into mg[j] (combined with earlier chunks unless t0 = 0).
+template <uint32_t NT>
+__device__ __forceinline__ void dm8_chunk(const uint4 *__restrict__ Qm8, const float *__restrict__ sqm, uint32_t R,
+                                          const uint32_t *__restrict__ plist, uint32_t pg, uint32_t ng, uint32_t L,
+                                          uint64_t off, uint32_t t0, const uint32_t *__restrict__ codes,
+                                          const uint8_t *__restrict__ residual, const uint8_t *__restrict__ C8,
+                                          const float *__restrict__ csc, const uint32_t *lut8, float sr, float init,
+                                          uint32_t lane, float *rsc, float (*mg)[kMaxQTok + 1]) {
+  const uint32_t h = lane >> 5, col = lane & 31;
+  uint32_t cd[NT];
+#pragma unroll
+  for (uint32_t u = 0; u < NT; ++u) cd[u] = codes[off + min(32 * (t0 + u) + col, L - 1)];
+  uint4 cv[NT][4];
+  uint4 rb[NT];
+  float scl[NT];
+#pragma unroll
+  for (uint32_t u = 0; u < NT; ++u) {
+    const uint64_t g = off + min(32 * (t0 + u) + col, L - 1);
+    rb[u] = *reinterpret_cast<const uint4 *>(residual + g * kResBytes + 16 * h);
+    const uint4 *cp = reinterpret_cast<const uint4 *>(C8 + (size_t)cd[u] * kDim + 64 * h);
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) cv[u][k] = cp[k];
+    scl[u] = csc[cd[u]];
+  }
+  // row scales of the chunk for the combine (rows are lane-local in the
+  // MFMA input, spread over lanes in its output)
+  if (h == 0) {
+#pragma unroll
+    for (uint32_t u = 0; u < NT; ++u) rsc[32 * u + col] = scl[u];
+  }
+  // residual operands, decoded once for all pairs
+  long rv[NT][8];
+#pragma unroll
+  for (uint32_t u = 0; u < NT; ++u) {
+    const uint32_t rw[4] = {rb[u].x, rb[u].y, rb[u].z, rb[u].w};
+#pragma unroll
+    for (uint32_t s = 0; s < 8; ++s) {
+      const uint32_t word = rw[s >> 1];
+      const uint32_t b0 = (word >> (16 * (s & 1))) & 0xFFu;
+      const uint32_t b1 = (word >> (16 * (s & 1) + 8)) & 0xFFu;
+      const uint32_t pos = 16 * h + 2 * s;
+      rv[u][s] = (long)(((uint64_t)lut8[(pos + 1) * 256 + b1] << 32) | lut8[pos * 256 + b0]);
+    }
+  }
+  wave_lds_sync();
+  // the next pair's query operand and scale are loaded while the current
+  // pair's MFMA chains run (17 registers; the load latency is not exposed)
+  uint4 qn[4];
+  float sqn;
+  auto load_pair = [&](uint32_t j) {
+    const uint32_t q = plist[pg + j] / R;
+    const uint4 *qp = Qm8 + (size_t)q * 4 * 64 + lane;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) qn[k] = qp[64 * k];
+    sqn = sqm[(size_t)q * 32 + col];
+  };
+  load_pair(0);
+  for (uint32_t j = 0; j < ng; ++j) {
+    long qf8[8];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; ++k) {
+      qf8[2 * k] = (long)(((uint64_t)qn[k].y << 32) | qn[k].x);
+      qf8[2 * k + 1] = (long)(((uint64_t)qn[k].w << 32) | qn[k].z);
+    }
+    const float sq = sqn;
+    if (j + 1 < ng) load_pair(j + 1);
+    float M = init;
+#pragma unroll
+    for (uint32_t u = 0; u < NT; ++u) {
+      floatx16_t accC = floatx16_t{}, accR = floatx16_t{};
+#pragma unroll
+      for (uint32_t s = 0; s < 8; ++s) {
+        const uint4 c4 = cv[u][s >> 1];
+        const long a = (long)((s & 1) ? (((uint64_t)c4.w << 32) | c4.z) : (((uint64_t)c4.y << 32) | c4.x));
+        accC = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(a, qf8[s], accC, 0, 0, 0);
+        accR = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8(rv[u][s], qf8[s], accR, 0, 0, 0);
+      }
+      const int valid = (int)L - (int)(32 * (t0 + u));  // rows of this tile that hold tokens
+#pragma unroll
+      for (uint32_t e = 0; e < 16; ++e) {
+        const uint32_t m = 8 * (e >> 2) + 4 * h + (e & 3);
+        if ((int)m < valid) M = fmaxf(M, sq * (rsc[32 * u + m] * accC[e] + sr * accR[e]));
+      }
+    }
+    M = fmaxf(M, __shfl_xor(M, 32));
+    if (h == 0) mg[j][col] = t0 != 0 ? fmaxf(M, mg[j][col]) : M;
+  }
+  wave_lds_sync();  // rsc is rewritten by the next chunk
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_rerank_fp8_dm(
+    const uint4 *__restrict__ Qm8, const float *__restrict__ sqm, uint32_t t_max,
+    const uint32_t *__restrict__ q_ntok, uint32_t R, const uint2 *__restrict__ work, uint32_t *__restrict__ tot,
+    const uint32_t *__restrict__ pend, const uint32_t *__restrict__ plist, const uint32_t *__restrict__ doclens,
+    const uint64_t *__restrict__ docoff, const uint32_t *__restrict__ codes,
+    const uint8_t *__restrict__ residual, const uint8_t *__restrict__ C8, const float *__restrict__ csc,
+    const uint32_t *__restrict__ lut_g, float sr, int clip_zero, float *__restrict__ sel_score) {
+  static_assert(kDm8Tiles == 4, "dm8_chunk is dispatched for 1 .. 4 tiles");
+  __shared__ uint32_t lut[kResBytes * 256];
+  __shared__ float rs[4][32 * kDm8Tiles];
+  __shared__ float Mg[4][kDmGroup][kMaxQTok + 1];
+  const uint32_t tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  for (uint32_t i = tid; i < kResBytes * 256; i += 256) lut[i] = lut_g[i];
+  __syncthreads();
+  float(*mg)[kMaxQTok + 1] = Mg[wave];
+  const uint32_t n_work = tot[1];
+  const float init = clip_zero ? 0.0f : -FLT_MAX;
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&tot[2], kDmGrab);
+    base = __builtin_amdgcn_readfirstlane(base);
+    if (base >= n_work) break;
+    const uint32_t w_end = min(base + kDmGrab, n_work);
+    for (uint32_t wi = base; wi < w_end; ++wi) {
+      const uint2 it = work[wi];
+      const uint32_t doc = it.x, p0 = it.y, p1 = pend[doc];
+      const uint32_t L = doclens[doc];
+      const uint64_t off = docoff[doc];
+      const uint32_t n_t = (L + 31) >> 5;
+      for (uint32_t pg = p0; pg < p1; pg += kDmGroup) {
+        const uint32_t ng = min(kDmGroup, p1 - pg);
+        if (n_t == 0 && lane < 32)  // an empty doc: the identity for every token
+          for (uint32_t j = 0; j < ng; ++j) mg[j][lane] = init;
+        for (uint32_t t0 = 0; t0 < n_t; t0 += kDm8Tiles) {
+#define MVR_DM8_CHUNK(N)                                                                                   \
+  dm8_chunk<N>(Qm8, sqm, R, plist, pg, ng, L, off, t0, codes, residual, C8, csc, lut, sr, init, lane, rs[wave], \
+               mg)
+          switch (min(kDm8Tiles, n_t - t0)) {
+            case 1: MVR_DM8_CHUNK(1); break;
+            case 2: MVR_DM8_CHUNK(2); break;
+            case 3: MVR_DM8_CHUNK(3); break;
+            default: MVR_DM8_CHUNK(4); break;
+          }
+#undef MVR_DM8_CHUNK
+        }
+        // scores of the group: lane j sums pair j's maxima left to right
+        wave_lds_sync();
+        if (lane < ng) {
+          const uint32_t pq = plist[pg + lane], q = pq / R;
+          const uint32_t T = q_ntok ? q_ntok[q] : t_max;
+          float sc = 0.0f;
+          for (uint32_t tt = 0; tt < T; ++tt) sc += mg[lane][tt];
+          sel_score[pq] = sc;
+        }
+        wave_lds_sync();
+      }
+    }
+  }
+}
+
 // FP8 copies of the centroids (one wave per centroid) and of the residual
 // weight table.
 __global__ void k_to_fp8_rows(const float *__restrict__ C, uint32_t n_c, uint8_t *__restrict__ C8,
@@ -3152,6 +3340,37 @@ hipError_t launch_rerank_bf16_dm(const float *Q, uint32_t t_max, const uint32_t 
   k_rerank_bf16_dm<<<grid, 256, 0, s>>>(b.Qm, t_max, q_ntok, R, b.work, b.tot, b.cnt, b.plist, ix.doclens,
                                         ix.docoff, ix.codes, ix.residual, ix.C_bf16m, ix.W, clip_zero,
                                         sel_score);
+  return hipGetLastError();
+}
+
+hipError_t launch_rerank_fp8_dm(const float *Q, uint32_t t_max, const uint32_t *q_ntok, uint32_t n_q,
+                                const uint32_t *sel_ids, const uint32_t *sel_n, uint32_t R,
+                                const DevIndex &ix, int clip_zero, const DocMajorBufs &b,
+                                float *sel_score, hipStream_t s) {
+  if (n_q == 0 || ix.n_doc == 0) return hipSuccess;
+  if (ix.n_bit != 2) return hipErrorNotSupported;
+  if (!ix.C_fp8 || !ix.lut_fp8 || (uint64_t)n_q * R > UINT32_MAX) return hipErrorInvalidValue;
+  const uint32_t nblk = (ix.n_doc + kDmBlockDocs - 1) / kDmBlockDocs;
+  static const int occ = [] {
+    int o = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, k_rerank_fp8_dm, 256, 0) != hipSuccess || o < 1) o = 1;
+    return o;
+  }();
+  // b.Qm: the FP8 operands (n_q x 4 KB), then the token scales (n_q x 128 B)
+  uint4 *Qm8 = b.Qm;
+  float *sqm = reinterpret_cast<float *>(b.Qm + (size_t)n_q * 256);
+  k_fill_u32<<<(ix.n_doc + 255) / 256, 256, 0, s>>>(b.cnt, 0u, ix.n_doc);
+  k_q_fp8m<<<(n_q + 3) / 4, 256, 0, s>>>(Q, t_max, q_ntok, n_q, Qm8, sqm);
+  const dim3 gq(std::min((R + 255) / 256, 16u), n_q);
+  k_dm_count<<<gq, 256, 0, s>>>(sel_ids, sel_n, R, b.cnt);
+  k_dm_bsum<<<nblk, 256, 0, s>>>(b.cnt, ix.n_doc, b.b_p, b.b_d);
+  k_dm_bscan<<<1, 1024, 0, s>>>(b.b_p, b.b_d, nblk, b.tot);
+  k_dm_offsets<<<nblk, 256, 0, s>>>(b.cnt, ix.n_doc, b.b_p, b.b_d, b.work);
+  k_dm_scatter<<<gq, 256, 0, s>>>(sel_ids, sel_n, R, b.cnt, b.plist);
+  const uint32_t grid = std::max(1u, ix.n_cu) * (uint32_t)occ;
+  k_rerank_fp8_dm<<<grid, 256, 0, s>>>(Qm8, sqm, t_max, q_ntok, R, b.work, b.tot, b.cnt, b.plist, ix.doclens,
+                                       ix.docoff, ix.codes, ix.residual, ix.C_fp8, ix.C_fp8_scale, ix.lut_fp8,
+                                       ix.fp8_res_scale, clip_zero, sel_score);
   return hipGetLastError();
 }
 

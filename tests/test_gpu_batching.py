@@ -73,10 +73,11 @@ def test_autotuner_picks_a_valid_setting(small):
     np.testing.assert_array_equal(got[1], ref[1])
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
 @pytest.mark.parametrize("order", ["doc", "auto"])
 @pytest.mark.parametrize("rescore", [0, 64])
-def test_persistent_doc_major_first_phase(c1, rescore, order):
-    """The persistent BF16 phase doc-major (a work queue of docs, then the
+def test_persistent_doc_major_first_phase(c1, rescore, order, prec):
+    """The persistent BF16/FP8 phase doc-major (a work queue of docs, then the
     per-query top-R2 or top-k), forced or as the autotuner's choice: same
     results as the launch path, eagerly and replayed from a graph.  Forced,
     only the FP32 phase has a persistent setting to tune; automatic, the
@@ -85,16 +86,16 @@ def test_persistent_doc_major_first_phase(c1, rescore, order):
     g = _index(ix)
     g.set_maxsim_order(order)
     qt, qp = _ragged(Q, 3)
-    kw = dict(precision="bf16", rescore=rescore, q_ntok=qt, q_nprobe=qp)
+    kw = dict(precision=prec, rescore=rescore, q_ntok=qt, q_nprobe=qp)
     ref_ids, ref_sc = g.search(Q, 8, 256, 10, **kw)
     for graph in (False, True, True, True):
         ids, sc = g.search(Q, 8, 256, 10, persistent=True, graph=graph, **kw)
         np.testing.assert_array_equal(ids, ref_ids, err_msg=f"graph={graph}")
         np.testing.assert_array_equal(sc, ref_sc, err_msg=f"graph={graph}")
     if order == "doc":
-        assert g.get_tuning("bf16") is None
+        assert g.get_tuning(prec) is None
     else:
-        t = g.get_tuning_ex("bf16")
+        t = g.get_tuning_ex(prec)
         assert t["order"] in ("query", "doc") and t["wgs_per_cu"] >= 1 and t["tiles"] in (1, 2)
     assert (g.get_tuning("fp32_mfma") is not None) == (rescore > 0)
     # the query order uses the per-query kernel whatever the tuner chose
