@@ -206,10 +206,18 @@ def main():
     achieved_tf = flops / (avg_launch_ms * 1e-3) / 1e12 if avg_launch_ms > 0 else None
     kname = {"fp32": "k_rerank_f32", "fp32_mfma": "k_rerank_f32mfma"}.get(args.precision,
                                                                        f"k_rerank_{args.precision}")
-    # the library's rule for the BF16 loop order (mvr_index_set_maxsim_order)
-    if args.precision == "bf16" and not args.persistent and (
-            args.maxsim_order == "doc" or (args.maxsim_order == "auto" and batch * R >= cfg["n_doc"])):
-        kname = "k_rerank_bf16_dm"
+    # the library's rule for the BF16/FP8 loop order (mvr_index_set_maxsim_order:
+    # doc-major from a quarter as many pairs per pass as docs); persistent:
+    # the autotuner's choice of the first phase
+    if args.precision in ("bf16", "fp8"):
+        if args.persistent:
+            t = ix.get_tuning_ex(args.precision)
+            if args.maxsim_order == "doc" or (args.maxsim_order == "auto" and t and t["order"] == "doc"):
+                kname = f"k_rerank_{args.precision}_dm"
+            else:
+                kname = "k_rerank_persist"
+        elif args.maxsim_order == "doc" or (args.maxsim_order == "auto" and min(batch, n_q) * R * 4 >= cfg["n_doc"]):
+            kname = f"k_rerank_{args.precision}_dm"
     traffic = measured_traffic(kname, args.config, args.precision)
 
     # the plain FP32 pipelines on the same batch (the reference of the
@@ -276,7 +284,9 @@ def main():
         "recall": recall,
         "pipelines": {p: {"qps": round(r["qps"], 1), "ms_per_step": round(r["ms_per_step"], 3),
                           "speedup_of_headline": round(qps / r["qps"], 3),
-                          "stages_ms_per_step": {f: round(v, 3) for f, v in r["stages"].items()}}
+                          "stages_ms_per_step": {f: round(v, 3) for f, v in r["stages"].items()},
+                          # the same figures for this pipeline's MaxSim kernel
+                          "roofline": pipeline_roofline(r, p, args.config)}
                       for p, r in others.items()},
         "stages_ms_per_step": {f: round(stages[f], 3) for f in stage_keys},
         "candidates": cands,
@@ -723,6 +733,20 @@ def roofline(kname, achieved_tf, peak_tf, bytes_alg, avg_launch_ms, n_launch, to
             "doc_tokens_per_launch": int(tok_per_launch), "flop_per_launch": flops,
             "flop_per_doc_token_and_query_token": 2 * 128, "alg_bytes_per_doc_token": 36,
             "measured_GBs": round(traffic / (avg_launch_ms * 1e-3) / 1e9, 1) if traffic else None}
+
+
+def pipeline_roofline(r, precision, config):
+    """roofline() of a pipeline's MaxSim kernel from its own timed run."""
+    tm, n_launch = r["tm"], r["n_launch"]
+    if not n_launch or not tm or not tm["n_pass"]:
+        return None
+    avg_ms = r["acc"]["rerank_ms"] / n_launch
+    tok = tm["rerank_tokens"] / tm["n_pass"]
+    flops = 2.0 * 128 * tm["rerank_pairs"] / tm["n_pass"]
+    kname = {"fp32": "k_rerank_f32", "fp32_mfma": "k_rerank_f32mfma"}[precision]
+    achieved = flops / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else None
+    return roofline(kname, achieved, PEAK_TFLOPS[precision], 36.0 * tok, avg_ms, n_launch, tok, flops,
+                    measured_traffic(kname, config, precision))
 
 
 def measured_traffic(kname, config, precision):
