@@ -1418,6 +1418,7 @@ int mvr_index_set_tuning(mvr_index *ix, uint32_t precision, uint32_t wgs_per_cu,
     return fail(MVR_ERR_UNSUPPORTED, "no work-queue kernel for precision %u", precision);
   if (tiles < 1 || tiles > 2) return fail(MVR_ERR_INVALID, "tiles %u not in {1,2}", tiles);
   const uint32_t wmax = persist_max_wgs((int)precision, tiles);
+  if (wmax == 0) return fail(MVR_ERR_INVALID, "no work-queue kernel with %u tiles for precision %u", tiles, precision);
   if (wgs_per_cu < 1 || wgs_per_cu > wmax)
     return fail(MVR_ERR_INVALID, "wgs_per_cu %u not in 1..%u (residency of that kernel)", wgs_per_cu, wmax);
   PersistCfg c;
