@@ -475,7 +475,7 @@ def bench_northstar(torch, args, cfg, corpus, Q, doclens):
                "exhaustive_2bit_vs_fp64": ceiling, "queries": nr}
         if name == "fp32":
             base_qps = run["qps"]
-        errs = score_errors(torch, corpus, Q, ids, sc, doclens, min(nr, 64))
+        errs = score_errors(torch, corpus, Q, ids, sc, doclens, nr)
         line = {"metric": BASELINE_METRIC, "mode": name, "value": round(run["qps"], 1), "unit": "queries/s",
                 "n_gpus": 1, "steps": steps, "warmup": warm, "ms_per_step": round(run["ms_per_step"], 3),
                 "higher_is_better": True, "dtype": mode["precision"] if not mode["precision"].startswith("fp32") else "fp32",
