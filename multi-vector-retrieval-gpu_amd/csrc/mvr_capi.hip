@@ -784,9 +784,11 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   P = std::min(P, d.n_c);
   if (learned && P < kProbeFeat) return fail(MVR_ERR_INVALID, "learned probing needs >= 32 centroids");
   const bool fast_ct = !force_scan || P > 64;  // the all-FP32 scan keeps 64 per row
-  // queries per device pass; the candidate pool holds a 4,096-slot region
-  // per (query, doc segment), so a pass is also capped by the pool budget
-  const uint64_t pool_per_q = (uint64_t)d.n_seg * kSeg * 8;
+  // queries per device pass; the candidate pool holds each query's keys (at
+  // most the docs its t_max x P probed lists hold), so a pass is also capped
+  // by the pool budget
+  const uint64_t keys_per_q = candidate_keys_per_query(d, t_max, P);
+  const uint64_t pool_per_q = keys_per_q * 8;
   const uint32_t B = (uint32_t)std::max<uint64_t>(
       1, std::min<uint64_t>(prm->batch ? prm->batch : 1024, ix->pool_budget / pool_per_q));
   DeviceGuard g(ix->device);
@@ -838,7 +840,7 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
   HIP_TRY(ws.part_s.ensure((size_t)n_split * n_rows_max * P * 4));
   HIP_TRY(ws.top_ids.ensure(Bc * t_max * P * 4));
   HIP_TRY(ws.top_s.ensure(Bc * t_max * P * 4));
-  HIP_TRY(ws.cand_cnt.ensure(Bc * d.n_seg * 4));
+  HIP_TRY(ws.cand_cnt.ensure(Bc * 4));  // per-query fill counters of the pool
   HIP_TRY(ws.cand_keys.ensure(Bc * pool_per_q));
   if (!fp32_mode || fast_ct) HIP_TRY(ws.bmax.ensure(Bc * t_max * ((d.n_c + 31) / 32) * 4));
   if (learned) {
@@ -993,9 +995,10 @@ int mvr_search(mvr_index *ix, const float *Q, uint32_t n_q, uint32_t t_max, cons
       q_r = ws.dyn_r.as<uint32_t>();
     }
     HIP_TRY(launch_filter(ws.top_ids.as<uint32_t>(), ws.top_s.as<float>(), t_max, P, bnt, bnp, nb,
-                          d, prm->clip_zero ? 0 : 1, ws.cand_cnt.as<uint32_t>(), ws.cand_keys.as<uint64_t>(), st));
+                          d, prm->clip_zero ? 0 : 1, ws.cand_cnt.as<uint32_t>(), keys_per_q,
+                          ws.cand_keys.as<uint64_t>(), st));
     HIP_TRY(mark(2));
-    HIP_TRY(launch_select(ws.cand_keys.as<uint64_t>(), ws.cand_cnt.as<uint32_t>(), d.n_seg, q_r, nb, R,
+    HIP_TRY(launch_select(ws.cand_keys.as<uint64_t>(), ws.cand_cnt.as<uint32_t>(), keys_per_q, q_r, nb, R,
                           sl.ids, sl.F, sl.n, sl.tot, st));
     HIP_TRY(mark(3));
     return MVR_OK;

@@ -80,17 +80,32 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t v, size_t n, hipStream_t s);
 
 // clamp0: per-(doc, token) maxima are max(0, .) (the PP filter); else the
 // first hit's score as it is (the basic variant)
+// Candidate keys go to cand_keys[q * keys_per_q ..], q_fill[q] of them (q_fill:
+// n_q counters, zeroed by the launch); keys_per_q must be at least the number
+// of distinct docs a query's probed lists can hold (candidate_keys_per_query).
 hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t_max, uint32_t P,
                          const uint32_t *q_ntok, const uint32_t *q_np, uint32_t n_q,
-                         const DevIndex &ix, int clamp0, uint32_t *seg_cnt, uint64_t *cand_keys,
-                         hipStream_t s);
+                         const DevIndex &ix, int clamp0, uint32_t *q_fill, uint64_t keys_per_q,
+                         uint64_t *cand_keys, hipStream_t s);
+// min(docs in the index rounded up to whole segments, t_max * P * longest IVF
+// list): a query's candidates are distinct docs of its t_max * P probed lists
+inline uint64_t candidate_keys_per_query(const DevIndex &ix, uint32_t t_max, uint32_t P) {
+  const uint64_t all = (uint64_t)ix.n_seg * kSeg;
+  const uint64_t lists = (uint64_t)t_max * P * (ix.max_list ? ix.max_list : 1u);
+  return lists < all ? lists : all;
+}
 
 // top-R per query (q_r, nullable: per-query R <= R; R is the output row stride)
-// over the per-(query, segment) candidate regions; cand_tot[q] receives the
-// query's candidate count
-hipError_t launch_select(const uint64_t *cand_keys, const uint32_t *seg_cnt, uint32_t n_seg,
+// over each query's candidate keys (launch_filter's layout); cand_tot[q]
+// receives the query's candidate count
+hipError_t launch_select(const uint64_t *cand_keys, const uint32_t *q_fill, uint64_t keys_per_q,
                          const uint32_t *q_r, uint32_t n_q, uint32_t R, uint32_t *sel_ids,
                          float *sel_F, uint32_t *sel_n, uint32_t *cand_tot, hipStream_t s);
+// The same over per-(query, segment) regions: seg_cnt[q][sg] keys at
+// cand_keys + (q * n_seg + sg) * kSeg (the PLAID filter's layout).
+hipError_t launch_select_segments(const uint64_t *cand_keys, const uint32_t *seg_cnt, uint32_t n_seg,
+                                  const uint32_t *q_r, uint32_t n_q, uint32_t R, uint32_t *sel_ids,
+                                  float *sel_F, uint32_t *sel_n, uint32_t *cand_tot, hipStream_t s);
 
 // persistent work-queue MaxSim + top-kk (kk <= 256) over whole queries;
 // precision 1 (BF16) or 3 (FP32 MFMA).  queue [1] is scratch.

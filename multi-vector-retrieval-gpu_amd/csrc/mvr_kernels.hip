@@ -256,11 +256,12 @@ __global__ __launch_bounds__(256) void k_topk_merge(const uint32_t *__restrict__
 // group) tiles; neighbouring tiles of one query then share the L2 lines of
 // the seg_off rows and of the query's probe lists.
 //
-// Candidates go to the segment's own 4,096-slot region of the candidate pool
-// (query-major, then segment) as 64-bit selection keys, with the count in
-// seg_cnt[q][seg]: no atomics on shared counters, and each wave writes its
-// candidates as contiguous runs.  The emission clears the LDS state it reads,
-// ready for the next segment.
+// Candidates go to the query's region of the candidate pool (keys_per_q
+// 64-bit selection keys per query: at most the docs its probed lists hold) as
+// one contiguous run per (segment, wave), reserved with one atomic on the
+// query's fill counter q_fill[q].  Runs land in any order (the selection is
+// by key value).  The emission clears the LDS state it reads, ready for the
+// next segment.
 // ---------------------------------------------------------------------------
 constexpr uint32_t kFiltGroup = 8;                    // segments per workgroup
 constexpr uint32_t kFiltHold = 1;                     // list entries per lane and rank loaded ahead
@@ -296,14 +297,18 @@ __device__ __forceinline__ long long filter_fx(float s, int clamp0) {
 // Emission of one segment: slot k of every lane in turn; a wave's hits
 // (slot-major, then lane) are compacted into a list of doc offsets (hl, kSeg
 // 16-bit entries per workgroup), so the conversion and key packing run once
-// per hit rather than once per slot, and written as one contiguous run, waves
-// in order.  wsum holds the per-wave counts.  The caller's barrier separates
-// the last LDS update of the segment from this; the emission ends with a
-// barrier and leaves mask and acc cleared.
+// per hit rather than once per slot.  The segment's keys are one contiguous
+// run in the query's key region, reserved by one atomic on q_fill[q]; the
+// waves build their keys in registers (at most kSeg / kFtThreads per lane)
+// while that atomic is in flight.  wsum: the per-wave counts, then the run's
+// start (slot kFtThreads / kWave).  The caller's barrier separates the last
+// LDS update of the segment from this; the emission ends with a barrier and
+// leaves mask and acc cleared.
 __device__ __forceinline__ void filter_emit(uint32_t *mask, unsigned long long *acc, uint32_t *wsum,
-                                            uint16_t *hl_all, uint32_t q, uint32_t sg, uint32_t n_seg,
-                                            uint32_t n_doc, uint32_t *__restrict__ seg_cnt,
+                                            uint16_t *hl_all, uint32_t q, uint32_t sg, uint32_t n_doc,
+                                            uint32_t *__restrict__ q_fill, uint64_t keys_per_q,
                                             uint64_t *__restrict__ cand_keys) {
+  constexpr uint32_t kPer = kSeg / kFtThreads;  // slots (and at most hits) per lane
   const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
   const uint32_t seg_lo = sg * kSeg;
   uint16_t *hl = hl_all + wv * (kSeg / (kFtThreads / kWave));
@@ -311,7 +316,7 @@ __device__ __forceinline__ void filter_emit(uint32_t *mask, unsigned long long *
   const bool tail = seg_lo + kSeg > n_doc;  // the last segment, partly past the corpus
   uint32_t wcount = 0;  // hits of this wave so far (uniform)
 #pragma unroll
-  for (uint32_t k = 0; k < kSeg / kFtThreads; ++k) {
+  for (uint32_t k = 0; k < kPer; ++k) {
     const uint32_t dl = lane + kWave * (wv + (kFtThreads / kWave) * k);
     bool hit = false;
     if (mask[dl] != 0u) {
@@ -330,16 +335,31 @@ __device__ __forceinline__ void filter_emit(uint32_t *mask, unsigned long long *
   if (tid == 0) {
     uint32_t tot = 0;
     for (uint32_t w = 0; w < kFtThreads / kWave; ++w) tot += wsum[w];
-    seg_cnt[(size_t)q * n_seg + sg] = tot;
+    wsum[kFtThreads / kWave] = tot ? atomicAdd(&q_fill[q], tot) : 0u;
   }
-  uint64_t *out = cand_keys + ((size_t)q * n_seg + sg) * kSeg + pos;
-  for (uint32_t i = lane; i < wcount; i += kWave) {
-    const uint32_t dl = hl[i];
-    const float F = (float)((double)(long long)acc[dl] * (1.0 / kFxScale));
-    acc[dl] = 0ull;
-    out[i] = make_key(F, seg_lo + dl);
+  uint64_t kv[kPer];
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; ++k) {
+    const uint32_t i = lane + kWave * k;
+    kv[k] = 0;
+    if (i < wcount) {
+      const uint32_t dl = hl[i];
+      const float F = (float)((double)(long long)acc[dl] * (1.0 / kFxScale));
+      acc[dl] = 0ull;
+      kv[k] = make_key(F, seg_lo + dl);
+    }
   }
-  __syncthreads();  // wsum and the cleared state before the next segment
+  __syncthreads();  // the run's start
+  const uint64_t base = (uint64_t)wsum[kFtThreads / kWave] + pos;
+  uint64_t *out = cand_keys + (size_t)q * keys_per_q + base;
+#pragma unroll
+  for (uint32_t k = 0; k < kPer; ++k) {
+    const uint32_t i = lane + kWave * k;
+    // (the region holds every doc the query's lists can reach; the check
+    // keeps a wrong bound from writing past it)
+    if (i < wcount && base + i < keys_per_q) out[i] = kv[k];
+  }
+  __syncthreads();  // wsum read before the next segment reuses it
 }
 
 // The XCD-aware workgroup order of the filter kernels: consecutive ids go to
@@ -358,7 +378,7 @@ __global__ __launch_bounds__(kFtThreads) __attribute__((amdgpu_waves_per_eu(NS <
     uint32_t P, const uint32_t *__restrict__ q_ntok, const uint32_t *__restrict__ q_np,
     const uint64_t *__restrict__ ivf_off, const uint32_t *__restrict__ ivf_ids,
     const uint32_t *__restrict__ seg_off, uint32_t n_seg, uint32_t n_doc, uint32_t n_q, int clamp0,
-    uint32_t *__restrict__ seg_cnt, uint64_t *__restrict__ cand_keys) {
+    uint32_t *__restrict__ q_fill, uint64_t keys_per_q, uint64_t *__restrict__ cand_keys) {
   // run tables for kR ranks (32, or 64 when a lane holds four ranks)
   constexpr uint32_t kR = NS * kFtLanes > 32 ? NS * kFtLanes : 32;
   static_assert(NS * kFtLanes <= kFtTokMax, "ranks per lane");
@@ -503,7 +523,7 @@ __global__ __launch_bounds__(kFtThreads) __attribute__((amdgpu_waves_per_eu(NS <
     // the emission's per-wave counts and hit lists go to this segment's run
     // tables, whose last reader was the loop above and whose next writer runs
     // after the emission's closing barrier
-    filter_emit(mask, acc, tb_len[b], reinterpret_cast<uint16_t *>(tb_st[b]), q, sg, n_seg, n_doc, seg_cnt,
+    filter_emit(mask, acc, tb_len[b], reinterpret_cast<uint16_t *>(tb_st[b]), q, sg, n_doc, q_fill, keys_per_q,
                 cand_keys);
   }
 }
@@ -519,14 +539,14 @@ __global__ __launch_bounds__(kFtThreads) void k_filter_wide(
     uint32_t P, const uint32_t *__restrict__ q_ntok, const uint32_t *__restrict__ q_np,
     const uint64_t *__restrict__ ivf_off, const uint32_t *__restrict__ ivf_ids,
     const uint32_t *__restrict__ seg_off, uint32_t n_seg, uint32_t n_doc, uint32_t n_q, int clamp0,
-    uint32_t *__restrict__ seg_cnt, uint64_t *__restrict__ cand_keys) {
+    uint32_t *__restrict__ q_fill, uint64_t keys_per_q, uint64_t *__restrict__ cand_keys) {
   constexpr uint32_t kG = 2 * kFtLanes;  // ranks per group
+  __shared__ uint32_t wsum[kFtThreads / kWave + 1];  // per-wave counts, then the segment's run start
   __shared__ uint32_t mask[kSeg];
   __shared__ unsigned long long acc[kSeg];
   __shared__ uint64_t tb_st[kG * kMaxQTok];  // also the emission's hit list
   __shared__ uint32_t tb_len[kG * kMaxQTok];
   __shared__ long long tb_fx[kG * kMaxQTok];
-  __shared__ uint32_t wsum[kFtThreads / kWave];
   static_assert(kG * kMaxQTok * 8 >= kSeg * 2, "hit list size");
 
   const uint32_t n_grp = (n_seg + kFiltGroup - 1) / kFiltGroup;
@@ -584,7 +604,7 @@ __global__ __launch_bounds__(kFtThreads) void k_filter_wide(
       wave_lds_sync();  // table reads done before the next group rewrites them
     }
     __syncthreads();
-    filter_emit(mask, acc, wsum, reinterpret_cast<uint16_t *>(tb_st), q, sg, n_seg, n_doc, seg_cnt, cand_keys);
+    filter_emit(mask, acc, wsum, reinterpret_cast<uint16_t *>(tb_st), q, sg, n_doc, q_fill, keys_per_q, cand_keys);
   }
 }
 
@@ -625,13 +645,17 @@ constexpr uint32_t kSelSmall = 4096;  // key capacity of the common variant (R <
 // pivot scratch and indexes up to 4,096 segments (16.7M docs); the
 // 16,384-key one (128 KB of keys) a 1,024-key scratch and 2,048 segments,
 // walking larger lists segment by segment.
-template <uint32_t CAP>
+// SEG: the candidate keys of query q lie in per-segment regions (seg_cnt[q][sg]
+// keys at keys + sg * kSeg, the PLAID filter's layout); otherwise they are
+// contiguous (q_fill[q] keys from q * keys_per_q, the IGP filter's layout).
+template <uint32_t CAP, bool SEG>
 __global__ __launch_bounds__(kSelThreads) void k_select_topr(
     const uint64_t *__restrict__ cand_keys, const uint32_t *__restrict__ seg_cnt, uint32_t n_seg,
+    const uint32_t *__restrict__ q_fill, uint64_t keys_per_q,
     const uint32_t *__restrict__ q_r, uint32_t R_out, uint32_t *__restrict__ sel_ids,
     float *__restrict__ sel_F, uint32_t *__restrict__ sel_n, uint32_t *__restrict__ cand_tot) {
   constexpr uint32_t kSelScratch = CAP > kSelSmall ? 1024 : 2048;
-  constexpr uint32_t kSelMaxSeg = CAP > kSelSmall ? 2048 : 4096;
+  constexpr uint32_t kSelMaxSeg = SEG ? (CAP > kSelSmall ? 2048 : 4096) : 1;
   __shared__ unsigned long long outk[CAP];
   __shared__ unsigned long long scr[kSelScratch];
   __shared__ uint32_t hist[2048];
@@ -642,50 +666,64 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
   const uint32_t q = blockIdx.x, tid = threadIdx.x;
   const uint32_t sel_lane = tid & (kWave - 1);
   const uint64_t sel_lt_mask = sel_lane == 0 ? 0ull : (~0ull >> (kWave - sel_lane));
-  const uint32_t *cnt = seg_cnt + (size_t)q * n_seg;
-  const uint64_t *keys = cand_keys + (size_t)q * n_seg * kSeg;
-  const bool flat = n_seg <= kSelMaxSeg;
-  // the query's candidates: cnt[sg] keys at keys + sg * kSeg for every segment
-  if (flat) {
-    // prefix sum of the counts: each thread sums a run of segments, then a
-    // scan over the threads' run totals
-    const uint32_t per = (n_seg + kSelThreads - 1) / kSelThreads;
-    const uint32_t s0 = min(n_seg, tid * per), s1 = min(n_seg, s0 + per);
-    uint32_t part = 0;
-    for (uint32_t sg = s0; sg < s1; ++sg) part += cnt[sg];
-    const uint32_t lane = tid & (kWave - 1), wv = tid / kWave;
-    uint32_t incl = part;
+  const uint32_t *cnt = SEG ? seg_cnt + (size_t)q * n_seg : nullptr;
+  const uint64_t *keys = SEG ? cand_keys + (size_t)q * n_seg * kSeg : cand_keys + (size_t)q * keys_per_q;
+  const bool flat = !SEG || n_seg <= kSelMaxSeg;
+  if constexpr (SEG) {
+    // the query's candidates: cnt[sg] keys at keys + sg * kSeg for every segment
+    if (flat) {
+      // prefix sum of the counts: each thread sums a run of segments, then a
+      // scan over the threads' run totals
+      const uint32_t per = (n_seg + kSelThreads - 1) / kSelThreads;
+      const uint32_t s0 = min(n_seg, tid * per), s1 = min(n_seg, s0 + per);
+      uint32_t part = 0;
+      for (uint32_t sg = s0; sg < s1; ++sg) part += cnt[sg];
+      const uint32_t lane = tid & (kWave - 1), wv = tid / kWave;
+      uint32_t incl = part;
 #pragma unroll
-    for (uint32_t o = 1; o < kWave; o <<= 1) {
-      const uint32_t u = __shfl_up(incl, o);
-      if (lane >= o) incl += u;
-    }
-    if (lane == kWave - 1) s_wsum[wv] = incl;
-    __syncthreads();
-    uint32_t run = incl - part;
-    for (uint32_t w = 0; w < wv; ++w) run += s_wsum[w];
-    for (uint32_t sg = s0; sg < s1; ++sg) {
-      spfx[sg] = run;
-      run += cnt[sg];
-    }
-    if (tid == kSelThreads - 1) {
-      spfx[n_seg] = run;
-      s_n = run;
+      for (uint32_t o = 1; o < kWave; o <<= 1) {
+        const uint32_t u = __shfl_up(incl, o);
+        if (lane >= o) incl += u;
+      }
+      if (lane == kWave - 1) s_wsum[wv] = incl;
+      __syncthreads();
+      uint32_t run = incl - part;
+      for (uint32_t w = 0; w < wv; ++w) run += s_wsum[w];
+      for (uint32_t sg = s0; sg < s1; ++sg) {
+        spfx[sg] = run;
+        run += cnt[sg];
+      }
+      if (tid == kSelThreads - 1) {
+        spfx[n_seg] = run;
+        s_n = run;
+      }
+    } else {
+      if (tid == 0) s_n = 0;
+      __syncthreads();
+      uint32_t part = 0;
+      for (uint32_t sg = tid; sg < n_seg; sg += kSelThreads) part += cnt[sg];
+      if (part) atomicAdd(&s_n, part);
     }
   } else {
-    if (tid == 0) s_n = 0;
-    __syncthreads();
-    uint32_t part = 0;
-    for (uint32_t sg = tid; sg < n_seg; sg += kSelThreads) part += cnt[sg];
-    if (part) atomicAdd(&s_n, part);
+    if (tid == 0) s_n = (uint32_t)min((uint64_t)q_fill[q], keys_per_q);
   }
   __syncthreads();
   const uint32_t n = s_n;
   if (tid == 0 && cand_tot) cand_tot[q] = n;
-  // visit every candidate key.  Indexed: key i of the query's list lies in
-  // the segment sg with spfx[sg] <= i < spfx[sg + 1]; thread tid takes
-  // i = tid, tid + kSelThreads, ... with a segment cursor that only moves
-  // forward, eight loads in flight.  Otherwise segment by segment.
+  // key i of the query's list: contiguous, or (segment layout) in the
+  // segment sg with spfx[sg] <= i < spfx[sg + 1], found with a cursor that
+  // only moves forward
+  auto key_at = [&](uint32_t i, uint32_t &sg) -> uint64_t {
+    if constexpr (!SEG) {
+      return keys[i];
+    } else {
+      while (spfx[sg + 1] <= i) ++sg;
+      return keys[(size_t)sg * kSeg + (i - spfx[sg])];
+    }
+  };
+  // visit every candidate key: thread tid takes i = tid, tid + kSelThreads,
+  // ..., eight loads in flight; a segment layout with too many segments for
+  // the prefix table goes segment by segment
   auto for_each_key = [&](auto &&body) {
     if (flat) {
       constexpr uint32_t U = 8;
@@ -695,10 +733,7 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u) {
           const uint32_t i = i0 + u * kSelThreads;
-          if (i < n) {
-            while (spfx[sg + 1] <= i) ++sg;
-            kv[u] = keys[(size_t)sg * kSeg + (i - spfx[sg])];
-          }
+          if (i < n) kv[u] = key_at(i, sg);
         }
 #pragma unroll
         for (uint32_t u = 0; u < U; ++u)
@@ -723,10 +758,7 @@ __global__ __launch_bounds__(kSelThreads) void k_select_topr(
     const uint32_t np = pow2_ceil(max(n, 2u));
     if (flat) {
       uint32_t sg = 0;
-      for (uint32_t i = tid; i < n; i += kSelThreads) {
-        while (spfx[sg + 1] <= i) ++sg;
-        outk[i] = keys[(size_t)sg * kSeg + (i - spfx[sg])];
-      }
+      for (uint32_t i = tid; i < n; i += kSelThreads) outk[i] = key_at(i, sg);
     } else {
       uint32_t base = 0;
       for (uint32_t sg = 0; sg < n_seg; ++sg) {
@@ -3240,16 +3272,18 @@ hipError_t launch_centroid_topk_fp8(const float *Q, uint32_t n_q, uint32_t t_max
 
 hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t_max, uint32_t P,
                          const uint32_t *q_ntok, const uint32_t *q_np, uint32_t n_q,
-                         const DevIndex &ix, int clamp0, uint32_t *seg_cnt, uint64_t *cand_keys,
-                         hipStream_t s) {
+                         const DevIndex &ix, int clamp0, uint32_t *q_fill, uint64_t keys_per_q,
+                         uint64_t *cand_keys, hipStream_t s) {
   if (n_q == 0) return hipSuccess;
   const uint64_t n_wg = (uint64_t)ix.n_seg * n_q;
+  // the fill counters start at zero (a fill kernel: graph-safe)
+  k_fill_u32<<<(n_q + 255) / 256, 256, 0, s>>>(q_fill, 0u, n_q);
   if (n_wg > 0x7FFFFFF8ull || P == 0 || P > kMaxProbe) return hipErrorInvalidValue;
   const uint64_t n_grp = ((uint64_t)ix.n_seg + kFiltGroup - 1) / kFiltGroup * n_q;
   const uint32_t grid = (uint32_t)((n_grp + 7) / 8 * 8);  // the kernel's XCD remap needs a multiple of 8
 #define MVR_FILTER(KERNEL)                                                                                  \
   KERNEL<<<grid, kFtThreads, 0, s>>>(top_ids, top_s, t_max, P, q_ntok, q_np, ix.ivf_off, ix.ivf_ids, ix.seg_off, \
-                                     ix.n_seg, ix.n_doc, n_q, clamp0, seg_cnt, cand_keys)
+                                     ix.n_seg, ix.n_doc, n_q, clamp0, q_fill, keys_per_q, cand_keys)
   // probe ranks per lane: the fewest that hold P
   if (P <= kFtLanes) MVR_FILTER(k_filter_tok<1>);
   else if (P <= 2 * kFtLanes) MVR_FILTER(k_filter_tok<2>);
@@ -3259,16 +3293,29 @@ hipError_t launch_filter(const uint32_t *top_ids, const float *top_s, uint32_t t
   return hipGetLastError();
 }
 
-hipError_t launch_select(const uint64_t *cand_keys, const uint32_t *seg_cnt, uint32_t n_seg,
+hipError_t launch_select(const uint64_t *cand_keys, const uint32_t *q_fill, uint64_t keys_per_q,
                          const uint32_t *q_r, uint32_t n_q, uint32_t R, uint32_t *sel_ids,
                          float *sel_F, uint32_t *sel_n, uint32_t *cand_tot, hipStream_t s) {
   if (n_q == 0) return hipSuccess;
   if (R <= kSelSmall)
-    k_select_topr<kSelSmall><<<n_q, kSelThreads, 0, s>>>(cand_keys, seg_cnt, n_seg, q_r, R, sel_ids, sel_F,
-                                                         sel_n, cand_tot);
+    k_select_topr<kSelSmall, false><<<n_q, kSelThreads, 0, s>>>(cand_keys, nullptr, 0, q_fill, keys_per_q, q_r,
+                                                                R, sel_ids, sel_F, sel_n, cand_tot);
   else
-    k_select_topr<kMaxSelect><<<n_q, kSelThreads, 0, s>>>(cand_keys, seg_cnt, n_seg, q_r, R, sel_ids, sel_F,
-                                                          sel_n, cand_tot);
+    k_select_topr<kMaxSelect, false><<<n_q, kSelThreads, 0, s>>>(cand_keys, nullptr, 0, q_fill, keys_per_q, q_r,
+                                                                 R, sel_ids, sel_F, sel_n, cand_tot);
+  return hipGetLastError();
+}
+
+hipError_t launch_select_segments(const uint64_t *cand_keys, const uint32_t *seg_cnt, uint32_t n_seg,
+                                  const uint32_t *q_r, uint32_t n_q, uint32_t R, uint32_t *sel_ids,
+                                  float *sel_F, uint32_t *sel_n, uint32_t *cand_tot, hipStream_t s) {
+  if (n_q == 0) return hipSuccess;
+  if (R <= kSelSmall)
+    k_select_topr<kSelSmall, true><<<n_q, kSelThreads, 0, s>>>(cand_keys, seg_cnt, n_seg, nullptr, 0, q_r, R,
+                                                               sel_ids, sel_F, sel_n, cand_tot);
+  else
+    k_select_topr<kMaxSelect, true><<<n_q, kSelThreads, 0, s>>>(cand_keys, seg_cnt, n_seg, nullptr, 0, q_r, R,
+                                                                sel_ids, sel_F, sel_n, cand_tot);
   return hipGetLastError();
 }
 

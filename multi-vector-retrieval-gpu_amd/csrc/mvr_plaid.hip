@@ -1285,8 +1285,8 @@ hipError_t launch_filter_stages(const uint32_t *in_ids, const uint32_t *in_n, ui
   else
     k_pl_segcnt<<<grid_for((uint64_t)n_q * n_seg1, 256), 256, 0, s>>>(in_n, n_fixed, n_seg1, n_q, seg1);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  if ((e = launch_select(keys1, seg1, n_seg1, nullptr, n_q, ndocs, sel1_ids, sel1_F, sel1_n, nullptr, s)) !=
-      hipSuccess)
+  if ((e = launch_select_segments(keys1, seg1, n_seg1, nullptr, n_q, ndocs, sel1_ids, sel1_F, sel1_n, nullptr,
+                                  s)) != hipSuccess)
     return e;
   // stage 2: one key per stage-1 pid, contiguous per query over n_seg2
   // segments (ndocs may exceed one segment); the stage-1 counts buffer
@@ -1300,7 +1300,7 @@ hipError_t launch_filter_stages(const uint32_t *in_ids, const uint32_t *in_n, ui
   if ((e = hipGetLastError()) != hipSuccess) return e;
   k_pl_segcnt<<<grid_for((uint64_t)n_q * n_seg2, 256), 256, 0, s>>>(sel1_n, 0, n_seg2, n_q, seg1);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  return launch_select(keys2, seg1, n_seg2, nullptr, n_q, ndocs / 4, sel2_ids, sel2_F, sel2_n, nullptr, s);
+  return launch_select_segments(keys2, seg1, n_seg2, nullptr, n_q, ndocs / 4, sel2_ids, sel2_F, sel2_n, nullptr, s);
 }
 
 }  // namespace
