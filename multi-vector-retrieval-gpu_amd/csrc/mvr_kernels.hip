@@ -298,9 +298,8 @@ __device__ __forceinline__ long long filter_fx(float s, int clamp0) {
 // (slot-major, then lane) are compacted into a list of doc offsets (hl, kSeg
 // 16-bit entries per workgroup), so the conversion and key packing run once
 // per hit rather than once per slot.  The segment's keys are one contiguous
-// run in the query's key region, reserved by one atomic on q_fill[q]; the
-// waves build their keys in registers (at most kSeg / kFtThreads per lane)
-// while that atomic is in flight.  wsum: the per-wave counts, then the run's
+// run in the query's key region, reserved by one atomic on q_fill[q].  wsum:
+// the per-wave counts, then the run's
 // start (slot kFtThreads / kWave).  The caller's barrier separates the last
 // LDS update of the segment from this; the emission ends with a barrier and
 // leaves mask and acc cleared.
@@ -308,7 +307,7 @@ __device__ __forceinline__ void filter_emit(uint32_t *mask, unsigned long long *
                                             uint16_t *hl_all, uint32_t q, uint32_t sg, uint32_t n_doc,
                                             uint32_t *__restrict__ q_fill, uint64_t keys_per_q,
                                             uint64_t *__restrict__ cand_keys) {
-  constexpr uint32_t kPer = kSeg / kFtThreads;  // slots (and at most hits) per lane
+  constexpr uint32_t kPer = kSeg / kFtThreads;  // slots per lane
   const uint32_t tid = threadIdx.x, lane = tid & (kWave - 1), wv = tid >> 6;
   const uint32_t seg_lo = sg * kSeg;
   uint16_t *hl = hl_all + wv * (kSeg / (kFtThreads / kWave));
@@ -337,29 +336,18 @@ __device__ __forceinline__ void filter_emit(uint32_t *mask, unsigned long long *
     for (uint32_t w = 0; w < kFtThreads / kWave; ++w) tot += wsum[w];
     wsum[kFtThreads / kWave] = tot ? atomicAdd(&q_fill[q], tot) : 0u;
   }
-  uint64_t kv[kPer];
-#pragma unroll
-  for (uint32_t k = 0; k < kPer; ++k) {
-    const uint32_t i = lane + kWave * k;
-    kv[k] = 0;
-    if (i < wcount) {
-      const uint32_t dl = hl[i];
-      const float F = (float)((double)(long long)acc[dl] * (1.0 / kFxScale));
-      acc[dl] = 0ull;
-      kv[k] = make_key(F, seg_lo + dl);
-    }
-  }
   __syncthreads();  // the run's start
   const uint64_t base = (uint64_t)wsum[kFtThreads / kWave] + pos;
   uint64_t *out = cand_keys + (size_t)q * keys_per_q + base;
-#pragma unroll
-  for (uint32_t k = 0; k < kPer; ++k) {
-    const uint32_t i = lane + kWave * k;
+  for (uint32_t i = lane; i < wcount; i += kWave) {
+    const uint32_t dl = hl[i];
+    const float F = (float)((double)(long long)acc[dl] * (1.0 / kFxScale));
+    acc[dl] = 0ull;
     // (the region holds every doc the query's lists can reach; the check
     // keeps a wrong bound from writing past it)
-    if (i < wcount && base + i < keys_per_q) out[i] = kv[k];
+    if (base + i < keys_per_q) out[i] = make_key(F, seg_lo + dl);
   }
-  __syncthreads();  // wsum read before the next segment reuses it
+  __syncthreads();  // wsum and the cleared state before the next segment
 }
 
 // The XCD-aware workgroup order of the filter kernels: consecutive ids go to
